@@ -1,0 +1,199 @@
+"""Benchmark of the plain hash group-by on one MI355X.
+
+Workload (the high-cardinality configuration): 100M rows; int64 key uniform
+over 10M values with 1% null keys (excluded); float64 N(0,1) and int32
+U[0,1e4] value columns, each 5% null; sum, mean, count, min, max, var
+(ddof=1) on both.  Inputs are generated on the device and stay resident in
+HBM; a step is one `GroupBy(keys).aggregate(requests)` call including the
+materialised output columns.
+
+Prints one JSON line.  `roofline` is for the dominant kernel (the accumulate
+pass), timed with hipEvents the library records on the stream it launches
+on; `pipeline` gives the same ratio for the whole step.  `cpu_baseline` times
+the CPU oracle on a bounded sample; `pandas_baseline` times pandas (the
+cudf.pandas CPU fallback) on the same sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "groupby rows/s + %HBM roofline at 100M rows; batched vs plain queries/s, p99"
+HBM_PEAK = 8.0e12
+N_ROWS = 100_000_000
+CARD = 10_000_000
+
+
+def pack_bits(valid):
+    import torch
+    n = valid.numel()
+    pad = (-n) % 32
+    v = torch.nn.functional.pad(valid.to(torch.int64), (0, pad)).view(-1, 32)
+    w = (v << torch.arange(32, device=valid.device, dtype=torch.int64)).sum(1)
+    w = torch.where(w >= 2**31, w - 2**32, w)
+    return w.to(torch.int32).contiguous()
+
+
+def make_c2(n, card, seed=0):
+    import torch
+    from cudf_amd import Column, TypeId
+    dev = torch.device("cuda", 0)
+    g = torch.Generator(device=dev).manual_seed(seed)
+    key = torch.randint(0, card, (n,), device=dev, dtype=torch.int64, generator=g)
+    kvalid = torch.rand(n, device=dev, generator=g) >= 0.01
+    f64 = torch.randn(n, device=dev, dtype=torch.float64, generator=g)
+    fvalid = torch.rand(n, device=dev, generator=g) >= 0.05
+    i32 = torch.randint(0, 10_001, (n,), device=dev, dtype=torch.int32, generator=g)
+    ivalid = torch.rand(n, device=dev, generator=g) >= 0.05
+
+    def col(t, valid, tid):
+        nulls = int((~valid).sum().item())
+        return Column(tid, n, t.view(torch.uint8), pack_bits(valid), nulls)
+
+    cols = (col(key, kvalid, TypeId.INT64), col(f64, fvalid, TypeId.FLOAT64), col(i32, ivalid, TypeId.INT32))
+    del kvalid, fvalid, ivalid
+    return cols, (key, f64, i32)
+
+
+def input_bytes(n):
+    return n * (8 + 8 + 4) + 3 * 4 * ((n + 31) // 32)
+
+
+def output_bytes(key_t, res):
+    from cudf_amd.types import width
+    total = 0
+    for c in list(key_t.columns()) + [c for t in res for c in t.columns()]:
+        total += c.size * width(c.type_id)
+        if c.mask is not None:
+            total += 4 * ((c.size + 31) // 32)
+    return total
+
+
+def cpu_baselines(sample_rows, card, seed=1):
+    """Time the CPU oracle (one thread) and pandas on a bounded sample of the
+    same distribution."""
+    from oracle import oracle as orc
+    from tests.gbutil import C2_AGGS, c2_data
+    key, f64, i32 = c2_data(n=sample_rows, card=card, seed=seed)
+    t0 = time.perf_counter()
+    orc.groupby([key], [(f64, C2_AGGS), (i32, C2_AGGS)], 0)
+    t_oracle = time.perf_counter() - t0
+    out = {"cpu_baseline": {"value": sample_rows / t_oracle, "unit": "rows/s", "cores": 1, "kind": "port",
+                            "sample": f"{sample_rows} rows, {card} distinct keys, same distribution and "
+                                      f"aggregations, C oracle (oracle/groupby_oracle.c), 1 thread, "
+                                      f"{t_oracle:.2f} s"}}
+    try:
+        import pandas as pd
+        df = pd.DataFrame({"k": pd.arrays.IntegerArray(key.data, ~key.valid),
+                           "f": pd.arrays.FloatingArray(f64.data, ~f64.valid),
+                           "i": pd.arrays.IntegerArray(i32.data, ~i32.valid)})
+        aggs = ["sum", "mean", "count", "min", "max", "var"]
+        df.groupby("k", sort=False, dropna=True).agg({"f": aggs, "i": aggs})  # warm-up
+        t0 = time.perf_counter()
+        df.groupby("k", sort=False, dropna=True).agg({"f": aggs, "i": aggs})
+        t_pd = time.perf_counter() - t0
+        out["pandas_baseline"] = {"value": sample_rows / t_pd, "unit": "rows/s", "cores": 1,
+                                  "version": pd.__version__, "cpu_count": os.cpu_count(),
+                                  "sample": f"{sample_rows} rows, nullable Int64/Float64/Int32, "
+                                            f"groupby(sort=False, dropna=True).agg, {t_pd:.2f} s"}
+    except ImportError:
+        pass
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--rows", type=int, default=N_ROWS)
+    ap.add_argument("--card", type=int, default=CARD)
+    ap.add_argument("--cpu-sample", type=int, default=4_000_000)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the group-by is measured on a "
+              f"single MI355X and the multi-GPU sharding is described, not run.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import cudf_amd as ca
+    from cudf_amd import _lib
+    from cudf_amd.aggregation import Aggregation, Kind
+
+    torch.cuda.set_device(0)
+    (kcol, fcol, icol), _keep = make_c2(args.rows, args.card)
+    aggs = lambda: [Aggregation(Kind.SUM), Aggregation(Kind.MEAN), Aggregation(Kind.COUNT_VALID),
+                    Aggregation(Kind.MIN), Aggregation(Kind.MAX), Aggregation(Kind.VARIANCE, 1)]
+    gb = ca.GroupBy(ca.Table([kcol]))
+    reqs = [ca.GroupByRequest(fcol, aggs()), ca.GroupByRequest(icol, aggs())]
+    _lib.set_phase_timing(True)
+
+    for _ in range(args.warmup):
+        out = gb.aggregate(reqs)
+        del out
+    torch.cuda.synchronize()
+    acc_ms, phase = [], {}
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        keys_t, res = gb.aggregate(reqs)
+        st = _lib.last_stats()
+        acc_ms.append(st["t_accumulate_ms"])
+        for k, v in st.items():
+            if k.startswith("t_"):
+                phase[k] = phase.get(k, 0.0) + v / args.steps
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    G = keys_t.num_rows()
+    in_b = input_bytes(args.rows)
+    out_b = output_bytes(keys_t, res)
+    t_acc = float(np.mean(acc_ms)) * 1e-3
+    achieved = in_b / t_acc if t_acc > 0 else 0.0
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "traffic_accumulate_c2.json")
+    if os.path.exists(tf):
+        with open(tf) as f:
+            traffic = json.load(f).get("bytes_per_launch")
+    line = {
+        "metric": METRIC,
+        "value": args.rows / (ms_per_step * 1e-3),
+        "unit": "rows/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int64+f64",
+        "data": "synthetic",
+        "config": {"workload": "high-cardinality hash group-by: 100M rows, int64 key uniform over 10M "
+                               "values (1% null, excluded), float64 N(0,1) and int32 U[0,1e4] values "
+                               "(5% null each); sum, mean, count, min, max, var(ddof=1) on both",
+                   "rows": args.rows, "distinct_keys": args.card, "groups": G,
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "kernel": "k_aggregate_global (accumulate pass)",
+                     "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK, "traffic": traffic,
+                     "algorithmic_bytes": in_b, "avg_ms": t_acc * 1e3},
+        "pipeline": {"bytes_alg": in_b + out_b, "bytes_per_row": (in_b + out_b) / args.rows,
+                     "achieved_GBps": (in_b + out_b) / (ms_per_step * 1e-3) / 1e9,
+                     "frac": (in_b + out_b) / (ms_per_step * 1e-3) / HBM_PEAK,
+                     "phase_ms": {k: round(v, 4) for k, v in phase.items()}},
+    }
+    if not args.no_cpu:
+        line.update(cpu_baselines(args.cpu_sample, max(1, args.cpu_sample // 10)))
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,87 @@
+"""Aggregation factories, shaped like pylibcudf.aggregation
+(python/cudf/cudf/_lib/pylibcudf/aggregation.pyx:157-340) and numbered as
+cudf::aggregation::Kind (cudf/aggregation.hpp:86-122)."""
+from __future__ import annotations
+
+import enum
+
+from .types import NullPolicy
+
+
+class Kind(enum.IntEnum):
+    SUM = 0
+    PRODUCT = 1
+    MIN = 2
+    MAX = 3
+    COUNT_VALID = 4
+    COUNT_ALL = 5
+    ANY = 6
+    ALL = 7
+    SUM_OF_SQUARES = 8
+    MEAN = 9
+    M2 = 10
+    VARIANCE = 11
+    STD = 12
+    MEDIAN = 13
+    QUANTILE = 14
+    ARGMAX = 15
+    ARGMIN = 16
+    NUNIQUE = 17
+
+
+class Aggregation:
+    def __init__(self, kind: Kind, ddof: int = 1):
+        self._kind = Kind(kind)
+        self.ddof = int(ddof)
+
+    def kind(self) -> Kind:
+        return self._kind
+
+    def __repr__(self) -> str:
+        extra = f", ddof={self.ddof}" if self._kind in (Kind.VARIANCE, Kind.STD) else ""
+        return f"Aggregation({self._kind.name}{extra})"
+
+
+def sum() -> Aggregation:  # noqa: A001 - mirrors pylibcudf
+    return Aggregation(Kind.SUM)
+
+
+def product() -> Aggregation:
+    return Aggregation(Kind.PRODUCT)
+
+
+def min() -> Aggregation:  # noqa: A001
+    return Aggregation(Kind.MIN)
+
+
+def max() -> Aggregation:  # noqa: A001
+    return Aggregation(Kind.MAX)
+
+
+def count(null_handling: NullPolicy = NullPolicy.EXCLUDE) -> Aggregation:
+    """COUNT_VALID under EXCLUDE, COUNT_ALL under INCLUDE (aggregation.cpp make_count_aggregation)."""
+    return Aggregation(Kind.COUNT_VALID if null_handling == NullPolicy.EXCLUDE else Kind.COUNT_ALL)
+
+
+def sum_of_squares() -> Aggregation:
+    return Aggregation(Kind.SUM_OF_SQUARES)
+
+
+def mean() -> Aggregation:
+    return Aggregation(Kind.MEAN)
+
+
+def variance(ddof: int = 1) -> Aggregation:
+    return Aggregation(Kind.VARIANCE, ddof)
+
+
+def std(ddof: int = 1) -> Aggregation:
+    return Aggregation(Kind.STD, ddof)
+
+
+def argmax() -> Aggregation:
+    return Aggregation(Kind.ARGMAX)
+
+
+def argmin() -> Aggregation:
+    return Aggregation(Kind.ARGMIN)

@@ -1,0 +1,808 @@
+// Host driver of the hash group-by: validates the request, flattens the
+// aggregations into per-column accumulators, sizes the hash table, launches
+// the kernels and assembles the result columns.
+//
+// Reference behaviour restated here:
+//   validation        cpp/src/groupby/groupby.cu:175-221
+//   hash eligibility  cpp/src/groupby/hash/groupby.cu:73-84, 635-651
+//   flattening        cpp/src/groupby/hash/groupby.cu:113-181, 351-383
+//                     (MEAN and VARIANCE/STD reduce to SUM + COUNT_VALID)
+//   target types      cpp/include/cudf/detail/aggregation/aggregation.hpp:1211-1331
+//   result nullability cpp/src/groupby/hash/groupby.cu:431-433 (SUM/MIN/MAX are
+//                     nullable iff the values have nulls, VARIANCE/STD always,
+//                     COUNT never) and the DIV of MEAN (hash/groupby.cu:298-304)
+//   result order      cpp/src/groupby/common/utils.hpp:32-60 (request order)
+#include "groupby.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <atomic>
+#include <mutex>
+#include <unordered_map>
+
+#include "kernels.hpp"
+
+namespace mig {
+
+double* thread_stats()
+{
+  static thread_local double stats[STAT_COUNT] = {0};
+  return stats;
+}
+
+namespace {
+std::atomic<bool> g_phase_timing{false};
+
+// hipEvents marking phase boundaries on the call's stream.
+struct phase_timer {
+  static constexpr int N = 8;
+  bool on;
+  hipStream_t stream;
+  hipEvent_t ev[N];
+  bool recorded[N] = {false};
+  explicit phase_timer(hipStream_t s) : on(g_phase_timing.load()), stream(s)
+  {
+    if (!on) return;
+    static thread_local std::unordered_map<int, std::vector<hipEvent_t>> pool;
+    int dev = 0;
+    MIG_HIP_TRY(hipGetDevice(&dev));
+    auto& v = pool[dev];
+    if (v.empty()) {
+      v.resize(N);
+      for (auto& e : v) MIG_HIP_TRY(hipEventCreate(&e));
+    }
+    for (int i = 0; i < N; i++) ev[i] = v[i];
+  }
+  void mark(int i)
+  {
+    if (!on) return;
+    MIG_HIP_TRY(hipEventRecord(ev[i], stream));
+    recorded[i] = true;
+  }
+  double between(int a, int b)
+  {
+    if (!on || !recorded[a] || !recorded[b]) return 0.0;
+    float ms = 0.f;
+    MIG_HIP_TRY(hipEventElapsedTime(&ms, ev[a], ev[b]));
+    return ms;
+  }
+};
+}  // namespace
+
+void set_phase_timing(bool on) { g_phase_timing.store(on); }
+
+// ---------------------------------------------------------------------------
+void* device_allocator::allocate(size_t bytes)
+{
+  if (bytes == 0) bytes = 1;
+  void* p = nullptr;
+  if (has_mr) {
+    p = mr.allocate(mr.ctx, bytes, stream);
+    if (!p) fail(MIG_ERROR_ALLOC, "memory resource returned null for " + std::to_string(bytes) + " bytes");
+  } else {
+    hipError_t e = hipMallocAsync(&p, bytes, stream);
+    if (e != hipSuccess) fail(MIG_ERROR_ALLOC, std::string("hipMallocAsync failed: ") + hipGetErrorString(e));
+  }
+  return p;
+}
+
+void device_allocator::deallocate(void* p, size_t bytes)
+{
+  if (!p) return;
+  if (bytes == 0) bytes = 1;
+  if (has_mr) mr.deallocate(mr.ctx, p, bytes, stream);
+  else (void)hipFreeAsync(p, stream);
+}
+
+namespace {
+
+struct device_props {
+  int num_cus = 256;
+};
+
+device_props const& props()
+{
+  static std::mutex mu;
+  static std::unordered_map<int, device_props> cache;
+  int dev = 0;
+  MIG_HIP_TRY(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = cache.find(dev);
+  if (it != cache.end()) return it->second;
+  device_props p;
+  MIG_HIP_TRY(hipDeviceGetAttribute(&p.num_cus, hipDeviceAttributeMultiprocessorCount, dev));
+  return cache.emplace(dev, p).first->second;
+}
+
+// Scratch memory of one call, freed in stream order when the call ends.
+struct scratch {
+  hipStream_t stream;
+  std::vector<void*> ptrs;
+  explicit scratch(hipStream_t s) : stream(s) {}
+  template <typename T>
+  T* get(size_t count)
+  {
+    void* p = nullptr;
+    size_t bytes = std::max<size_t>(1, count * sizeof(T));
+    hipError_t e = hipMallocAsync(&p, bytes, stream);
+    if (e != hipSuccess)
+      fail(MIG_ERROR_ALLOC, "scratch allocation of " + std::to_string(bytes) + " bytes failed");
+    ptrs.push_back(p);
+    return static_cast<T*>(p);
+  }
+  ~scratch()
+  {
+    for (void* p : ptrs) (void)hipFreeAsync(p, stream);
+  }
+};
+
+// Small pinned host buffer per thread for parameter uploads and readbacks.
+struct pinned_host {
+  void* p = nullptr;
+  size_t bytes = 0;
+  void* get(size_t need)
+  {
+    if (need > bytes) {
+      if (p) (void)hipHostFree(p);
+      p = nullptr;
+      MIG_HIP_TRY(hipHostMalloc(&p, need, hipHostMallocDefault));
+      bytes = need;
+    }
+    return p;
+  }
+  ~pinned_host()
+  {
+    if (p) (void)hipHostFree(p);
+  }
+};
+
+pinned_host& thread_pinned()
+{
+  static thread_local pinned_host h;
+  return h;
+}
+
+bool same_view(mig_column_view const& a, mig_column_view const& b)
+{
+  return a.type_id == b.type_id && a.size == b.size && a.offset == b.offset && a.data == b.data &&
+         a.null_mask == b.null_mask && a.offsets == b.offsets;
+}
+
+int32_t null_count_of(mig_column_view const& v, hipStream_t s)
+{
+  if (v.null_mask == nullptr) return 0;
+  if (v.null_count >= 0) return v.null_count;
+  dcol c{};
+  c.mask        = v.null_mask;
+  c.mask_offset = v.offset;
+  uint32_t* d   = nullptr;
+  MIG_HIP_TRY(hipMallocAsync(reinterpret_cast<void**>(&d), sizeof(uint32_t), s));
+  MIG_HIP_TRY(hipMemsetAsync(d, 0, sizeof(uint32_t), s));
+  launch_count_nulls(c, v.size, d, s);
+  uint32_t valid = 0;
+  MIG_HIP_TRY(hipMemcpyAsync(&valid, d, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  MIG_HIP_TRY(hipStreamSynchronize(s));
+  MIG_HIP_TRY(hipFreeAsync(d, s));
+  return v.size - int32_t(valid);
+}
+
+dcol make_dcol(mig_column_view const& v, bool use_mask)
+{
+  dcol c{};
+  c.storage     = storage_of(v.type_id);
+  c.mask        = use_mask ? v.null_mask : nullptr;
+  c.mask_offset = v.offset;
+  if (c.storage == ST_STRING) {
+    c.data      = static_cast<uint8_t const*>(v.data);
+    c.offsets64 = v.offsets_type_id == MIG_TYPE_INT64;
+    int w       = c.offsets64 ? 8 : 4;
+    c.offsets   = static_cast<uint8_t const*>(v.offsets) + int64_t(w) * v.offset;
+  } else {
+    c.data = static_cast<uint8_t const*>(v.data) + int64_t(storage_width(c.storage)) * v.offset;
+  }
+  return c;
+}
+
+bool is_numeric_type(int32_t t)
+{
+  return (t >= MIG_TYPE_INT8 && t <= MIG_TYPE_FLOAT64) || t == MIG_TYPE_BOOL8;
+}
+bool is_chrono_type(int32_t t) { return t >= MIG_TYPE_TIMESTAMP_DAYS && t <= MIG_TYPE_DURATION_NANOSECONDS; }
+
+char const* kind_name(int32_t k)
+{
+  switch (k) {
+    case MIG_AGG_SUM: return "SUM";
+    case MIG_AGG_PRODUCT: return "PRODUCT";
+    case MIG_AGG_MIN: return "MIN";
+    case MIG_AGG_MAX: return "MAX";
+    case MIG_AGG_COUNT_VALID: return "COUNT_VALID";
+    case MIG_AGG_COUNT_ALL: return "COUNT_ALL";
+    case MIG_AGG_SUM_OF_SQUARES: return "SUM_OF_SQUARES";
+    case MIG_AGG_MEAN: return "MEAN";
+    case MIG_AGG_VARIANCE: return "VARIANCE";
+    case MIG_AGG_STD: return "STD";
+    case MIG_AGG_ARGMIN: return "ARGMIN";
+    case MIG_AGG_ARGMAX: return "ARGMAX";
+    default: return "UNKNOWN";
+  }
+}
+
+// Result type of (source type, aggregation): aggregation.hpp:1211-1331.
+int32_t target_type_of(int32_t src, int32_t kind)
+{
+  switch (kind) {
+    case MIG_AGG_MIN:
+    case MIG_AGG_MAX: return src;
+    case MIG_AGG_COUNT_VALID:
+    case MIG_AGG_COUNT_ALL: return MIG_TYPE_INT32;
+    case MIG_AGG_MEAN:
+    case MIG_AGG_VARIANCE:
+    case MIG_AGG_STD: return MIG_TYPE_FLOAT64;
+    case MIG_AGG_SUM:
+    case MIG_AGG_SUM_OF_SQUARES:
+      if (src == MIG_TYPE_FLOAT32 || src == MIG_TYPE_FLOAT64) return src;
+      return MIG_TYPE_INT64;
+    default: return MIG_TYPE_EMPTY;
+  }
+}
+
+// Validity of (source type, aggregation) on this hash path.  Mirrors the
+// type rules of cudf's is_valid_aggregation for the kinds implemented here.
+void check_supported(int32_t src, int32_t kind)
+{
+  bool ok = false;
+  switch (kind) {
+    case MIG_AGG_COUNT_VALID:
+    case MIG_AGG_COUNT_ALL: ok = src != MIG_TYPE_EMPTY; break;
+    case MIG_AGG_MIN:
+    case MIG_AGG_MAX: ok = is_numeric_type(src) || is_chrono_type(src); break;
+    case MIG_AGG_SUM:
+    case MIG_AGG_SUM_OF_SQUARES:
+    case MIG_AGG_MEAN:
+    case MIG_AGG_VARIANCE:
+    case MIG_AGG_STD: ok = is_numeric_type(src); break;
+    default:
+      fail(MIG_ERROR_LOGIC, std::string("aggregation ") + kind_name(kind) +
+                              " is not supported by the hash group-by");
+  }
+  if (!ok)
+    fail(MIG_ERROR_DATA_TYPE, std::string("unsupported type ") + std::to_string(src) +
+                                " for aggregation " + kind_name(kind));
+}
+
+// Per distinct value column: which accumulators it needs.
+struct value_plan {
+  mig_column_view view{};
+  int32_t storage   = ST_NONE;
+  bool has_nulls    = false;
+  bool want_sum = false, want_sumsq = false, want_min = false, want_max = false;
+  bool want_count = false, want_m2 = false;
+  int32_t f_sum = -1, f_sumsq = -1, f_min = -1, f_max = -1, f_count = -1, f_m2 = -1;
+};
+
+int32_t sum_field_kind(int32_t st)
+{
+  if (st == ST_F32) return FK_SUM_F32;
+  if (st == ST_F64) return FK_SUM_F64;
+  return FK_SUM_I64;
+}
+int32_t sumsq_field_kind(int32_t st)
+{
+  if (st == ST_F32) return FK_SUMSQ_F32;
+  if (st == ST_F64) return FK_SUMSQ_F64;
+  return FK_SUMSQ_I64;
+}
+int32_t minmax_field_kind(int32_t st, bool is_min)
+{
+  switch (st) {
+    case ST_I8:
+    case ST_I16:
+    case ST_I32: return is_min ? FK_MIN_S32 : FK_MAX_S32;
+    case ST_I64: return is_min ? FK_MIN_S64 : FK_MAX_S64;
+    case ST_U8:
+    case ST_U16:
+    case ST_U32:
+    case ST_BOOL: return is_min ? FK_MIN_U32 : FK_MAX_U32;
+    case ST_U64: return is_min ? FK_MIN_U64 : FK_MAX_U64;
+    case ST_F32: return is_min ? FK_MIN_F32 : FK_MAX_F32;
+    default: return is_min ? FK_MIN_F64 : FK_MAX_F64;
+  }
+}
+int32_t field_width(int32_t fk)
+{
+  switch (fk) {
+    case FK_SUM_F32:
+    case FK_SUMSQ_F32:
+    case FK_MIN_S32:
+    case FK_MAX_S32:
+    case FK_MIN_U32:
+    case FK_MAX_U32:
+    case FK_MIN_F32:
+    case FK_MAX_F32:
+    case FK_COUNT: return 4;
+    default: return 8;
+  }
+}
+
+uint64_t next_pow2(uint64_t x)
+{
+  uint64_t p = 1;
+  while (p < x) p <<= 1;
+  return p;
+}
+
+// Distinct values D of a population from `d` distinct among `n` samples,
+// assuming equally frequent values: solve D (1 - exp(-n / D)) = d.
+double estimate_distinct(double n, double d, double population)
+{
+  if (d >= n) return population;
+  double lo = d, hi = population;
+  if (hi * (1.0 - std::exp(-n / hi)) <= d) return population;
+  for (int it = 0; it < 100; it++) {
+    double mid = 0.5 * (lo + hi);
+    double f   = mid * (1.0 - std::exp(-n / mid));
+    if (f < d) lo = mid;
+    else hi = mid;
+  }
+  return hi;
+}
+
+constexpr int64_t SAMPLE_ROWS = 65536;
+
+owned_column make_owned_fixed(device_allocator& A, int32_t type_id, int64_t n, bool with_mask)
+{
+  owned_column oc;
+  oc.col.type_id    = type_id;
+  oc.col.size       = int32_t(n);
+  oc.data_bytes     = size_t(type_width(type_id)) * size_t(n);
+  oc.col.data       = n > 0 ? A.allocate(oc.data_bytes) : nullptr;
+  oc.col.data_bytes = int64_t(oc.data_bytes);
+  if (with_mask) {
+    oc.mask_bytes    = std::max<size_t>(4, size_t((n + 31) / 32) * 4);
+    oc.col.null_mask = static_cast<uint32_t*>(A.allocate(oc.mask_bytes));
+    if (n == 0) MIG_HIP_TRY(hipMemsetAsync(oc.col.null_mask, 0, oc.mask_bytes, A.stream));
+  }
+  return oc;
+}
+
+owned_column make_empty_column(device_allocator& A, int32_t type_id)
+{
+  owned_column oc;
+  oc.col.type_id = type_id;
+  if (type_id == MIG_TYPE_STRING) {
+    oc.col.offsets_type_id = MIG_TYPE_INT32;
+    oc.offsets_bytes       = sizeof(int32_t);
+    oc.col.offsets         = A.allocate(oc.offsets_bytes);
+    MIG_HIP_TRY(hipMemsetAsync(oc.col.offsets, 0, oc.offsets_bytes, A.stream));
+  }
+  return oc;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+void hash_rows(mig_column_view const* cols, int32_t ncols, uint32_t seed, uint32_t* out,
+               hipStream_t stream)
+{
+  MIG_EXPECTS(ncols >= 0 && ncols <= MAX_KEYS, "too many columns to hash");
+  if (ncols == 0) return;
+  int64_t const n = cols[0].size;
+  for (int i = 0; i < ncols; i++) {
+    MIG_EXPECTS(cols[i].size == n, "all columns must have the same number of rows");
+    if (storage_of(cols[i].type_id) == ST_NONE)
+      fail(MIG_ERROR_DATA_TYPE, "unsupported column type " + std::to_string(cols[i].type_id) + " for hashing");
+  }
+  if (n == 0) return;
+  dcol h[MAX_KEYS];
+  for (int i = 0; i < ncols; i++) h[i] = make_dcol(cols[i], null_count_of(cols[i], stream) > 0);
+  scratch S(stream);
+  dcol* d = S.get<dcol>(ncols);
+  void* pin = thread_pinned().get(sizeof(h));
+  std::memcpy(pin, h, sizeof(dcol) * ncols);
+  MIG_HIP_TRY(hipMemcpyAsync(d, pin, sizeof(dcol) * ncols, hipMemcpyHostToDevice, stream));
+  launch_hash_rows(d, ncols, n, seed, out, stream);
+  MIG_HIP_TRY(hipGetLastError());
+  MIG_HIP_TRY(hipStreamSynchronize(stream));
+}
+
+// ---------------------------------------------------------------------------
+std::unique_ptr<mig_groupby_result> groupby_aggregate(mig_column_view const* keys, int32_t num_keys,
+                                                      int32_t null_policy,
+                                                      mig_aggregation_request const* requests,
+                                                      int32_t num_requests, hipStream_t stream,
+                                                      mig_memory_resource const* mr)
+{
+  double* stats = thread_stats();
+  std::fill(stats, stats + STAT_COUNT, 0.0);
+
+  MIG_EXPECTS(num_keys >= 0 && num_keys <= MAX_KEYS, "number of key columns out of range");
+  MIG_EXPECTS(num_requests >= 0, "negative number of requests");
+  MIG_EXPECTS(null_policy == MIG_NULL_EXCLUDE || null_policy == MIG_NULL_INCLUDE, "invalid null policy");
+  int64_t const N = num_keys > 0 ? keys[0].size : 0;
+  for (int i = 0; i < num_keys; i++) {
+    MIG_EXPECTS(keys[i].size == N, "key columns must have the same number of rows");
+    int32_t st = storage_of(keys[i].type_id);
+    if (st == ST_NONE)
+      fail(MIG_ERROR_DATA_TYPE, "unsupported key column type " + std::to_string(keys[i].type_id));
+  }
+  // groupby.cu:210-214 and :175-192
+  for (int r = 0; r < num_requests; r++) {
+    MIG_EXPECTS(requests[r].values.size == N,
+                "Size mismatch between request values and groupby keys.");
+    MIG_EXPECTS(requests[r].num_aggregations >= 0, "negative number of aggregations");
+    for (int a = 0; a < requests[r].num_aggregations; a++)
+      check_supported(requests[r].values.type_id, requests[r].aggregations[a].kind);
+  }
+
+  auto res = std::make_unique<mig_groupby_result>();
+  res->alloc.stream = stream;
+  if (mr) {
+    res->alloc.mr     = *mr;
+    res->alloc.has_mr = true;
+  }
+  device_allocator& A = res->alloc;
+
+  // ---- empty input: groupby.cu:150-172 ---------------------------------
+  if (N == 0) {
+    for (int i = 0; i < num_keys; i++) res->keys.push_back(make_empty_column(A, keys[i].type_id));
+    for (int r = 0; r < num_requests; r++) {
+      std::vector<owned_column> cols;
+      for (int a = 0; a < requests[r].num_aggregations; a++)
+        cols.push_back(make_empty_column(
+          A, target_type_of(requests[r].values.type_id, requests[r].aggregations[a].kind)));
+      res->results.push_back(std::move(cols));
+    }
+    MIG_HIP_TRY(hipStreamSynchronize(stream));
+    return res;
+  }
+
+  device_props const& dp = props();
+  auto P_ptr = std::make_unique<gb_params>();
+  gb_params& P = *P_ptr;
+  std::memset(&P, 0, sizeof(gb_params));
+  P.num_rows = N;
+  P.num_keys = num_keys;
+
+  // ---- keys -------------------------------------------------------------
+  bool any_key_nulls = false;
+  for (int i = 0; i < num_keys; i++) {
+    bool has_nulls = null_count_of(keys[i], stream) > 0;
+    any_key_nulls |= has_nulls;
+    P.keys[i] = make_dcol(keys[i], has_nulls);
+  }
+  P.skip_null_keys = (null_policy == MIG_NULL_EXCLUDE) && any_key_nulls;
+  bool const inline_keys = num_keys == 1 && storage_of(keys[0].type_id) != ST_STRING;
+  P.key_mode    = inline_keys ? KM_INLINE : KM_GENERAL;
+  P.key_storage = inline_keys ? storage_of(keys[0].type_id) : ST_NONE;
+
+  // ---- flatten requests into per-column accumulators -------------------
+  std::vector<value_plan> vals;
+  std::vector<std::vector<int>> req_vcol(num_requests);
+  std::vector<int> vcol_of_request(num_requests);
+  for (int r = 0; r < num_requests; r++) {
+    mig_column_view const& v = requests[r].values;
+    int idx = -1;
+    for (size_t j = 0; j < vals.size(); j++)
+      if (same_view(vals[j].view, v)) idx = int(j);
+    if (idx < 0) {
+      value_plan vp;
+      vp.view      = v;
+      vp.storage   = storage_of(v.type_id);
+      vp.has_nulls = null_count_of(v, stream) > 0;
+      vals.push_back(vp);
+      idx = int(vals.size()) - 1;
+    }
+    vcol_of_request[r] = idx;
+    value_plan& vp = vals[idx];
+    for (int a = 0; a < requests[r].num_aggregations; a++) {
+      switch (requests[r].aggregations[a].kind) {
+        case MIG_AGG_SUM: vp.want_sum = true; vp.want_count |= vp.has_nulls; break;
+        case MIG_AGG_SUM_OF_SQUARES: vp.want_sumsq = true; vp.want_count |= vp.has_nulls; break;
+        case MIG_AGG_MIN: vp.want_min = true; vp.want_count |= vp.has_nulls; break;
+        case MIG_AGG_MAX: vp.want_max = true; vp.want_count |= vp.has_nulls; break;
+        case MIG_AGG_COUNT_VALID: vp.want_count = true; break;
+        case MIG_AGG_COUNT_ALL: P.need_count_all = 1; break;
+        case MIG_AGG_MEAN: vp.want_sum = vp.want_count = true; break;
+        case MIG_AGG_VARIANCE:
+        case MIG_AGG_STD: vp.want_sum = vp.want_count = vp.want_m2 = true; break;
+        default: break;
+      }
+    }
+  }
+  MIG_EXPECTS(vals.size() <= size_t(MAX_VALS), "too many distinct value columns");
+
+  // A column without nulls counts its valid values with the group row count.
+  int const COUNT_FROM_ROWS = -2;
+  int nf = 0;
+  auto add_field = [&](int vc, int32_t kind) {
+    MIG_EXPECTS(nf < MAX_FIELDS, "too many accumulators");
+    field_desc& F = P.fields[nf];
+    F.kind        = kind;
+    F.vcol        = vc;
+    F.width       = field_width(kind);
+    F.aux_sum     = -1;
+    F.aux_count   = -1;
+    return nf++;
+  };
+  for (size_t j = 0; j < vals.size(); j++) {
+    value_plan& vp = vals[j];
+    P.val_field_begin[j] = nf;
+    P.vals[j]            = make_dcol(vp.view, vp.has_nulls);
+    if (vp.want_sum) vp.f_sum = add_field(int(j), sum_field_kind(vp.storage));
+    if (vp.want_sumsq) vp.f_sumsq = add_field(int(j), sumsq_field_kind(vp.storage));
+    if (vp.want_min) vp.f_min = add_field(int(j), minmax_field_kind(vp.storage, true));
+    if (vp.want_max) vp.f_max = add_field(int(j), minmax_field_kind(vp.storage, false));
+    if (vp.want_count) {
+      if (vp.has_nulls) vp.f_count = add_field(int(j), FK_COUNT);
+      else {
+        vp.f_count       = COUNT_FROM_ROWS;
+        P.need_count_all = 1;
+      }
+    }
+    if (vp.want_m2) {
+      vp.f_m2                   = add_field(int(j), FK_M2);
+      P.fields[vp.f_m2].aux_sum   = vp.f_sum;
+      P.fields[vp.f_m2].aux_count = vp.f_count;
+      P.need_second_pass        = 1;
+    }
+  }
+  P.val_field_begin[vals.size()] = nf;
+  P.num_vals   = int32_t(vals.size());
+  P.num_fields = nf;
+
+  // ---- capacity -----------------------------------------------------------
+  scratch S(stream);
+  gb_params* dP = S.get<gb_params>(1);
+  auto* pin = static_cast<uint8_t*>(thread_pinned().get(2 * sizeof(gb_params) + 4 * SAMPLE_ROWS + 64));
+  auto upload = [&](int which) {
+    void* h = pin + which * sizeof(gb_params);
+    std::memcpy(h, &P, sizeof(gb_params));
+    MIG_HIP_TRY(hipMemcpyAsync(dP, h, sizeof(gb_params), hipMemcpyHostToDevice, stream));
+  };
+  phase_timer T(stream);
+  T.mark(0);
+  uint64_t const max_cap = next_pow2(uint64_t(2 * N + 2));
+  double est = double(N);
+  if (N > 4096) {
+    int64_t ns = std::min<int64_t>(N, SAMPLE_ROWS);
+    uint32_t* d_sample = S.get<uint32_t>(ns);
+    upload(0);
+    launch_sample_hash(dP, N, ns, d_sample, stream);
+    auto* h_sample = reinterpret_cast<uint32_t*>(pin + 2 * sizeof(gb_params));
+    MIG_HIP_TRY(hipMemcpyAsync(h_sample, d_sample, 4 * ns, hipMemcpyDeviceToHost, stream));
+    MIG_HIP_TRY(hipStreamSynchronize(stream));
+    std::vector<uint32_t> hs(h_sample, h_sample + ns);
+    std::sort(hs.begin(), hs.end());
+    double d = double(std::unique(hs.begin(), hs.end()) - hs.begin());
+    est = ns == N ? d : estimate_distinct(double(ns), d, double(N));
+  }
+  stats[STAT_EST_GROUPS] = est;
+  T.mark(1);
+  uint64_t cap = std::max<uint64_t>(64, next_pow2(uint64_t(est * 2.6) + 1));
+  cap = std::min(cap, max_cap);
+
+  // ---- build + accumulate, growing the table on overflow ---------------------
+  int32_t* d_flags = S.get<int32_t>(4);  // [0] overflow, [1..2] special occupancy
+  uint32_t* map = nullptr;
+  uint32_t* block_counts = nullptr;
+  int64_t G = 0;
+  int retries = 0;
+  for (;;) {
+    P.cap         = cap;
+    P.cap_mask    = cap - 1;
+    P.probe_limit = uint32_t(cap >= max_cap ? std::min<uint64_t>(cap, 0xffffffffull)
+                                            : std::min<uint64_t>(cap, 4096));
+    P.overflow    = d_flags;
+    P.special     = reinterpret_cast<uint32_t*>(d_flags + 1);
+    size_t state_bytes = 0;
+    uint64_t const total = cap + 2;
+    if (P.key_mode == KM_INLINE) P.slot_keys = S.get<uint64_t>(cap);
+    else P.slot_rows = S.get<uint64_t>(cap);
+    state_bytes += 8 * cap;
+    P.slot_count = P.need_count_all ? S.get<uint32_t>(total) : nullptr;
+    if (P.slot_count) state_bytes += 4 * total;
+    for (int f = 0; f < nf; f++) {
+      P.fields[f].ptr = P.fields[f].width == 8 ? static_cast<void*>(S.get<uint64_t>(total))
+                                               : static_cast<void*>(S.get<uint32_t>(total));
+      state_bytes += size_t(P.fields[f].width) * total;
+    }
+    stats[STAT_STATE_BYTES] = double(state_bytes);
+    upload(0);
+    launch_init_state(dP, P, stream);
+    T.mark(2);
+    launch_aggregate(dP, P, dp.num_cus, stream);
+    T.mark(3);
+    MIG_HIP_TRY(hipGetLastError());
+
+    int64_t nb   = (int64_t(cap) + 2 + SCAN_TILE - 1) / SCAN_TILE;
+    block_counts = S.get<uint32_t>(nb + 1);
+    launch_count_occupied(dP, P, block_counts, stream);
+    launch_scan_blocks(block_counts, nb, block_counts + nb, stream);
+    auto* h_flags = reinterpret_cast<uint32_t*>(pin + 2 * sizeof(gb_params));
+    MIG_HIP_TRY(hipMemcpyAsync(h_flags, d_flags, 4, hipMemcpyDeviceToHost, stream));
+    MIG_HIP_TRY(hipMemcpyAsync(h_flags + 1, block_counts + nb, 4, hipMemcpyDeviceToHost, stream));
+    MIG_HIP_TRY(hipStreamSynchronize(stream));
+    if (h_flags[0] == 0) {
+      G = h_flags[1];
+      break;
+    }
+    MIG_EXPECTS(cap < max_cap, "hash table overflow at maximum capacity");
+    cap = std::min(max_cap, cap * 4);
+    retries++;
+  }
+  stats[STAT_KEY_MODE] = P.key_mode;
+  stats[STAT_CAPACITY] = double(cap);
+  stats[STAT_RETRIES]  = retries;
+  stats[STAT_GROUPS]   = double(G);
+
+  T.mark(4);
+  if (P.need_second_pass) launch_second_pass(dP, P, dp.num_cus, stream);
+  T.mark(5);
+  map = S.get<uint32_t>(G);
+  launch_compact(dP, P, block_counts, map, stream);
+  T.mark(6);
+
+  // ---- outputs ----------------------------------------------------------------
+  res->num_groups = int32_t(G);
+  // keys
+  int n_str_keys = 0;
+  for (int i = 0; i < num_keys; i++) {
+    bool const with_mask = keys[i].null_mask != nullptr;
+    if (storage_of(keys[i].type_id) == ST_STRING) {
+      owned_column oc;
+      oc.col.type_id         = MIG_TYPE_STRING;
+      oc.col.size            = int32_t(G);
+      oc.col.offsets_type_id = MIG_TYPE_INT32;
+      oc.offsets_bytes       = sizeof(int32_t) * size_t(G + 1);
+      oc.col.offsets         = A.allocate(oc.offsets_bytes);
+      if (with_mask) {
+        oc.mask_bytes    = std::max<size_t>(4, size_t((G + 31) / 32) * 4);
+        oc.col.null_mask = static_cast<uint32_t*>(A.allocate(oc.mask_bytes));
+      }
+      res->keys.push_back(oc);
+      n_str_keys++;
+    } else {
+      res->keys.push_back(make_owned_fixed(A, keys[i].type_id, G, with_mask));
+    }
+    P.out_keys[i]     = storage_of(keys[i].type_id) == ST_STRING ? nullptr : res->keys.back().col.data;
+    P.out_key_mask[i] = res->keys.back().col.null_mask;
+  }
+  // aggregation results in request order (duplicates computed again from the
+  // same accumulators, as extract_results copies them)
+  int no = 0;
+  for (int r = 0; r < num_requests; r++) {
+    value_plan const& vp = vals[vcol_of_request[r]];
+    std::vector<owned_column> cols;
+    for (int a = 0; a < requests[r].num_aggregations; a++) {
+      int32_t const kind = requests[r].aggregations[a].kind;
+      int32_t const tt   = target_type_of(vp.view.type_id, kind);
+      MIG_EXPECTS(no < MAX_OUTS, "too many aggregation results");
+      out_desc& O   = P.outs[no++];
+      O.count_field = -1;
+      O.src_storage = vp.storage;
+      O.ddof        = requests[r].aggregations[a].ddof;
+      O.out_width   = type_width(tt);
+      bool with_mask = false;
+      switch (kind) {
+        case MIG_AGG_SUM:
+        case MIG_AGG_SUM_OF_SQUARES:
+          O.field = kind == MIG_AGG_SUM ? vp.f_sum : vp.f_sumsq;
+          O.kind  = tt == MIG_TYPE_FLOAT32 ? OK_COPY_F32 : (tt == MIG_TYPE_FLOAT64 ? OK_COPY_F64 : OK_COPY_I64);
+          if (vp.has_nulls) { O.count_field = vp.f_count; with_mask = true; }
+          break;
+        case MIG_AGG_MIN:
+        case MIG_AGG_MAX:
+          O.field = kind == MIG_AGG_MIN ? vp.f_min : vp.f_max;
+          O.kind  = OK_MINMAX;
+          if (vp.has_nulls) { O.count_field = vp.f_count; with_mask = true; }
+          break;
+        case MIG_AGG_COUNT_VALID:
+          if (vp.f_count >= 0) { O.kind = OK_COUNT; O.field = vp.f_count; }
+          else O.kind = OK_COUNT_ALL;
+          break;
+        case MIG_AGG_COUNT_ALL: O.kind = OK_COUNT_ALL; break;
+        case MIG_AGG_MEAN:
+          O.kind = OK_MEAN;
+          O.field = vp.f_sum;
+          O.count_field = vp.f_count;
+          with_mask = vp.has_nulls;
+          break;
+        case MIG_AGG_VARIANCE:
+        case MIG_AGG_STD:
+          O.kind = kind == MIG_AGG_VARIANCE ? OK_VAR : OK_STD;
+          O.field = vp.f_m2;
+          O.count_field = vp.f_count;
+          with_mask = true;
+          break;
+        default: fail(MIG_ERROR_LOGIC, "unreachable aggregation kind");
+      }
+      owned_column oc = make_owned_fixed(A, tt, G, with_mask);
+      O.data = oc.col.data;
+      O.mask = oc.col.null_mask;
+      cols.push_back(oc);
+    }
+    res->results.push_back(std::move(cols));
+  }
+  P.num_outs = no;
+
+  uint32_t* rep_rows = P.key_mode == KM_GENERAL ? S.get<uint32_t>(G) : nullptr;
+  upload(1);  // same device block: the previous kernels are stream-ordered before this copy
+  launch_outputs(dP, P, map, G, rep_rows, stream);
+  if (P.key_mode == KM_GENERAL) {
+    int32_t* str_sizes = n_str_keys > 0 ? S.get<int32_t>(int64_t(n_str_keys) * G + 1) : nullptr;
+    launch_gather_keys(dP, P, rep_rows, G, str_sizes, stream);
+    int si = 0;
+    for (int i = 0; i < num_keys; i++) {
+      if (storage_of(keys[i].type_id) != ST_STRING) continue;
+      owned_column& oc = res->keys[i];
+      auto* offs       = static_cast<int32_t*>(oc.col.offsets);
+      if (G == 0) {
+        MIG_HIP_TRY(hipMemsetAsync(offs, 0, sizeof(int32_t), stream));
+        si++;
+        continue;
+      }
+      MIG_HIP_TRY(hipMemcpyAsync(offs, str_sizes + int64_t(si) * G, sizeof(int32_t) * G,
+                                 hipMemcpyDeviceToDevice, stream));
+      launch_exclusive_scan_i32(offs, G, offs + G, stream);
+      int32_t total_chars = 0;
+      auto* h = reinterpret_cast<int32_t*>(pin + 2 * sizeof(gb_params));
+      MIG_HIP_TRY(hipMemcpyAsync(h, offs + G, 4, hipMemcpyDeviceToHost, stream));
+      MIG_HIP_TRY(hipStreamSynchronize(stream));
+      total_chars       = *h;
+      MIG_EXPECTS(total_chars >= 0, "unique string keys exceed 2^31 characters");
+      oc.data_bytes     = size_t(total_chars);
+      oc.col.data       = A.allocate(oc.data_bytes);
+      oc.col.data_bytes = total_chars;
+      launch_gather_string_chars(P.keys[i], rep_rows, G, offs, static_cast<uint8_t*>(oc.col.data), stream);
+      si++;
+    }
+  }
+  MIG_HIP_TRY(hipGetLastError());
+
+  // null counts of the outputs are computed on the device, then read back
+  std::vector<mig_column*> masked;
+  for (auto& k : res->keys)
+    if (k.col.null_mask) masked.push_back(&k.col);
+  for (auto& rc : res->results)
+    for (auto& c : rc)
+      if (c.col.null_mask) masked.push_back(&c.col);
+  uint32_t* d_valid = masked.empty() ? nullptr : S.get<uint32_t>(masked.size());
+  if (d_valid) {
+    MIG_HIP_TRY(hipMemsetAsync(d_valid, 0, 4 * masked.size(), stream));
+    for (size_t m = 0; m < masked.size(); m++) {
+      dcol c{};
+      c.mask = masked[m]->null_mask;
+      launch_count_nulls(c, G, d_valid + m, stream);
+    }
+    std::vector<uint32_t> hv(masked.size());
+    MIG_HIP_TRY(hipMemcpyAsync(hv.data(), d_valid, 4 * masked.size(), hipMemcpyDeviceToHost, stream));
+    MIG_HIP_TRY(hipStreamSynchronize(stream));
+    for (size_t m = 0; m < masked.size(); m++) masked[m]->null_count = int32_t(G - hv[m]);
+  }
+  T.mark(7);
+  MIG_HIP_TRY(hipStreamSynchronize(stream));
+  stats[STAT_T_ESTIMATE]    = T.between(0, 1);
+  stats[STAT_T_INIT]        = T.between(1, 2);
+  stats[STAT_T_ACCUMULATE]  = T.between(2, 3);
+  stats[STAT_T_SECOND_PASS] = T.between(4, 5);
+  stats[STAT_T_COMPACT]     = T.between(3, 4) + T.between(5, 6);
+  stats[STAT_T_OUTPUTS]     = T.between(6, 7);
+  return res;
+}
+
+}  // namespace mig
+
+mig_groupby_result::~mig_groupby_result()
+{
+  auto release = [&](mig::owned_column& c) {
+    alloc.deallocate(c.col.data, c.data_bytes);
+    alloc.deallocate(c.col.null_mask, c.mask_bytes);
+    alloc.deallocate(c.col.offsets, c.offsets_bytes);
+    c = mig::owned_column{};
+  };
+  for (auto& k : keys) release(k);
+  for (auto& r : results)
+    for (auto& c : r) release(c);
+}

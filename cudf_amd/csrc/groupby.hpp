@@ -1,0 +1,68 @@
+// Host-side group-by driver (C++), called by the C ABI in capi.cpp.
+#pragma once
+
+#include <memory>
+#include <vector>
+
+#include "common.hpp"
+
+namespace mig {
+
+// A column owned by a result: every buffer came from `alloc`.
+struct owned_column {
+  mig_column col{};
+  size_t data_bytes    = 0;
+  size_t mask_bytes    = 0;
+  size_t offsets_bytes = 0;
+};
+
+struct device_allocator {
+  mig_memory_resource mr{};
+  bool has_mr = false;
+  hipStream_t stream = nullptr;
+  void* allocate(size_t bytes);
+  void deallocate(void* p, size_t bytes);
+};
+
+}  // namespace mig
+
+struct mig_groupby_result {
+  int32_t num_groups = 0;
+  std::vector<mig::owned_column> keys;
+  std::vector<std::vector<mig::owned_column>> results;
+  mig::device_allocator alloc;
+  ~mig_groupby_result();
+};
+
+namespace mig {
+
+// Statistics of the last call on this thread (exposed by mig_last_stats).
+enum stat_index : int {
+  STAT_STRATEGY = 0,   // 0 = global table, 1 = LDS pre-aggregation
+  STAT_KEY_MODE,       // key_mode
+  STAT_CAPACITY,       // table slots
+  STAT_EST_GROUPS,     // estimated groups
+  STAT_GROUPS,         // actual groups
+  STAT_RETRIES,        // table rebuilds after overflow
+  STAT_STATE_BYTES,    // bytes of table + accumulators
+  STAT_T_ESTIMATE,     // phase times (ms) when phase timing is on
+  STAT_T_INIT,
+  STAT_T_ACCUMULATE,
+  STAT_T_SECOND_PASS,
+  STAT_T_COMPACT,
+  STAT_T_OUTPUTS,
+  STAT_COUNT
+};
+double* thread_stats();
+void set_phase_timing(bool on);
+
+std::unique_ptr<mig_groupby_result> groupby_aggregate(mig_column_view const* keys, int32_t num_keys,
+                                                      int32_t null_policy,
+                                                      mig_aggregation_request const* requests,
+                                                      int32_t num_requests, hipStream_t stream,
+                                                      mig_memory_resource const* mr);
+
+void hash_rows(mig_column_view const* cols, int32_t ncols, uint32_t seed, uint32_t* out,
+               hipStream_t stream);
+
+}  // namespace mig

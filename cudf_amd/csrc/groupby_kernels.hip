@@ -1,0 +1,900 @@
+// gfx950 kernels of the hash group-by.
+//
+// Row hashing follows cudf/table/experimental/row_operators.cuh:1858-1883 and
+// the MurmurHash3_x86_32 element hash; equality follows row_operators.cuh:
+// 1352-1364, 1429-1448 (nulls equal, NaN equal NaN, -0.0 equal 0.0).  The
+// accumulate step replaces compute_single_pass_aggs_fn
+// (groupby/hash/groupby_kernels.cuh:98-105) and the variance pass replaces
+// var_hash_functor (groupby/hash/multi_pass_kernels.cuh:74-111).
+#include "kernels.hpp"
+#include "murmur.hpp"
+
+namespace mig {
+
+constexpr int BLOCK = 256;
+
+// ---------------------------------------------------------------------------
+// atomics (agent scope: the table is shared by all XCDs of the device)
+template <typename T>
+__device__ __forceinline__ T aload(T const* p)
+{
+  return __hip_atomic_load(const_cast<T*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ void astore(T* p, T v)
+{
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint64_t acas64(uint64_t* p, uint64_t expected, uint64_t desired)
+{
+  __hip_atomic_compare_exchange_strong(p, &expected, desired, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+  return expected;
+}
+template <typename T>
+__device__ __forceinline__ void aadd(T* p, T v)
+{
+  __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ void amin(T* p, T v)
+{
+  __hip_atomic_fetch_min(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ void amax(T* p, T v)
+{
+  __hip_atomic_fetch_max(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Order-preserving unsigned encodings of float bit patterns.
+__device__ __forceinline__ uint32_t ord32(uint32_t b) { return (b & 0x80000000u) ? ~b : (b | 0x80000000u); }
+__device__ __forceinline__ uint32_t unord32(uint32_t o) { return (o & 0x80000000u) ? (o & 0x7fffffffu) : ~o; }
+__device__ __forceinline__ uint64_t ord64(uint64_t b)
+{
+  return (b & 0x8000000000000000ull) ? ~b : (b | 0x8000000000000000ull);
+}
+__device__ __forceinline__ uint64_t unord64(uint64_t o)
+{
+  return (o & 0x8000000000000000ull) ? (o & 0x7fffffffffffffffull) : ~o;
+}
+
+// ---------------------------------------------------------------------------
+// element access
+__device__ __forceinline__ uint64_t load_fixed_bits(dcol const& c, int64_t row)
+{
+  switch (c.storage) {
+    case ST_I8:
+    case ST_U8: return c.data[row];
+    case ST_BOOL: return c.data[row] != 0;
+    case ST_I16:
+    case ST_U16: return reinterpret_cast<uint16_t const*>(c.data)[row];
+    case ST_I32:
+    case ST_U32: return reinterpret_cast<uint32_t const*>(c.data)[row];
+    case ST_F32: return normalize_f32_bits(reinterpret_cast<uint32_t const*>(c.data)[row]);
+    case ST_I64:
+    case ST_U64: return reinterpret_cast<uint64_t const*>(c.data)[row];
+    case ST_F64: return normalize_f64_bits(reinterpret_cast<uint64_t const*>(c.data)[row]);
+    default: return 0;
+  }
+}
+
+__device__ __forceinline__ uint32_t hash_fixed_bits(int32_t st, uint64_t bits, uint32_t seed)
+{
+  switch (storage_width(st)) {
+    case 1: return mm_hash_small(static_cast<uint32_t>(bits), 1, seed);
+    case 2: return mm_hash_small(static_cast<uint32_t>(bits), 2, seed);
+    case 4: return mm_hash4(static_cast<uint32_t>(bits), seed);
+    default: return mm_hash8(bits, seed);
+  }
+}
+
+__device__ __forceinline__ void str_span(dcol const& c, int64_t row, int64_t& b, int64_t& e)
+{
+  if (c.offsets64) {
+    auto o = reinterpret_cast<int64_t const*>(c.offsets);
+    b = o[row];
+    e = o[row + 1];
+  } else {
+    auto o = reinterpret_cast<int32_t const*>(c.offsets);
+    b = o[row];
+    e = o[row + 1];
+  }
+}
+
+__device__ __forceinline__ uint32_t elem_hash(dcol const& c, int64_t row, uint32_t seed)
+{
+  if (!dcol_valid(c, row)) return NULL_HASH;
+  if (c.storage == ST_STRING) {
+    int64_t b, e;
+    str_span(c, row, b, e);
+    return mm_hash_bytes(c.data + b, e - b, seed);
+  }
+  return hash_fixed_bits(c.storage, load_fixed_bits(c, row), seed);
+}
+
+__device__ __forceinline__ uint32_t row_hash(dcol const* cols, int ncols, int64_t row, uint32_t seed)
+{
+  uint32_t h = seed;
+  for (int c = 0; c < ncols; c++) h = hash_combine(h, elem_hash(cols[c], row, seed));
+  return h;
+}
+
+__device__ __forceinline__ bool elem_equal(dcol const& c, int64_t a, int64_t b)
+{
+  bool const va = dcol_valid(c, a), vb = dcol_valid(c, b);
+  if (!va || !vb) return va == vb;
+  if (c.storage == ST_STRING) {
+    int64_t ab, ae, bb, be;
+    str_span(c, a, ab, ae);
+    str_span(c, b, bb, be);
+    if (ae - ab != be - bb) return false;
+    for (int64_t i = 0; i < ae - ab; i++)
+      if (c.data[ab + i] != c.data[bb + i]) return false;
+    return true;
+  }
+  return load_fixed_bits(c, a) == load_fixed_bits(c, b);
+}
+
+__device__ __forceinline__ bool rows_equal(gb_params const& P, int64_t a, int64_t b)
+{
+  for (int c = 0; c < P.num_keys; c++)
+    if (!elem_equal(P.keys[c], a, b)) return false;
+  return true;
+}
+
+__device__ __forceinline__ bool any_key_null(gb_params const& P, int64_t row)
+{
+  for (int c = 0; c < P.num_keys; c++)
+    if (!dcol_valid(P.keys[c], row)) return true;
+  return false;
+}
+
+// ---------------------------------------------------------------------------
+// public row hash: cudf::hashing::murmurhash3_x86_32 (src/hash/murmurhash3_x86_32.cu:34-57)
+__global__ void __launch_bounds__(BLOCK)
+  k_hash_rows(dcol const* __restrict__ cols, int ncols, int64_t n, uint32_t seed, uint32_t* out)
+{
+  for (int64_t r = int64_t(blockIdx.x) * BLOCK + threadIdx.x; r < n; r += int64_t(gridDim.x) * BLOCK)
+    out[r] = row_hash(cols, ncols, r, seed);
+}
+
+void launch_hash_rows(dcol const* d_cols, int ncols, int64_t n, uint32_t seed, uint32_t* out,
+                      hipStream_t s)
+{
+  if (n <= 0) return;
+  int64_t blocks = (n + BLOCK - 1) / BLOCK;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(k_hash_rows, dim3(blocks), dim3(BLOCK), 0, s, d_cols, ncols, n, seed, out);
+}
+
+// ---------------------------------------------------------------------------
+// cardinality sample: row hashes of `nsample` evenly strided rows
+__global__ void __launch_bounds__(BLOCK)
+  k_sample_hash(gb_params const* __restrict__ Pp, int64_t n, int64_t nsample, uint32_t* out)
+{
+  gb_params const& P = *Pp;
+  int64_t i = int64_t(blockIdx.x) * BLOCK + threadIdx.x;
+  if (i >= nsample) return;
+  int64_t row = (i * n) / nsample;
+  out[i] = row_hash(P.keys, P.num_keys, row, 0);
+}
+
+void launch_sample_hash(gb_params const* dP, int64_t n, int64_t nsample, uint32_t* out, hipStream_t s)
+{
+  int64_t blocks = (nsample + BLOCK - 1) / BLOCK;
+  hipLaunchKernelGGL(k_sample_hash, dim3(blocks), dim3(BLOCK), 0, s, dP, n, nsample, out);
+}
+
+// ---------------------------------------------------------------------------
+// state initialisation: table, row counts, special flags and accumulator identities
+__device__ __forceinline__ uint64_t field_identity(int32_t kind)
+{
+  switch (kind) {
+    case FK_MIN_S32: return 0x7fffffffull;
+    case FK_MAX_S32: return 0x80000000ull;
+    case FK_MIN_S64: return 0x7fffffffffffffffull;
+    case FK_MAX_S64: return 0x8000000000000000ull;
+    case FK_MIN_U32: return 0xffffffffull;
+    case FK_MIN_U64: return 0xffffffffffffffffull;
+    case FK_MIN_F32: return 0xff800000ull;              // ord32(+inf)
+    case FK_MAX_F32: return 0x007fffffull;              // ord32(-inf)
+    case FK_MIN_F64: return 0xfff0000000000000ull;      // ord64(+inf)
+    case FK_MAX_F64: return 0x000fffffffffffffull;      // ord64(-inf)
+    default: return 0;                                  // sums, counts, max of unsigned
+  }
+}
+
+__global__ void __launch_bounds__(BLOCK) k_init_state(gb_params const* __restrict__ Pp)
+{
+  gb_params const& P = *Pp;
+  int64_t const total = int64_t(P.cap) + 2;
+  int const which = blockIdx.y;  // 0 = table, 1 = row counts, 2.. = fields
+  int64_t const stride = int64_t(gridDim.x) * BLOCK;
+  int64_t i = int64_t(blockIdx.x) * BLOCK + threadIdx.x;
+  if (which == 0) {
+    if (P.key_mode == KM_INLINE) {
+      for (int64_t j = i; j < int64_t(P.cap); j += stride) P.slot_keys[j] = EMPTY_KEY;
+    } else {
+      for (int64_t j = i; j < int64_t(P.cap); j += stride) P.slot_rows[j] = EMPTY_ROW;
+    }
+    if (i == 0) {
+      P.special[0] = 0;
+      P.special[1] = 0;
+      *P.overflow  = 0;
+    }
+  } else if (which == 1) {
+    if (P.slot_count)
+      for (int64_t j = i; j < total; j += stride) P.slot_count[j] = 0;
+  } else {
+    field_desc const& F = P.fields[which - 2];
+    uint64_t const id = field_identity(F.kind);
+    if (F.width == 8) {
+      auto p = reinterpret_cast<uint64_t*>(F.ptr);
+      for (int64_t j = i; j < total; j += stride) p[j] = id;
+    } else {
+      auto p = reinterpret_cast<uint32_t*>(F.ptr);
+      for (int64_t j = i; j < total; j += stride) p[j] = static_cast<uint32_t>(id);
+    }
+  }
+}
+
+void launch_init_state(gb_params const* dP, gb_params const& hP, hipStream_t s)
+{
+  int64_t total = int64_t(hP.cap) + 2;
+  int64_t blocks = (total + BLOCK - 1) / BLOCK;
+  if (blocks > 2048) blocks = 2048;
+  hipLaunchKernelGGL(k_init_state, dim3(blocks, 2 + hP.num_fields), dim3(BLOCK), 0, s, dP);
+}
+
+// ---------------------------------------------------------------------------
+// hash table probing
+__device__ __forceinline__ void mark_special(gb_params const& P, int which)
+{
+  if (aload(&P.special[which]) == 0u) astore(&P.special[which], 1u);
+}
+
+// Insert-or-find of an inline key; returns the slot, or -1 on overflow.
+__device__ __forceinline__ int64_t insert_inline(gb_params const& P, uint64_t key, uint32_t h)
+{
+  if (key == EMPTY_KEY) {
+    mark_special(P, 0);
+    return int64_t(P.cap);
+  }
+  uint64_t s = h & P.cap_mask;
+  for (uint32_t i = 0; i < P.probe_limit; i++) {
+    uint64_t cur = aload(&P.slot_keys[s]);
+    if (cur == key) return int64_t(s);
+    if (cur == EMPTY_KEY) {
+      uint64_t old = acas64(&P.slot_keys[s], EMPTY_KEY, key);
+      if (old == EMPTY_KEY || old == key) return int64_t(s);
+    }
+    s = (s + 1) & P.cap_mask;
+  }
+  astore(P.overflow, 1);
+  return -1;
+}
+
+__device__ __forceinline__ int64_t find_inline(gb_params const& P, uint64_t key, uint32_t h)
+{
+  if (key == EMPTY_KEY) return int64_t(P.cap);
+  uint64_t s = h & P.cap_mask;
+  for (uint32_t i = 0; i < P.probe_limit; i++) {
+    uint64_t cur = P.slot_keys[s];
+    if (cur == key) return int64_t(s);
+    if (cur == EMPTY_KEY) return -1;
+    s = (s + 1) & P.cap_mask;
+  }
+  return -1;
+}
+
+__device__ __forceinline__ int64_t insert_general(gb_params const& P, int64_t row, uint32_t h)
+{
+  uint64_t const mine = (uint64_t(h) << 32) | uint32_t(row);
+  uint64_t s = h & P.cap_mask;
+  for (uint32_t i = 0; i < P.probe_limit; i++) {
+    uint64_t cur = aload(&P.slot_rows[s]);
+    if (cur == EMPTY_ROW) {
+      uint64_t old = acas64(&P.slot_rows[s], EMPTY_ROW, mine);
+      if (old == EMPTY_ROW) return int64_t(s);
+      cur = old;
+    }
+    if (uint32_t(cur >> 32) == h && rows_equal(P, int64_t(uint32_t(cur)), row)) return int64_t(s);
+    s = (s + 1) & P.cap_mask;
+  }
+  astore(P.overflow, 1);
+  return -1;
+}
+
+__device__ __forceinline__ int64_t find_general(gb_params const& P, int64_t row, uint32_t h)
+{
+  uint64_t s = h & P.cap_mask;
+  for (uint32_t i = 0; i < P.probe_limit; i++) {
+    uint64_t cur = P.slot_rows[s];
+    if (cur == EMPTY_ROW) return -1;
+    if (uint32_t(cur >> 32) == h && rows_equal(P, int64_t(uint32_t(cur)), row)) return int64_t(s);
+    s = (s + 1) & P.cap_mask;
+  }
+  return -1;
+}
+
+// Slot of `row`'s group (inserting it when INSERT), or -1 when the row is
+// skipped (null key under EXCLUDE) or the table overflowed.
+template <int KM, bool INSERT>
+__device__ __forceinline__ int64_t locate(gb_params const& P, int64_t row)
+{
+  if constexpr (KM == KM_INLINE) {
+    dcol const& kc = P.keys[0];
+    if (!dcol_valid(kc, row)) {
+      if (P.skip_null_keys) return -1;
+      if (INSERT) mark_special(P, 1);
+      return int64_t(P.cap) + 1;
+    }
+    uint64_t const bits = load_fixed_bits(kc, row);
+    uint32_t const h    = hash_combine(0u, hash_fixed_bits(kc.storage, bits, 0u));
+    return INSERT ? insert_inline(P, bits, h) : find_inline(P, bits, h);
+  } else {
+    if (P.skip_null_keys && any_key_null(P, row)) return -1;
+    uint32_t const h = row_hash(P.keys, P.num_keys, row, 0u);
+    return INSERT ? insert_general(P, row, h) : find_general(P, row, h);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// accumulate one row into the group's accumulators
+struct loaded_value {
+  int64_t i;   // integer sources: sign- or zero-extended
+  double d;    // value as double (all sources)
+  float f;     // float32 sources
+  uint64_t b;  // float sources: raw bits
+};
+
+__device__ __forceinline__ loaded_value load_value(dcol const& c, int64_t row)
+{
+  loaded_value v{0, 0.0, 0.0f, 0};
+  switch (c.storage) {
+    case ST_I8: v.i = reinterpret_cast<int8_t const*>(c.data)[row]; v.d = double(v.i); break;
+    case ST_I16: v.i = reinterpret_cast<int16_t const*>(c.data)[row]; v.d = double(v.i); break;
+    case ST_I32: v.i = reinterpret_cast<int32_t const*>(c.data)[row]; v.d = double(v.i); break;
+    case ST_I64: v.i = reinterpret_cast<int64_t const*>(c.data)[row]; v.d = double(v.i); break;
+    case ST_U8: v.i = c.data[row]; v.d = double(v.i); break;
+    case ST_BOOL: v.i = c.data[row] != 0; v.d = double(v.i); break;
+    case ST_U16: v.i = reinterpret_cast<uint16_t const*>(c.data)[row]; v.d = double(v.i); break;
+    case ST_U32: v.i = reinterpret_cast<uint32_t const*>(c.data)[row]; v.d = double(v.i); break;
+    case ST_U64: {
+      uint64_t u = reinterpret_cast<uint64_t const*>(c.data)[row];
+      v.i = int64_t(u);
+      v.d = double(u);
+      break;
+    }
+    case ST_F32: {
+      uint32_t u = reinterpret_cast<uint32_t const*>(c.data)[row];
+      v.b = u;
+      v.f = __uint_as_float(u);
+      v.d = double(v.f);
+      break;
+    }
+    case ST_F64: {
+      uint64_t u = reinterpret_cast<uint64_t const*>(c.data)[row];
+      v.b = u;
+      v.d = __longlong_as_double(int64_t(u));
+      break;
+    }
+    default: break;
+  }
+  return v;
+}
+
+__device__ __forceinline__ void apply_field(field_desc const& F, int64_t slot, loaded_value const& v)
+{
+  switch (F.kind) {
+    case FK_SUM_I64: aadd(reinterpret_cast<uint64_t*>(F.ptr) + slot, uint64_t(v.i)); break;
+    case FK_SUM_F32: unsafeAtomicAdd(reinterpret_cast<float*>(F.ptr) + slot, v.f); break;
+    case FK_SUM_F64: unsafeAtomicAdd(reinterpret_cast<double*>(F.ptr) + slot, v.d); break;
+    case FK_SUMSQ_I64: aadd(reinterpret_cast<uint64_t*>(F.ptr) + slot, uint64_t(v.i) * uint64_t(v.i)); break;
+    case FK_SUMSQ_F32: unsafeAtomicAdd(reinterpret_cast<float*>(F.ptr) + slot, v.f * v.f); break;
+    case FK_SUMSQ_F64: unsafeAtomicAdd(reinterpret_cast<double*>(F.ptr) + slot, v.d * v.d); break;
+    case FK_MIN_S32: amin(reinterpret_cast<int32_t*>(F.ptr) + slot, int32_t(v.i)); break;
+    case FK_MAX_S32: amax(reinterpret_cast<int32_t*>(F.ptr) + slot, int32_t(v.i)); break;
+    case FK_MIN_S64: amin(reinterpret_cast<int64_t*>(F.ptr) + slot, v.i); break;
+    case FK_MAX_S64: amax(reinterpret_cast<int64_t*>(F.ptr) + slot, v.i); break;
+    case FK_MIN_U32: amin(reinterpret_cast<uint32_t*>(F.ptr) + slot, uint32_t(v.i)); break;
+    case FK_MAX_U32: amax(reinterpret_cast<uint32_t*>(F.ptr) + slot, uint32_t(v.i)); break;
+    case FK_MIN_U64: amin(reinterpret_cast<uint64_t*>(F.ptr) + slot, uint64_t(v.i)); break;
+    case FK_MAX_U64: amax(reinterpret_cast<uint64_t*>(F.ptr) + slot, uint64_t(v.i)); break;
+    case FK_MIN_F32:
+      if (!isnan(v.f)) amin(reinterpret_cast<uint32_t*>(F.ptr) + slot, ord32(uint32_t(v.b)));
+      break;
+    case FK_MAX_F32:
+      if (!isnan(v.f)) amax(reinterpret_cast<uint32_t*>(F.ptr) + slot, ord32(uint32_t(v.b)));
+      break;
+    case FK_MIN_F64:
+      if (!isnan(v.d)) amin(reinterpret_cast<uint64_t*>(F.ptr) + slot, ord64(v.b));
+      break;
+    case FK_MAX_F64:
+      if (!isnan(v.d)) amax(reinterpret_cast<uint64_t*>(F.ptr) + slot, ord64(v.b));
+      break;
+    case FK_COUNT: aadd(reinterpret_cast<uint32_t*>(F.ptr) + slot, 1u); break;
+    default: break;  // FK_M2: second pass
+  }
+}
+
+__device__ __forceinline__ void update_row(gb_params const& P, int64_t slot, int64_t row)
+{
+  if (P.need_count_all) aadd(P.slot_count + slot, 1u);
+  for (int vc = 0; vc < P.num_vals; vc++) {
+    int const fb = P.val_field_begin[vc], fe = P.val_field_begin[vc + 1];
+    if (fb == fe) continue;
+    dcol const& c = P.vals[vc];
+    if (!dcol_valid(c, row)) continue;
+    loaded_value const v = load_value(c, row);
+    for (int f = fb; f < fe; f++) apply_field(P.fields[f], slot, v);
+  }
+}
+
+template <int KM>
+__global__ void __launch_bounds__(BLOCK) k_aggregate_global(gb_params const* __restrict__ Pp)
+{
+  gb_params const& P = *Pp;
+  int64_t const n = P.num_rows;
+  for (int64_t row = int64_t(blockIdx.x) * BLOCK + threadIdx.x; row < n;
+       row += int64_t(gridDim.x) * BLOCK) {
+    int64_t const slot = locate<KM, true>(P, row);
+    if (slot < 0) continue;
+    update_row(P, slot, row);
+  }
+}
+
+static int64_t row_grid(int64_t n, int num_cus, int per_cu)
+{
+  int64_t blocks = (n + BLOCK - 1) / BLOCK;
+  int64_t cap = int64_t(num_cus) * per_cu;
+  if (blocks > cap) blocks = cap;
+  if (blocks < 1) blocks = 1;
+  return blocks;
+}
+
+void launch_aggregate(gb_params const* dP, gb_params const& hP, int num_cus, hipStream_t s)
+{
+  if (hP.num_rows <= 0) return;
+  int64_t blocks = row_grid(hP.num_rows, num_cus, 16);
+  if (hP.key_mode == KM_INLINE)
+    hipLaunchKernelGGL(k_aggregate_global<KM_INLINE>, dim3(blocks), dim3(BLOCK), 0, s, dP);
+  else
+    hipLaunchKernelGGL(k_aggregate_global<KM_GENERAL>, dim3(blocks), dim3(BLOCK), 0, s, dP);
+}
+
+// ---------------------------------------------------------------------------
+// second pass: sum of squared deviations from the group mean (VARIANCE / STD)
+__device__ __forceinline__ double field_as_double(field_desc const& F, int64_t slot)
+{
+  switch (F.kind) {
+    case FK_SUM_I64: return double(reinterpret_cast<int64_t const*>(F.ptr)[slot]);
+    case FK_SUM_F32: return double(reinterpret_cast<float const*>(F.ptr)[slot]);
+    case FK_SUM_F64: return reinterpret_cast<double const*>(F.ptr)[slot];
+    case FK_COUNT: return double(reinterpret_cast<uint32_t const*>(F.ptr)[slot]);
+    default: return 0.0;
+  }
+}
+
+// Count accumulator `f`, or the group row count when f == -2 (columns
+// without nulls count their values with the row count).
+__device__ __forceinline__ uint32_t count_of(gb_params const& P, int32_t f, int64_t slot)
+{
+  return f == -2 ? P.slot_count[slot] : reinterpret_cast<uint32_t const*>(P.fields[f].ptr)[slot];
+}
+
+template <int KM>
+__global__ void __launch_bounds__(BLOCK) k_second_pass(gb_params const* __restrict__ Pp)
+{
+  gb_params const& P = *Pp;
+  int64_t const n = P.num_rows;
+  for (int64_t row = int64_t(blockIdx.x) * BLOCK + threadIdx.x; row < n;
+       row += int64_t(gridDim.x) * BLOCK) {
+    int64_t slot = -2;
+    for (int vc = 0; vc < P.num_vals; vc++) {
+      int const fb = P.val_field_begin[vc], fe = P.val_field_begin[vc + 1];
+      bool has_m2 = false;
+      for (int f = fb; f < fe; f++) has_m2 |= (P.fields[f].kind == FK_M2);
+      if (!has_m2) continue;
+      dcol const& c = P.vals[vc];
+      if (!dcol_valid(c, row)) continue;
+      if (slot == -2) slot = locate<KM, false>(P, row);
+      if (slot < 0) break;
+      loaded_value const v = load_value(c, row);
+      for (int f = fb; f < fe; f++) {
+        field_desc const& F = P.fields[f];
+        if (F.kind != FK_M2) continue;
+        double const cnt  = double(count_of(P, F.aux_count, slot));
+        double const mean = field_as_double(P.fields[F.aux_sum], slot) / cnt;
+        double const d    = v.d - mean;
+        unsafeAtomicAdd(reinterpret_cast<double*>(F.ptr) + slot, d * d);
+      }
+    }
+  }
+}
+
+void launch_second_pass(gb_params const* dP, gb_params const& hP, int num_cus, hipStream_t s)
+{
+  if (hP.num_rows <= 0) return;
+  int64_t blocks = row_grid(hP.num_rows, num_cus, 16);
+  if (hP.key_mode == KM_INLINE)
+    hipLaunchKernelGGL(k_second_pass<KM_INLINE>, dim3(blocks), dim3(BLOCK), 0, s, dP);
+  else
+    hipLaunchKernelGGL(k_second_pass<KM_GENERAL>, dim3(blocks), dim3(BLOCK), 0, s, dP);
+}
+
+// ---------------------------------------------------------------------------
+// compaction of occupied slots -> dense group ids
+__device__ __forceinline__ bool slot_occupied(gb_params const& P, uint64_t slot)
+{
+  if (slot < P.cap) {
+    return P.key_mode == KM_INLINE ? (P.slot_keys[slot] != EMPTY_KEY)
+                                   : (P.slot_rows[slot] != EMPTY_ROW);
+  }
+  return P.special[slot - P.cap] != 0u;
+}
+
+__device__ __forceinline__ uint32_t block_sum_u32(uint32_t v, uint32_t* lds)
+{
+  // wave64 reduction then across the block's 4 waves
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  int const wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (lane == 0) lds[wave] = v;
+  __syncthreads();
+  uint32_t t = 0;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < BLOCK / 64; w++) t += lds[w];
+  __syncthreads();
+  return t;
+}
+
+__global__ void __launch_bounds__(BLOCK)
+  k_count_occupied(gb_params const* __restrict__ Pp, uint32_t* block_counts)
+{
+  __shared__ uint32_t red[BLOCK / 64];
+  gb_params const& P = *Pp;
+  uint64_t const total = P.cap + 2;
+  uint64_t const base  = uint64_t(blockIdx.x) * SCAN_TILE;
+  uint32_t c = 0;
+  for (int k = threadIdx.x; k < SCAN_TILE; k += BLOCK) {
+    uint64_t slot = base + k;
+    if (slot < total && slot_occupied(P, slot)) c++;
+  }
+  c = block_sum_u32(c, red);
+  if (threadIdx.x == 0) block_counts[blockIdx.x] = c;
+}
+
+void launch_count_occupied(gb_params const* dP, gb_params const& hP, uint32_t* block_counts,
+                           hipStream_t s)
+{
+  int64_t nb = (int64_t(hP.cap) + 2 + SCAN_TILE - 1) / SCAN_TILE;
+  hipLaunchKernelGGL(k_count_occupied, dim3(nb), dim3(BLOCK), 0, s, dP, block_counts);
+}
+
+// Exclusive scan of `n` u32 values in place by one block of 1024 threads.
+__global__ void __launch_bounds__(1024) k_scan_small(uint32_t* data, int64_t n, uint32_t* total)
+{
+  __shared__ uint32_t part[1024];
+  int64_t const per = (n + 1023) / 1024;
+  int64_t const b   = int64_t(threadIdx.x) * per;
+  int64_t const e   = (b + per < n) ? b + per : n;
+  uint32_t s = 0;
+  for (int64_t i = b; i < e; i++) s += data[i];
+  part[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 1; o < 1024; o <<= 1) {
+    uint32_t v = threadIdx.x >= o ? part[threadIdx.x - o] : 0;
+    __syncthreads();
+    part[threadIdx.x] += v;
+    __syncthreads();
+  }
+  uint32_t run = threadIdx.x ? part[threadIdx.x - 1] : 0;
+  for (int64_t i = b; i < e; i++) {
+    uint32_t v = data[i];
+    data[i]    = run;
+    run += v;
+  }
+  if (threadIdx.x == 1023 && total) *total = part[1023];
+}
+
+void launch_scan_blocks(uint32_t* block_counts, int64_t nblocks, uint32_t* total, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_scan_small, dim3(1), dim3(1024), 0, s, block_counts, nblocks, total);
+}
+
+// Writes the gather map: rep_rows[g] = slot of dense group g (slot order).
+__global__ void __launch_bounds__(BLOCK)
+  k_compact(gb_params const* __restrict__ Pp, uint32_t const* block_offsets, uint32_t* map)
+{
+  __shared__ uint32_t wave_tot[BLOCK / 64];
+  gb_params const& P = *Pp;
+  uint64_t const total = P.cap + 2;
+  uint64_t const base  = uint64_t(blockIdx.x) * SCAN_TILE;
+  uint32_t running     = block_offsets[blockIdx.x];
+  int const wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int k = 0; k < SCAN_TILE; k += BLOCK) {
+    uint64_t const slot = base + k + threadIdx.x;
+    bool const occ     = slot < total && slot_occupied(P, slot);
+    uint64_t const bal = __ballot(occ);
+    uint32_t const below = __popcll(bal & ((1ull << lane) - 1ull));
+    if (lane == 0) wave_tot[wave] = __popcll(bal);
+    __syncthreads();
+    uint32_t before = 0, all = 0;
+    for (int w = 0; w < BLOCK / 64; w++) {
+      if (w < wave) before += wave_tot[w];
+      all += wave_tot[w];
+    }
+    if (occ) map[running + before + below] = uint32_t(slot);
+    running += all;
+    __syncthreads();
+  }
+}
+
+void launch_compact(gb_params const* dP, gb_params const& hP, uint32_t const* block_offsets,
+                    uint32_t* map, hipStream_t s)
+{
+  int64_t nb = (int64_t(hP.cap) + 2 + SCAN_TILE - 1) / SCAN_TILE;
+  hipLaunchKernelGGL(k_compact, dim3(nb), dim3(BLOCK), 0, s, dP, block_offsets, map);
+}
+
+// ---------------------------------------------------------------------------
+// dense outputs: keys and aggregation results for group g = blockIdx*BLOCK+tid
+__device__ __forceinline__ uint64_t field_bits(field_desc const& F, int64_t slot)
+{
+  return F.width == 8 ? reinterpret_cast<uint64_t const*>(F.ptr)[slot]
+                      : reinterpret_cast<uint32_t const*>(F.ptr)[slot];
+}
+
+__device__ __forceinline__ void store_width(void* data, int64_t g, int w, uint64_t bits)
+{
+  switch (w) {
+    case 1: reinterpret_cast<uint8_t*>(data)[g] = uint8_t(bits); break;
+    case 2: reinterpret_cast<uint16_t*>(data)[g] = uint16_t(bits); break;
+    case 4: reinterpret_cast<uint32_t*>(data)[g] = uint32_t(bits); break;
+    default: reinterpret_cast<uint64_t*>(data)[g] = bits; break;
+  }
+}
+
+// Writes the validity bit of group g for the whole wave with one ballot.
+__device__ __forceinline__ void store_valid_bits(uint32_t* mask, int64_t g, int64_t G, bool valid)
+{
+  uint64_t const bal = __ballot(g < G && valid);
+  int const lane     = threadIdx.x & 63;
+  int64_t const g0   = g - lane;  // multiple of 64
+  if (lane == 0 && g0 < G) mask[g0 >> 5] = uint32_t(bal);
+  if (lane == 32 && g0 + 32 < G) mask[(g0 >> 5) + 1] = uint32_t(bal >> 32);
+}
+
+template <int KM>
+__global__ void __launch_bounds__(BLOCK)
+  k_outputs(gb_params const* __restrict__ Pp, uint32_t const* __restrict__ map, int64_t G,
+            uint32_t* rep_rows)
+{
+  gb_params const& P = *Pp;
+  int64_t const g    = int64_t(blockIdx.x) * BLOCK + threadIdx.x;
+  bool const live    = g < G;
+  int64_t const slot = live ? int64_t(map[g]) : 0;
+
+  // keys
+  if constexpr (KM == KM_INLINE) {
+    uint64_t bits = 0;
+    bool kvalid   = true;
+    if (live) {
+      if (slot < int64_t(P.cap)) bits = P.slot_keys[slot];
+      else if (slot == int64_t(P.cap)) bits = EMPTY_KEY;
+      else kvalid = false;
+      store_width(P.out_keys[0], g, storage_width(P.key_storage), kvalid ? bits : 0);
+    }
+    if (P.out_key_mask[0]) store_valid_bits(P.out_key_mask[0], g, G, kvalid);
+  } else {
+    if (live) rep_rows[g] = uint32_t(P.slot_rows[slot]);
+  }
+
+  // aggregation results
+  for (int o = 0; o < P.num_outs; o++) {
+    out_desc const& O = P.outs[o];
+    bool valid        = true;
+    if (live) {
+      uint32_t cnt = 0;
+      if (O.count_field != -1) {
+        cnt   = count_of(P, O.count_field, slot);
+        valid = cnt > 0;
+      }
+      switch (O.kind) {
+        case OK_COPY_I64:
+        case OK_COPY_F64: reinterpret_cast<uint64_t*>(O.data)[g] = field_bits(P.fields[O.field], slot); break;
+        case OK_COPY_F32: reinterpret_cast<uint32_t*>(O.data)[g] = uint32_t(field_bits(P.fields[O.field], slot)); break;
+        case OK_COUNT:
+          reinterpret_cast<int32_t*>(O.data)[g] =
+            int32_t(reinterpret_cast<uint32_t const*>(P.fields[O.field].ptr)[slot]);
+          valid = true;
+          break;
+        case OK_COUNT_ALL:
+          reinterpret_cast<int32_t*>(O.data)[g] = int32_t(P.slot_count[slot]);
+          valid = true;
+          break;
+        case OK_MINMAX: {
+          field_desc const& F = P.fields[O.field];
+          uint64_t b          = field_bits(F, slot);
+          if (F.kind == FK_MIN_F32 || F.kind == FK_MAX_F32) b = unord32(uint32_t(b));
+          else if (F.kind == FK_MIN_F64 || F.kind == FK_MAX_F64) b = unord64(b);
+          store_width(O.data, g, O.out_width, b);
+          break;
+        }
+        case OK_MEAN: {
+          double const sum = field_as_double(P.fields[O.field], slot);
+          reinterpret_cast<double*>(O.data)[g] = valid ? sum / double(cnt) : 0.0;
+          break;
+        }
+        case OK_VAR:
+        case OK_STD: {
+          int64_t const dn = int64_t(cnt) - O.ddof;
+          valid            = cnt > 0 && dn > 0;
+          double var       = 0.0;
+          if (valid) var = reinterpret_cast<double const*>(P.fields[O.field].ptr)[slot] / double(dn);
+          reinterpret_cast<double*>(O.data)[g] = (O.kind == OK_STD) ? sqrt(var) : var;
+          break;
+        }
+        default: break;
+      }
+    }
+    if (O.mask) store_valid_bits(O.mask, g, G, valid);
+  }
+}
+
+void launch_outputs(gb_params const* dP, gb_params const& hP, uint32_t const* map, int64_t G,
+                    uint32_t* rep_rows, hipStream_t s)
+{
+  if (G <= 0) return;
+  int64_t blocks = (G + BLOCK - 1) / BLOCK;
+  if (hP.key_mode == KM_INLINE)
+    hipLaunchKernelGGL(k_outputs<KM_INLINE>, dim3(blocks), dim3(BLOCK), 0, s, dP, map, G, rep_rows);
+  else
+    hipLaunchKernelGGL(k_outputs<KM_GENERAL>, dim3(blocks), dim3(BLOCK), 0, s, dP, map, G, rep_rows);
+}
+
+// ---------------------------------------------------------------------------
+// key gather for the general table (fixed-width keys and string lengths)
+__global__ void __launch_bounds__(BLOCK)
+  k_gather_keys(gb_params const* __restrict__ Pp, uint32_t const* __restrict__ rep_rows, int64_t G,
+                int32_t* str_sizes)
+{
+  gb_params const& P = *Pp;
+  int64_t const g    = int64_t(blockIdx.x) * BLOCK + threadIdx.x;
+  bool const live    = g < G;
+  int64_t const r    = live ? int64_t(rep_rows[g]) : 0;
+  int nstr           = 0;
+  for (int c = 0; c < P.num_keys; c++) {
+    dcol const& kc = P.keys[c];
+    bool const v   = live && dcol_valid(kc, r);
+    if (kc.storage == ST_STRING) {
+      if (live) {
+        int64_t b = 0, e = 0;
+        if (v) str_span(kc, r, b, e);
+        str_sizes[int64_t(nstr) * G + g] = int32_t(e - b);
+      }
+      nstr++;
+    } else if (live) {
+      int const w = storage_width(kc.storage);
+      uint64_t bits;
+      switch (w) {
+        case 1: bits = kc.data[r]; break;
+        case 2: bits = reinterpret_cast<uint16_t const*>(kc.data)[r]; break;
+        case 4: bits = reinterpret_cast<uint32_t const*>(kc.data)[r]; break;
+        default: bits = reinterpret_cast<uint64_t const*>(kc.data)[r]; break;
+      }
+      store_width(P.out_keys[c], g, w, bits);
+    }
+    if (P.out_key_mask[c]) store_valid_bits(P.out_key_mask[c], g, G, v);
+  }
+}
+
+void launch_gather_keys(gb_params const* dP, gb_params const& hP, uint32_t const* rep_rows,
+                        int64_t G, int32_t* str_sizes, hipStream_t s)
+{
+  if (G <= 0) return;
+  int64_t blocks = (G + BLOCK - 1) / BLOCK;
+  hipLaunchKernelGGL(k_gather_keys, dim3(blocks), dim3(BLOCK), 0, s, dP, rep_rows, G, str_sizes);
+}
+
+// Exclusive scan of int32 values (three phases: tile sums, scan of sums, apply).
+constexpr int I32_TILE = 4096;
+__global__ void __launch_bounds__(BLOCK) k_tile_sums_i32(int32_t const* data, int64_t n, uint32_t* sums)
+{
+  __shared__ uint32_t red[BLOCK / 64];
+  int64_t const base = int64_t(blockIdx.x) * I32_TILE;
+  uint32_t s = 0;
+  for (int k = threadIdx.x; k < I32_TILE; k += BLOCK)
+    if (base + k < n) s += uint32_t(data[base + k]);
+  s = block_sum_u32(s, red);
+  if (threadIdx.x == 0) sums[blockIdx.x] = s;
+}
+
+__global__ void __launch_bounds__(BLOCK)
+  k_apply_scan_i32(int32_t* data, int64_t n, uint32_t const* tile_offsets)
+{
+  __shared__ uint32_t wave_tot[BLOCK / 64];
+  int64_t const base = int64_t(blockIdx.x) * I32_TILE;
+  uint32_t running   = tile_offsets[blockIdx.x];
+  int const wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int k = 0; k < I32_TILE; k += BLOCK) {
+    int64_t const i = base + k + threadIdx.x;
+    uint32_t v      = i < n ? uint32_t(data[i]) : 0u;
+    // inclusive scan within the wave
+    uint32_t x = v;
+    for (int o = 1; o < 64; o <<= 1) {
+      uint32_t y = __shfl_up(x, o, 64);
+      if (lane >= o) x += y;
+    }
+    if (lane == 63) wave_tot[wave] = x;
+    __syncthreads();
+    uint32_t before = 0, all = 0;
+    for (int w = 0; w < BLOCK / 64; w++) {
+      if (w < wave) before += wave_tot[w];
+      all += wave_tot[w];
+    }
+    if (i < n) data[i] = int32_t(running + before + x - v);
+    running += all;
+    __syncthreads();
+  }
+}
+
+void launch_exclusive_scan_i32(int32_t* data, int64_t n, int32_t* total, hipStream_t s)
+{
+  // `total` must point at data[n] (one extra element) or be a separate slot.
+  int64_t const nt = (n + I32_TILE - 1) / I32_TILE;
+  uint32_t* sums   = nullptr;
+  MIG_HIP_TRY(hipMallocAsync(reinterpret_cast<void**>(&sums), sizeof(uint32_t) * (nt + 1), s));
+  hipLaunchKernelGGL(k_tile_sums_i32, dim3(nt), dim3(BLOCK), 0, s, data, n, sums);
+  hipLaunchKernelGGL(k_scan_small, dim3(1), dim3(1024), 0, s, sums, nt, reinterpret_cast<uint32_t*>(total));
+  hipLaunchKernelGGL(k_apply_scan_i32, dim3(nt), dim3(BLOCK), 0, s, data, n, sums);
+  MIG_HIP_TRY(hipFreeAsync(sums, s));
+}
+
+// One wave copies the chars of one group's representative string.
+__global__ void __launch_bounds__(BLOCK)
+  k_gather_chars(dcol src, uint32_t const* __restrict__ rep_rows, int64_t G,
+                 int32_t const* __restrict__ out_offsets, uint8_t* out_chars)
+{
+  int64_t const g = (int64_t(blockIdx.x) * BLOCK + threadIdx.x) >> 6;
+  int const lane  = threadIdx.x & 63;
+  if (g >= G) return;
+  int64_t const r = rep_rows[g];
+  int64_t const o = out_offsets[g];
+  int64_t const len = int64_t(out_offsets[g + 1]) - o;
+  if (len == 0) return;
+  int64_t b, e;
+  str_span(src, r, b, e);
+  for (int64_t i = lane; i < len; i += 64) out_chars[o + i] = src.data[b + i];
+}
+
+void launch_gather_string_chars(dcol const& src, uint32_t const* rep_rows, int64_t G,
+                                int32_t const* out_offsets, uint8_t* out_chars, hipStream_t s)
+{
+  if (G <= 0) return;
+  int64_t blocks = (G * 64 + BLOCK - 1) / BLOCK;
+  hipLaunchKernelGGL(k_gather_chars, dim3(blocks), dim3(BLOCK), 0, s, src, rep_rows, G, out_offsets,
+                     out_chars);
+}
+
+// Number of valid rows among [0, n) of a column view (null_count == -1 input).
+__global__ void __launch_bounds__(BLOCK) k_count_valid(dcol c, int64_t n, uint32_t* out)
+{
+  __shared__ uint32_t red[BLOCK / 64];
+  uint32_t cnt = 0;
+  for (int64_t r = int64_t(blockIdx.x) * BLOCK + threadIdx.x; r < n; r += int64_t(gridDim.x) * BLOCK)
+    cnt += dcol_valid(c, r) ? 1u : 0u;
+  cnt = block_sum_u32(cnt, red);
+  if (threadIdx.x == 0) aadd(out, cnt);
+}
+
+void launch_count_nulls(dcol const& c, int64_t n, uint32_t* d_count, hipStream_t s)
+{
+  if (n <= 0) return;
+  int64_t blocks = (n + BLOCK - 1) / BLOCK;
+  if (blocks > 4096) blocks = 4096;
+  hipLaunchKernelGGL(k_count_valid, dim3(blocks), dim3(BLOCK), 0, s, c, n, d_count);
+}
+
+}  // namespace mig

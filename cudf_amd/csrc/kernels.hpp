@@ -1,0 +1,149 @@
+// Parameter blocks and launcher entry points of the group-by kernels.
+//
+// The host side (groupby.cpp) fills a `gb_params` block once per call and
+// copies it to device memory; every kernel of the call reads it through a
+// uniform pointer, so the descriptors live in the scalar cache.
+#pragma once
+
+#include "common.hpp"
+
+namespace mig {
+
+constexpr int MAX_KEYS   = 16;
+constexpr int MAX_VALS   = 32;
+constexpr int MAX_FIELDS = 128;
+constexpr int MAX_OUTS   = 128;
+
+// Empty marker of the inline key table.  Normalised float keys can never take
+// this value (it is a signalling-NaN pattern and NaN keys are canonicalised);
+// an integer key equal to it is routed to the dedicated slot `cap`.
+constexpr uint64_t EMPTY_KEY = 0xFFF1A5A5A5A5A5A5ull;
+// Empty marker of the general (row index) table: (hash << 32) | row.
+constexpr uint64_t EMPTY_ROW = 0xFFFFFFFFFFFFFFFFull;
+
+enum key_mode : int32_t {
+  KM_INLINE  = 0,  // one fixed-width key column, normalised bits stored in the table
+  KM_GENERAL = 1   // any keys: the table stores (row hash, representative row)
+};
+
+// Accumulator kinds.  Small integer sources are widened to 32 bits for
+// min/max; floats use an order-preserving unsigned encoding for min/max.
+enum field_kind : int32_t {
+  FK_SUM_I64 = 0,
+  FK_SUM_F32,
+  FK_SUM_F64,
+  FK_SUMSQ_I64,
+  FK_SUMSQ_F32,
+  FK_SUMSQ_F64,
+  FK_MIN_S32,
+  FK_MAX_S32,
+  FK_MIN_S64,
+  FK_MAX_S64,
+  FK_MIN_U32,
+  FK_MAX_U32,
+  FK_MIN_U64,
+  FK_MAX_U64,
+  FK_MIN_F32,
+  FK_MAX_F32,
+  FK_MIN_F64,
+  FK_MAX_F64,
+  FK_COUNT,  // valid values (u32)
+  FK_M2      // sum of squared deviations from the group mean (f64, second pass)
+};
+
+struct field_desc {
+  void* ptr;        // global accumulator array, cap + 2 entries
+  int32_t kind;     // field_kind
+  int32_t vcol;     // value column it reads
+  int32_t width;    // 4 or 8 bytes
+  int32_t lds_off;  // byte offset of this field's array in the LDS table
+  int32_t aux_sum;    // FK_M2: index of the column's sum field
+  int32_t aux_count;  // FK_M2: index of the column's count field
+};
+
+// Output program: how one result column is produced from the accumulators.
+enum out_kind : int32_t {
+  OK_COPY_I64 = 0,  // int64 sum / sum of squares
+  OK_COPY_F32,
+  OK_COPY_F64,
+  OK_COUNT,      // int32 from a u32 count field
+  OK_COUNT_ALL,  // int32 from the group row count
+  OK_MINMAX,     // decode a min/max field into the source type
+  OK_MEAN,       // double(sum) / count
+  OK_VAR,        // m2 / (count - ddof)
+  OK_STD         // sqrt(var)
+};
+
+struct out_desc {
+  void* data;         // G elements of the output type
+  uint32_t* mask;     // validity words or nullptr
+  int32_t kind;       // out_kind
+  int32_t field;      // primary field (sum / min / max / count / m2)
+  int32_t count_field;  // count field for validity and mean/var, -1 = always valid
+  int32_t src_storage;  // source storage (min/max decode, mean of ints)
+  int32_t ddof;
+  int32_t out_width;
+};
+
+struct gb_params {
+  int64_t num_rows;
+  uint64_t cap;       // table slots (power of two); slots cap, cap+1 are special
+  uint64_t cap_mask;
+  uint32_t probe_limit;
+  int32_t num_keys;
+  int32_t num_vals;
+  int32_t num_fields;
+  int32_t num_outs;
+  int32_t key_mode;
+  int32_t skip_null_keys;  // null_policy EXCLUDE and some key column has nulls
+  int32_t key_storage;     // inline mode: storage of the key column
+  int32_t need_count_all;
+  int32_t need_second_pass;  // some field is FK_M2
+  uint64_t* slot_keys;   // inline table
+  uint64_t* slot_rows;   // general table
+  uint32_t* slot_count;  // group row count (COUNT_ALL), cap + 2 entries, or null
+  uint32_t* special;     // [2] occupancy of slots cap (sentinel key) and cap+1 (null key)
+  int32_t* overflow;     // set when a probe sequence exceeds probe_limit
+  dcol keys[MAX_KEYS];
+  dcol vals[MAX_VALS];
+  int32_t val_field_begin[MAX_VALS + 1];  // fields are grouped by value column
+  field_desc fields[MAX_FIELDS];
+  out_desc outs[MAX_OUTS];
+  // key output (finalize)
+  void* out_keys[MAX_KEYS];       // fixed-width key outputs (or string offsets)
+  uint32_t* out_key_mask[MAX_KEYS];
+};
+
+// ---- launchers (groupby_kernels.hip) --------------------------------------
+struct launch_cfg {
+  int num_cus;
+};
+
+void launch_hash_rows(dcol const* d_cols, int ncols, int64_t n, uint32_t seed, uint32_t* out,
+                      hipStream_t s);
+void launch_sample_hash(gb_params const* dP, int64_t n, int64_t nsample, uint32_t* out,
+                        hipStream_t s);
+void launch_init_state(gb_params const* dP, gb_params const& hP, hipStream_t s);
+void launch_aggregate(gb_params const* dP, gb_params const& hP, int num_cus, hipStream_t s);
+void launch_second_pass(gb_params const* dP, gb_params const& hP, int num_cus, hipStream_t s);
+// Occupancy scan: block_counts has ceil((cap+2)/SCAN_TILE) entries.
+constexpr int SCAN_TILE = 4096;
+void launch_count_occupied(gb_params const* dP, gb_params const& hP, uint32_t* block_counts,
+                           hipStream_t s);
+void launch_scan_blocks(uint32_t* block_counts, int64_t nblocks, uint32_t* total, hipStream_t s);
+// Writes map[g] = slot of dense group g, in slot order.
+void launch_compact(gb_params const* dP, gb_params const& hP, uint32_t const* block_offsets,
+                    uint32_t* map, hipStream_t s);
+// Writes keys (inline table) or representative rows (general table) and all
+// aggregation results of the G dense groups.
+void launch_outputs(gb_params const* dP, gb_params const& hP, uint32_t const* map, int64_t G,
+                    uint32_t* rep_rows, hipStream_t s);
+// Gathers fixed-width key columns / string lengths by representative row.
+void launch_gather_keys(gb_params const* dP, gb_params const& hP, uint32_t const* rep_rows,
+                        int64_t G, int32_t* str_sizes_scratch, hipStream_t s);
+void launch_exclusive_scan_i32(int32_t* data, int64_t n, int32_t* total, hipStream_t s);
+void launch_gather_string_chars(dcol const& src, uint32_t const* rep_rows, int64_t G,
+                                int32_t const* out_offsets, uint8_t* out_chars, hipStream_t s);
+void launch_count_nulls(dcol const& c, int64_t n, uint32_t* d_count, hipStream_t s);
+
+}  // namespace mig

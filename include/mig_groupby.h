@@ -1,0 +1,232 @@
+/*
+ * mig_groupby.h -- C ABI of the MI355X (gfx950) hash group-by library.
+ *
+ * This is the drop-in boundary for libcudf's hash group-by path
+ * (`cudf::groupby::groupby(keys).aggregate(requests)`).  Every entry point takes
+ * plain pointers and sizes; no C++ or HIP types appear in the signatures, so the
+ * header can be bound from ctypes, JNI, cgo or Cython alike.  The C++ facade in
+ * `include/cudf/` (namespace `cudf`) is written on top of these functions.
+ *
+ * Conventions
+ *   - Every function that can fail returns a `mig_status`; on failure the
+ *     calling thread's message is available from `mig_last_error()`.
+ *   - Device pointers are HIP device addresses on the current device.
+ *   - `stream` arguments are `hipStream_t` values passed as `void*`; NULL means
+ *     the device's null stream.
+ *   - Validity bitmasks are Arrow/cudf style: 32-bit words, least significant
+ *     bit first, bit set = row valid (cudf/utilities/bit.hpp:128-132).
+ */
+#ifndef MIG_GROUPBY_H
+#define MIG_GROUPBY_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MIG_ABI_VERSION 1
+
+/* ---- status codes ------------------------------------------------------ */
+/* Error classes mirror cudf/utilities/error.hpp:64-127 (logic_error,
+ * cuda_error, data_type_error) plus allocation failure. */
+typedef enum mig_status {
+  MIG_OK              = 0,
+  MIG_ERROR_LOGIC     = 1, /* cudf::logic_error: invalid request / sizes      */
+  MIG_ERROR_HIP       = 2, /* cudf::cuda_error equivalent: a HIP call failed  */
+  MIG_ERROR_DATA_TYPE = 3, /* cudf::data_type_error: unsupported type combo   */
+  MIG_ERROR_ALLOC     = 4, /* device allocation failed                        */
+  MIG_ERROR_OVERFLOW  = 5  /* a fixed-capacity buffer would be exceeded       */
+} mig_status;
+
+/* ---- type ids: same numbering as cudf::type_id (cudf/types.hpp:203-235) -- */
+enum {
+  MIG_TYPE_EMPTY = 0,
+  MIG_TYPE_INT8,
+  MIG_TYPE_INT16,
+  MIG_TYPE_INT32,
+  MIG_TYPE_INT64,
+  MIG_TYPE_UINT8,
+  MIG_TYPE_UINT16,
+  MIG_TYPE_UINT32,
+  MIG_TYPE_UINT64,
+  MIG_TYPE_FLOAT32,
+  MIG_TYPE_FLOAT64,
+  MIG_TYPE_BOOL8,
+  MIG_TYPE_TIMESTAMP_DAYS,
+  MIG_TYPE_TIMESTAMP_SECONDS,
+  MIG_TYPE_TIMESTAMP_MILLISECONDS,
+  MIG_TYPE_TIMESTAMP_MICROSECONDS,
+  MIG_TYPE_TIMESTAMP_NANOSECONDS,
+  MIG_TYPE_DURATION_DAYS,
+  MIG_TYPE_DURATION_SECONDS,
+  MIG_TYPE_DURATION_MILLISECONDS,
+  MIG_TYPE_DURATION_MICROSECONDS,
+  MIG_TYPE_DURATION_NANOSECONDS,
+  MIG_TYPE_DICTIONARY32,
+  MIG_TYPE_STRING,
+  MIG_TYPE_LIST,
+  MIG_TYPE_DECIMAL32,
+  MIG_TYPE_DECIMAL64,
+  MIG_TYPE_DECIMAL128,
+  MIG_TYPE_STRUCT,
+  MIG_NUM_TYPE_IDS
+};
+
+/* ---- aggregation kinds: same numbering as cudf::aggregation::Kind
+ *      (cudf/aggregation.hpp:86-122).  The hash path accepts the kinds
+ *      listed in hash/groupby.cu:73-84 that this library implements:
+ *      SUM, MIN, MAX, COUNT_VALID, COUNT_ALL, SUM_OF_SQUARES, MEAN,
+ *      VARIANCE, STD. ---------------------------------------------------- */
+enum {
+  MIG_AGG_SUM            = 0,
+  MIG_AGG_PRODUCT        = 1,
+  MIG_AGG_MIN            = 2,
+  MIG_AGG_MAX            = 3,
+  MIG_AGG_COUNT_VALID    = 4,
+  MIG_AGG_COUNT_ALL      = 5,
+  MIG_AGG_ANY            = 6,
+  MIG_AGG_ALL            = 7,
+  MIG_AGG_SUM_OF_SQUARES = 8,
+  MIG_AGG_MEAN           = 9,
+  MIG_AGG_M2             = 10,
+  MIG_AGG_VARIANCE       = 11,
+  MIG_AGG_STD            = 12,
+  MIG_AGG_ARGMAX         = 15,
+  MIG_AGG_ARGMIN         = 16
+};
+
+/* cudf::null_policy (cudf/types.hpp:126-129). */
+enum { MIG_NULL_EXCLUDE = 0, MIG_NULL_INCLUDE = 1 };
+
+/* Non-owning view of one column; mirrors cudf::column_view_base fields
+ * (cudf/column/column_view.hpp:240-247).  For STRING columns `data` is the
+ * chars buffer and `offsets` the offsets child (child 0, size+1 entries,
+ * INT32 or INT64, indices absolute into `data`), as in
+ * cudf/column/column_device_view.cuh:444-453. */
+typedef struct mig_column_view {
+  int32_t type_id;          /* MIG_TYPE_*                                      */
+  int32_t scale;            /* fixed-point scale (unused by this library)      */
+  int32_t size;             /* rows visible through the view (cudf::size_type) */
+  int32_t offset;           /* first row of the view inside data / null_mask   */
+  const void* data;         /* element array (fixed width) or chars (STRING)   */
+  const uint32_t* null_mask;/* validity words, NULL = no mask (all valid)      */
+  int32_t null_count;       /* nulls in [offset, offset+size); -1 = unknown    */
+  int32_t offsets_type_id;  /* STRING only: MIG_TYPE_INT32 or MIG_TYPE_INT64   */
+  const void* offsets;      /* STRING only: offsets child                      */
+} mig_column_view;
+
+/* One aggregation of a request (cudf::groupby_aggregation).  `ddof` is used
+ * by VARIANCE and STD (cudf/aggregation.hpp make_variance_aggregation). */
+typedef struct mig_aggregation {
+  int32_t kind; /* MIG_AGG_* */
+  int32_t ddof;
+} mig_aggregation;
+
+/* cudf::groupby::aggregation_request (cudf/groupby.hpp:61-64). */
+typedef struct mig_aggregation_request {
+  mig_column_view values;
+  const mig_aggregation* aggregations;
+  int32_t num_aggregations;
+} mig_aggregation_request;
+
+/* Device memory resource used for result columns (the `mr` argument of
+ * groupby::aggregate, cudf/groupby.hpp:187-199).  A NULL resource selects the
+ * library's own stream-ordered allocator (hipMallocAsync). */
+typedef struct mig_memory_resource {
+  void* ctx;
+  void* (*allocate)(void* ctx, size_t bytes, void* stream);
+  void (*deallocate)(void* ctx, void* ptr, size_t bytes, void* stream);
+} mig_memory_resource;
+
+/* Column owned by a result handle (borrowed by the caller). */
+typedef struct mig_column {
+  int32_t type_id;
+  int32_t size;
+  void* data;               /* elements, or chars for STRING                  */
+  uint32_t* null_mask;      /* NULL if the column has no mask                 */
+  int32_t null_count;
+  int32_t offsets_type_id;  /* STRING: MIG_TYPE_INT32                         */
+  void* offsets;            /* STRING: size+1 offsets                         */
+  int64_t data_bytes;       /* bytes in `data`                                */
+} mig_column;
+
+typedef struct mig_groupby_result mig_groupby_result;
+
+/* ---- library -------------------------------------------------------------- */
+
+/* Message for the last failed call on this thread ("" if none). */
+const char* mig_last_error(void);
+
+/* MIG_ABI_VERSION of the loaded library. */
+int32_t mig_abi_version(void);
+
+/* ---- row hashing -----------------------------------------------------------
+ * Replaces cudf::hashing::murmurhash3_x86_32(table_view, seed, stream, mr)
+ * (cpp/include/cudf/hashing.hpp:61; src/hash/murmurhash3_x86_32.cu:34-57):
+ * MurmurHash3_x86_32 of every row, elements combined with boost hash_combine,
+ * null elements hashing to 0xFFFFFFFF.  `out` is a device array of
+ * columns[0].size uint32 values. */
+mig_status mig_hash_murmurhash3_x86_32(const mig_column_view* columns,
+                                       int32_t num_columns,
+                                       uint32_t seed,
+                                       uint32_t* out,
+                                       void* stream);
+
+/* ---- hash group-by -----------------------------------------------------------
+ * Replaces cudf::groupby::groupby(keys, null_policy).aggregate(requests, stream,
+ * mr) on the hash path (cpp/src/groupby/groupby.cu:53-64, 204-221;
+ * cpp/src/groupby/hash/groupby.cu:548-667).  On success `*out` owns the unique
+ * keys (output row order unspecified, as in cudf/groupby.hpp:155-157) and one
+ * column per requested aggregation, in request order.  Blocks until the
+ * results are complete on `stream`. */
+mig_status mig_groupby_aggregate(const mig_column_view* keys,
+                                 int32_t num_keys,
+                                 int32_t null_policy,
+                                 const mig_aggregation_request* requests,
+                                 int32_t num_requests,
+                                 void* stream,
+                                 const mig_memory_resource* mr,
+                                 mig_groupby_result** out);
+
+int32_t mig_result_num_groups(const mig_groupby_result* r);
+int32_t mig_result_num_keys(const mig_groupby_result* r);
+int32_t mig_result_num_requests(const mig_groupby_result* r);
+int32_t mig_result_num_aggregations(const mig_groupby_result* r, int32_t request);
+/* Borrow key column `i` / result column (`request`, `agg`). */
+mig_status mig_result_key(const mig_groupby_result* r, int32_t i, mig_column* out);
+mig_status mig_result_column(const mig_groupby_result* r,
+                             int32_t request,
+                             int32_t agg,
+                             mig_column* out);
+/* Take ownership of a key / result column: afterwards the caller frees its
+ * buffers with the resource they came from and the handle forgets them. */
+mig_status mig_result_release_key(mig_groupby_result* r, int32_t i, mig_column* out);
+mig_status mig_result_release_column(mig_groupby_result* r,
+                                     int32_t request,
+                                     int32_t agg,
+                                     mig_column* out);
+/* Free the handle and every column it still owns. */
+void mig_result_free(mig_groupby_result* r);
+
+/* Statistics of the last mig_groupby_aggregate call on this thread, for
+ * benchmarks and the autotuner (strategy chosen, table capacity, time per
+ * phase).  Writes at most `n` doubles; returns the number available.
+ * Index: 0 strategy, 1 key mode, 2 table slots, 3 estimated groups,
+ * 4 groups, 5 table rebuilds, 6 state bytes, then phase times in ms
+ * (7 estimate, 8 init, 9 accumulate, 10 second pass, 11 compact,
+ * 12 outputs) when phase timing is on. */
+int32_t mig_last_stats(double* values, int32_t n);
+
+/* Turns hipEvent timing of each phase of mig_groupby_aggregate on (1) or off
+ * (0) for the calling process; events are recorded on the call's stream.
+ * Instrumentation only, with no counterpart in the reference (which marks
+ * the call with an NVTX range, cpp/src/groupby/groupby.cu:209). */
+void mig_set_phase_timing(int32_t enable);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MIG_GROUPBY_H */

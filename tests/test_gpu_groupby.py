@@ -1,0 +1,281 @@
+"""GPU parity tests: the gfx950 path, called through the C ABI, against the
+CPU oracle on the same seeded inputs (bit-exact for keys, counts, integer
+sums, min and max; floating-point tolerances as stated in tests/gbutil.py)."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+from tests.gbutil import (AGG, C2_AGGS, assert_parity, c1_data, c2_data, c3_data, gpu_groupby,
+                          host_col, sort_order)
+from tests.test_oracle import KATS, NP_TYPES, _kat_value_types, _run_kat
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def _loaded_library_is_native():
+    from cudf_amd import _lib
+    assert _lib.lib() is not None
+
+
+# ------------------------------------------------------------------ hashing --
+def test_hash_docstring_vectors():
+    import cudf_amd as ca
+    with open(os.path.join(GOLDEN, "hash_vectors.json")) as f:
+        cases = json.load(f)["docstring"]
+    for case in cases:
+        if case["seed"] != 0:
+            continue
+        cols = [ca.Column.from_numpy(np.array(c["values"], dtype=NP_TYPES[c["type"]]))
+                for c in case["columns"]]
+        h, _ = ca.murmurhash3_x86_32(ca.Table(cols)).to_numpy()
+        assert list(h.view(np.int32)) == case["expect_int32"], case["source"]
+
+
+@pytest.mark.parametrize("dtype", [np.int8, np.int16, np.int32, np.int64, np.uint8, np.uint16,
+                                   np.uint32, np.uint64, np.float32, np.float64, np.bool_])
+def test_hash_matches_oracle_every_type(dtype):
+    import cudf_amd as ca
+    rng = np.random.default_rng(7)
+    n = 5000
+    if dtype == np.bool_:
+        a = rng.random(n) > 0.5
+    elif np.dtype(dtype).kind == "f":
+        a = rng.standard_normal(n).astype(dtype)
+        a[:10] = [0.0, -0.0, np.nan, -np.nan, np.inf, -np.inf, 1.0, -1.0, 1e-30, 3.5]
+    else:
+        info = np.iinfo(dtype)
+        a = rng.integers(info.min, info.max, n, dtype=dtype, endpoint=True)
+    valid = rng.random(n) > 0.1
+    hc = host_col(a, valid)
+    for seed in (0, 1234):
+        h, _ = ca.murmurhash3_x86_32(ca.Table([ca.Column.from_numpy(a, valid)]), seed).to_numpy()
+        assert np.array_equal(h, orc.hash_rows([hc], seed=seed))
+
+
+def test_hash_strings_and_multicolumn():
+    import cudf_amd as ca
+    rng = np.random.default_rng(8)
+    n = 3000
+    strs = [bytes(rng.integers(0, 256, int(rng.integers(0, 40)), dtype=np.uint8)) for _ in range(n)]
+    sv = rng.random(n) > 0.1
+    ints = rng.integers(-1000, 1000, n, dtype=np.int32)
+    iv = rng.random(n) > 0.2
+    s_host = orc.HostColumn.from_strings(strs, sv)
+    i_host = host_col(ints, iv)
+    dev = ca.Table([ca.Column.from_strings(strs, sv), ca.Column.from_numpy(ints, iv)])
+    h, _ = ca.murmurhash3_x86_32(dev).to_numpy()
+    assert np.array_equal(h, orc.hash_rows([s_host, i_host]))
+    # large (int64) offsets give the same hashes
+    dev64 = ca.Table([ca.Column.from_strings(strs, sv, large=True), ca.Column.from_numpy(ints, iv)])
+    h64, _ = ca.murmurhash3_x86_32(dev64).to_numpy()
+    assert np.array_equal(h64, h)
+
+
+# ----------------------------------------------------- known-answer tests --
+@pytest.mark.parametrize("case", KATS["cases"], ids=[c["name"] for c in KATS["cases"]])
+def test_known_answers(case):
+    for vt in _kat_value_types(case):
+        _run_kat(case, vt, gpu_groupby)
+
+
+def test_string_keys_known_answer():
+    case = KATS["string_key_cases"][0]
+    keys = orc.HostColumn.from_strings([s.encode() for s in case["keys"]])
+    vals = host_col(np.array(case["vals"], np.int32))
+    kcols, res = gpu_groupby([keys], [(vals, [(AGG["SUM"], 1)])])
+    got = {bytes(kcols[0][0][i]): int(res[0][0][0][i]) for i in range(len(kcols[0][0]))}
+    assert got == {s.encode(): v for s, v in zip(case["expect_keys"], case["expect_vals"])}
+
+
+def test_empty_input():
+    keys = host_col(np.zeros(0, np.int32))
+    vals = host_col(np.zeros(0, np.float64))
+    kcols, res = gpu_groupby([keys], [(vals, [(AGG["SUM"], 1), (AGG["VARIANCE"], 1)])])
+    assert len(kcols[0][0]) == 0 and len(res[0][0][0]) == 0 and res[0][1][0].dtype == np.float64
+
+
+# ------------------------------------------------------ randomized parity --
+ALL_NUMERIC_AGGS = [(AGG["SUM"], 1), (AGG["MIN"], 1), (AGG["MAX"], 1), (AGG["COUNT_VALID"], 1),
+                    (AGG["COUNT_ALL"], 1), (AGG["MEAN"], 1), (AGG["VARIANCE"], 1), (AGG["STD"], 2),
+                    (AGG["SUM_OF_SQUARES"], 1)]
+
+
+@pytest.mark.parametrize("key_dtype", [np.int8, np.int16, np.int32, np.int64, np.uint32, np.uint64,
+                                       np.float32, np.float64, np.bool_])
+@pytest.mark.parametrize("policy", [0, 1])
+def test_key_types(key_dtype, policy):
+    rng = np.random.default_rng(11)
+    n = 20_000
+    if key_dtype == np.bool_:
+        k = rng.random(n) > 0.5
+    elif np.dtype(key_dtype).kind == "f":
+        k = rng.integers(-50, 50, n).astype(key_dtype) / 4
+        k[rng.random(n) < 0.02] = np.nan
+        k[rng.random(n) < 0.02] = -0.0
+    else:
+        k = rng.integers(0, min(200, np.iinfo(key_dtype).max), n).astype(key_dtype)
+    kvalid = rng.random(n) > 0.05
+    v = rng.integers(-1000, 1000, n).astype(np.int32)
+    vvalid = rng.random(n) > 0.1
+    assert_parity([host_col(k, kvalid)], [(host_col(v, vvalid), ALL_NUMERIC_AGGS)], policy)
+
+
+@pytest.mark.parametrize("val_dtype", [np.int8, np.int16, np.int32, np.int64, np.uint8, np.uint16,
+                                       np.uint32, np.uint64, np.float32, np.float64, np.bool_])
+def test_value_types(val_dtype):
+    rng = np.random.default_rng(12)
+    n = 30_000
+    k = rng.integers(0, 500, n).astype(np.int64)
+    if val_dtype == np.bool_:
+        v = rng.random(n) > 0.5
+    elif np.dtype(val_dtype).kind == "f":
+        v = (rng.standard_normal(n) * 10).astype(val_dtype)
+    else:
+        info = np.iinfo(val_dtype)
+        lo, hi = max(info.min, -10**6), min(info.max, 10**6)
+        v = rng.integers(lo, hi, n, endpoint=True).astype(val_dtype)
+    vvalid = rng.random(n) > 0.1
+    aggs = ALL_NUMERIC_AGGS
+    assert_parity([host_col(k)], [(host_col(v, vvalid), aggs)])
+
+
+def test_extreme_integer_values():
+    rng = np.random.default_rng(13)
+    n = 10_000
+    k = rng.integers(0, 50, n).astype(np.int32)
+    v = rng.choice(np.array([np.iinfo(np.int64).min, np.iinfo(np.int64).max, -1, 0, 1], np.int64), n)
+    u = rng.choice(np.array([0, np.iinfo(np.uint64).max, 2**63, 1], np.uint64), n)
+    aggs = [(AGG["SUM"], 1), (AGG["MIN"], 1), (AGG["MAX"], 1), (AGG["SUM_OF_SQUARES"], 1)]
+    assert_parity([host_col(k)], [(host_col(v), aggs), (host_col(u), aggs)])
+
+
+def test_sentinel_valued_int64_key():
+    """An int64 key equal to the inline table's empty marker still forms its
+    own group."""
+    marker = np.array([0xFFF1A5A5A5A5A5A5], dtype=np.uint64).view(np.int64)[0]
+    k = np.array([marker, 1, marker, 2, 1, marker], dtype=np.int64)
+    v = np.arange(6, dtype=np.int64)
+    ng = assert_parity([host_col(k)], [(host_col(v), [(AGG["SUM"], 1), (AGG["COUNT_ALL"], 1)])])
+    assert ng == 3
+
+
+def test_multicolumn_and_string_keys():
+    rng = np.random.default_rng(14)
+    n = 40_000
+    k1 = rng.integers(0, 20, n).astype(np.int32)
+    k1v = rng.random(n) > 0.03
+    pool = [bytes(rng.integers(97, 123, int(rng.integers(0, 12)), dtype=np.uint8)) for _ in range(60)]
+    k2 = orc.HostColumn.from_strings([pool[i] for i in rng.integers(0, 60, n)], rng.random(n) > 0.03)
+    k3 = (rng.integers(0, 3, n) * 0.5).astype(np.float64)
+    v = rng.standard_normal(n)
+    for policy in (0, 1):
+        assert_parity([host_col(k1, k1v), k2, host_col(k3)],
+                      [(host_col(v, rng.random(n) > 0.1), ALL_NUMERIC_AGGS)], policy)
+
+
+def test_duplicate_requests_and_columns():
+    rng = np.random.default_rng(15)
+    n = 5000
+    k = host_col(rng.integers(0, 30, n).astype(np.int32))
+    v = host_col(rng.integers(0, 100, n).astype(np.int32))
+    aggs = [(AGG["SUM"], 1), (AGG["SUM"], 1), (AGG["MEAN"], 1), (AGG["COUNT_VALID"], 1)]
+    assert_parity([k], [(v, aggs), (v, aggs[:2])])
+
+
+def test_sliced_columns():
+    """Column views with a row offset (cudf slicing): masks are read from the
+    view's bit offset."""
+    import cudf_amd as ca
+    from cudf_amd.aggregation import Aggregation
+    rng = np.random.default_rng(16)
+    n = 10_000
+    k = rng.integers(0, 40, n).astype(np.int32)
+    kv = rng.random(n) > 0.1
+    v = rng.integers(-100, 100, n).astype(np.int64)
+    vv = rng.random(n) > 0.1
+    off, size = 37, 7000
+    dk = ca.Column.from_numpy(k, kv).slice(off, size)
+    dv = ca.Column.from_numpy(v, vv).slice(off, size)
+    out_k, out_r = ca.GroupBy(ca.Table([dk])).aggregate(
+        [ca.GroupByRequest(dv, [Aggregation(AGG["SUM"]), Aggregation(AGG["COUNT_VALID"])])])
+    gpu = ([c.to_numpy() for c in out_k.columns()], [[c.to_numpy() for c in t.columns()] for t in out_r])
+    hk = host_col(k[off:off + size], kv[off:off + size])
+    hv = host_col(v[off:off + size], vv[off:off + size])
+    assert_parity([hk], [(hv, [(AGG["SUM"], 1), (AGG["COUNT_VALID"], 1)])], gpu=gpu)
+
+
+def test_count_of_string_values():
+    rng = np.random.default_rng(17)
+    n = 2000
+    k = host_col(rng.integers(0, 10, n).astype(np.int32))
+    s = orc.HostColumn.from_strings([b"x" * int(i) for i in rng.integers(0, 5, n)], rng.random(n) > 0.3)
+    assert_parity([k], [(s, [(AGG["COUNT_VALID"], 1), (AGG["COUNT_ALL"], 1)])])
+
+
+def test_errors_are_raised():
+    import cudf_amd as ca
+    from cudf_amd import _lib
+    from cudf_amd.aggregation import Aggregation
+    k = ca.Column.from_numpy(np.arange(10, dtype=np.int32))
+    v = ca.Column.from_numpy(np.arange(5, dtype=np.int32))
+    with pytest.raises(_lib.LogicError):
+        ca.GroupBy(ca.Table([k])).aggregate([ca.GroupByRequest(v, [Aggregation(AGG["SUM"])])])
+    s = ca.Column.from_strings(["a"] * 10)
+    with pytest.raises(_lib.DataTypeError):
+        ca.GroupBy(ca.Table([k])).aggregate([ca.GroupByRequest(s, [Aggregation(AGG["SUM"])])])
+
+
+# ------------------------------------------------------ workload configs --
+def test_c1_full_size():
+    key, v = c1_data()
+    ng = assert_parity([key], [(v, [(AGG["SUM"], 1), (AGG["COUNT_VALID"], 1), (AGG["MIN"], 1),
+                                    (AGG["MAX"], 1)])])
+    assert ng == 1000
+
+
+def test_c2_shape_reduced():
+    key, f64, i32 = c2_data(n=2_000_000, card=200_000, seed=5)
+    assert_parity([key], [(f64, C2_AGGS), (i32, C2_AGGS)])
+
+
+def test_c2_high_cardinality_reduced():
+    # ~10 rows per group, like the 100M-row / 10M-key workload
+    key, f64, i32 = c2_data(n=1_000_000, card=100_000, seed=6)
+    assert_parity([key], [(f64, C2_AGGS), (i32, C2_AGGS)])
+
+
+def test_c3_shape_reduced():
+    key1, key2, key3, f32, i64 = c3_data(n=300_000, seed=9, pool_size=2000)
+    aggs = [(AGG["SUM"], 1), (AGG["COUNT_VALID"], 1), (AGG["MAX"], 1)]
+    assert_parity([key1, key2, key3], [(f32, aggs), (i64, aggs)])
+
+
+def test_c2_full_size_properties():
+    """100M rows, 10M keys: properties independent of size (exact totals of
+    counts and integer sums, global min/max, group count)."""
+    key, f64, i32 = c2_data()
+    kcols, res = gpu_groupby([key], [(f64, C2_AGGS), (i32, C2_AGGS)])
+    G = len(kcols[0][0])
+    kvalid = key.valid
+    # every valid key value appears exactly once
+    assert len(np.unique(kcols[0][0])) == G
+    assert G == len(np.unique(key.data[kvalid]))
+    both = kvalid & i32.valid
+    cnt, cval = res[1][2]
+    assert int(cnt.sum()) == int(both.sum())
+    s, sval = res[1][0]
+    assert int(s[sval].sum()) == int(i32.data[both].astype(np.int64).sum())
+    mn, mnv = res[1][3]
+    mx, mxv = res[1][4]
+    assert int(mn[mnv].min()) == int(i32.data[both].min())
+    assert int(mx[mxv].max()) == int(i32.data[both].max())
+    fboth = kvalid & f64.valid
+    fcnt, _ = res[0][2]
+    assert int(fcnt.sum()) == int(fboth.sum())
+    fs, fsv = res[0][0]
+    assert abs(float(fs[fsv].sum()) - float(f64.data[fboth].sum())) < 1e-6 * float(np.abs(f64.data[fboth]).sum())

@@ -1,0 +1,354 @@
+"""Pins the CPU oracle before it is trusted as the GPU's checker.
+
+Each oracle function is checked against something other than itself:
+  - MurmurHash3_x86_32: smhasher's published verification value;
+  - the row hash: the reference's docstring row-hash vectors;
+  - group-by semantics: the reference's C++ known-answer tests (fixtures in
+    tests/golden/groupby_kats.json), an exact brute force (Python integers,
+    fractions and math.fsum) on small random inputs, and pandas -- the CPU
+    fallback of cudf.pandas -- on seeded data.
+A last test plants mistakes into a Python mirror of the row hash to show the
+golden vectors catch them.
+"""
+import json
+import math
+import os
+import struct
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+from tests.gbutil import AGG, host_col, oracle_groupby, sort_order
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+NP_TYPES = {"INT8": np.int8, "INT16": np.int16, "INT32": np.int32, "INT64": np.int64,
+            "UINT8": np.uint8, "UINT16": np.uint16, "UINT32": np.uint32, "UINT64": np.uint64,
+            "FLOAT32": np.float32, "FLOAT64": np.float64}
+
+
+def load(name):
+    with open(os.path.join(GOLDEN, name)) as f:
+        return json.load(f)
+
+
+# ---------------------------------------------------------------- hashing ----
+def test_smhasher_verification_value():
+    blob = b""
+    for i in range(256):
+        blob += struct.pack("<I", orc.murmur3_x86_32(bytes(range(i)), 256 - i))
+    expect = int(load("hash_vectors.json")["smhasher_verification_x86_32"], 16)
+    assert orc.murmur3_x86_32(blob, 0) == expect
+
+
+def _cols(spec):
+    return [host_col(np.array(c["values"], dtype=NP_TYPES[c["type"]])) for c in spec]
+
+
+def test_docstring_row_hashes_seed0():
+    for case in load("hash_vectors.json")["docstring"]:
+        if case["seed"] != 0:
+            continue
+        h = orc.hash_rows(_cols(case["columns"]), seed=0)
+        assert list(h.view(np.int32)) == case["expect_int32"], case["source"]
+
+
+def test_docstring_row_hash_seed42_documented_discrepancy():
+    case = [c for c in load("hash_vectors.json")["docstring"] if c["seed"] == 42][0]
+    cols = _cols(case["columns"])
+    assert list(orc.hash_rows(cols, seed=42, elem_seed=0)) == case["expect_uint32"]
+    assert list(orc.hash_rows(cols, seed=42)) != case["expect_uint32"]
+
+
+def _py_murmur(data: bytes, seed: int, mutate=None) -> int:
+    """Independent Python MurmurHash3_x86_32 used only to show that the
+    golden vectors reject plausible mistakes."""
+    m = 0xFFFFFFFF
+    c1, c2 = 0xCC9E2D51, 0x1B873593
+    rot1 = 15 if mutate != "rot" else 16
+    h = seed
+    n = len(data) // 4
+    for i in range(n):
+        k = int.from_bytes(data[4 * i:4 * i + 4], "little")
+        k = (k * c1) & m
+        k = ((k << rot1) | (k >> (32 - rot1))) & m
+        k = (k * c2) & m
+        h ^= k
+        h = ((h << 13) | (h >> 19)) & m
+        h = (h * 5 + 0xE6546B64) & m
+    tail = data[4 * n:]
+    k = 0
+    for j in reversed(range(len(tail))):
+        k ^= tail[j] << (8 * j)
+    if tail:
+        k = (k * c1) & m
+        k = ((k << rot1) | (k >> (32 - rot1))) & m
+        k = (k * c2) & m
+        h ^= k
+    if mutate != "len":
+        h ^= len(data)
+    h ^= h >> 16
+    h = (h * 0x85EBCA6B) & m
+    h ^= h >> 13
+    h = (h * 0xC2B2AE35) & m
+    h ^= h >> 16
+    return h
+
+
+def _py_row_hash(vals, mutate=None):
+    h = 0
+    for v in vals:
+        e = _py_murmur(struct.pack("<q", v), 0, mutate)
+        if mutate == "combine":
+            h = h ^ (e + 0x9E3779B9 + (h << 6) + (h >> 2)) & 0xFFFFFFFF  # wrong precedence
+        else:
+            h = (h ^ ((e + 0x9E3779B9 + (h << 6) + (h >> 2)) & 0xFFFFFFFF)) & 0xFFFFFFFF
+    return h
+
+
+@pytest.mark.parametrize("mutate", [None, "rot", "len"])
+def test_golden_vectors_reject_planted_mistakes(mutate):
+    case = load("hash_vectors.json")["docstring"][0]
+    got = [np.uint32(_py_row_hash([v], mutate)).view(np.int32) for v in case["columns"][0]["values"]]
+    ok = [int(x) for x in got] == case["expect_int32"]
+    assert ok == (mutate is None)
+
+
+def test_oracle_hash_matches_independent_python():
+    rng = np.random.default_rng(1)
+    for _ in range(50):
+        L = int(rng.integers(0, 40))
+        b = bytes(rng.integers(0, 256, L, dtype=np.uint8))
+        s = int(rng.integers(0, 2**32))
+        assert orc.murmur3_x86_32(b, s) == _py_murmur(b, s)
+
+
+def test_float_key_normalisation():
+    # -0.0 hashes like 0.0 and every NaN like the canonical NaN
+    # (hash_functions.cuh:28-47)
+    a = host_col(np.array([0.0, -0.0, np.nan, -np.nan, 1.5]))
+    h = orc.hash_rows([a])
+    assert h[0] == h[1] and h[2] == h[3] and h[0] != h[4]
+    nan_payload = np.array([0x7FF8000000000123], dtype=np.uint64).view(np.float64)
+    assert orc.hash_rows([host_col(nan_payload)])[0] == h[2]
+
+
+def test_null_hash_is_all_ones_combined():
+    c = host_col(np.array([5], np.int32), np.array([False]))
+    # hash_combine(0, 0xFFFFFFFF)
+    expect = (0 ^ ((0xFFFFFFFF + 0x9E3779B9) & 0xFFFFFFFF))
+    assert int(orc.hash_rows([c])[0]) == expect
+
+
+# --------------------------------------------------------------- group-by ----
+def _kat_value_types(case):
+    if "val_types" in case:
+        return case["val_types"]
+    types = ["INT8", "INT16", "INT32", "INT64", "UINT32", "FLOAT32", "FLOAT64"]
+    if case["agg"] in ("MIN", "MAX", "COUNT_VALID", "COUNT_ALL"):
+        types += ["UINT8", "UINT64"]
+    return types
+
+
+def _run_kat(case, vt, groupby_fn):
+    keys = host_col(np.array(case["keys"], np.int32),
+                    None if case["key_valid"] is None else np.array(case["key_valid"], bool))
+    vals = host_col(np.array(case["vals"], NP_TYPES[vt]),
+                    None if case["val_valid"] is None else np.array(case["val_valid"], bool))
+    policy = 0 if case["null_policy"] == "EXCLUDE" else 1
+    kcols, res = groupby_fn([keys], [(vals, [(AGG[case["agg"]], case["ddof"])])], policy)
+    order = sort_order(kcols)
+    kv, kvalid = kcols[0]
+    got_keys = [None if (kvalid is not None and not kvalid[i]) else int(kv[i]) for i in order]
+    exp_keys = [None if (case["expect_key_valid"] is not None and not case["expect_key_valid"][j])
+                else case["expect_keys"][j] for j in range(len(case["expect_keys"]))]
+    # expected lists put the null key last, as the sort token does
+    assert got_keys == exp_keys
+    v, valid = res[0][0]
+    valid = np.ones(len(v), bool) if valid is None else valid
+    exp_valid = case["expect_val_valid"] or [1] * len(case["expect_vals"])
+    for j, i in enumerate(order):
+        assert bool(valid[i]) == bool(exp_valid[j]), (case["name"], vt, j)
+        if exp_valid[j]:
+            e = case["expect_vals"][j]
+            if isinstance(e, float):
+                assert math.isclose(float(v[i]), e, rel_tol=1e-15), (case["name"], vt, v[i], e)
+            else:
+                assert int(v[i]) == e, (case["name"], vt, v[i], e)
+
+
+KATS = load("groupby_kats.json")
+
+
+@pytest.mark.parametrize("case", KATS["cases"], ids=[c["name"] for c in KATS["cases"]])
+def test_oracle_known_answers(case):
+    for vt in _kat_value_types(case):
+        _run_kat(case, vt, oracle_groupby)
+
+
+def test_oracle_string_keys_known_answer():
+    case = KATS["string_key_cases"][0]
+    keys = orc.HostColumn.from_strings([s.encode() for s in case["keys"]])
+    vals = host_col(np.array(case["vals"], np.int32))
+    kcols, res = oracle_groupby([keys], [(vals, [(AGG["SUM"], 1)])])
+    order = sort_order(kcols)
+    assert [kcols[0][0][i] for i in order] == sorted(s.encode() for s in case["expect_keys"])
+    exp = dict(zip([s.encode() for s in case["expect_keys"]], case["expect_vals"]))
+    for i in order:
+        assert int(res[0][0][0][i]) == exp[kcols[0][0][i]]
+
+
+def _brute(keys, vals, valid_k, valid_v, kind, ddof, policy):
+    """Exact group-by with Python integers / Fractions / fsum."""
+    groups = {}
+    order = []
+    for r in range(len(keys)):
+        kv = bool(valid_k[r])
+        if not kv and policy == 0:
+            continue
+        k = (kv, keys[r] if kv else None)
+        if k not in groups:
+            groups[k] = []
+            order.append(k)
+        groups[k].append(r)
+    out = {}
+    for k in order:
+        rows = groups[k]
+        xs = [float(vals[r]) if isinstance(vals[r], (float, np.floating)) else int(vals[r])
+              for r in rows if valid_v[r]]
+        if kind == "COUNT_ALL":
+            out[k] = (len(rows), True)
+        elif kind == "COUNT_VALID":
+            out[k] = (len(xs), True)
+        elif not xs:
+            out[k] = (None, False)
+        elif kind == "SUM":
+            if isinstance(xs[0], float):
+                out[k] = (math.fsum(xs), True)
+            else:
+                s = sum(int(x) for x in xs)
+                s = (s + 2**63) % 2**64 - 2**63
+                out[k] = (s, True)
+        elif kind == "MIN":
+            out[k] = (min(xs), True)
+        elif kind == "MAX":
+            out[k] = (max(xs), True)
+        elif kind == "MEAN":
+            out[k] = (float(sum(Fraction(x) for x in xs) / len(xs)), True)
+        elif kind == "VARIANCE":
+            n = len(xs)
+            if n - ddof <= 0:
+                out[k] = (None, False)
+            else:
+                fx = [Fraction(x) for x in xs]
+                m = sum(fx) / n
+                out[k] = (float(sum((x - m) ** 2 for x in fx) / (n - ddof)), True)
+    return out
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_oracle_matches_exact_brute_force(seed):
+    rng = np.random.default_rng(seed)
+    n = int(rng.integers(1, 300))
+    nk = int(rng.integers(1, 40))
+    keys = rng.integers(-5, nk, n).astype(np.int64)
+    valid_k = rng.random(n) > 0.1
+    is_float = seed % 2 == 0
+    vals = rng.standard_normal(n) * 100 if is_float else rng.integers(-10**12, 10**12, n)
+    valid_v = rng.random(n) > 0.2
+    policy = seed % 3 == 0
+    for kind in ["SUM", "MIN", "MAX", "COUNT_VALID", "COUNT_ALL", "MEAN", "VARIANCE"]:
+        ddof = int(rng.integers(0, 3)) if kind == "VARIANCE" else 1
+        kc = host_col(keys, valid_k)
+        vc = host_col(vals, valid_v)
+        kcols, res = oracle_groupby([kc], [(vc, [(AGG[kind], ddof)])], int(policy))
+        expect = _brute(list(keys), list(vals), valid_k, valid_v, kind, ddof, int(policy))
+        kv, kvalid = kcols[0]
+        got = {}
+        for i in range(len(kv)):
+            valid = True if kvalid is None else bool(kvalid[i])
+            got[(valid, kv[i] if valid else None)] = (res[0][0][0][i], bool(res[0][0][1][i]))
+        assert set(got) == set(expect)
+        for k, (e, ev) in expect.items():
+            g, gv = got[k]
+            assert gv == ev, (kind, k)
+            if not ev:
+                continue
+            if kind in ("MEAN", "VARIANCE"):
+                assert math.isclose(float(g), e, rel_tol=1e-14, abs_tol=1e-300), (kind, g, e)
+            elif isinstance(e, float):
+                assert float(g) == e, (kind, g, e)
+            else:
+                assert int(g) == int(e), (kind, g, e)
+
+
+def test_oracle_min_max_ignore_nan_like_reference():
+    # device_atomics.cuh:107-153 with std::min/std::max keeps the identity
+    # against NaN, so an all-NaN group yields +inf (MIN) / -inf (MAX).
+    keys = host_col(np.array([1, 1, 2, 2], np.int32))
+    vals = host_col(np.array([np.nan, np.nan, np.nan, 3.0]))
+    kcols, res = oracle_groupby([keys], [(vals, [(AGG["MIN"], 1), (AGG["MAX"], 1)])])
+    got = {int(k): (res[0][0][0][i], res[0][1][0][i]) for i, k in enumerate(kcols[0][0])}
+    assert got[1][0] == np.inf and got[1][1] == -np.inf
+    assert got[2] == (3.0, 3.0)
+
+
+def test_oracle_vs_pandas_c1():
+    pd = pytest.importorskip("pandas")
+    rng = np.random.default_rng(0)
+    n = 200_000
+    key = rng.integers(0, 1000, n, dtype=np.int32)
+    v = rng.integers(-10**6, 10**6, n, endpoint=True, dtype=np.int64)
+    kcols, res = oracle_groupby([host_col(key)], [(host_col(v), [(AGG["SUM"], 1), (AGG["COUNT_VALID"], 1),
+                                                                 (AGG["MIN"], 1), (AGG["MAX"], 1)])])
+    df = pd.DataFrame({"k": key, "v": v})
+    e = df.groupby("k", sort=True)["v"].agg(["sum", "count", "min", "max"])
+    order = np.argsort(kcols[0][0])
+    assert np.array_equal(kcols[0][0][order], e.index.to_numpy())
+    for a, name in enumerate(["sum", "count", "min", "max"]):
+        assert np.array_equal(res[0][a][0][order].astype(np.int64), e[name].to_numpy().astype(np.int64))
+
+
+def test_oracle_vs_pandas_nullable_c2_shape():
+    pd = pytest.importorskip("pandas")
+    rng = np.random.default_rng(3)
+    n, card = 100_000, 10_000
+    key = rng.integers(0, card, n, dtype=np.int64)
+    kvalid = rng.random(n) >= 0.01
+    f64 = rng.standard_normal(n)
+    fvalid = rng.random(n) >= 0.05
+    i32 = rng.integers(0, 10_000, n, endpoint=True, dtype=np.int32)
+    ivalid = rng.random(n) >= 0.05
+    aggs = [(AGG["SUM"], 1), (AGG["MEAN"], 1), (AGG["COUNT_VALID"], 1), (AGG["MIN"], 1),
+            (AGG["MAX"], 1), (AGG["VARIANCE"], 1)]
+    kcols, res = oracle_groupby([host_col(key, kvalid)],
+                                [(host_col(f64, fvalid), aggs), (host_col(i32, ivalid), aggs)])
+    df = pd.DataFrame({
+        "k": pd.arrays.IntegerArray(key, ~kvalid),
+        "f": pd.arrays.FloatingArray(f64, ~fvalid),
+        "i": pd.arrays.IntegerArray(i32, ~ivalid),
+    })
+    g = df.groupby("k", sort=True, dropna=True)
+    order = np.argsort(kcols[0][0])
+    assert np.array_equal(kcols[0][0][order], g.size().index.to_numpy().astype(np.int64))
+    for r, col in enumerate(["f", "i"]):
+        s = g[col]
+        expect = {
+            0: s.sum(min_count=1), 1: s.mean(), 2: s.count(), 3: s.min(), 4: s.max(), 5: s.var(ddof=1),
+        }
+        for a in range(6):
+            ov, ovalid = res[r][a]
+            ov, ovalid = ov[order], ovalid[order]
+            ex = expect[a]
+            exv = ex.notna().to_numpy()
+            assert np.array_equal(ovalid, exv), (col, a)
+            exn = ex.to_numpy(dtype=np.float64, na_value=np.nan)[exv]
+            got = ov[ovalid].astype(np.float64)
+            if a in (2, 3, 4) or (a == 0 and col == "i"):
+                assert np.array_equal(got, exn), (col, a)
+            elif a == 5:
+                assert np.allclose(got, exn, rtol=1e-10, atol=0), (col, a)
+            else:
+                assert np.allclose(got, exn, rtol=1e-12, atol=1e-12), (col, a)
