@@ -118,6 +118,8 @@ def main():
     ap.add_argument("--card", type=int, default=CARD)
     ap.add_argument("--cpu-sample", type=int, default=4_000_000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--strategy", choices=["global", "partitioned"], default=None,
+                    help="force an aggregation strategy (default: chosen per call)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the group-by is measured on a "
@@ -137,16 +139,17 @@ def main():
     reqs = [ca.GroupByRequest(fcol, aggs()), ca.GroupByRequest(icol, aggs())]
     _lib.set_phase_timing(True)
 
+    if args.strategy:
+        _lib.set_strategy({"global": 1, "partitioned": 2}[args.strategy])
     for _ in range(args.warmup):
         out = gb.aggregate(reqs)
         del out
     torch.cuda.synchronize()
-    acc_ms, phase = [], {}
+    phase = {}
     t0 = time.perf_counter()
     for _ in range(args.steps):
         keys_t, res = gb.aggregate(reqs)
         st = _lib.last_stats()
-        acc_ms.append(st["t_accumulate_ms"])
         for k, v in st.items():
             if k.startswith("t_"):
                 phase[k] = phase.get(k, 0.0) + v / args.steps
@@ -156,10 +159,25 @@ def main():
     G = keys_t.num_rows()
     in_b = input_bytes(args.rows)
     out_b = output_bytes(keys_t, res)
-    t_acc = float(np.mean(acc_ms)) * 1e-3
-    achieved = in_b / t_acc if t_acc > 0 else 0.0
+    # compulsory bytes of each kernel of the strategy that ran (rows moved by
+    # the partition passes carry key + values + one flags byte)
+    nv = args.rows - kcol.null_count
+    rec = 8 + 8 + 4 + 1
+    if st["strategy"] == 2:
+        kernels = {
+            "k_rpa_hist": (phase.get("t_rpa_hist_ms", 0), args.rows * 8 + 4 * ((args.rows + 31) // 32)),
+            "k_rpa_scatter1": (phase.get("t_rpa_scatter1_ms", 0), in_b + nv * rec),
+            "k_rpa_scatter2": (phase.get("t_rpa_scatter2_ms", 0), 2 * nv * rec),
+            "k_rpa_aggregate": (phase.get("t_rpa_aggregate_ms", 0), nv * rec + out_b),
+        }
+    else:
+        kernels = {"k_aggregate_global": (phase.get("t_accumulate_ms", 0), in_b)}
+    dom = max(kernels, key=lambda k: kernels[k][0])
+    t_dom = kernels[dom][0] * 1e-3
+    dom_bytes = kernels[dom][1]
+    achieved = dom_bytes / t_dom if t_dom > 0 else 0.0
     traffic = None
-    tf = os.path.join(ROOT, "profiles", "traffic_accumulate_c2.json")
+    tf = os.path.join(ROOT, "profiles", f"traffic_{dom}_c2.json")
     if os.path.exists(tf):
         with open(tf) as f:
             traffic = json.load(f).get("bytes_per_launch")
@@ -181,13 +199,15 @@ def main():
                                "(5% null each); sum, mean, count, min, max, var(ddof=1) on both",
                    "rows": args.rows, "distinct_keys": args.card, "groups": G,
                    "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "kernel": "k_aggregate_global (accumulate pass)",
+        "roofline": {"bound": "hbm", "kernel": dom,
                      "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": traffic,
-                     "algorithmic_bytes": in_b, "avg_ms": t_acc * 1e3},
-        "pipeline": {"bytes_alg": in_b + out_b, "bytes_per_row": (in_b + out_b) / args.rows,
+                     "algorithmic_bytes": dom_bytes, "avg_ms": t_dom * 1e3},
+        "pipeline": {"strategy": {1: "global", 2: "partitioned"}.get(int(st["strategy"]), "?"),
+                     "bytes_alg": in_b + out_b, "bytes_per_row": (in_b + out_b) / args.rows,
                      "achieved_GBps": (in_b + out_b) / (ms_per_step * 1e-3) / 1e9,
                      "frac": (in_b + out_b) / (ms_per_step * 1e-3) / HBM_PEAK,
+                     "kernels_ms": {k: round(v[0], 4) for k, v in kernels.items()},
                      "phase_ms": {k: round(v, 4) for k, v in phase.items()}},
     }
     if not args.no_cpu:

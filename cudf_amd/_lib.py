@@ -110,11 +110,20 @@ SIGNATURES = {
     "mig_result_free": (None, [ctypes.c_void_p]),
     "mig_last_stats": (ctypes.c_int32, [ctypes.POINTER(ctypes.c_double), ctypes.c_int32]),
     "mig_set_phase_timing": (None, [ctypes.c_int32]),
+    "mig_set_strategy": (None, [ctypes.c_int32]),
 }
 
 STAT_NAMES = ["strategy", "key_mode", "capacity", "est_groups", "groups", "retries", "state_bytes",
               "t_estimate_ms", "t_init_ms", "t_accumulate_ms", "t_second_pass_ms", "t_compact_ms",
-              "t_outputs_ms"]
+              "t_outputs_ms", "t_rpa_hist_ms", "t_rpa_scatter1_ms", "t_rpa_scatter2_ms",
+              "t_rpa_aggregate_ms"]
+
+STRATEGY_AUTO, STRATEGY_GLOBAL, STRATEGY_PARTITIONED = 0, 1, 2
+
+
+def set_strategy(strategy: int) -> None:
+    """Force an aggregation strategy process-wide (0 = automatic)."""
+    lib().mig_set_strategy(int(strategy))
 
 _lib = None
 

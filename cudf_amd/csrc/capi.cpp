@@ -148,4 +148,9 @@ int32_t mig_last_stats(double* values, int32_t n)
 
 void mig_set_phase_timing(int32_t enable) { mig::set_phase_timing(enable != 0); }
 
+void mig_set_strategy(int32_t strategy)
+{
+  if (strategy >= MIG_STRATEGY_AUTO && strategy <= MIG_STRATEGY_PARTITIONED) mig::set_strategy(strategy);
+}
+
 }  // extern "C"

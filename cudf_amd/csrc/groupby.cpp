@@ -33,10 +33,18 @@ double* thread_stats()
 
 namespace {
 std::atomic<bool> g_phase_timing{false};
+std::atomic<int> g_strategy{MIG_STRATEGY_AUTO};
+
+// Partitioned path: LDS hash slots and byte budget of the aggregating
+// workgroup (two 512-thread workgroups fit in a CU's 160 KB of LDS).
+constexpr int32_t RPA_LDS_SLOTS      = 2048;
+constexpr int32_t RPA_LDS_BUDGET     = 78 * 1024;
+constexpr int RPA_MAX_LOG_BINS       = 15;
+constexpr int RPA_MAX_LOG_FANOUT     = 8;
 
 // hipEvents marking phase boundaries on the call's stream.
 struct phase_timer {
-  static constexpr int N = 8;
+  static constexpr int N = 16;
   bool on;
   hipStream_t stream;
   hipEvent_t ev[N];
@@ -71,6 +79,7 @@ struct phase_timer {
 }  // namespace
 
 void set_phase_timing(bool on) { g_phase_timing.store(on); }
+void set_strategy(int s) { g_strategy.store(s); }
 
 // ---------------------------------------------------------------------------
 void* device_allocator::allocate(size_t bytes)
@@ -112,6 +121,15 @@ device_props const& props()
   if (it != cache.end()) return it->second;
   device_props p;
   MIG_HIP_TRY(hipDeviceGetAttribute(&p.num_cus, hipDeviceAttributeMultiprocessorCount, dev));
+  // Keep freed scratch in the stream-ordered pool instead of returning it to
+  // the driver at every synchronisation: a group-by's table and partition
+  // buffers are gigabytes, and re-mapping them per call costs far more than
+  // the kernels.
+  hipMemPool_t pool;
+  if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) {
+    uint64_t keep = ~0ull;
+    (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+  }
   return cache.emplace(dev, p).first->second;
 }
 
@@ -552,7 +570,7 @@ std::unique_ptr<mig_groupby_result> groupby_aggregate(mig_column_view const* key
   P.num_vals   = int32_t(vals.size());
   P.num_fields = nf;
 
-  // ---- capacity -----------------------------------------------------------
+  // ---- cardinality estimate ------------------------------------------------
   scratch S(stream);
   gb_params* dP = S.get<gb_params>(1);
   auto* pin = static_cast<uint8_t*>(thread_pinned().get(2 * sizeof(gb_params) + 4 * SAMPLE_ROWS + 64));
@@ -580,186 +598,370 @@ std::unique_ptr<mig_groupby_result> groupby_aggregate(mig_column_view const* key
   }
   stats[STAT_EST_GROUPS] = est;
   T.mark(1);
-  uint64_t cap = std::max<uint64_t>(64, next_pow2(uint64_t(est * 2.6) + 1));
-  cap = std::min(cap, max_cap);
 
-  // ---- build + accumulate, growing the table on overflow ---------------------
-  int32_t* d_flags = S.get<int32_t>(4);  // [0] overflow, [1..2] special occupancy
-  uint32_t* map = nullptr;
-  uint32_t* block_counts = nullptr;
-  int64_t G = 0;
-  int retries = 0;
-  for (;;) {
-    P.cap         = cap;
-    P.cap_mask    = cap - 1;
-    P.probe_limit = uint32_t(cap >= max_cap ? std::min<uint64_t>(cap, 0xffffffffull)
-                                            : std::min<uint64_t>(cap, 4096));
-    P.overflow    = d_flags;
-    P.special     = reinterpret_cast<uint32_t*>(d_flags + 1);
-    size_t state_bytes = 0;
-    uint64_t const total = cap + 2;
-    if (P.key_mode == KM_INLINE) P.slot_keys = S.get<uint64_t>(cap);
-    else P.slot_rows = S.get<uint64_t>(cap);
-    state_bytes += 8 * cap;
-    P.slot_count = P.need_count_all ? S.get<uint32_t>(total) : nullptr;
-    if (P.slot_count) state_bytes += 4 * total;
-    for (int f = 0; f < nf; f++) {
-      P.fields[f].ptr = P.fields[f].width == 8 ? static_cast<void*>(S.get<uint64_t>(total))
-                                               : static_cast<void*>(S.get<uint32_t>(total));
-      state_bytes += size_t(P.fields[f].width) * total;
-    }
-    stats[STAT_STATE_BYTES] = double(state_bytes);
-    upload(0);
-    launch_init_state(dP, P, stream);
-    T.mark(2);
-    launch_aggregate(dP, P, dp.num_cus, stream);
-    T.mark(3);
-    MIG_HIP_TRY(hipGetLastError());
-
-    int64_t nb   = (int64_t(cap) + 2 + SCAN_TILE - 1) / SCAN_TILE;
-    block_counts = S.get<uint32_t>(nb + 1);
-    launch_count_occupied(dP, P, block_counts, stream);
-    launch_scan_blocks(block_counts, nb, block_counts + nb, stream);
-    auto* h_flags = reinterpret_cast<uint32_t*>(pin + 2 * sizeof(gb_params));
-    MIG_HIP_TRY(hipMemcpyAsync(h_flags, d_flags, 4, hipMemcpyDeviceToHost, stream));
-    MIG_HIP_TRY(hipMemcpyAsync(h_flags + 1, block_counts + nb, 4, hipMemcpyDeviceToHost, stream));
-    MIG_HIP_TRY(hipStreamSynchronize(stream));
-    if (h_flags[0] == 0) {
-      G = h_flags[1];
-      break;
-    }
-    MIG_EXPECTS(cap < max_cap, "hash table overflow at maximum capacity");
-    cap = std::min(max_cap, cap * 4);
-    retries++;
-  }
-  stats[STAT_KEY_MODE] = P.key_mode;
-  stats[STAT_CAPACITY] = double(cap);
-  stats[STAT_RETRIES]  = retries;
-  stats[STAT_GROUPS]   = double(G);
-
-  T.mark(4);
-  if (P.need_second_pass) launch_second_pass(dP, P, dp.num_cus, stream);
-  T.mark(5);
-  map = S.get<uint32_t>(G);
-  launch_compact(dP, P, block_counts, map, stream);
-  T.mark(6);
-
-  // ---- outputs ----------------------------------------------------------------
-  res->num_groups = int32_t(G);
-  // keys
+  // ---- output columns ------------------------------------------------------
+  // Fixed-width columns get room for `rows_cap` rows and report `G` rows (the
+  // partitioned path allocates before the exact group count is known).  Masks
+  // are zeroed when `zero_masks` (they are then set bit by bit).
   int n_str_keys = 0;
-  for (int i = 0; i < num_keys; i++) {
-    bool const with_mask = keys[i].null_mask != nullptr;
-    if (storage_of(keys[i].type_id) == ST_STRING) {
-      owned_column oc;
-      oc.col.type_id         = MIG_TYPE_STRING;
-      oc.col.size            = int32_t(G);
-      oc.col.offsets_type_id = MIG_TYPE_INT32;
-      oc.offsets_bytes       = sizeof(int32_t) * size_t(G + 1);
-      oc.col.offsets         = A.allocate(oc.offsets_bytes);
-      if (with_mask) {
-        oc.mask_bytes    = std::max<size_t>(4, size_t((G + 31) / 32) * 4);
-        oc.col.null_mask = static_cast<uint32_t*>(A.allocate(oc.mask_bytes));
-      }
-      res->keys.push_back(oc);
-      n_str_keys++;
-    } else {
-      res->keys.push_back(make_owned_fixed(A, keys[i].type_id, G, with_mask));
-    }
-    P.out_keys[i]     = storage_of(keys[i].type_id) == ST_STRING ? nullptr : res->keys.back().col.data;
-    P.out_key_mask[i] = res->keys.back().col.null_mask;
-  }
-  // aggregation results in request order (duplicates computed again from the
-  // same accumulators, as extract_results copies them)
-  int no = 0;
-  for (int r = 0; r < num_requests; r++) {
-    value_plan const& vp = vals[vcol_of_request[r]];
-    std::vector<owned_column> cols;
-    for (int a = 0; a < requests[r].num_aggregations; a++) {
-      int32_t const kind = requests[r].aggregations[a].kind;
-      int32_t const tt   = target_type_of(vp.view.type_id, kind);
-      MIG_EXPECTS(no < MAX_OUTS, "too many aggregation results");
-      out_desc& O   = P.outs[no++];
-      O.count_field = -1;
-      O.src_storage = vp.storage;
-      O.ddof        = requests[r].aggregations[a].ddof;
-      O.out_width   = type_width(tt);
-      bool with_mask = false;
-      switch (kind) {
-        case MIG_AGG_SUM:
-        case MIG_AGG_SUM_OF_SQUARES:
-          O.field = kind == MIG_AGG_SUM ? vp.f_sum : vp.f_sumsq;
-          O.kind  = tt == MIG_TYPE_FLOAT32 ? OK_COPY_F32 : (tt == MIG_TYPE_FLOAT64 ? OK_COPY_F64 : OK_COPY_I64);
-          if (vp.has_nulls) { O.count_field = vp.f_count; with_mask = true; }
-          break;
-        case MIG_AGG_MIN:
-        case MIG_AGG_MAX:
-          O.field = kind == MIG_AGG_MIN ? vp.f_min : vp.f_max;
-          O.kind  = OK_MINMAX;
-          if (vp.has_nulls) { O.count_field = vp.f_count; with_mask = true; }
-          break;
-        case MIG_AGG_COUNT_VALID:
-          if (vp.f_count >= 0) { O.kind = OK_COUNT; O.field = vp.f_count; }
-          else O.kind = OK_COUNT_ALL;
-          break;
-        case MIG_AGG_COUNT_ALL: O.kind = OK_COUNT_ALL; break;
-        case MIG_AGG_MEAN:
-          O.kind = OK_MEAN;
-          O.field = vp.f_sum;
-          O.count_field = vp.f_count;
-          with_mask = vp.has_nulls;
-          break;
-        case MIG_AGG_VARIANCE:
-        case MIG_AGG_STD:
-          O.kind = kind == MIG_AGG_VARIANCE ? OK_VAR : OK_STD;
-          O.field = vp.f_m2;
-          O.count_field = vp.f_count;
-          with_mask = true;
-          break;
-        default: fail(MIG_ERROR_LOGIC, "unreachable aggregation kind");
-      }
-      owned_column oc = make_owned_fixed(A, tt, G, with_mask);
-      O.data = oc.col.data;
-      O.mask = oc.col.null_mask;
-      cols.push_back(oc);
-    }
-    res->results.push_back(std::move(cols));
-  }
-  P.num_outs = no;
-
-  uint32_t* rep_rows = P.key_mode == KM_GENERAL ? S.get<uint32_t>(G) : nullptr;
-  upload(1);  // same device block: the previous kernels are stream-ordered before this copy
-  launch_outputs(dP, P, map, G, rep_rows, stream);
-  if (P.key_mode == KM_GENERAL) {
-    int32_t* str_sizes = n_str_keys > 0 ? S.get<int32_t>(int64_t(n_str_keys) * G + 1) : nullptr;
-    launch_gather_keys(dP, P, rep_rows, G, str_sizes, stream);
-    int si = 0;
+  auto alloc_outputs = [&](int64_t rows_cap, int64_t G, bool zero_masks) {
+    res->keys.clear();
+    res->results.clear();
+    n_str_keys = 0;
+    auto fixed = [&](int32_t type_id, bool with_mask) {
+      owned_column oc = make_owned_fixed(A, type_id, rows_cap, with_mask);
+      oc.col.size     = int32_t(G);
+      if (with_mask && zero_masks && rows_cap > 0)
+        MIG_HIP_TRY(hipMemsetAsync(oc.col.null_mask, 0, oc.mask_bytes, stream));
+      return oc;
+    };
     for (int i = 0; i < num_keys; i++) {
-      if (storage_of(keys[i].type_id) != ST_STRING) continue;
-      owned_column& oc = res->keys[i];
-      auto* offs       = static_cast<int32_t*>(oc.col.offsets);
-      if (G == 0) {
-        MIG_HIP_TRY(hipMemsetAsync(offs, 0, sizeof(int32_t), stream));
-        si++;
-        continue;
+      bool const with_mask = keys[i].null_mask != nullptr;
+      if (storage_of(keys[i].type_id) == ST_STRING) {
+        owned_column oc;
+        oc.col.type_id         = MIG_TYPE_STRING;
+        oc.col.size            = int32_t(G);
+        oc.col.offsets_type_id = MIG_TYPE_INT32;
+        oc.offsets_bytes       = sizeof(int32_t) * size_t(G + 1);
+        oc.col.offsets         = A.allocate(oc.offsets_bytes);
+        if (with_mask) {
+          oc.mask_bytes    = std::max<size_t>(4, size_t((G + 31) / 32) * 4);
+          oc.col.null_mask = static_cast<uint32_t*>(A.allocate(oc.mask_bytes));
+        }
+        res->keys.push_back(oc);
+        n_str_keys++;
+      } else {
+        res->keys.push_back(fixed(keys[i].type_id, with_mask));
       }
-      MIG_HIP_TRY(hipMemcpyAsync(offs, str_sizes + int64_t(si) * G, sizeof(int32_t) * G,
-                                 hipMemcpyDeviceToDevice, stream));
-      launch_exclusive_scan_i32(offs, G, offs + G, stream);
-      int32_t total_chars = 0;
-      auto* h = reinterpret_cast<int32_t*>(pin + 2 * sizeof(gb_params));
-      MIG_HIP_TRY(hipMemcpyAsync(h, offs + G, 4, hipMemcpyDeviceToHost, stream));
+      P.out_keys[i]     = storage_of(keys[i].type_id) == ST_STRING ? nullptr : res->keys.back().col.data;
+      P.out_key_mask[i] = res->keys.back().col.null_mask;
+    }
+    // aggregation results in request order (duplicates computed again from
+    // the same accumulators, as extract_results copies them)
+    int no = 0;
+    for (int r = 0; r < num_requests; r++) {
+      value_plan const& vp = vals[vcol_of_request[r]];
+      std::vector<owned_column> cols;
+      for (int a = 0; a < requests[r].num_aggregations; a++) {
+        int32_t const kind = requests[r].aggregations[a].kind;
+        int32_t const tt   = target_type_of(vp.view.type_id, kind);
+        MIG_EXPECTS(no < MAX_OUTS, "too many aggregation results");
+        out_desc& O   = P.outs[no++];
+        O             = out_desc{};
+        O.count_field = -1;
+        O.src_storage = vp.storage;
+        O.ddof        = requests[r].aggregations[a].ddof;
+        O.out_width   = type_width(tt);
+        bool with_mask = false;
+        switch (kind) {
+          case MIG_AGG_SUM:
+          case MIG_AGG_SUM_OF_SQUARES:
+            O.field = kind == MIG_AGG_SUM ? vp.f_sum : vp.f_sumsq;
+            O.kind  = tt == MIG_TYPE_FLOAT32 ? OK_COPY_F32 : (tt == MIG_TYPE_FLOAT64 ? OK_COPY_F64 : OK_COPY_I64);
+            if (vp.has_nulls) { O.count_field = vp.f_count; with_mask = true; }
+            break;
+          case MIG_AGG_MIN:
+          case MIG_AGG_MAX:
+            O.field = kind == MIG_AGG_MIN ? vp.f_min : vp.f_max;
+            O.kind  = OK_MINMAX;
+            if (vp.has_nulls) { O.count_field = vp.f_count; with_mask = true; }
+            break;
+          case MIG_AGG_COUNT_VALID:
+            if (vp.f_count >= 0) { O.kind = OK_COUNT; O.field = vp.f_count; }
+            else O.kind = OK_COUNT_ALL;
+            break;
+          case MIG_AGG_COUNT_ALL: O.kind = OK_COUNT_ALL; break;
+          case MIG_AGG_MEAN:
+            O.kind        = OK_MEAN;
+            O.field       = vp.f_sum;
+            O.count_field = vp.f_count;
+            with_mask     = vp.has_nulls;
+            break;
+          case MIG_AGG_VARIANCE:
+          case MIG_AGG_STD:
+            O.kind        = kind == MIG_AGG_VARIANCE ? OK_VAR : OK_STD;
+            O.field       = vp.f_m2;
+            O.count_field = vp.f_count;
+            with_mask     = true;
+            break;
+          default: fail(MIG_ERROR_LOGIC, "unreachable aggregation kind");
+        }
+        owned_column oc = fixed(tt, with_mask);
+        O.data          = oc.col.data;
+        O.mask          = oc.col.null_mask;
+        cols.push_back(oc);
+      }
+      res->results.push_back(std::move(cols));
+    }
+    P.num_outs = no;
+  };
+  auto release_outputs = [&]() {
+    for (auto& k : res->keys) {
+      A.deallocate(k.col.data, k.data_bytes);
+      A.deallocate(k.col.null_mask, k.mask_bytes);
+      A.deallocate(k.col.offsets, k.offsets_bytes);
+    }
+    for (auto& rc : res->results)
+      for (auto& c : rc) {
+        A.deallocate(c.col.data, c.data_bytes);
+        A.deallocate(c.col.null_mask, c.mask_bytes);
+      }
+    res->keys.clear();
+    res->results.clear();
+  };
+
+  int64_t G = -1;
+
+  // ---- strategy: radix-partitioned LDS aggregation for high cardinality -----
+  int const forced = g_strategy.load();
+  bool rpa_ok      = P.key_mode == KM_INLINE;
+  int n_flagged    = 0;
+  for (int v = 0; v < P.num_vals && rpa_ok; v++) {
+    if (P.val_field_begin[v] == P.val_field_begin[v + 1]) continue;
+    if (vals[v].storage == ST_STRING) rpa_ok = false;
+    if (vals[v].has_nulls) n_flagged++;
+  }
+  rpa_ok = rpa_ok && n_flagged <= RPA_MAX_FLAGGED;
+  // LDS layout of the aggregating workgroup
+  rpa_params R{};
+  int32_t acc_bytes = 0;
+  for (int f = 0; f < nf; f++) acc_bytes += P.fields[f].width;
+  if (P.need_count_all) acc_bytes += 4;
+  int32_t const S_slots = RPA_LDS_SLOTS;
+  int32_t const fixed_lds = S_slots * 8 + S_slots * 2 + int32_t(sizeof(uint64_t)) * 4 + 64;
+  int32_t D = acc_bytes > 0 ? (RPA_LDS_BUDGET - fixed_lds) / acc_bytes : 4096;
+  D = std::min<int32_t>(D, 4096);
+  rpa_ok = rpa_ok && D >= 64;
+  bool use_rpa = false;
+  if (forced == MIG_STRATEGY_PARTITIONED) use_rpa = rpa_ok;
+  else if (forced == MIG_STRATEGY_AUTO) use_rpa = rpa_ok && N >= (1 << 20) && est >= 65536;
+
+  if (use_rpa) {
+    // LDS offsets (8-byte aligned)
+    int32_t off = 0;
+    off += S_slots * 8;
+    R.ids_off = off;
+    off += S_slots * 2;
+    off = (off + 7) & ~7;
+    for (int f = 0; f < nf; f++) {
+      P.fields[f].lds_off = off;
+      off += ((D * P.fields[f].width) + 7) & ~7;
+    }
+    R.count_all_off = -1;
+    if (P.need_count_all) {
+      R.count_all_off = off;
+      off += ((D * 4) + 7) & ~7;
+    }
+    R.misc_off  = off;
+    off += 64;
+    R.lds_bytes = off;
+    R.lds_slots = S_slots;
+    R.max_ids   = D;
+    int flag_bit = 0;
+    for (int v = 0; v < P.num_vals; v++)
+      R.val_flag_bit[v] = (P.val_field_begin[v] < P.val_field_begin[v + 1] && vals[v].has_nulls) ? flag_bit++ : -1;
+    double const per_bin = 0.75 * double(D - 2);
+    int log_bins = 1;
+    while (log_bins < RPA_MAX_LOG_BINS && double(int64_t(1) << log_bins) * per_bin < est) log_bins++;
+    for (int attempt = 0; attempt < 3 && G < 0; attempt++) {
+      R.log_bins  = log_bins;
+      R.two_level = log_bins > RPA_MAX_LOG_FANOUT;
+      R.log_f1    = R.two_level ? (log_bins + 1) / 2 : log_bins;
+      int64_t const B  = int64_t(1) << log_bins;
+      int64_t const F1 = int64_t(1) << R.log_f1;
+      int const lf2    = log_bins - R.log_f1;
+      R.hist        = S.get<uint32_t>(B);
+      R.bin_start   = S.get<int64_t>(B + 1);
+      R.cursor1     = S.get<unsigned long long>(F1);
+      R.cursor2     = S.get<unsigned long long>(B);
+      R.tile_start1 = S.get<int64_t>(F1 + 1);
+      int32_t* d_small = S.get<int32_t>(4);
+      R.overflow      = d_small;
+      R.group_counter = reinterpret_cast<unsigned long long*>(S.get<uint64_t>(1));
+      rpa_params* dR  = S.get<rpa_params>(1);
+      MIG_HIP_TRY(hipMemsetAsync(R.hist, 0, sizeof(uint32_t) * B, stream));
+      MIG_HIP_TRY(hipMemsetAsync(d_small, 0, sizeof(int32_t) * 4, stream));
+      MIG_HIP_TRY(hipMemsetAsync(R.group_counter, 0, sizeof(uint64_t), stream));
+      upload(0);
+      MIG_HIP_TRY(hipMemcpyAsync(dR, &R, sizeof(rpa_params), hipMemcpyHostToDevice, stream));
+      T.mark(8);
+      launch_rpa_hist(dP, dR, R, N, dp.num_cus, stream);
+      T.mark(9);
+      std::vector<uint32_t> hist(B);
+      MIG_HIP_TRY(hipMemcpyAsync(hist.data(), R.hist, sizeof(uint32_t) * B, hipMemcpyDeviceToHost, stream));
       MIG_HIP_TRY(hipStreamSynchronize(stream));
-      total_chars       = *h;
-      MIG_EXPECTS(total_chars >= 0, "unique string keys exceed 2^31 characters");
-      oc.data_bytes     = size_t(total_chars);
-      oc.col.data       = A.allocate(oc.data_bytes);
-      oc.col.data_bytes = total_chars;
-      launch_gather_string_chars(P.keys[i], rep_rows, G, offs, static_cast<uint8_t*>(oc.col.data), stream);
-      si++;
+      // bin layout, cursors and the second-level tile table
+      std::vector<int64_t> start(B + 1), t1(F1 + 1);
+      std::vector<unsigned long long> c1(F1), c2(B);
+      int64_t rows_cap = 0;
+      start[0] = 0;
+      for (int64_t b = 0; b < B; b++) {
+        start[b + 1] = start[b] + hist[b];
+        c2[b]        = (unsigned long long)start[b];
+        rows_cap += std::min<int64_t>(hist[b], D);
+      }
+      int64_t const nrows = start[B];
+      t1[0] = 0;
+      for (int64_t b1 = 0; b1 < F1; b1++) {
+        c1[b1]      = (unsigned long long)start[b1 << lf2];
+        int64_t len = start[(b1 + 1) << lf2] - start[b1 << lf2];
+        t1[b1 + 1]  = t1[b1] + (len + RPA_TILE - 1) / RPA_TILE;
+      }
+      R.num_tiles1 = t1[F1];
+      MIG_HIP_TRY(hipMemcpyAsync(R.bin_start, start.data(), sizeof(int64_t) * (B + 1), hipMemcpyHostToDevice, stream));
+      MIG_HIP_TRY(hipMemcpyAsync(R.cursor1, c1.data(), sizeof(unsigned long long) * F1, hipMemcpyHostToDevice, stream));
+      MIG_HIP_TRY(hipMemcpyAsync(R.cursor2, c2.data(), sizeof(unsigned long long) * B, hipMemcpyHostToDevice, stream));
+      MIG_HIP_TRY(hipMemcpyAsync(R.tile_start1, t1.data(), sizeof(int64_t) * (F1 + 1), hipMemcpyHostToDevice, stream));
+      // partition buffers
+      size_t part_bytes = 0;
+      R.keysB  = S.get<uint64_t>(nrows);
+      R.flagsB = S.get<uint8_t>(nrows);
+      part_bytes += 9 * size_t(nrows);
+      if (R.two_level) {
+        R.keysA  = S.get<uint64_t>(nrows);
+        R.flagsA = S.get<uint8_t>(nrows);
+        part_bytes += 9 * size_t(nrows);
+      }
+      for (int v = 0; v < P.num_vals; v++) {
+        if (P.val_field_begin[v] == P.val_field_begin[v + 1]) continue;
+        size_t w  = size_t(storage_width(vals[v].storage));
+        R.valsB[v] = S.get<uint8_t>(w * nrows);
+        part_bytes += w * nrows;
+        if (R.two_level) {
+          R.valsA[v] = S.get<uint8_t>(w * nrows);
+          part_bytes += w * nrows;
+        }
+      }
+      stats[STAT_STATE_BYTES] = double(part_bytes);
+      alloc_outputs(rows_cap, 0, true);
+      R.out_capacity = uint64_t(rows_cap);
+      upload(0);
+      MIG_HIP_TRY(hipMemcpyAsync(dR, &R, sizeof(rpa_params), hipMemcpyHostToDevice, stream));
+      T.mark(10);
+      launch_rpa_scatter1(dP, dR, N, dp.num_cus, stream);
+      T.mark(11);
+      if (R.two_level) launch_rpa_scatter2(dP, dR, R, dp.num_cus, stream);
+      T.mark(12);
+      launch_rpa_aggregate(dP, dR, R, stream);
+      T.mark(13);
+      MIG_HIP_TRY(hipGetLastError());
+      auto* h = reinterpret_cast<uint64_t*>(pin + 2 * sizeof(gb_params));
+      MIG_HIP_TRY(hipMemcpyAsync(h, R.group_counter, 8, hipMemcpyDeviceToHost, stream));
+      MIG_HIP_TRY(hipMemcpyAsync(h + 1, d_small, 4, hipMemcpyDeviceToHost, stream));
+      MIG_HIP_TRY(hipStreamSynchronize(stream));
+      if (uint32_t(h[1]) == 0) {
+        G = int64_t(h[0]);
+        for (auto& k : res->keys) k.col.size = int32_t(G);
+        for (auto& rc : res->results)
+          for (auto& c : rc) c.col.size = int32_t(G);
+        stats[STAT_STRATEGY] = MIG_STRATEGY_PARTITIONED;
+        stats[STAT_CAPACITY] = double(B);
+        stats[STAT_RETRIES]  = attempt;
+      } else {
+        release_outputs();
+        log_bins = std::min(RPA_MAX_LOG_BINS, log_bins + 2);
+      }
+    }
+  }
+
+  // ---- global hash table, growing on overflow --------------------------------
+  if (G < 0) {
+    uint64_t cap = std::max<uint64_t>(64, next_pow2(uint64_t(est * 2.6) + 1));
+    cap = std::min(cap, max_cap);
+    int32_t* d_flags = S.get<int32_t>(4);  // [0] overflow, [1..2] special occupancy
+    uint32_t* block_counts = nullptr;
+    int retries = 0;
+    for (;;) {
+      P.cap         = cap;
+      P.cap_mask    = cap - 1;
+      P.probe_limit = uint32_t(cap >= max_cap ? std::min<uint64_t>(cap, 0xffffffffull)
+                                              : std::min<uint64_t>(cap, 4096));
+      P.overflow    = d_flags;
+      P.special     = reinterpret_cast<uint32_t*>(d_flags + 1);
+      size_t state_bytes = 0;
+      uint64_t const total = cap + 2;
+      if (P.key_mode == KM_INLINE) P.slot_keys = S.get<uint64_t>(cap);
+      else P.slot_rows = S.get<uint64_t>(cap);
+      state_bytes += 8 * cap;
+      P.slot_count = P.need_count_all ? S.get<uint32_t>(total) : nullptr;
+      if (P.slot_count) state_bytes += 4 * total;
+      for (int f = 0; f < nf; f++) {
+        P.fields[f].ptr = P.fields[f].width == 8 ? static_cast<void*>(S.get<uint64_t>(total))
+                                                 : static_cast<void*>(S.get<uint32_t>(total));
+        state_bytes += size_t(P.fields[f].width) * total;
+      }
+      stats[STAT_STATE_BYTES] = double(state_bytes);
+      upload(0);
+      launch_init_state(dP, P, stream);
+      T.mark(2);
+      launch_aggregate(dP, P, dp.num_cus, stream);
+      T.mark(3);
+      MIG_HIP_TRY(hipGetLastError());
+
+      int64_t nb   = (int64_t(cap) + 2 + SCAN_TILE - 1) / SCAN_TILE;
+      block_counts = S.get<uint32_t>(nb + 1);
+      launch_count_occupied(dP, P, block_counts, stream);
+      launch_scan_blocks(block_counts, nb, block_counts + nb, stream);
+      auto* h_flags = reinterpret_cast<uint32_t*>(pin + 2 * sizeof(gb_params));
+      MIG_HIP_TRY(hipMemcpyAsync(h_flags, d_flags, 4, hipMemcpyDeviceToHost, stream));
+      MIG_HIP_TRY(hipMemcpyAsync(h_flags + 1, block_counts + nb, 4, hipMemcpyDeviceToHost, stream));
+      MIG_HIP_TRY(hipStreamSynchronize(stream));
+      if (h_flags[0] == 0) {
+        G = h_flags[1];
+        break;
+      }
+      MIG_EXPECTS(cap < max_cap, "hash table overflow at maximum capacity");
+      cap = std::min(max_cap, cap * 4);
+      retries++;
+    }
+    stats[STAT_STRATEGY] = MIG_STRATEGY_GLOBAL;
+    stats[STAT_CAPACITY] = double(cap);
+    stats[STAT_RETRIES]  = retries;
+
+    T.mark(4);
+    if (P.need_second_pass) launch_second_pass(dP, P, dp.num_cus, stream);
+    T.mark(5);
+    uint32_t* map = S.get<uint32_t>(G);
+    launch_compact(dP, P, block_counts, map, stream);
+    T.mark(6);
+
+    alloc_outputs(G, G, false);
+    uint32_t* rep_rows = P.key_mode == KM_GENERAL ? S.get<uint32_t>(G) : nullptr;
+    upload(1);  // stream-ordered after the kernels that read the previous block
+    launch_outputs(dP, P, map, G, rep_rows, stream);
+    if (P.key_mode == KM_GENERAL) {
+      int32_t* str_sizes = n_str_keys > 0 ? S.get<int32_t>(int64_t(n_str_keys) * G + 1) : nullptr;
+      launch_gather_keys(dP, P, rep_rows, G, str_sizes, stream);
+      int si = 0;
+      for (int i = 0; i < num_keys; i++) {
+        if (storage_of(keys[i].type_id) != ST_STRING) continue;
+        owned_column& oc = res->keys[i];
+        auto* offs       = static_cast<int32_t*>(oc.col.offsets);
+        if (G == 0) {
+          MIG_HIP_TRY(hipMemsetAsync(offs, 0, sizeof(int32_t), stream));
+          si++;
+          continue;
+        }
+        MIG_HIP_TRY(hipMemcpyAsync(offs, str_sizes + int64_t(si) * G, sizeof(int32_t) * G,
+                                   hipMemcpyDeviceToDevice, stream));
+        launch_exclusive_scan_i32(offs, G, offs + G, stream);
+        auto* h = reinterpret_cast<int32_t*>(pin + 2 * sizeof(gb_params));
+        MIG_HIP_TRY(hipMemcpyAsync(h, offs + G, 4, hipMemcpyDeviceToHost, stream));
+        MIG_HIP_TRY(hipStreamSynchronize(stream));
+        int32_t const total_chars = *h;
+        MIG_EXPECTS(total_chars >= 0, "unique string keys exceed 2^31 characters");
+        oc.data_bytes     = size_t(total_chars);
+        oc.col.data       = A.allocate(oc.data_bytes);
+        oc.col.data_bytes = total_chars;
+        launch_gather_string_chars(P.keys[i], rep_rows, G, offs, static_cast<uint8_t*>(oc.col.data), stream);
+        si++;
+      }
     }
   }
   MIG_HIP_TRY(hipGetLastError());
+  stats[STAT_KEY_MODE] = P.key_mode;
+  stats[STAT_GROUPS]   = double(G);
+  res->num_groups      = int32_t(G);
 
   // null counts of the outputs are computed on the device, then read back
   std::vector<mig_column*> masked;
@@ -788,7 +990,11 @@ std::unique_ptr<mig_groupby_result> groupby_aggregate(mig_column_view const* key
   stats[STAT_T_ACCUMULATE]  = T.between(2, 3);
   stats[STAT_T_SECOND_PASS] = T.between(4, 5);
   stats[STAT_T_COMPACT]     = T.between(3, 4) + T.between(5, 6);
-  stats[STAT_T_OUTPUTS]     = T.between(6, 7);
+  stats[STAT_T_OUTPUTS]     = T.between(6, 7) + T.between(13, 7);
+  stats[STAT_T_RPA_HIST]      = T.between(8, 9);
+  stats[STAT_T_RPA_SCATTER1]  = T.between(10, 11);
+  stats[STAT_T_RPA_SCATTER2]  = T.between(11, 12);
+  stats[STAT_T_RPA_AGGREGATE] = T.between(12, 13);
   return res;
 }
 

@@ -51,10 +51,15 @@ enum stat_index : int {
   STAT_T_SECOND_PASS,
   STAT_T_COMPACT,
   STAT_T_OUTPUTS,
+  STAT_T_RPA_HIST,       // partitioned path kernels (ms)
+  STAT_T_RPA_SCATTER1,
+  STAT_T_RPA_SCATTER2,
+  STAT_T_RPA_AGGREGATE,
   STAT_COUNT
 };
 double* thread_stats();
 void set_phase_timing(bool on);
+void set_strategy(int s);
 
 std::unique_ptr<mig_groupby_result> groupby_aggregate(mig_column_view const* keys, int32_t num_keys,
                                                       int32_t null_policy,

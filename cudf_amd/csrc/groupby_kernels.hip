@@ -6,149 +6,11 @@
 // accumulate step replaces compute_single_pass_aggs_fn
 // (groupby/hash/groupby_kernels.cuh:98-105) and the variance pass replaces
 // var_hash_functor (groupby/hash/multi_pass_kernels.cuh:74-111).
-#include "kernels.hpp"
-#include "murmur.hpp"
+#include "device_ops.hpp"
 
 namespace mig {
 
 constexpr int BLOCK = 256;
-
-// ---------------------------------------------------------------------------
-// atomics (agent scope: the table is shared by all XCDs of the device)
-template <typename T>
-__device__ __forceinline__ T aload(T const* p)
-{
-  return __hip_atomic_load(const_cast<T*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-template <typename T>
-__device__ __forceinline__ void astore(T* p, T v)
-{
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ uint64_t acas64(uint64_t* p, uint64_t expected, uint64_t desired)
-{
-  __hip_atomic_compare_exchange_strong(p, &expected, desired, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_AGENT);
-  return expected;
-}
-template <typename T>
-__device__ __forceinline__ void aadd(T* p, T v)
-{
-  __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-template <typename T>
-__device__ __forceinline__ void amin(T* p, T v)
-{
-  __hip_atomic_fetch_min(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-template <typename T>
-__device__ __forceinline__ void amax(T* p, T v)
-{
-  __hip_atomic_fetch_max(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Order-preserving unsigned encodings of float bit patterns.
-__device__ __forceinline__ uint32_t ord32(uint32_t b) { return (b & 0x80000000u) ? ~b : (b | 0x80000000u); }
-__device__ __forceinline__ uint32_t unord32(uint32_t o) { return (o & 0x80000000u) ? (o & 0x7fffffffu) : ~o; }
-__device__ __forceinline__ uint64_t ord64(uint64_t b)
-{
-  return (b & 0x8000000000000000ull) ? ~b : (b | 0x8000000000000000ull);
-}
-__device__ __forceinline__ uint64_t unord64(uint64_t o)
-{
-  return (o & 0x8000000000000000ull) ? (o & 0x7fffffffffffffffull) : ~o;
-}
-
-// ---------------------------------------------------------------------------
-// element access
-__device__ __forceinline__ uint64_t load_fixed_bits(dcol const& c, int64_t row)
-{
-  switch (c.storage) {
-    case ST_I8:
-    case ST_U8: return c.data[row];
-    case ST_BOOL: return c.data[row] != 0;
-    case ST_I16:
-    case ST_U16: return reinterpret_cast<uint16_t const*>(c.data)[row];
-    case ST_I32:
-    case ST_U32: return reinterpret_cast<uint32_t const*>(c.data)[row];
-    case ST_F32: return normalize_f32_bits(reinterpret_cast<uint32_t const*>(c.data)[row]);
-    case ST_I64:
-    case ST_U64: return reinterpret_cast<uint64_t const*>(c.data)[row];
-    case ST_F64: return normalize_f64_bits(reinterpret_cast<uint64_t const*>(c.data)[row]);
-    default: return 0;
-  }
-}
-
-__device__ __forceinline__ uint32_t hash_fixed_bits(int32_t st, uint64_t bits, uint32_t seed)
-{
-  switch (storage_width(st)) {
-    case 1: return mm_hash_small(static_cast<uint32_t>(bits), 1, seed);
-    case 2: return mm_hash_small(static_cast<uint32_t>(bits), 2, seed);
-    case 4: return mm_hash4(static_cast<uint32_t>(bits), seed);
-    default: return mm_hash8(bits, seed);
-  }
-}
-
-__device__ __forceinline__ void str_span(dcol const& c, int64_t row, int64_t& b, int64_t& e)
-{
-  if (c.offsets64) {
-    auto o = reinterpret_cast<int64_t const*>(c.offsets);
-    b = o[row];
-    e = o[row + 1];
-  } else {
-    auto o = reinterpret_cast<int32_t const*>(c.offsets);
-    b = o[row];
-    e = o[row + 1];
-  }
-}
-
-__device__ __forceinline__ uint32_t elem_hash(dcol const& c, int64_t row, uint32_t seed)
-{
-  if (!dcol_valid(c, row)) return NULL_HASH;
-  if (c.storage == ST_STRING) {
-    int64_t b, e;
-    str_span(c, row, b, e);
-    return mm_hash_bytes(c.data + b, e - b, seed);
-  }
-  return hash_fixed_bits(c.storage, load_fixed_bits(c, row), seed);
-}
-
-__device__ __forceinline__ uint32_t row_hash(dcol const* cols, int ncols, int64_t row, uint32_t seed)
-{
-  uint32_t h = seed;
-  for (int c = 0; c < ncols; c++) h = hash_combine(h, elem_hash(cols[c], row, seed));
-  return h;
-}
-
-__device__ __forceinline__ bool elem_equal(dcol const& c, int64_t a, int64_t b)
-{
-  bool const va = dcol_valid(c, a), vb = dcol_valid(c, b);
-  if (!va || !vb) return va == vb;
-  if (c.storage == ST_STRING) {
-    int64_t ab, ae, bb, be;
-    str_span(c, a, ab, ae);
-    str_span(c, b, bb, be);
-    if (ae - ab != be - bb) return false;
-    for (int64_t i = 0; i < ae - ab; i++)
-      if (c.data[ab + i] != c.data[bb + i]) return false;
-    return true;
-  }
-  return load_fixed_bits(c, a) == load_fixed_bits(c, b);
-}
-
-__device__ __forceinline__ bool rows_equal(gb_params const& P, int64_t a, int64_t b)
-{
-  for (int c = 0; c < P.num_keys; c++)
-    if (!elem_equal(P.keys[c], a, b)) return false;
-  return true;
-}
-
-__device__ __forceinline__ bool any_key_null(gb_params const& P, int64_t row)
-{
-  for (int c = 0; c < P.num_keys; c++)
-    if (!dcol_valid(P.keys[c], row)) return true;
-  return false;
-}
 
 // ---------------------------------------------------------------------------
 // public row hash: cudf::hashing::murmurhash3_x86_32 (src/hash/murmurhash3_x86_32.cu:34-57)
@@ -188,22 +50,6 @@ void launch_sample_hash(gb_params const* dP, int64_t n, int64_t nsample, uint32_
 
 // ---------------------------------------------------------------------------
 // state initialisation: table, row counts, special flags and accumulator identities
-__device__ __forceinline__ uint64_t field_identity(int32_t kind)
-{
-  switch (kind) {
-    case FK_MIN_S32: return 0x7fffffffull;
-    case FK_MAX_S32: return 0x80000000ull;
-    case FK_MIN_S64: return 0x7fffffffffffffffull;
-    case FK_MAX_S64: return 0x8000000000000000ull;
-    case FK_MIN_U32: return 0xffffffffull;
-    case FK_MIN_U64: return 0xffffffffffffffffull;
-    case FK_MIN_F32: return 0xff800000ull;              // ord32(+inf)
-    case FK_MAX_F32: return 0x007fffffull;              // ord32(-inf)
-    case FK_MIN_F64: return 0xfff0000000000000ull;      // ord64(+inf)
-    case FK_MAX_F64: return 0x000fffffffffffffull;      // ord64(-inf)
-    default: return 0;                                  // sums, counts, max of unsigned
-  }
-}
 
 __global__ void __launch_bounds__(BLOCK) k_init_state(gb_params const* __restrict__ Pp)
 {
@@ -342,48 +188,6 @@ __device__ __forceinline__ int64_t locate(gb_params const& P, int64_t row)
 
 // ---------------------------------------------------------------------------
 // accumulate one row into the group's accumulators
-struct loaded_value {
-  int64_t i;   // integer sources: sign- or zero-extended
-  double d;    // value as double (all sources)
-  float f;     // float32 sources
-  uint64_t b;  // float sources: raw bits
-};
-
-__device__ __forceinline__ loaded_value load_value(dcol const& c, int64_t row)
-{
-  loaded_value v{0, 0.0, 0.0f, 0};
-  switch (c.storage) {
-    case ST_I8: v.i = reinterpret_cast<int8_t const*>(c.data)[row]; v.d = double(v.i); break;
-    case ST_I16: v.i = reinterpret_cast<int16_t const*>(c.data)[row]; v.d = double(v.i); break;
-    case ST_I32: v.i = reinterpret_cast<int32_t const*>(c.data)[row]; v.d = double(v.i); break;
-    case ST_I64: v.i = reinterpret_cast<int64_t const*>(c.data)[row]; v.d = double(v.i); break;
-    case ST_U8: v.i = c.data[row]; v.d = double(v.i); break;
-    case ST_BOOL: v.i = c.data[row] != 0; v.d = double(v.i); break;
-    case ST_U16: v.i = reinterpret_cast<uint16_t const*>(c.data)[row]; v.d = double(v.i); break;
-    case ST_U32: v.i = reinterpret_cast<uint32_t const*>(c.data)[row]; v.d = double(v.i); break;
-    case ST_U64: {
-      uint64_t u = reinterpret_cast<uint64_t const*>(c.data)[row];
-      v.i = int64_t(u);
-      v.d = double(u);
-      break;
-    }
-    case ST_F32: {
-      uint32_t u = reinterpret_cast<uint32_t const*>(c.data)[row];
-      v.b = u;
-      v.f = __uint_as_float(u);
-      v.d = double(v.f);
-      break;
-    }
-    case ST_F64: {
-      uint64_t u = reinterpret_cast<uint64_t const*>(c.data)[row];
-      v.b = u;
-      v.d = __longlong_as_double(int64_t(u));
-      break;
-    }
-    default: break;
-  }
-  return v;
-}
 
 __device__ __forceinline__ void apply_field(field_desc const& F, int64_t slot, loaded_value const& v)
 {
@@ -535,19 +339,6 @@ __device__ __forceinline__ bool slot_occupied(gb_params const& P, uint64_t slot)
   return P.special[slot - P.cap] != 0u;
 }
 
-__device__ __forceinline__ uint32_t block_sum_u32(uint32_t v, uint32_t* lds)
-{
-  // wave64 reduction then across the block's 4 waves
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
-  int const wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  if (lane == 0) lds[wave] = v;
-  __syncthreads();
-  uint32_t t = 0;
-  if (threadIdx.x == 0)
-    for (int w = 0; w < BLOCK / 64; w++) t += lds[w];
-  __syncthreads();
-  return t;
-}
 
 __global__ void __launch_bounds__(BLOCK)
   k_count_occupied(gb_params const* __restrict__ Pp, uint32_t* block_counts)
@@ -646,15 +437,6 @@ __device__ __forceinline__ uint64_t field_bits(field_desc const& F, int64_t slot
                       : reinterpret_cast<uint32_t const*>(F.ptr)[slot];
 }
 
-__device__ __forceinline__ void store_width(void* data, int64_t g, int w, uint64_t bits)
-{
-  switch (w) {
-    case 1: reinterpret_cast<uint8_t*>(data)[g] = uint8_t(bits); break;
-    case 2: reinterpret_cast<uint16_t*>(data)[g] = uint16_t(bits); break;
-    case 4: reinterpret_cast<uint32_t*>(data)[g] = uint32_t(bits); break;
-    default: reinterpret_cast<uint64_t*>(data)[g] = bits; break;
-  }
-}
 
 // Writes the validity bit of group g for the whole wave with one ballot.
 __device__ __forceinline__ void store_valid_bits(uint32_t* mask, int64_t g, int64_t G, bool valid)

@@ -114,6 +114,50 @@ struct gb_params {
   uint32_t* out_key_mask[MAX_KEYS];
 };
 
+// ---------------------------------------------------------------------------
+// Radix-partitioned aggregation (high cardinality, one fixed-width key).
+// Rows are partitioned by the top bits of their row hash into B = 2^log_bins
+// bins small enough that all groups of a bin fit in one workgroup's LDS
+// (one or two scatter passes), then one workgroup aggregates each bin in LDS.
+// Partition buffers are structure-of-arrays: normalised key bits (u64), the
+// raw bytes of each value column, and a flags byte per row (bit v = value
+// column v valid, bit 7 = key null under INCLUDE).
+constexpr int RPA_MAX_FLAGGED = 7;
+constexpr uint8_t RPA_KEY_NULL = 0x80;
+constexpr int RPA_TILE = 2048;    // rows per scatter tile
+constexpr int RPA_BLOCK = 256;    // scatter block size
+constexpr int RPA_AGG_BLOCK = 512;
+
+struct rpa_params {
+  int32_t log_bins;   // final bins B = 1 << log_bins
+  int32_t log_f1;     // first-level fanout F1 = 1 << log_f1 (== log_bins when one level)
+  int32_t two_level;  // 1 when the second scatter pass runs
+  int32_t lds_slots;  // LDS hash slots per aggregating workgroup (power of two)
+  int32_t max_ids;    // dense ids per workgroup, including the two reserved ones
+  int32_t lds_bytes;  // dynamic LDS of the aggregate kernel
+  int32_t ids_off;    // LDS byte offsets
+  int32_t count_all_off;
+  int32_t misc_off;
+  int32_t pad0;
+  uint32_t* hist;                    // [B] rows per final bin
+  int64_t* bin_start;                // [B + 1]
+  unsigned long long* cursor1;       // [F1] next free row of each first-level bin (buffer A)
+  unsigned long long* cursor2;       // [B] next free row of each final bin (buffer B)
+  int64_t* tile_start1;              // [F1 + 1] scatter-2 tiles before each first-level bin
+  int64_t num_tiles1;
+  uint64_t* keysA;
+  uint64_t* keysB;
+  uint8_t* flagsA;
+  uint8_t* flagsB;
+  uint8_t* valsA[MAX_VALS];
+  uint8_t* valsB[MAX_VALS];
+  int32_t val_flag_bit[MAX_VALS];    // flags bit of value column v, -1 if it has no nulls
+  unsigned long long* group_counter; // groups emitted so far
+  int32_t* overflow;                 // a bin had more groups than max_ids
+  int32_t pad1;
+  uint64_t out_capacity;             // rows allocated in every output column
+};
+
 // ---- launchers (groupby_kernels.hip) --------------------------------------
 struct launch_cfg {
   int num_cus;
@@ -145,5 +189,15 @@ void launch_exclusive_scan_i32(int32_t* data, int64_t n, int32_t* total, hipStre
 void launch_gather_string_chars(dcol const& src, uint32_t const* rep_rows, int64_t G,
                                 int32_t const* out_offsets, uint8_t* out_chars, hipStream_t s);
 void launch_count_nulls(dcol const& c, int64_t n, uint32_t* d_count, hipStream_t s);
+
+// ---- radix-partitioned aggregation (partition_kernels.hip) ------------------
+void launch_rpa_hist(gb_params const* dP, rpa_params const* dR, rpa_params const& hR, int64_t n,
+                     int num_cus, hipStream_t s);
+void launch_rpa_scatter1(gb_params const* dP, rpa_params const* dR, int64_t n, int num_cus,
+                         hipStream_t s);
+void launch_rpa_scatter2(gb_params const* dP, rpa_params const* dR, rpa_params const& hR,
+                         int num_cus, hipStream_t s);
+void launch_rpa_aggregate(gb_params const* dP, rpa_params const* dR, rpa_params const& hR,
+                          hipStream_t s);
 
 }  // namespace mig

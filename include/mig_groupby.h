@@ -225,6 +225,17 @@ int32_t mig_last_stats(double* values, int32_t n);
  * the call with an NVTX range, cpp/src/groupby/groupby.cu:209). */
 void mig_set_phase_timing(int32_t enable);
 
+/* Aggregation strategies.  AUTO picks per call from the key layout and a
+ * sampled estimate of the number of groups; the others force one (used by
+ * tests and the autotuner).  No counterpart in the reference, whose hash
+ * path has a single strategy (cpp/src/groupby/hash/groupby.cu:548-622). */
+enum {
+  MIG_STRATEGY_AUTO        = 0,
+  MIG_STRATEGY_GLOBAL      = 1, /* device-wide hash table, global atomics      */
+  MIG_STRATEGY_PARTITIONED = 2  /* radix partition into LDS-sized bins          */
+};
+void mig_set_strategy(int32_t strategy);
+
 #ifdef __cplusplus
 }
 #endif

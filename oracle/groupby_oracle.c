@@ -300,7 +300,7 @@ int64_t orc_groupby(const orc_col* keys, int32_t nkeys, int32_t null_policy,
     int k = kinds[a];
     int src = v->type_id;
     int tgt = orc_target_type(src, k);
-    if (tgt < 0 || src == T_STRING) { ok = 0; break; }
+    if (tgt < 0 || (src == T_STRING && k != A_COUNT_VALID && k != A_COUNT_ALL)) { ok = 0; break; }
     uint8_t* ov = out_valid[a];
     for (int64_t g = 0; g < G; g++) { acc[g] = 0; iacc[g] = 0; cnt[g] = 0; cnt_all[g] = 0; }
     int fl = is_float(src);

@@ -1,0 +1,121 @@
+"""Parity of the radix-partitioned (LDS-bin) aggregation path, forced on
+through mig_set_strategy, against the CPU oracle: one- and two-level
+partitions, every key and value storage width, null keys under both
+policies, the empty-marker key value, and the fallback to the global table
+when the sampled group estimate is far too low."""
+import numpy as np
+import pytest
+
+from tests.gbutil import AGG, C2_AGGS, assert_parity, c2_data, gpu_groupby, host_col
+from tests.test_gpu_groupby import ALL_NUMERIC_AGGS
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(autouse=True)
+def partitioned():
+    from cudf_amd import _lib
+    _lib.set_strategy(_lib.STRATEGY_PARTITIONED)
+    yield
+    _lib.set_strategy(_lib.STRATEGY_AUTO)
+
+
+def _stats():
+    from cudf_amd import _lib
+    return _lib.last_stats()
+
+
+@pytest.mark.parametrize("key_dtype", [np.int8, np.int16, np.int32, np.int64, np.uint64,
+                                       np.float32, np.float64])
+@pytest.mark.parametrize("policy", [0, 1])
+def test_key_types_partitioned(key_dtype, policy):
+    rng = np.random.default_rng(21)
+    n = 50_000
+    if np.dtype(key_dtype).kind == "f":
+        k = (rng.integers(-3000, 3000, n) / 8).astype(key_dtype)
+        k[rng.random(n) < 0.01] = np.nan
+        k[rng.random(n) < 0.01] = -0.0
+    else:
+        hi = min(20_000, np.iinfo(key_dtype).max)
+        k = rng.integers(0, hi, n).astype(key_dtype)
+    kvalid = rng.random(n) > 0.05
+    v = rng.integers(-1000, 1000, n).astype(np.int32)
+    vvalid = rng.random(n) > 0.1
+    assert_parity([host_col(k, kvalid)], [(host_col(v, vvalid), ALL_NUMERIC_AGGS)], policy)
+    assert _stats()["strategy"] == 2
+
+
+@pytest.mark.parametrize("val_dtype", [np.int8, np.int16, np.int32, np.int64, np.uint8, np.uint32,
+                                       np.uint64, np.float32, np.float64, np.bool_])
+def test_value_types_partitioned(val_dtype):
+    rng = np.random.default_rng(22)
+    n = 60_000
+    k = rng.integers(0, 8000, n).astype(np.int64)
+    if val_dtype == np.bool_:
+        v = rng.random(n) > 0.5
+    elif np.dtype(val_dtype).kind == "f":
+        v = (rng.standard_normal(n) * 10).astype(val_dtype)
+    else:
+        info = np.iinfo(val_dtype)
+        v = rng.integers(max(info.min, -10**6), min(info.max, 10**6), n, endpoint=True).astype(val_dtype)
+    assert_parity([host_col(k)], [(host_col(v, rng.random(n) > 0.1), ALL_NUMERIC_AGGS)])
+    assert _stats()["strategy"] == 2
+
+
+def test_one_level_partition_c2_shape():
+    key, f64, i32 = c2_data(n=1_000_000, card=100_000, seed=31)
+    assert_parity([key], [(f64, C2_AGGS), (i32, C2_AGGS)])
+    st = _stats()
+    assert st["strategy"] == 2 and st["capacity"] <= 256
+
+
+def test_two_level_partition_c2_shape():
+    key, f64, i32 = c2_data(n=3_000_000, card=400_000, seed=32)
+    assert_parity([key], [(f64, C2_AGGS), (i32, C2_AGGS)])
+    st = _stats()
+    assert st["strategy"] == 2 and st["capacity"] > 256
+
+
+def test_many_value_columns_with_nulls():
+    rng = np.random.default_rng(33)
+    n = 80_000
+    k = host_col(rng.integers(0, 20_000, n).astype(np.int32))
+    reqs = []
+    for i in range(6):
+        v = rng.integers(-50, 50, n).astype([np.int16, np.int32, np.int64, np.float32, np.float64, np.uint16][i])
+        reqs.append((host_col(v, rng.random(n) > 0.2), [(AGG["SUM"], 1), (AGG["MAX"], 1), (AGG["VARIANCE"], 1)]))
+    assert_parity([k], reqs)
+
+
+def test_empty_marker_key_and_null_group():
+    marker = np.array([0xFFF1A5A5A5A5A5A5], dtype=np.uint64).view(np.int64)[0]
+    rng = np.random.default_rng(34)
+    n = 20_000
+    k = rng.integers(0, 3000, n).astype(np.int64)
+    k[rng.random(n) < 0.05] = marker
+    kvalid = rng.random(n) > 0.05
+    v = rng.integers(0, 100, n).astype(np.int64)
+    for policy in (0, 1):
+        assert_parity([host_col(k, kvalid)], [(host_col(v), [(AGG["SUM"], 1), (AGG["COUNT_ALL"], 1)])], policy)
+
+
+def test_underestimated_groups_fall_back_correctly():
+    """99% of rows share one key and the last 1% are all distinct: the strided
+    sample sees few distinct keys, the LDS bins overflow, and the call must
+    still return the exact result (larger partitions, then the global table)."""
+    n = 2_000_000
+    k = np.zeros(n, dtype=np.int64)
+    tail = n // 100
+    k[-tail:] = np.arange(1, tail + 1)
+    v = np.arange(n, dtype=np.int64) % 1000
+    ng = assert_parity([host_col(k)], [(host_col(v), [(AGG["SUM"], 1), (AGG["COUNT_VALID"], 1),
+                                                       (AGG["MIN"], 1)])])
+    assert ng == tail + 1
+
+
+def test_all_keys_null_excluded():
+    n = 100_000
+    k = np.arange(n, dtype=np.int64)
+    kv = np.zeros(n, dtype=bool)
+    kcols, res = gpu_groupby([host_col(k, kv)], [(host_col(np.ones(n, np.int32)), [(AGG["SUM"], 1)])])
+    assert len(kcols[0][0]) == 0 and len(res[0][0][0]) == 0
