@@ -208,6 +208,7 @@ def main():
                      "achieved_GBps": (in_b + out_b) / (ms_per_step * 1e-3) / 1e9,
                      "frac": (in_b + out_b) / (ms_per_step * 1e-3) / HBM_PEAK,
                      "kernels_ms": {k: round(v[0], 4) for k, v in kernels.items()},
+                     "last_call": {k: v for k, v in st.items() if not k.startswith("t_")},
                      "phase_ms": {k: round(v, 4) for k, v in phase.items()}},
     }
     if not args.no_cpu:

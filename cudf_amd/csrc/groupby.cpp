@@ -19,6 +19,7 @@
 #include <cstring>
 #include <atomic>
 #include <mutex>
+#include <map>
 #include <unordered_map>
 
 #include "kernels.hpp"
@@ -38,9 +39,10 @@ std::atomic<int> g_strategy{MIG_STRATEGY_AUTO};
 // Partitioned path: LDS hash slots and byte budget of the aggregating
 // workgroup (two 512-thread workgroups fit in a CU's 160 KB of LDS).
 constexpr int32_t RPA_LDS_SLOTS      = 2048;
-constexpr int32_t RPA_LDS_BUDGET     = 78 * 1024;
+constexpr int32_t RPA_LDS_BUDGET     = 76 * 1024;
 constexpr int RPA_MAX_LOG_BINS       = 15;
 constexpr int RPA_MAX_LOG_FANOUT     = 8;
+constexpr int32_t RPA_STAGE_BUDGET   = 48 * 1024;  // LDS staging of a scatter tile
 
 // hipEvents marking phase boundaries on the call's stream.
 struct phase_timer {
@@ -133,25 +135,76 @@ device_props const& props()
   return cache.emplace(dev, p).first->second;
 }
 
-// Scratch memory of one call, freed in stream order when the call ends.
+// Device scratch blocks kept across calls, per device.  A call's hash table,
+// accumulators and partition buffers total gigabytes; mapping fresh device
+// memory for them on every call costs ~40 ms per GB, far more than the
+// kernels, so freed blocks are cached and reused by later calls (best fit
+// within 2x of the request).  Every call synchronises its stream before its
+// blocks return to the cache, so a reused block is never still in use.
+struct scratch_cache {
+  std::mutex mu;
+  std::unordered_map<int, std::multimap<size_t, void*>> free_blocks;
+
+  static scratch_cache& get()
+  {
+    static scratch_cache c;
+    return c;
+  }
+  void* take(int dev, size_t bytes)
+  {
+    {
+      std::lock_guard<std::mutex> lock(mu);
+      auto& m = free_blocks[dev];
+      auto it = m.lower_bound(bytes);
+      if (it != m.end() && it->first <= 2 * bytes + (1u << 20)) {
+        void* p = it->second;
+        m.erase(it);
+        return p;
+      }
+    }
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+      // release cached blocks of this device and try once more
+      release_all(dev);
+      e = hipMalloc(&p, bytes);
+    }
+    if (e != hipSuccess)
+      fail(MIG_ERROR_ALLOC, "scratch allocation of " + std::to_string(bytes) + " bytes failed");
+    return p;
+  }
+  void give(int dev, void* p, size_t bytes)
+  {
+    std::lock_guard<std::mutex> lock(mu);
+    free_blocks[dev].emplace(bytes, p);
+  }
+  void release_all(int dev)
+  {
+    std::lock_guard<std::mutex> lock(mu);
+    for (auto& kv : free_blocks[dev]) (void)hipFree(kv.second);
+    free_blocks[dev].clear();
+  }
+};
+
+// Scratch memory of one call; blocks go back to the cache when the call ends
+// (after its stream has been synchronised).
 struct scratch {
   hipStream_t stream;
-  std::vector<void*> ptrs;
-  explicit scratch(hipStream_t s) : stream(s) {}
+  int dev = 0;
+  std::vector<std::pair<void*, size_t>> blocks;
+  explicit scratch(hipStream_t s) : stream(s) { MIG_HIP_TRY(hipGetDevice(&dev)); }
   template <typename T>
   T* get(size_t count)
   {
-    void* p = nullptr;
-    size_t bytes = std::max<size_t>(1, count * sizeof(T));
-    hipError_t e = hipMallocAsync(&p, bytes, stream);
-    if (e != hipSuccess)
-      fail(MIG_ERROR_ALLOC, "scratch allocation of " + std::to_string(bytes) + " bytes failed");
-    ptrs.push_back(p);
+    size_t bytes = std::max<size_t>(256, ((count * sizeof(T)) + 255) & ~size_t(255));
+    void* p      = scratch_cache::get().take(dev, bytes);
+    blocks.emplace_back(p, bytes);
     return static_cast<T*>(p);
   }
   ~scratch()
   {
-    for (void* p : ptrs) (void)hipFreeAsync(p, stream);
+    (void)hipStreamSynchronize(stream);
+    for (auto& b : blocks) scratch_cache::get().give(dev, b.first, b.second);
   }
 };
 
@@ -584,17 +637,21 @@ std::unique_ptr<mig_groupby_result> groupby_aggregate(mig_column_view const* key
   uint64_t const max_cap = next_pow2(uint64_t(2 * N + 2));
   double est = double(N);
   if (N > 4096) {
-    int64_t ns = std::min<int64_t>(N, SAMPLE_ROWS);
-    uint32_t* d_sample = S.get<uint32_t>(ns);
+    int64_t const ns = std::min<int64_t>(N, SAMPLE_ROWS);
+    uint32_t const tslots = uint32_t(next_pow2(uint64_t(2 * ns)));
+    auto* table    = S.get<uint64_t>(tslots);
+    auto* counters = S.get<uint32_t>(2);
+    MIG_HIP_TRY(hipMemsetAsync(table, 0, sizeof(uint64_t) * tslots, stream));
+    MIG_HIP_TRY(hipMemsetAsync(counters, 0, sizeof(uint32_t) * 2, stream));
     upload(0);
-    launch_sample_hash(dP, N, ns, d_sample, stream);
-    auto* h_sample = reinterpret_cast<uint32_t*>(pin + 2 * sizeof(gb_params));
-    MIG_HIP_TRY(hipMemcpyAsync(h_sample, d_sample, 4 * ns, hipMemcpyDeviceToHost, stream));
+    launch_sample_hash(dP, N, ns, table, tslots - 1, counters, stream);
+    auto* h = reinterpret_cast<uint32_t*>(pin + 2 * sizeof(gb_params));
+    MIG_HIP_TRY(hipMemcpyAsync(h, counters, 8, hipMemcpyDeviceToHost, stream));
     MIG_HIP_TRY(hipStreamSynchronize(stream));
-    std::vector<uint32_t> hs(h_sample, h_sample + ns);
-    std::sort(hs.begin(), hs.end());
-    double d = double(std::unique(hs.begin(), hs.end()) - hs.begin());
-    est = ns == N ? d : estimate_distinct(double(ns), d, double(N));
+    double const sampled = double(h[0]), d = double(h[1]);
+    // population: rows that are not dropped, scaled from the sample
+    double const pop = double(N) * (sampled / double(ns));
+    est = ns == N ? d : (sampled > 0 ? estimate_distinct(sampled, d, std::max(pop, d)) : 1.0);
   }
   stats[STAT_EST_GROUPS] = est;
   T.mark(1);
@@ -728,8 +785,9 @@ std::unique_ptr<mig_groupby_result> groupby_aggregate(mig_column_view const* key
   for (int f = 0; f < nf; f++) acc_bytes += P.fields[f].width;
   if (P.need_count_all) acc_bytes += 4;
   int32_t const S_slots = RPA_LDS_SLOTS;
-  int32_t const fixed_lds = S_slots * 8 + S_slots * 2 + int32_t(sizeof(uint64_t)) * 4 + 64;
-  int32_t D = acc_bytes > 0 ? (RPA_LDS_BUDGET - fixed_lds) / acc_bytes : 4096;
+  int32_t const fixed_lds = S_slots * 8 + S_slots * 2 + int32_t(sizeof(uint64_t)) * 4 + 64 + 64;
+  // per dense id: its accumulators plus its slot (u16) for the output pass
+  int32_t D = (RPA_LDS_BUDGET - fixed_lds) / (acc_bytes + 2);
   D = std::min<int32_t>(D, 4096);
   rpa_ok = rpa_ok && D >= 64;
   bool use_rpa = false;
@@ -752,11 +810,33 @@ std::unique_ptr<mig_groupby_result> groupby_aggregate(mig_column_view const* key
       R.count_all_off = off;
       off += ((D * 4) + 7) & ~7;
     }
+    R.slot_of_id_off = off;
+    off += ((D * 2) + 7) & ~7;
     R.misc_off  = off;
     off += 64;
     R.lds_bytes = off;
     R.lds_slots = S_slots;
     R.max_ids   = D;
+    // scatter staging layout: a tile of keys, copied values, flags and bins
+    int32_t rec = 8 + 1 + 2;
+    for (int v = 0; v < P.num_vals; v++)
+      if (P.val_field_begin[v] < P.val_field_begin[v + 1]) rec += storage_width(vals[v].storage);
+    R.tile_rows = RPA_TILE;
+    while (R.tile_rows > RPA_SCATTER_BLOCK && R.tile_rows * (rec + 4) > RPA_STAGE_BUDGET) R.tile_rows /= 2;
+    {
+      int32_t so = R.tile_rows * 8;
+      for (int v = 0; v < P.num_vals; v++) {
+        if (P.val_field_begin[v] == P.val_field_begin[v + 1]) continue;
+        R.stage_val_off[v] = so;
+        so += (R.tile_rows * storage_width(vals[v].storage) + 7) & ~7;
+      }
+      R.stage_flags_off = so;
+      so += (R.tile_rows + 7) & ~7;
+      R.stage_bin_off = so;
+      so += R.tile_rows * 2;
+      R.stage_bytes = so;
+    }
+    R.hist_blocks = dp.num_cus * 2;
     int flag_bit = 0;
     for (int v = 0; v < P.num_vals; v++)
       R.val_flag_bit[v] = (P.val_field_begin[v] < P.val_field_begin[v + 1] && vals[v].has_nulls) ? flag_bit++ : -1;
@@ -771,6 +851,7 @@ std::unique_ptr<mig_groupby_result> groupby_aggregate(mig_column_view const* key
       int64_t const F1 = int64_t(1) << R.log_f1;
       int const lf2    = log_bins - R.log_f1;
       R.hist        = S.get<uint32_t>(B);
+      R.hist_partial = S.get<uint32_t>(size_t(R.hist_blocks) * B);
       R.bin_start   = S.get<int64_t>(B + 1);
       R.cursor1     = S.get<unsigned long long>(F1);
       R.cursor2     = S.get<unsigned long long>(B);
@@ -805,7 +886,7 @@ std::unique_ptr<mig_groupby_result> groupby_aggregate(mig_column_view const* key
       for (int64_t b1 = 0; b1 < F1; b1++) {
         c1[b1]      = (unsigned long long)start[b1 << lf2];
         int64_t len = start[(b1 + 1) << lf2] - start[b1 << lf2];
-        t1[b1 + 1]  = t1[b1] + (len + RPA_TILE - 1) / RPA_TILE;
+        t1[b1 + 1]  = t1[b1] + (len + R.tile_rows - 1) / R.tile_rows;
       }
       R.num_tiles1 = t1[F1];
       MIG_HIP_TRY(hipMemcpyAsync(R.bin_start, start.data(), sizeof(int64_t) * (B + 1), hipMemcpyHostToDevice, stream));
@@ -838,7 +919,7 @@ std::unique_ptr<mig_groupby_result> groupby_aggregate(mig_column_view const* key
       upload(0);
       MIG_HIP_TRY(hipMemcpyAsync(dR, &R, sizeof(rpa_params), hipMemcpyHostToDevice, stream));
       T.mark(10);
-      launch_rpa_scatter1(dP, dR, N, dp.num_cus, stream);
+      launch_rpa_scatter1(dP, dR, R, N, dp.num_cus, stream);
       T.mark(11);
       if (R.two_level) launch_rpa_scatter2(dP, dR, R, dp.num_cus, stream);
       T.mark(12);

@@ -33,19 +33,41 @@ void launch_hash_rows(dcol const* d_cols, int ncols, int64_t n, uint32_t seed, u
 // ---------------------------------------------------------------------------
 // cardinality sample: row hashes of `nsample` evenly strided rows
 __global__ void __launch_bounds__(BLOCK)
-  k_sample_hash(gb_params const* __restrict__ Pp, int64_t n, int64_t nsample, uint32_t* out)
+  k_sample_hash(gb_params const* __restrict__ Pp, int64_t n, int64_t nsample, uint64_t* table,
+                uint32_t table_mask, uint32_t* counters)
 {
   gb_params const& P = *Pp;
   int64_t i = int64_t(blockIdx.x) * BLOCK + threadIdx.x;
   if (i >= nsample) return;
   int64_t row = (i * n) / nsample;
-  out[i] = row_hash(P.keys, P.num_keys, row, 0);
+  // rows the group-by drops (null keys under EXCLUDE) are not part of the
+  // population whose distinct keys are estimated
+  if (P.skip_null_keys && any_key_null(P, row)) return;
+  aadd(counters, 1u);
+  uint32_t const h   = row_hash(P.keys, P.num_keys, row, 0);
+  uint64_t const tag = (1ull << 32) | h;
+  uint32_t s         = mm_fmix32(h) & table_mask;
+  for (uint32_t k = 0; k <= table_mask; k++) {
+    uint64_t cur = aload(table + s);
+    if (cur == tag) return;
+    if (cur == 0) {
+      uint64_t old = acas64(table + s, 0ull, tag);
+      if (old == 0) {
+        aadd(counters + 1, 1u);
+        return;
+      }
+      if (old == tag) return;
+    }
+    s = (s + 1) & table_mask;
+  }
 }
 
-void launch_sample_hash(gb_params const* dP, int64_t n, int64_t nsample, uint32_t* out, hipStream_t s)
+void launch_sample_hash(gb_params const* dP, int64_t n, int64_t nsample, uint64_t* table,
+                        uint32_t table_mask, uint32_t* counters, hipStream_t s)
 {
   int64_t blocks = (nsample + BLOCK - 1) / BLOCK;
-  hipLaunchKernelGGL(k_sample_hash, dim3(blocks), dim3(BLOCK), 0, s, dP, n, nsample, out);
+  hipLaunchKernelGGL(k_sample_hash, dim3(blocks), dim3(BLOCK), 0, s, dP, n, nsample, table, table_mask,
+                     counters);
 }
 
 // ---------------------------------------------------------------------------

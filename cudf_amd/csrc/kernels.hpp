@@ -124,8 +124,10 @@ struct gb_params {
 // column v valid, bit 7 = key null under INCLUDE).
 constexpr int RPA_MAX_FLAGGED = 7;
 constexpr uint8_t RPA_KEY_NULL = 0x80;
-constexpr int RPA_TILE = 2048;    // rows per scatter tile
-constexpr int RPA_BLOCK = 256;    // scatter block size
+constexpr int RPA_TILE = 2048;          // rows per scatter tile (at most)
+constexpr int RPA_SMALL_BLOCK = 256;    // small helper kernels
+constexpr int RPA_SCATTER_BLOCK = 512;  // scatter kernels
+constexpr int RPA_HIST_BLOCK = 1024;    // histogram kernel
 constexpr int RPA_AGG_BLOCK = 512;
 
 struct rpa_params {
@@ -135,10 +137,19 @@ struct rpa_params {
   int32_t lds_slots;  // LDS hash slots per aggregating workgroup (power of two)
   int32_t max_ids;    // dense ids per workgroup, including the two reserved ones
   int32_t lds_bytes;  // dynamic LDS of the aggregate kernel
-  int32_t ids_off;    // LDS byte offsets
+  int32_t ids_off;    // LDS byte offsets of the aggregate kernel
   int32_t count_all_off;
   int32_t misc_off;
+  int32_t slot_of_id_off;
+  // scatter staging: a tile of rows sorted by bin in LDS before it is written
+  int32_t tile_rows;
+  int32_t stage_bytes;
+  int32_t stage_flags_off;
+  int32_t stage_bin_off;
+  int32_t stage_val_off[MAX_VALS];
+  int32_t hist_blocks;               // blocks of the histogram kernel
   int32_t pad0;
+  uint32_t* hist_partial;            // [hist_blocks][B] per-block histograms
   uint32_t* hist;                    // [B] rows per final bin
   int64_t* bin_start;                // [B + 1]
   unsigned long long* cursor1;       // [F1] next free row of each first-level bin (buffer A)
@@ -165,8 +176,11 @@ struct launch_cfg {
 
 void launch_hash_rows(dcol const* d_cols, int ncols, int64_t n, uint32_t seed, uint32_t* out,
                       hipStream_t s);
-void launch_sample_hash(gb_params const* dP, int64_t n, int64_t nsample, uint32_t* out,
-                        hipStream_t s);
+// Distinct row hashes among `nsample` evenly strided rows: counters[0] =
+// rows sampled (dropped rows excluded), counters[1] = distinct hashes.
+// `table` has table_mask + 1 zeroed slots.
+void launch_sample_hash(gb_params const* dP, int64_t n, int64_t nsample, uint64_t* table,
+                        uint32_t table_mask, uint32_t* counters, hipStream_t s);
 void launch_init_state(gb_params const* dP, gb_params const& hP, hipStream_t s);
 void launch_aggregate(gb_params const* dP, gb_params const& hP, int num_cus, hipStream_t s);
 void launch_second_pass(gb_params const* dP, gb_params const& hP, int num_cus, hipStream_t s);
@@ -193,8 +207,8 @@ void launch_count_nulls(dcol const& c, int64_t n, uint32_t* d_count, hipStream_t
 // ---- radix-partitioned aggregation (partition_kernels.hip) ------------------
 void launch_rpa_hist(gb_params const* dP, rpa_params const* dR, rpa_params const& hR, int64_t n,
                      int num_cus, hipStream_t s);
-void launch_rpa_scatter1(gb_params const* dP, rpa_params const* dR, int64_t n, int num_cus,
-                         hipStream_t s);
+void launch_rpa_scatter1(gb_params const* dP, rpa_params const* dR, rpa_params const& hR, int64_t n,
+                         int num_cus, hipStream_t s);
 void launch_rpa_scatter2(gb_params const* dP, rpa_params const* dR, rpa_params const& hR,
                          int num_cus, hipStream_t s);
 void launch_rpa_aggregate(gb_params const* dP, rpa_params const* dR, rpa_params const& hR,
