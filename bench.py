@@ -120,6 +120,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--strategy", choices=["global", "partitioned"], default=None,
                     help="force an aggregation strategy (default: chosen per call)")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="run one device copy of MIB MiB first (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the group-by is measured on a "
@@ -141,6 +143,13 @@ def main():
 
     if args.strategy:
         _lib.set_strategy({"global": 1, "partitioned": 2}[args.strategy])
+    if args.calibrate_copy:
+        # a plain device copy of known size, for scaling FETCH_SIZE / WRITE_SIZE
+        src = torch.ones(args.calibrate_copy << 20, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     for _ in range(args.warmup):
         out = gb.aggregate(reqs)
         del out

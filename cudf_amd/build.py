@@ -30,7 +30,7 @@ COMMON_FLAGS = [
     f"-I{os.path.join(ROOT, 'include')}",
 ]
 
-SOURCES = ["groupby_kernels.hip", "partition_kernels.hip", "groupby.cpp", "capi.cpp"]
+SOURCES = ["groupby_kernels.hip", "partition_kernels.hip", "partition_aggregate.hip", "groupby.cpp", "capi.cpp"]
 
 
 def _compile(src: str) -> str:

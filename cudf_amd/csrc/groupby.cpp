@@ -38,7 +38,7 @@ std::atomic<int> g_strategy{MIG_STRATEGY_AUTO};
 
 // Partitioned path: LDS hash slots and byte budget of the aggregating
 // workgroup (two 512-thread workgroups fit in a CU's 160 KB of LDS).
-constexpr int32_t RPA_LDS_SLOTS      = 2048;
+constexpr int32_t RPA_LDS_SLOTS      = 4096;
 constexpr int32_t RPA_LDS_BUDGET     = 76 * 1024;
 constexpr int RPA_MAX_LOG_BINS       = 15;
 constexpr int RPA_MAX_LOG_FANOUT     = 8;
@@ -784,39 +784,42 @@ std::unique_ptr<mig_groupby_result> groupby_aggregate(mig_column_view const* key
   int32_t acc_bytes = 0;
   for (int f = 0; f < nf; f++) acc_bytes += P.fields[f].width;
   if (P.need_count_all) acc_bytes += 4;
-  int32_t const S_slots = RPA_LDS_SLOTS;
-  int32_t const fixed_lds = S_slots * 8 + S_slots * 2 + int32_t(sizeof(uint64_t)) * 4 + 64 + 64;
-  // per dense id: its accumulators plus its slot (u16) for the output pass
-  int32_t D = (RPA_LDS_BUDGET - fixed_lds) / (acc_bytes + 2);
-  D = std::min<int32_t>(D, 4096);
-  rpa_ok = rpa_ok && D >= 64;
+  // LDS of the aggregating workgroup for S hash slots: keys, S + 2 rows of
+  // every accumulator, the compaction map and the shared counters.
+  auto agg_lds_bytes = [&](int32_t S) {
+    int32_t bytes = S * 8;
+    for (int f = 0; f < nf; f++) bytes += ((S + 2) * P.fields[f].width + 7) & ~7;
+    if (P.need_count_all) bytes += ((S + 2) * 4 + 7) & ~7;
+    bytes += ((S + 2) * 2 + 7) & ~7;
+    return bytes + 128;
+  };
+  int32_t S_slots = RPA_LDS_SLOTS;
+  while (S_slots > 64 && agg_lds_bytes(S_slots) > RPA_LDS_BUDGET) S_slots /= 2;
+  rpa_ok = rpa_ok && agg_lds_bytes(S_slots) <= RPA_LDS_BUDGET;
+  int32_t const max_fill = int32_t(0.85 * S_slots);
   bool use_rpa = false;
   if (forced == MIG_STRATEGY_PARTITIONED) use_rpa = rpa_ok;
   else if (forced == MIG_STRATEGY_AUTO) use_rpa = rpa_ok && N >= (1 << 20) && est >= 65536;
 
   if (use_rpa) {
     // LDS offsets (8-byte aligned)
-    int32_t off = 0;
-    off += S_slots * 8;
-    R.ids_off = off;
-    off += S_slots * 2;
-    off = (off + 7) & ~7;
+    int32_t off = S_slots * 8;
     for (int f = 0; f < nf; f++) {
       P.fields[f].lds_off = off;
-      off += ((D * P.fields[f].width) + 7) & ~7;
+      off += ((S_slots + 2) * P.fields[f].width + 7) & ~7;
     }
     R.count_all_off = -1;
     if (P.need_count_all) {
       R.count_all_off = off;
-      off += ((D * 4) + 7) & ~7;
+      off += ((S_slots + 2) * 4 + 7) & ~7;
     }
-    R.slot_of_id_off = off;
-    off += ((D * 2) + 7) & ~7;
+    R.slot_of_j_off = off;
+    off += ((S_slots + 2) * 2 + 7) & ~7;
     R.misc_off  = off;
-    off += 64;
+    off += 128;
     R.lds_bytes = off;
     R.lds_slots = S_slots;
-    R.max_ids   = D;
+    R.max_fill  = max_fill;
     // scatter staging layout: a tile of keys, copied values, flags and bins
     int32_t rec = 8 + 1 + 2;
     for (int v = 0; v < P.num_vals; v++)
@@ -840,7 +843,7 @@ std::unique_ptr<mig_groupby_result> groupby_aggregate(mig_column_view const* key
     int flag_bit = 0;
     for (int v = 0; v < P.num_vals; v++)
       R.val_flag_bit[v] = (P.val_field_begin[v] < P.val_field_begin[v + 1] && vals[v].has_nulls) ? flag_bit++ : -1;
-    double const per_bin = 0.75 * double(D - 2);
+    double const per_bin = 0.6 * double(S_slots);
     int log_bins = 1;
     while (log_bins < RPA_MAX_LOG_BINS && double(int64_t(1) << log_bins) * per_bin < est) log_bins++;
     for (int attempt = 0; attempt < 3 && G < 0; attempt++) {
@@ -879,7 +882,7 @@ std::unique_ptr<mig_groupby_result> groupby_aggregate(mig_column_view const* key
       for (int64_t b = 0; b < B; b++) {
         start[b + 1] = start[b] + hist[b];
         c2[b]        = (unsigned long long)start[b];
-        rows_cap += std::min<int64_t>(hist[b], D);
+        rows_cap += std::min<int64_t>(hist[b], max_fill + 2);
       }
       int64_t const nrows = start[B];
       t1[0] = 0;

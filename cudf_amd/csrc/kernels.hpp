@@ -134,13 +134,15 @@ struct rpa_params {
   int32_t log_bins;   // final bins B = 1 << log_bins
   int32_t log_f1;     // first-level fanout F1 = 1 << log_f1 (== log_bins when one level)
   int32_t two_level;  // 1 when the second scatter pass runs
-  int32_t lds_slots;  // LDS hash slots per aggregating workgroup (power of two)
-  int32_t max_ids;    // dense ids per workgroup, including the two reserved ones
+  int32_t lds_slots;  // LDS hash slots S per aggregating workgroup (power of two);
+                      // accumulator rows S and S+1 hold the null-key group and
+                      // the group of keys equal to EMPTY_KEY
+  int32_t max_fill;   // keys a bin may insert before it counts as overflowing
   int32_t lds_bytes;  // dynamic LDS of the aggregate kernel
-  int32_t ids_off;    // LDS byte offsets of the aggregate kernel
-  int32_t count_all_off;
+  int32_t count_all_off;  // LDS byte offsets of the aggregate kernel
+  int32_t slot_of_j_off;
   int32_t misc_off;
-  int32_t slot_of_id_off;
+  int32_t pad_agg;
   // scatter staging: a tile of rows sorted by bin in LDS before it is written
   int32_t tile_rows;
   int32_t stage_bytes;

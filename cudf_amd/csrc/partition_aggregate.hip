@@ -1,0 +1,503 @@
+// k_rpa_aggregate: one workgroup aggregates one final bin of the radix
+// partition entirely in LDS.
+//
+//   pass A  every row of the bin: find-or-insert its key in an LDS open-
+//           addressing table of S slots (CAS on 64-bit LDS words), then apply
+//           every accumulator of the request at the slot with LDS atomics.
+//           Accumulators are indexed by slot, so a row never waits for
+//           another thread's insert to publish anything.
+//   pass B  VARIANCE / STD: squared deviations from the group mean, with the
+//           slots of the first rows of every thread kept in registers and the
+//           values re-read from L2.
+//   output  occupied slots are compacted by a block scan; consecutive lanes
+//           write consecutive output rows (coalesced) and validity words are
+//           assembled by wave ballots.
+//
+// Replaces the per-row device-wide atomics of the reference
+// (groupby/hash/groupby_kernels.cuh:98-105, detail/aggregation/
+// aggregation.cuh:131-490) and its second VARIANCE pass over the whole input
+// (multi_pass_kernels.cuh:74-111), with the same results per group.
+#include "partition_common.hpp"
+
+namespace mig {
+
+constexpr int WG_SCOPE = __HIP_MEMORY_SCOPE_WORKGROUP;
+
+__device__ __forceinline__ void apply_field_lds(field_desc const& F, uint8_t* lds, int id,
+                                                loaded_value const& v)
+{
+  switch (F.kind) {
+    case FK_SUM_I64:
+      __hip_atomic_fetch_add(lds_at<uint64_t>(lds, F.lds_off, id), uint64_t(v.i), __ATOMIC_RELAXED, WG_SCOPE);
+      break;
+    case FK_SUM_F32:
+      __hip_atomic_fetch_add(lds_at<float>(lds, F.lds_off, id), v.f, __ATOMIC_RELAXED, WG_SCOPE);
+      break;
+    case FK_SUM_F64:
+      __hip_atomic_fetch_add(lds_at<double>(lds, F.lds_off, id), v.d, __ATOMIC_RELAXED, WG_SCOPE);
+      break;
+    case FK_SUMSQ_I64:
+      __hip_atomic_fetch_add(lds_at<uint64_t>(lds, F.lds_off, id), uint64_t(v.i) * uint64_t(v.i),
+                             __ATOMIC_RELAXED, WG_SCOPE);
+      break;
+    case FK_SUMSQ_F32:
+      __hip_atomic_fetch_add(lds_at<float>(lds, F.lds_off, id), v.f * v.f, __ATOMIC_RELAXED, WG_SCOPE);
+      break;
+    case FK_SUMSQ_F64:
+      __hip_atomic_fetch_add(lds_at<double>(lds, F.lds_off, id), v.d * v.d, __ATOMIC_RELAXED, WG_SCOPE);
+      break;
+    case FK_MIN_S32:
+      __hip_atomic_fetch_min(lds_at<int32_t>(lds, F.lds_off, id), int32_t(v.i), __ATOMIC_RELAXED, WG_SCOPE);
+      break;
+    case FK_MAX_S32:
+      __hip_atomic_fetch_max(lds_at<int32_t>(lds, F.lds_off, id), int32_t(v.i), __ATOMIC_RELAXED, WG_SCOPE);
+      break;
+    case FK_MIN_S64:
+      __hip_atomic_fetch_min(lds_at<int64_t>(lds, F.lds_off, id), v.i, __ATOMIC_RELAXED, WG_SCOPE);
+      break;
+    case FK_MAX_S64:
+      __hip_atomic_fetch_max(lds_at<int64_t>(lds, F.lds_off, id), v.i, __ATOMIC_RELAXED, WG_SCOPE);
+      break;
+    case FK_MIN_U32:
+      __hip_atomic_fetch_min(lds_at<uint32_t>(lds, F.lds_off, id), uint32_t(v.i), __ATOMIC_RELAXED, WG_SCOPE);
+      break;
+    case FK_MAX_U32:
+      __hip_atomic_fetch_max(lds_at<uint32_t>(lds, F.lds_off, id), uint32_t(v.i), __ATOMIC_RELAXED, WG_SCOPE);
+      break;
+    case FK_MIN_U64:
+      __hip_atomic_fetch_min(lds_at<uint64_t>(lds, F.lds_off, id), uint64_t(v.i), __ATOMIC_RELAXED, WG_SCOPE);
+      break;
+    case FK_MAX_U64:
+      __hip_atomic_fetch_max(lds_at<uint64_t>(lds, F.lds_off, id), uint64_t(v.i), __ATOMIC_RELAXED, WG_SCOPE);
+      break;
+    case FK_MIN_F32:
+      if (!isnan(v.f))
+        __hip_atomic_fetch_min(lds_at<uint32_t>(lds, F.lds_off, id), ord32(uint32_t(v.b)), __ATOMIC_RELAXED,
+                               WG_SCOPE);
+      break;
+    case FK_MAX_F32:
+      if (!isnan(v.f))
+        __hip_atomic_fetch_max(lds_at<uint32_t>(lds, F.lds_off, id), ord32(uint32_t(v.b)), __ATOMIC_RELAXED,
+                               WG_SCOPE);
+      break;
+    case FK_MIN_F64:
+      if (!isnan(v.d))
+        __hip_atomic_fetch_min(lds_at<uint64_t>(lds, F.lds_off, id), ord64(v.b), __ATOMIC_RELAXED, WG_SCOPE);
+      break;
+    case FK_MAX_F64:
+      if (!isnan(v.d))
+        __hip_atomic_fetch_max(lds_at<uint64_t>(lds, F.lds_off, id), ord64(v.b), __ATOMIC_RELAXED, WG_SCOPE);
+      break;
+    case FK_COUNT:
+      __hip_atomic_fetch_add(lds_at<uint32_t>(lds, F.lds_off, id), 1u, __ATOMIC_RELAXED, WG_SCOPE);
+      break;
+    default: break;
+  }
+}
+
+__device__ __forceinline__ uint64_t lds_field_bits(field_desc const& F, uint8_t* lds, int id)
+{
+  return F.width == 8 ? *lds_at<uint64_t>(lds, F.lds_off, id) : *lds_at<uint32_t>(lds, F.lds_off, id);
+}
+
+__device__ __forceinline__ double lds_field_double(field_desc const& F, uint8_t* lds, int id)
+{
+  switch (F.kind) {
+    case FK_SUM_I64: return double(*lds_at<int64_t>(lds, F.lds_off, id));
+    case FK_SUM_F32: return double(*lds_at<float>(lds, F.lds_off, id));
+    case FK_SUM_F64: return *lds_at<double>(lds, F.lds_off, id);
+    case FK_COUNT: return double(*lds_at<uint32_t>(lds, F.lds_off, id));
+    default: return 0.0;
+  }
+}
+
+__device__ __forceinline__ uint32_t lds_count(gb_params const& P, rpa_params const& R, uint8_t* lds,
+                                              int32_t f, int id)
+{
+  return f == -2 ? *lds_at<uint32_t>(lds, R.count_all_off, id)
+                 : *lds_at<uint32_t>(lds, P.fields[f].lds_off, id);
+}
+
+// Validity bits of the 64 output rows [p0, p0 + 64) of one wave, clipped to
+// the workgroup's rows [.., end): words wholly inside are stored, words shared
+// with a neighbouring workgroup are OR-ed in (the mask starts zeroed).
+__device__ __forceinline__ void write_valid_bits(uint32_t* mask, uint64_t p0, uint64_t end, bool valid)
+{
+  uint64_t const bal = __ballot(valid);
+  if ((threadIdx.x & 63) != 0) return;
+  uint64_t const last = (p0 + 64 < end) ? p0 + 64 : end;
+  if (last <= p0) return;
+  for (uint64_t w = p0 >> 5; w <= (last - 1) >> 5; w++) {
+    int64_t const shift = int64_t(w * 32) - int64_t(p0);
+    uint32_t bits       = shift >= 0 ? uint32_t(bal >> shift) : uint32_t(bal << (-shift));
+    bool const whole    = (w * 32 >= p0) && (w * 32 + 32 <= last);
+    if (whole) mask[w] = bits;
+    else if (bits) __hip_atomic_fetch_or(mask + w, bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+struct rpa_misc {
+  uint32_t fill;      // keys inserted into the slot table
+  uint32_t present0;  // a null key (INCLUDE) was seen
+  uint32_t present1;  // a key equal to EMPTY_KEY was seen
+  int32_t overflow;
+  uint32_t g_out;
+  uint32_t pad;
+  unsigned long long base;
+  uint32_t wave_sum[RPA_AGG_BLOCK / 64];
+};
+
+// Slot of a partitioned row: S = null key, S + 1 = key equal to EMPTY_KEY,
+// else its LDS table slot (inserted when INSERT); -1 when the table is full.
+template <bool INSERT>
+__device__ __forceinline__ int row_slot(gb_params const& P, rpa_params const& R, uint64_t* tkeys,
+                                        rpa_misc* misc, int S, uint8_t fl, uint64_t key)
+{
+  if (fl & RPA_KEY_NULL) {
+    if (INSERT) misc->present0 = 1;
+    return S;
+  }
+  if (key == EMPTY_KEY) {
+    if (INSERT) misc->present1 = 1;
+    return S + 1;
+  }
+  uint32_t slot = inline_key_hash(P.key_storage, key) & uint32_t(S - 1);
+  for (int i = 0; i < S; i++) {
+    uint64_t const cur = __hip_atomic_load(tkeys + slot, __ATOMIC_RELAXED, WG_SCOPE);
+    if (cur == key) return int(slot);
+    if (cur == EMPTY_KEY) {
+      if (!INSERT) return -1;
+      if (__hip_atomic_load(&misc->fill, __ATOMIC_RELAXED, WG_SCOPE) >= uint32_t(R.max_fill)) {
+        __hip_atomic_store(&misc->overflow, 1, __ATOMIC_RELAXED, WG_SCOPE);
+        return -1;
+      }
+      uint64_t expected = EMPTY_KEY;
+      if (__hip_atomic_compare_exchange_strong(tkeys + slot, &expected, key, __ATOMIC_RELAXED,
+                                               __ATOMIC_RELAXED, WG_SCOPE)) {
+        __hip_atomic_fetch_add(&misc->fill, 1u, __ATOMIC_RELAXED, WG_SCOPE);
+        return int(slot);
+      }
+      if (expected == key) return int(slot);
+    }
+    slot = (slot + 1) & uint32_t(S - 1);
+  }
+  if (INSERT) __hip_atomic_store(&misc->overflow, 1, __ATOMIC_RELAXED, WG_SCOPE);
+  return -1;
+}
+
+constexpr int RPA_CACHED = 8;  // rows per thread per chunk; their slots stay in registers
+
+// Raw 8 bytes of element r of a partition value column (narrow types are
+// zero-extended; load_value-style decoding happens in decode_raw).
+__device__ __forceinline__ uint64_t load_raw(uint8_t const* base, int w, int64_t r)
+{
+  switch (w) {
+    case 1: return base[r];
+    case 2: return reinterpret_cast<uint16_t const*>(base)[r];
+    case 4: return reinterpret_cast<uint32_t const*>(base)[r];
+    default: return reinterpret_cast<uint64_t const*>(base)[r];
+  }
+}
+
+__device__ __forceinline__ loaded_value decode_raw(int32_t st, uint64_t raw)
+{
+  loaded_value v{0, 0.0, 0.0f, 0};
+  switch (st) {
+    case ST_I8: v.i = int8_t(raw); v.d = double(v.i); break;
+    case ST_I16: v.i = int16_t(raw); v.d = double(v.i); break;
+    case ST_I32: v.i = int32_t(raw); v.d = double(v.i); break;
+    case ST_I64: v.i = int64_t(raw); v.d = double(v.i); break;
+    case ST_BOOL: v.i = raw != 0; v.d = double(v.i); break;
+    case ST_U8:
+    case ST_U16:
+    case ST_U32: v.i = int64_t(raw); v.d = double(v.i); break;
+    case ST_U64: v.i = int64_t(raw); v.d = double(raw); break;
+    case ST_F32: v.b = raw; v.f = __uint_as_float(uint32_t(raw)); v.d = double(v.f); break;
+    case ST_F64: v.b = raw; v.d = __longlong_as_double(int64_t(raw)); break;
+    default: break;
+  }
+  return v;
+}
+
+// Slots of the rows [r0 + tid + k * block) of one chunk: all keys and flags
+// are loaded first (RPA_CACHED loads in flight per thread), then probed.
+template <bool INSERT>
+__device__ __forceinline__ void chunk_slots(gb_params const& P, rpa_params const& R, uint64_t* tkeys,
+                                            rpa_misc* misc, int S, int64_t r0, int64_t e,
+                                            int (&slot)[RPA_CACHED], uint8_t (&fl)[RPA_CACHED])
+{
+  uint64_t key[RPA_CACHED];
+#pragma unroll
+  for (int k = 0; k < RPA_CACHED; k++) {
+    int64_t const r = r0 + int64_t(k) * RPA_AGG_BLOCK;
+    fl[k]  = r < e ? R.flagsB[r] : 0;
+    key[k] = r < e ? R.keysB[r] : 0;
+  }
+#pragma unroll
+  for (int k = 0; k < RPA_CACHED; k++) {
+    int64_t const r = r0 + int64_t(k) * RPA_AGG_BLOCK;
+    slot[k]         = -1;
+    if (r < e && !__hip_atomic_load(&misc->overflow, __ATOMIC_RELAXED, WG_SCOPE))
+      slot[k] = row_slot<INSERT>(P, R, tkeys, misc, S, fl[k], key[k]);
+  }
+}
+
+// Pass A of one chunk: every non-variance accumulator, column by column, with
+// the chunk's values of a column loaded together.
+__device__ __forceinline__ void chunk_accumulate(gb_params const& P, rpa_params const& R, uint8_t* lds,
+                                                 int64_t r0, int64_t e, int const (&slot)[RPA_CACHED],
+                                                 uint8_t const (&fl)[RPA_CACHED])
+{
+  if (R.count_all_off >= 0) {
+#pragma unroll
+    for (int k = 0; k < RPA_CACHED; k++)
+      if (slot[k] >= 0)
+        __hip_atomic_fetch_add(lds_at<uint32_t>(lds, R.count_all_off, slot[k]), 1u, __ATOMIC_RELAXED, WG_SCOPE);
+  }
+  for (int v = 0; v < P.num_vals; v++) {
+    int const fb = P.val_field_begin[v], fe = P.val_field_begin[v + 1];
+    if (fb == fe) continue;
+    int const bit = R.val_flag_bit[v];
+    int const st  = P.vals[v].storage;
+    int const w   = storage_width(st);
+    uint64_t raw[RPA_CACHED];
+#pragma unroll
+    for (int k = 0; k < RPA_CACHED; k++) {
+      int64_t const r = r0 + int64_t(k) * RPA_AGG_BLOCK;
+      raw[k]          = (r < e) ? load_raw(R.valsB[v], w, r) : 0;
+    }
+#pragma unroll
+    for (int k = 0; k < RPA_CACHED; k++) {
+      if (slot[k] < 0) continue;
+      if (bit >= 0 && !((fl[k] >> bit) & 1)) continue;
+      loaded_value const val = decode_raw(st, raw[k]);
+      for (int f = fb; f < fe; f++)
+        if (P.fields[f].kind != FK_M2) apply_field_lds(P.fields[f], lds, slot[k], val);
+    }
+  }
+}
+
+// Pass B of one chunk: squared deviations from the group mean.
+__device__ __forceinline__ void chunk_deviations(gb_params const& P, rpa_params const& R, uint8_t* lds,
+                                                 int64_t r0, int64_t e, int const (&slot)[RPA_CACHED],
+                                                 uint8_t const (&fl)[RPA_CACHED])
+{
+  for (int v = 0; v < P.num_vals; v++) {
+    int const fb = P.val_field_begin[v], fe = P.val_field_begin[v + 1];
+    int m2 = -1;
+    for (int f = fb; f < fe; f++)
+      if (P.fields[f].kind == FK_M2) m2 = f;
+    if (m2 < 0) continue;
+    int const bit       = R.val_flag_bit[v];
+    int const st        = P.vals[v].storage;
+    int const w         = storage_width(st);
+    field_desc const& F = P.fields[m2];
+    uint64_t raw[RPA_CACHED];
+#pragma unroll
+    for (int k = 0; k < RPA_CACHED; k++) {
+      int64_t const r = r0 + int64_t(k) * RPA_AGG_BLOCK;
+      raw[k]          = (r < e) ? load_raw(R.valsB[v], w, r) : 0;
+    }
+#pragma unroll
+    for (int k = 0; k < RPA_CACHED; k++) {
+      if (slot[k] < 0) continue;
+      if (bit >= 0 && !((fl[k] >> bit) & 1)) continue;
+      double const x    = decode_raw(st, raw[k]).d;
+      double const cnt  = double(lds_count(P, R, lds, F.aux_count, slot[k]));
+      double const mean = lds_field_double(P.fields[F.aux_sum], lds, slot[k]) / cnt;
+      double const d    = x - mean;
+      __hip_atomic_fetch_add(lds_at<double>(lds, F.lds_off, slot[k]), d * d, __ATOMIC_RELAXED, WG_SCOPE);
+    }
+  }
+}
+
+__global__ void __launch_bounds__(RPA_AGG_BLOCK)
+  k_rpa_aggregate(gb_params const* __restrict__ Pp, rpa_params const* __restrict__ Rp)
+{
+  extern __shared__ __attribute__((aligned(16))) uint8_t lds[];
+  gb_params const& P = *Pp;
+  rpa_params const& R = *Rp;
+  int const b         = blockIdx.x;
+  int64_t const s     = R.bin_start[b];
+  int64_t const e     = R.bin_start[b + 1];
+  if (s == e) return;
+  int const S       = R.lds_slots;
+  int const NS      = S + 2;
+  auto* tkeys       = reinterpret_cast<uint64_t*>(lds);
+  auto* slot_of_j   = reinterpret_cast<uint16_t*>(lds + R.slot_of_j_off);
+  auto* misc        = reinterpret_cast<rpa_misc*>(lds + R.misc_off);
+  int const tid     = threadIdx.x;
+
+  // ---- init
+  for (int i = tid; i < S; i += RPA_AGG_BLOCK) tkeys[i] = EMPTY_KEY;
+  for (int f = 0; f < P.num_fields; f++) {
+    field_desc const& F = P.fields[f];
+    uint64_t const idv  = field_identity(F.kind);
+    if (F.width == 8)
+      for (int i = tid; i < NS; i += RPA_AGG_BLOCK) *lds_at<uint64_t>(lds, F.lds_off, i) = idv;
+    else
+      for (int i = tid; i < NS; i += RPA_AGG_BLOCK) *lds_at<uint32_t>(lds, F.lds_off, i) = uint32_t(idv);
+  }
+  if (R.count_all_off >= 0)
+    for (int i = tid; i < NS; i += RPA_AGG_BLOCK) *lds_at<uint32_t>(lds, R.count_all_off, i) = 0;
+  if (tid == 0) {
+    misc->fill     = 0;
+    misc->present0 = 0;
+    misc->present1 = 0;
+    misc->overflow = 0;
+  }
+  __syncthreads();
+
+  // ---- pass A: find-or-insert and accumulate, chunk by chunk; the slots of
+  // the first chunk stay in registers for pass B
+  constexpr int64_t CHUNK = int64_t(RPA_CACHED) * RPA_AGG_BLOCK;
+  int slot0[RPA_CACHED];
+  uint8_t fl0[RPA_CACHED];
+  chunk_slots<true>(P, R, tkeys, misc, S, s + tid, e, slot0, fl0);
+  chunk_accumulate(P, R, lds, s + tid, e, slot0, fl0);
+  for (int64_t c = s + CHUNK; c < e; c += CHUNK) {
+    if (__hip_atomic_load(&misc->overflow, __ATOMIC_RELAXED, WG_SCOPE)) break;
+    int slot[RPA_CACHED];
+    uint8_t fl[RPA_CACHED];
+    chunk_slots<true>(P, R, tkeys, misc, S, c + tid, e, slot, fl);
+    chunk_accumulate(P, R, lds, c + tid, e, slot, fl);
+  }
+  __syncthreads();
+  if (misc->overflow) {
+    if (tid == 0) astore(R.overflow, 1);
+    return;
+  }
+
+  // ---- pass B: squared deviations from the group mean
+  if (P.need_second_pass) {
+    chunk_deviations(P, R, lds, s + tid, e, slot0, fl0);
+    for (int64_t c = s + CHUNK; c < e; c += CHUNK) {
+      int slot[RPA_CACHED];
+      uint8_t fl[RPA_CACHED];
+      chunk_slots<false>(P, R, tkeys, misc, S, c + tid, e, slot, fl);
+      chunk_deviations(P, R, lds, c + tid, e, slot, fl);
+    }
+    __syncthreads();
+  }
+
+  // ---- compact occupied slots (null-key group first, EMPTY_KEY group next)
+  uint32_t const p0 = misc->present0, p1 = misc->present1;
+  int const per     = (NS + RPA_AGG_BLOCK - 1) / RPA_AGG_BLOCK;
+  int const i0      = tid * per;
+  int const i1      = (i0 + per < NS) ? i0 + per : NS;
+  // order: S (null), S+1 (EMPTY_KEY), then table slots 0..S-1
+  auto occupied = [&](int k) -> bool {
+    int const slot = k < 2 ? S + k : k - 2;
+    if (slot == S) return p0 != 0;
+    if (slot == S + 1) return p1 != 0;
+    return tkeys[slot] != EMPTY_KEY;
+  };
+  uint32_t c = 0;
+  for (int k = i0; k < i1; k++) c += occupied(k) ? 1u : 0u;
+  // block exclusive scan of c
+  int const lane = tid & 63, wave = tid >> 6;
+  uint32_t incl = c;
+  for (int o = 1; o < 64; o <<= 1) {
+    uint32_t y = __shfl_up(incl, o, 64);
+    if (lane >= o) incl += y;
+  }
+  if (lane == 63) misc->wave_sum[wave] = incl;
+  __syncthreads();
+  uint32_t before = 0, total = 0;
+  for (int w = 0; w < RPA_AGG_BLOCK / 64; w++) {
+    if (w < wave) before += misc->wave_sum[w];
+    total += misc->wave_sum[w];
+  }
+  uint32_t j = before + incl - c;
+  for (int k = i0; k < i1; k++)
+    if (occupied(k)) slot_of_j[j++] = uint16_t(k < 2 ? S + k : k - 2);
+  if (tid == 0) {
+    misc->g_out = total;
+    misc->base  = __hip_atomic_fetch_add(R.group_counter, (unsigned long long)total, __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __syncthreads();
+
+  // ---- outputs: output row base + j for j in [0, g_out)
+  uint32_t const g_out           = misc->g_out;
+  unsigned long long const base  = misc->base;
+  uint64_t const end             = base + g_out;
+  int const kw                   = storage_width(P.key_storage);
+  for (uint32_t j0 = uint32_t(tid) & ~63u; j0 < g_out; j0 += RPA_AGG_BLOCK) {
+    uint32_t const jj   = j0 + lane;
+    bool const live     = jj < g_out;
+    uint64_t const pos  = base + jj;
+    int const slot      = live ? int(slot_of_j[jj]) : 0;
+    bool const in_range = live && pos < R.out_capacity;
+    bool const kvalid   = slot != S;
+    if (live && !in_range) astore(R.overflow, 1);
+    if (in_range) {
+      uint64_t const key = slot < S ? tkeys[slot] : (slot == S ? 0ull : EMPTY_KEY);
+      store_width(P.out_keys[0], int64_t(pos), kw, key);
+    }
+    if (P.out_key_mask[0]) write_valid_bits(P.out_key_mask[0], base + j0, end, in_range && kvalid);
+    for (int o = 0; o < P.num_outs; o++) {
+      out_desc const& O = P.outs[o];
+      bool valid        = true;
+      if (in_range) {
+        uint32_t cnt = 0;
+        if (O.count_field != -1) {
+          cnt   = lds_count(P, R, lds, O.count_field, slot);
+          valid = cnt > 0;
+        }
+        switch (O.kind) {
+          case OK_COPY_I64:
+          case OK_COPY_F64:
+            reinterpret_cast<uint64_t*>(O.data)[pos] = lds_field_bits(P.fields[O.field], lds, slot);
+            break;
+          case OK_COPY_F32:
+            reinterpret_cast<uint32_t*>(O.data)[pos] = uint32_t(lds_field_bits(P.fields[O.field], lds, slot));
+            break;
+          case OK_COUNT:
+            reinterpret_cast<int32_t*>(O.data)[pos] =
+              int32_t(*lds_at<uint32_t>(lds, P.fields[O.field].lds_off, slot));
+            valid = true;
+            break;
+          case OK_COUNT_ALL:
+            reinterpret_cast<int32_t*>(O.data)[pos] = int32_t(*lds_at<uint32_t>(lds, R.count_all_off, slot));
+            valid = true;
+            break;
+          case OK_MINMAX: {
+            field_desc const& F = P.fields[O.field];
+            uint64_t bits       = lds_field_bits(F, lds, slot);
+            if (F.kind == FK_MIN_F32 || F.kind == FK_MAX_F32) bits = unord32(uint32_t(bits));
+            else if (F.kind == FK_MIN_F64 || F.kind == FK_MAX_F64) bits = unord64(bits);
+            store_width(O.data, int64_t(pos), O.out_width, bits);
+            break;
+          }
+          case OK_MEAN: {
+            double const sum = lds_field_double(P.fields[O.field], lds, slot);
+            reinterpret_cast<double*>(O.data)[pos] = valid ? sum / double(cnt) : 0.0;
+            break;
+          }
+          case OK_VAR:
+          case OK_STD: {
+            int64_t const dn = int64_t(cnt) - O.ddof;
+            valid            = cnt > 0 && dn > 0;
+            double var = valid ? *lds_at<double>(lds, P.fields[O.field].lds_off, slot) / double(dn) : 0.0;
+            reinterpret_cast<double*>(O.data)[pos] = (O.kind == OK_STD) ? sqrt(var) : var;
+            break;
+          }
+          default: break;
+        }
+      }
+      if (O.mask) write_valid_bits(O.mask, base + j0, end, in_range && valid);
+    }
+  }
+}
+
+void launch_rpa_aggregate(gb_params const* dP, rpa_params const* dR, rpa_params const& hR, hipStream_t s)
+{
+  int64_t const B = int64_t(1) << hR.log_bins;
+  if (hR.lds_bytes > 64 * 1024)
+    MIG_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<void const*>(&k_rpa_aggregate),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, hR.lds_bytes));
+  hipLaunchKernelGGL(k_rpa_aggregate, dim3(B), dim3(RPA_AGG_BLOCK), hR.lds_bytes, s, dP, dR);
+}
+
+}  // namespace mig
