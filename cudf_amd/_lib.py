@@ -84,12 +84,16 @@ class OutColumn(ctypes.Structure):
         ("offsets_type_id", ctypes.c_int32),
         ("offsets", ctypes.c_void_p),
         ("data_bytes", ctypes.c_int64),
+        ("mask_bytes", ctypes.c_int64),
+        ("offsets_bytes", ctypes.c_int64),
     ]
 
 
 # every function the header declares, with its ctypes signature
 SIGNATURES = {
     "mig_last_error": (ctypes.c_char_p, []),
+    "mig_device_allocate": (ctypes.c_void_p, [ctypes.c_size_t, ctypes.c_void_p]),
+    "mig_device_deallocate": (None, [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]),
     "mig_abi_version": (ctypes.c_int32, []),
     "mig_hash_murmurhash3_x86_32": (ctypes.c_int, [ctypes.POINTER(ColumnView), ctypes.c_int32,
                                                    ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p]),

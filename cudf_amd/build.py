@@ -42,6 +42,22 @@ def _compile(src: str) -> str:
     return obj
 
 
+CXX_TEST_SRC = os.path.join(ROOT, "tests", "cpp", "test_cxx_api.cpp")
+CXX_TEST_BIN = os.path.join(ROOT, "tests", "cpp", "test_cxx_api")
+
+
+def build_cxx_test() -> str:
+    """The C++ API test: plain g++ against include/ (the cudf:: facade) and
+    libmig_groupby.so; the HIP runtime only moves test data."""
+    cmd = ["g++", "-std=c++17", "-O1", "-D__HIP_PLATFORM_AMD__", f"-I{os.path.join(ROOT, 'include')}",
+           "-I/opt/rocm/include", CXX_TEST_SRC, "-o", CXX_TEST_BIN, f"-L{LIBDIR}", "-lmig_groupby",
+           "-L/opt/rocm/lib", "-lamdhip64", f"-Wl,-rpath,{LIBDIR}", "-Wl,-rpath,$ORIGIN/../../cudf_amd/lib"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"C++ API test build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    return CXX_TEST_BIN
+
+
 def build(verbose: bool = False) -> str:
     os.makedirs(BUILD, exist_ok=True)
     os.makedirs(LIBDIR, exist_ok=True)
@@ -53,8 +69,9 @@ def build(verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    build_cxx_test()
     if verbose:
-        print(f"built {LIB}")
+        print(f"built {LIB} and {CXX_TEST_BIN}")
     return LIB
 
 

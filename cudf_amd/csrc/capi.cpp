@@ -49,6 +49,16 @@ mig::owned_column* result_col(mig_groupby_result const* r, int32_t request, int3
   return &v[agg];
 }
 
+// The C view of an owned column, with the allocation sizes of its buffers.
+mig_column exported(mig::owned_column const& c)
+{
+  mig_column out    = c.col;
+  out.data_bytes    = int64_t(c.data_bytes);
+  out.mask_bytes    = int64_t(c.mask_bytes);
+  out.offsets_bytes = int64_t(c.offsets_bytes);
+  return out;
+}
+
 mig::owned_column* key_col(mig_groupby_result const* r, int32_t i)
 {
   if (!r) mig::fail(MIG_ERROR_LOGIC, "null result handle");
@@ -63,6 +73,21 @@ extern "C" {
 const char* mig_last_error(void) { return last_error().c_str(); }
 
 int32_t mig_abi_version(void) { return MIG_ABI_VERSION; }
+
+void* mig_device_allocate(size_t bytes, void* stream)
+{
+  void* p = nullptr;
+  if (hipMallocAsync(&p, bytes == 0 ? 1 : bytes, static_cast<hipStream_t>(stream)) != hipSuccess) {
+    last_error() = "hipMallocAsync failed";
+    return nullptr;
+  }
+  return p;
+}
+
+void mig_device_deallocate(void* ptr, size_t, void* stream)
+{
+  if (ptr) (void)hipFreeAsync(ptr, static_cast<hipStream_t>(stream));
+}
 
 mig_status mig_hash_murmurhash3_x86_32(const mig_column_view* columns, int32_t num_columns,
                                        uint32_t seed, uint32_t* out, void* stream)
@@ -105,7 +130,7 @@ mig_status mig_result_key(const mig_groupby_result* r, int32_t i, mig_column* ou
 {
   return guarded([&] {
     if (!out) mig::fail(MIG_ERROR_LOGIC, "null output column");
-    *out = key_col(r, i)->col;
+    *out = exported(*key_col(r, i));
   });
 }
 
@@ -113,7 +138,7 @@ mig_status mig_result_column(const mig_groupby_result* r, int32_t request, int32
 {
   return guarded([&] {
     if (!out) mig::fail(MIG_ERROR_LOGIC, "null output column");
-    *out = result_col(r, request, agg)->col;
+    *out = exported(*result_col(r, request, agg));
   });
 }
 
@@ -122,7 +147,7 @@ mig_status mig_result_release_key(mig_groupby_result* r, int32_t i, mig_column* 
   return guarded([&] {
     if (!out) mig::fail(MIG_ERROR_LOGIC, "null output column");
     auto* c = key_col(r, i);
-    *out    = c->col;
+    *out    = exported(*c);
     *c      = mig::owned_column{};
   });
 }
@@ -132,7 +157,7 @@ mig_status mig_result_release_column(mig_groupby_result* r, int32_t request, int
   return guarded([&] {
     if (!out) mig::fail(MIG_ERROR_LOGIC, "null output column");
     auto* c = result_col(r, request, agg);
-    *out    = c->col;
+    *out    = exported(*c);
     *c      = mig::owned_column{};
   });
 }

@@ -1,0 +1,97 @@
+// Aggregation requests (drop-in for cpp/include/cudf/aggregation.hpp:86-337:
+// aggregation::Kind numbering and the make_*_aggregation factories used by
+// group-by).
+#pragma once
+
+#include <memory>
+
+#include "types.hpp"
+
+namespace cudf {
+
+class aggregation {
+ public:
+  enum Kind {
+    SUM,
+    PRODUCT,
+    MIN,
+    MAX,
+    COUNT_VALID,
+    COUNT_ALL,
+    ANY,
+    ALL,
+    SUM_OF_SQUARES,
+    MEAN,
+    M2,
+    VARIANCE,
+    STD,
+    MEDIAN,
+    QUANTILE,
+    ARGMAX,
+    ARGMIN,
+    NUNIQUE
+  };
+  explicit aggregation(Kind k, size_type ddof = 1) : kind{k}, _ddof{ddof} {}
+  virtual ~aggregation() = default;
+  [[nodiscard]] virtual std::unique_ptr<aggregation> clone() const
+  {
+    return std::make_unique<aggregation>(*this);
+  }
+  Kind kind;
+  size_type _ddof;  // VARIANCE / STD
+};
+
+/// Aggregations valid in a group-by (aggregation.hpp groupby_aggregation).
+class groupby_aggregation : public virtual aggregation {
+ public:
+  explicit groupby_aggregation(Kind k, size_type ddof = 1) : aggregation(k, ddof) {}
+  [[nodiscard]] std::unique_ptr<aggregation> clone() const override
+  {
+    return std::make_unique<groupby_aggregation>(*this);
+  }
+};
+
+template <typename Base = aggregation>
+std::unique_ptr<Base> make_sum_aggregation()
+{
+  return std::make_unique<Base>(aggregation::SUM);
+}
+template <typename Base = aggregation>
+std::unique_ptr<Base> make_min_aggregation()
+{
+  return std::make_unique<Base>(aggregation::MIN);
+}
+template <typename Base = aggregation>
+std::unique_ptr<Base> make_max_aggregation()
+{
+  return std::make_unique<Base>(aggregation::MAX);
+}
+/// COUNT_VALID under EXCLUDE, COUNT_ALL under INCLUDE (aggregation.cpp:452-463).
+template <typename Base = aggregation>
+std::unique_ptr<Base> make_count_aggregation(null_policy null_handling = null_policy::EXCLUDE)
+{
+  return std::make_unique<Base>(null_handling == null_policy::EXCLUDE ? aggregation::COUNT_VALID
+                                                                       : aggregation::COUNT_ALL);
+}
+template <typename Base = aggregation>
+std::unique_ptr<Base> make_sum_of_squares_aggregation()
+{
+  return std::make_unique<Base>(aggregation::SUM_OF_SQUARES);
+}
+template <typename Base = aggregation>
+std::unique_ptr<Base> make_mean_aggregation()
+{
+  return std::make_unique<Base>(aggregation::MEAN);
+}
+template <typename Base = aggregation>
+std::unique_ptr<Base> make_variance_aggregation(size_type ddof = 1)
+{
+  return std::make_unique<Base>(aggregation::VARIANCE, ddof);
+}
+template <typename Base = aggregation>
+std::unique_ptr<Base> make_std_aggregation(size_type ddof = 1)
+{
+  return std::make_unique<Base>(aggregation::STD, ddof);
+}
+
+}  // namespace cudf

@@ -149,7 +149,9 @@ typedef struct mig_column {
   int32_t null_count;
   int32_t offsets_type_id;  /* STRING: MIG_TYPE_INT32                         */
   void* offsets;            /* STRING: size+1 offsets                         */
-  int64_t data_bytes;       /* bytes in `data`                                */
+  int64_t data_bytes;       /* bytes allocated for `data` (>= size * width)   */
+  int64_t mask_bytes;       /* bytes allocated for `null_mask`                */
+  int64_t offsets_bytes;    /* bytes allocated for `offsets`                  */
 } mig_column;
 
 typedef struct mig_groupby_result mig_groupby_result;
@@ -161,6 +163,14 @@ const char* mig_last_error(void);
 
 /* MIG_ABI_VERSION of the loaded library. */
 int32_t mig_abi_version(void);
+
+/* Device memory of the library's default resource (stream-ordered
+ * hipMallocAsync / hipFreeAsync): what result columns use when no resource
+ * is passed, and what a caller uses to free them.  Plays the role of
+ * rmm::mr::get_current_device_resource() in the reference API
+ * (cudf/groupby.hpp:187-199).  Returns NULL on failure. */
+void* mig_device_allocate(size_t bytes, void* stream);
+void mig_device_deallocate(void* ptr, size_t bytes, void* stream);
 
 /* ---- row hashing -----------------------------------------------------------
  * Replaces cudf::hashing::murmurhash3_x86_32(table_view, seed, stream, mr)
