@@ -109,6 +109,108 @@ def cpu_baselines(sample_rows, card, seed=1):
     return out
 
 
+def c4_shapes(n_inst, seed=4):
+    """Row counts log-uniform 1e3..4e6, cardinality log-uniform 1..rows/2."""
+    rng = np.random.default_rng(seed)
+    rows = np.round(np.exp(rng.uniform(np.log(1e3), np.log(4e6), n_inst))).astype(np.int64)
+    card = np.maximum(1, np.round(np.exp(rng.uniform(0.0, np.log(0.5 * rows))))).astype(np.int64)
+    return rows, card
+
+
+def make_c4(n_inst, seed=4):
+    """The batched workload on the device: per instance an int32 key uniform
+    over [0, card), an int64 value uniform in [-1e9, 1e9] and a float32 value
+    uniform in [0, 1); instances are views into three concatenated buffers."""
+    import torch
+    from cudf_amd import Column, TypeId
+    rows, card = c4_shapes(n_inst, seed)
+    total = int(rows.sum())
+    dev = torch.device("cuda", 0)
+    g = torch.Generator(device=dev).manual_seed(seed)
+    keys = torch.empty(total, dtype=torch.int32, device=dev)
+    starts = np.concatenate([[0], np.cumsum(rows)[:-1]])
+    for s, r, c in zip(starts, rows, card):
+        keys[s:s + r].random_(0, int(c), generator=g)
+    v0 = torch.randint(-1_000_000_000, 1_000_000_001, (total,), dtype=torch.int64, device=dev, generator=g)
+    v1 = torch.rand(total, dtype=torch.float32, device=dev, generator=g)
+    kb, v0b, v1b = keys.view(torch.uint8), v0.view(torch.uint8), v1.view(torch.uint8)
+    inst = []
+    for s, r in zip(starts, rows):
+        inst.append((Column(TypeId.INT32, int(r), kb, None, 0, int(s)),
+                     [Column(TypeId.INT64, int(r), v0b, None, 0, int(s)),
+                      Column(TypeId.FLOAT32, int(r), v1b, None, 0, int(s))]))
+    return inst, rows, card, (keys, v0, v1)
+
+
+def bench_batched(n_inst, reps, autotune=True, plain=True):
+    """Batched form against the plain one-call-per-instance loop on the same
+    instances (sum, count, min, max on both values)."""
+    import torch
+    import cudf_amd as ca
+    from cudf_amd import autotune as at
+    from cudf_amd.aggregation import Aggregation, Kind
+    inst, rows, card, keep = make_c4(n_inst)
+    torch.cuda.synchronize()
+    mk = lambda: [Aggregation(Kind.SUM), Aggregation(Kind.COUNT_VALID), Aggregation(Kind.MIN),
+                  Aggregation(Kind.MAX)]
+    aggs = [mk(), mk()]
+    t0 = time.perf_counter()
+    opts = at.tune(inst, aggs, sample=16) if autotune else ca.BatchOptions()
+    t_tune = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    plan = ca.BatchedGroupBy(inst, aggs, ca.NullPolicy.EXCLUDE, opts)
+    t_plan = time.perf_counter() - t0
+    plan.run()  # warm-up
+    times = []
+    for _ in range(reps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        plan.run()
+        times.append(time.perf_counter() - t0)
+    t_batch = float(np.median(times))
+    groups = sum(plan.num_groups(i) for i in range(n_inst))
+    plan.close()
+    total = int(rows.sum())
+    in_b = total * (4 + 8 + 4)
+    out_b = groups * (4 + 8 + 4 + 8 + 8 + 4 + 4 + 4 + 4)
+    res = {
+        "workload": f"{n_inst} independent group-bys in one plan: rows log-uniform 1e3..4e6 "
+                    f"(total {total}), cardinality log-uniform 1..rows/2 (total {groups} groups), int32 "
+                    f"keys, int64 and float32 values, sum/count/min/max on both",
+        "instances": n_inst, "rows": total, "groups": groups,
+        "options": {k: getattr(opts, k) for k in ("load_factor", "lds_max_groups", "block_size",
+                                                   "tile_rows")},
+        "autotune_s": round(t_tune, 3), "plan_create_s": round(t_plan, 3),
+        "ms_per_batch": t_batch * 1e3,
+        "queries_per_s": n_inst / t_batch,
+        "queries_per_s_with_plan": n_inst / (t_batch + t_plan),
+        "rows_per_s": total / t_batch,
+        "bytes_alg": in_b + out_b,
+        "hbm_frac": (in_b + out_b) / t_batch / HBM_PEAK,
+    }
+    if plain:
+        reqs = lambda k, vs: (ca.GroupBy(ca.Table([k])),
+                              [ca.GroupByRequest(vs[0], mk()), ca.GroupByRequest(vs[1], mk())])
+        for k, vs in inst[:8]:  # warm-up
+            gb, rq = reqs(k, vs)
+            gb.aggregate(rq)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k, vs in inst:
+            gb, rq = reqs(k, vs)
+            out = gb.aggregate(rq)
+            del out
+        torch.cuda.synchronize()
+        t_plain = time.perf_counter() - t0
+        res["plain_s"] = t_plain
+        res["plain_queries_per_s"] = n_inst / t_plain
+        res["speedup_vs_plain"] = t_plain / t_batch
+        res["speedup_vs_plain_with_plan"] = t_plain / (t_batch + t_plan)
+    del keep
+    torch.cuda.empty_cache()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -122,6 +224,12 @@ def main():
                     help="force an aggregation strategy (default: chosen per call)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="run one device copy of MIB MiB first (counter calibration)")
+    ap.add_argument("--batched-instances", type=int, default=4096,
+                    help="instances of the batched workload (0 skips it)")
+    ap.add_argument("--batched-reps", type=int, default=3)
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain per-instance loop")
+    ap.add_argument("--only-batched", action="store_true", help="run the batched workload alone")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the group-by is measured on a "
@@ -134,6 +242,10 @@ def main():
     from cudf_amd.aggregation import Aggregation, Kind
 
     torch.cuda.set_device(0)
+    if args.only_batched:
+        print(json.dumps({"batched": bench_batched(args.batched_instances, args.batched_reps,
+                                                   not args.no_autotune, not args.no_plain)}), flush=True)
+        return
     (kcol, fcol, icol), _keep = make_c2(args.rows, args.card)
     aggs = lambda: [Aggregation(Kind.SUM), Aggregation(Kind.MEAN), Aggregation(Kind.COUNT_VALID),
                     Aggregation(Kind.MIN), Aggregation(Kind.MAX), Aggregation(Kind.VARIANCE, 1)]
@@ -222,6 +334,11 @@ def main():
     }
     if not args.no_cpu:
         line.update(cpu_baselines(args.cpu_sample, max(1, args.cpu_sample // 10)))
+    if args.batched_instances > 0:
+        del keys_t, res, gb, reqs, kcol, fcol, icol, _keep
+        torch.cuda.empty_cache()
+        line["batched"] = bench_batched(args.batched_instances, args.batched_reps, not args.no_autotune,
+                                        not args.no_plain)
     print(json.dumps(line), flush=True)
 
 

@@ -5,11 +5,12 @@ the C ABI of ``include/mig_groupby.h``); this package is the Python mirror of
 pylibcudf's ``GroupBy`` / ``aggregation`` API on top of it.
 """
 from . import aggregation
+from .batch import BatchedGroupBy, BatchOptions
 from .column import Column, Table
 from .groupby import GroupBy, GroupByRequest, murmurhash3_x86_32
 from .types import NullPolicy, Sorted, TypeId
 
 __all__ = [
-    "aggregation", "Column", "Table", "GroupBy", "GroupByRequest", "murmurhash3_x86_32",
+    "aggregation", "BatchedGroupBy", "BatchOptions", "Column", "Table", "GroupBy", "GroupByRequest", "murmurhash3_x86_32",
     "NullPolicy", "Sorted", "TypeId",
 ]

@@ -89,6 +89,20 @@ class OutColumn(ctypes.Structure):
     ]
 
 
+class BatchInstance(ctypes.Structure):
+    _fields_ = [("keys", ctypes.POINTER(ColumnView)), ("values", ctypes.POINTER(ColumnView))]
+
+
+class BatchOptions(ctypes.Structure):
+    _fields_ = [
+        ("load_factor", ctypes.c_double),
+        ("lds_max_groups", ctypes.c_int32),
+        ("block_size", ctypes.c_int32),
+        ("tile_rows", ctypes.c_int32),
+        ("use_graph", ctypes.c_int32),
+    ]
+
+
 # every function the header declares, with its ctypes signature
 SIGNATURES = {
     "mig_last_error": (ctypes.c_char_p, []),
@@ -115,6 +129,18 @@ SIGNATURES = {
     "mig_last_stats": (ctypes.c_int32, [ctypes.POINTER(ctypes.c_double), ctypes.c_int32]),
     "mig_set_phase_timing": (None, [ctypes.c_int32]),
     "mig_set_strategy": (None, [ctypes.c_int32]),
+    "mig_batch_plan_create": (ctypes.c_int, [ctypes.POINTER(BatchInstance), ctypes.c_int32, ctypes.c_int32,
+                                             ctypes.c_int32, ctypes.POINTER(Aggregation),
+                                             ctypes.POINTER(ctypes.c_int32), ctypes.c_int32,
+                                             ctypes.POINTER(BatchOptions), ctypes.c_void_p,
+                                             ctypes.POINTER(ctypes.c_void_p)]),
+    "mig_batch_execute": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "mig_batch_synchronize": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "mig_batch_num_groups": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32]),
+    "mig_batch_result_key": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(OutColumn)]),
+    "mig_batch_result_column": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32,
+                                               ctypes.c_int32, ctypes.POINTER(OutColumn)]),
+    "mig_batch_plan_free": (None, [ctypes.c_void_p]),
 }
 
 STAT_NAMES = ["strategy", "key_mode", "capacity", "est_groups", "groups", "retries", "state_bytes",

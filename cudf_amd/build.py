@@ -30,7 +30,8 @@ COMMON_FLAGS = [
     f"-I{os.path.join(ROOT, 'include')}",
 ]
 
-SOURCES = ["groupby_kernels.hip", "partition_kernels.hip", "partition_aggregate.hip", "groupby.cpp", "capi.cpp"]
+SOURCES = ["groupby_kernels.hip", "partition_kernels.hip", "partition_aggregate.hip", "batch_kernels.hip",
+           "groupby.cpp", "batch.cpp", "capi.cpp"]
 
 
 def _compile(src: str) -> str:

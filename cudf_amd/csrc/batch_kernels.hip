@@ -1,0 +1,591 @@
+// Batched hash group-by kernels: many independent group-bys (one fixed-width
+// key each) in one persistent launch.
+//
+//   k_batch_sample     per instance: distinct row hashes of a strided sample
+//   k_batch_init       every table, accumulator, flag and output mask reset
+//                      (a fill kernel: the sequence is captured in a hipGraph,
+//                      where memset nodes are avoided)
+//   k_batch_aggregate  persistent workgroups pull (instance, row tile) items
+//                      from a device queue; low-cardinality instances are
+//                      pre-aggregated per tile in LDS and flushed into their
+//                      table, the others update their table directly
+//   k_batch_count / k_batch_scan / k_batch_outputs
+//                      per-instance compaction into dense output rows
+//
+// Per row the semantics are those of the single group-by
+// (groupby_kernels.hip; reference groupby/hash/groupby_kernels.cuh:98-105).
+#include "batch.hpp"
+#include "device_ops.hpp"
+
+namespace mig {
+
+constexpr int BBLOCK = 256;
+
+__device__ __forceinline__ uint32_t b_key_hash(int32_t st, uint64_t bits)
+{
+  return hash_combine(0u, hash_fixed_bits(st, bits, 0u));
+}
+
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(BBLOCK)
+  k_batch_sample(batch_params const* __restrict__ Bp, int32_t per_inst, uint64_t* tables, uint32_t table_slots,
+                 uint32_t* counters)
+{
+  batch_params const& B = *Bp;
+  int const i           = blockIdx.x;
+  batch_inst const I    = B.inst[i];
+  int64_t const ns      = I.rows < per_inst ? I.rows : per_inst;
+  uint64_t* table       = tables + uint64_t(i) * table_slots;
+  for (int64_t k = threadIdx.x; k < ns; k += BBLOCK) {
+    int64_t const row = (k * I.rows) / ns;
+    uint32_t h;
+    if (!dcol_valid(I.key, row)) {
+      if (B.skip_null_keys) continue;
+      h = hash_combine(0u, NULL_HASH);
+    } else {
+      h = b_key_hash(I.key.storage, load_fixed_bits(I.key, row));
+    }
+    aadd(counters + 2 * i, 1u);
+    uint64_t const tag = (1ull << 32) | h;
+    uint32_t s         = mm_fmix32(h) & (table_slots - 1);
+    for (uint32_t p = 0; p < table_slots; p++) {
+      uint64_t cur = aload(table + s);
+      if (cur == tag) break;
+      if (cur == 0) {
+        uint64_t old = acas64(table + s, 0ull, tag);
+        if (old == 0) {
+          aadd(counters + 2 * i + 1, 1u);
+          break;
+        }
+        if (old == tag) break;
+      }
+      s = (s + 1) & (table_slots - 1);
+    }
+  }
+}
+
+void launch_batch_sample(batch_params const* dB, int32_t n_inst, int32_t per_inst, uint64_t* tables,
+                         uint32_t table_slots, uint32_t* counters, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_batch_sample, dim3(n_inst), dim3(BBLOCK), 0, s, dB, per_inst, tables, table_slots,
+                     counters);
+}
+
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(BBLOCK) k_batch_init(batch_params const* __restrict__ Bp)
+{
+  batch_params const& B = *Bp;
+  int const which       = blockIdx.y;  // 0 keys, 1 row counts, 2 small state, 3 out masks, 4.. fields
+  int64_t const stride  = int64_t(gridDim.x) * BBLOCK;
+  int64_t const i0      = int64_t(blockIdx.x) * BBLOCK + threadIdx.x;
+  uint64_t const total  = B.total_slots;
+  if (which == 0) {
+    for (int64_t j = i0; j < int64_t(total); j += stride) B.slot_keys[j] = EMPTY_KEY;
+  } else if (which == 1) {
+    if (B.slot_count)
+      for (int64_t j = i0; j < int64_t(total); j += stride) B.slot_count[j] = 0;
+  } else if (which == 2) {
+    for (int64_t j = i0; j < int64_t(B.n_inst); j += stride) {
+      B.special[2 * j]     = 0;
+      B.special[2 * j + 1] = 0;
+      B.groups[j]          = 0;
+      B.overflow[j]        = 0;
+    }
+    int64_t const nv = int64_t(B.n_inst) * (B.num_outs + 1);
+    for (int64_t j = i0; j < nv; j += stride) B.valid_counts[j] = 0;
+    if (i0 == 0) B.queue[0] = 0;
+  } else if (which == 3) {
+    int64_t const words = int64_t((B.out_rows + 31) / 32);
+    for (int o = 0; o < B.num_outs; o++)
+      if (B.outs[o].mask)
+        for (int64_t j = i0; j < words; j += stride) B.outs[o].mask[j] = 0;
+    if (B.out_key_mask)
+      for (int64_t j = i0; j < words; j += stride) B.out_key_mask[j] = 0;
+  } else {
+    field_desc const& F = B.fields[which - 4];
+    uint64_t const idv  = field_identity(F.kind);
+    if (F.width == 8) {
+      auto p = reinterpret_cast<uint64_t*>(F.ptr);
+      for (int64_t j = i0; j < int64_t(total); j += stride) p[j] = idv;
+    } else {
+      auto p = reinterpret_cast<uint32_t*>(F.ptr);
+      for (int64_t j = i0; j < int64_t(total); j += stride) p[j] = uint32_t(idv);
+    }
+  }
+}
+
+void launch_batch_init(batch_params const* dB, batch_params const& hB, int num_cus, hipStream_t s)
+{
+  uint64_t const longest = hB.total_slots > hB.out_rows / 32 ? hB.total_slots : hB.out_rows / 32;
+  int64_t blocks         = (int64_t(longest) + BBLOCK - 1) / BBLOCK;
+  if (blocks > int64_t(num_cus) * 8) blocks = int64_t(num_cus) * 8;
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL(k_batch_init, dim3(blocks, 4 + hB.num_fields), dim3(BBLOCK), 0, s, dB);
+}
+
+// ---------------------------------------------------------------------------
+// table of one instance
+__device__ __forceinline__ int64_t b_locate(batch_params const& B, batch_inst const& I, int inst,
+                                            uint64_t key, uint32_t h)
+{
+  if (key == EMPTY_KEY) {
+    uint32_t* f = B.special + 2 * inst;
+    if (aload(f) == 0u) astore(f, 1u);
+    return int64_t(I.slot_base + I.cap);
+  }
+  uint64_t* keys   = B.slot_keys + I.slot_base;
+  uint64_t const m = I.cap - 1;
+  uint64_t s       = h & m;
+  for (uint64_t p = 0; p < I.cap; p++) {
+    uint64_t cur = aload(keys + s);
+    if (cur == key) return int64_t(I.slot_base + s);
+    if (cur == EMPTY_KEY) {
+      uint64_t old = acas64(keys + s, EMPTY_KEY, key);
+      if (old == EMPTY_KEY || old == key) return int64_t(I.slot_base + s);
+    }
+    s = (s + 1) & m;
+  }
+  astore(B.overflow + inst, 1);
+  return -1;
+}
+
+__device__ __forceinline__ int64_t b_null_slot(batch_params const& B, batch_inst const& I, int inst)
+{
+  uint32_t* f = B.special + 2 * inst + 1;
+  if (aload(f) == 0u) astore(f, 1u);
+  return int64_t(I.slot_base + I.cap + 1);
+}
+
+// Combine a pre-aggregated partial (same encoding as the table) into slot.
+__device__ __forceinline__ void combine_field(field_desc const& F, int64_t slot, uint64_t part)
+{
+  switch (F.kind) {
+    case FK_SUM_I64:
+    case FK_SUMSQ_I64: aadd(reinterpret_cast<uint64_t*>(F.ptr) + slot, part); break;
+    case FK_SUM_F32:
+    case FK_SUMSQ_F32:
+      unsafeAtomicAdd(reinterpret_cast<float*>(F.ptr) + slot, __uint_as_float(uint32_t(part)));
+      break;
+    case FK_SUM_F64:
+    case FK_SUMSQ_F64:
+      unsafeAtomicAdd(reinterpret_cast<double*>(F.ptr) + slot, __longlong_as_double(int64_t(part)));
+      break;
+    case FK_MIN_S32: amin(reinterpret_cast<int32_t*>(F.ptr) + slot, int32_t(part)); break;
+    case FK_MAX_S32: amax(reinterpret_cast<int32_t*>(F.ptr) + slot, int32_t(part)); break;
+    case FK_MIN_S64: amin(reinterpret_cast<int64_t*>(F.ptr) + slot, int64_t(part)); break;
+    case FK_MAX_S64: amax(reinterpret_cast<int64_t*>(F.ptr) + slot, int64_t(part)); break;
+    case FK_MIN_U32:
+    case FK_MIN_F32: amin(reinterpret_cast<uint32_t*>(F.ptr) + slot, uint32_t(part)); break;
+    case FK_MAX_U32:
+    case FK_MAX_F32: amax(reinterpret_cast<uint32_t*>(F.ptr) + slot, uint32_t(part)); break;
+    case FK_MIN_U64:
+    case FK_MIN_F64: amin(reinterpret_cast<uint64_t*>(F.ptr) + slot, part); break;
+    case FK_MAX_U64:
+    case FK_MAX_F64: amax(reinterpret_cast<uint64_t*>(F.ptr) + slot, part); break;
+    case FK_COUNT: aadd(reinterpret_cast<uint32_t*>(F.ptr) + slot, uint32_t(part)); break;
+    default: break;
+  }
+}
+
+// The same update into LDS partials (workgroup-scope atomics).
+__device__ __forceinline__ void lds_update(field_desc const& F, uint8_t* lds, int32_t off, int id,
+                                           loaded_value const& v)
+{
+  constexpr int WG = __HIP_MEMORY_SCOPE_WORKGROUP;
+  switch (F.kind) {
+    case FK_SUM_I64: __hip_atomic_fetch_add(lds_at<uint64_t>(lds, off, id), uint64_t(v.i), __ATOMIC_RELAXED, WG); break;
+    case FK_SUM_F32: __hip_atomic_fetch_add(lds_at<float>(lds, off, id), v.f, __ATOMIC_RELAXED, WG); break;
+    case FK_SUM_F64: __hip_atomic_fetch_add(lds_at<double>(lds, off, id), v.d, __ATOMIC_RELAXED, WG); break;
+    case FK_SUMSQ_I64:
+      __hip_atomic_fetch_add(lds_at<uint64_t>(lds, off, id), uint64_t(v.i) * uint64_t(v.i), __ATOMIC_RELAXED, WG);
+      break;
+    case FK_SUMSQ_F32: __hip_atomic_fetch_add(lds_at<float>(lds, off, id), v.f * v.f, __ATOMIC_RELAXED, WG); break;
+    case FK_SUMSQ_F64: __hip_atomic_fetch_add(lds_at<double>(lds, off, id), v.d * v.d, __ATOMIC_RELAXED, WG); break;
+    case FK_MIN_S32: __hip_atomic_fetch_min(lds_at<int32_t>(lds, off, id), int32_t(v.i), __ATOMIC_RELAXED, WG); break;
+    case FK_MAX_S32: __hip_atomic_fetch_max(lds_at<int32_t>(lds, off, id), int32_t(v.i), __ATOMIC_RELAXED, WG); break;
+    case FK_MIN_S64: __hip_atomic_fetch_min(lds_at<int64_t>(lds, off, id), v.i, __ATOMIC_RELAXED, WG); break;
+    case FK_MAX_S64: __hip_atomic_fetch_max(lds_at<int64_t>(lds, off, id), v.i, __ATOMIC_RELAXED, WG); break;
+    case FK_MIN_U32: __hip_atomic_fetch_min(lds_at<uint32_t>(lds, off, id), uint32_t(v.i), __ATOMIC_RELAXED, WG); break;
+    case FK_MAX_U32: __hip_atomic_fetch_max(lds_at<uint32_t>(lds, off, id), uint32_t(v.i), __ATOMIC_RELAXED, WG); break;
+    case FK_MIN_U64: __hip_atomic_fetch_min(lds_at<uint64_t>(lds, off, id), uint64_t(v.i), __ATOMIC_RELAXED, WG); break;
+    case FK_MAX_U64: __hip_atomic_fetch_max(lds_at<uint64_t>(lds, off, id), uint64_t(v.i), __ATOMIC_RELAXED, WG); break;
+    case FK_MIN_F32:
+      if (!isnan(v.f))
+        __hip_atomic_fetch_min(lds_at<uint32_t>(lds, off, id), ord32(uint32_t(v.b)), __ATOMIC_RELAXED, WG);
+      break;
+    case FK_MAX_F32:
+      if (!isnan(v.f))
+        __hip_atomic_fetch_max(lds_at<uint32_t>(lds, off, id), ord32(uint32_t(v.b)), __ATOMIC_RELAXED, WG);
+      break;
+    case FK_MIN_F64:
+      if (!isnan(v.d)) __hip_atomic_fetch_min(lds_at<uint64_t>(lds, off, id), ord64(v.b), __ATOMIC_RELAXED, WG);
+      break;
+    case FK_MAX_F64:
+      if (!isnan(v.d)) __hip_atomic_fetch_max(lds_at<uint64_t>(lds, off, id), ord64(v.b), __ATOMIC_RELAXED, WG);
+      break;
+    case FK_COUNT: __hip_atomic_fetch_add(lds_at<uint32_t>(lds, off, id), 1u, __ATOMIC_RELAXED, WG); break;
+    default: break;
+  }
+}
+
+__device__ __forceinline__ void b_update_global(batch_params const& B, int inst, int64_t slot, int64_t row)
+{
+  if (B.need_count_all) aadd(B.slot_count + slot, 1u);
+  for (int v = 0; v < B.num_vals; v++) {
+    int const fb = B.val_field_begin[v], fe = B.val_field_begin[v + 1];
+    if (fb == fe) continue;
+    dcol const& c = B.vals[int64_t(inst) * B.num_vals + v];
+    if (!dcol_valid(c, row)) continue;
+    loaded_value const val = load_value(c, row);
+    for (int f = fb; f < fe; f++) apply_field(B.fields[f], slot, val);
+  }
+}
+
+template <int NT>
+__global__ void __launch_bounds__(NT) k_batch_aggregate(batch_params const* __restrict__ Bp)
+{
+  extern __shared__ __attribute__((aligned(16))) uint8_t lds[];
+  __shared__ uint64_t item_sh;
+  __shared__ uint32_t fill_sh, present_sh[2];
+  batch_params const& B = *Bp;
+  constexpr int S       = BATCH_LDS_SLOTS;
+  constexpr int WG      = __HIP_MEMORY_SCOPE_WORKGROUP;
+  auto* tkeys           = reinterpret_cast<uint64_t*>(lds);
+  for (;;) {
+    if (threadIdx.x == 0) {
+      uint32_t it = __hip_atomic_fetch_add(B.queue, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      item_sh     = it < uint64_t(B.n_items) ? B.items[it] : ~0ull;
+    }
+    __syncthreads();
+    uint64_t const item = item_sh;
+    if (item == ~0ull) break;
+    int const inst     = int(item >> 40);
+    int64_t const r0   = int64_t(item & ((1ull << 40) - 1));
+    batch_inst const I = B.inst[inst];
+    int64_t const r1   = (r0 + B.tile_rows < I.rows) ? r0 + B.tile_rows : I.rows;
+
+    if (!I.lds) {
+      for (int64_t row = r0 + threadIdx.x; row < r1; row += NT) {
+        int64_t slot;
+        if (!dcol_valid(I.key, row)) {
+          if (B.skip_null_keys) continue;
+          slot = b_null_slot(B, I, inst);
+        } else {
+          uint64_t const bits = load_fixed_bits(I.key, row);
+          slot                = b_locate(B, I, inst, bits, b_key_hash(I.key.storage, bits));
+        }
+        if (slot >= 0) b_update_global(B, inst, slot, row);
+      }
+      __syncthreads();
+      continue;
+    }
+
+    // ---- LDS pre-aggregation of the tile (slots S, S+1: EMPTY_KEY, null key)
+    for (int i = threadIdx.x; i < S; i += NT) tkeys[i] = EMPTY_KEY;
+    for (int f = 0; f < B.num_fields; f++) {
+      uint64_t const idv = field_identity(B.fields[f].kind);
+      int32_t const off  = B.lds_field_off[f];
+      if (B.fields[f].width == 8)
+        for (int i = threadIdx.x; i < S + 2; i += NT) *lds_at<uint64_t>(lds, off, i) = idv;
+      else
+        for (int i = threadIdx.x; i < S + 2; i += NT) *lds_at<uint32_t>(lds, off, i) = uint32_t(idv);
+    }
+    if (B.lds_count_off >= 0)
+      for (int i = threadIdx.x; i < S + 2; i += NT) *lds_at<uint32_t>(lds, B.lds_count_off, i) = 0;
+    if (threadIdx.x == 0) {
+      fill_sh       = 0;
+      present_sh[0] = 0;
+      present_sh[1] = 0;
+    }
+    __syncthreads();
+    for (int64_t row = r0 + threadIdx.x; row < r1; row += NT) {
+      int id = -1;
+      uint64_t bits = 0;
+      uint32_t h    = 0;
+      bool const kv = dcol_valid(I.key, row);
+      if (!kv) {
+        if (B.skip_null_keys) continue;
+        id = S + 1;
+        present_sh[1] = 1;
+      } else {
+        bits = load_fixed_bits(I.key, row);
+        h    = b_key_hash(I.key.storage, bits);
+        if (bits == EMPTY_KEY) {
+          id = S;
+          present_sh[0] = 1;
+        } else {
+          uint32_t s = h & uint32_t(S - 1);
+          for (int p = 0; p < S; p++) {
+            uint64_t cur = __hip_atomic_load(tkeys + s, __ATOMIC_RELAXED, WG);
+            if (cur == bits) { id = int(s); break; }
+            if (cur == EMPTY_KEY) {
+              if (__hip_atomic_load(&fill_sh, __ATOMIC_RELAXED, WG) >= uint32_t(BATCH_LDS_FILL)) break;
+              uint64_t expected = EMPTY_KEY;
+              if (__hip_atomic_compare_exchange_strong(tkeys + s, &expected, bits, __ATOMIC_RELAXED,
+                                                       __ATOMIC_RELAXED, WG)) {
+                __hip_atomic_fetch_add(&fill_sh, 1u, __ATOMIC_RELAXED, WG);
+                id = int(s);
+                break;
+              }
+              if (expected == bits) { id = int(s); break; }
+            }
+            s = (s + 1) & uint32_t(S - 1);
+          }
+        }
+      }
+      if (id < 0) {
+        // the LDS table is full: this row goes straight to the instance table
+        int64_t const slot = b_locate(B, I, inst, bits, h);
+        if (slot >= 0) b_update_global(B, inst, slot, row);
+        continue;
+      }
+      if (B.lds_count_off >= 0)
+        __hip_atomic_fetch_add(lds_at<uint32_t>(lds, B.lds_count_off, id), 1u, __ATOMIC_RELAXED, WG);
+      for (int v = 0; v < B.num_vals; v++) {
+        int const fb = B.val_field_begin[v], fe = B.val_field_begin[v + 1];
+        if (fb == fe) continue;
+        dcol const& c = B.vals[int64_t(inst) * B.num_vals + v];
+        if (!dcol_valid(c, row)) continue;
+        loaded_value const val = load_value(c, row);
+        for (int f = fb; f < fe; f++) lds_update(B.fields[f], lds, B.lds_field_off[f], id, val);
+      }
+    }
+    __syncthreads();
+    // ---- flush the tile's groups into the instance table
+    for (int i = threadIdx.x; i < S + 2; i += NT) {
+      int64_t slot;
+      if (i < S) {
+        uint64_t const k = tkeys[i];
+        if (k == EMPTY_KEY) continue;
+        slot = b_locate(B, I, inst, k, b_key_hash(I.key.storage, k));
+      } else if (i == S) {
+        if (!present_sh[0]) continue;
+        slot = b_locate(B, I, inst, EMPTY_KEY, 0);
+      } else {
+        if (!present_sh[1]) continue;
+        slot = b_null_slot(B, I, inst);
+      }
+      if (slot < 0) continue;
+      if (B.lds_count_off >= 0) aadd(B.slot_count + slot, *lds_at<uint32_t>(lds, B.lds_count_off, i));
+      for (int f = 0; f < B.num_fields; f++) {
+        field_desc const& F = B.fields[f];
+        uint64_t const part = F.width == 8 ? *lds_at<uint64_t>(lds, B.lds_field_off[f], i)
+                                           : *lds_at<uint32_t>(lds, B.lds_field_off[f], i);
+        uint64_t const idv  = F.width == 8 ? field_identity(F.kind) : uint32_t(field_identity(F.kind));
+        // an identity partial leaves the accumulator as it is (the float sums
+        // start at +0.0, which adding +0.0 keeps)
+        if (part != idv) combine_field(F, slot, part);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+void launch_batch_aggregate(batch_params const* dB, batch_params const& hB, int num_cus, int block,
+                            hipStream_t s)
+{
+  int64_t blocks = int64_t(num_cus) * (block == 512 ? 2 : 4);
+  if (blocks > hB.n_items) blocks = hB.n_items;
+  if (blocks < 1) blocks = 1;
+  if (block == 512)
+    hipLaunchKernelGGL(k_batch_aggregate<512>, dim3(blocks), dim3(512), hB.lds_bytes, s, dB);
+  else
+    hipLaunchKernelGGL(k_batch_aggregate<256>, dim3(blocks), dim3(256), hB.lds_bytes, s, dB);
+}
+
+// ---------------------------------------------------------------------------
+// compaction: occupied slots of each instance, in slot order
+__device__ __forceinline__ bool b_occupied(batch_params const& B, batch_inst const& I, int inst, uint64_t local)
+{
+  if (local < I.cap) return B.slot_keys[I.slot_base + local] != EMPTY_KEY;
+  return B.special[2 * inst + (local - I.cap)] != 0u;
+}
+
+__global__ void __launch_bounds__(BBLOCK) k_batch_count(batch_params const* __restrict__ Bp)
+{
+  __shared__ uint32_t red[BBLOCK / 64];
+  batch_params const& B = *Bp;
+  uint64_t const t      = B.slot_tiles[blockIdx.x];
+  int const inst        = int(t >> 40);
+  uint64_t const l0     = t & ((1ull << 40) - 1);
+  batch_inst const I    = B.inst[inst];
+  uint32_t c            = 0;
+  for (int k = threadIdx.x; k < SCAN_TILE; k += BBLOCK) {
+    uint64_t const l = l0 + k;
+    if (l < I.cap + 2 && b_occupied(B, I, inst, l)) c++;
+  }
+  c = block_sum_u32<BBLOCK>(c, red);
+  if (threadIdx.x == 0) B.tile_offsets[blockIdx.x] = c;
+}
+
+// One block per instance: exclusive scan of its tiles' counts (BBLOCK tiles
+// per step: wave64 prefix sums, then across the block's waves), total groups.
+__global__ void __launch_bounds__(BBLOCK) k_batch_scan(batch_params const* __restrict__ Bp)
+{
+  __shared__ uint32_t wave_tot[BBLOCK / 64];
+  batch_params const& B = *Bp;
+  int const inst        = blockIdx.x;
+  int const wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  int64_t const t0 = B.inst_tile_begin[inst], t1 = B.inst_tile_begin[inst + 1];
+  uint32_t run = 0;
+  for (int64_t base = t0; base < t1; base += BBLOCK) {
+    int64_t const t  = base + threadIdx.x;
+    uint32_t const c = t < t1 ? B.tile_offsets[t] : 0u;
+    uint32_t incl    = c;
+    for (int o = 1; o < 64; o <<= 1) {
+      uint32_t const up = __shfl_up(incl, o, 64);
+      if (lane >= o) incl += up;
+    }
+    if (lane == 63) wave_tot[wave] = incl;
+    __syncthreads();
+    uint32_t before = 0, all = 0;
+    for (int w = 0; w < BBLOCK / 64; w++) {
+      if (w < wave) before += wave_tot[w];
+      all += wave_tot[w];
+    }
+    if (t < t1) B.tile_offsets[t] = run + before + incl - c;
+    run += all;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) B.groups[inst] = run;
+}
+
+__global__ void __launch_bounds__(BBLOCK) k_batch_outputs(batch_params const* __restrict__ Bp)
+{
+  __shared__ uint32_t wave_tot[BBLOCK / 64];
+  batch_params const& B = *Bp;
+  uint64_t const t      = B.slot_tiles[blockIdx.x];
+  int const inst        = int(t >> 40);
+  uint64_t const l0     = t & ((1ull << 40) - 1);
+  batch_inst const I    = B.inst[inst];
+  uint32_t running      = B.tile_offsets[blockIdx.x];
+  int const wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  int const kw = storage_width(B.key_storage);
+  for (int k = 0; k < SCAN_TILE; k += BBLOCK) {
+    uint64_t const l   = l0 + k + threadIdx.x;
+    bool const occ     = l < I.cap + 2 && b_occupied(B, I, inst, l);
+    uint64_t const bal = __ballot(occ);
+    uint32_t const below = __popcll(bal & ((1ull << lane) - 1ull));
+    if (lane == 0) wave_tot[wave] = __popcll(bal);
+    __syncthreads();
+    uint32_t before = 0, all = 0;
+    for (int w = 0; w < BBLOCK / 64; w++) {
+      if (w < wave) before += wave_tot[w];
+      all += wave_tot[w];
+    }
+    if (occ) {
+      uint64_t const pos  = I.out_base + running + before + below;
+      int64_t const slot  = int64_t(I.slot_base + l);
+      bool const kvalid   = l != I.cap + 1;
+      uint64_t const key  = l < I.cap ? B.slot_keys[slot] : (l == I.cap ? EMPTY_KEY : 0ull);
+      store_width(B.out_key, int64_t(pos), kw, kvalid ? key : 0ull);
+      if (B.out_key_mask && kvalid)
+        __hip_atomic_fetch_or(B.out_key_mask + (pos >> 5), 1u << (pos & 31), __ATOMIC_RELAXED,
+                              __HIP_MEMORY_SCOPE_AGENT);
+      for (int o = 0; o < B.num_outs; o++) {
+        out_desc const& O = B.outs[o];
+        bool valid        = true;
+        uint32_t cnt      = 0;
+        if (O.count_field != -1) {
+          cnt   = O.count_field == -2 ? B.slot_count[slot]
+                                      : reinterpret_cast<uint32_t const*>(B.fields[O.count_field].ptr)[slot];
+          valid = cnt > 0;
+        }
+        switch (O.kind) {
+          case OK_COPY_I64:
+          case OK_COPY_F64:
+            reinterpret_cast<uint64_t*>(O.data)[pos] = reinterpret_cast<uint64_t const*>(B.fields[O.field].ptr)[slot];
+            break;
+          case OK_COPY_F32:
+            reinterpret_cast<uint32_t*>(O.data)[pos] = reinterpret_cast<uint32_t const*>(B.fields[O.field].ptr)[slot];
+            break;
+          case OK_COUNT:
+            reinterpret_cast<int32_t*>(O.data)[pos] =
+              int32_t(reinterpret_cast<uint32_t const*>(B.fields[O.field].ptr)[slot]);
+            valid = true;
+            break;
+          case OK_COUNT_ALL:
+            reinterpret_cast<int32_t*>(O.data)[pos] = int32_t(B.slot_count[slot]);
+            valid = true;
+            break;
+          case OK_MINMAX: {
+            field_desc const& F = B.fields[O.field];
+            uint64_t bits = F.width == 8 ? reinterpret_cast<uint64_t const*>(F.ptr)[slot]
+                                         : reinterpret_cast<uint32_t const*>(F.ptr)[slot];
+            if (F.kind == FK_MIN_F32 || F.kind == FK_MAX_F32) bits = unord32(uint32_t(bits));
+            else if (F.kind == FK_MIN_F64 || F.kind == FK_MAX_F64) bits = unord64(bits);
+            store_width(O.data, int64_t(pos), O.out_width, bits);
+            break;
+          }
+          case OK_MEAN: {
+            field_desc const& F = B.fields[O.field];
+            double sum          = 0.0;
+            if (F.kind == FK_SUM_I64) sum = double(reinterpret_cast<int64_t const*>(F.ptr)[slot]);
+            else if (F.kind == FK_SUM_F32) sum = double(reinterpret_cast<float const*>(F.ptr)[slot]);
+            else sum = reinterpret_cast<double const*>(F.ptr)[slot];
+            reinterpret_cast<double*>(O.data)[pos] = valid ? sum / double(cnt) : 0.0;
+            break;
+          }
+          default: break;
+        }
+        if (O.mask && valid)
+          __hip_atomic_fetch_or(O.mask + (pos >> 5), 1u << (pos & 31), __ATOMIC_RELAXED,
+                                __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
+    running += all;
+    __syncthreads();
+  }
+}
+
+void launch_batch_count(batch_params const* dB, batch_params const& hB, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_batch_count, dim3(hB.n_slot_tiles), dim3(BBLOCK), 0, s, dB);
+}
+void launch_batch_scan(batch_params const* dB, batch_params const& hB, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_batch_scan, dim3(hB.n_inst), dim3(BBLOCK), 0, s, dB);
+}
+void launch_batch_outputs(batch_params const* dB, batch_params const& hB, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_batch_outputs, dim3(hB.n_slot_tiles), dim3(BBLOCK), 0, s, dB);
+}
+
+// Set bits of each instance's region of each output mask (grid: instances x
+// (masked outputs + key)); the host turns them into null counts.
+__global__ void __launch_bounds__(BBLOCK) k_batch_valid_counts(batch_params const* __restrict__ Bp)
+{
+  __shared__ uint32_t red[BBLOCK / 64];
+  batch_params const& B = *Bp;
+  int const inst        = blockIdx.x;
+  int const o           = blockIdx.y;
+  uint32_t const* mask  = o < B.num_outs ? B.outs[o].mask : B.out_key_mask;
+  if (mask == nullptr) return;
+  uint64_t const base = B.inst[inst].out_base;  // multiple of 64
+  uint32_t const G    = B.groups[inst];
+  uint32_t c          = 0;
+  for (uint32_t w = threadIdx.x; w < (G + 31) / 32; w += BBLOCK) {
+    uint32_t bits = mask[base / 32 + w];
+    if (w == G / 32) bits &= (1u << (G & 31)) - 1u;
+    c += __popc(bits);
+  }
+  c = block_sum_u32<BBLOCK>(c, red);
+  if (threadIdx.x == 0) B.valid_counts[int64_t(inst) * (B.num_outs + 1) + o] = c;
+}
+
+void launch_batch_valid_counts(batch_params const* dB, batch_params const& hB, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_batch_valid_counts, dim3(hB.n_inst, hB.num_outs + 1), dim3(BBLOCK), 0, s, dB);
+}
+
+void prepare_batch_aggregate(int32_t lds_bytes)
+{
+  if (lds_bytes > 65536) {
+    (void)hipFuncSetAttribute(reinterpret_cast<void const*>(&k_batch_aggregate<256>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, lds_bytes);
+    (void)hipFuncSetAttribute(reinterpret_cast<void const*>(&k_batch_aggregate<512>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, lds_bytes);
+  }
+}
+
+}  // namespace mig

@@ -7,6 +7,7 @@
 #include <new>
 #include <string>
 
+#include "batch_host.hpp"
 #include "groupby.hpp"
 
 namespace {
@@ -177,5 +178,53 @@ void mig_set_strategy(int32_t strategy)
 {
   if (strategy >= MIG_STRATEGY_AUTO && strategy <= MIG_STRATEGY_PARTITIONED) mig::set_strategy(strategy);
 }
+
+mig_status mig_batch_plan_create(const mig_batch_instance* instances, int32_t num_instances,
+                                 int32_t num_keys, int32_t null_policy,
+                                 const mig_aggregation* aggregations, const int32_t* num_aggregations,
+                                 int32_t num_requests, const mig_batch_options* options, void* stream,
+                                 mig_batch_plan** out)
+{
+  return guarded([&] {
+    if (!out) mig::fail(MIG_ERROR_LOGIC, "null output handle");
+    *out = nullptr;
+    *out = mig::batch_plan_create(instances, num_instances, num_keys, null_policy, aggregations,
+                                  num_aggregations, num_requests, options, static_cast<hipStream_t>(stream));
+  });
+}
+
+mig_status mig_batch_execute(mig_batch_plan* plan, void* stream)
+{
+  return guarded([&] { mig::batch_execute(plan, static_cast<hipStream_t>(stream)); });
+}
+
+mig_status mig_batch_synchronize(mig_batch_plan* plan, void* stream)
+{
+  return guarded([&] { mig::batch_synchronize(plan, static_cast<hipStream_t>(stream)); });
+}
+
+int32_t mig_batch_num_groups(const mig_batch_plan* plan, int32_t instance)
+{
+  return mig::batch_num_groups(plan, instance);
+}
+
+mig_status mig_batch_result_key(const mig_batch_plan* plan, int32_t instance, mig_column* out)
+{
+  return guarded([&] {
+    if (!out) mig::fail(MIG_ERROR_LOGIC, "null output column");
+    *out = mig::batch_result_key(plan, instance);
+  });
+}
+
+mig_status mig_batch_result_column(const mig_batch_plan* plan, int32_t instance, int32_t request,
+                                   int32_t agg, mig_column* out)
+{
+  return guarded([&] {
+    if (!out) mig::fail(MIG_ERROR_LOGIC, "null output column");
+    *out = mig::batch_result_column(plan, instance, request, agg);
+  });
+}
+
+void mig_batch_plan_free(mig_batch_plan* plan) { mig::batch_plan_free(plan); }
 
 }  // extern "C"

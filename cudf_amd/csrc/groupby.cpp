@@ -22,6 +22,7 @@
 #include <map>
 #include <unordered_map>
 
+#include "host_plan.hpp"
 #include "kernels.hpp"
 
 namespace mig {
@@ -234,6 +235,10 @@ pinned_host& thread_pinned()
   return h;
 }
 
+}  // namespace
+
+// Planning helpers shared with the batched driver (declared in host_plan.hpp).
+
 bool same_view(mig_column_view const& a, mig_column_view const& b)
 {
   return a.type_id == b.type_id && a.size == b.size && a.offset == b.offset && a.data == b.data &&
@@ -419,6 +424,8 @@ double estimate_distinct(double n, double d, double population)
   }
   return hi;
 }
+
+namespace {
 
 constexpr int64_t SAMPLE_ROWS = 65536;
 

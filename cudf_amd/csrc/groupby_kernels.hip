@@ -209,41 +209,7 @@ __device__ __forceinline__ int64_t locate(gb_params const& P, int64_t row)
 }
 
 // ---------------------------------------------------------------------------
-// accumulate one row into the group's accumulators
-
-__device__ __forceinline__ void apply_field(field_desc const& F, int64_t slot, loaded_value const& v)
-{
-  switch (F.kind) {
-    case FK_SUM_I64: aadd(reinterpret_cast<uint64_t*>(F.ptr) + slot, uint64_t(v.i)); break;
-    case FK_SUM_F32: unsafeAtomicAdd(reinterpret_cast<float*>(F.ptr) + slot, v.f); break;
-    case FK_SUM_F64: unsafeAtomicAdd(reinterpret_cast<double*>(F.ptr) + slot, v.d); break;
-    case FK_SUMSQ_I64: aadd(reinterpret_cast<uint64_t*>(F.ptr) + slot, uint64_t(v.i) * uint64_t(v.i)); break;
-    case FK_SUMSQ_F32: unsafeAtomicAdd(reinterpret_cast<float*>(F.ptr) + slot, v.f * v.f); break;
-    case FK_SUMSQ_F64: unsafeAtomicAdd(reinterpret_cast<double*>(F.ptr) + slot, v.d * v.d); break;
-    case FK_MIN_S32: amin(reinterpret_cast<int32_t*>(F.ptr) + slot, int32_t(v.i)); break;
-    case FK_MAX_S32: amax(reinterpret_cast<int32_t*>(F.ptr) + slot, int32_t(v.i)); break;
-    case FK_MIN_S64: amin(reinterpret_cast<int64_t*>(F.ptr) + slot, v.i); break;
-    case FK_MAX_S64: amax(reinterpret_cast<int64_t*>(F.ptr) + slot, v.i); break;
-    case FK_MIN_U32: amin(reinterpret_cast<uint32_t*>(F.ptr) + slot, uint32_t(v.i)); break;
-    case FK_MAX_U32: amax(reinterpret_cast<uint32_t*>(F.ptr) + slot, uint32_t(v.i)); break;
-    case FK_MIN_U64: amin(reinterpret_cast<uint64_t*>(F.ptr) + slot, uint64_t(v.i)); break;
-    case FK_MAX_U64: amax(reinterpret_cast<uint64_t*>(F.ptr) + slot, uint64_t(v.i)); break;
-    case FK_MIN_F32:
-      if (!isnan(v.f)) amin(reinterpret_cast<uint32_t*>(F.ptr) + slot, ord32(uint32_t(v.b)));
-      break;
-    case FK_MAX_F32:
-      if (!isnan(v.f)) amax(reinterpret_cast<uint32_t*>(F.ptr) + slot, ord32(uint32_t(v.b)));
-      break;
-    case FK_MIN_F64:
-      if (!isnan(v.d)) amin(reinterpret_cast<uint64_t*>(F.ptr) + slot, ord64(v.b));
-      break;
-    case FK_MAX_F64:
-      if (!isnan(v.d)) amax(reinterpret_cast<uint64_t*>(F.ptr) + slot, ord64(v.b));
-      break;
-    case FK_COUNT: aadd(reinterpret_cast<uint32_t*>(F.ptr) + slot, 1u); break;
-    default: break;  // FK_M2: second pass
-  }
-}
+// accumulate one row into the group's accumulators (apply_field: device_ops.hpp)
 
 __device__ __forceinline__ void update_row(gb_params const& P, int64_t slot, int64_t row)
 {

@@ -1,0 +1,29 @@
+// Host-side planning helpers shared by the single-call driver (groupby.cpp)
+// and the batched driver (batch.cpp).
+#pragma once
+
+#include "common.hpp"
+
+namespace mig {
+
+bool same_view(mig_column_view const& a, mig_column_view const& b);
+// Nulls of a view (its null_count when known, else counted on the device;
+// synchronises `s` in that case).
+int32_t null_count_of(mig_column_view const& v, hipStream_t s);
+dcol make_dcol(mig_column_view const& v, bool use_mask);
+bool is_numeric_type(int32_t t);
+bool is_chrono_type(int32_t t);
+char const* kind_name(int32_t k);
+// Result type of (source type, aggregation): aggregation.hpp:1211-1331.
+int32_t target_type_of(int32_t src, int32_t kind);
+// Throws MIG_ERROR_LOGIC / MIG_ERROR_DATA_TYPE for unsupported combinations.
+void check_supported(int32_t src, int32_t kind);
+int32_t sum_field_kind(int32_t st);
+int32_t sumsq_field_kind(int32_t st);
+int32_t minmax_field_kind(int32_t st, bool is_min);
+int32_t field_width(int32_t fk);
+uint64_t next_pow2(uint64_t x);
+// Distinct values of a population from `d` distinct among `n` samples.
+double estimate_distinct(double n, double d, double population);
+
+}  // namespace mig

@@ -156,6 +156,58 @@ typedef struct mig_column {
 
 typedef struct mig_groupby_result mig_groupby_result;
 
+/* ---- batched group-by -------------------------------------------------------
+ * Many independent group-bys over different inputs with the same key type,
+ * the same value types and the same aggregation requests, executed together:
+ * one persistent work-queue kernel aggregates every instance, captured with
+ * the kernels around it into one hipGraph that each execute replays.  No
+ * counterpart in the reference, which runs one aggregate() per table
+ * (cpp/src/groupby/groupby.cu:204-221); each instance's result equals what
+ * mig_groupby_aggregate returns for it. */
+typedef struct mig_batch_instance {
+  const mig_column_view* keys;   /* num_keys key columns (one fixed-width key) */
+  const mig_column_view* values; /* one values column per request             */
+} mig_batch_instance;
+
+/* Tuning knobs (0 = automatic); chosen by the autotuner in cudf_amd.autotune. */
+typedef struct mig_batch_options {
+  double load_factor;        /* target groups per table slot (default 0.5)      */
+  int32_t lds_max_groups;    /* LDS pre-aggregation for instances estimated at
+                                or below this many groups (default 512; -1 off) */
+  int32_t block_size;        /* 256 or 512 (default 256)                        */
+  int32_t tile_rows;         /* rows per work item (default 4096)               */
+  int32_t use_graph;         /* -1 = launch the kernels one by one instead of
+                                replaying the captured hipGraph (default on)    */
+} mig_batch_options;
+
+typedef struct mig_batch_plan mig_batch_plan;
+
+/* Builds a plan: estimates every instance's group count on the device, sizes
+ * tables and output regions, and (by default) captures the execution graph.
+ * The instance columns must stay valid and unchanged while the plan is used. */
+mig_status mig_batch_plan_create(const mig_batch_instance* instances,
+                                 int32_t num_instances,
+                                 int32_t num_keys,
+                                 int32_t null_policy,
+                                 const mig_aggregation* aggregations,
+                                 const int32_t* num_aggregations,
+                                 int32_t num_requests,
+                                 const mig_batch_options* options,
+                                 void* stream,
+                                 mig_batch_plan** out);
+/* Runs every instance on `stream` (enqueued; use mig_batch_synchronize or the
+ * stream to wait). */
+mig_status mig_batch_execute(mig_batch_plan* plan, void* stream);
+/* Waits for the last execute and reads back the per-instance group counts. */
+mig_status mig_batch_synchronize(mig_batch_plan* plan, void* stream);
+int32_t mig_batch_num_groups(const mig_batch_plan* plan, int32_t instance);
+/* Borrowed views of an instance's results, valid until the next execute or
+ * mig_batch_plan_free. */
+mig_status mig_batch_result_key(const mig_batch_plan* plan, int32_t instance, mig_column* out);
+mig_status mig_batch_result_column(const mig_batch_plan* plan, int32_t instance, int32_t request,
+                                   int32_t agg, mig_column* out);
+void mig_batch_plan_free(mig_batch_plan* plan);
+
 /* ---- library -------------------------------------------------------------- */
 
 /* Message for the last failed call on this thread ("" if none). */
