@@ -573,7 +573,8 @@ mig_column batch_result_key(mig_batch_plan const* plan, int32_t instance)
   c.size       = int32_t(G);
   c.data       = static_cast<uint8_t*>(p.P.out_key) + I.out_base * uint64_t(kw);
   c.data_bytes = int64_t(capr * uint64_t(kw));
-  if (p.key_views[size_t(instance)].null_mask != nullptr) {
+  // an empty instance returns empty columns without masks, as the single call does
+  if (p.key_views[size_t(instance)].null_mask != nullptr && I.rows > 0) {
     c.null_mask  = p.P.out_key_mask + I.out_base / 32;
     c.mask_bytes = int64_t(capr / 8);
     c.null_count = int32_t(G - p.valid[size_t(instance) * size_t(p.P.num_outs + 1) + size_t(p.P.num_outs)]);
@@ -601,7 +602,7 @@ mig_column batch_result_column(mig_batch_plan const* plan, int32_t instance, int
   c.size       = int32_t(G);
   c.data       = static_cast<uint8_t*>(p.P.outs[o].data) + I.out_base * uint64_t(w);
   c.data_bytes = int64_t(capr * uint64_t(w));
-  if (oi.mask_if_nulls && p.val_nulls[size_t(instance) * size_t(p.num_vals) + size_t(oi.vcol)]) {
+  if (oi.mask_if_nulls && I.rows > 0 && p.val_nulls[size_t(instance) * size_t(p.num_vals) + size_t(oi.vcol)]) {
     c.null_mask  = p.P.outs[o].mask + I.out_base / 32;
     c.mask_bytes = int64_t(capr / 8);
     c.null_count = int32_t(G - p.valid[size_t(instance) * size_t(p.P.num_outs + 1) + size_t(o)]);
