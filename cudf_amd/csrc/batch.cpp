@@ -1,13 +1,14 @@
 // Host driver of the batched group-by: many independent single-key group-bys
-// (same key type, value types and requests; any row counts and
-// cardinalities) sized from one device-side sample pass, laid out in
-// concatenated tables and output columns, and executed by one persistent
-// work-queue kernel plus a few compaction kernels captured in a hipGraph.
+// (same requests and aggregations; any row counts, cardinalities, key types
+// and value types) sized from one device-side sample pass, laid out in
+// concatenated tables and output regions, and executed by persistent
+// work-queue kernels plus a few compaction kernels captured in a hipGraph.
 //
 // Each instance's result is what the single-call path returns for it
 // (groupby.cpp): same result types, nullability (a SUM/MIN/MAX/MEAN result
-// has a mask iff that instance's values have nulls, the key result iff its
-// key view has a mask) and group set.  VARIANCE/STD are not offered here.
+// has a mask iff that instance's values have nulls, VARIANCE/STD always, the
+// key result iff its key view has a mask; an empty instance returns columns
+// without masks, groupby.cu:150-172) and group set.
 #include "batch.hpp"
 
 #include <algorithm>
@@ -21,16 +22,16 @@
 #include "host_plan.hpp"
 
 struct mig_batch_plan {
+  enum mask_mode : int32_t { MASK_NONE = 0, MASK_IF_NULLS = 1, MASK_ALWAYS = 2 };
   struct out_info {
-    int32_t type_id;
+    int32_t agg_kind;  // MIG_AGG_*
     int32_t vcol;
-    bool mask_if_nulls;  // exposed mask iff the instance's values have nulls
+    int32_t mask;      // mask_mode
   };
 
   int dev     = 0;
   int num_cus = 256;
   int32_t n_inst = 0, null_policy = 0, num_requests = 0, num_vals = 0;
-  int32_t key_type = 0;
   mig_batch_options opt{};
   std::vector<mig_column_view> key_views;          // [n_inst]
   std::vector<mig_column_view> val_views;          // [n_inst * num_vals]
@@ -38,6 +39,8 @@ struct mig_batch_plan {
   std::vector<int32_t> agg_begin;                  // [num_requests + 1]
   std::vector<int32_t> out_of_agg;                 // flattened (request, agg) -> output
   std::vector<out_info> outs;
+  std::vector<int32_t> kinds;                      // [n_inst * num_fields]
+  std::vector<int32_t> widths;                     // [n_inst * (num_outs + 1)]
   std::vector<double> est;                         // estimated groups per instance
   std::vector<uint64_t> cap;                       // table slots per instance
   std::vector<mig::batch_inst> hinst;
@@ -48,7 +51,6 @@ struct mig_batch_plan {
   hipGraph_t graph           = nullptr;
   hipGraphExec_t exec        = nullptr;
   uint32_t* pinned           = nullptr;  // groups | overflow | valid counts
-  size_t pinned_words        = 0;
   std::vector<uint32_t> groups, valid;
   bool executed = false, synced = false;
   int32_t retries = 0;
@@ -80,6 +82,11 @@ struct mig_batch_plan {
     blocks.push_back(p);
     return static_cast<T*>(p);
   }
+  uint64_t region_rows(int32_t i) const
+  {
+    mig::batch_inst const& I = hinst[size_t(i)];
+    return (std::min<uint64_t>(I.cap + 2, uint64_t(I.rows)) + 63) & ~uint64_t(63);
+  }
 };
 
 namespace mig {
@@ -102,6 +109,7 @@ void enqueue(mig_batch_plan& p, hipStream_t s)
 {
   launch_batch_init(p.dP, p.P, p.num_cus, s);
   launch_batch_aggregate(p.dP, p.P, p.num_cus, p.opt.block_size, s);
+  launch_batch_second_pass(p.dP, p.P, p.num_cus, p.opt.block_size, s);
   launch_batch_count(p.dP, p.P, s);
   launch_batch_scan(p.dP, p.P, s);
   launch_batch_outputs(p.dP, p.P, s);
@@ -110,12 +118,11 @@ void enqueue(mig_batch_plan& p, hipStream_t s)
 }
 
 // Tables, outputs, work lists and the parameter block for the current `cap`.
-void layout(mig_batch_plan& p, hipStream_t stream)
+void layout(mig_batch_plan& p)
 {
   p.release_layout();
   batch_params& P = p.P;
   int32_t const n = p.n_inst;
-  int32_t const kw = type_width(p.key_type);
 
   uint64_t total_slots = 0, out_rows = 0;
   int64_t n_items = 0, n_tiles = 0;
@@ -125,9 +132,8 @@ void layout(mig_batch_plan& p, hipStream_t stream)
     I.cap         = p.cap[i];
     I.slot_base   = total_slots;
     total_slots += I.cap + 2;
-    I.out_base  = out_rows;
-    uint64_t const out_cap = std::min<uint64_t>(I.cap + 2, uint64_t(I.rows));
-    out_rows += (out_cap + 63) & ~uint64_t(63);
+    I.out_base = out_rows;
+    out_rows += p.region_rows(i);
     n_items += (I.rows + P.tile_rows - 1) / P.tile_rows;
     inst_tile_begin[i] = n_tiles;
     n_tiles += int64_t((I.cap + 2 + SCAN_TILE - 1) / SCAN_TILE);
@@ -147,28 +153,28 @@ void layout(mig_batch_plan& p, hipStream_t stream)
     for (uint64_t l = 0; l < p.hinst[i].cap + 2; l += SCAN_TILE) tiles.push_back((uint64_t(i) << 40) | l);
   }
 
-  P.queue           = p.alloc<uint32_t>(1);
+  P.queue           = p.alloc<uint32_t>(2);
   P.items           = p.alloc<uint64_t>(std::max<size_t>(1, items.size()));
   P.slot_tiles      = p.alloc<uint64_t>(tiles.size());
   P.tile_offsets    = p.alloc<uint32_t>(tiles.size());
   P.inst_tile_begin = p.alloc<int64_t>(size_t(n) + 1);
   P.inst            = p.alloc<batch_inst>(size_t(n));
   P.vals            = p.alloc<dcol>(std::max<size_t>(1, size_t(n) * size_t(p.num_vals)));
+  P.inst_kinds      = p.alloc<int32_t>(std::max<size_t>(1, p.kinds.size()));
+  P.inst_widths     = p.alloc<int32_t>(p.widths.size());
   P.slot_keys       = p.alloc<uint64_t>(total_slots);
   P.slot_count      = P.need_count_all ? p.alloc<uint32_t>(total_slots) : nullptr;
   P.special         = p.alloc<uint32_t>(2 * size_t(n));
   P.groups          = p.alloc<uint32_t>(size_t(n));
   P.overflow        = p.alloc<int32_t>(size_t(n));
   P.valid_counts    = p.alloc<uint32_t>(size_t(n) * size_t(P.num_outs + 1));
-  for (int f = 0; f < P.num_fields; f++)
-    P.fields[f].ptr = P.fields[f].width == 8 ? static_cast<void*>(p.alloc<uint64_t>(total_slots))
-                                             : static_cast<void*>(p.alloc<uint32_t>(total_slots));
+  for (int f = 0; f < P.num_fields; f++) P.fields[f].ptr = p.alloc<uint64_t>(total_slots);
   size_t const mask_words = size_t((out_rows + 31) / 32);
-  P.out_key      = p.alloc<uint8_t>(size_t(out_rows) * size_t(kw));
+  P.out_key      = p.alloc<uint64_t>(size_t(out_rows));
   P.out_key_mask = p.alloc<uint32_t>(mask_words);
   for (int o = 0; o < P.num_outs; o++) {
-    P.outs[o].data = p.alloc<uint8_t>(size_t(out_rows) * size_t(type_width(p.outs[o].type_id)));
-    P.outs[o].mask = p.outs[o].mask_if_nulls ? p.alloc<uint32_t>(mask_words) : nullptr;
+    P.outs[o].data = p.alloc<uint64_t>(size_t(out_rows));
+    P.outs[o].mask = p.outs[o].mask != mig_batch_plan::MASK_NONE ? p.alloc<uint32_t>(mask_words) : nullptr;
   }
 
   std::vector<dcol> vals(size_t(n) * size_t(p.num_vals));
@@ -178,15 +184,18 @@ void layout(mig_batch_plan& p, hipStream_t stream)
       vals[k]        = make_dcol(p.val_views[k], p.val_nulls[k] != 0);
     }
 
-  MIG_HIP_TRY(hipMemcpy(P.items, items.data(), items.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
-  MIG_HIP_TRY(hipMemcpy(P.slot_tiles, tiles.data(), tiles.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
-  MIG_HIP_TRY(hipMemcpy(P.inst_tile_begin, inst_tile_begin.data(), inst_tile_begin.size() * sizeof(int64_t),
-                        hipMemcpyHostToDevice));
-  MIG_HIP_TRY(hipMemcpy(P.inst, p.hinst.data(), size_t(n) * sizeof(batch_inst), hipMemcpyHostToDevice));
-  if (!vals.empty())
-    MIG_HIP_TRY(hipMemcpy(P.vals, vals.data(), vals.size() * sizeof(dcol), hipMemcpyHostToDevice));
+  auto up = [](void* dst, void const* src, size_t bytes) {
+    if (bytes) MIG_HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  };
+  up(P.items, items.data(), items.size() * sizeof(uint64_t));
+  up(P.slot_tiles, tiles.data(), tiles.size() * sizeof(uint64_t));
+  up(P.inst_tile_begin, inst_tile_begin.data(), inst_tile_begin.size() * sizeof(int64_t));
+  up(P.inst, p.hinst.data(), size_t(n) * sizeof(batch_inst));
+  up(P.vals, vals.data(), vals.size() * sizeof(dcol));
+  up(P.inst_kinds, p.kinds.data(), p.kinds.size() * sizeof(int32_t));
+  up(P.inst_widths, p.widths.data(), p.widths.size() * sizeof(int32_t));
   p.dP = p.alloc<batch_params>(1);
-  MIG_HIP_TRY(hipMemcpy(p.dP, &P, sizeof(batch_params), hipMemcpyHostToDevice));
+  up(p.dP, &P, sizeof(batch_params));
 
   prepare_batch_aggregate(P.lds_bytes);
   if (p.opt.use_graph) {
@@ -203,7 +212,6 @@ void layout(mig_batch_plan& p, hipStream_t stream)
     MIG_HIP_TRY(hipStreamEndCapture(p.capture_stream, &p.graph));
     MIG_HIP_TRY(hipGraphInstantiate(&p.exec, p.graph, nullptr, nullptr, 0));
   }
-  (void)stream;
 }
 
 // Per-instance group estimates from a strided sample of BATCH_SAMPLE rows
@@ -232,8 +240,8 @@ void estimate(mig_batch_plan& p, hipStream_t stream)
   P.inst = nullptr;
   p.est.assign(size_t(n), 1.0);
   for (int32_t i = 0; i < n; i++) {
-    int64_t const rows = p.hinst[i].rows;
-    int64_t const ns   = std::min<int64_t>(rows, BATCH_SAMPLE);
+    int64_t const rows   = p.hinst[i].rows;
+    int64_t const ns     = std::min<int64_t>(rows, BATCH_SAMPLE);
     double const sampled = double(h[2 * i]), d = double(h[2 * i + 1]);
     if (ns == rows) p.est[i] = d;
     else if (sampled > 0) {
@@ -241,6 +249,12 @@ void estimate(mig_batch_plan& p, hipStream_t stream)
       p.est[i]         = estimate_distinct(sampled, d, std::max(pop, d));
     } else p.est[i] = 1.0;
   }
+}
+
+bool fixed_width(int32_t type_id)
+{
+  int32_t const st = storage_of(type_id);
+  return st != ST_NONE && st != ST_STRING;
 }
 
 }  // namespace
@@ -258,7 +272,7 @@ mig_batch_plan* batch_plan_create(mig_batch_instance const* instances, int32_t n
   MIG_EXPECTS(num_requests >= 0 && num_requests <= MAX_VALS, "number of requests out of range");
   MIG_EXPECTS(num_requests == 0 || (aggregations && num_aggregations), "null aggregation arrays");
 
-  auto plan       = std::make_unique<mig_batch_plan>();
+  auto plan         = std::make_unique<mig_batch_plan>();
   mig_batch_plan& p = *plan;
   MIG_HIP_TRY(hipGetDevice(&p.dev));
   MIG_HIP_TRY(hipDeviceGetAttribute(&p.num_cus, hipDeviceAttributeMultiprocessorCount, p.dev));
@@ -276,21 +290,18 @@ mig_batch_plan* batch_plan_create(mig_batch_instance const* instances, int32_t n
   MIG_EXPECTS(p.opt.tile_rows >= 256 && p.opt.tile_rows <= (1 << 24), "tile_rows out of range");
   p.opt.use_graph = p.opt.use_graph < 0 ? 0 : 1;
 
-  // ---- keys -----------------------------------------------------------------
-  p.key_type = instances[0].keys ? instances[0].keys[0].type_id : MIG_TYPE_EMPTY;
-  int32_t const kst = storage_of(p.key_type);
-  if (kst == ST_NONE || kst == ST_STRING)
-    fail(MIG_ERROR_DATA_TYPE, "the batched group-by takes one fixed-width key column");
+  // ---- keys: one fixed-width column per instance ------------------------------
   p.key_views.resize(size_t(num_instances));
   p.hinst.assign(size_t(num_instances), batch_inst{});
   for (int32_t i = 0; i < num_instances; i++) {
     MIG_EXPECTS(instances[i].keys != nullptr, "null key array of an instance");
     mig_column_view const& k = instances[i].keys[0];
-    MIG_EXPECTS(k.type_id == p.key_type, "all instances must have the same key type");
+    if (!fixed_width(k.type_id))
+      fail(MIG_ERROR_DATA_TYPE, "the batched group-by takes one fixed-width key column per instance");
     MIG_EXPECTS(k.size >= 0, "negative column size");
-    p.key_views[i]     = k;
-    p.hinst[i].rows    = k.size;
-    p.hinst[i].key     = make_dcol(k, k.null_mask != nullptr);
+    p.key_views[i]  = k;
+    p.hinst[i].rows = k.size;
+    p.hinst[i].key  = make_dcol(k, k.null_mask != nullptr);
   }
 
   // ---- requests: one value column per distinct request column -------------
@@ -299,6 +310,8 @@ mig_batch_plan* batch_plan_create(mig_batch_instance const* instances, int32_t n
     MIG_EXPECTS(num_aggregations[r] >= 0, "negative number of aggregations");
     p.agg_begin[r + 1] = p.agg_begin[r] + num_aggregations[r];
   }
+  for (int32_t i = 0; i < num_instances; i++)
+    MIG_EXPECTS(num_requests == 0 || instances[i].values != nullptr, "null values array of an instance");
   // request r reads value column vcol_of[r]; columns are shared when every
   // instance passes the same view for two requests
   std::vector<int32_t> vcol_of(size_t(num_requests), -1), req_of_vcol;
@@ -315,28 +328,20 @@ mig_batch_plan* batch_plan_create(mig_batch_instance const* instances, int32_t n
     }
   }
   p.num_vals = int32_t(req_of_vcol.size());
-  std::vector<int32_t> val_type(size_t(p.num_vals));
   p.val_views.resize(size_t(num_instances) * size_t(p.num_vals));
-  for (int32_t v = 0; v < p.num_vals; v++) {
-    int32_t const r = req_of_vcol[v];
-    val_type[v]     = instances[0].values ? instances[0].values[r].type_id : MIG_TYPE_EMPTY;
+  for (int32_t v = 0; v < p.num_vals; v++)
     for (int32_t i = 0; i < num_instances; i++) {
-      MIG_EXPECTS(instances[i].values != nullptr, "null values array of an instance");
-      mig_column_view const& c = instances[i].values[r];
-      MIG_EXPECTS(c.type_id == val_type[v], "all instances must have the same value types");
+      mig_column_view const& c = instances[i].values[req_of_vcol[v]];
+      if (!fixed_width(c.type_id))
+        fail(MIG_ERROR_DATA_TYPE, "the batched group-by takes fixed-width value columns");
       MIG_EXPECTS(c.size == p.key_views[i].size, "Size mismatch between request values and groupby keys.");
       p.val_views[size_t(i) * p.num_vals + v] = c;
     }
-    if (storage_of(val_type[v]) == ST_STRING || storage_of(val_type[v]) == ST_NONE)
-      fail(MIG_ERROR_DATA_TYPE, "the batched group-by takes fixed-width value columns");
-  }
   for (int32_t r = 0; r < num_requests; r++)
     for (int32_t a = 0; a < num_aggregations[r]; a++) {
       int32_t const kind = aggregations[p.agg_begin[r] + a].kind;
-      if (kind == MIG_AGG_VARIANCE || kind == MIG_AGG_STD)
-        fail(MIG_ERROR_LOGIC, std::string("aggregation ") + kind_name(kind) +
-                                " is not supported by the batched group-by");
-      check_supported(val_type[vcol_of[r]], kind);
+      for (int32_t i = 0; i < num_instances; i++)
+        check_supported(p.val_views[size_t(i) * p.num_vals + vcol_of[r]].type_id, kind);
     }
 
   // null counts of the value views (counted on the device where unknown)
@@ -371,92 +376,115 @@ mig_batch_plan* batch_plan_create(mig_batch_instance const* instances, int32_t n
   for (int32_t i = 0; i < num_instances; i++)
     for (int32_t v = 0; v < p.num_vals; v++) any_nulls[v] |= p.val_nulls[size_t(i) * p.num_vals + v];
 
-  // ---- accumulators ----------------------------------------------------------
+  // ---- accumulator template: fields grouped by value column ------------------
   batch_params& P = p.P;
   std::memset(&P, 0, sizeof(batch_params));
   P.n_inst         = num_instances;
-  P.key_storage    = kst;
   P.skip_null_keys = null_policy == MIG_NULL_EXCLUDE;
   P.tile_rows      = p.opt.tile_rows;
+  enum field_class : int32_t { FC_SUM, FC_SUMSQ, FC_MIN, FC_MAX, FC_COUNT, FC_M2 };
   struct vplan {
-    bool sum = false, sumsq = false, mn = false, mx = false, count = false;
-    int32_t f_sum = -1, f_sumsq = -1, f_min = -1, f_max = -1, f_count = -1;
+    bool sum = false, sumsq = false, mn = false, mx = false, count = false, m2 = false;
+    int32_t f_sum = -1, f_sumsq = -1, f_min = -1, f_max = -1, f_count = -1, f_m2 = -1;
   };
   std::vector<vplan> vp(size_t(p.num_vals));
   for (int32_t r = 0; r < num_requests; r++) {
-    vplan& q = vp[vcol_of[r]];
+    vplan& q      = vp[vcol_of[r]];
+    bool const nn = any_nulls[vcol_of[r]] != 0;
     for (int32_t a = 0; a < num_aggregations[r]; a++) switch (aggregations[p.agg_begin[r] + a].kind) {
-        case MIG_AGG_SUM: q.sum = true; q.count |= any_nulls[vcol_of[r]] != 0; break;
-        case MIG_AGG_SUM_OF_SQUARES: q.sumsq = true; q.count |= any_nulls[vcol_of[r]] != 0; break;
-        case MIG_AGG_MIN: q.mn = true; q.count |= any_nulls[vcol_of[r]] != 0; break;
-        case MIG_AGG_MAX: q.mx = true; q.count |= any_nulls[vcol_of[r]] != 0; break;
+        case MIG_AGG_SUM: q.sum = true; q.count |= nn; break;
+        case MIG_AGG_SUM_OF_SQUARES: q.sumsq = true; q.count |= nn; break;
+        case MIG_AGG_MIN: q.mn = true; q.count |= nn; break;
+        case MIG_AGG_MAX: q.mx = true; q.count |= nn; break;
         case MIG_AGG_COUNT_VALID: q.count = true; break;
         case MIG_AGG_COUNT_ALL: P.need_count_all = 1; break;
         case MIG_AGG_MEAN: q.sum = q.count = true; break;
+        case MIG_AGG_VARIANCE:
+        case MIG_AGG_STD: q.sum = q.count = q.m2 = true; break;
         default: break;
       }
   }
+  std::vector<int32_t> fclass;  // class of each field
   int nf = 0;
-  auto add_field = [&](int32_t v, int32_t kind) {
+  auto add_field = [&](int32_t v, int32_t cls) {
     MIG_EXPECTS(nf < MAX_FIELDS, "too many accumulators");
     field_desc& F = P.fields[nf];
-    F.kind        = kind;
+    F.kind        = cls == FC_M2 ? FK_M2 : FK_SUM_I64;  // template marker; per-instance kinds below
     F.vcol        = v;
-    F.width       = field_width(kind);
+    F.width       = BATCH_ELEM;
     F.aux_sum     = -1;
     F.aux_count   = -1;
+    fclass.push_back(cls);
     return nf++;
   };
   int32_t const COUNT_FROM_ROWS = -2;
   for (int32_t v = 0; v < p.num_vals; v++) {
     vplan& q             = vp[v];
-    int32_t const st     = storage_of(val_type[v]);
     P.val_field_begin[v] = nf;
-    if (q.sum) q.f_sum = add_field(v, sum_field_kind(st));
-    if (q.sumsq) q.f_sumsq = add_field(v, sumsq_field_kind(st));
-    if (q.mn) q.f_min = add_field(v, minmax_field_kind(st, true));
-    if (q.mx) q.f_max = add_field(v, minmax_field_kind(st, false));
+    if (q.sum) q.f_sum = add_field(v, FC_SUM);
+    if (q.sumsq) q.f_sumsq = add_field(v, FC_SUMSQ);
+    if (q.mn) q.f_min = add_field(v, FC_MIN);
+    if (q.mx) q.f_max = add_field(v, FC_MAX);
     if (q.count) {
-      if (any_nulls[v]) q.f_count = add_field(v, FK_COUNT);
+      if (any_nulls[v]) q.f_count = add_field(v, FC_COUNT);
       else {
         q.f_count        = COUNT_FROM_ROWS;
         P.need_count_all = 1;
       }
     }
+    if (q.m2) {
+      q.f_m2                     = add_field(v, FC_M2);
+      P.fields[q.f_m2].aux_sum   = q.f_sum;
+      P.fields[q.f_m2].aux_count = q.f_count;
+      P.has_m2                   = 1;
+    }
   }
   P.val_field_begin[p.num_vals] = nf;
   P.num_vals                    = p.num_vals;
   P.num_fields                  = nf;
+  // per-instance accumulator kinds
+  p.kinds.assign(size_t(num_instances) * size_t(nf), 0);
+  for (int32_t i = 0; i < num_instances; i++)
+    for (int f = 0; f < nf; f++) {
+      int32_t const st = storage_of(p.val_views[size_t(i) * p.num_vals + P.fields[f].vcol].type_id);
+      int32_t k        = FK_M2;
+      switch (fclass[f]) {
+        case FC_SUM: k = sum_field_kind(st); break;
+        case FC_SUMSQ: k = sumsq_field_kind(st); break;
+        case FC_MIN: k = minmax_field_kind(st, true); break;
+        case FC_MAX: k = minmax_field_kind(st, false); break;
+        case FC_COUNT: k = FK_COUNT; break;
+        default: break;
+      }
+      p.kinds[size_t(i) * nf + f] = k;
+    }
 
-  // ---- output programs ---------------------------------------------------------
+  // ---- output programs -----------------------------------------------------------
   p.out_of_agg.assign(size_t(p.agg_begin[num_requests]), -1);
   int no = 0;
   for (int32_t r = 0; r < num_requests; r++) {
-    int32_t const v  = vcol_of[r];
-    vplan const& q   = vp[v];
-    int32_t const st = storage_of(val_type[v]);
+    int32_t const v = vcol_of[r];
+    vplan const& q  = vp[v];
     for (int32_t a = 0; a < num_aggregations[r]; a++) {
       int32_t const kind = aggregations[p.agg_begin[r] + a].kind;
-      int32_t const tt   = target_type_of(val_type[v], kind);
       MIG_EXPECTS(no < MAX_OUTS, "too many aggregation results");
       out_desc& O   = P.outs[no];
       O             = out_desc{};
       O.count_field = -1;
-      O.src_storage = st;
-      O.out_width   = type_width(tt);
-      bool masked   = false;
+      O.ddof        = aggregations[p.agg_begin[r] + a].ddof;
+      int32_t mask  = mig_batch_plan::MASK_NONE;
       switch (kind) {
         case MIG_AGG_SUM:
         case MIG_AGG_SUM_OF_SQUARES:
           O.field = kind == MIG_AGG_SUM ? q.f_sum : q.f_sumsq;
-          O.kind  = tt == MIG_TYPE_FLOAT32 ? OK_COPY_F32 : (tt == MIG_TYPE_FLOAT64 ? OK_COPY_F64 : OK_COPY_I64);
-          if (any_nulls[v]) { O.count_field = q.f_count; masked = true; }
+          O.kind  = OK_COPY_I64;  // copies the instance's result width
+          if (any_nulls[v]) { O.count_field = q.f_count; mask = mig_batch_plan::MASK_IF_NULLS; }
           break;
         case MIG_AGG_MIN:
         case MIG_AGG_MAX:
           O.field = kind == MIG_AGG_MIN ? q.f_min : q.f_max;
           O.kind  = OK_MINMAX;
-          if (any_nulls[v]) { O.count_field = q.f_count; masked = true; }
+          if (any_nulls[v]) { O.count_field = q.f_count; mask = mig_batch_plan::MASK_IF_NULLS; }
           break;
         case MIG_AGG_COUNT_VALID:
           if (q.f_count >= 0) { O.kind = OK_COUNT; O.field = q.f_count; }
@@ -467,28 +495,52 @@ mig_batch_plan* batch_plan_create(mig_batch_instance const* instances, int32_t n
           O.kind        = OK_MEAN;
           O.field       = q.f_sum;
           O.count_field = q.f_count;
-          masked        = any_nulls[v] != 0;
+          if (any_nulls[v]) mask = mig_batch_plan::MASK_IF_NULLS;
+          break;
+        case MIG_AGG_VARIANCE:
+        case MIG_AGG_STD:
+          O.kind        = kind == MIG_AGG_VARIANCE ? OK_VAR : OK_STD;
+          O.field       = q.f_m2;
+          O.count_field = q.f_count;
+          mask          = mig_batch_plan::MASK_ALWAYS;
           break;
         default: fail(MIG_ERROR_LOGIC, "unreachable aggregation kind");
       }
-      p.outs.push_back({tt, v, masked});
+      p.outs.push_back({kind, v, mask});
       p.out_of_agg[size_t(p.agg_begin[r] + a)] = no++;
     }
   }
   P.num_outs = no;
+  p.widths.assign(size_t(num_instances) * size_t(no + 1), 0);
+  for (int32_t i = 0; i < num_instances; i++) {
+    for (int o = 0; o < no; o++) {
+      int32_t const src = p.val_views[size_t(i) * p.num_vals + p.outs[o].vcol].type_id;
+      p.widths[size_t(i) * (no + 1) + o] = type_width(target_type_of(src, p.outs[o].agg_kind));
+    }
+    p.widths[size_t(i) * (no + 1) + no] = type_width(p.key_views[i].type_id);
+  }
 
   // ---- LDS layout of the pre-aggregating tiles -----------------------------------
-  int32_t off = BATCH_LDS_SLOTS * 8;
+  // first pass: key table, one partial array per non-M2 field, row counts;
+  // second pass (within the same bytes): slot table, one array per M2 field
+  int32_t off = BATCH_LDS_SLOTS * 8, m2_off = BATCH_LDS_SLOTS * 8;
   for (int f = 0; f < nf; f++) {
-    P.lds_field_off[f] = off;
-    off += ((BATCH_LDS_SLOTS + 2) * P.fields[f].width + 7) & ~7;
+    if (P.fields[f].kind == FK_M2) {
+      P.lds_field_off[f] = -1;
+      P.lds_m2_off[f]    = m2_off;
+      m2_off += BATCH_LDS_SLOTS * 8;
+    } else {
+      P.lds_field_off[f] = off;
+      P.lds_m2_off[f]    = -1;
+      off += (BATCH_LDS_SLOTS + 2) * 8;
+    }
   }
   P.lds_count_off = -1;
   if (P.need_count_all) {
     P.lds_count_off = off;
     off += ((BATCH_LDS_SLOTS + 2) * 4 + 7) & ~7;
   }
-  P.lds_bytes = off;
+  P.lds_bytes = std::max(off, m2_off);
   if (P.lds_bytes > 160 * 1024 || p.opt.lds_max_groups < 0) {
     p.opt.lds_max_groups = -1;  // off, or does not fit: direct updates only
     P.lds_bytes          = 0;
@@ -498,14 +550,13 @@ mig_batch_plan* batch_plan_create(mig_batch_instance const* instances, int32_t n
   estimate(p, stream);
   p.cap.resize(size_t(num_instances));
   for (int32_t i = 0; i < num_instances; i++) {
-    p.cap[i]         = table_slots(p.est[i], p.hinst[i].rows, p.opt.load_factor);
-    p.hinst[i].lds   = p.opt.lds_max_groups > 0 && p.est[i] <= double(p.opt.lds_max_groups) ? 1 : 0;
+    p.cap[i]       = table_slots(p.est[i], p.hinst[i].rows, p.opt.load_factor);
+    p.hinst[i].lds = p.opt.lds_max_groups > 0 && p.est[i] <= double(p.opt.lds_max_groups) ? 1 : 0;
   }
-  layout(p, stream);
+  layout(p);
 
-  p.pinned_words = size_t(num_instances) * (2 + size_t(no + 1));
-  MIG_HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&p.pinned), p.pinned_words * sizeof(uint32_t),
-                            hipHostMallocDefault));
+  size_t const words = size_t(num_instances) * (2 + size_t(no + 1));
+  MIG_HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&p.pinned), words * sizeof(uint32_t), hipHostMallocDefault));
   return plan.release();
 }
 
@@ -544,7 +595,7 @@ void batch_synchronize(mig_batch_plan* plan, hipStream_t stream)
     if (!grew) break;
     MIG_EXPECTS(attempt < 16, "batched group-by: tables kept overflowing");
     p.retries++;
-    layout(p, stream);
+    layout(p);
     batch_execute(plan, stream);
   }
   p.groups.assign(p.pinned, p.pinned + n);
@@ -566,13 +617,13 @@ mig_column batch_result_key(mig_batch_plan const* plan, int32_t instance)
   MIG_EXPECTS(instance >= 0 && instance < p.n_inst, "instance index out of range");
   batch_inst const& I = p.hinst[size_t(instance)];
   uint32_t const G    = p.groups[size_t(instance)];
-  int32_t const kw    = type_width(p.key_type);
-  uint64_t const capr = (std::min<uint64_t>(I.cap + 2, uint64_t(I.rows)) + 63) & ~uint64_t(63);
+  int32_t const tid   = p.key_views[size_t(instance)].type_id;
+  uint64_t const capr = p.region_rows(instance);
   mig_column c{};
-  c.type_id    = p.key_type;
+  c.type_id    = tid;
   c.size       = int32_t(G);
-  c.data       = static_cast<uint8_t*>(p.P.out_key) + I.out_base * uint64_t(kw);
-  c.data_bytes = int64_t(capr * uint64_t(kw));
+  c.data       = static_cast<uint8_t*>(p.P.out_key) + I.out_base * BATCH_ELEM;
+  c.data_bytes = int64_t(capr * uint64_t(type_width(tid)));
   // an empty instance returns empty columns without masks, as the single call does
   if (p.key_views[size_t(instance)].null_mask != nullptr && I.rows > 0) {
     c.null_mask  = p.P.out_key_mask + I.out_base / 32;
@@ -591,18 +642,22 @@ mig_column batch_result_column(mig_batch_plan const* plan, int32_t instance, int
   MIG_EXPECTS(request >= 0 && request < p.num_requests, "request index out of range");
   MIG_EXPECTS(agg >= 0 && agg < p.agg_begin[size_t(request) + 1] - p.agg_begin[size_t(request)],
               "aggregation index out of range");
-  int32_t const o          = p.out_of_agg[size_t(p.agg_begin[size_t(request)] + agg)];
-  auto const& oi           = p.outs[size_t(o)];
-  batch_inst const& I      = p.hinst[size_t(instance)];
-  uint32_t const G         = p.groups[size_t(instance)];
-  int32_t const w          = type_width(oi.type_id);
-  uint64_t const capr      = (std::min<uint64_t>(I.cap + 2, uint64_t(I.rows)) + 63) & ~uint64_t(63);
+  int32_t const o     = p.out_of_agg[size_t(p.agg_begin[size_t(request)] + agg)];
+  auto const& oi      = p.outs[size_t(o)];
+  batch_inst const& I = p.hinst[size_t(instance)];
+  uint32_t const G    = p.groups[size_t(instance)];
+  int32_t const src   = p.val_views[size_t(instance) * size_t(p.num_vals) + size_t(oi.vcol)].type_id;
+  int32_t const tid   = target_type_of(src, oi.agg_kind);
+  uint64_t const capr = p.region_rows(instance);
   mig_column c{};
-  c.type_id    = oi.type_id;
+  c.type_id    = tid;
   c.size       = int32_t(G);
-  c.data       = static_cast<uint8_t*>(p.P.outs[o].data) + I.out_base * uint64_t(w);
-  c.data_bytes = int64_t(capr * uint64_t(w));
-  if (oi.mask_if_nulls && I.rows > 0 && p.val_nulls[size_t(instance) * size_t(p.num_vals) + size_t(oi.vcol)]) {
+  c.data       = static_cast<uint8_t*>(p.P.outs[o].data) + I.out_base * BATCH_ELEM;
+  c.data_bytes = int64_t(capr * uint64_t(type_width(tid)));
+  bool const nulls  = p.val_nulls[size_t(instance) * size_t(p.num_vals) + size_t(oi.vcol)] != 0;
+  bool const masked = I.rows > 0 && (oi.mask == mig_batch_plan::MASK_ALWAYS ||
+                                     (oi.mask == mig_batch_plan::MASK_IF_NULLS && nulls));
+  if (masked) {
     c.null_mask  = p.P.outs[o].mask + I.out_base / 32;
     c.mask_bytes = int64_t(capr / 8);
     c.null_count = int32_t(G - p.valid[size_t(instance) * size_t(p.P.num_outs + 1) + size_t(o)]);

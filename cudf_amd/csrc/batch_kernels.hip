@@ -1,15 +1,18 @@
 // Batched hash group-by kernels: many independent group-bys (one fixed-width
-// key each) in one persistent launch.
+// key each, per-instance key and value types) in one persistent launch.
 //
 //   k_batch_sample     per instance: distinct row hashes of a strided sample
-//   k_batch_init       every table, accumulator, flag and output mask reset
-//                      (a fill kernel: the sequence is captured in a hipGraph,
-//                      where memset nodes are avoided)
+//   k_batch_init_*     tables, accumulators (each instance's identities),
+//                      flags and output masks reset by fill kernels (the
+//                      sequence is captured in a hipGraph, where memset nodes
+//                      are avoided)
 //   k_batch_aggregate  persistent workgroups pull (instance, row tile) items
-//                      from a device queue; low-cardinality instances are
-//                      pre-aggregated per tile in LDS and flushed into their
-//                      table, the others update their table directly
-//   k_batch_count / k_batch_scan / k_batch_outputs
+//                      from a device queue; instances estimated at few groups
+//                      are pre-aggregated per tile in LDS and flushed into
+//                      their table, the others update their table directly
+//   k_batch_second     variance: squared deviations from the group means,
+//                      over the same work items
+//   k_batch_count / k_batch_scan / k_batch_outputs / k_batch_valid_counts
 //                      per-instance compaction into dense output rows
 //
 // Per row the semantics are those of the single group-by
@@ -24,6 +27,40 @@ constexpr int BBLOCK = 256;
 __device__ __forceinline__ uint32_t b_key_hash(int32_t st, uint64_t bits)
 {
   return hash_combine(0u, hash_fixed_bits(st, bits, 0u));
+}
+
+__device__ __forceinline__ int fk_width(int32_t k)
+{
+  switch (k) {
+    case FK_SUM_F32:
+    case FK_SUMSQ_F32:
+    case FK_MIN_S32:
+    case FK_MAX_S32:
+    case FK_MIN_U32:
+    case FK_MAX_U32:
+    case FK_MIN_F32:
+    case FK_MAX_F32:
+    case FK_COUNT: return 4;
+    default: return 8;
+  }
+}
+
+// Accumulator element `slot` of field `f` (8-byte element slots).
+__device__ __forceinline__ uint8_t* fa(field_desc const& F, int64_t slot)
+{
+  return static_cast<uint8_t*>(F.ptr) + slot * BATCH_ELEM;
+}
+
+__device__ __forceinline__ uint64_t read_kind(int32_t k, void const* a)
+{
+  return fk_width(k) == 4 ? uint64_t(*static_cast<uint32_t const*>(a)) : *static_cast<uint64_t const*>(a);
+}
+
+__device__ __forceinline__ double sum_as_double(int32_t k, void const* a)
+{
+  if (k == FK_SUM_I64) return double(*static_cast<int64_t const*>(a));
+  if (k == FK_SUM_F32) return double(*static_cast<float const*>(a));
+  return *static_cast<double const*>(a);
 }
 
 // ---------------------------------------------------------------------------
@@ -72,55 +109,60 @@ void launch_batch_sample(batch_params const* dB, int32_t n_inst, int32_t per_ins
 }
 
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(BBLOCK) k_batch_init(batch_params const* __restrict__ Bp)
+// One block per slot tile: keys, row counts and every field's identity of
+// the tile's instance.
+__global__ void __launch_bounds__(BBLOCK) k_batch_init_slots(batch_params const* __restrict__ Bp)
 {
   batch_params const& B = *Bp;
-  int const which       = blockIdx.y;  // 0 keys, 1 row counts, 2 small state, 3 out masks, 4.. fields
+  uint64_t const t      = B.slot_tiles[blockIdx.x];
+  int const inst        = int(t >> 40);
+  uint64_t const l0     = t & ((1ull << 40) - 1);
+  batch_inst const I    = B.inst[inst];
+  int32_t const* kinds  = B.inst_kinds + int64_t(inst) * B.num_fields;
+  for (int k = threadIdx.x; k < SCAN_TILE; k += BBLOCK) {
+    uint64_t const l = l0 + k;
+    if (l >= I.cap + 2) break;
+    int64_t const slot = int64_t(I.slot_base + l);
+    B.slot_keys[slot]  = EMPTY_KEY;
+    if (B.slot_count) B.slot_count[slot] = 0;
+    for (int f = 0; f < B.num_fields; f++)
+      *reinterpret_cast<uint64_t*>(fa(B.fields[f], slot)) = kinds[f] == FK_M2 ? 0ull : field_identity(kinds[f]);
+  }
+}
+
+// Small per-instance state, the work queues and the output masks.
+__global__ void __launch_bounds__(BBLOCK) k_batch_init_misc(batch_params const* __restrict__ Bp)
+{
+  batch_params const& B = *Bp;
   int64_t const stride  = int64_t(gridDim.x) * BBLOCK;
   int64_t const i0      = int64_t(blockIdx.x) * BBLOCK + threadIdx.x;
-  uint64_t const total  = B.total_slots;
-  if (which == 0) {
-    for (int64_t j = i0; j < int64_t(total); j += stride) B.slot_keys[j] = EMPTY_KEY;
-  } else if (which == 1) {
-    if (B.slot_count)
-      for (int64_t j = i0; j < int64_t(total); j += stride) B.slot_count[j] = 0;
-  } else if (which == 2) {
-    for (int64_t j = i0; j < int64_t(B.n_inst); j += stride) {
-      B.special[2 * j]     = 0;
-      B.special[2 * j + 1] = 0;
-      B.groups[j]          = 0;
-      B.overflow[j]        = 0;
-    }
-    int64_t const nv = int64_t(B.n_inst) * (B.num_outs + 1);
-    for (int64_t j = i0; j < nv; j += stride) B.valid_counts[j] = 0;
-    if (i0 == 0) B.queue[0] = 0;
-  } else if (which == 3) {
-    int64_t const words = int64_t((B.out_rows + 31) / 32);
-    for (int o = 0; o < B.num_outs; o++)
-      if (B.outs[o].mask)
-        for (int64_t j = i0; j < words; j += stride) B.outs[o].mask[j] = 0;
-    if (B.out_key_mask)
-      for (int64_t j = i0; j < words; j += stride) B.out_key_mask[j] = 0;
-  } else {
-    field_desc const& F = B.fields[which - 4];
-    uint64_t const idv  = field_identity(F.kind);
-    if (F.width == 8) {
-      auto p = reinterpret_cast<uint64_t*>(F.ptr);
-      for (int64_t j = i0; j < int64_t(total); j += stride) p[j] = idv;
-    } else {
-      auto p = reinterpret_cast<uint32_t*>(F.ptr);
-      for (int64_t j = i0; j < int64_t(total); j += stride) p[j] = uint32_t(idv);
-    }
+  for (int64_t j = i0; j < int64_t(B.n_inst); j += stride) {
+    B.special[2 * j]     = 0;
+    B.special[2 * j + 1] = 0;
+    B.groups[j]          = 0;
+    B.overflow[j]        = 0;
   }
+  int64_t const nv = int64_t(B.n_inst) * (B.num_outs + 1);
+  for (int64_t j = i0; j < nv; j += stride) B.valid_counts[j] = 0;
+  if (i0 == 0) {
+    B.queue[0] = 0;
+    B.queue[1] = 0;
+  }
+  int64_t const words = int64_t((B.out_rows + 31) / 32);
+  for (int o = 0; o < B.num_outs; o++)
+    if (B.outs[o].mask)
+      for (int64_t j = i0; j < words; j += stride) B.outs[o].mask[j] = 0;
+  if (B.out_key_mask)
+    for (int64_t j = i0; j < words; j += stride) B.out_key_mask[j] = 0;
 }
 
 void launch_batch_init(batch_params const* dB, batch_params const& hB, int num_cus, hipStream_t s)
 {
-  uint64_t const longest = hB.total_slots > hB.out_rows / 32 ? hB.total_slots : hB.out_rows / 32;
-  int64_t blocks         = (int64_t(longest) + BBLOCK - 1) / BBLOCK;
-  if (blocks > int64_t(num_cus) * 8) blocks = int64_t(num_cus) * 8;
+  hipLaunchKernelGGL(k_batch_init_slots, dim3(hB.n_slot_tiles), dim3(BBLOCK), 0, s, dB);
+  int64_t blocks = int64_t((hB.out_rows / 32 + BBLOCK - 1) / BBLOCK);
+  if (blocks > int64_t(num_cus) * 4) blocks = int64_t(num_cus) * 4;
   if (blocks < 1) blocks = 1;
-  hipLaunchKernelGGL(k_batch_init, dim3(blocks, 4 + hB.num_fields), dim3(BBLOCK), 0, s, dB);
+  hipLaunchKernelGGL(k_batch_init_misc, dim3(blocks), dim3(BBLOCK), 0, s, dB);
 }
 
 // ---------------------------------------------------------------------------
@@ -149,6 +191,23 @@ __device__ __forceinline__ int64_t b_locate(batch_params const& B, batch_inst co
   return -1;
 }
 
+// Slot of a key inserted by the first pass (-1 if absent: only after an
+// overflow, whose results are discarded).
+__device__ __forceinline__ int64_t b_find(batch_params const& B, batch_inst const& I, uint64_t key, uint32_t h)
+{
+  if (key == EMPTY_KEY) return int64_t(I.slot_base + I.cap);
+  uint64_t const* keys = B.slot_keys + I.slot_base;
+  uint64_t const m     = I.cap - 1;
+  uint64_t s           = h & m;
+  for (uint64_t p = 0; p < I.cap; p++) {
+    uint64_t const cur = keys[s];
+    if (cur == key) return int64_t(I.slot_base + s);
+    if (cur == EMPTY_KEY) return -1;
+    s = (s + 1) & m;
+  }
+  return -1;
+}
+
 __device__ __forceinline__ int64_t b_null_slot(batch_params const& B, batch_inst const& I, int inst)
 {
   uint32_t* f = B.special + 2 * inst + 1;
@@ -156,79 +215,73 @@ __device__ __forceinline__ int64_t b_null_slot(batch_params const& B, batch_inst
   return int64_t(I.slot_base + I.cap + 1);
 }
 
-// Combine a pre-aggregated partial (same encoding as the table) into slot.
-__device__ __forceinline__ void combine_field(field_desc const& F, int64_t slot, uint64_t part)
+// Combine a pre-aggregated partial (same encoding as the table) into `a`.
+__device__ __forceinline__ void acc_combine(int32_t kind, void* a, uint64_t part)
 {
-  switch (F.kind) {
+  switch (kind) {
     case FK_SUM_I64:
-    case FK_SUMSQ_I64: aadd(reinterpret_cast<uint64_t*>(F.ptr) + slot, part); break;
+    case FK_SUMSQ_I64: aadd(static_cast<uint64_t*>(a), part); break;
     case FK_SUM_F32:
-    case FK_SUMSQ_F32:
-      unsafeAtomicAdd(reinterpret_cast<float*>(F.ptr) + slot, __uint_as_float(uint32_t(part)));
-      break;
+    case FK_SUMSQ_F32: unsafeAtomicAdd(static_cast<float*>(a), __uint_as_float(uint32_t(part))); break;
     case FK_SUM_F64:
-    case FK_SUMSQ_F64:
-      unsafeAtomicAdd(reinterpret_cast<double*>(F.ptr) + slot, __longlong_as_double(int64_t(part)));
-      break;
-    case FK_MIN_S32: amin(reinterpret_cast<int32_t*>(F.ptr) + slot, int32_t(part)); break;
-    case FK_MAX_S32: amax(reinterpret_cast<int32_t*>(F.ptr) + slot, int32_t(part)); break;
-    case FK_MIN_S64: amin(reinterpret_cast<int64_t*>(F.ptr) + slot, int64_t(part)); break;
-    case FK_MAX_S64: amax(reinterpret_cast<int64_t*>(F.ptr) + slot, int64_t(part)); break;
+    case FK_SUMSQ_F64: unsafeAtomicAdd(static_cast<double*>(a), __longlong_as_double(int64_t(part))); break;
+    case FK_MIN_S32: amin(static_cast<int32_t*>(a), int32_t(part)); break;
+    case FK_MAX_S32: amax(static_cast<int32_t*>(a), int32_t(part)); break;
+    case FK_MIN_S64: amin(static_cast<int64_t*>(a), int64_t(part)); break;
+    case FK_MAX_S64: amax(static_cast<int64_t*>(a), int64_t(part)); break;
     case FK_MIN_U32:
-    case FK_MIN_F32: amin(reinterpret_cast<uint32_t*>(F.ptr) + slot, uint32_t(part)); break;
+    case FK_MIN_F32: amin(static_cast<uint32_t*>(a), uint32_t(part)); break;
     case FK_MAX_U32:
-    case FK_MAX_F32: amax(reinterpret_cast<uint32_t*>(F.ptr) + slot, uint32_t(part)); break;
+    case FK_MAX_F32: amax(static_cast<uint32_t*>(a), uint32_t(part)); break;
     case FK_MIN_U64:
-    case FK_MIN_F64: amin(reinterpret_cast<uint64_t*>(F.ptr) + slot, part); break;
+    case FK_MIN_F64: amin(static_cast<uint64_t*>(a), part); break;
     case FK_MAX_U64:
-    case FK_MAX_F64: amax(reinterpret_cast<uint64_t*>(F.ptr) + slot, part); break;
-    case FK_COUNT: aadd(reinterpret_cast<uint32_t*>(F.ptr) + slot, uint32_t(part)); break;
+    case FK_MAX_F64: amax(static_cast<uint64_t*>(a), part); break;
+    case FK_COUNT: aadd(static_cast<uint32_t*>(a), uint32_t(part)); break;
     default: break;
   }
 }
 
-// The same update into LDS partials (workgroup-scope atomics).
-__device__ __forceinline__ void lds_update(field_desc const& F, uint8_t* lds, int32_t off, int id,
-                                           loaded_value const& v)
+// The per-row update into LDS partials (workgroup-scope atomics).
+__device__ __forceinline__ void lds_apply(int32_t kind, void* a, loaded_value const& v)
 {
   constexpr int WG = __HIP_MEMORY_SCOPE_WORKGROUP;
-  switch (F.kind) {
-    case FK_SUM_I64: __hip_atomic_fetch_add(lds_at<uint64_t>(lds, off, id), uint64_t(v.i), __ATOMIC_RELAXED, WG); break;
-    case FK_SUM_F32: __hip_atomic_fetch_add(lds_at<float>(lds, off, id), v.f, __ATOMIC_RELAXED, WG); break;
-    case FK_SUM_F64: __hip_atomic_fetch_add(lds_at<double>(lds, off, id), v.d, __ATOMIC_RELAXED, WG); break;
+  switch (kind) {
+    case FK_SUM_I64: __hip_atomic_fetch_add(static_cast<uint64_t*>(a), uint64_t(v.i), __ATOMIC_RELAXED, WG); break;
+    case FK_SUM_F32: __hip_atomic_fetch_add(static_cast<float*>(a), v.f, __ATOMIC_RELAXED, WG); break;
+    case FK_SUM_F64: __hip_atomic_fetch_add(static_cast<double*>(a), v.d, __ATOMIC_RELAXED, WG); break;
     case FK_SUMSQ_I64:
-      __hip_atomic_fetch_add(lds_at<uint64_t>(lds, off, id), uint64_t(v.i) * uint64_t(v.i), __ATOMIC_RELAXED, WG);
+      __hip_atomic_fetch_add(static_cast<uint64_t*>(a), uint64_t(v.i) * uint64_t(v.i), __ATOMIC_RELAXED, WG);
       break;
-    case FK_SUMSQ_F32: __hip_atomic_fetch_add(lds_at<float>(lds, off, id), v.f * v.f, __ATOMIC_RELAXED, WG); break;
-    case FK_SUMSQ_F64: __hip_atomic_fetch_add(lds_at<double>(lds, off, id), v.d * v.d, __ATOMIC_RELAXED, WG); break;
-    case FK_MIN_S32: __hip_atomic_fetch_min(lds_at<int32_t>(lds, off, id), int32_t(v.i), __ATOMIC_RELAXED, WG); break;
-    case FK_MAX_S32: __hip_atomic_fetch_max(lds_at<int32_t>(lds, off, id), int32_t(v.i), __ATOMIC_RELAXED, WG); break;
-    case FK_MIN_S64: __hip_atomic_fetch_min(lds_at<int64_t>(lds, off, id), v.i, __ATOMIC_RELAXED, WG); break;
-    case FK_MAX_S64: __hip_atomic_fetch_max(lds_at<int64_t>(lds, off, id), v.i, __ATOMIC_RELAXED, WG); break;
-    case FK_MIN_U32: __hip_atomic_fetch_min(lds_at<uint32_t>(lds, off, id), uint32_t(v.i), __ATOMIC_RELAXED, WG); break;
-    case FK_MAX_U32: __hip_atomic_fetch_max(lds_at<uint32_t>(lds, off, id), uint32_t(v.i), __ATOMIC_RELAXED, WG); break;
-    case FK_MIN_U64: __hip_atomic_fetch_min(lds_at<uint64_t>(lds, off, id), uint64_t(v.i), __ATOMIC_RELAXED, WG); break;
-    case FK_MAX_U64: __hip_atomic_fetch_max(lds_at<uint64_t>(lds, off, id), uint64_t(v.i), __ATOMIC_RELAXED, WG); break;
+    case FK_SUMSQ_F32: __hip_atomic_fetch_add(static_cast<float*>(a), v.f * v.f, __ATOMIC_RELAXED, WG); break;
+    case FK_SUMSQ_F64: __hip_atomic_fetch_add(static_cast<double*>(a), v.d * v.d, __ATOMIC_RELAXED, WG); break;
+    case FK_MIN_S32: __hip_atomic_fetch_min(static_cast<int32_t*>(a), int32_t(v.i), __ATOMIC_RELAXED, WG); break;
+    case FK_MAX_S32: __hip_atomic_fetch_max(static_cast<int32_t*>(a), int32_t(v.i), __ATOMIC_RELAXED, WG); break;
+    case FK_MIN_S64: __hip_atomic_fetch_min(static_cast<int64_t*>(a), v.i, __ATOMIC_RELAXED, WG); break;
+    case FK_MAX_S64: __hip_atomic_fetch_max(static_cast<int64_t*>(a), v.i, __ATOMIC_RELAXED, WG); break;
+    case FK_MIN_U32: __hip_atomic_fetch_min(static_cast<uint32_t*>(a), uint32_t(v.i), __ATOMIC_RELAXED, WG); break;
+    case FK_MAX_U32: __hip_atomic_fetch_max(static_cast<uint32_t*>(a), uint32_t(v.i), __ATOMIC_RELAXED, WG); break;
+    case FK_MIN_U64: __hip_atomic_fetch_min(static_cast<uint64_t*>(a), uint64_t(v.i), __ATOMIC_RELAXED, WG); break;
+    case FK_MAX_U64: __hip_atomic_fetch_max(static_cast<uint64_t*>(a), uint64_t(v.i), __ATOMIC_RELAXED, WG); break;
     case FK_MIN_F32:
-      if (!isnan(v.f))
-        __hip_atomic_fetch_min(lds_at<uint32_t>(lds, off, id), ord32(uint32_t(v.b)), __ATOMIC_RELAXED, WG);
+      if (!isnan(v.f)) __hip_atomic_fetch_min(static_cast<uint32_t*>(a), ord32(uint32_t(v.b)), __ATOMIC_RELAXED, WG);
       break;
     case FK_MAX_F32:
-      if (!isnan(v.f))
-        __hip_atomic_fetch_max(lds_at<uint32_t>(lds, off, id), ord32(uint32_t(v.b)), __ATOMIC_RELAXED, WG);
+      if (!isnan(v.f)) __hip_atomic_fetch_max(static_cast<uint32_t*>(a), ord32(uint32_t(v.b)), __ATOMIC_RELAXED, WG);
       break;
     case FK_MIN_F64:
-      if (!isnan(v.d)) __hip_atomic_fetch_min(lds_at<uint64_t>(lds, off, id), ord64(v.b), __ATOMIC_RELAXED, WG);
+      if (!isnan(v.d)) __hip_atomic_fetch_min(static_cast<uint64_t*>(a), ord64(v.b), __ATOMIC_RELAXED, WG);
       break;
     case FK_MAX_F64:
-      if (!isnan(v.d)) __hip_atomic_fetch_max(lds_at<uint64_t>(lds, off, id), ord64(v.b), __ATOMIC_RELAXED, WG);
+      if (!isnan(v.d)) __hip_atomic_fetch_max(static_cast<uint64_t*>(a), ord64(v.b), __ATOMIC_RELAXED, WG);
       break;
-    case FK_COUNT: __hip_atomic_fetch_add(lds_at<uint32_t>(lds, off, id), 1u, __ATOMIC_RELAXED, WG); break;
+    case FK_COUNT: __hip_atomic_fetch_add(static_cast<uint32_t*>(a), 1u, __ATOMIC_RELAXED, WG); break;
     default: break;
   }
 }
 
-__device__ __forceinline__ void b_update_global(batch_params const& B, int inst, int64_t slot, int64_t row)
+__device__ __forceinline__ void b_update_global(batch_params const& B, int32_t const* kinds, int inst,
+                                                int64_t slot, int64_t row)
 {
   if (B.need_count_all) aadd(B.slot_count + slot, 1u);
   for (int v = 0; v < B.num_vals; v++) {
@@ -237,8 +290,53 @@ __device__ __forceinline__ void b_update_global(batch_params const& B, int inst,
     dcol const& c = B.vals[int64_t(inst) * B.num_vals + v];
     if (!dcol_valid(c, row)) continue;
     loaded_value const val = load_value(c, row);
-    for (int f = fb; f < fe; f++) apply_field(B.fields[f], slot, val);
+    for (int f = fb; f < fe; f++) acc_apply(kinds[f], fa(B.fields[f], slot), val);
   }
+}
+
+// Insert `key` into the LDS table (S slots of u64 at `tkeys`, at most
+// BATCH_LDS_FILL keys); returns its LDS slot, or -1 when the table is full.
+__device__ __forceinline__ int lds_insert(uint64_t* tkeys, uint32_t* fill, uint64_t key, uint32_t h,
+                                          uint64_t empty)
+{
+  constexpr int S  = BATCH_LDS_SLOTS;
+  constexpr int WG = __HIP_MEMORY_SCOPE_WORKGROUP;
+  uint32_t s       = h & uint32_t(S - 1);
+  for (int p = 0; p < S; p++) {
+    uint64_t const cur = __hip_atomic_load(tkeys + s, __ATOMIC_RELAXED, WG);
+    if (cur == key) return int(s);
+    if (cur == empty) {
+      if (__hip_atomic_load(fill, __ATOMIC_RELAXED, WG) >= uint32_t(BATCH_LDS_FILL)) return -1;
+      uint64_t expected = empty;
+      if (__hip_atomic_compare_exchange_strong(tkeys + s, &expected, key, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                               WG)) {
+        __hip_atomic_fetch_add(fill, 1u, __ATOMIC_RELAXED, WG);
+        return int(s);
+      }
+      if (expected == key) return int(s);
+    }
+    s = (s + 1) & uint32_t(S - 1);
+  }
+  return -1;
+}
+
+// Pull the next work item of queue `q` into `item_sh` (all threads), and
+// the instance's accumulator kinds into `kinds_sh`.  Returns false when the
+// queue is drained.
+__device__ __forceinline__ bool next_item(batch_params const& B, int q, uint64_t& item_sh, int32_t* kinds_sh,
+                                          int nt)
+{
+  if (threadIdx.x == 0) {
+    uint32_t it = __hip_atomic_fetch_add(B.queue + q, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    item_sh     = it < uint64_t(B.n_items) ? B.items[it] : ~0ull;
+  }
+  __syncthreads();
+  uint64_t const item = item_sh;
+  if (item == ~0ull) return false;
+  int const inst = int(item >> 40);
+  for (int f = threadIdx.x; f < B.num_fields; f += nt) kinds_sh[f] = B.inst_kinds[int64_t(inst) * B.num_fields + f];
+  __syncthreads();
+  return true;
 }
 
 template <int NT>
@@ -247,22 +345,17 @@ __global__ void __launch_bounds__(NT) k_batch_aggregate(batch_params const* __re
   extern __shared__ __attribute__((aligned(16))) uint8_t lds[];
   __shared__ uint64_t item_sh;
   __shared__ uint32_t fill_sh, present_sh[2];
+  __shared__ int32_t kinds_sh[MAX_FIELDS];
   batch_params const& B = *Bp;
   constexpr int S       = BATCH_LDS_SLOTS;
   constexpr int WG      = __HIP_MEMORY_SCOPE_WORKGROUP;
   auto* tkeys           = reinterpret_cast<uint64_t*>(lds);
-  for (;;) {
-    if (threadIdx.x == 0) {
-      uint32_t it = __hip_atomic_fetch_add(B.queue, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      item_sh     = it < uint64_t(B.n_items) ? B.items[it] : ~0ull;
-    }
-    __syncthreads();
+  while (next_item(B, 0, item_sh, kinds_sh, NT)) {
     uint64_t const item = item_sh;
-    if (item == ~0ull) break;
-    int const inst     = int(item >> 40);
-    int64_t const r0   = int64_t(item & ((1ull << 40) - 1));
-    batch_inst const I = B.inst[inst];
-    int64_t const r1   = (r0 + B.tile_rows < I.rows) ? r0 + B.tile_rows : I.rows;
+    int const inst      = int(item >> 40);
+    int64_t const r0    = int64_t(item & ((1ull << 40) - 1));
+    batch_inst const I  = B.inst[inst];
+    int64_t const r1    = (r0 + B.tile_rows < I.rows) ? r0 + B.tile_rows : I.rows;
 
     if (!I.lds) {
       for (int64_t row = r0 + threadIdx.x; row < r1; row += NT) {
@@ -274,21 +367,19 @@ __global__ void __launch_bounds__(NT) k_batch_aggregate(batch_params const* __re
           uint64_t const bits = load_fixed_bits(I.key, row);
           slot                = b_locate(B, I, inst, bits, b_key_hash(I.key.storage, bits));
         }
-        if (slot >= 0) b_update_global(B, inst, slot, row);
+        if (slot >= 0) b_update_global(B, kinds_sh, inst, slot, row);
       }
       __syncthreads();
       continue;
     }
 
-    // ---- LDS pre-aggregation of the tile (slots S, S+1: EMPTY_KEY, null key)
+    // ---- LDS pre-aggregation of the tile (rows S, S+1: EMPTY_KEY, null key)
     for (int i = threadIdx.x; i < S; i += NT) tkeys[i] = EMPTY_KEY;
     for (int f = 0; f < B.num_fields; f++) {
-      uint64_t const idv = field_identity(B.fields[f].kind);
-      int32_t const off  = B.lds_field_off[f];
-      if (B.fields[f].width == 8)
-        for (int i = threadIdx.x; i < S + 2; i += NT) *lds_at<uint64_t>(lds, off, i) = idv;
-      else
-        for (int i = threadIdx.x; i < S + 2; i += NT) *lds_at<uint32_t>(lds, off, i) = uint32_t(idv);
+      int32_t const off = B.lds_field_off[f];
+      if (off < 0) continue;
+      uint64_t const idv = field_identity(kinds_sh[f]);
+      for (int i = threadIdx.x; i < S + 2; i += NT) *lds_at<uint64_t>(lds, off, i) = idv;
     }
     if (B.lds_count_off >= 0)
       for (int i = threadIdx.x; i < S + 2; i += NT) *lds_at<uint32_t>(lds, B.lds_count_off, i) = 0;
@@ -299,44 +390,27 @@ __global__ void __launch_bounds__(NT) k_batch_aggregate(batch_params const* __re
     }
     __syncthreads();
     for (int64_t row = r0 + threadIdx.x; row < r1; row += NT) {
-      int id = -1;
+      int id        = -1;
       uint64_t bits = 0;
       uint32_t h    = 0;
-      bool const kv = dcol_valid(I.key, row);
-      if (!kv) {
+      if (!dcol_valid(I.key, row)) {
         if (B.skip_null_keys) continue;
-        id = S + 1;
+        id            = S + 1;
         present_sh[1] = 1;
       } else {
         bits = load_fixed_bits(I.key, row);
         h    = b_key_hash(I.key.storage, bits);
         if (bits == EMPTY_KEY) {
-          id = S;
+          id            = S;
           present_sh[0] = 1;
         } else {
-          uint32_t s = h & uint32_t(S - 1);
-          for (int p = 0; p < S; p++) {
-            uint64_t cur = __hip_atomic_load(tkeys + s, __ATOMIC_RELAXED, WG);
-            if (cur == bits) { id = int(s); break; }
-            if (cur == EMPTY_KEY) {
-              if (__hip_atomic_load(&fill_sh, __ATOMIC_RELAXED, WG) >= uint32_t(BATCH_LDS_FILL)) break;
-              uint64_t expected = EMPTY_KEY;
-              if (__hip_atomic_compare_exchange_strong(tkeys + s, &expected, bits, __ATOMIC_RELAXED,
-                                                       __ATOMIC_RELAXED, WG)) {
-                __hip_atomic_fetch_add(&fill_sh, 1u, __ATOMIC_RELAXED, WG);
-                id = int(s);
-                break;
-              }
-              if (expected == bits) { id = int(s); break; }
-            }
-            s = (s + 1) & uint32_t(S - 1);
-          }
+          id = lds_insert(tkeys, &fill_sh, bits, h, EMPTY_KEY);
         }
       }
       if (id < 0) {
         // the LDS table is full: this row goes straight to the instance table
         int64_t const slot = b_locate(B, I, inst, bits, h);
-        if (slot >= 0) b_update_global(B, inst, slot, row);
+        if (slot >= 0) b_update_global(B, kinds_sh, inst, slot, row);
         continue;
       }
       if (B.lds_count_off >= 0)
@@ -347,7 +421,8 @@ __global__ void __launch_bounds__(NT) k_batch_aggregate(batch_params const* __re
         dcol const& c = B.vals[int64_t(inst) * B.num_vals + v];
         if (!dcol_valid(c, row)) continue;
         loaded_value const val = load_value(c, row);
-        for (int f = fb; f < fe; f++) lds_update(B.fields[f], lds, B.lds_field_off[f], id, val);
+        for (int f = fb; f < fe; f++)
+          if (B.lds_field_off[f] >= 0) lds_apply(kinds_sh[f], lds_at<uint64_t>(lds, B.lds_field_off[f], id), val);
       }
     }
     __syncthreads();
@@ -368,18 +443,19 @@ __global__ void __launch_bounds__(NT) k_batch_aggregate(batch_params const* __re
       if (slot < 0) continue;
       if (B.lds_count_off >= 0) aadd(B.slot_count + slot, *lds_at<uint32_t>(lds, B.lds_count_off, i));
       for (int f = 0; f < B.num_fields; f++) {
-        field_desc const& F = B.fields[f];
-        uint64_t const part = F.width == 8 ? *lds_at<uint64_t>(lds, B.lds_field_off[f], i)
-                                           : *lds_at<uint32_t>(lds, B.lds_field_off[f], i);
-        uint64_t const idv  = F.width == 8 ? field_identity(F.kind) : uint32_t(field_identity(F.kind));
-        // an identity partial leaves the accumulator as it is (the float sums
-        // start at +0.0, which adding +0.0 keeps)
-        if (part != idv) combine_field(F, slot, part);
+        if (B.lds_field_off[f] < 0) continue;
+        int32_t const k     = kinds_sh[f];
+        uint64_t const part = read_kind(k, lds_at<uint64_t>(lds, B.lds_field_off[f], i));
+        // an identity partial leaves the accumulator as it is (the float
+        // sums start at +0.0, which adding +0.0 keeps)
+        if (part != field_identity(k)) acc_combine(k, fa(B.fields[f], slot), part);
       }
     }
     __syncthreads();
   }
 }
+
+void prepare_batch_aggregate(int32_t lds_bytes);
 
 void launch_batch_aggregate(batch_params const* dB, batch_params const& hB, int num_cus, int block,
                             hipStream_t s)
@@ -391,6 +467,115 @@ void launch_batch_aggregate(batch_params const* dB, batch_params const& hB, int 
     hipLaunchKernelGGL(k_batch_aggregate<512>, dim3(blocks), dim3(512), hB.lds_bytes, s, dB);
   else
     hipLaunchKernelGGL(k_batch_aggregate<256>, dim3(blocks), dim3(256), hB.lds_bytes, s, dB);
+}
+
+// ---------------------------------------------------------------------------
+// second pass: squared deviations from the group mean (FK_M2 fields)
+
+__device__ __forceinline__ uint32_t b_count_of(batch_params const& B, int32_t count_field, int64_t slot)
+{
+  if (count_field == -2) return B.slot_count[slot];
+  return *reinterpret_cast<uint32_t const*>(fa(B.fields[count_field], slot));
+}
+
+// The group slot of `row` for the second pass (-1: row dropped or absent).
+__device__ __forceinline__ int64_t b_row_slot(batch_params const& B, batch_inst const& I, int64_t row)
+{
+  if (!dcol_valid(I.key, row)) return B.skip_null_keys ? -1 : int64_t(I.slot_base + I.cap + 1);
+  uint64_t const bits = load_fixed_bits(I.key, row);
+  return b_find(B, I, bits, b_key_hash(I.key.storage, bits));
+}
+
+// Deviation term of row `row` for M2 field `f` at `slot`.
+__device__ __forceinline__ double b_dev2(batch_params const& B, int32_t const* kinds, int f, int64_t slot,
+                                         loaded_value const& val)
+{
+  field_desc const& F = B.fields[f];
+  double const cnt    = double(b_count_of(B, F.aux_count, slot));
+  double const mean   = sum_as_double(kinds[F.aux_sum], fa(B.fields[F.aux_sum], slot)) / cnt;
+  double const d      = val.d - mean;
+  return d * d;
+}
+
+template <int NT>
+__global__ void __launch_bounds__(NT) k_batch_second(batch_params const* __restrict__ Bp)
+{
+  extern __shared__ __attribute__((aligned(16))) uint8_t lds[];
+  __shared__ uint64_t item_sh;
+  __shared__ uint32_t fill_sh;
+  __shared__ int32_t kinds_sh[MAX_FIELDS];
+  batch_params const& B = *Bp;
+  constexpr int S       = BATCH_LDS_SLOTS;
+  constexpr int WG      = __HIP_MEMORY_SCOPE_WORKGROUP;
+  constexpr uint64_t NO = ~0ull;  // empty marker of the LDS slot table
+  auto* tslots          = reinterpret_cast<uint64_t*>(lds);
+  while (next_item(B, 1, item_sh, kinds_sh, NT)) {
+    uint64_t const item = item_sh;
+    int const inst      = int(item >> 40);
+    int64_t const r0    = int64_t(item & ((1ull << 40) - 1));
+    batch_inst const I  = B.inst[inst];
+    int64_t const r1    = (r0 + B.tile_rows < I.rows) ? r0 + B.tile_rows : I.rows;
+    bool const use_lds  = I.lds != 0;
+    if (use_lds) {
+      for (int i = threadIdx.x; i < S; i += NT) tslots[i] = NO;
+      for (int f = 0; f < B.num_fields; f++) {
+        if (B.fields[f].kind != FK_M2) continue;
+        for (int i = threadIdx.x; i < S; i += NT) *lds_at<double>(lds, B.lds_m2_off[f], i) = 0.0;
+      }
+      if (threadIdx.x == 0) fill_sh = 0;
+      __syncthreads();
+    }
+    for (int64_t row = r0 + threadIdx.x; row < r1; row += NT) {
+      int64_t slot = -2;
+      int id       = -2;
+      for (int v = 0; v < B.num_vals; v++) {
+        int const fb = B.val_field_begin[v], fe = B.val_field_begin[v + 1];
+        bool has_m2  = false;
+        for (int f = fb; f < fe; f++) has_m2 |= (B.fields[f].kind == FK_M2);
+        if (!has_m2) continue;
+        dcol const& c = B.vals[int64_t(inst) * B.num_vals + v];
+        if (!dcol_valid(c, row)) continue;
+        if (slot == -2) {
+          slot = b_row_slot(B, I, row);
+          if (slot >= 0 && use_lds) id = lds_insert(tslots, &fill_sh, uint64_t(slot), mm_fmix32(uint32_t(slot)), NO);
+        }
+        if (slot < 0) break;
+        loaded_value const val = load_value(c, row);
+        for (int f = fb; f < fe; f++) {
+          if (B.fields[f].kind != FK_M2) continue;
+          double const t = b_dev2(B, kinds_sh, f, slot, val);
+          if (id >= 0) __hip_atomic_fetch_add(lds_at<double>(lds, B.lds_m2_off[f], id), t, __ATOMIC_RELAXED, WG);
+          else unsafeAtomicAdd(reinterpret_cast<double*>(fa(B.fields[f], slot)), t);
+        }
+      }
+    }
+    __syncthreads();
+    if (use_lds) {
+      for (int i = threadIdx.x; i < S; i += NT) {
+        uint64_t const slot = tslots[i];
+        if (slot == NO) continue;
+        for (int f = 0; f < B.num_fields; f++) {
+          if (B.fields[f].kind != FK_M2) continue;
+          double const part = *lds_at<double>(lds, B.lds_m2_off[f], i);
+          if (part != 0.0) unsafeAtomicAdd(reinterpret_cast<double*>(fa(B.fields[f], int64_t(slot))), part);
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+void launch_batch_second_pass(batch_params const* dB, batch_params const& hB, int num_cus, int block,
+                              hipStream_t s)
+{
+  if (!hB.has_m2) return;
+  int64_t blocks = int64_t(num_cus) * (block == 512 ? 2 : 4);
+  if (blocks > hB.n_items) blocks = hB.n_items;
+  if (blocks < 1) blocks = 1;
+  if (block == 512)
+    hipLaunchKernelGGL(k_batch_second<512>, dim3(blocks), dim3(512), hB.lds_bytes, s, dB);
+  else
+    hipLaunchKernelGGL(k_batch_second<256>, dim3(blocks), dim3(256), hB.lds_bytes, s, dB);
 }
 
 // ---------------------------------------------------------------------------
@@ -460,11 +645,13 @@ __global__ void __launch_bounds__(BBLOCK) k_batch_outputs(batch_params const* __
   batch_inst const I    = B.inst[inst];
   uint32_t running      = B.tile_offsets[blockIdx.x];
   int const wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  int const kw = storage_width(B.key_storage);
+  int32_t const* kinds  = B.inst_kinds + int64_t(inst) * B.num_fields;
+  int32_t const* widths = B.inst_widths + int64_t(inst) * (B.num_outs + 1);
+  uint64_t const rbase  = I.out_base * BATCH_ELEM;  // byte offset of the instance's regions
   for (int k = 0; k < SCAN_TILE; k += BBLOCK) {
-    uint64_t const l   = l0 + k + threadIdx.x;
-    bool const occ     = l < I.cap + 2 && b_occupied(B, I, inst, l);
-    uint64_t const bal = __ballot(occ);
+    uint64_t const l     = l0 + k + threadIdx.x;
+    bool const occ       = l < I.cap + 2 && b_occupied(B, I, inst, l);
+    uint64_t const bal   = __ballot(occ);
     uint32_t const below = __popcll(bal & ((1ull << lane) - 1ull));
     if (lane == 0) wave_tot[wave] = __popcll(bal);
     __syncthreads();
@@ -474,63 +661,65 @@ __global__ void __launch_bounds__(BBLOCK) k_batch_outputs(batch_params const* __
       all += wave_tot[w];
     }
     if (occ) {
-      uint64_t const pos  = I.out_base + running + before + below;
-      int64_t const slot  = int64_t(I.slot_base + l);
-      bool const kvalid   = l != I.cap + 1;
-      uint64_t const key  = l < I.cap ? B.slot_keys[slot] : (l == I.cap ? EMPTY_KEY : 0ull);
-      store_width(B.out_key, int64_t(pos), kw, kvalid ? key : 0ull);
+      uint64_t const lp  = running + before + below;  // row within the instance
+      uint64_t const pos = I.out_base + lp;           // row of the concatenated masks
+      int64_t const slot = int64_t(I.slot_base + l);
+      bool const kvalid  = l != I.cap + 1;
+      uint64_t const key = l < I.cap ? B.slot_keys[slot] : (l == I.cap ? EMPTY_KEY : 0ull);
+      store_width(static_cast<uint8_t*>(B.out_key) + rbase, int64_t(lp), widths[B.num_outs], kvalid ? key : 0ull);
       if (B.out_key_mask && kvalid)
         __hip_atomic_fetch_or(B.out_key_mask + (pos >> 5), 1u << (pos & 31), __ATOMIC_RELAXED,
                               __HIP_MEMORY_SCOPE_AGENT);
       for (int o = 0; o < B.num_outs; o++) {
         out_desc const& O = B.outs[o];
-        bool valid        = true;
-        uint32_t cnt      = 0;
+        uint8_t* const dst = static_cast<uint8_t*>(O.data) + rbase;
+        int const w        = widths[o];
+        bool valid         = true;
+        uint32_t cnt       = 0;
         if (O.count_field != -1) {
-          cnt   = O.count_field == -2 ? B.slot_count[slot]
-                                      : reinterpret_cast<uint32_t const*>(B.fields[O.count_field].ptr)[slot];
+          cnt   = b_count_of(B, O.count_field, slot);
           valid = cnt > 0;
         }
         switch (O.kind) {
           case OK_COPY_I64:
-          case OK_COPY_F64:
-            reinterpret_cast<uint64_t*>(O.data)[pos] = reinterpret_cast<uint64_t const*>(B.fields[O.field].ptr)[slot];
-            break;
           case OK_COPY_F32:
-            reinterpret_cast<uint32_t*>(O.data)[pos] = reinterpret_cast<uint32_t const*>(B.fields[O.field].ptr)[slot];
+          case OK_COPY_F64:
+            store_width(dst, int64_t(lp), w, *reinterpret_cast<uint64_t const*>(fa(B.fields[O.field], slot)));
             break;
           case OK_COUNT:
-            reinterpret_cast<int32_t*>(O.data)[pos] =
-              int32_t(reinterpret_cast<uint32_t const*>(B.fields[O.field].ptr)[slot]);
+            store_width(dst, int64_t(lp), 4, *reinterpret_cast<uint32_t const*>(fa(B.fields[O.field], slot)));
             valid = true;
             break;
           case OK_COUNT_ALL:
-            reinterpret_cast<int32_t*>(O.data)[pos] = int32_t(B.slot_count[slot]);
+            store_width(dst, int64_t(lp), 4, B.slot_count[slot]);
             valid = true;
             break;
           case OK_MINMAX: {
-            field_desc const& F = B.fields[O.field];
-            uint64_t bits = F.width == 8 ? reinterpret_cast<uint64_t const*>(F.ptr)[slot]
-                                         : reinterpret_cast<uint32_t const*>(F.ptr)[slot];
-            if (F.kind == FK_MIN_F32 || F.kind == FK_MAX_F32) bits = unord32(uint32_t(bits));
-            else if (F.kind == FK_MIN_F64 || F.kind == FK_MAX_F64) bits = unord64(bits);
-            store_width(O.data, int64_t(pos), O.out_width, bits);
+            int32_t const fk = kinds[O.field];
+            uint64_t bits    = read_kind(fk, fa(B.fields[O.field], slot));
+            if (fk == FK_MIN_F32 || fk == FK_MAX_F32) bits = unord32(uint32_t(bits));
+            else if (fk == FK_MIN_F64 || fk == FK_MAX_F64) bits = unord64(bits);
+            store_width(dst, int64_t(lp), w, bits);
             break;
           }
           case OK_MEAN: {
-            field_desc const& F = B.fields[O.field];
-            double sum          = 0.0;
-            if (F.kind == FK_SUM_I64) sum = double(reinterpret_cast<int64_t const*>(F.ptr)[slot]);
-            else if (F.kind == FK_SUM_F32) sum = double(reinterpret_cast<float const*>(F.ptr)[slot]);
-            else sum = reinterpret_cast<double const*>(F.ptr)[slot];
-            reinterpret_cast<double*>(O.data)[pos] = valid ? sum / double(cnt) : 0.0;
+            double const sum = sum_as_double(kinds[O.field], fa(B.fields[O.field], slot));
+            reinterpret_cast<double*>(dst)[lp] = valid ? sum / double(cnt) : 0.0;
+            break;
+          }
+          case OK_VAR:
+          case OK_STD: {
+            int64_t const dn = int64_t(cnt) - O.ddof;
+            valid            = cnt > 0 && dn > 0;
+            double var       = 0.0;
+            if (valid) var = *reinterpret_cast<double const*>(fa(B.fields[O.field], slot)) / double(dn);
+            reinterpret_cast<double*>(dst)[lp] = O.kind == OK_STD ? sqrt(var) : var;
             break;
           }
           default: break;
         }
         if (O.mask && valid)
-          __hip_atomic_fetch_or(O.mask + (pos >> 5), 1u << (pos & 31), __ATOMIC_RELAXED,
-                                __HIP_MEMORY_SCOPE_AGENT);
+          __hip_atomic_fetch_or(O.mask + (pos >> 5), 1u << (pos & 31), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       }
     }
     running += all;
@@ -552,7 +741,7 @@ void launch_batch_outputs(batch_params const* dB, batch_params const& hB, hipStr
 }
 
 // Set bits of each instance's region of each output mask (grid: instances x
-// (masked outputs + key)); the host turns them into null counts.
+// (outputs + key)); the host turns them into null counts.
 __global__ void __launch_bounds__(BBLOCK) k_batch_valid_counts(batch_params const* __restrict__ Bp)
 {
   __shared__ uint32_t red[BBLOCK / 64];
@@ -584,6 +773,10 @@ void prepare_batch_aggregate(int32_t lds_bytes)
     (void)hipFuncSetAttribute(reinterpret_cast<void const*>(&k_batch_aggregate<256>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, lds_bytes);
     (void)hipFuncSetAttribute(reinterpret_cast<void const*>(&k_batch_aggregate<512>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, lds_bytes);
+    (void)hipFuncSetAttribute(reinterpret_cast<void const*>(&k_batch_second<256>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, lds_bytes);
+    (void)hipFuncSetAttribute(reinterpret_cast<void const*>(&k_batch_second<512>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, lds_bytes);
   }
 }

@@ -208,40 +208,46 @@ __device__ __forceinline__ loaded_value load_value(dcol const& c, int64_t row)
   return v;
 }
 
-// Accumulate one valid value into slot `slot` of a global accumulator array
-// (agent-scope atomics; FK_M2 is handled by the second pass).
-__device__ __forceinline__ void apply_field(field_desc const& F, int64_t slot, loaded_value const& v)
+// Accumulate one valid value into the accumulator at `a` (an element of the
+// kind's width; agent-scope atomics; FK_M2 is handled by a second pass).
+__device__ __forceinline__ void acc_apply(int32_t kind, void* a, loaded_value const& v)
 {
-  switch (F.kind) {
-    case FK_SUM_I64: aadd(reinterpret_cast<uint64_t*>(F.ptr) + slot, uint64_t(v.i)); break;
-    case FK_SUM_F32: unsafeAtomicAdd(reinterpret_cast<float*>(F.ptr) + slot, v.f); break;
-    case FK_SUM_F64: unsafeAtomicAdd(reinterpret_cast<double*>(F.ptr) + slot, v.d); break;
-    case FK_SUMSQ_I64: aadd(reinterpret_cast<uint64_t*>(F.ptr) + slot, uint64_t(v.i) * uint64_t(v.i)); break;
-    case FK_SUMSQ_F32: unsafeAtomicAdd(reinterpret_cast<float*>(F.ptr) + slot, v.f * v.f); break;
-    case FK_SUMSQ_F64: unsafeAtomicAdd(reinterpret_cast<double*>(F.ptr) + slot, v.d * v.d); break;
-    case FK_MIN_S32: amin(reinterpret_cast<int32_t*>(F.ptr) + slot, int32_t(v.i)); break;
-    case FK_MAX_S32: amax(reinterpret_cast<int32_t*>(F.ptr) + slot, int32_t(v.i)); break;
-    case FK_MIN_S64: amin(reinterpret_cast<int64_t*>(F.ptr) + slot, v.i); break;
-    case FK_MAX_S64: amax(reinterpret_cast<int64_t*>(F.ptr) + slot, v.i); break;
-    case FK_MIN_U32: amin(reinterpret_cast<uint32_t*>(F.ptr) + slot, uint32_t(v.i)); break;
-    case FK_MAX_U32: amax(reinterpret_cast<uint32_t*>(F.ptr) + slot, uint32_t(v.i)); break;
-    case FK_MIN_U64: amin(reinterpret_cast<uint64_t*>(F.ptr) + slot, uint64_t(v.i)); break;
-    case FK_MAX_U64: amax(reinterpret_cast<uint64_t*>(F.ptr) + slot, uint64_t(v.i)); break;
+  switch (kind) {
+    case FK_SUM_I64: aadd(static_cast<uint64_t*>(a), uint64_t(v.i)); break;
+    case FK_SUM_F32: unsafeAtomicAdd(static_cast<float*>(a), v.f); break;
+    case FK_SUM_F64: unsafeAtomicAdd(static_cast<double*>(a), v.d); break;
+    case FK_SUMSQ_I64: aadd(static_cast<uint64_t*>(a), uint64_t(v.i) * uint64_t(v.i)); break;
+    case FK_SUMSQ_F32: unsafeAtomicAdd(static_cast<float*>(a), v.f * v.f); break;
+    case FK_SUMSQ_F64: unsafeAtomicAdd(static_cast<double*>(a), v.d * v.d); break;
+    case FK_MIN_S32: amin(static_cast<int32_t*>(a), int32_t(v.i)); break;
+    case FK_MAX_S32: amax(static_cast<int32_t*>(a), int32_t(v.i)); break;
+    case FK_MIN_S64: amin(static_cast<int64_t*>(a), v.i); break;
+    case FK_MAX_S64: amax(static_cast<int64_t*>(a), v.i); break;
+    case FK_MIN_U32: amin(static_cast<uint32_t*>(a), uint32_t(v.i)); break;
+    case FK_MAX_U32: amax(static_cast<uint32_t*>(a), uint32_t(v.i)); break;
+    case FK_MIN_U64: amin(static_cast<uint64_t*>(a), uint64_t(v.i)); break;
+    case FK_MAX_U64: amax(static_cast<uint64_t*>(a), uint64_t(v.i)); break;
     case FK_MIN_F32:
-      if (!isnan(v.f)) amin(reinterpret_cast<uint32_t*>(F.ptr) + slot, ord32(uint32_t(v.b)));
+      if (!isnan(v.f)) amin(static_cast<uint32_t*>(a), ord32(uint32_t(v.b)));
       break;
     case FK_MAX_F32:
-      if (!isnan(v.f)) amax(reinterpret_cast<uint32_t*>(F.ptr) + slot, ord32(uint32_t(v.b)));
+      if (!isnan(v.f)) amax(static_cast<uint32_t*>(a), ord32(uint32_t(v.b)));
       break;
     case FK_MIN_F64:
-      if (!isnan(v.d)) amin(reinterpret_cast<uint64_t*>(F.ptr) + slot, ord64(v.b));
+      if (!isnan(v.d)) amin(static_cast<uint64_t*>(a), ord64(v.b));
       break;
     case FK_MAX_F64:
-      if (!isnan(v.d)) amax(reinterpret_cast<uint64_t*>(F.ptr) + slot, ord64(v.b));
+      if (!isnan(v.d)) amax(static_cast<uint64_t*>(a), ord64(v.b));
       break;
-    case FK_COUNT: aadd(reinterpret_cast<uint32_t*>(F.ptr) + slot, 1u); break;
+    case FK_COUNT: aadd(static_cast<uint32_t*>(a), 1u); break;
     default: break;
   }
+}
+
+// Accumulate into slot `slot` of a field's global array.
+__device__ __forceinline__ void apply_field(field_desc const& F, int64_t slot, loaded_value const& v)
+{
+  acc_apply(F.kind, static_cast<uint8_t*>(F.ptr) + slot * F.width, v);
 }
 
 __device__ __forceinline__ void store_width(void* data, int64_t g, int w, uint64_t bits)

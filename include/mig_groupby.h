@@ -157,10 +157,11 @@ typedef struct mig_column {
 typedef struct mig_groupby_result mig_groupby_result;
 
 /* ---- batched group-by -------------------------------------------------------
- * Many independent group-bys over different inputs with the same key type,
- * the same value types and the same aggregation requests, executed together:
- * one persistent work-queue kernel aggregates every instance, captured with
- * the kernels around it into one hipGraph that each execute replays.  No
+ * Many independent group-bys over different inputs with the same requests and
+ * aggregations, executed together: one persistent work-queue kernel
+ * aggregates every instance, captured with the kernels around it into one
+ * hipGraph that each execute replays.  Instances may differ in row count,
+ * key type and value types (fixed-width columns; one key column each).  No
  * counterpart in the reference, which runs one aggregate() per table
  * (cpp/src/groupby/groupby.cu:204-221); each instance's result equals what
  * mig_groupby_aggregate returns for it. */

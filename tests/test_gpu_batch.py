@@ -171,24 +171,63 @@ def test_batch_empty_instances():
         assert len(kcols[0][0]) == 0 and all(len(c[0]) == 0 for t in r for c in t)
 
 
+def make_mixed(seed, n_inst, max_rows):
+    """Instances whose key and value types differ (int32/int64/float32/float64),
+    with nulls in some keys and values."""
+    rng = np.random.default_rng(seed)
+    types = [np.int32, np.int64, np.float32, np.float64]
+    out = []
+    for i in range(n_inst):
+        rows = 0 if i == 0 else int(round(math.exp(rng.uniform(0, math.log(max_rows)))))
+        card = max(1, int(round(math.exp(rng.uniform(0, math.log(max(1.0, rows / 2)))))))
+        kt, t0, t1 = (types[j] for j in rng.integers(0, 4, 3))
+        keys = (rng.integers(0, card, rows) - card // 3).astype(kt)
+        kvalid = rng.random(rows) > 0.04 if i % 3 == 0 else None
+
+        def val(t):
+            if np.issubdtype(t, np.integer):
+                return rng.integers(-1_000_000, 1_000_000, rows).astype(t)
+            return (rng.standard_normal(rows) * 100).astype(t)
+        v0valid = rng.random(rows) > 0.05 if i % 2 == 1 else None
+        v1valid = rng.random(rows) > 0.05 if i % 4 == 2 else None
+        out.append((host_col(keys, kvalid), [host_col(val(t0), v0valid), host_col(val(t1), v1valid)]))
+    return out
+
+
+REQ_VAR0 = [(AGG["SUM"], 1), (AGG["MEAN"], 1), (AGG["COUNT_VALID"], 1), (AGG["MIN"], 1), (AGG["MAX"], 1),
+            (AGG["VARIANCE"], 1)]
+REQ_VAR1 = [(AGG["STD"], 0), (AGG["VARIANCE"], 2), (AGG["SUM"], 1), (AGG["COUNT_ALL"], 1)]
+
+
+@pytest.mark.parametrize("null_policy", [0, 1])
+def test_batch_mixed_types_with_variance(null_policy):
+    """Instances with different key and value types in one plan, with
+    variance / std (second pass) next to the first-pass aggregations."""
+    inst = make_mixed(29 + null_policy, 36, 50_000)
+    inst += make_mixed(31, 2, 2)  # tiny instances: var null where count <= ddof
+    plan = run_batch(inst, [REQ_VAR0, REQ_VAR1], null_policy)
+    check_all(plan, inst, [REQ_VAR0, REQ_VAR1], null_policy)
+
+
+def test_batch_mixed_types_options():
+    from cudf_amd import BatchOptions
+    inst = make_mixed(37, 20, 30_000)
+    for opts in (dict(lds_max_groups=-1), dict(block_size=512, tile_rows=1024), dict(use_graph=-1)):
+        plan = run_batch(inst, [REQ_VAR0, REQ_VAR1], 0, BatchOptions(**opts))
+        check_all(plan, inst, [REQ_VAR0, REQ_VAR1], 0)
+
+
 def test_batch_errors():
     from cudf_amd import _lib
+    from oracle import oracle as orc
     inst = make_instances(23, 3, 1_000)
     with pytest.raises(_lib.LogicError):
-        run_batch(inst, [[(AGG["VARIANCE"], 1)], REQ1], 0)
-    mixed = list(inst)
-    k, vs = mixed[2]
-    mixed[2] = (host_col(k.data.astype(np.int64)), vs)
-    with pytest.raises(_lib.LogicError):
-        run_batch(mixed, [REQ0, REQ1], 0)
+        run_batch(inst, [[(AGG["PRODUCT"], 1)], REQ1], 0)
     bad = list(inst)
-    k, vs = bad[1]
-    bad[1] = (k, [host_col(vs[0].data[:-1]) if len(vs[0].data) else vs[0], vs[1]])
-    if len(k.data):
-        with pytest.raises(_lib.LogicError):
-            run_batch(bad, [REQ0, REQ1], 0)
-    from oracle import oracle as orc
-    k, vs = inst[2]
+    k, vs = bad[2]
+    bad[2] = (k, [host_col(vs[0].data[:-1]), vs[1]])
+    with pytest.raises(_lib.LogicError):
+        run_batch(bad, [REQ0, REQ1], 0)
     strs = [(orc.HostColumn.from_strings([b"x%d" % (j % 7) for j in range(len(k.data))]), vs)]
     with pytest.raises(_lib.DataTypeError):
         run_batch(strs, [REQ0, REQ1], 0)
