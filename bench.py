@@ -211,6 +211,160 @@ def bench_batched(n_inst, reps, autotune=True, plain=True):
     return res
 
 
+SERVE_TYPES = ["int32", "int64", "float32", "float64"]
+
+
+def _dev_values(g, n, dt, lo, hi):
+    import torch
+    dev = torch.device("cuda", 0)
+    if dt.startswith("int"):
+        return torch.randint(lo, hi + 1, (n,), dtype=getattr(torch, dt), device=dev, generator=g)
+    return torch.randn(n, dtype=getattr(torch, dt), device=dev, generator=g)
+
+
+def _dev_col(t, valid=None):
+    import torch
+    from cudf_amd import Column, TypeId
+    tid = {torch.int32: TypeId.INT32, torch.int64: TypeId.INT64, torch.float32: TypeId.FLOAT32,
+           torch.float64: TypeId.FLOAT64}[t.dtype]
+    if valid is None:
+        return Column(tid, t.numel(), t.view(torch.uint8))
+    return Column(tid, t.numel(), t.view(torch.uint8), pack_bits(valid), int((~valid).sum().item()))
+
+
+def make_served(n_req, seed=5):
+    """The served request stream: templates 1-3 drawn uniformly, rows
+    log-uniform 1e4..5e7, random key/value dtypes, 0-5% nulls.  Returns
+    (query factories, per-request (template, rows))."""
+    import torch
+    import cudf_amd as ca
+    from cudf_amd import Column, TypeId
+    from cudf_amd.aggregation import Aggregation, Kind
+    rng = np.random.default_rng(seed)
+    dev = torch.device("cuda", 0)
+    g = torch.Generator(device=dev).manual_seed(seed)
+    A = lambda *ks: [Aggregation(k, 1) for k in ks]
+    # the C3 string pool and Zipf(1.1) table over 1M values
+    lens = rng.integers(4, 33, 10_000)
+    pool_chars = torch.from_numpy(rng.integers(33, 127, int(lens.sum()), dtype=np.uint8)).to(dev)
+    pool_off = torch.from_numpy(np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)).to(dev)
+    pool_len = torch.from_numpy(lens.astype(np.int64)).to(dev)
+    w = np.arange(1, 1_000_001, dtype=np.float64) ** -1.1
+    zipf_cdf = torch.from_numpy(np.cumsum(w) / w.sum()).to(dev)
+    factories, shapes = [], []
+    for q in range(n_req):
+        tpl = int(rng.integers(1, 4))
+        n = int(round(np.exp(rng.uniform(np.log(1e4), np.log(5e7)))))
+        nulls = lambda: (torch.rand(n, device=dev, generator=g) >= rng.uniform(0, 0.05))
+        if tpl == 1:
+            kt, vt = rng.choice(SERVE_TYPES, 2)
+            key = torch.randint(0, min(1000, n), (n,), device=dev, generator=g).to(getattr(torch, kt))
+            keys = [_dev_col(key, nulls())]
+            reqs = [(_dev_col(_dev_values(g, n, vt, -10**6, 10**6), nulls()),
+                     (Kind.SUM, Kind.COUNT_VALID, Kind.MIN, Kind.MAX))]
+        elif tpl == 2:
+            kt, ft, it = rng.choice(SERVE_TYPES, 3)
+            key = torch.randint(0, max(1, n // 10), (n,), device=dev, generator=g).to(getattr(torch, kt))
+            keys = [_dev_col(key, torch.rand(n, device=dev, generator=g) >= 0.01)]
+            c2 = (Kind.SUM, Kind.MEAN, Kind.COUNT_VALID, Kind.MIN, Kind.MAX, Kind.VARIANCE)
+            reqs = [(_dev_col(_dev_values(g, n, ft, -10**6, 10**6), nulls()), c2),
+                    (_dev_col(_dev_values(g, n, it, 0, 10**4), nulls()), c2)]
+        else:
+            ft, it = rng.choice(SERVE_TYPES, 2)
+            k1 = torch.randint(1, 101, (n,), dtype=torch.int32, device=dev, generator=g)
+            k2 = torch.searchsorted(zipf_cdf, torch.rand(n, dtype=torch.float64, device=dev, generator=g)) + 1
+            idx = torch.randint(0, 10_000, (n,), device=dev, generator=g)
+            ln = pool_len[idx]
+            offs = torch.zeros(n + 1, dtype=torch.int64, device=dev)
+            offs[1:] = torch.cumsum(ln, 0)
+            total = int(offs[-1].item())
+            src = torch.repeat_interleave(pool_off[idx] - offs[:-1], ln, output_size=total)
+            chars = pool_chars[src + torch.arange(total, device=dev)]
+            k3 = Column(TypeId.STRING, n, chars, None, 0, 0, offs.to(torch.int32))
+            del src, ln, idx
+            keys = [_dev_col(k1), _dev_col(k2.to(torch.int64)), k3]
+            reqs = [(_dev_col(_dev_values(g, n, ft, -10**9, 10**9), nulls()), (Kind.SUM, Kind.COUNT_VALID, Kind.MAX)),
+                    (_dev_col(_dev_values(g, n, it, -10**9, 10**9), nulls()), (Kind.SUM, Kind.COUNT_VALID, Kind.MAX))]
+
+        def factory(keys=keys, reqs=reqs):
+            from cudf_amd.serve import Query
+            return Query(ca.Table(keys), [ca.GroupByRequest(c, A(*ks)) for c, ks in reqs])
+        factories.append(factory)
+        shapes.append((tpl, n))
+    torch.cuda.synchronize()
+    return factories, shapes
+
+
+def bench_served(n_req, loads=(0.5, 1.0, 2.0, 4.0), workers=4):
+    """Plain kernel run serially over the stream, then the continuous-batching
+    server at Poisson arrival rates that are multiples of the serial rate and
+    with every request arriving at once; p50/p99 latency and achieved rate."""
+    import threading
+    import torch
+    from cudf_amd import GroupBy
+    from cudf_amd.serve import GroupByServer, latency_stats
+    factories, shapes = make_served(n_req)
+    rows = sum(n for _, n in shapes)
+    # warm-up of both paths
+    for f in factories[:4]:
+        q = f()
+        GroupBy(q.keys).aggregate(q.requests)
+    torch.cuda.synchronize()
+    groups_serial, t_serial = [], []
+    for f in factories:
+        q = f()
+        t0 = time.perf_counter()
+        k, _ = GroupBy(q.keys).aggregate(q.requests)
+        torch.cuda.synchronize()
+        t_serial.append(time.perf_counter() - t0)
+        groups_serial.append(k.num_rows())
+    serial_qps = n_req / sum(t_serial)
+    lat_serial = np.array(t_serial) * 1e3
+    out = {"workload": f"{n_req} requests from the three templates (1M-row int32 key/int64 value sum,count,"
+                       f"min,max; 100M-row int64 key high-cardinality with var; 50M-row three-column "
+                       f"int32/Zipf int64/string key), rows log-uniform 1e4..5e7 (total {rows}), random "
+                       f"int32/int64/float32/float64 key and value types, 0-5% nulls; inputs resident in HBM",
+           "requests": n_req, "rows": rows, "workers": workers, "memory_budget_GB": 200,
+           "templates": {str(t): sum(1 for s, _ in shapes if s == t) for t in (1, 2, 3)},
+           "plain_serial": {"queries_per_s": serial_qps, "p50_ms": float(np.percentile(lat_serial, 50)),
+                            "p99_ms": float(np.percentile(lat_serial, 99)), "note": "service time only"},
+           "sweep": []}
+    mismatches = 0
+    for load in list(loads) + ["burst"]:
+        queries = [f() for f in factories]
+        rng = np.random.default_rng(11)
+        srv = GroupByServer(workers=workers, memory_budget=200 * 10**9).start()
+        t0 = time.perf_counter()
+        if load == "burst":
+            for q in queries:
+                q.arrival = t0
+                srv.submit(q)
+        else:
+            rate = serial_qps * load
+            at = t0 + np.cumsum(rng.exponential(1.0 / rate, n_req))
+            for q, a in zip(queries, at):
+                d = a - time.perf_counter()
+                if d > 0:
+                    time.sleep(d)
+                q.arrival = float(a)
+                srv.submit(q)
+        res = [q.future.result(timeout=600) for q in queries]
+        srv.stop()
+        mismatches += sum(int(r[0].num_rows() != g) for r, g in zip(res, groups_serial))
+        st = latency_stats(queries)
+        st["offered_qps"] = None if load == "burst" else serial_qps * load
+        st["load"] = load
+        st["units"] = srv.units
+        st["batched_units"] = srv.batched_units
+        out["sweep"].append(st)
+        del res, queries
+    best = max(s["queries_per_s"] for s in out["sweep"])
+    out["max_queries_per_s"] = best
+    out["speedup_vs_plain_serial"] = best / serial_qps
+    out["group_count_mismatches_vs_serial"] = mismatches
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -230,6 +384,10 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain per-instance loop")
     ap.add_argument("--only-batched", action="store_true", help="run the batched workload alone")
+    ap.add_argument("--served-requests", type=int, default=200,
+                    help="requests of the served workload (0 skips it)")
+    ap.add_argument("--served-workers", type=int, default=4)
+    ap.add_argument("--only-served", action="store_true", help="run the served workload alone")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the group-by is measured on a "
@@ -245,6 +403,10 @@ def main():
     if args.only_batched:
         print(json.dumps({"batched": bench_batched(args.batched_instances, args.batched_reps,
                                                    not args.no_autotune, not args.no_plain)}), flush=True)
+        return
+    if args.only_served:
+        print(json.dumps({"served": bench_served(args.served_requests, workers=args.served_workers)}),
+              flush=True)
         return
     (kcol, fcol, icol), _keep = make_c2(args.rows, args.card)
     aggs = lambda: [Aggregation(Kind.SUM), Aggregation(Kind.MEAN), Aggregation(Kind.COUNT_VALID),
@@ -339,6 +501,9 @@ def main():
         torch.cuda.empty_cache()
         line["batched"] = bench_batched(args.batched_instances, args.batched_reps, not args.no_autotune,
                                         not args.no_plain)
+        torch.cuda.empty_cache()
+    if args.served_requests > 0:
+        line["served"] = bench_served(args.served_requests, workers=args.served_workers)
     print(json.dumps(line), flush=True)
 
 

@@ -46,7 +46,8 @@ struct mig_batch_plan {
   std::vector<mig::batch_inst> hinst;
   mig::batch_params P{};
   mig::batch_params* dP = nullptr;
-  std::vector<void*> blocks;
+  std::vector<std::pair<void*, size_t>> blocks;  // from the scratch cache
+  hipStream_t last_stream    = nullptr;           // stream of the last execute
   hipStream_t capture_stream = nullptr;
   hipGraph_t graph           = nullptr;
   hipGraphExec_t exec        = nullptr;
@@ -63,23 +64,22 @@ struct mig_batch_plan {
   }
   void release_layout()
   {
+    // blocks go back to the cache only once the last execute has finished
+    if (executed && !synced && last_stream != nullptr) (void)hipStreamSynchronize(last_stream);
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
     exec  = nullptr;
     graph = nullptr;
-    for (void* p : blocks) (void)hipFree(p);
+    for (auto& b : blocks) mig::scratch_give(dev, b.first, b.second);
     blocks.clear();
     dP = nullptr;
   }
   template <typename T>
   T* alloc(size_t count)
   {
-    void* p = nullptr;
     size_t const bytes = std::max<size_t>(256, (count * sizeof(T) + 255) & ~size_t(255));
-    if (hipMalloc(&p, bytes) != hipSuccess)
-      mig::fail(MIG_ERROR_ALLOC, "batched plan: device allocation of " + std::to_string(bytes) +
-                                   " bytes failed");
-    blocks.push_back(p);
+    void* p            = mig::scratch_take(dev, bytes);
+    blocks.emplace_back(p, bytes);
     return static_cast<T*>(p);
   }
   uint64_t region_rows(int32_t i) const
@@ -221,6 +221,7 @@ void estimate(mig_batch_plan& p, hipStream_t stream)
 {
   int32_t const n = p.n_inst;
   mig_batch_plan tmp;  // owns the sample's device memory
+  tmp.dev = p.dev;
   batch_params& P = p.P;
   P.inst          = tmp.alloc<batch_inst>(size_t(n));
   MIG_HIP_TRY(hipMemcpyAsync(P.inst, p.hinst.data(), size_t(n) * sizeof(batch_inst), hipMemcpyHostToDevice,
@@ -356,6 +357,7 @@ mig_batch_plan* batch_plan_create(mig_batch_instance const* instances, int32_t n
     }
     if (!unknown.empty()) {
       mig_batch_plan tmp;
+      tmp.dev = p.dev;
       auto* d = tmp.alloc<uint32_t>(unknown.size());
       MIG_HIP_TRY(hipMemsetAsync(d, 0, unknown.size() * sizeof(uint32_t), stream));
       for (size_t u = 0; u < unknown.size(); u++) {
@@ -566,8 +568,9 @@ void batch_execute(mig_batch_plan* plan, hipStream_t stream)
   mig_batch_plan& p = *plan;
   if (p.exec) MIG_HIP_TRY(hipGraphLaunch(p.exec, stream));
   else enqueue(p, stream);
-  p.executed = true;
-  p.synced   = false;
+  p.executed    = true;
+  p.synced      = false;
+  p.last_stream = stream;
 }
 
 void batch_synchronize(mig_batch_plan* plan, hipStream_t stream)

@@ -239,6 +239,9 @@ pinned_host& thread_pinned()
 
 // Planning helpers shared with the batched driver (declared in host_plan.hpp).
 
+void* scratch_take(int dev, size_t bytes) { return scratch_cache::get().take(dev, bytes); }
+void scratch_give(int dev, void* p, size_t bytes) { scratch_cache::get().give(dev, p, bytes); }
+
 bool same_view(mig_column_view const& a, mig_column_view const& b)
 {
   return a.type_id == b.type_id && a.size == b.size && a.offset == b.offset && a.data == b.data &&

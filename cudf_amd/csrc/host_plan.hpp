@@ -6,6 +6,12 @@
 
 namespace mig {
 
+// Device blocks of the per-device scratch cache (kept across calls: mapping
+// fresh device memory costs far more than the kernels).  A block is given
+// back only once no queued work uses it.
+void* scratch_take(int dev, size_t bytes);
+void scratch_give(int dev, void* p, size_t bytes);
+
 bool same_view(mig_column_view const& a, mig_column_view const& b);
 // Nulls of a view (its null_count when known, else counted on the device;
 // synchronises `s` in that case).

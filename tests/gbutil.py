@@ -252,3 +252,44 @@ def c3_data(n=50_000_000, seed=0, pool_size=10_000):
     f32 = rng.random(n, dtype=np.float32)
     i64 = rng.integers(-10**9, 10**9, n, endpoint=True, dtype=np.int64)
     return host_col(key1), host_col(key2), key3, host_col(f32), host_col(i64)
+
+
+# ---------------------------------------------------------------------------
+# served request stream: requests drawn from the three plain templates with
+# random key / value dtypes and 0-5% nulls
+
+SERVE_DTYPES = [np.int32, np.int64, np.float32, np.float64]
+C1_AGGS = [(AGG["SUM"], 1), (AGG["COUNT_VALID"], 1), (AGG["MIN"], 1), (AGG["MAX"], 1)]
+C3_AGGS = [(AGG["SUM"], 1), (AGG["COUNT_VALID"], 1), (AGG["MAX"], 1)]
+
+
+def _typed_values(rng, n, dt, int_lo, int_hi):
+    if np.issubdtype(dt, np.integer):
+        return rng.integers(int_lo, int_hi, n, endpoint=True).astype(dt)
+    return rng.standard_normal(n).astype(dt)
+
+
+def _nulls(rng, n):
+    frac = rng.uniform(0.0, 0.05)
+    return (rng.random(n) >= frac) if frac > 0.005 else None
+
+
+def served_request(rng, template: int, n: int):
+    """(keys, requests, null_policy) of one request of template 1, 2 or 3."""
+    if template == 1:
+        kt, vt = (SERVE_DTYPES[i] for i in rng.integers(0, 4, 2))
+        key = rng.integers(0, max(1, min(1000, n)), n).astype(kt)
+        v = _typed_values(rng, n, vt, -10**6, 10**6)
+        return [host_col(key, _nulls(rng, n))], [(host_col(v, _nulls(rng, n)), C1_AGGS)], 0
+    if template == 2:
+        kt, ft, it = (SERVE_DTYPES[i] for i in rng.integers(0, 4, 3))
+        key = rng.integers(0, max(1, n // 10), n).astype(kt)
+        f = _typed_values(rng, n, ft, -10**6, 10**6)
+        i = _typed_values(rng, n, it, 0, 10**4)
+        return ([host_col(key, rng.random(n) >= 0.01)],
+                [(host_col(f, _nulls(rng, n)), C2_AGGS), (host_col(i, _nulls(rng, n)), C2_AGGS)], 0)
+    k1, k2, k3, f32, i64 = c3_data(n=n, seed=int(rng.integers(1 << 30)), pool_size=min(10_000, max(1, n)))
+    ft, it = (SERVE_DTYPES[i] for i in rng.integers(0, 4, 2))
+    f = host_col(f32.data.astype(ft), _nulls(rng, n))
+    i = host_col(i64.data.astype(it) if np.issubdtype(it, np.integer) else i64.data.astype(it), _nulls(rng, n))
+    return [k1, k2, k3], [(f, C3_AGGS), (i, C3_AGGS)], 0
