@@ -648,22 +648,35 @@ void launch_gather_string_chars(dcol const& src, uint32_t const* rep_rows, int64
                      out_chars);
 }
 
-// Number of valid rows among [0, n) of a column view (null_count == -1 input).
+// Number of valid rows among [0, n) of a column view: popcount of the mask
+// words covering bits [mask_offset, mask_offset + n), the partial first and
+// last words masked.
 __global__ void __launch_bounds__(BLOCK) k_count_valid(dcol c, int64_t n, uint32_t* out)
 {
   __shared__ uint32_t red[BLOCK / 64];
+  if (c.mask == nullptr) {  // no mask: every row is valid
+    if (blockIdx.x == 0 && threadIdx.x == 0) aadd(out, uint32_t(n));
+    return;
+  }
+  int64_t const b0 = c.mask_offset, b1 = int64_t(c.mask_offset) + n;
+  int64_t const w0 = b0 >> 5, w1 = (b1 + 31) >> 5;
   uint32_t cnt = 0;
-  for (int64_t r = int64_t(blockIdx.x) * BLOCK + threadIdx.x; r < n; r += int64_t(gridDim.x) * BLOCK)
-    cnt += dcol_valid(c, r) ? 1u : 0u;
+  for (int64_t w = w0 + int64_t(blockIdx.x) * BLOCK + threadIdx.x; w < w1; w += int64_t(gridDim.x) * BLOCK) {
+    uint32_t bits = c.mask[w];
+    if (w == w0) bits &= ~0u << (b0 & 31);
+    if (w == w1 - 1 && (b1 & 31) != 0) bits &= (1u << (b1 & 31)) - 1u;
+    cnt += __popc(bits);
+  }
   cnt = block_sum_u32(cnt, red);
-  if (threadIdx.x == 0) aadd(out, cnt);
+  if (threadIdx.x == 0 && cnt != 0) aadd(out, cnt);
 }
 
 void launch_count_nulls(dcol const& c, int64_t n, uint32_t* d_count, hipStream_t s)
 {
   if (n <= 0) return;
-  int64_t blocks = (n + BLOCK - 1) / BLOCK;
-  if (blocks > 4096) blocks = 4096;
+  int64_t const words = (int64_t(c.mask_offset & 31) + n + 31) / 32;
+  int64_t blocks      = c.mask == nullptr ? 1 : (words + BLOCK - 1) / BLOCK;
+  if (blocks > 1024) blocks = 1024;
   hipLaunchKernelGGL(k_count_valid, dim3(blocks), dim3(BLOCK), 0, s, c, n, d_count);
 }
 

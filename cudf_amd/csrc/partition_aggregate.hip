@@ -23,75 +23,97 @@ namespace mig {
 
 constexpr int WG_SCOPE = __HIP_MEMORY_SCOPE_WORKGROUP;
 
-__device__ __forceinline__ void apply_field_lds(field_desc const& F, uint8_t* lds, int id,
-                                                loaded_value const& v)
+constexpr int RPA_CACHED = 8;  // rows per thread per chunk; their slots stay in registers
+
+// Apply `f(k)` to the rows k of a chunk whose bit is set in `ok` (the loop is
+// unrolled; the caller's switch on the accumulator kind stays outside it).
+template <typename F>
+__device__ __forceinline__ void for_rows(uint32_t ok, F&& f)
 {
+#pragma unroll
+  for (int k = 0; k < RPA_CACHED; k++)
+    if ((ok >> k) & 1u) f(k);
+}
+
+// One accumulator over the rows of a chunk: `c[k]` holds row k's value in
+// canonical form (integers sign- or zero-extended to 64 bits, float bits
+// unchanged), `slot[k]` its group slot, bit k of `ok` whether it counts.
+__device__ __forceinline__ void acc_rows_lds(field_desc const& F, uint8_t* lds, int const (&slot)[RPA_CACHED],
+                                             uint32_t ok, uint64_t const (&c)[RPA_CACHED])
+{
+  uint8_t* const base = lds + F.lds_off;
+  auto u64 = [&](int k) { return reinterpret_cast<uint64_t*>(base) + slot[k]; };
+  auto u32 = [&](int k) { return reinterpret_cast<uint32_t*>(base) + slot[k]; };
+  auto fl  = [&](int k) { return __uint_as_float(uint32_t(c[k])); };
+  auto db  = [&](int k) { return __longlong_as_double(int64_t(c[k])); };
+  constexpr int R = __ATOMIC_RELAXED;
   switch (F.kind) {
-    case FK_SUM_I64:
-      __hip_atomic_fetch_add(lds_at<uint64_t>(lds, F.lds_off, id), uint64_t(v.i), __ATOMIC_RELAXED, WG_SCOPE);
-      break;
+    case FK_SUM_I64: for_rows(ok, [&](int k) { __hip_atomic_fetch_add(u64(k), c[k], R, WG_SCOPE); }); break;
     case FK_SUM_F32:
-      __hip_atomic_fetch_add(lds_at<float>(lds, F.lds_off, id), v.f, __ATOMIC_RELAXED, WG_SCOPE);
+      for_rows(ok, [&](int k) { __hip_atomic_fetch_add(reinterpret_cast<float*>(u32(k)), fl(k), R, WG_SCOPE); });
       break;
     case FK_SUM_F64:
-      __hip_atomic_fetch_add(lds_at<double>(lds, F.lds_off, id), v.d, __ATOMIC_RELAXED, WG_SCOPE);
+      for_rows(ok, [&](int k) { __hip_atomic_fetch_add(reinterpret_cast<double*>(u64(k)), db(k), R, WG_SCOPE); });
       break;
     case FK_SUMSQ_I64:
-      __hip_atomic_fetch_add(lds_at<uint64_t>(lds, F.lds_off, id), uint64_t(v.i) * uint64_t(v.i),
-                             __ATOMIC_RELAXED, WG_SCOPE);
+      for_rows(ok, [&](int k) { __hip_atomic_fetch_add(u64(k), c[k] * c[k], R, WG_SCOPE); });
       break;
     case FK_SUMSQ_F32:
-      __hip_atomic_fetch_add(lds_at<float>(lds, F.lds_off, id), v.f * v.f, __ATOMIC_RELAXED, WG_SCOPE);
+      for_rows(ok, [&](int k) {
+        __hip_atomic_fetch_add(reinterpret_cast<float*>(u32(k)), fl(k) * fl(k), R, WG_SCOPE);
+      });
       break;
     case FK_SUMSQ_F64:
-      __hip_atomic_fetch_add(lds_at<double>(lds, F.lds_off, id), v.d * v.d, __ATOMIC_RELAXED, WG_SCOPE);
+      for_rows(ok, [&](int k) {
+        __hip_atomic_fetch_add(reinterpret_cast<double*>(u64(k)), db(k) * db(k), R, WG_SCOPE);
+      });
       break;
     case FK_MIN_S32:
-      __hip_atomic_fetch_min(lds_at<int32_t>(lds, F.lds_off, id), int32_t(v.i), __ATOMIC_RELAXED, WG_SCOPE);
+      for_rows(ok, [&](int k) {
+        __hip_atomic_fetch_min(reinterpret_cast<int32_t*>(u32(k)), int32_t(c[k]), R, WG_SCOPE);
+      });
       break;
     case FK_MAX_S32:
-      __hip_atomic_fetch_max(lds_at<int32_t>(lds, F.lds_off, id), int32_t(v.i), __ATOMIC_RELAXED, WG_SCOPE);
+      for_rows(ok, [&](int k) {
+        __hip_atomic_fetch_max(reinterpret_cast<int32_t*>(u32(k)), int32_t(c[k]), R, WG_SCOPE);
+      });
       break;
     case FK_MIN_S64:
-      __hip_atomic_fetch_min(lds_at<int64_t>(lds, F.lds_off, id), v.i, __ATOMIC_RELAXED, WG_SCOPE);
+      for_rows(ok, [&](int k) {
+        __hip_atomic_fetch_min(reinterpret_cast<int64_t*>(u64(k)), int64_t(c[k]), R, WG_SCOPE);
+      });
       break;
     case FK_MAX_S64:
-      __hip_atomic_fetch_max(lds_at<int64_t>(lds, F.lds_off, id), v.i, __ATOMIC_RELAXED, WG_SCOPE);
+      for_rows(ok, [&](int k) {
+        __hip_atomic_fetch_max(reinterpret_cast<int64_t*>(u64(k)), int64_t(c[k]), R, WG_SCOPE);
+      });
       break;
-    case FK_MIN_U32:
-      __hip_atomic_fetch_min(lds_at<uint32_t>(lds, F.lds_off, id), uint32_t(v.i), __ATOMIC_RELAXED, WG_SCOPE);
-      break;
-    case FK_MAX_U32:
-      __hip_atomic_fetch_max(lds_at<uint32_t>(lds, F.lds_off, id), uint32_t(v.i), __ATOMIC_RELAXED, WG_SCOPE);
-      break;
-    case FK_MIN_U64:
-      __hip_atomic_fetch_min(lds_at<uint64_t>(lds, F.lds_off, id), uint64_t(v.i), __ATOMIC_RELAXED, WG_SCOPE);
-      break;
-    case FK_MAX_U64:
-      __hip_atomic_fetch_max(lds_at<uint64_t>(lds, F.lds_off, id), uint64_t(v.i), __ATOMIC_RELAXED, WG_SCOPE);
-      break;
+    case FK_MIN_U32: for_rows(ok, [&](int k) { __hip_atomic_fetch_min(u32(k), uint32_t(c[k]), R, WG_SCOPE); }); break;
+    case FK_MAX_U32: for_rows(ok, [&](int k) { __hip_atomic_fetch_max(u32(k), uint32_t(c[k]), R, WG_SCOPE); }); break;
+    case FK_MIN_U64: for_rows(ok, [&](int k) { __hip_atomic_fetch_min(u64(k), c[k], R, WG_SCOPE); }); break;
+    case FK_MAX_U64: for_rows(ok, [&](int k) { __hip_atomic_fetch_max(u64(k), c[k], R, WG_SCOPE); }); break;
     case FK_MIN_F32:
-      if (!isnan(v.f))
-        __hip_atomic_fetch_min(lds_at<uint32_t>(lds, F.lds_off, id), ord32(uint32_t(v.b)), __ATOMIC_RELAXED,
-                               WG_SCOPE);
+      for_rows(ok, [&](int k) {
+        if (!isnan(fl(k))) __hip_atomic_fetch_min(u32(k), ord32(uint32_t(c[k])), R, WG_SCOPE);
+      });
       break;
     case FK_MAX_F32:
-      if (!isnan(v.f))
-        __hip_atomic_fetch_max(lds_at<uint32_t>(lds, F.lds_off, id), ord32(uint32_t(v.b)), __ATOMIC_RELAXED,
-                               WG_SCOPE);
+      for_rows(ok, [&](int k) {
+        if (!isnan(fl(k))) __hip_atomic_fetch_max(u32(k), ord32(uint32_t(c[k])), R, WG_SCOPE);
+      });
       break;
     case FK_MIN_F64:
-      if (!isnan(v.d))
-        __hip_atomic_fetch_min(lds_at<uint64_t>(lds, F.lds_off, id), ord64(v.b), __ATOMIC_RELAXED, WG_SCOPE);
+      for_rows(ok, [&](int k) {
+        if (!isnan(db(k))) __hip_atomic_fetch_min(u64(k), ord64(c[k]), R, WG_SCOPE);
+      });
       break;
     case FK_MAX_F64:
-      if (!isnan(v.d))
-        __hip_atomic_fetch_max(lds_at<uint64_t>(lds, F.lds_off, id), ord64(v.b), __ATOMIC_RELAXED, WG_SCOPE);
+      for_rows(ok, [&](int k) {
+        if (!isnan(db(k))) __hip_atomic_fetch_max(u64(k), ord64(c[k]), R, WG_SCOPE);
+      });
       break;
-    case FK_COUNT:
-      __hip_atomic_fetch_add(lds_at<uint32_t>(lds, F.lds_off, id), 1u, __ATOMIC_RELAXED, WG_SCOPE);
-      break;
-    default: break;
+    case FK_COUNT: for_rows(ok, [&](int k) { __hip_atomic_fetch_add(u32(k), 1u, R, WG_SCOPE); }); break;
+    default: break;  // FK_M2: pass B
   }
 }
 
@@ -185,10 +207,8 @@ __device__ __forceinline__ int row_slot(gb_params const& P, rpa_params const& R,
   return -1;
 }
 
-constexpr int RPA_CACHED = 8;  // rows per thread per chunk; their slots stay in registers
-
 // Raw 8 bytes of element r of a partition value column (narrow types are
-// zero-extended; load_value-style decoding happens in decode_raw).
+// zero-extended; canon_rows gives the canonical form).
 __device__ __forceinline__ uint64_t load_raw(uint8_t const* base, int w, int64_t r)
 {
   switch (w) {
@@ -199,24 +219,38 @@ __device__ __forceinline__ uint64_t load_raw(uint8_t const* base, int w, int64_t
   }
 }
 
-__device__ __forceinline__ loaded_value decode_raw(int32_t st, uint64_t raw)
+// Canonical form of a chunk's raw elements of storage `st`: integers sign-
+// or zero-extended to 64 bits (bool as 0/1), float bits unchanged.
+__device__ __forceinline__ void canon_rows(int32_t st, uint64_t (&c)[RPA_CACHED])
 {
-  loaded_value v{0, 0.0, 0.0f, 0};
   switch (st) {
-    case ST_I8: v.i = int8_t(raw); v.d = double(v.i); break;
-    case ST_I16: v.i = int16_t(raw); v.d = double(v.i); break;
-    case ST_I32: v.i = int32_t(raw); v.d = double(v.i); break;
-    case ST_I64: v.i = int64_t(raw); v.d = double(v.i); break;
-    case ST_BOOL: v.i = raw != 0; v.d = double(v.i); break;
-    case ST_U8:
-    case ST_U16:
-    case ST_U32: v.i = int64_t(raw); v.d = double(v.i); break;
-    case ST_U64: v.i = int64_t(raw); v.d = double(raw); break;
-    case ST_F32: v.b = raw; v.f = __uint_as_float(uint32_t(raw)); v.d = double(v.f); break;
-    case ST_F64: v.b = raw; v.d = __longlong_as_double(int64_t(raw)); break;
+    case ST_I8:
+#pragma unroll
+      for (int k = 0; k < RPA_CACHED; k++) c[k] = uint64_t(int64_t(int8_t(c[k])));
+      break;
+    case ST_I16:
+#pragma unroll
+      for (int k = 0; k < RPA_CACHED; k++) c[k] = uint64_t(int64_t(int16_t(c[k])));
+      break;
+    case ST_I32:
+#pragma unroll
+      for (int k = 0; k < RPA_CACHED; k++) c[k] = uint64_t(int64_t(int32_t(c[k])));
+      break;
+    case ST_BOOL:
+#pragma unroll
+      for (int k = 0; k < RPA_CACHED; k++) c[k] = c[k] != 0;
+      break;
     default: break;
   }
-  return v;
+}
+
+// A canonical element as double.
+__device__ __forceinline__ double canon_double(int32_t st, uint64_t c)
+{
+  if (st == ST_F64) return __longlong_as_double(int64_t(c));
+  if (st == ST_F32) return double(__uint_as_float(uint32_t(c)));
+  if (st == ST_U64) return double(c);
+  return double(int64_t(c));
 }
 
 // Slots of the rows [r0 + tid + k * block) of one chunk: all keys and flags
@@ -224,9 +258,10 @@ __device__ __forceinline__ loaded_value decode_raw(int32_t st, uint64_t raw)
 template <bool INSERT>
 __device__ __forceinline__ void chunk_slots(gb_params const& P, rpa_params const& R, uint64_t* tkeys,
                                             rpa_misc* misc, int S, int64_t r0, int64_t e,
-                                            int (&slot)[RPA_CACHED], uint8_t (&fl)[RPA_CACHED])
+                                            int (&slot)[RPA_CACHED], uint64_t& flp)
 {
   uint64_t key[RPA_CACHED];
+  uint8_t fl[RPA_CACHED];
 #pragma unroll
   for (int k = 0; k < RPA_CACHED; k++) {
     int64_t const r = r0 + int64_t(k) * RPA_AGG_BLOCK;
@@ -240,48 +275,55 @@ __device__ __forceinline__ void chunk_slots(gb_params const& P, rpa_params const
     if (r < e && !__hip_atomic_load(&misc->overflow, __ATOMIC_RELAXED, WG_SCOPE))
       slot[k] = row_slot<INSERT>(P, R, tkeys, misc, S, fl[k], key[k]);
   }
+  flp = 0;
+#pragma unroll
+  for (int k = 0; k < RPA_CACHED; k++) flp |= uint64_t(fl[k]) << (8 * k);
 }
 
-// Pass A of one chunk: every non-variance accumulator, column by column, with
-// the chunk's values of a column loaded together.
+// Pass A of one chunk: every non-variance accumulator, column by column,
+// with the chunk's values of a column loaded together and each accumulator
+// applied to all rows of the chunk at once.
 __device__ __forceinline__ void chunk_accumulate(gb_params const& P, rpa_params const& R, uint8_t* lds,
                                                  int64_t r0, int64_t e, int const (&slot)[RPA_CACHED],
-                                                 uint8_t const (&fl)[RPA_CACHED])
+                                                 uint64_t flp)
 {
-  if (R.count_all_off >= 0) {
+  uint32_t live = 0;
 #pragma unroll
-    for (int k = 0; k < RPA_CACHED; k++)
-      if (slot[k] >= 0)
-        __hip_atomic_fetch_add(lds_at<uint32_t>(lds, R.count_all_off, slot[k]), 1u, __ATOMIC_RELAXED, WG_SCOPE);
-  }
+  for (int k = 0; k < RPA_CACHED; k++) live |= uint32_t(slot[k] >= 0) << k;
+  if (R.count_all_off >= 0)
+    for_rows(live, [&](int k) {
+      __hip_atomic_fetch_add(lds_at<uint32_t>(lds, R.count_all_off, slot[k]), 1u, __ATOMIC_RELAXED, WG_SCOPE);
+    });
   for (int v = 0; v < P.num_vals; v++) {
     int const fb = P.val_field_begin[v], fe = P.val_field_begin[v + 1];
     if (fb == fe) continue;
     int const bit = R.val_flag_bit[v];
     int const st  = P.vals[v].storage;
     int const w   = storage_width(st);
-    uint64_t raw[RPA_CACHED];
+    uint64_t c[RPA_CACHED];
 #pragma unroll
     for (int k = 0; k < RPA_CACHED; k++) {
       int64_t const r = r0 + int64_t(k) * RPA_AGG_BLOCK;
-      raw[k]          = (r < e) ? load_raw(R.valsB[v], w, r) : 0;
+      c[k]            = (r < e) ? load_raw(R.valsB[v], w, r) : 0;
     }
+    canon_rows(st, c);
+    uint32_t ok = live;
+    if (bit >= 0) {
 #pragma unroll
-    for (int k = 0; k < RPA_CACHED; k++) {
-      if (slot[k] < 0) continue;
-      if (bit >= 0 && !((fl[k] >> bit) & 1)) continue;
-      loaded_value const val = decode_raw(st, raw[k]);
-      for (int f = fb; f < fe; f++)
-        if (P.fields[f].kind != FK_M2) apply_field_lds(P.fields[f], lds, slot[k], val);
+      for (int k = 0; k < RPA_CACHED; k++) ok &= ~(uint32_t(((flp >> (8 * k + bit)) & 1) == 0) << k);
     }
+    for (int f = fb; f < fe; f++) acc_rows_lds(P.fields[f], lds, slot, ok, c);
   }
 }
 
 // Pass B of one chunk: squared deviations from the group mean.
 __device__ __forceinline__ void chunk_deviations(gb_params const& P, rpa_params const& R, uint8_t* lds,
                                                  int64_t r0, int64_t e, int const (&slot)[RPA_CACHED],
-                                                 uint8_t const (&fl)[RPA_CACHED])
+                                                 uint64_t flp)
 {
+  uint32_t live = 0;
+#pragma unroll
+  for (int k = 0; k < RPA_CACHED; k++) live |= uint32_t(slot[k] >= 0) << k;
   for (int v = 0; v < P.num_vals; v++) {
     int const fb = P.val_field_begin[v], fe = P.val_field_begin[v + 1];
     int m2 = -1;
@@ -292,26 +334,29 @@ __device__ __forceinline__ void chunk_deviations(gb_params const& P, rpa_params 
     int const st        = P.vals[v].storage;
     int const w         = storage_width(st);
     field_desc const& F = P.fields[m2];
-    uint64_t raw[RPA_CACHED];
+    uint64_t c[RPA_CACHED];
 #pragma unroll
     for (int k = 0; k < RPA_CACHED; k++) {
       int64_t const r = r0 + int64_t(k) * RPA_AGG_BLOCK;
-      raw[k]          = (r < e) ? load_raw(R.valsB[v], w, r) : 0;
+      c[k]            = (r < e) ? load_raw(R.valsB[v], w, r) : 0;
     }
+    canon_rows(st, c);
+    uint32_t ok = live;
+    if (bit >= 0) {
 #pragma unroll
-    for (int k = 0; k < RPA_CACHED; k++) {
-      if (slot[k] < 0) continue;
-      if (bit >= 0 && !((fl[k] >> bit) & 1)) continue;
-      double const x    = decode_raw(st, raw[k]).d;
+      for (int k = 0; k < RPA_CACHED; k++) ok &= ~(uint32_t(((flp >> (8 * k + bit)) & 1) == 0) << k);
+    }
+    for_rows(ok, [&](int k) {
+      double const x    = canon_double(st, c[k]);
       double const cnt  = double(lds_count(P, R, lds, F.aux_count, slot[k]));
       double const mean = lds_field_double(P.fields[F.aux_sum], lds, slot[k]) / cnt;
       double const d    = x - mean;
       __hip_atomic_fetch_add(lds_at<double>(lds, F.lds_off, slot[k]), d * d, __ATOMIC_RELAXED, WG_SCOPE);
-    }
+    });
   }
 }
 
-__global__ void __launch_bounds__(RPA_AGG_BLOCK)
+__global__ void __launch_bounds__(RPA_AGG_BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
   k_rpa_aggregate(gb_params const* __restrict__ Pp, rpa_params const* __restrict__ Rp)
 {
   extern __shared__ __attribute__((aligned(16))) uint8_t lds[];
@@ -351,14 +396,20 @@ __global__ void __launch_bounds__(RPA_AGG_BLOCK)
   // ---- pass A: find-or-insert and accumulate, chunk by chunk; the slots of
   // the first chunk stay in registers for pass B
   constexpr int64_t CHUNK = int64_t(RPA_CACHED) * RPA_AGG_BLOCK;
-  int slot0[RPA_CACHED];
-  uint8_t fl0[RPA_CACHED];
-  chunk_slots<true>(P, R, tkeys, misc, S, s + tid, e, slot0, fl0);
-  chunk_accumulate(P, R, lds, s + tid, e, slot0, fl0);
+  uint32_t sp0[RPA_CACHED / 2];  // first chunk's slots + 1, two 16-bit halves per word
+  uint64_t fl0;
+  {
+    int slot0[RPA_CACHED];
+    chunk_slots<true>(P, R, tkeys, misc, S, s + tid, e, slot0, fl0);
+    chunk_accumulate(P, R, lds, s + tid, e, slot0, fl0);
+#pragma unroll
+    for (int k = 0; k < RPA_CACHED / 2; k++)
+      sp0[k] = uint32_t(slot0[2 * k] + 1) | (uint32_t(slot0[2 * k + 1] + 1) << 16);
+  }
   for (int64_t c = s + CHUNK; c < e; c += CHUNK) {
     if (__hip_atomic_load(&misc->overflow, __ATOMIC_RELAXED, WG_SCOPE)) break;
     int slot[RPA_CACHED];
-    uint8_t fl[RPA_CACHED];
+    uint64_t fl;
     chunk_slots<true>(P, R, tkeys, misc, S, c + tid, e, slot, fl);
     chunk_accumulate(P, R, lds, c + tid, e, slot, fl);
   }
@@ -370,10 +421,13 @@ __global__ void __launch_bounds__(RPA_AGG_BLOCK)
 
   // ---- pass B: squared deviations from the group mean
   if (P.need_second_pass) {
+    int slot0[RPA_CACHED];
+#pragma unroll
+    for (int k = 0; k < RPA_CACHED; k++) slot0[k] = int((sp0[k / 2] >> (16 * (k & 1))) & 0xffffu) - 1;
     chunk_deviations(P, R, lds, s + tid, e, slot0, fl0);
     for (int64_t c = s + CHUNK; c < e; c += CHUNK) {
       int slot[RPA_CACHED];
-      uint8_t fl[RPA_CACHED];
+      uint64_t fl;
       chunk_slots<false>(P, R, tkeys, misc, S, c + tid, e, slot, fl);
       chunk_deviations(P, R, lds, c + tid, e, slot, fl);
     }

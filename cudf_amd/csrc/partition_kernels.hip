@@ -57,15 +57,22 @@ __global__ void __launch_bounds__(RPA_HIST_BLOCK)
   for (int i = threadIdx.x; i < B; i += RPA_HIST_BLOCK) out[i] = hist[i];
 }
 
+// Column sums of the per-block histogram rows: blockIdx.y takes one slice of
+// the rows and adds its partial sums into the (zeroed) histogram.
+constexpr int RPA_HIST_SLICES = 16;
+
 __global__ void __launch_bounds__(RPA_SMALL_BLOCK) k_rpa_hist_reduce(rpa_params const* __restrict__ Rp)
 {
   rpa_params const& R = *Rp;
   int const B         = 1 << R.log_bins;
   int const b         = blockIdx.x * RPA_SMALL_BLOCK + threadIdx.x;
   if (b >= B) return;
-  uint32_t s = 0;
-  for (int k = 0; k < R.hist_blocks; k++) s += R.hist_partial[int64_t(k) * B + b];
-  R.hist[b] = s;
+  int const per = (R.hist_blocks + RPA_HIST_SLICES - 1) / RPA_HIST_SLICES;
+  int const k0  = int(blockIdx.y) * per;
+  int const k1  = (k0 + per < R.hist_blocks) ? k0 + per : R.hist_blocks;
+  uint32_t s    = 0;
+  for (int k = k0; k < k1; k++) s += R.hist_partial[int64_t(k) * B + b];
+  if (s) aadd(R.hist + b, s);
 }
 
 void launch_rpa_hist(gb_params const* dP, rpa_params const* dR, rpa_params const& hR, int64_t n,
@@ -77,7 +84,7 @@ void launch_rpa_hist(gb_params const* dP, rpa_params const* dR, rpa_params const
                                     hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
   hipLaunchKernelGGL(k_rpa_hist, dim3(hR.hist_blocks), dim3(RPA_HIST_BLOCK), lds, s, dP, dR, n);
   int64_t const B = int64_t(1) << hR.log_bins;
-  hipLaunchKernelGGL(k_rpa_hist_reduce, dim3((B + RPA_SMALL_BLOCK - 1) / RPA_SMALL_BLOCK),
+  hipLaunchKernelGGL(k_rpa_hist_reduce, dim3((B + RPA_SMALL_BLOCK - 1) / RPA_SMALL_BLOCK, RPA_HIST_SLICES),
                      dim3(RPA_SMALL_BLOCK), 0, s, dR);
 }
 
@@ -287,21 +294,36 @@ static void allow_large_lds(void const* fn, int bytes)
     MIG_HIP_TRY(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
 }
 
+// Workgroups of a scatter kernel that are resident at once: the tiles are
+// strided over the grid, so a grid larger than what fits would leave its last
+// workgroups to run after the others (LDS: 160 KB per CU; 4 x 512 threads).
+static int64_t resident_grid(void const* fn, int dyn_lds, int num_cus)
+{
+  hipFuncAttributes fa{};
+  MIG_HIP_TRY(hipFuncGetAttributes(&fa, fn));
+  int64_t const per_wg = int64_t(dyn_lds) + int64_t(fa.sharedSizeBytes);
+  int64_t per_cu       = per_wg > 0 ? (160 * 1024) / per_wg : 4;
+  if (per_cu > 4) per_cu = 4;
+  if (per_cu < 1) per_cu = 1;
+  return int64_t(num_cus) * per_cu;
+}
+
 void launch_rpa_scatter1(gb_params const* dP, rpa_params const* dR, rpa_params const& hR, int64_t n,
                          int num_cus, hipStream_t s)
 {
+  void const* fn = reinterpret_cast<void const*>(&k_rpa_scatter<false>);
   int64_t tiles  = (n + hR.tile_rows - 1) / hR.tile_rows;
-  int64_t blocks = int64_t(num_cus) * 4;
+  int64_t blocks = resident_grid(fn, hR.stage_bytes, num_cus);
   if (blocks > tiles) blocks = tiles;
   if (blocks < 1) blocks = 1;
-  allow_large_lds(reinterpret_cast<void const*>(&k_rpa_scatter<false>), hR.stage_bytes);
+  allow_large_lds(fn, hR.stage_bytes);
   hipLaunchKernelGGL(k_rpa_scatter<false>, dim3(blocks), dim3(RPA_SCATTER_BLOCK), hR.stage_bytes, s, dP, dR, n);
 }
 
 void launch_rpa_scatter2(gb_params const* dP, rpa_params const* dR, rpa_params const& hR,
                          int num_cus, hipStream_t s)
 {
-  int64_t blocks = int64_t(num_cus) * 4;
+  int64_t blocks = resident_grid(reinterpret_cast<void const*>(&k_rpa_scatter<true>), hR.stage_bytes, num_cus);
   if (blocks > hR.num_tiles1) blocks = hR.num_tiles1;
   if (blocks < 1) blocks = 1;
   allow_large_lds(reinterpret_cast<void const*>(&k_rpa_scatter<true>), hR.stage_bytes);
