@@ -327,7 +327,10 @@ def bench_served(n_req, loads=(0.5, 1.0, 2.0, 4.0), workers=4):
            "requests": n_req, "rows": rows, "workers": workers, "memory_budget_GB": 200,
            "templates": {str(t): sum(1 for s, _ in shapes if s == t) for t in (1, 2, 3)},
            "plain_serial": {"queries_per_s": serial_qps, "p50_ms": float(np.percentile(lat_serial, 50)),
-                            "p99_ms": float(np.percentile(lat_serial, 99)), "note": "service time only"},
+                            "p99_ms": float(np.percentile(lat_serial, 99)), "note": "service time only",
+                            "ms_by_template": {str(t): round(1e3 * sum(x for x, (s, _) in zip(t_serial, shapes)
+                                                                       if s == t), 2) for t in (1, 2, 3)},
+                            "rows_by_template": {str(t): sum(n for s, n in shapes if s == t) for t in (1, 2, 3)}},
            "sweep": []}
     mismatches = 0
     for load in list(loads) + ["burst"]:

@@ -215,71 +215,6 @@ __device__ __forceinline__ int64_t b_null_slot(batch_params const& B, batch_inst
   return int64_t(I.slot_base + I.cap + 1);
 }
 
-// Combine a pre-aggregated partial (same encoding as the table) into `a`.
-__device__ __forceinline__ void acc_combine(int32_t kind, void* a, uint64_t part)
-{
-  switch (kind) {
-    case FK_SUM_I64:
-    case FK_SUMSQ_I64: aadd(static_cast<uint64_t*>(a), part); break;
-    case FK_SUM_F32:
-    case FK_SUMSQ_F32: unsafeAtomicAdd(static_cast<float*>(a), __uint_as_float(uint32_t(part))); break;
-    case FK_SUM_F64:
-    case FK_SUMSQ_F64: unsafeAtomicAdd(static_cast<double*>(a), __longlong_as_double(int64_t(part))); break;
-    case FK_MIN_S32: amin(static_cast<int32_t*>(a), int32_t(part)); break;
-    case FK_MAX_S32: amax(static_cast<int32_t*>(a), int32_t(part)); break;
-    case FK_MIN_S64: amin(static_cast<int64_t*>(a), int64_t(part)); break;
-    case FK_MAX_S64: amax(static_cast<int64_t*>(a), int64_t(part)); break;
-    case FK_MIN_U32:
-    case FK_MIN_F32: amin(static_cast<uint32_t*>(a), uint32_t(part)); break;
-    case FK_MAX_U32:
-    case FK_MAX_F32: amax(static_cast<uint32_t*>(a), uint32_t(part)); break;
-    case FK_MIN_U64:
-    case FK_MIN_F64: amin(static_cast<uint64_t*>(a), part); break;
-    case FK_MAX_U64:
-    case FK_MAX_F64: amax(static_cast<uint64_t*>(a), part); break;
-    case FK_COUNT: aadd(static_cast<uint32_t*>(a), uint32_t(part)); break;
-    default: break;
-  }
-}
-
-// The per-row update into LDS partials (workgroup-scope atomics).
-__device__ __forceinline__ void lds_apply(int32_t kind, void* a, loaded_value const& v)
-{
-  constexpr int WG = __HIP_MEMORY_SCOPE_WORKGROUP;
-  switch (kind) {
-    case FK_SUM_I64: __hip_atomic_fetch_add(static_cast<uint64_t*>(a), uint64_t(v.i), __ATOMIC_RELAXED, WG); break;
-    case FK_SUM_F32: __hip_atomic_fetch_add(static_cast<float*>(a), v.f, __ATOMIC_RELAXED, WG); break;
-    case FK_SUM_F64: __hip_atomic_fetch_add(static_cast<double*>(a), v.d, __ATOMIC_RELAXED, WG); break;
-    case FK_SUMSQ_I64:
-      __hip_atomic_fetch_add(static_cast<uint64_t*>(a), uint64_t(v.i) * uint64_t(v.i), __ATOMIC_RELAXED, WG);
-      break;
-    case FK_SUMSQ_F32: __hip_atomic_fetch_add(static_cast<float*>(a), v.f * v.f, __ATOMIC_RELAXED, WG); break;
-    case FK_SUMSQ_F64: __hip_atomic_fetch_add(static_cast<double*>(a), v.d * v.d, __ATOMIC_RELAXED, WG); break;
-    case FK_MIN_S32: __hip_atomic_fetch_min(static_cast<int32_t*>(a), int32_t(v.i), __ATOMIC_RELAXED, WG); break;
-    case FK_MAX_S32: __hip_atomic_fetch_max(static_cast<int32_t*>(a), int32_t(v.i), __ATOMIC_RELAXED, WG); break;
-    case FK_MIN_S64: __hip_atomic_fetch_min(static_cast<int64_t*>(a), v.i, __ATOMIC_RELAXED, WG); break;
-    case FK_MAX_S64: __hip_atomic_fetch_max(static_cast<int64_t*>(a), v.i, __ATOMIC_RELAXED, WG); break;
-    case FK_MIN_U32: __hip_atomic_fetch_min(static_cast<uint32_t*>(a), uint32_t(v.i), __ATOMIC_RELAXED, WG); break;
-    case FK_MAX_U32: __hip_atomic_fetch_max(static_cast<uint32_t*>(a), uint32_t(v.i), __ATOMIC_RELAXED, WG); break;
-    case FK_MIN_U64: __hip_atomic_fetch_min(static_cast<uint64_t*>(a), uint64_t(v.i), __ATOMIC_RELAXED, WG); break;
-    case FK_MAX_U64: __hip_atomic_fetch_max(static_cast<uint64_t*>(a), uint64_t(v.i), __ATOMIC_RELAXED, WG); break;
-    case FK_MIN_F32:
-      if (!isnan(v.f)) __hip_atomic_fetch_min(static_cast<uint32_t*>(a), ord32(uint32_t(v.b)), __ATOMIC_RELAXED, WG);
-      break;
-    case FK_MAX_F32:
-      if (!isnan(v.f)) __hip_atomic_fetch_max(static_cast<uint32_t*>(a), ord32(uint32_t(v.b)), __ATOMIC_RELAXED, WG);
-      break;
-    case FK_MIN_F64:
-      if (!isnan(v.d)) __hip_atomic_fetch_min(static_cast<uint64_t*>(a), ord64(v.b), __ATOMIC_RELAXED, WG);
-      break;
-    case FK_MAX_F64:
-      if (!isnan(v.d)) __hip_atomic_fetch_max(static_cast<uint64_t*>(a), ord64(v.b), __ATOMIC_RELAXED, WG);
-      break;
-    case FK_COUNT: __hip_atomic_fetch_add(static_cast<uint32_t*>(a), 1u, __ATOMIC_RELAXED, WG); break;
-    default: break;
-  }
-}
-
 __device__ __forceinline__ void b_update_global(batch_params const& B, int32_t const* kinds, int inst,
                                                 int64_t slot, int64_t row)
 {
@@ -292,32 +227,6 @@ __device__ __forceinline__ void b_update_global(batch_params const& B, int32_t c
     loaded_value const val = load_value(c, row);
     for (int f = fb; f < fe; f++) acc_apply(kinds[f], fa(B.fields[f], slot), val);
   }
-}
-
-// Insert `key` into the LDS table (S slots of u64 at `tkeys`, at most
-// BATCH_LDS_FILL keys); returns its LDS slot, or -1 when the table is full.
-__device__ __forceinline__ int lds_insert(uint64_t* tkeys, uint32_t* fill, uint64_t key, uint32_t h,
-                                          uint64_t empty)
-{
-  constexpr int S  = BATCH_LDS_SLOTS;
-  constexpr int WG = __HIP_MEMORY_SCOPE_WORKGROUP;
-  uint32_t s       = h & uint32_t(S - 1);
-  for (int p = 0; p < S; p++) {
-    uint64_t const cur = __hip_atomic_load(tkeys + s, __ATOMIC_RELAXED, WG);
-    if (cur == key) return int(s);
-    if (cur == empty) {
-      if (__hip_atomic_load(fill, __ATOMIC_RELAXED, WG) >= uint32_t(BATCH_LDS_FILL)) return -1;
-      uint64_t expected = empty;
-      if (__hip_atomic_compare_exchange_strong(tkeys + s, &expected, key, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
-                                               WG)) {
-        __hip_atomic_fetch_add(fill, 1u, __ATOMIC_RELAXED, WG);
-        return int(s);
-      }
-      if (expected == key) return int(s);
-    }
-    s = (s + 1) & uint32_t(S - 1);
-  }
-  return -1;
 }
 
 // Pull the next work item of queue `q` into `item_sh` (all threads), and
@@ -404,7 +313,7 @@ __global__ void __launch_bounds__(NT) k_batch_aggregate(batch_params const* __re
           id            = S;
           present_sh[0] = 1;
         } else {
-          id = lds_insert(tkeys, &fill_sh, bits, h, EMPTY_KEY);
+          id = lds_insert(tkeys, &fill_sh, bits, h, EMPTY_KEY, S, BATCH_LDS_FILL);
         }
       }
       if (id < 0) {
@@ -537,7 +446,7 @@ __global__ void __launch_bounds__(NT) k_batch_second(batch_params const* __restr
         if (!dcol_valid(c, row)) continue;
         if (slot == -2) {
           slot = b_row_slot(B, I, row);
-          if (slot >= 0 && use_lds) id = lds_insert(tslots, &fill_sh, uint64_t(slot), mm_fmix32(uint32_t(slot)), NO);
+          if (slot >= 0 && use_lds) id = lds_insert(tslots, &fill_sh, uint64_t(slot), mm_fmix32(uint32_t(slot)), NO, S, BATCH_LDS_FILL);
         }
         if (slot < 0) break;
         loaded_value const val = load_value(c, row);

@@ -250,6 +250,97 @@ __device__ __forceinline__ void apply_field(field_desc const& F, int64_t slot, l
   acc_apply(F.kind, static_cast<uint8_t*>(F.ptr) + slot * F.width, v);
 }
 
+// Combine a pre-aggregated partial (same encoding as the table) into `a`.
+__device__ __forceinline__ void acc_combine(int32_t kind, void* a, uint64_t part)
+{
+  switch (kind) {
+    case FK_SUM_I64:
+    case FK_SUMSQ_I64: aadd(static_cast<uint64_t*>(a), part); break;
+    case FK_SUM_F32:
+    case FK_SUMSQ_F32: unsafeAtomicAdd(static_cast<float*>(a), __uint_as_float(uint32_t(part))); break;
+    case FK_SUM_F64:
+    case FK_SUMSQ_F64: unsafeAtomicAdd(static_cast<double*>(a), __longlong_as_double(int64_t(part))); break;
+    case FK_MIN_S32: amin(static_cast<int32_t*>(a), int32_t(part)); break;
+    case FK_MAX_S32: amax(static_cast<int32_t*>(a), int32_t(part)); break;
+    case FK_MIN_S64: amin(static_cast<int64_t*>(a), int64_t(part)); break;
+    case FK_MAX_S64: amax(static_cast<int64_t*>(a), int64_t(part)); break;
+    case FK_MIN_U32:
+    case FK_MIN_F32: amin(static_cast<uint32_t*>(a), uint32_t(part)); break;
+    case FK_MAX_U32:
+    case FK_MAX_F32: amax(static_cast<uint32_t*>(a), uint32_t(part)); break;
+    case FK_MIN_U64:
+    case FK_MIN_F64: amin(static_cast<uint64_t*>(a), part); break;
+    case FK_MAX_U64:
+    case FK_MAX_F64: amax(static_cast<uint64_t*>(a), part); break;
+    case FK_COUNT: aadd(static_cast<uint32_t*>(a), uint32_t(part)); break;
+    default: break;
+  }
+}
+
+// The per-row update into LDS partials (workgroup-scope atomics).
+__device__ __forceinline__ void lds_apply(int32_t kind, void* a, loaded_value const& v)
+{
+  constexpr int WG = __HIP_MEMORY_SCOPE_WORKGROUP;
+  switch (kind) {
+    case FK_SUM_I64: __hip_atomic_fetch_add(static_cast<uint64_t*>(a), uint64_t(v.i), __ATOMIC_RELAXED, WG); break;
+    case FK_SUM_F32: __hip_atomic_fetch_add(static_cast<float*>(a), v.f, __ATOMIC_RELAXED, WG); break;
+    case FK_SUM_F64: __hip_atomic_fetch_add(static_cast<double*>(a), v.d, __ATOMIC_RELAXED, WG); break;
+    case FK_SUMSQ_I64:
+      __hip_atomic_fetch_add(static_cast<uint64_t*>(a), uint64_t(v.i) * uint64_t(v.i), __ATOMIC_RELAXED, WG);
+      break;
+    case FK_SUMSQ_F32: __hip_atomic_fetch_add(static_cast<float*>(a), v.f * v.f, __ATOMIC_RELAXED, WG); break;
+    case FK_SUMSQ_F64: __hip_atomic_fetch_add(static_cast<double*>(a), v.d * v.d, __ATOMIC_RELAXED, WG); break;
+    case FK_MIN_S32: __hip_atomic_fetch_min(static_cast<int32_t*>(a), int32_t(v.i), __ATOMIC_RELAXED, WG); break;
+    case FK_MAX_S32: __hip_atomic_fetch_max(static_cast<int32_t*>(a), int32_t(v.i), __ATOMIC_RELAXED, WG); break;
+    case FK_MIN_S64: __hip_atomic_fetch_min(static_cast<int64_t*>(a), v.i, __ATOMIC_RELAXED, WG); break;
+    case FK_MAX_S64: __hip_atomic_fetch_max(static_cast<int64_t*>(a), v.i, __ATOMIC_RELAXED, WG); break;
+    case FK_MIN_U32: __hip_atomic_fetch_min(static_cast<uint32_t*>(a), uint32_t(v.i), __ATOMIC_RELAXED, WG); break;
+    case FK_MAX_U32: __hip_atomic_fetch_max(static_cast<uint32_t*>(a), uint32_t(v.i), __ATOMIC_RELAXED, WG); break;
+    case FK_MIN_U64: __hip_atomic_fetch_min(static_cast<uint64_t*>(a), uint64_t(v.i), __ATOMIC_RELAXED, WG); break;
+    case FK_MAX_U64: __hip_atomic_fetch_max(static_cast<uint64_t*>(a), uint64_t(v.i), __ATOMIC_RELAXED, WG); break;
+    case FK_MIN_F32:
+      if (!isnan(v.f)) __hip_atomic_fetch_min(static_cast<uint32_t*>(a), ord32(uint32_t(v.b)), __ATOMIC_RELAXED, WG);
+      break;
+    case FK_MAX_F32:
+      if (!isnan(v.f)) __hip_atomic_fetch_max(static_cast<uint32_t*>(a), ord32(uint32_t(v.b)), __ATOMIC_RELAXED, WG);
+      break;
+    case FK_MIN_F64:
+      if (!isnan(v.d)) __hip_atomic_fetch_min(static_cast<uint64_t*>(a), ord64(v.b), __ATOMIC_RELAXED, WG);
+      break;
+    case FK_MAX_F64:
+      if (!isnan(v.d)) __hip_atomic_fetch_max(static_cast<uint64_t*>(a), ord64(v.b), __ATOMIC_RELAXED, WG);
+      break;
+    case FK_COUNT: __hip_atomic_fetch_add(static_cast<uint32_t*>(a), 1u, __ATOMIC_RELAXED, WG); break;
+    default: break;
+  }
+}
+
+// Insert `key` into an LDS open-addressing table of S (a power of two) u64
+// slots at `tkeys` whose free slots hold `empty`, with at most `limit` keys
+// (`fill` counts them); returns its slot, or -1 when the table is full.
+__device__ __forceinline__ int lds_insert(uint64_t* tkeys, uint32_t* fill, uint64_t key, uint32_t h,
+                                          uint64_t empty, int S, uint32_t limit)
+{
+  constexpr int WG = __HIP_MEMORY_SCOPE_WORKGROUP;
+  uint32_t s       = h & uint32_t(S - 1);
+  for (int p = 0; p < S; p++) {
+    uint64_t const cur = __hip_atomic_load(tkeys + s, __ATOMIC_RELAXED, WG);
+    if (cur == key) return int(s);
+    if (cur == empty) {
+      if (__hip_atomic_load(fill, __ATOMIC_RELAXED, WG) >= limit) return -1;
+      uint64_t expected = empty;
+      if (__hip_atomic_compare_exchange_strong(tkeys + s, &expected, key, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                               WG)) {
+        __hip_atomic_fetch_add(fill, 1u, __ATOMIC_RELAXED, WG);
+        return int(s);
+      }
+      if (expected == key) return int(s);
+    }
+    s = (s + 1) & uint32_t(S - 1);
+  }
+  return -1;
+}
+
 __device__ __forceinline__ void store_width(void* data, int64_t g, int w, uint64_t bits)
 {
   switch (w) {

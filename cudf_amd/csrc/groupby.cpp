@@ -44,6 +44,11 @@ constexpr int32_t RPA_LDS_BUDGET     = 76 * 1024;
 constexpr int RPA_MAX_LOG_BINS       = 15;
 constexpr int RPA_MAX_LOG_FANOUT     = 8;
 constexpr int32_t RPA_STAGE_BUDGET   = 48 * 1024;  // LDS staging of a scatter tile
+// Global path, low cardinality: LDS partial tables of up to 2048 slots (3/4
+// fillable), used when the estimate is at most LDS_PREAGG_MAX_GROUPS groups.
+constexpr int32_t LDS_PREAGG_SLOTS      = 2048;
+constexpr int32_t LDS_PREAGG_MAX_GROUPS = 1024;
+constexpr int32_t LDS_PREAGG_BUDGET     = 96 * 1024;
 
 // hipEvents marking phase boundaries on the call's stream.
 struct phase_timer {
@@ -960,6 +965,43 @@ std::unique_ptr<mig_groupby_result> groupby_aggregate(mig_column_view const* key
 
   // ---- global hash table, growing on overflow --------------------------------
   if (G < 0) {
+    // Low cardinality with one fixed-width key: per-workgroup LDS partial
+    // tables merged into the global table (one global atomic per workgroup,
+    // group and field instead of one per row and field).
+    int64_t lds_blocks = 0;
+    P.lds_bytes        = 0;
+    if (P.key_mode == KM_INLINE && est <= double(LDS_PREAGG_MAX_GROUPS) && forced != MIG_STRATEGY_PARTITIONED) {
+      int32_t S_l = 256;
+      while (S_l < LDS_PREAGG_SLOTS && double(S_l) * 0.75 < est * 1.5) S_l *= 2;
+      int32_t off = S_l * 8, m2_off = S_l * 8;
+      for (int f = 0; f < nf; f++) {
+        field_desc& F = P.fields[f];
+        if (F.kind == FK_M2) {
+          P.lds_m2_off[f] = m2_off;
+          m2_off += S_l * 8;
+        } else {
+          F.lds_off = off;
+          off += ((S_l + 2) * F.width + 7) & ~7;
+        }
+      }
+      P.lds_count_off = -1;
+      if (P.need_count_all) {
+        P.lds_count_off = off;
+        off += ((S_l + 2) * 4 + 7) & ~7;
+      }
+      int32_t const bytes = std::max(off, m2_off);
+      if (bytes <= LDS_PREAGG_BUDGET) {
+        P.lds_slots = S_l;
+        P.lds_fill  = S_l * 3 / 4;
+        P.lds_bytes = bytes;
+        int64_t const per_cu = std::max<int64_t>(1, std::min<int64_t>(8, (160 * 1024) / bytes));
+        // at least ~16 rows per group and workgroup, so that merging pays off
+        int64_t const rows_per_wg = std::max<int64_t>(4096, int64_t(16.0 * std::max(est, 1.0)));
+        lds_blocks = std::max<int64_t>(1, std::min<int64_t>(int64_t(dp.num_cus) * per_cu,
+                                                             (N + rows_per_wg - 1) / rows_per_wg));
+      }
+    }
+    stats[STAT_LDS_PREAGG] = lds_blocks > 0 ? 1.0 : 0.0;
     uint64_t cap = std::max<uint64_t>(64, next_pow2(uint64_t(est * 2.6) + 1));
     cap = std::min(cap, max_cap);
     int32_t* d_flags = S.get<int32_t>(4);  // [0] overflow, [1..2] special occupancy
@@ -988,7 +1030,8 @@ std::unique_ptr<mig_groupby_result> groupby_aggregate(mig_column_view const* key
       upload(0);
       launch_init_state(dP, P, stream);
       T.mark(2);
-      launch_aggregate(dP, P, dp.num_cus, stream);
+      if (lds_blocks > 0) launch_aggregate_lds(dP, P, lds_blocks, stream);
+      else launch_aggregate(dP, P, dp.num_cus, stream);
       T.mark(3);
       MIG_HIP_TRY(hipGetLastError());
 
@@ -1013,7 +1056,10 @@ std::unique_ptr<mig_groupby_result> groupby_aggregate(mig_column_view const* key
     stats[STAT_RETRIES]  = retries;
 
     T.mark(4);
-    if (P.need_second_pass) launch_second_pass(dP, P, dp.num_cus, stream);
+    if (P.need_second_pass) {
+      if (lds_blocks > 0) launch_second_pass_lds(dP, P, lds_blocks, stream);
+      else launch_second_pass(dP, P, dp.num_cus, stream);
+    }
     T.mark(5);
     uint32_t* map = S.get<uint32_t>(G);
     launch_compact(dP, P, block_counts, map, stream);

@@ -55,6 +55,7 @@ enum stat_index : int {
   STAT_T_RPA_SCATTER1,
   STAT_T_RPA_SCATTER2,
   STAT_T_RPA_AGGREGATE,
+  STAT_LDS_PREAGG,       // 1 when the global path pre-aggregated in LDS
   STAT_COUNT
 };
 double* thread_stats();

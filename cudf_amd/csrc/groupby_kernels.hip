@@ -317,6 +317,185 @@ void launch_second_pass(gb_params const* dP, gb_params const& hP, int num_cus, h
 }
 
 // ---------------------------------------------------------------------------
+// Low cardinality (one fixed-width key, few groups): per-workgroup partial
+// tables in LDS merged into the global table once per workgroup, so a hot
+// group costs one global atomic per workgroup and field instead of one per
+// row.  Rows whose key finds the LDS table full go straight to the global
+// table.  The variance pass keys an LDS table by group slot.
+constexpr int LBLOCK = 256;
+
+__global__ void __launch_bounds__(LBLOCK) k_aggregate_lds(gb_params const* __restrict__ Pp)
+{
+  extern __shared__ __attribute__((aligned(16))) uint8_t lds[];
+  __shared__ uint32_t fill_sh, present_sh[2];
+  gb_params const& P = *Pp;
+  constexpr int WG   = __HIP_MEMORY_SCOPE_WORKGROUP;
+  int const S        = P.lds_slots;
+  auto* tkeys        = reinterpret_cast<uint64_t*>(lds);
+  for (int i = threadIdx.x; i < S; i += LBLOCK) tkeys[i] = EMPTY_KEY;
+  for (int f = 0; f < P.num_fields; f++) {
+    field_desc const& F = P.fields[f];
+    if (F.kind == FK_M2) continue;
+    uint64_t const idv = field_identity(F.kind);
+    if (F.width == 8)
+      for (int i = threadIdx.x; i < S + 2; i += LBLOCK) *lds_at<uint64_t>(lds, F.lds_off, i) = idv;
+    else
+      for (int i = threadIdx.x; i < S + 2; i += LBLOCK) *lds_at<uint32_t>(lds, F.lds_off, i) = uint32_t(idv);
+  }
+  if (P.lds_count_off >= 0)
+    for (int i = threadIdx.x; i < S + 2; i += LBLOCK) *lds_at<uint32_t>(lds, P.lds_count_off, i) = 0;
+  if (threadIdx.x == 0) {
+    fill_sh       = 0;
+    present_sh[0] = 0;
+    present_sh[1] = 0;
+  }
+  __syncthreads();
+  dcol const& kc = P.keys[0];
+  for (int64_t row = int64_t(blockIdx.x) * LBLOCK + threadIdx.x; row < P.num_rows;
+       row += int64_t(gridDim.x) * LBLOCK) {
+    int id        = -1;
+    uint64_t bits = 0;
+    uint32_t h    = 0;
+    if (!dcol_valid(kc, row)) {
+      if (P.skip_null_keys) continue;
+      id            = S + 1;
+      present_sh[1] = 1;
+    } else {
+      bits = load_fixed_bits(kc, row);
+      h    = hash_combine(0u, hash_fixed_bits(kc.storage, bits, 0u));
+      if (bits == EMPTY_KEY) {
+        id            = S;
+        present_sh[0] = 1;
+      } else {
+        id = lds_insert(tkeys, &fill_sh, bits, h, EMPTY_KEY, S, uint32_t(P.lds_fill));
+      }
+    }
+    if (id < 0) {
+      int64_t const slot = insert_inline(P, bits, h);
+      if (slot >= 0) update_row(P, slot, row);
+      continue;
+    }
+    if (P.lds_count_off >= 0)
+      __hip_atomic_fetch_add(lds_at<uint32_t>(lds, P.lds_count_off, id), 1u, __ATOMIC_RELAXED, WG);
+    for (int vc = 0; vc < P.num_vals; vc++) {
+      int const fb = P.val_field_begin[vc], fe = P.val_field_begin[vc + 1];
+      if (fb == fe) continue;
+      dcol const& c = P.vals[vc];
+      if (!dcol_valid(c, row)) continue;
+      loaded_value const v = load_value(c, row);
+      for (int f = fb; f < fe; f++) {
+        field_desc const& F = P.fields[f];
+        if (F.kind != FK_M2) lds_apply(F.kind, lds + F.lds_off + int64_t(id) * F.width, v);
+      }
+    }
+  }
+  __syncthreads();
+  // merge the partials into the global table
+  for (int i = threadIdx.x; i < S + 2; i += LBLOCK) {
+    int64_t slot;
+    if (i < S) {
+      uint64_t const k = tkeys[i];
+      if (k == EMPTY_KEY) continue;
+      slot = insert_inline(P, k, hash_combine(0u, hash_fixed_bits(kc.storage, k, 0u)));
+    } else if (i == S) {
+      if (!present_sh[0]) continue;
+      slot = insert_inline(P, EMPTY_KEY, 0u);
+    } else {
+      if (!present_sh[1]) continue;
+      mark_special(P, 1);
+      slot = int64_t(P.cap) + 1;
+    }
+    if (slot < 0) continue;
+    if (P.lds_count_off >= 0) aadd(P.slot_count + slot, *lds_at<uint32_t>(lds, P.lds_count_off, i));
+    for (int f = 0; f < P.num_fields; f++) {
+      field_desc const& F = P.fields[f];
+      if (F.kind == FK_M2) continue;
+      uint64_t const part = F.width == 8 ? *lds_at<uint64_t>(lds, F.lds_off, i)
+                                         : uint64_t(*lds_at<uint32_t>(lds, F.lds_off, i));
+      // an identity partial changes nothing (float sums start at +0.0)
+      if (part != field_identity(F.kind))
+        acc_combine(F.kind, static_cast<uint8_t*>(F.ptr) + slot * F.width, part);
+    }
+  }
+}
+
+__global__ void __launch_bounds__(LBLOCK) k_second_pass_lds(gb_params const* __restrict__ Pp)
+{
+  extern __shared__ __attribute__((aligned(16))) uint8_t lds[];
+  __shared__ uint32_t fill_sh;
+  gb_params const& P = *Pp;
+  constexpr int WG   = __HIP_MEMORY_SCOPE_WORKGROUP;
+  constexpr uint64_t NO = ~0ull;  // free slot of the LDS table of group slots
+  int const S        = P.lds_slots;
+  auto* tslots       = reinterpret_cast<uint64_t*>(lds);
+  for (int i = threadIdx.x; i < S; i += LBLOCK) tslots[i] = NO;
+  for (int f = 0; f < P.num_fields; f++) {
+    if (P.fields[f].kind != FK_M2) continue;
+    for (int i = threadIdx.x; i < S; i += LBLOCK) *lds_at<double>(lds, P.lds_m2_off[f], i) = 0.0;
+  }
+  if (threadIdx.x == 0) fill_sh = 0;
+  __syncthreads();
+  for (int64_t row = int64_t(blockIdx.x) * LBLOCK + threadIdx.x; row < P.num_rows;
+       row += int64_t(gridDim.x) * LBLOCK) {
+    int64_t slot = -2;
+    int id       = -1;
+    for (int vc = 0; vc < P.num_vals; vc++) {
+      int const fb = P.val_field_begin[vc], fe = P.val_field_begin[vc + 1];
+      bool has_m2  = false;
+      for (int f = fb; f < fe; f++) has_m2 |= (P.fields[f].kind == FK_M2);
+      if (!has_m2) continue;
+      dcol const& c = P.vals[vc];
+      if (!dcol_valid(c, row)) continue;
+      if (slot == -2) {
+        slot = locate<KM_INLINE, false>(P, row);
+        if (slot >= 0) id = lds_insert(tslots, &fill_sh, uint64_t(slot), mm_fmix32(uint32_t(slot)), NO, S,
+                                       uint32_t(P.lds_fill));
+      }
+      if (slot < 0) break;
+      loaded_value const v = load_value(c, row);
+      for (int f = fb; f < fe; f++) {
+        field_desc const& F = P.fields[f];
+        if (F.kind != FK_M2) continue;
+        double const cnt  = double(count_of(P, F.aux_count, slot));
+        double const mean = field_as_double(P.fields[F.aux_sum], slot) / cnt;
+        double const d    = v.d - mean;
+        if (id >= 0) __hip_atomic_fetch_add(lds_at<double>(lds, P.lds_m2_off[f], id), d * d, __ATOMIC_RELAXED, WG);
+        else unsafeAtomicAdd(reinterpret_cast<double*>(F.ptr) + slot, d * d);
+      }
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < S; i += LBLOCK) {
+    uint64_t const slot = tslots[i];
+    if (slot == NO) continue;
+    for (int f = 0; f < P.num_fields; f++) {
+      if (P.fields[f].kind != FK_M2) continue;
+      double const part = *lds_at<double>(lds, P.lds_m2_off[f], i);
+      if (part != 0.0) unsafeAtomicAdd(reinterpret_cast<double*>(P.fields[f].ptr) + int64_t(slot), part);
+    }
+  }
+}
+
+static void allow_lds(void const* fn, int bytes)
+{
+  if (bytes > 64 * 1024) MIG_HIP_TRY(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+}
+
+void launch_aggregate_lds(gb_params const* dP, gb_params const& hP, int64_t blocks, hipStream_t s)
+{
+  if (hP.num_rows <= 0) return;
+  allow_lds(reinterpret_cast<void const*>(&k_aggregate_lds), hP.lds_bytes);
+  hipLaunchKernelGGL(k_aggregate_lds, dim3(blocks), dim3(LBLOCK), hP.lds_bytes, s, dP);
+}
+
+void launch_second_pass_lds(gb_params const* dP, gb_params const& hP, int64_t blocks, hipStream_t s)
+{
+  if (hP.num_rows <= 0) return;
+  allow_lds(reinterpret_cast<void const*>(&k_second_pass_lds), hP.lds_bytes);
+  hipLaunchKernelGGL(k_second_pass_lds, dim3(blocks), dim3(LBLOCK), hP.lds_bytes, s, dP);
+}
+
+// ---------------------------------------------------------------------------
 // compaction of occupied slots -> dense group ids
 __device__ __forceinline__ bool slot_occupied(gb_params const& P, uint64_t slot)
 {

@@ -112,6 +112,17 @@ struct gb_params {
   // key output (finalize)
   void* out_keys[MAX_KEYS];       // fixed-width key outputs (or string offsets)
   uint32_t* out_key_mask[MAX_KEYS];
+  // low-cardinality pre-aggregation (inline keys): each workgroup keeps an
+  // LDS table of lds_slots keys (at most lds_fill) with partials at
+  // fields[f].lds_off (lds_slots + 2 elements of the field's width; slots
+  // lds_slots and lds_slots + 1 hold the EMPTY_KEY and null-key groups) and
+  // the row counts at lds_count_off (-1: none); the variance pass reuses the
+  // bytes for a table of group slots and the M2 partials at lds_m2_off[f]
+  int32_t lds_slots;
+  int32_t lds_fill;
+  int32_t lds_count_off;
+  int32_t lds_bytes;
+  int32_t lds_m2_off[MAX_FIELDS];
 };
 
 // ---------------------------------------------------------------------------
@@ -185,6 +196,10 @@ void launch_sample_hash(gb_params const* dP, int64_t n, int64_t nsample, uint64_
                         uint32_t table_mask, uint32_t* counters, hipStream_t s);
 void launch_init_state(gb_params const* dP, gb_params const& hP, hipStream_t s);
 void launch_aggregate(gb_params const* dP, gb_params const& hP, int num_cus, hipStream_t s);
+// Low-cardinality form of launch_aggregate / launch_second_pass (inline keys,
+// hP.lds_bytes > 0): `blocks` workgroups pre-aggregate their rows in LDS.
+void launch_aggregate_lds(gb_params const* dP, gb_params const& hP, int64_t blocks, hipStream_t s);
+void launch_second_pass_lds(gb_params const* dP, gb_params const& hP, int64_t blocks, hipStream_t s);
 void launch_second_pass(gb_params const* dP, gb_params const& hP, int num_cus, hipStream_t s);
 // Occupancy scan: block_counts has ceil((cap+2)/SCAN_TILE) entries.
 constexpr int SCAN_TILE = 4096;

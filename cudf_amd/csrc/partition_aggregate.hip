@@ -23,7 +23,7 @@ namespace mig {
 
 constexpr int WG_SCOPE = __HIP_MEMORY_SCOPE_WORKGROUP;
 
-constexpr int RPA_CACHED = 8;  // rows per thread per chunk; their slots stay in registers
+constexpr int RPA_CACHED = 6;  // rows per thread per chunk; their slots stay in registers
 
 // Apply `f(k)` to the rows k of a chunk whose bit is set in `ok` (the loop is
 // unrolled; the caller's switch on the accumulator kind stays outside it).

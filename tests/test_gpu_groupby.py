@@ -279,3 +279,48 @@ def test_c2_full_size_properties():
     assert int(fcnt.sum()) == int(fboth.sum())
     fs, fsv = res[0][0]
     assert abs(float(fs[fsv].sum()) - float(f64.data[fboth].sum())) < 1e-6 * float(np.abs(f64.data[fboth]).sum())
+
+
+# ------------------------------------- low cardinality: LDS partial tables --
+def _last_stats():
+    from cudf_amd import _lib
+    return _lib.last_stats()
+
+
+def test_lds_preaggregation_is_used_for_low_cardinality():
+    key, v = c1_data(n=2_000_000, seed=3)
+    assert_parity([key], [(v, [(AGG["SUM"], 1), (AGG["COUNT_VALID"], 1), (AGG["MIN"], 1),
+                               (AGG["MAX"], 1), (AGG["MEAN"], 1)])])
+    assert _last_stats()["lds_preagg"] == 1.0
+
+
+@pytest.mark.parametrize("policy", [0, 1])
+def test_lds_preaggregation_variance_nulls_sentinel(policy):
+    """50 int64 keys incl. the table's empty marker, null keys, nullable
+    float64 and int32 values, every C2 aggregation (LDS first and second pass)."""
+    rng = np.random.default_rng(61 + policy)
+    n = 400_000
+    sentinel = np.array([0xFFF1A5A5A5A5A5A5], dtype=np.uint64).view(np.int64)[0]
+    k = rng.integers(-25, 25, n).astype(np.int64)
+    k[rng.random(n) < 0.05] = sentinel
+    key = host_col(k, rng.random(n) >= 0.03)
+    f = host_col(rng.standard_normal(n) * 1e3 + 5e3, rng.random(n) >= 0.05)
+    i = host_col(rng.integers(0, 10_000, n).astype(np.int32), rng.random(n) >= 0.05)
+    assert_parity([key], [(f, C2_AGGS), (i, C2_AGGS)], policy)
+    assert _last_stats()["lds_preagg"] == 1.0
+
+
+def test_lds_preaggregation_table_full_falls_back_to_global():
+    """The strided sample sees one key while the input has ~20k: the LDS
+    tables fill and the remaining keys go straight to the global table."""
+    n, ns = 400_000, 65536
+    k = (np.arange(n, dtype=np.int64) % 20_000) + 1
+    k[(np.arange(ns, dtype=np.int64) * n) // ns] = 0
+    rng = np.random.default_rng(67)
+    v = host_col(rng.integers(-100, 100, n).astype(np.int64))
+    f = host_col(rng.standard_normal(n))
+    ng = assert_parity([host_col(k)], [(v, [(AGG["SUM"], 1), (AGG["MIN"], 1), (AGG["COUNT_ALL"], 1)]),
+                                       (f, [(AGG["SUM"], 1), (AGG["VARIANCE"], 1)])])
+    assert ng == len(np.unique(k))
+    st = _last_stats()
+    assert st["lds_preagg"] == 1.0 and st["est_groups"] < 100
