@@ -244,10 +244,20 @@ __device__ __forceinline__ void acc_apply(int32_t kind, void* a, loaded_value co
   }
 }
 
-// Accumulate into slot `slot` of a field's global array.
+// Global accumulator of field F at `slot`, and a slot's row count.
+__device__ __forceinline__ uint8_t* field_at(field_desc const& F, int64_t slot)
+{
+  return static_cast<uint8_t*>(F.ptr) + slot * F.stride;
+}
+__device__ __forceinline__ uint32_t* count_at(gb_params const& P, int64_t slot)
+{
+  return reinterpret_cast<uint32_t*>(reinterpret_cast<uint8_t*>(P.slot_count) + slot * P.count_stride);
+}
+
+// Accumulate into slot `slot` of a field's global accumulators.
 __device__ __forceinline__ void apply_field(field_desc const& F, int64_t slot, loaded_value const& v)
 {
-  acc_apply(F.kind, static_cast<uint8_t*>(F.ptr) + slot * F.width, v);
+  acc_apply(F.kind, field_at(F, slot), v);
 }
 
 // Combine a pre-aggregated partial (same encoding as the table) into `a`.

@@ -608,6 +608,7 @@ std::unique_ptr<mig_groupby_result> groupby_aggregate(mig_column_view const* key
     F.kind        = kind;
     F.vcol        = vc;
     F.width       = field_width(kind);
+    F.stride      = F.width;
     F.aux_sum     = -1;
     F.aux_count   = -1;
     return nf++;
@@ -1019,11 +1020,16 @@ std::unique_ptr<mig_groupby_result> groupby_aggregate(mig_column_view const* key
       if (P.key_mode == KM_INLINE) P.slot_keys = S.get<uint64_t>(cap);
       else P.slot_rows = S.get<uint64_t>(cap);
       state_bytes += 8 * cap;
-      P.slot_count = P.need_count_all ? S.get<uint32_t>(total) : nullptr;
+      // one array per accumulator (structure of arrays): the atomics of a
+      // row then go to different lines, which the memory system serves in
+      // parallel (a shared row of all accumulators measured slower)
+      P.slot_count   = P.need_count_all ? S.get<uint32_t>(total) : nullptr;
+      P.count_stride = 4;
       if (P.slot_count) state_bytes += 4 * total;
       for (int f = 0; f < nf; f++) {
         P.fields[f].ptr = P.fields[f].width == 8 ? static_cast<void*>(S.get<uint64_t>(total))
                                                  : static_cast<void*>(S.get<uint32_t>(total));
+        P.fields[f].stride = P.fields[f].width;
         state_bytes += size_t(P.fields[f].width) * total;
       }
       stats[STAT_STATE_BYTES] = double(state_bytes);

@@ -93,16 +93,14 @@ __global__ void __launch_bounds__(BLOCK) k_init_state(gb_params const* __restric
     }
   } else if (which == 1) {
     if (P.slot_count)
-      for (int64_t j = i; j < total; j += stride) P.slot_count[j] = 0;
+      for (int64_t j = i; j < total; j += stride) *count_at(P, j) = 0;
   } else {
     field_desc const& F = P.fields[which - 2];
     uint64_t const id = field_identity(F.kind);
     if (F.width == 8) {
-      auto p = reinterpret_cast<uint64_t*>(F.ptr);
-      for (int64_t j = i; j < total; j += stride) p[j] = id;
+      for (int64_t j = i; j < total; j += stride) *reinterpret_cast<uint64_t*>(field_at(F, j)) = id;
     } else {
-      auto p = reinterpret_cast<uint32_t*>(F.ptr);
-      for (int64_t j = i; j < total; j += stride) p[j] = static_cast<uint32_t>(id);
+      for (int64_t j = i; j < total; j += stride) *reinterpret_cast<uint32_t*>(field_at(F, j)) = uint32_t(id);
     }
   }
 }
@@ -213,7 +211,7 @@ __device__ __forceinline__ int64_t locate(gb_params const& P, int64_t row)
 
 __device__ __forceinline__ void update_row(gb_params const& P, int64_t slot, int64_t row)
 {
-  if (P.need_count_all) aadd(P.slot_count + slot, 1u);
+  if (P.need_count_all) aadd(count_at(P, slot), 1u);
   for (int vc = 0; vc < P.num_vals; vc++) {
     int const fb = P.val_field_begin[vc], fe = P.val_field_begin[vc + 1];
     if (fb == fe) continue;
@@ -261,10 +259,10 @@ void launch_aggregate(gb_params const* dP, gb_params const& hP, int num_cus, hip
 __device__ __forceinline__ double field_as_double(field_desc const& F, int64_t slot)
 {
   switch (F.kind) {
-    case FK_SUM_I64: return double(reinterpret_cast<int64_t const*>(F.ptr)[slot]);
-    case FK_SUM_F32: return double(reinterpret_cast<float const*>(F.ptr)[slot]);
-    case FK_SUM_F64: return reinterpret_cast<double const*>(F.ptr)[slot];
-    case FK_COUNT: return double(reinterpret_cast<uint32_t const*>(F.ptr)[slot]);
+    case FK_SUM_I64: return double(*reinterpret_cast<int64_t const*>(field_at(F, slot)));
+    case FK_SUM_F32: return double(*reinterpret_cast<float const*>(field_at(F, slot)));
+    case FK_SUM_F64: return *reinterpret_cast<double const*>(field_at(F, slot));
+    case FK_COUNT: return double(*reinterpret_cast<uint32_t const*>(field_at(F, slot)));
     default: return 0.0;
   }
 }
@@ -273,7 +271,7 @@ __device__ __forceinline__ double field_as_double(field_desc const& F, int64_t s
 // without nulls count their values with the row count).
 __device__ __forceinline__ uint32_t count_of(gb_params const& P, int32_t f, int64_t slot)
 {
-  return f == -2 ? P.slot_count[slot] : reinterpret_cast<uint32_t const*>(P.fields[f].ptr)[slot];
+  return f == -2 ? *count_at(P, slot) : *reinterpret_cast<uint32_t const*>(field_at(P.fields[f], slot));
 }
 
 template <int KM>
@@ -300,7 +298,7 @@ __global__ void __launch_bounds__(BLOCK) k_second_pass(gb_params const* __restri
         double const cnt  = double(count_of(P, F.aux_count, slot));
         double const mean = field_as_double(P.fields[F.aux_sum], slot) / cnt;
         double const d    = v.d - mean;
-        unsafeAtomicAdd(reinterpret_cast<double*>(F.ptr) + slot, d * d);
+        unsafeAtomicAdd(reinterpret_cast<double*>(field_at(F, slot)), d * d);
       }
     }
   }
@@ -406,7 +404,7 @@ __global__ void __launch_bounds__(LBLOCK) k_aggregate_lds(gb_params const* __res
       slot = int64_t(P.cap) + 1;
     }
     if (slot < 0) continue;
-    if (P.lds_count_off >= 0) aadd(P.slot_count + slot, *lds_at<uint32_t>(lds, P.lds_count_off, i));
+    if (P.lds_count_off >= 0) aadd(count_at(P, slot), *lds_at<uint32_t>(lds, P.lds_count_off, i));
     for (int f = 0; f < P.num_fields; f++) {
       field_desc const& F = P.fields[f];
       if (F.kind == FK_M2) continue;
@@ -414,7 +412,7 @@ __global__ void __launch_bounds__(LBLOCK) k_aggregate_lds(gb_params const* __res
                                          : uint64_t(*lds_at<uint32_t>(lds, F.lds_off, i));
       // an identity partial changes nothing (float sums start at +0.0)
       if (part != field_identity(F.kind))
-        acc_combine(F.kind, static_cast<uint8_t*>(F.ptr) + slot * F.width, part);
+        acc_combine(F.kind, field_at(F, slot), part);
     }
   }
 }
@@ -460,7 +458,7 @@ __global__ void __launch_bounds__(LBLOCK) k_second_pass_lds(gb_params const* __r
         double const mean = field_as_double(P.fields[F.aux_sum], slot) / cnt;
         double const d    = v.d - mean;
         if (id >= 0) __hip_atomic_fetch_add(lds_at<double>(lds, P.lds_m2_off[f], id), d * d, __ATOMIC_RELAXED, WG);
-        else unsafeAtomicAdd(reinterpret_cast<double*>(F.ptr) + slot, d * d);
+        else unsafeAtomicAdd(reinterpret_cast<double*>(field_at(F, slot)), d * d);
       }
     }
   }
@@ -471,7 +469,7 @@ __global__ void __launch_bounds__(LBLOCK) k_second_pass_lds(gb_params const* __r
     for (int f = 0; f < P.num_fields; f++) {
       if (P.fields[f].kind != FK_M2) continue;
       double const part = *lds_at<double>(lds, P.lds_m2_off[f], i);
-      if (part != 0.0) unsafeAtomicAdd(reinterpret_cast<double*>(P.fields[f].ptr) + int64_t(slot), part);
+      if (part != 0.0) unsafeAtomicAdd(reinterpret_cast<double*>(field_at(P.fields[f], int64_t(slot))), part);
     }
   }
 }
@@ -600,8 +598,8 @@ void launch_compact(gb_params const* dP, gb_params const& hP, uint32_t const* bl
 // dense outputs: keys and aggregation results for group g = blockIdx*BLOCK+tid
 __device__ __forceinline__ uint64_t field_bits(field_desc const& F, int64_t slot)
 {
-  return F.width == 8 ? reinterpret_cast<uint64_t const*>(F.ptr)[slot]
-                      : reinterpret_cast<uint32_t const*>(F.ptr)[slot];
+  return F.width == 8 ? *reinterpret_cast<uint64_t const*>(field_at(F, slot))
+                      : *reinterpret_cast<uint32_t const*>(field_at(F, slot));
 }
 
 
@@ -656,11 +654,11 @@ __global__ void __launch_bounds__(BLOCK)
         case OK_COPY_F32: reinterpret_cast<uint32_t*>(O.data)[g] = uint32_t(field_bits(P.fields[O.field], slot)); break;
         case OK_COUNT:
           reinterpret_cast<int32_t*>(O.data)[g] =
-            int32_t(reinterpret_cast<uint32_t const*>(P.fields[O.field].ptr)[slot]);
+            int32_t(*reinterpret_cast<uint32_t const*>(field_at(P.fields[O.field], slot)));
           valid = true;
           break;
         case OK_COUNT_ALL:
-          reinterpret_cast<int32_t*>(O.data)[g] = int32_t(P.slot_count[slot]);
+          reinterpret_cast<int32_t*>(O.data)[g] = int32_t(*count_at(P, slot));
           valid = true;
           break;
         case OK_MINMAX: {
@@ -681,7 +679,7 @@ __global__ void __launch_bounds__(BLOCK)
           int64_t const dn = int64_t(cnt) - O.ddof;
           valid            = cnt > 0 && dn > 0;
           double var       = 0.0;
-          if (valid) var = reinterpret_cast<double const*>(P.fields[O.field].ptr)[slot] / double(dn);
+          if (valid) var = *reinterpret_cast<double const*>(field_at(P.fields[O.field], slot)) / double(dn);
           reinterpret_cast<double*>(O.data)[g] = (O.kind == OK_STD) ? sqrt(var) : var;
           break;
         }

@@ -52,10 +52,11 @@ enum field_kind : int32_t {
 };
 
 struct field_desc {
-  void* ptr;        // global accumulator array, cap + 2 entries
+  void* ptr;        // global accumulator of slot 0 (cap + 2 slots)
   int32_t kind;     // field_kind
   int32_t vcol;     // value column it reads
   int32_t width;    // 4 or 8 bytes
+  int32_t stride;   // bytes between the accumulators of consecutive slots
   int32_t lds_off;  // byte offset of this field's array in the LDS table
   int32_t aux_sum;    // FK_M2: index of the column's sum field
   int32_t aux_count;  // FK_M2: index of the column's count field
@@ -101,7 +102,9 @@ struct gb_params {
   int32_t need_second_pass;  // some field is FK_M2
   uint64_t* slot_keys;   // inline table
   uint64_t* slot_rows;   // general table
-  uint32_t* slot_count;  // group row count (COUNT_ALL), cap + 2 entries, or null
+  uint32_t* slot_count;  // group row count (COUNT_ALL) of slot 0, or null
+  int32_t count_stride;  // bytes between the row counts of consecutive slots
+  int32_t pad_stride;
   uint32_t* special;     // [2] occupancy of slots cap (sentinel key) and cap+1 (null key)
   int32_t* overflow;     // set when a probe sequence exceeds probe_limit
   dcol keys[MAX_KEYS];
