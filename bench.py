@@ -295,6 +295,65 @@ def make_served(n_req, seed=5):
     return factories, shapes
 
 
+def make_c3(n, seed=3):
+    """Skewed multi-column keys on the device: key1 int32 uniform 1..100, key2
+    int64 Zipf(1.1) over 1M values, key3 a string from a pool of 10k random
+    ASCII strings of length 4..32; float32 U[0,1) and int64 U[-1e9,1e9]
+    values; sum, count, max on both."""
+    import torch
+    import cudf_amd as ca
+    from cudf_amd import Column, TypeId
+    from cudf_amd.aggregation import Aggregation, Kind
+    rng = np.random.default_rng(seed)
+    dev = torch.device("cuda", 0)
+    g = torch.Generator(device=dev).manual_seed(seed)
+    lens = rng.integers(4, 33, 10_000)
+    pool_chars = torch.from_numpy(rng.integers(33, 127, int(lens.sum()), dtype=np.uint8)).to(dev)
+    pool_off = torch.from_numpy(np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)).to(dev)
+    pool_len = torch.from_numpy(lens.astype(np.int64)).to(dev)
+    w = np.arange(1, 1_000_001, dtype=np.float64) ** -1.1
+    zipf_cdf = torch.from_numpy(np.cumsum(w) / w.sum()).to(dev)
+    k1 = torch.randint(1, 101, (n,), dtype=torch.int32, device=dev, generator=g)
+    k2 = torch.searchsorted(zipf_cdf, torch.rand(n, dtype=torch.float64, device=dev, generator=g)) + 1
+    idx = torch.randint(0, 10_000, (n,), device=dev, generator=g)
+    ln = pool_len[idx]
+    offs = torch.zeros(n + 1, dtype=torch.int64, device=dev)
+    offs[1:] = torch.cumsum(ln, 0)
+    total = int(offs[-1].item())
+    src = torch.repeat_interleave(pool_off[idx] - offs[:-1], ln, output_size=total)
+    chars = pool_chars[src + torch.arange(total, device=dev)]
+    del src, ln, idx
+    k3 = Column(TypeId.STRING, n, chars, None, 0, 0, offs.to(torch.int32))
+    f32 = torch.rand(n, dtype=torch.float32, device=dev, generator=g)
+    i64 = torch.randint(-10**9, 10**9 + 1, (n,), dtype=torch.int64, device=dev, generator=g)
+    keys = ca.Table([_dev_col(k1), _dev_col(k2.to(torch.int64)), k3])
+    aggs = lambda: [Aggregation(Kind.SUM), Aggregation(Kind.COUNT_VALID), Aggregation(Kind.MAX)]
+    reqs = [ca.GroupByRequest(_dev_col(f32), aggs()), ca.GroupByRequest(_dev_col(i64), aggs())]
+    return keys, reqs, (chars, offs, k1, k2, f32, i64)
+
+
+def bench_c3(n=50_000_000, steps=3):
+    import torch
+    import cudf_amd as ca
+    from cudf_amd import _lib
+    keys, reqs, keep = make_c3(n)
+    gb = ca.GroupBy(keys)
+    _lib.set_phase_timing(True)
+    out = gb.aggregate(reqs)
+    del out
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        k, r = gb.aggregate(reqs)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    st = _lib.last_stats()
+    return {"workload": f"{n} rows, keys (int32 1..100, int64 Zipf(1.1) over 1M, string of 4..32 chars "
+                        f"from 10k), float32 and int64 values, sum/count/max",
+            "rows": n, "groups": k.num_rows(), "ms_per_step": ms, "rows_per_s": n / (ms * 1e-3),
+            "last_call": {kk: v for kk, v in st.items()}}
+
+
 def bench_served(n_req, loads=(0.5, 1.0, 2.0, 4.0), workers=4):
     """Plain kernel run serially over the stream, then the continuous-batching
     server at Poisson arrival rates that are multiples of the serial rate and
@@ -391,6 +450,8 @@ def main():
                     help="requests of the served workload (0 skips it)")
     ap.add_argument("--served-workers", type=int, default=4)
     ap.add_argument("--only-served", action="store_true", help="run the served workload alone")
+    ap.add_argument("--only-c3", action="store_true",
+                    help="run the 50M-row multi-column (string) key workload alone")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the group-by is measured on a "
@@ -406,6 +467,9 @@ def main():
     if args.only_batched:
         print(json.dumps({"batched": bench_batched(args.batched_instances, args.batched_reps,
                                                    not args.no_autotune, not args.no_plain)}), flush=True)
+        return
+    if args.only_c3:
+        print(json.dumps({"c3": bench_c3()}), flush=True)
         return
     if args.only_served:
         print(json.dumps({"served": bench_served(args.served_requests, workers=args.served_workers)}),

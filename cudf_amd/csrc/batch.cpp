@@ -134,7 +134,7 @@ void layout(mig_batch_plan& p)
     total_slots += I.cap + 2;
     I.out_base = out_rows;
     out_rows += p.region_rows(i);
-    n_items += (I.rows + P.tile_rows - 1) / P.tile_rows;
+    n_items += ((I.rows + P.tile_rows - 1) / P.tile_rows) * std::max(1, I.lds);
     inst_tile_begin[i] = n_tiles;
     n_tiles += int64_t((I.cap + 2 + SCAN_TILE - 1) / SCAN_TILE);
   }
@@ -149,7 +149,11 @@ void layout(mig_batch_plan& p)
   std::vector<uint64_t> tiles;
   tiles.reserve(size_t(n_tiles));
   for (int32_t i = 0; i < n; i++) {
-    for (int64_t r = 0; r < p.hinst[i].rows; r += P.tile_rows) items.push_back((uint64_t(i) << 40) | uint64_t(r));
+    // the parts of a tile are consecutive items, so the workgroups that read
+    // the tile for its parts run at about the same time (cache reuse)
+    for (int64_t r = 0; r < p.hinst[i].rows; r += P.tile_rows)
+      for (int32_t part = 0; part < std::max(1, p.hinst[i].lds); part++)
+        items.push_back((uint64_t(i) << 40) | (uint64_t(part) << 32) | uint64_t(r));
     for (uint64_t l = 0; l < p.hinst[i].cap + 2; l += SCAN_TILE) tiles.push_back((uint64_t(i) << 40) | l);
   }
 
@@ -200,7 +204,9 @@ void layout(mig_batch_plan& p)
   prepare_batch_aggregate(P.lds_bytes);
   if (p.opt.use_graph) {
     if (!p.capture_stream) MIG_HIP_TRY(hipStreamCreateWithFlags(&p.capture_stream, hipStreamNonBlocking));
-    MIG_HIP_TRY(hipStreamBeginCapture(p.capture_stream, hipStreamCaptureModeThreadLocal));
+    // relaxed: other threads (single calls, other plans) keep working while
+    // this thread captures
+    MIG_HIP_TRY(hipStreamBeginCapture(p.capture_stream, hipStreamCaptureModeRelaxed));
     try {
       enqueue(p, p.capture_stream);
     } catch (...) {
@@ -267,6 +273,7 @@ mig_batch_plan* batch_plan_create(mig_batch_instance const* instances, int32_t n
                                   hipStream_t stream)
 {
   MIG_EXPECTS(num_instances > 0, "a batch needs at least one instance");
+  MIG_EXPECTS(num_instances < (1 << 24), "too many instances in one batch");
   MIG_EXPECTS(instances != nullptr, "null instance array");
   MIG_EXPECTS(num_keys == 1, "the batched group-by takes exactly one key column");
   MIG_EXPECTS(null_policy == MIG_NULL_EXCLUDE || null_policy == MIG_NULL_INCLUDE, "invalid null policy");
@@ -553,7 +560,15 @@ mig_batch_plan* batch_plan_create(mig_batch_instance const* instances, int32_t n
   p.cap.resize(size_t(num_instances));
   for (int32_t i = 0; i < num_instances; i++) {
     p.cap[i]       = table_slots(p.est[i], p.hinst[i].rows, p.opt.load_factor);
-    p.hinst[i].lds = p.opt.lds_max_groups > 0 && p.est[i] <= double(p.opt.lds_max_groups) ? 1 : 0;
+    // LDS pre-aggregation with the keys split over as many hash parts as
+    // needed for each part to fit the LDS table (each part re-reads the rows,
+    // mostly from cache); beyond BATCH_MAX_PARTS the table is updated directly
+    int32_t parts = 0;
+    if (p.opt.lds_max_groups > 0) {
+      double const need = std::ceil(std::max(p.est[i], 1.0) / double(p.opt.lds_max_groups));
+      if (need <= double(BATCH_MAX_PARTS)) parts = int32_t(need);
+    }
+    p.hinst[i].lds = parts;
   }
   layout(p);
 

@@ -17,6 +17,7 @@ namespace mig {
 constexpr int BATCH_LDS_SLOTS = 1024;  // LDS table of the pre-aggregating path
 constexpr int BATCH_LDS_FILL  = 768;   // keys a tile may insert before rows bypass LDS
 constexpr int BATCH_ELEM      = 8;     // bytes per accumulator / output element slot
+constexpr int BATCH_MAX_PARTS = 64;    // hash parts of an LDS-aggregated instance
 
 // One instance.  Its table occupies slots [slot_base, slot_base + cap + 2) of
 // the concatenated slot arrays (slot_base + cap: key equal to EMPTY_KEY,
@@ -27,7 +28,8 @@ struct batch_inst {
   uint64_t cap;
   uint64_t slot_base;
   uint64_t out_base;
-  int32_t lds;         // 1 = pre-aggregate each tile in LDS
+  int32_t lds;         // >= 1: pre-aggregate in LDS, the keys split over this many
+                       // hash parts (one work item per tile and part); 0: direct
   int32_t pad;
   dcol key;
 };
@@ -46,7 +48,7 @@ struct batch_params {
   uint64_t total_slots;
   uint64_t out_rows;        // rows of the concatenated output regions
   uint32_t* queue;          // [0] next first-pass item, [1] next second-pass item
-  uint64_t* items;          // (instance << 40) | first row
+  uint64_t* items;          // (instance << 40) | (hash part << 32) | first row
   uint64_t* slot_tiles;     // (instance << 40) | first local slot
   uint32_t* tile_offsets;   // per slot tile: groups before it within its instance
   int64_t* inst_tile_begin; // per instance: first slot tile (n_inst + 1 entries)

@@ -63,6 +63,29 @@ __device__ __forceinline__ double sum_as_double(int32_t k, void const* a)
   return *static_cast<double const*>(a);
 }
 
+// Hash part of a row key for instances split over `parts` LDS passes
+// (multiplicative range map of a remixed hash, independent of the low bits
+// that place the key in the LDS and global tables).
+__device__ __forceinline__ uint32_t part_of(uint32_t h, uint32_t parts)
+{
+  return uint32_t((uint64_t(mm_fmix32(h ^ 0x9e3779b9u)) * parts) >> 32);
+}
+
+// Key hash of a row (null keys hash as the reference hashes a null).
+__device__ __forceinline__ uint32_t b_row_hash(batch_inst const& I, int64_t row)
+{
+  if (!dcol_valid(I.key, row)) return hash_combine(0u, NULL_HASH);
+  return b_key_hash(I.key.storage, load_fixed_bits(I.key, row));
+}
+
+// Work item: (instance << 40) | (hash part << 32) | first row.
+__device__ __forceinline__ void decode_item(uint64_t item, int& inst, uint32_t& part, int64_t& r0)
+{
+  inst = int(item >> 40);
+  part = uint32_t(item >> 32) & 0xffu;
+  r0   = int64_t(item & 0xffffffffull);
+}
+
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(BBLOCK)
   k_batch_sample(batch_params const* __restrict__ Bp, int32_t per_inst, uint64_t* tables, uint32_t table_slots,
@@ -260,9 +283,10 @@ __global__ void __launch_bounds__(NT) k_batch_aggregate(batch_params const* __re
   constexpr int WG      = __HIP_MEMORY_SCOPE_WORKGROUP;
   auto* tkeys           = reinterpret_cast<uint64_t*>(lds);
   while (next_item(B, 0, item_sh, kinds_sh, NT)) {
-    uint64_t const item = item_sh;
-    int const inst      = int(item >> 40);
-    int64_t const r0    = int64_t(item & ((1ull << 40) - 1));
+    int inst;
+    uint32_t part;
+    int64_t r0;
+    decode_item(item_sh, inst, part, r0);
     batch_inst const I  = B.inst[inst];
     int64_t const r1    = (r0 + B.tile_rows < I.rows) ? r0 + B.tile_rows : I.rows;
 
@@ -298,17 +322,20 @@ __global__ void __launch_bounds__(NT) k_batch_aggregate(batch_params const* __re
       present_sh[1] = 0;
     }
     __syncthreads();
+    uint32_t const parts = uint32_t(I.lds);
     for (int64_t row = r0 + threadIdx.x; row < r1; row += NT) {
       int id        = -1;
       uint64_t bits = 0;
       uint32_t h    = 0;
       if (!dcol_valid(I.key, row)) {
         if (B.skip_null_keys) continue;
+        if (parts > 1 && part_of(hash_combine(0u, NULL_HASH), parts) != part) continue;
         id            = S + 1;
         present_sh[1] = 1;
       } else {
         bits = load_fixed_bits(I.key, row);
         h    = b_key_hash(I.key.storage, bits);
+        if (parts > 1 && part_of(h, parts) != part) continue;  // another item's part
         if (bits == EMPTY_KEY) {
           id            = S;
           present_sh[0] = 1;
@@ -419,9 +446,10 @@ __global__ void __launch_bounds__(NT) k_batch_second(batch_params const* __restr
   constexpr uint64_t NO = ~0ull;  // empty marker of the LDS slot table
   auto* tslots          = reinterpret_cast<uint64_t*>(lds);
   while (next_item(B, 1, item_sh, kinds_sh, NT)) {
-    uint64_t const item = item_sh;
-    int const inst      = int(item >> 40);
-    int64_t const r0    = int64_t(item & ((1ull << 40) - 1));
+    int inst;
+    uint32_t part;
+    int64_t r0;
+    decode_item(item_sh, inst, part, r0);
     batch_inst const I  = B.inst[inst];
     int64_t const r1    = (r0 + B.tile_rows < I.rows) ? r0 + B.tile_rows : I.rows;
     bool const use_lds  = I.lds != 0;
@@ -434,7 +462,9 @@ __global__ void __launch_bounds__(NT) k_batch_second(batch_params const* __restr
       if (threadIdx.x == 0) fill_sh = 0;
       __syncthreads();
     }
+    uint32_t const parts = uint32_t(I.lds);
     for (int64_t row = r0 + threadIdx.x; row < r1; row += NT) {
+      if (parts > 1 && part_of(b_row_hash(I, row), parts) != part) continue;  // another item's part
       int64_t slot = -2;
       int id       = -2;
       for (int v = 0; v < B.num_vals; v++) {
@@ -678,16 +708,10 @@ void launch_batch_valid_counts(batch_params const* dB, batch_params const& hB, h
 
 void prepare_batch_aggregate(int32_t lds_bytes)
 {
-  if (lds_bytes > 65536) {
-    (void)hipFuncSetAttribute(reinterpret_cast<void const*>(&k_batch_aggregate<256>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, lds_bytes);
-    (void)hipFuncSetAttribute(reinterpret_cast<void const*>(&k_batch_aggregate<512>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, lds_bytes);
-    (void)hipFuncSetAttribute(reinterpret_cast<void const*>(&k_batch_second<256>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, lds_bytes);
-    (void)hipFuncSetAttribute(reinterpret_cast<void const*>(&k_batch_second<512>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, lds_bytes);
-  }
+  allow_dynamic_lds(reinterpret_cast<void const*>(&k_batch_aggregate<256>), lds_bytes);
+  allow_dynamic_lds(reinterpret_cast<void const*>(&k_batch_aggregate<512>), lds_bytes);
+  allow_dynamic_lds(reinterpret_cast<void const*>(&k_batch_second<256>), lds_bytes);
+  allow_dynamic_lds(reinterpret_cast<void const*>(&k_batch_second<512>), lds_bytes);
 }
 
 }  // namespace mig

@@ -8,6 +8,9 @@
 // var_hash_functor (groupby/hash/multi_pass_kernels.cuh:74-111).
 #include "device_ops.hpp"
 
+#include <map>
+#include <mutex>
+
 namespace mig {
 
 constexpr int BLOCK = 256;
@@ -474,22 +477,34 @@ __global__ void __launch_bounds__(LBLOCK) k_second_pass_lds(gb_params const* __r
   }
 }
 
-static void allow_lds(void const* fn, int bytes)
+void allow_dynamic_lds(void const* fn, int bytes)
 {
-  if (bytes > 64 * 1024) MIG_HIP_TRY(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+  if (bytes <= 64 * 1024) return;
+  // Raised once per kernel and device (and only upwards): setting a function
+  // attribute is not a stream operation, but other threads may be capturing
+  // graphs, so it is kept off the per-call path.
+  static std::mutex mu;
+  static std::map<std::pair<void const*, int>, int> done;
+  int dev = 0;
+  MIG_HIP_TRY(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(mu);
+  int& cur = done[{fn, dev}];
+  if (bytes <= cur) return;
+  MIG_HIP_TRY(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+  cur = bytes;
 }
 
 void launch_aggregate_lds(gb_params const* dP, gb_params const& hP, int64_t blocks, hipStream_t s)
 {
   if (hP.num_rows <= 0) return;
-  allow_lds(reinterpret_cast<void const*>(&k_aggregate_lds), hP.lds_bytes);
+  allow_dynamic_lds(reinterpret_cast<void const*>(&k_aggregate_lds), hP.lds_bytes);
   hipLaunchKernelGGL(k_aggregate_lds, dim3(blocks), dim3(LBLOCK), hP.lds_bytes, s, dP);
 }
 
 void launch_second_pass_lds(gb_params const* dP, gb_params const& hP, int64_t blocks, hipStream_t s)
 {
   if (hP.num_rows <= 0) return;
-  allow_lds(reinterpret_cast<void const*>(&k_second_pass_lds), hP.lds_bytes);
+  allow_dynamic_lds(reinterpret_cast<void const*>(&k_second_pass_lds), hP.lds_bytes);
   hipLaunchKernelGGL(k_second_pass_lds, dim3(blocks), dim3(LBLOCK), hP.lds_bytes, s, dP);
 }
 
@@ -799,28 +814,31 @@ void launch_exclusive_scan_i32(int32_t* data, int64_t n, int32_t* total, hipStre
   MIG_HIP_TRY(hipFreeAsync(sums, s));
 }
 
-// One wave copies the chars of one group's representative string.
+// Chars of each group's representative string: GATHER_LANES lanes per group
+// (strings are short; a wave serves 64 / GATHER_LANES groups).
+constexpr int GATHER_LANES = 4;
+
 __global__ void __launch_bounds__(BLOCK)
   k_gather_chars(dcol src, uint32_t const* __restrict__ rep_rows, int64_t G,
                  int32_t const* __restrict__ out_offsets, uint8_t* out_chars)
 {
-  int64_t const g = (int64_t(blockIdx.x) * BLOCK + threadIdx.x) >> 6;
-  int const lane  = threadIdx.x & 63;
+  int64_t const t = int64_t(blockIdx.x) * BLOCK + threadIdx.x;
+  int64_t const g = t / GATHER_LANES;
+  int const lane  = int(t % GATHER_LANES);
   if (g >= G) return;
-  int64_t const r = rep_rows[g];
-  int64_t const o = out_offsets[g];
+  int64_t const o   = out_offsets[g];
   int64_t const len = int64_t(out_offsets[g + 1]) - o;
   if (len == 0) return;
   int64_t b, e;
-  str_span(src, r, b, e);
-  for (int64_t i = lane; i < len; i += 64) out_chars[o + i] = src.data[b + i];
+  str_span(src, int64_t(rep_rows[g]), b, e);
+  for (int64_t i = lane; i < len; i += GATHER_LANES) out_chars[o + i] = src.data[b + i];
 }
 
 void launch_gather_string_chars(dcol const& src, uint32_t const* rep_rows, int64_t G,
                                 int32_t const* out_offsets, uint8_t* out_chars, hipStream_t s)
 {
   if (G <= 0) return;
-  int64_t blocks = (G * 64 + BLOCK - 1) / BLOCK;
+  int64_t blocks = (G * GATHER_LANES + BLOCK - 1) / BLOCK;
   hipLaunchKernelGGL(k_gather_chars, dim3(blocks), dim3(BLOCK), 0, s, src, rep_rows, G, out_offsets,
                      out_chars);
 }

@@ -198,6 +198,9 @@ void launch_hash_rows(dcol const* d_cols, int ncols, int64_t n, uint32_t seed, u
 void launch_sample_hash(gb_params const* dP, int64_t n, int64_t nsample, uint64_t* table,
                         uint32_t table_mask, uint32_t* counters, hipStream_t s);
 void launch_init_state(gb_params const* dP, gb_params const& hP, hipStream_t s);
+// Allows `bytes` of dynamic LDS for kernel `fn` (above the 64 KB default;
+// gfx950 has 160 KB per CU).
+void allow_dynamic_lds(void const* fn, int bytes);
 void launch_aggregate(gb_params const* dP, gb_params const& hP, int num_cus, hipStream_t s);
 // Low-cardinality form of launch_aggregate / launch_second_pass (inline keys,
 // hP.lds_bytes > 0): `blocks` workgroups pre-aggregate their rows in LDS.

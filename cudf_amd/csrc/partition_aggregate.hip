@@ -548,9 +548,7 @@ __global__ void __launch_bounds__(RPA_AGG_BLOCK) __attribute__((amdgpu_waves_per
 void launch_rpa_aggregate(gb_params const* dP, rpa_params const* dR, rpa_params const& hR, hipStream_t s)
 {
   int64_t const B = int64_t(1) << hR.log_bins;
-  if (hR.lds_bytes > 64 * 1024)
-    MIG_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<void const*>(&k_rpa_aggregate),
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, hR.lds_bytes));
+  allow_dynamic_lds(reinterpret_cast<void const*>(&k_rpa_aggregate), hR.lds_bytes);
   hipLaunchKernelGGL(k_rpa_aggregate, dim3(B), dim3(RPA_AGG_BLOCK), hR.lds_bytes, s, dP, dR);
 }
 

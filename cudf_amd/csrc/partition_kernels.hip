@@ -79,9 +79,7 @@ void launch_rpa_hist(gb_params const* dP, rpa_params const* dR, rpa_params const
                      int num_cus, hipStream_t s)
 {
   size_t lds = sizeof(uint32_t) << hR.log_bins;
-  if (lds > 64 * 1024)
-    MIG_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<void const*>(&k_rpa_hist),
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
+  allow_dynamic_lds(reinterpret_cast<void const*>(&k_rpa_hist), int(lds));
   hipLaunchKernelGGL(k_rpa_hist, dim3(hR.hist_blocks), dim3(RPA_HIST_BLOCK), lds, s, dP, dR, n);
   int64_t const B = int64_t(1) << hR.log_bins;
   hipLaunchKernelGGL(k_rpa_hist_reduce, dim3((B + RPA_SMALL_BLOCK - 1) / RPA_SMALL_BLOCK, RPA_HIST_SLICES),
@@ -288,11 +286,7 @@ __global__ void __launch_bounds__(RPA_SCATTER_BLOCK)
 }
 
 // Dynamic LDS above 64 KB must be opted into per kernel (gfx950 has 160 KB).
-static void allow_large_lds(void const* fn, int bytes)
-{
-  if (bytes > 64 * 1024)
-    MIG_HIP_TRY(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
-}
+static void allow_large_lds(void const* fn, int bytes) { allow_dynamic_lds(fn, bytes); }
 
 // Workgroups of a scatter kernel that are resident at once: the tiles are
 // strided over the grid, so a grid larger than what fits would leave its last

@@ -73,6 +73,7 @@ def test_batch_matches_oracle(null_policy):
 
 @pytest.mark.parametrize("opts", [
     dict(lds_max_groups=-1), dict(block_size=512), dict(tile_rows=256), dict(use_graph=-1),
+    dict(lds_max_groups=64, tile_rows=1024),
     dict(load_factor=0.9, lds_max_groups=768, tile_rows=16384),
 ])
 def test_batch_options(opts):
@@ -212,7 +213,8 @@ def test_batch_mixed_types_with_variance(null_policy):
 def test_batch_mixed_types_options():
     from cudf_amd import BatchOptions
     inst = make_mixed(37, 20, 30_000)
-    for opts in (dict(lds_max_groups=-1), dict(block_size=512, tile_rows=1024), dict(use_graph=-1)):
+    for opts in (dict(lds_max_groups=-1), dict(block_size=512, tile_rows=1024), dict(use_graph=-1),
+                 dict(lds_max_groups=96)):
         plan = run_batch(inst, [REQ_VAR0, REQ_VAR1], 0, BatchOptions(**opts))
         check_all(plan, inst, [REQ_VAR0, REQ_VAR1], 0)
 
