@@ -117,8 +117,9 @@ void enqueue(mig_batch_plan& p, hipStream_t s)
   MIG_HIP_TRY(hipGetLastError());
 }
 
-// Tables, outputs, work lists and the parameter block for the current `cap`.
-void layout(mig_batch_plan& p)
+// Tables, outputs, work lists and the parameter block for the current `cap`
+// (uploaded on `stream`, which is synchronised before the graph is captured).
+void layout(mig_batch_plan& p, hipStream_t stream)
 {
   p.release_layout();
   batch_params& P = p.P;
@@ -188,8 +189,8 @@ void layout(mig_batch_plan& p)
       vals[k]        = make_dcol(p.val_views[k], p.val_nulls[k] != 0);
     }
 
-  auto up = [](void* dst, void const* src, size_t bytes) {
-    if (bytes) MIG_HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  auto up = [&](void* dst, void const* src, size_t bytes) {
+    if (bytes) MIG_HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream));
   };
   up(P.items, items.data(), items.size() * sizeof(uint64_t));
   up(P.slot_tiles, tiles.data(), tiles.size() * sizeof(uint64_t));
@@ -201,6 +202,7 @@ void layout(mig_batch_plan& p)
   p.dP = p.alloc<batch_params>(1);
   up(p.dP, &P, sizeof(batch_params));
 
+  MIG_HIP_TRY(hipStreamSynchronize(stream));
   prepare_batch_aggregate(P.lds_bytes);
   if (p.opt.use_graph) {
     if (!p.capture_stream) MIG_HIP_TRY(hipStreamCreateWithFlags(&p.capture_stream, hipStreamNonBlocking));
@@ -213,9 +215,14 @@ void layout(mig_batch_plan& p)
       hipGraph_t g = nullptr;
       (void)hipStreamEndCapture(p.capture_stream, &g);
       if (g) (void)hipGraphDestroy(g);
+      (void)hipGetLastError();  // leave no sticky capture error to the thread's next call
       throw;
     }
-    MIG_HIP_TRY(hipStreamEndCapture(p.capture_stream, &p.graph));
+    hipError_t const e = hipStreamEndCapture(p.capture_stream, &p.graph);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      fail(MIG_ERROR_HIP, std::string("graph capture failed: ") + hipGetErrorName(e));
+    }
     MIG_HIP_TRY(hipGraphInstantiate(&p.exec, p.graph, nullptr, nullptr, 0));
   }
 }
@@ -280,6 +287,7 @@ mig_batch_plan* batch_plan_create(mig_batch_instance const* instances, int32_t n
   MIG_EXPECTS(num_requests >= 0 && num_requests <= MAX_VALS, "number of requests out of range");
   MIG_EXPECTS(num_requests == 0 || (aggregations && num_aggregations), "null aggregation arrays");
 
+  (void)hipGetLastError();  // a launch error reported below belongs to this call
   auto plan         = std::make_unique<mig_batch_plan>();
   mig_batch_plan& p = *plan;
   MIG_HIP_TRY(hipGetDevice(&p.dev));
@@ -570,7 +578,7 @@ mig_batch_plan* batch_plan_create(mig_batch_instance const* instances, int32_t n
     }
     p.hinst[i].lds = parts;
   }
-  layout(p);
+  layout(p, stream);
 
   size_t const words = size_t(num_instances) * (2 + size_t(no + 1));
   MIG_HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&p.pinned), words * sizeof(uint32_t), hipHostMallocDefault));
@@ -613,7 +621,7 @@ void batch_synchronize(mig_batch_plan* plan, hipStream_t stream)
     if (!grew) break;
     MIG_EXPECTS(attempt < 16, "batched group-by: tables kept overflowing");
     p.retries++;
-    layout(p);
+    layout(p, stream);
     batch_execute(plan, stream);
   }
   p.groups.assign(p.pinned, p.pinned + n);

@@ -502,6 +502,7 @@ std::unique_ptr<mig_groupby_result> groupby_aggregate(mig_column_view const* key
 {
   double* stats = thread_stats();
   std::fill(stats, stats + STAT_COUNT, 0.0);
+  (void)hipGetLastError();  // a launch error reported below belongs to this call
 
   MIG_EXPECTS(num_keys >= 0 && num_keys <= MAX_KEYS, "number of key columns out of range");
   MIG_EXPECTS(num_requests >= 0, "negative number of requests");

@@ -30,7 +30,7 @@ from __future__ import annotations
 import threading
 import time
 from concurrent.futures import Future
-from dataclasses import dataclass, field
+from dataclasses import dataclass, field, replace
 from typing import List, Optional, Sequence, Tuple
 
 import torch
@@ -233,7 +233,10 @@ class GroupByServer:
     def _run_batch(self, queries: List[Query], stream) -> None:
         aggs = [list(r.aggregations) for r in queries[0].requests]
         inst = [(q.keys.columns()[0], [r.values for r in q.requests]) for q in queries]
-        plan = BatchedGroupBy(inst, aggs, queries[0].null_policy, self.options, stream=stream)
+        # a unit's plan runs once: launching its kernels directly is cheaper
+        # than capturing a graph for one replay
+        opts = replace(self.options or BatchOptions(), use_graph=-1)
+        plan = BatchedGroupBy(inst, aggs, queries[0].null_policy, opts, stream=stream)
         try:
             plan.run(stream)
             outs = [plan.result(i) for i in range(len(queries))]
