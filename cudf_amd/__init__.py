@@ -4,7 +4,7 @@ The compute path is ``cudf_amd/lib/libmig_groupby.so`` (HIP kernels behind
 the C ABI of ``include/mig_groupby.h``); this package is the Python mirror of
 pylibcudf's ``GroupBy`` / ``aggregation`` API on top of it.
 """
-from . import aggregation
+from . import aggregation, interop, pandas_compat
 from .batch import BatchedGroupBy, BatchOptions
 from .column import Column, Table
 from .groupby import GroupBy, GroupByRequest, murmurhash3_x86_32
