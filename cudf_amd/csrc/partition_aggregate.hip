@@ -38,8 +38,11 @@ __device__ __forceinline__ void for_rows(uint32_t ok, F&& f)
 // One accumulator over the rows of a chunk: `c[k]` holds row k's value in
 // canonical form (integers sign- or zero-extended to 64 bits, float bits
 // unchanged), `slot[k]` its group slot, bit k of `ok` whether it counts.
-__device__ __forceinline__ void acc_rows_lds(field_desc const& F, uint8_t* lds, int const (&slot)[RPA_CACHED],
-                                             uint32_t ok, uint64_t const (&c)[RPA_CACHED])
+__device__ __forceinline__ double canon_double(int32_t st, uint64_t c);
+
+__device__ __forceinline__ void acc_rows_lds(field_desc const& F, int32_t st, uint8_t* lds,
+                                             int const (&slot)[RPA_CACHED], uint32_t ok,
+                                             uint64_t const (&c)[RPA_CACHED])
 {
   uint8_t* const base = lds + F.lds_off;
   auto u64 = [&](int k) { return reinterpret_cast<uint64_t*>(base) + slot[k]; };
@@ -113,7 +116,13 @@ __device__ __forceinline__ void acc_rows_lds(field_desc const& F, uint8_t* lds, 
       });
       break;
     case FK_COUNT: for_rows(ok, [&](int k) { __hip_atomic_fetch_add(u32(k), 1u, R, WG_SCOPE); }); break;
-    default: break;  // FK_M2: pass B
+    case FK_M2:  // pass A: sum of squares in double (finalised into M2 after the pass)
+      for_rows(ok, [&](int k) {
+        double const x = canon_double(st, c[k]);
+        __hip_atomic_fetch_add(reinterpret_cast<double*>(u64(k)), x * x, R, WG_SCOPE);
+      });
+      break;
+    default: break;
   }
 }
 
@@ -164,7 +173,7 @@ struct rpa_misc {
   uint32_t present1;  // a key equal to EMPTY_KEY was seen
   int32_t overflow;
   uint32_t g_out;
-  uint32_t pad;
+  uint32_t m2_pending;  // some group's M2 needs the exact second pass
   unsigned long long base;
   uint32_t wave_sum[RPA_AGG_BLOCK / 64];
 };
@@ -312,7 +321,7 @@ __device__ __forceinline__ void chunk_accumulate(gb_params const& P, rpa_params 
 #pragma unroll
       for (int k = 0; k < RPA_CACHED; k++) ok &= ~(uint32_t(((flp >> (8 * k + bit)) & 1) == 0) << k);
     }
-    for (int f = fb; f < fe; f++) acc_rows_lds(P.fields[f], lds, slot, ok, c);
+    for (int f = fb; f < fe; f++) acc_rows_lds(P.fields[f], st, lds, slot, ok, c);
   }
 }
 
@@ -321,9 +330,14 @@ __device__ __forceinline__ void chunk_deviations(gb_params const& P, rpa_params 
                                                  int64_t r0, int64_t e, int const (&slot)[RPA_CACHED],
                                                  uint64_t flp)
 {
-  uint32_t live = 0;
+  uint8_t const* m2_flag = lds + R.slot_of_j_off;  // groups whose M2 is recomputed
+  uint32_t live = 0, flagged = 0;
 #pragma unroll
-  for (int k = 0; k < RPA_CACHED; k++) live |= uint32_t(slot[k] >= 0) << k;
+  for (int k = 0; k < RPA_CACHED; k++) {
+    live |= uint32_t(slot[k] >= 0) << k;
+    flagged |= uint32_t(slot[k] >= 0 && m2_flag[slot[k]] != 0) << k;
+  }
+  if (flagged == 0) return;
   for (int v = 0; v < P.num_vals; v++) {
     int const fb = P.val_field_begin[v], fe = P.val_field_begin[v + 1];
     int m2 = -1;
@@ -346,7 +360,7 @@ __device__ __forceinline__ void chunk_deviations(gb_params const& P, rpa_params 
 #pragma unroll
       for (int k = 0; k < RPA_CACHED; k++) ok &= ~(uint32_t(((flp >> (8 * k + bit)) & 1) == 0) << k);
     }
-    for_rows(ok, [&](int k) {
+    for_rows(ok & flagged, [&](int k) {
       double const x    = canon_double(st, c[k]);
       double const cnt  = double(lds_count(P, R, lds, F.aux_count, slot[k]));
       double const mean = lds_field_double(P.fields[F.aux_sum], lds, slot[k]) / cnt;
@@ -354,6 +368,37 @@ __device__ __forceinline__ void chunk_deviations(gb_params const& P, rpa_params 
       __hip_atomic_fetch_add(lds_at<double>(lds, F.lds_off, slot[k]), d * d, __ATOMIC_RELAXED, WG_SCOPE);
     });
   }
+}
+
+// M2 of a group from its pass-A sums: `*m2` holds Σx² (double) on entry and
+// Σ(x - mean)² on success.  Integer sources whose Σx² stays below 2^53 (then
+// every partial sum and square is exact in double) use the exact integer
+// identity n·M2 = n·Σx² − (Σx)² in 128 bits.  Float64 sources use
+// Σx² − (Σx)²/n when its rounding error, at most ~3·n·ε·Σx², is below 1e-11
+// of the result.  Anything else (float32 sums, large or ill-conditioned
+// groups) returns false: the exact second pass computes that group.
+__device__ __forceinline__ bool m2_from_sums(field_desc const& SUMF, int32_t st, uint8_t* lds, int i, uint32_t n,
+                                             double* m2)
+{
+  double const s2 = *m2;
+  if (n <= 1) {  // no deviation from the mean of one value
+    *m2 = 0.0;
+    return true;
+  }
+  if (st == ST_F32) return false;
+  if (st == ST_F64) {
+    if (n > 256) return false;
+    double const s1 = *lds_at<double>(lds, SUMF.lds_off, i);
+    double const v  = s2 - s1 * (s1 / double(n));
+    if (!(v > 0.0) || 3.0 * double(n) * 1.1102230246251565e-16 * s2 > 1e-11 * v) return v == 0.0 && s2 == 0.0;
+    *m2 = v;
+    return true;
+  }
+  if (SUMF.kind != FK_SUM_I64 || !(s2 < 9007199254740992.0)) return false;
+  int64_t const s1        = *lds_at<int64_t>(lds, SUMF.lds_off, i);
+  __int128 const num      = __int128(n) * __int128(uint64_t(s2)) - __int128(s1) * __int128(s1);
+  *m2 = double(num) / double(n);
+  return true;
 }
 
 __global__ void __launch_bounds__(RPA_AGG_BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
@@ -419,8 +464,33 @@ __global__ void __launch_bounds__(RPA_AGG_BLOCK) __attribute__((amdgpu_waves_per
     return;
   }
 
-  // ---- pass B: squared deviations from the group mean
+  // ---- M2 from the pass-A sums where that is exact or well conditioned;
+  // the other groups are flagged for the exact second pass
   if (P.need_second_pass) {
+    uint8_t* m2_flag = lds + R.slot_of_j_off;  // the compaction map's bytes, free until then
+    if (tid == 0) misc->m2_pending = 0;
+    __syncthreads();
+    for (int i = tid; i < NS; i += RPA_AGG_BLOCK) {
+      bool redo = false;
+      for (int f = 0; f < P.num_fields; f++) {
+        field_desc const& F = P.fields[f];
+        if (F.kind != FK_M2) continue;
+        double* m2       = lds_at<double>(lds, F.lds_off, i);
+        uint32_t const n = lds_count(P, R, lds, F.aux_count, i);
+        redo |= !m2_from_sums(P.fields[F.aux_sum], P.vals[F.vcol].storage, lds, i, n, m2);
+      }
+      m2_flag[i] = redo ? 1 : 0;
+      if (redo) {
+        for (int f = 0; f < P.num_fields; f++)
+          if (P.fields[f].kind == FK_M2) *lds_at<double>(lds, P.fields[f].lds_off, i) = 0.0;
+        __hip_atomic_store(&misc->m2_pending, 1u, __ATOMIC_RELAXED, WG_SCOPE);
+      }
+    }
+    __syncthreads();
+  }
+
+  // ---- pass B: squared deviations from the group mean, for flagged groups
+  if (P.need_second_pass && misc->m2_pending) {
     int slot0[RPA_CACHED];
 #pragma unroll
     for (int k = 0; k < RPA_CACHED; k++) slot0[k] = int((sp0[k / 2] >> (16 * (k & 1))) & 0xffffu) - 1;

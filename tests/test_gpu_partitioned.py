@@ -119,3 +119,59 @@ def test_all_keys_null_excluded():
     kv = np.zeros(n, dtype=bool)
     kcols, res = gpu_groupby([host_col(k, kv)], [(host_col(np.ones(n, np.int32)), [(AGG["SUM"], 1)])])
     assert len(kcols[0][0]) == 0 and len(res[0][0][0]) == 0
+
+
+def _exact_var(keys, vals, ddof):
+    """Per-key exact variance (Python fractions) of the valid values."""
+    from fractions import Fraction
+    groups = {}
+    for k, x in zip(keys, vals):
+        groups.setdefault(k, []).append(x)
+    out = {}
+    for k, xs in groups.items():
+        n = len(xs)
+        if n - ddof <= 0:
+            out[k] = None
+            continue
+        fx = [Fraction(int(x)) if isinstance(x, (int, np.integer)) else Fraction(float(x)) for x in xs]
+        mean = sum(fx) / n
+        out[k] = float(sum((x - mean) ** 2 for x in fx) / (n - ddof))
+    return out
+
+
+@pytest.mark.parametrize("case", ["f64_small_mean", "f64_large_mean", "i32_exact", "i64_huge",
+                                  "f32", "constant_groups"])
+def test_variance_one_pass_and_exact_fallback(case):
+    """The partitioned path finalises M2 from the first pass's sums where
+    that is exact (integers with Σx² < 2^53) or provably within 1e-11
+    (float64, small well-conditioned groups), and runs the exact second pass
+    for every other group.  Variance is checked strictly, relative 1e-10
+    against exact rational arithmetic (no absolute floor)."""
+    rng = np.random.default_rng({"f64_small_mean": 1, "f64_large_mean": 2, "i32_exact": 3,
+                                 "i64_huge": 4, "f32": 5, "constant_groups": 6}[case])
+    n, card = 60_000, 6_000
+    k = rng.integers(0, card, n).astype(np.int64)
+    if case == "f64_small_mean":
+        v = rng.standard_normal(n)
+    elif case == "f64_large_mean":
+        v = 1e6 + rng.standard_normal(n)          # Σx² / M2 ~ 1e12: ill-conditioned
+    elif case == "i32_exact":
+        v = rng.integers(-10_000, 10_000, n).astype(np.int32)
+    elif case == "i64_huge":
+        v = rng.integers(-10**15, 10**15, n).astype(np.int64)  # Σx² beyond 2^53
+    elif case == "f32":
+        v = rng.random(n).astype(np.float32)
+    else:
+        v = np.full(n, 3.25)
+    kcols, res = gpu_groupby([host_col(k)], [(host_col(v), [(AGG["VARIANCE"], 1), (AGG["COUNT_VALID"], 1)])])
+    assert _stats()["strategy"] == 2.0
+    want = _exact_var(k.tolist(), v.tolist() if v.dtype.kind != "f" else v, 1)
+    gk = kcols[0][0]
+    var, valid = res[0][0]
+    for key, got, ok in zip(gk.tolist(), var, valid):
+        w = want[key]
+        if w is None:
+            assert not ok
+            continue
+        assert ok
+        assert abs(got - w) <= 1e-10 * abs(w) + (0.0 if w != 0 else 1e-300), (case, key, got, w)
