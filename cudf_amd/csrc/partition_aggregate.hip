@@ -438,20 +438,9 @@ __global__ void __launch_bounds__(RPA_AGG_BLOCK) __attribute__((amdgpu_waves_per
   }
   __syncthreads();
 
-  // ---- pass A: find-or-insert and accumulate, chunk by chunk; the slots of
-  // the first chunk stay in registers for pass B
+  // ---- pass A: find-or-insert and accumulate, chunk by chunk
   constexpr int64_t CHUNK = int64_t(RPA_CACHED) * RPA_AGG_BLOCK;
-  uint32_t sp0[RPA_CACHED / 2];  // first chunk's slots + 1, two 16-bit halves per word
-  uint64_t fl0;
-  {
-    int slot0[RPA_CACHED];
-    chunk_slots<true>(P, R, tkeys, misc, S, s + tid, e, slot0, fl0);
-    chunk_accumulate(P, R, lds, s + tid, e, slot0, fl0);
-#pragma unroll
-    for (int k = 0; k < RPA_CACHED / 2; k++)
-      sp0[k] = uint32_t(slot0[2 * k] + 1) | (uint32_t(slot0[2 * k + 1] + 1) << 16);
-  }
-  for (int64_t c = s + CHUNK; c < e; c += CHUNK) {
+  for (int64_t c = s; c < e; c += CHUNK) {
     if (__hip_atomic_load(&misc->overflow, __ATOMIC_RELAXED, WG_SCOPE)) break;
     int slot[RPA_CACHED];
     uint64_t fl;
@@ -491,11 +480,7 @@ __global__ void __launch_bounds__(RPA_AGG_BLOCK) __attribute__((amdgpu_waves_per
 
   // ---- pass B: squared deviations from the group mean, for flagged groups
   if (P.need_second_pass && misc->m2_pending) {
-    int slot0[RPA_CACHED];
-#pragma unroll
-    for (int k = 0; k < RPA_CACHED; k++) slot0[k] = int((sp0[k / 2] >> (16 * (k & 1))) & 0xffffu) - 1;
-    chunk_deviations(P, R, lds, s + tid, e, slot0, fl0);
-    for (int64_t c = s + CHUNK; c < e; c += CHUNK) {
+    for (int64_t c = s; c < e; c += CHUNK) {
       int slot[RPA_CACHED];
       uint64_t fl;
       chunk_slots<false>(P, R, tkeys, misc, S, c + tid, e, slot, fl);
