@@ -484,7 +484,7 @@ __global__ void __launch_bounds__(NT) k_batch_second(batch_params const* __restr
           if (B.fields[f].kind != FK_M2) continue;
           double const t = b_dev2(B, kinds_sh, f, slot, val);
           if (id >= 0) __hip_atomic_fetch_add(lds_at<double>(lds, B.lds_m2_off[f], id), t, __ATOMIC_RELAXED, WG);
-          else unsafeAtomicAdd(reinterpret_cast<double*>(fa(B.fields[f], slot)), t);
+          else gadd_f64(reinterpret_cast<double*>(fa(B.fields[f], slot)), t);
         }
       }
     }
@@ -496,7 +496,7 @@ __global__ void __launch_bounds__(NT) k_batch_second(batch_params const* __restr
         for (int f = 0; f < B.num_fields; f++) {
           if (B.fields[f].kind != FK_M2) continue;
           double const part = *lds_at<double>(lds, B.lds_m2_off[f], i);
-          if (part != 0.0) unsafeAtomicAdd(reinterpret_cast<double*>(fa(B.fields[f], int64_t(slot))), part);
+          if (part != 0.0) gadd_f64(reinterpret_cast<double*>(fa(B.fields[f], int64_t(slot))), part);
         }
       }
       __syncthreads();

@@ -125,11 +125,25 @@ struct dcol {
   int32_t pad;
 };
 
+// Global-memory view of a device pointer.  Pointers read from parameter
+// structs in device memory are generic to the compiler, and a generic (flat)
+// access is counted by the LDS / scalar-load wait counter as well as the
+// vector-memory one: every wait for such a load would also wait for the LDS
+// operations in flight, and every wait for LDS data for the loads.  Accesses
+// through gp(...) compile to global_load / global_store / global_atomic.
+template <typename T>
+using gmem = __attribute__((address_space(1))) T;
+template <typename T>
+__device__ __forceinline__ gmem<T>* gp(T* p)
+{
+  return (gmem<T>*)p;
+}
+
 __device__ __forceinline__ bool dcol_valid(dcol const& c, int64_t row)
 {
   if (c.mask == nullptr) return true;
   int64_t b = row + c.mask_offset;
-  return (c.mask[b >> 5] >> (b & 31)) & 1u;
+  return (gp(c.mask)[b >> 5] >> (b & 31)) & 1u;
 }
 
 }  // namespace mig

@@ -15,33 +15,42 @@ namespace mig {
 template <typename T>
 __device__ __forceinline__ T aload(T const* p)
 {
-  return __hip_atomic_load(const_cast<T*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return __hip_atomic_load(gp(const_cast<T*>(p)), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 template <typename T>
 __device__ __forceinline__ void astore(T* p, T v)
 {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(gp(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ uint64_t acas64(uint64_t* p, uint64_t expected, uint64_t desired)
 {
-  __hip_atomic_compare_exchange_strong(p, &expected, desired, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+  __hip_atomic_compare_exchange_strong(gp(p), &expected, desired, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
                                        __HIP_MEMORY_SCOPE_AGENT);
   return expected;
 }
 template <typename T>
 __device__ __forceinline__ void aadd(T* p, T v)
 {
-  __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_fetch_add(gp(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 template <typename T>
 __device__ __forceinline__ void amin(T* p, T v)
 {
-  __hip_atomic_fetch_min(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_fetch_min(gp(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 template <typename T>
 __device__ __forceinline__ void amax(T* p, T v)
 {
-  __hip_atomic_fetch_max(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_fetch_max(gp(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void gadd_f32(float* p, float v)
+{
+  __hip_atomic_fetch_add(gp(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void gadd_f64(double* p, double v)
+{
+  __hip_atomic_fetch_add(gp(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Order-preserving unsigned encodings of float bit patterns.
@@ -60,18 +69,19 @@ __device__ __forceinline__ uint64_t unord64(uint64_t o)
 // element access
 __device__ __forceinline__ uint64_t load_fixed_bits(dcol const& c, int64_t row)
 {
+  auto const d = gp(c.data);
   switch (c.storage) {
     case ST_I8:
-    case ST_U8: return c.data[row];
-    case ST_BOOL: return c.data[row] != 0;
+    case ST_U8: return d[row];
+    case ST_BOOL: return d[row] != 0;
     case ST_I16:
-    case ST_U16: return reinterpret_cast<uint16_t const*>(c.data)[row];
+    case ST_U16: return reinterpret_cast<gmem<uint16_t const>*>(d)[row];
     case ST_I32:
-    case ST_U32: return reinterpret_cast<uint32_t const*>(c.data)[row];
-    case ST_F32: return normalize_f32_bits(reinterpret_cast<uint32_t const*>(c.data)[row]);
+    case ST_U32: return reinterpret_cast<gmem<uint32_t const>*>(d)[row];
+    case ST_F32: return normalize_f32_bits(reinterpret_cast<gmem<uint32_t const>*>(d)[row]);
     case ST_I64:
-    case ST_U64: return reinterpret_cast<uint64_t const*>(c.data)[row];
-    case ST_F64: return normalize_f64_bits(reinterpret_cast<uint64_t const*>(c.data)[row]);
+    case ST_U64: return reinterpret_cast<gmem<uint64_t const>*>(d)[row];
+    case ST_F64: return normalize_f64_bits(reinterpret_cast<gmem<uint64_t const>*>(d)[row]);
     default: return 0;
   }
 }
@@ -89,11 +99,11 @@ __device__ __forceinline__ uint32_t hash_fixed_bits(int32_t st, uint64_t bits, u
 __device__ __forceinline__ void str_span(dcol const& c, int64_t row, int64_t& b, int64_t& e)
 {
   if (c.offsets64) {
-    auto o = reinterpret_cast<int64_t const*>(c.offsets);
+    auto o = gp(reinterpret_cast<int64_t const*>(c.offsets));
     b = o[row];
     e = o[row + 1];
   } else {
-    auto o = reinterpret_cast<int32_t const*>(c.offsets);
+    auto o = gp(reinterpret_cast<int32_t const*>(c.offsets));
     b = o[row];
     e = o[row + 1];
   }
@@ -175,30 +185,31 @@ struct loaded_value {
 __device__ __forceinline__ loaded_value load_value(dcol const& c, int64_t row)
 {
   loaded_value v{0, 0.0, 0.0f, 0};
+  auto const d = gp(c.data);
   switch (c.storage) {
-    case ST_I8: v.i = reinterpret_cast<int8_t const*>(c.data)[row]; v.d = double(v.i); break;
-    case ST_I16: v.i = reinterpret_cast<int16_t const*>(c.data)[row]; v.d = double(v.i); break;
-    case ST_I32: v.i = reinterpret_cast<int32_t const*>(c.data)[row]; v.d = double(v.i); break;
-    case ST_I64: v.i = reinterpret_cast<int64_t const*>(c.data)[row]; v.d = double(v.i); break;
-    case ST_U8: v.i = c.data[row]; v.d = double(v.i); break;
-    case ST_BOOL: v.i = c.data[row] != 0; v.d = double(v.i); break;
-    case ST_U16: v.i = reinterpret_cast<uint16_t const*>(c.data)[row]; v.d = double(v.i); break;
-    case ST_U32: v.i = reinterpret_cast<uint32_t const*>(c.data)[row]; v.d = double(v.i); break;
+    case ST_I8: v.i = reinterpret_cast<gmem<int8_t const>*>(d)[row]; v.d = double(v.i); break;
+    case ST_I16: v.i = reinterpret_cast<gmem<int16_t const>*>(d)[row]; v.d = double(v.i); break;
+    case ST_I32: v.i = reinterpret_cast<gmem<int32_t const>*>(d)[row]; v.d = double(v.i); break;
+    case ST_I64: v.i = reinterpret_cast<gmem<int64_t const>*>(d)[row]; v.d = double(v.i); break;
+    case ST_U8: v.i = d[row]; v.d = double(v.i); break;
+    case ST_BOOL: v.i = d[row] != 0; v.d = double(v.i); break;
+    case ST_U16: v.i = reinterpret_cast<gmem<uint16_t const>*>(d)[row]; v.d = double(v.i); break;
+    case ST_U32: v.i = reinterpret_cast<gmem<uint32_t const>*>(d)[row]; v.d = double(v.i); break;
     case ST_U64: {
-      uint64_t u = reinterpret_cast<uint64_t const*>(c.data)[row];
+      uint64_t u = reinterpret_cast<gmem<uint64_t const>*>(d)[row];
       v.i = int64_t(u);
       v.d = double(u);
       break;
     }
     case ST_F32: {
-      uint32_t u = reinterpret_cast<uint32_t const*>(c.data)[row];
+      uint32_t u = reinterpret_cast<gmem<uint32_t const>*>(d)[row];
       v.b = u;
       v.f = __uint_as_float(u);
       v.d = double(v.f);
       break;
     }
     case ST_F64: {
-      uint64_t u = reinterpret_cast<uint64_t const*>(c.data)[row];
+      uint64_t u = reinterpret_cast<gmem<uint64_t const>*>(d)[row];
       v.b = u;
       v.d = __longlong_as_double(int64_t(u));
       break;
@@ -214,11 +225,11 @@ __device__ __forceinline__ void acc_apply(int32_t kind, void* a, loaded_value co
 {
   switch (kind) {
     case FK_SUM_I64: aadd(static_cast<uint64_t*>(a), uint64_t(v.i)); break;
-    case FK_SUM_F32: unsafeAtomicAdd(static_cast<float*>(a), v.f); break;
-    case FK_SUM_F64: unsafeAtomicAdd(static_cast<double*>(a), v.d); break;
+    case FK_SUM_F32: gadd_f32(static_cast<float*>(a), v.f); break;
+    case FK_SUM_F64: gadd_f64(static_cast<double*>(a), v.d); break;
     case FK_SUMSQ_I64: aadd(static_cast<uint64_t*>(a), uint64_t(v.i) * uint64_t(v.i)); break;
-    case FK_SUMSQ_F32: unsafeAtomicAdd(static_cast<float*>(a), v.f * v.f); break;
-    case FK_SUMSQ_F64: unsafeAtomicAdd(static_cast<double*>(a), v.d * v.d); break;
+    case FK_SUMSQ_F32: gadd_f32(static_cast<float*>(a), v.f * v.f); break;
+    case FK_SUMSQ_F64: gadd_f64(static_cast<double*>(a), v.d * v.d); break;
     case FK_MIN_S32: amin(static_cast<int32_t*>(a), int32_t(v.i)); break;
     case FK_MAX_S32: amax(static_cast<int32_t*>(a), int32_t(v.i)); break;
     case FK_MIN_S64: amin(static_cast<int64_t*>(a), v.i); break;
@@ -267,9 +278,9 @@ __device__ __forceinline__ void acc_combine(int32_t kind, void* a, uint64_t part
     case FK_SUM_I64:
     case FK_SUMSQ_I64: aadd(static_cast<uint64_t*>(a), part); break;
     case FK_SUM_F32:
-    case FK_SUMSQ_F32: unsafeAtomicAdd(static_cast<float*>(a), __uint_as_float(uint32_t(part))); break;
+    case FK_SUMSQ_F32: gadd_f32(static_cast<float*>(a), __uint_as_float(uint32_t(part))); break;
     case FK_SUM_F64:
-    case FK_SUMSQ_F64: unsafeAtomicAdd(static_cast<double*>(a), __longlong_as_double(int64_t(part))); break;
+    case FK_SUMSQ_F64: gadd_f64(static_cast<double*>(a), __longlong_as_double(int64_t(part))); break;
     case FK_MIN_S32: amin(static_cast<int32_t*>(a), int32_t(part)); break;
     case FK_MAX_S32: amax(static_cast<int32_t*>(a), int32_t(part)); break;
     case FK_MIN_S64: amin(static_cast<int64_t*>(a), int64_t(part)); break;
@@ -353,11 +364,12 @@ __device__ __forceinline__ int lds_insert(uint64_t* tkeys, uint32_t* fill, uint6
 
 __device__ __forceinline__ void store_width(void* data, int64_t g, int w, uint64_t bits)
 {
+  auto const d = gp(static_cast<uint8_t*>(data));
   switch (w) {
-    case 1: reinterpret_cast<uint8_t*>(data)[g] = uint8_t(bits); break;
-    case 2: reinterpret_cast<uint16_t*>(data)[g] = uint16_t(bits); break;
-    case 4: reinterpret_cast<uint32_t*>(data)[g] = uint32_t(bits); break;
-    default: reinterpret_cast<uint64_t*>(data)[g] = bits; break;
+    case 1: d[g] = uint8_t(bits); break;
+    case 2: reinterpret_cast<gmem<uint16_t>*>(d)[g] = uint16_t(bits); break;
+    case 4: reinterpret_cast<gmem<uint32_t>*>(d)[g] = uint32_t(bits); break;
+    default: reinterpret_cast<gmem<uint64_t>*>(d)[g] = bits; break;
   }
 }
 

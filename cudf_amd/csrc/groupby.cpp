@@ -796,6 +796,13 @@ std::unique_ptr<mig_groupby_result> groupby_aggregate(mig_column_view const* key
     if (vals[v].has_nulls) n_flagged++;
   }
   rpa_ok = rpa_ok && n_flagged <= RPA_MAX_FLAGGED;
+  // the aggregate kernel keeps its accumulator plan in registers
+  {
+    int ncols = 0;
+    for (int v = 0; v < P.num_vals; v++)
+      if (P.val_field_begin[v] < P.val_field_begin[v + 1]) ncols++;
+    rpa_ok = rpa_ok && ncols <= RPA_PLAN_COLS && nf <= RPA_PLAN_FIELDS;
+  }
   // LDS layout of the aggregating workgroup
   rpa_params R{};
   int32_t acc_bytes = 0;
@@ -837,6 +844,17 @@ std::unique_ptr<mig_groupby_result> groupby_aggregate(mig_column_view const* key
     R.lds_bytes = off;
     R.lds_slots = S_slots;
     R.max_fill  = max_fill;
+    R.plan_ncols = 0;
+    R.plan_fb[0] = 0;
+    for (int v = 0; v < P.num_vals; v++) {
+      if (P.val_field_begin[v] == P.val_field_begin[v + 1]) continue;
+      int const i   = R.plan_ncols++;
+      R.plan_col[i] = v;
+      R.plan_fb[i + 1] = P.val_field_begin[v + 1];
+      if (i == 0) R.plan_fb[0] = P.val_field_begin[v];
+    }
+    for (int f = 0; f < nf; f++)
+      R.plan_fd[f] = uint32_t(P.fields[f].kind) | (uint32_t(P.fields[f].lds_off) << 8);
     // scatter staging layout: a tile of keys, copied values, flags and bins
     int32_t rec = 8 + 1 + 2;
     for (int v = 0; v < P.num_vals; v++)

@@ -301,7 +301,7 @@ __global__ void __launch_bounds__(BLOCK) k_second_pass(gb_params const* __restri
         double const cnt  = double(count_of(P, F.aux_count, slot));
         double const mean = field_as_double(P.fields[F.aux_sum], slot) / cnt;
         double const d    = v.d - mean;
-        unsafeAtomicAdd(reinterpret_cast<double*>(field_at(F, slot)), d * d);
+        gadd_f64(reinterpret_cast<double*>(field_at(F, slot)), d * d);
       }
     }
   }
@@ -461,7 +461,7 @@ __global__ void __launch_bounds__(LBLOCK) k_second_pass_lds(gb_params const* __r
         double const mean = field_as_double(P.fields[F.aux_sum], slot) / cnt;
         double const d    = v.d - mean;
         if (id >= 0) __hip_atomic_fetch_add(lds_at<double>(lds, P.lds_m2_off[f], id), d * d, __ATOMIC_RELAXED, WG);
-        else unsafeAtomicAdd(reinterpret_cast<double*>(field_at(F, slot)), d * d);
+        else gadd_f64(reinterpret_cast<double*>(field_at(F, slot)), d * d);
       }
     }
   }
@@ -472,7 +472,7 @@ __global__ void __launch_bounds__(LBLOCK) k_second_pass_lds(gb_params const* __r
     for (int f = 0; f < P.num_fields; f++) {
       if (P.fields[f].kind != FK_M2) continue;
       double const part = *lds_at<double>(lds, P.lds_m2_off[f], i);
-      if (part != 0.0) unsafeAtomicAdd(reinterpret_cast<double*>(field_at(P.fields[f], int64_t(slot))), part);
+      if (part != 0.0) gadd_f64(reinterpret_cast<double*>(field_at(P.fields[f], int64_t(slot))), part);
     }
   }
 }

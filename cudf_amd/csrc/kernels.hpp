@@ -143,6 +143,8 @@ constexpr int RPA_SMALL_BLOCK = 256;    // small helper kernels
 constexpr int RPA_SCATTER_BLOCK = 512;  // scatter kernels
 constexpr int RPA_HIST_BLOCK = 1024;    // histogram kernel
 constexpr int RPA_AGG_BLOCK = 512;
+constexpr int RPA_PLAN_COLS   = 4;   // value columns with accumulators (partitioned path)
+constexpr int RPA_PLAN_FIELDS = 16;  // accumulator fields (partitioned path)
 
 struct rpa_params {
   int32_t log_bins;   // final bins B = 1 << log_bins
@@ -183,6 +185,14 @@ struct rpa_params {
   int32_t* overflow;                 // a bin had more groups than max_ids
   int32_t pad1;
   uint64_t out_capacity;             // rows allocated in every output column
+  // Pass-A plan of the aggregate kernel, read into registers once per
+  // workgroup: the accumulators of value column plan_col[i] are the fields
+  // [plan_fb[i], plan_fb[i + 1]), field f described by plan_fd[f] =
+  // kind | lds_off << 8.
+  int32_t plan_ncols;
+  int32_t plan_col[RPA_PLAN_COLS];
+  int32_t plan_fb[RPA_PLAN_COLS + 1];
+  uint32_t plan_fd[RPA_PLAN_FIELDS];
 };
 
 // ---- launchers (groupby_kernels.hip) --------------------------------------

@@ -23,7 +23,8 @@ namespace mig {
 
 constexpr int WG_SCOPE = __HIP_MEMORY_SCOPE_WORKGROUP;
 
-constexpr int RPA_CACHED = 6;  // rows per thread per chunk; their slots stay in registers
+constexpr int RPA_CACHED = 6;     // rows per thread per chunk; their slots stay in registers
+constexpr int RPA_OUT_BATCH = 4;  // output columns whose LDS reads are issued together
 
 // Apply `f(k)` to the rows k of a chunk whose bit is set in `ok` (the loop is
 // unrolled; the caller's switch on the accumulator kind stays outside it).
@@ -40,17 +41,17 @@ __device__ __forceinline__ void for_rows(uint32_t ok, F&& f)
 // unchanged), `slot[k]` its group slot, bit k of `ok` whether it counts.
 __device__ __forceinline__ double canon_double(int32_t st, uint64_t c);
 
-__device__ __forceinline__ void acc_rows_lds(field_desc const& F, int32_t st, uint8_t* lds,
+__device__ __forceinline__ void acc_rows_lds(int32_t kind, int32_t lds_off, int32_t st, uint8_t* lds,
                                              int const (&slot)[RPA_CACHED], uint32_t ok,
                                              uint64_t const (&c)[RPA_CACHED])
 {
-  uint8_t* const base = lds + F.lds_off;
+  uint8_t* const base = lds + lds_off;
   auto u64 = [&](int k) { return reinterpret_cast<uint64_t*>(base) + slot[k]; };
   auto u32 = [&](int k) { return reinterpret_cast<uint32_t*>(base) + slot[k]; };
   auto fl  = [&](int k) { return __uint_as_float(uint32_t(c[k])); };
   auto db  = [&](int k) { return __longlong_as_double(int64_t(c[k])); };
   constexpr int R = __ATOMIC_RELAXED;
-  switch (F.kind) {
+  switch (kind) {
     case FK_SUM_I64: for_rows(ok, [&](int k) { __hip_atomic_fetch_add(u64(k), c[k], R, WG_SCOPE); }); break;
     case FK_SUM_F32:
       for_rows(ok, [&](int k) { __hip_atomic_fetch_add(reinterpret_cast<float*>(u32(k)), fl(k), R, WG_SCOPE); });
@@ -142,6 +143,18 @@ __device__ __forceinline__ double lds_field_double(field_desc const& F, uint8_t*
   }
 }
 
+// A sum or count partial, given as its raw bits, as double.
+__device__ __forceinline__ double field_bits_double(int32_t kind, uint64_t bits)
+{
+  switch (kind) {
+    case FK_SUM_I64: return double(int64_t(bits));
+    case FK_SUM_F32: return double(__uint_as_float(uint32_t(bits)));
+    case FK_SUM_F64: return __longlong_as_double(int64_t(bits));
+    case FK_COUNT: return double(uint32_t(bits));
+    default: return 0.0;
+  }
+}
+
 __device__ __forceinline__ uint32_t lds_count(gb_params const& P, rpa_params const& R, uint8_t* lds,
                                               int32_t f, int id)
 {
@@ -162,8 +175,8 @@ __device__ __forceinline__ void write_valid_bits(uint32_t* mask, uint64_t p0, ui
     int64_t const shift = int64_t(w * 32) - int64_t(p0);
     uint32_t bits       = shift >= 0 ? uint32_t(bal >> shift) : uint32_t(bal << (-shift));
     bool const whole    = (w * 32 >= p0) && (w * 32 + 32 <= last);
-    if (whole) mask[w] = bits;
-    else if (bits) __hip_atomic_fetch_or(mask + w, bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (whole) gp(mask)[w] = bits;
+    else if (bits) __hip_atomic_fetch_or(gp(mask + w), bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 
@@ -178,53 +191,27 @@ struct rpa_misc {
   uint32_t wave_sum[RPA_AGG_BLOCK / 64];
 };
 
-// Slot of a partitioned row: S = null key, S + 1 = key equal to EMPTY_KEY,
-// else its LDS table slot (inserted when INSERT); -1 when the table is full.
-template <bool INSERT>
-__device__ __forceinline__ int row_slot(gb_params const& P, rpa_params const& R, uint64_t* tkeys,
-                                        rpa_misc* misc, int S, uint8_t fl, uint64_t key)
+// Raw elements of a partition value column for the rows [r0 + k * block)
+// of a chunk, zero-extended to 64 bits (canon_rows gives the canonical form).
+// Rows past the bin read its last row; their lanes take no part.
+template <typename T>
+__device__ __forceinline__ void load_rows(gmem<T const>* p, int64_t r0, int64_t e, uint64_t (&c)[RPA_CACHED])
 {
-  if (fl & RPA_KEY_NULL) {
-    if (INSERT) misc->present0 = 1;
-    return S;
+#pragma unroll
+  for (int k = 0; k < RPA_CACHED; k++) {
+    int64_t const r = r0 + int64_t(k) * RPA_AGG_BLOCK;
+    c[k]            = uint64_t(p[r < e ? r : e - 1]);
   }
-  if (key == EMPTY_KEY) {
-    if (INSERT) misc->present1 = 1;
-    return S + 1;
-  }
-  uint32_t slot = inline_key_hash(P.key_storage, key) & uint32_t(S - 1);
-  for (int i = 0; i < S; i++) {
-    uint64_t const cur = __hip_atomic_load(tkeys + slot, __ATOMIC_RELAXED, WG_SCOPE);
-    if (cur == key) return int(slot);
-    if (cur == EMPTY_KEY) {
-      if (!INSERT) return -1;
-      if (__hip_atomic_load(&misc->fill, __ATOMIC_RELAXED, WG_SCOPE) >= uint32_t(R.max_fill)) {
-        __hip_atomic_store(&misc->overflow, 1, __ATOMIC_RELAXED, WG_SCOPE);
-        return -1;
-      }
-      uint64_t expected = EMPTY_KEY;
-      if (__hip_atomic_compare_exchange_strong(tkeys + slot, &expected, key, __ATOMIC_RELAXED,
-                                               __ATOMIC_RELAXED, WG_SCOPE)) {
-        __hip_atomic_fetch_add(&misc->fill, 1u, __ATOMIC_RELAXED, WG_SCOPE);
-        return int(slot);
-      }
-      if (expected == key) return int(slot);
-    }
-    slot = (slot + 1) & uint32_t(S - 1);
-  }
-  if (INSERT) __hip_atomic_store(&misc->overflow, 1, __ATOMIC_RELAXED, WG_SCOPE);
-  return -1;
 }
 
-// Raw 8 bytes of element r of a partition value column (narrow types are
-// zero-extended; canon_rows gives the canonical form).
-__device__ __forceinline__ uint64_t load_raw(uint8_t const* base, int w, int64_t r)
+__device__ __forceinline__ void load_chunk(gmem<uint8_t const>* src, int w, int64_t r0, int64_t e,
+                                           uint64_t (&c)[RPA_CACHED])
 {
   switch (w) {
-    case 1: return base[r];
-    case 2: return reinterpret_cast<uint16_t const*>(base)[r];
-    case 4: return reinterpret_cast<uint32_t const*>(base)[r];
-    default: return reinterpret_cast<uint64_t const*>(base)[r];
+    case 1: load_rows(src, r0, e, c); break;
+    case 2: load_rows(reinterpret_cast<gmem<uint16_t const>*>(src), r0, e, c); break;
+    case 4: load_rows(reinterpret_cast<gmem<uint32_t const>*>(src), r0, e, c); break;
+    default: load_rows(reinterpret_cast<gmem<uint64_t const>*>(src), r0, e, c); break;
   }
 }
 
@@ -262,66 +249,162 @@ __device__ __forceinline__ double canon_double(int32_t st, uint64_t c)
   return double(int64_t(c));
 }
 
-// Slots of the rows [r0 + tid + k * block) of one chunk: all keys and flags
-// are loaded first (RPA_CACHED loads in flight per thread), then probed.
+// What pass A needs of the request, held in registers for the whole bin.
+// Every load from the kernel parameters is a scalar memory read, and waiting
+// for one also waits for all LDS operations in flight (one counter covers
+// both), so reading descriptors inside the row loop would drain the LDS
+// atomics of every field before the next one starts.  The per-column and
+// per-field descriptors sit one per lane of a VGPR and are fetched with
+// v_readlane (a uniform index into a register array would go to memory).
+struct acc_plan {
+  int32_t ncols;
+  int32_t key_storage;
+  int32_t count_all_off;
+  uint32_t cd_lane;     // lane i < ncols: storage | (flag bit + 1) << 8 | first field << 16 | end << 24
+  uint32_t cp_lo_lane;  // lane i < ncols: partitioned values of column i (low, high word)
+  uint32_t cp_hi_lane;
+  uint32_t fd_lane;     // lane f: kind | lds_off << 8 of field f
+  gmem<uint64_t const>* keys;
+  gmem<uint8_t const>* flags;
+};
+
+__device__ __forceinline__ acc_plan load_plan(gb_params const& P, rpa_params const& R)
+{
+  acc_plan A;
+  A.ncols         = R.plan_ncols;
+  A.key_storage   = P.key_storage;
+  A.count_all_off = R.count_all_off;
+  A.keys          = gp(static_cast<uint64_t const*>(R.keysB));
+  A.flags         = gp(static_cast<uint8_t const*>(R.flagsB));
+  int const lane  = int(threadIdx.x & 63);
+  int const i     = lane < A.ncols ? lane : 0;
+  int const v     = R.plan_col[i];
+  A.cd_lane       = uint32_t(P.vals[v].storage) | (uint32_t(R.val_flag_bit[v] + 1) << 8) |
+              (uint32_t(R.plan_fb[i]) << 16) | (uint32_t(R.plan_fb[i + 1]) << 24);
+  uint64_t const cp = reinterpret_cast<uint64_t>(R.valsB[v]);
+  A.cp_lo_lane      = uint32_t(cp);
+  A.cp_hi_lane      = uint32_t(cp >> 32);
+  A.fd_lane         = R.plan_fd[lane < RPA_PLAN_FIELDS ? lane : 0];
+  return A;
+}
+
+__device__ __forceinline__ uint32_t lane_value(uint32_t v, int i)
+{
+  return uint32_t(__builtin_amdgcn_readlane(int(v), i));
+}
+
+// Slots of the rows [r0 + tid + k * block) of one chunk: S = null key,
+// S + 1 = key equal to EMPTY_KEY, else the row's LDS table slot (inserted
+// when INSERT); -1 when the row is past the bin or its key is not found.
+// All keys and flags are loaded first (RPA_CACHED loads in flight), then the
+// rows are probed one after the other.  The table is open-addressed with
+// double hashing: the first slot takes the low bits of the row hash and the
+// step (odd, so every slot is reached) the next ones, which keeps probe
+// sequences short at the table's ~60% fill where linear probing clusters.
+// Inserted keys are counted once per chunk; a bin whose count passes
+// max_fill is retried with more bins.
 template <bool INSERT>
-__device__ __forceinline__ void chunk_slots(gb_params const& P, rpa_params const& R, uint64_t* tkeys,
-                                            rpa_misc* misc, int S, int64_t r0, int64_t e,
-                                            int (&slot)[RPA_CACHED], uint64_t& flp)
+__device__ __forceinline__ void chunk_slots(acc_plan const& A, uint64_t* tkeys, rpa_misc* misc, int S,
+                                            int64_t r0, int64_t e, int (&slot)[RPA_CACHED], uint64_t& flp)
 {
   uint64_t key[RPA_CACHED];
   uint8_t fl[RPA_CACHED];
 #pragma unroll
-  for (int k = 0; k < RPA_CACHED; k++) {
-    int64_t const r = r0 + int64_t(k) * RPA_AGG_BLOCK;
-    fl[k]  = r < e ? R.flagsB[r] : 0;
-    key[k] = r < e ? R.keysB[r] : 0;
+  for (int k = 0; k < RPA_CACHED; k++) {  // rows past the bin read its last row (e > r0 - tid)
+    int64_t const r  = r0 + int64_t(k) * RPA_AGG_BLOCK;
+    int64_t const rc = r < e ? r : e - 1;
+    fl[k]            = A.flags[rc];
+    key[k]           = A.keys[rc];
   }
+  uint32_t pending = 0;
+  uint32_t pos[RPA_CACHED], step[RPA_CACHED];
 #pragma unroll
   for (int k = 0; k < RPA_CACHED; k++) {
     int64_t const r = r0 + int64_t(k) * RPA_AGG_BLOCK;
     slot[k]         = -1;
-    if (r < e && !__hip_atomic_load(&misc->overflow, __ATOMIC_RELAXED, WG_SCOPE))
-      slot[k] = row_slot<INSERT>(P, R, tkeys, misc, S, fl[k], key[k]);
+    pos[k]          = 0;
+    step[k]         = 1;
+    if (r >= e) continue;
+    if (fl[k] & RPA_KEY_NULL) {
+      if (INSERT) misc->present0 = 1;
+      slot[k] = S;
+    } else if (key[k] == EMPTY_KEY) {
+      if (INSERT) misc->present1 = 1;
+      slot[k] = S + 1;
+    } else {
+      uint32_t const h = inline_key_hash(A.key_storage, key[k]);
+      pos[k]           = h & uint32_t(S - 1);
+      step[k]          = ((h >> 9) | 1u) & uint32_t(S - 1);
+      pending |= 1u << k;
+    }
   }
+  uint32_t inserted = 0;
+#pragma unroll
+  for (int k = 0; k < RPA_CACHED; k++) {
+    if (!((pending >> k) & 1u)) continue;
+    uint32_t p = pos[k];
+    int i      = 0;
+    for (; i < S; i++) {
+      uint64_t const cur = __hip_atomic_load(tkeys + p, __ATOMIC_RELAXED, WG_SCOPE);
+      if (cur == key[k]) break;
+      if (cur == EMPTY_KEY) {
+        if (!INSERT) {
+          i = S;
+          break;
+        }
+        uint64_t expected = EMPTY_KEY;
+        if (__hip_atomic_compare_exchange_strong(tkeys + p, &expected, key[k], __ATOMIC_RELAXED,
+                                                 __ATOMIC_RELAXED, WG_SCOPE)) {
+          inserted++;
+          break;
+        }
+        if (expected == key[k]) break;
+      }
+      p = (p + step[k]) & uint32_t(S - 1);
+    }
+    if (i < S) slot[k] = int(p);
+    else if (INSERT) __hip_atomic_store(&misc->overflow, 1, __ATOMIC_RELAXED, WG_SCOPE);  // table full
+  }
+  if (INSERT && inserted) __hip_atomic_fetch_add(&misc->fill, inserted, __ATOMIC_RELAXED, WG_SCOPE);
   flp = 0;
 #pragma unroll
   for (int k = 0; k < RPA_CACHED; k++) flp |= uint64_t(fl[k]) << (8 * k);
 }
 
-// Pass A of one chunk: every non-variance accumulator, column by column,
-// with the chunk's values of a column loaded together and each accumulator
-// applied to all rows of the chunk at once.
-__device__ __forceinline__ void chunk_accumulate(gb_params const& P, rpa_params const& R, uint8_t* lds,
-                                                 int64_t r0, int64_t e, int const (&slot)[RPA_CACHED],
-                                                 uint64_t flp)
+// Pass A of one chunk: every accumulator, column by column, with the
+// chunk's values of a column loaded together and each accumulator applied to
+// all rows of the chunk at once.
+__device__ __forceinline__ void chunk_accumulate(acc_plan const& A, uint8_t* lds, int64_t r0, int64_t e,
+                                                 int const (&slot)[RPA_CACHED], uint64_t flp)
 {
   uint32_t live = 0;
 #pragma unroll
   for (int k = 0; k < RPA_CACHED; k++) live |= uint32_t(slot[k] >= 0) << k;
-  if (R.count_all_off >= 0)
+  if (A.count_all_off >= 0)
     for_rows(live, [&](int k) {
-      __hip_atomic_fetch_add(lds_at<uint32_t>(lds, R.count_all_off, slot[k]), 1u, __ATOMIC_RELAXED, WG_SCOPE);
+      __hip_atomic_fetch_add(lds_at<uint32_t>(lds, A.count_all_off, slot[k]), 1u, __ATOMIC_RELAXED, WG_SCOPE);
     });
-  for (int v = 0; v < P.num_vals; v++) {
-    int const fb = P.val_field_begin[v], fe = P.val_field_begin[v + 1];
-    if (fb == fe) continue;
-    int const bit = R.val_flag_bit[v];
-    int const st  = P.vals[v].storage;
-    int const w   = storage_width(st);
+  for (int i = 0; i < A.ncols; i++) {
+    uint32_t const cd = lane_value(A.cd_lane, i);
+    auto const src    = reinterpret_cast<gmem<uint8_t const>*>(
+      (uint64_t(lane_value(A.cp_hi_lane, i)) << 32) | lane_value(A.cp_lo_lane, i));
+    int const st             = int(cd & 0xffu);
+    int const bit            = int((cd >> 8) & 0xffu) - 1;
+    int const fb             = int((cd >> 16) & 0xffu);
+    int const fe             = int(cd >> 24);
+    int const w              = storage_width(st);
     uint64_t c[RPA_CACHED];
-#pragma unroll
-    for (int k = 0; k < RPA_CACHED; k++) {
-      int64_t const r = r0 + int64_t(k) * RPA_AGG_BLOCK;
-      c[k]            = (r < e) ? load_raw(R.valsB[v], w, r) : 0;
-    }
+    load_chunk(src, w, r0, e, c);
     canon_rows(st, c);
     uint32_t ok = live;
     if (bit >= 0) {
 #pragma unroll
       for (int k = 0; k < RPA_CACHED; k++) ok &= ~(uint32_t(((flp >> (8 * k + bit)) & 1) == 0) << k);
     }
-    for (int f = fb; f < fe; f++) acc_rows_lds(P.fields[f], st, lds, slot, ok, c);
+    for (int f = fb; f < fe; f++) {
+      uint32_t const d = lane_value(A.fd_lane, f);
+      acc_rows_lds(int32_t(d & 0xffu), int32_t(d >> 8), st, lds, slot, ok, c);
+    }
   }
 }
 
@@ -349,11 +432,7 @@ __device__ __forceinline__ void chunk_deviations(gb_params const& P, rpa_params 
     int const w         = storage_width(st);
     field_desc const& F = P.fields[m2];
     uint64_t c[RPA_CACHED];
-#pragma unroll
-    for (int k = 0; k < RPA_CACHED; k++) {
-      int64_t const r = r0 + int64_t(k) * RPA_AGG_BLOCK;
-      c[k]            = (r < e) ? load_raw(R.valsB[v], w, r) : 0;
-    }
+    load_chunk(gp(static_cast<uint8_t const*>(R.valsB[v])), w, r0, e, c);
     canon_rows(st, c);
     uint32_t ok = live;
     if (bit >= 0) {
@@ -417,6 +496,7 @@ __global__ void __launch_bounds__(RPA_AGG_BLOCK) __attribute__((amdgpu_waves_per
   auto* slot_of_j   = reinterpret_cast<uint16_t*>(lds + R.slot_of_j_off);
   auto* misc        = reinterpret_cast<rpa_misc*>(lds + R.misc_off);
   int const tid     = threadIdx.x;
+  acc_plan const A  = load_plan(P, R);
 
   // ---- init
   for (int i = tid; i < S; i += RPA_AGG_BLOCK) tkeys[i] = EMPTY_KEY;
@@ -441,14 +521,16 @@ __global__ void __launch_bounds__(RPA_AGG_BLOCK) __attribute__((amdgpu_waves_per
   // ---- pass A: find-or-insert and accumulate, chunk by chunk
   constexpr int64_t CHUNK = int64_t(RPA_CACHED) * RPA_AGG_BLOCK;
   for (int64_t c = s; c < e; c += CHUNK) {
-    if (__hip_atomic_load(&misc->overflow, __ATOMIC_RELAXED, WG_SCOPE)) break;
+    if (__hip_atomic_load(&misc->overflow, __ATOMIC_RELAXED, WG_SCOPE) ||
+        __hip_atomic_load(&misc->fill, __ATOMIC_RELAXED, WG_SCOPE) > uint32_t(R.max_fill))
+      break;
     int slot[RPA_CACHED];
     uint64_t fl;
-    chunk_slots<true>(P, R, tkeys, misc, S, c + tid, e, slot, fl);
-    chunk_accumulate(P, R, lds, c + tid, e, slot, fl);
+    chunk_slots<true>(A, tkeys, misc, S, c + tid, e, slot, fl);
+    chunk_accumulate(A, lds, c + tid, e, slot, fl);
   }
   __syncthreads();
-  if (misc->overflow) {
+  if (misc->overflow || misc->fill > uint32_t(R.max_fill)) {
     if (tid == 0) astore(R.overflow, 1);
     return;
   }
@@ -483,7 +565,7 @@ __global__ void __launch_bounds__(RPA_AGG_BLOCK) __attribute__((amdgpu_waves_per
     for (int64_t c = s; c < e; c += CHUNK) {
       int slot[RPA_CACHED];
       uint64_t fl;
-      chunk_slots<false>(P, R, tkeys, misc, S, c + tid, e, slot, fl);
+      chunk_slots<false>(A, tkeys, misc, S, c + tid, e, slot, fl);
       chunk_deviations(P, R, lds, c + tid, e, slot, fl);
     }
     __syncthreads();
@@ -522,7 +604,7 @@ __global__ void __launch_bounds__(RPA_AGG_BLOCK) __attribute__((amdgpu_waves_per
     if (occupied(k)) slot_of_j[j++] = uint16_t(k < 2 ? S + k : k - 2);
   if (tid == 0) {
     misc->g_out = total;
-    misc->base  = __hip_atomic_fetch_add(R.group_counter, (unsigned long long)total, __ATOMIC_RELAXED,
+    misc->base  = __hip_atomic_fetch_add(gp(R.group_counter), (unsigned long long)total, __ATOMIC_RELAXED,
                                          __HIP_MEMORY_SCOPE_AGENT);
   }
   __syncthreads();
@@ -545,57 +627,64 @@ __global__ void __launch_bounds__(RPA_AGG_BLOCK) __attribute__((amdgpu_waves_per
       store_width(P.out_keys[0], int64_t(pos), kw, key);
     }
     if (P.out_key_mask[0]) write_valid_bits(P.out_key_mask[0], base + j0, end, in_range && kvalid);
-    for (int o = 0; o < P.num_outs; o++) {
-      out_desc const& O = P.outs[o];
-      bool valid        = true;
-      if (in_range) {
-        uint32_t cnt = 0;
-        if (O.count_field != -1) {
-          cnt   = lds_count(P, R, lds, O.count_field, slot);
-          valid = cnt > 0;
-        }
-        switch (O.kind) {
-          case OK_COPY_I64:
-          case OK_COPY_F64:
-            reinterpret_cast<uint64_t*>(O.data)[pos] = lds_field_bits(P.fields[O.field], lds, slot);
-            break;
-          case OK_COPY_F32:
-            reinterpret_cast<uint32_t*>(O.data)[pos] = uint32_t(lds_field_bits(P.fields[O.field], lds, slot));
-            break;
-          case OK_COUNT:
-            reinterpret_cast<int32_t*>(O.data)[pos] =
-              int32_t(*lds_at<uint32_t>(lds, P.fields[O.field].lds_off, slot));
-            valid = true;
-            break;
-          case OK_COUNT_ALL:
-            reinterpret_cast<int32_t*>(O.data)[pos] = int32_t(*lds_at<uint32_t>(lds, R.count_all_off, slot));
-            valid = true;
-            break;
-          case OK_MINMAX: {
-            field_desc const& F = P.fields[O.field];
-            uint64_t bits       = lds_field_bits(F, lds, slot);
-            if (F.kind == FK_MIN_F32 || F.kind == FK_MAX_F32) bits = unord32(uint32_t(bits));
-            else if (F.kind == FK_MIN_F64 || F.kind == FK_MAX_F64) bits = unord64(bits);
-            store_width(O.data, int64_t(pos), O.out_width, bits);
-            break;
-          }
-          case OK_MEAN: {
-            double const sum = lds_field_double(P.fields[O.field], lds, slot);
-            reinterpret_cast<double*>(O.data)[pos] = valid ? sum / double(cnt) : 0.0;
-            break;
-          }
-          case OK_VAR:
-          case OK_STD: {
-            int64_t const dn = int64_t(cnt) - O.ddof;
-            valid            = cnt > 0 && dn > 0;
-            double var = valid ? *lds_at<double>(lds, P.fields[O.field].lds_off, slot) / double(dn) : 0.0;
-            reinterpret_cast<double*>(O.data)[pos] = (O.kind == OK_STD) ? sqrt(var) : var;
-            break;
-          }
-          default: break;
-        }
+    // outputs in batches: the LDS reads of a batch are issued together, then
+    // the batch's values are converted and stored
+    for (int o0 = 0; o0 < P.num_outs; o0 += RPA_OUT_BATCH) {
+      uint64_t raw[RPA_OUT_BATCH];
+      uint32_t cnt[RPA_OUT_BATCH];
+#pragma unroll
+      for (int u = 0; u < RPA_OUT_BATCH; u++) {
+        raw[u] = 0;
+        cnt[u] = 0;
+        int const o = o0 + u;
+        if (o >= P.num_outs || !in_range) continue;
+        out_desc const& O = P.outs[o];
+        if (O.kind == OK_COUNT_ALL) raw[u] = *lds_at<uint32_t>(lds, R.count_all_off, slot);
+        else raw[u] = lds_field_bits(P.fields[O.field], lds, slot);
+        if (O.count_field != -1) cnt[u] = lds_count(P, R, lds, O.count_field, slot);
       }
-      if (O.mask) write_valid_bits(O.mask, base + j0, end, in_range && valid);
+#pragma unroll
+      for (int u = 0; u < RPA_OUT_BATCH; u++) {
+        int const o = o0 + u;
+        if (o >= P.num_outs) break;
+        out_desc const& O = P.outs[o];
+        bool valid        = O.count_field == -1 || cnt[u] > 0;
+        if (in_range) {
+          switch (O.kind) {
+            case OK_COPY_I64:
+            case OK_COPY_F64: gp(reinterpret_cast<uint64_t*>(O.data))[pos] = raw[u]; break;
+            case OK_COPY_F32: gp(reinterpret_cast<uint32_t*>(O.data))[pos] = uint32_t(raw[u]); break;
+            case OK_COUNT:
+            case OK_COUNT_ALL:
+              gp(reinterpret_cast<int32_t*>(O.data))[pos] = int32_t(uint32_t(raw[u]));
+              valid = true;
+              break;
+            case OK_MINMAX: {
+              int const fk  = P.fields[O.field].kind;
+              uint64_t bits = raw[u];
+              if (fk == FK_MIN_F32 || fk == FK_MAX_F32) bits = unord32(uint32_t(bits));
+              else if (fk == FK_MIN_F64 || fk == FK_MAX_F64) bits = unord64(bits);
+              store_width(O.data, int64_t(pos), O.out_width, bits);
+              break;
+            }
+            case OK_MEAN: {
+              double const sum = field_bits_double(P.fields[O.field].kind, raw[u]);
+              gp(reinterpret_cast<double*>(O.data))[pos] = valid ? sum / double(cnt[u]) : 0.0;
+              break;
+            }
+            case OK_VAR:
+            case OK_STD: {
+              int64_t const dn = int64_t(cnt[u]) - O.ddof;
+              valid            = cnt[u] > 0 && dn > 0;
+              double var       = valid ? __longlong_as_double(int64_t(raw[u])) / double(dn) : 0.0;
+              gp(reinterpret_cast<double*>(O.data))[pos] = (O.kind == OK_STD) ? sqrt(var) : var;
+              break;
+            }
+            default: break;
+          }
+        }
+        if (O.mask) write_valid_bits(O.mask, base + j0, end, in_range && valid);
+      }
     }
   }
 }

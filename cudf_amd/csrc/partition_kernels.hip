@@ -53,7 +53,7 @@ __global__ void __launch_bounds__(RPA_HIST_BLOCK)
         __hip_atomic_fetch_add(&hist[bin[u]], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
   }
   __syncthreads();
-  uint32_t* out = R.hist_partial + int64_t(blockIdx.x) * B;
+  auto out = gp(R.hist_partial + int64_t(blockIdx.x) * B);
   for (int i = threadIdx.x; i < B; i += RPA_HIST_BLOCK) out[i] = hist[i];
 }
 
@@ -71,7 +71,8 @@ __global__ void __launch_bounds__(RPA_SMALL_BLOCK) k_rpa_hist_reduce(rpa_params 
   int const k0  = int(blockIdx.y) * per;
   int const k1  = (k0 + per < R.hist_blocks) ? k0 + per : R.hist_blocks;
   uint32_t s    = 0;
-  for (int k = k0; k < k1; k++) s += R.hist_partial[int64_t(k) * B + b];
+  auto const hp = gp(static_cast<uint32_t const*>(R.hist_partial));
+  for (int k = k0; k < k1; k++) s += hp[int64_t(k) * B + b];
   if (s) aadd(R.hist + b, s);
 }
 
@@ -90,10 +91,10 @@ void launch_rpa_hist(gb_params const* dP, rpa_params const* dR, rpa_params const
 __device__ __forceinline__ void copy_width(uint8_t* dst, int64_t di, uint8_t const* src, int64_t si, int w)
 {
   switch (w) {
-    case 1: dst[di] = src[si]; break;
-    case 2: reinterpret_cast<uint16_t*>(dst)[di] = reinterpret_cast<uint16_t const*>(src)[si]; break;
-    case 4: reinterpret_cast<uint32_t*>(dst)[di] = reinterpret_cast<uint32_t const*>(src)[si]; break;
-    default: reinterpret_cast<uint64_t*>(dst)[di] = reinterpret_cast<uint64_t const*>(src)[si]; break;
+    case 1: gp(dst)[di] = src[si]; break;
+    case 2: gp(reinterpret_cast<uint16_t*>(dst))[di] = reinterpret_cast<uint16_t const*>(src)[si]; break;
+    case 4: gp(reinterpret_cast<uint32_t*>(dst))[di] = reinterpret_cast<uint32_t const*>(src)[si]; break;
+    default: gp(reinterpret_cast<uint64_t*>(dst))[di] = reinterpret_cast<uint64_t const*>(src)[si]; break;
   }
 }
 
@@ -156,7 +157,8 @@ __global__ void __launch_bounds__(RPA_SCATTER_BLOCK)
   uint8_t* const oflags = (SECOND || !R.two_level) ? R.flagsB : R.flagsA;
   int64_t ntiles;
   if constexpr (SECOND) {
-    for (int i = threadIdx.x; i <= F1; i += RPA_SCATTER_BLOCK) tstart[i] = R.tile_start1[i];
+    for (int i = threadIdx.x; i <= F1; i += RPA_SCATTER_BLOCK)
+      tstart[i] = gp(static_cast<int64_t const*>(R.tile_start1))[i];
     __syncthreads();
     ntiles = R.num_tiles1;
   } else {
@@ -174,8 +176,8 @@ __global__ void __launch_bounds__(RPA_SCATTER_BLOCK)
         else hi = mid - 1;
       }
       b1                 = lo;
-      int64_t const bs   = R.bin_start[int64_t(b1) << lf2];
-      int64_t const be   = R.bin_start[int64_t(b1 + 1) << lf2];
+      int64_t const bs   = gp(static_cast<int64_t const*>(R.bin_start))[int64_t(b1) << lf2];
+      int64_t const be   = gp(static_cast<int64_t const*>(R.bin_start))[int64_t(b1 + 1) << lf2];
       r0                 = bs + (t - tstart[b1]) * T;
       r1                 = (r0 + T < be) ? r0 + T : be;
     } else {
@@ -197,8 +199,8 @@ __global__ void __launch_bounds__(RPA_SCATTER_BLOCK)
       if (row >= r1) continue;
       uint32_t h;
       if constexpr (SECOND) {
-        kf[k] = R.flagsA[row];
-        kb[k] = R.keysA[row];
+        kf[k] = gp(static_cast<uint8_t const*>(R.flagsA))[row];
+        kb[k] = gp(static_cast<uint64_t const*>(R.keysA))[row];
         h     = (kf[k] & RPA_KEY_NULL) ? null_key_hash() : inline_key_hash(P.key_storage, kb[k]);
         bin[k] = int(top_bits(h, R.log_bins) & uint32_t(fan - 1));
       } else {
@@ -214,7 +216,7 @@ __global__ void __launch_bounds__(RPA_SCATTER_BLOCK)
     for (int i = threadIdx.x; i < fan; i += RPA_SCATTER_BLOCK) {
       if (cnt[i]) {
         unsigned long long* cur = SECOND ? R.cursor2 + ((int64_t(b1) << lf2) + i) : R.cursor1 + i;
-        base[i] = __hip_atomic_fetch_add(cur, (unsigned long long)cnt[i], __ATOMIC_RELAXED,
+        base[i] = __hip_atomic_fetch_add(gp(cur), (unsigned long long)cnt[i], __ATOMIC_RELAXED,
                                          __HIP_MEMORY_SCOPE_AGENT);
       }
     }
@@ -232,7 +234,7 @@ __global__ void __launch_bounds__(RPA_SCATTER_BLOCK)
     for (int v = 0; v < P.num_vals; v++) {
       if (P.val_field_begin[v] == P.val_field_begin[v + 1]) continue;
       int const w              = storage_width(P.vals[v].storage);
-      uint8_t const* const src = SECOND ? R.valsA[v] : P.vals[v].data;
+      auto const src = gp(static_cast<uint8_t const*>(SECOND ? R.valsA[v] : P.vals[v].data));
       uint8_t* const dst       = stage + R.stage_val_off[v];
       uint64_t raw[RPA_MAX_RPT];
 #pragma unroll
@@ -242,9 +244,9 @@ __global__ void __launch_bounds__(RPA_SCATTER_BLOCK)
         if (lp[k] >= 0) {
           switch (w) {
             case 1: raw[k] = src[row]; break;
-            case 2: raw[k] = reinterpret_cast<uint16_t const*>(src)[row]; break;
-            case 4: raw[k] = reinterpret_cast<uint32_t const*>(src)[row]; break;
-            default: raw[k] = reinterpret_cast<uint64_t const*>(src)[row]; break;
+            case 2: raw[k] = reinterpret_cast<gmem<uint16_t const>*>(src)[row]; break;
+            case 4: raw[k] = reinterpret_cast<gmem<uint32_t const>*>(src)[row]; break;
+            default: raw[k] = reinterpret_cast<gmem<uint64_t const>*>(src)[row]; break;
           }
         }
       }
@@ -273,8 +275,8 @@ __global__ void __launch_bounds__(RPA_SCATTER_BLOCK)
     for (int lp = threadIdx.x; lp < total; lp += RPA_SCATTER_BLOCK) {
       int const b       = sbins[lp];
       int64_t const dst = int64_t(base[b]) + (lp - int(toff[b]));
-      okeys[dst]        = skeys[lp];
-      oflags[dst]       = sflags[lp];
+      gp(okeys)[dst]    = skeys[lp];
+      gp(oflags)[dst]   = sflags[lp];
       for (int v = 0; v < P.num_vals; v++) {
         if (P.val_field_begin[v] == P.val_field_begin[v + 1]) continue;
         uint8_t* const ov = (SECOND || !R.two_level) ? R.valsB[v] : R.valsA[v];
