@@ -43,7 +43,7 @@ constexpr int32_t RPA_LDS_SLOTS      = 4096;
 constexpr int32_t RPA_LDS_BUDGET     = 76 * 1024;
 constexpr int RPA_MAX_LOG_BINS       = 15;
 constexpr int RPA_MAX_LOG_FANOUT     = 8;
-constexpr int32_t RPA_STAGE_BUDGET   = 48 * 1024;  // LDS staging of a scatter tile
+constexpr int32_t RPA_STAGE_BUDGET   = 56 * 1024;  // LDS staging of a scatter tile (3 workgroups per CU)
 // Global path, low cardinality: LDS partial tables of up to 2048 slots (3/4
 // fillable), used when the estimate is at most LDS_PREAGG_MAX_GROUPS groups.
 constexpr int32_t LDS_PREAGG_SLOTS      = 2048;
@@ -859,8 +859,9 @@ std::unique_ptr<mig_groupby_result> groupby_aggregate(mig_column_view const* key
     int32_t rec = 8 + 1 + 2;
     for (int v = 0; v < P.num_vals; v++)
       if (P.val_field_begin[v] < P.val_field_begin[v + 1]) rec += storage_width(vals[v].storage);
-    R.tile_rows = RPA_TILE;
-    while (R.tile_rows > RPA_SCATTER_BLOCK && R.tile_rows * (rec + 4) > RPA_STAGE_BUDGET) R.tile_rows /= 2;
+    // the largest multiple of the block whose staging fits the budget
+    R.tile_rows = std::max(1, std::min(RPA_TILE / RPA_SCATTER_BLOCK, RPA_STAGE_BUDGET / ((rec + 4) * RPA_SCATTER_BLOCK))) *
+                  RPA_SCATTER_BLOCK;
     {
       int32_t so = R.tile_rows * 8;
       for (int v = 0; v < P.num_vals; v++) {

@@ -129,13 +129,64 @@ __device__ __forceinline__ void scan_counts(uint32_t const* cnt, uint32_t* off, 
   if (lane == 63) off[fan] = incl;
 }
 
+constexpr int RPA_PREFETCH_COLS = 2;  // value columns loaded together with the keys
+
+// Value column v of the tile's rows (raw, zero-extended) and, in the first
+// pass, their validity; rows with bin < 0 are skipped.
+template <bool SECOND>
+__device__ __forceinline__ void load_vals(gb_params const& P, rpa_params const& R, int v, int64_t r0,
+                                          int const (&bin)[RPA_MAX_RPT], uint64_t (&raw)[RPA_MAX_RPT],
+                                          bool (&vv)[RPA_MAX_RPT])
+{
+  int const w    = storage_width(P.vals[v].storage);
+  auto const src = gp(static_cast<uint8_t const*>(SECOND ? R.valsA[v] : P.vals[v].data));
+  int const fb   = SECOND ? -1 : R.val_flag_bit[v];
+#pragma unroll
+  for (int k = 0; k < RPA_MAX_RPT; k++) {
+    int64_t const row = r0 + int64_t(k) * RPA_SCATTER_BLOCK + threadIdx.x;
+    raw[k]            = 0;
+    vv[k]             = false;
+    if (bin[k] < 0) continue;
+    switch (w) {
+      case 1: raw[k] = src[row]; break;
+      case 2: raw[k] = reinterpret_cast<gmem<uint16_t const>*>(src)[row]; break;
+      case 4: raw[k] = reinterpret_cast<gmem<uint32_t const>*>(src)[row]; break;
+      default: raw[k] = reinterpret_cast<gmem<uint64_t const>*>(src)[row]; break;
+    }
+    if (fb >= 0) vv[k] = dcol_valid(P.vals[v], row);
+  }
+}
+
+// Stage value column v of the tile at the rows' bin-order positions `lp`,
+// and fold its validity into the rows' flags.
+template <bool SECOND>
+__device__ __forceinline__ void stage_vals(gb_params const& P, rpa_params const& R, uint8_t* stage, int v,
+                                           int const (&lp)[RPA_MAX_RPT], uint64_t const (&raw)[RPA_MAX_RPT],
+                                           bool const (&vv)[RPA_MAX_RPT], uint8_t (&kf)[RPA_MAX_RPT])
+{
+  int const w        = storage_width(P.vals[v].storage);
+  uint8_t* const dst = stage + R.stage_val_off[v];
+  int const fb       = SECOND ? -1 : R.val_flag_bit[v];
+#pragma unroll
+  for (int k = 0; k < RPA_MAX_RPT; k++) {
+    if (lp[k] < 0) continue;
+    switch (w) {
+      case 1: dst[lp[k]] = uint8_t(raw[k]); break;
+      case 2: reinterpret_cast<uint16_t*>(dst)[lp[k]] = uint16_t(raw[k]); break;
+      case 4: reinterpret_cast<uint32_t*>(dst)[lp[k]] = uint32_t(raw[k]); break;
+      default: reinterpret_cast<uint64_t*>(dst)[lp[k]] = raw[k]; break;
+    }
+    if (!SECOND && vv[k]) kf[k] |= uint8_t(1u << (fb & 7));
+  }
+}
+
 // One tile of the scatter: rows are ranked per bin with LDS atomics, staged
 // in LDS in bin order, and written out so that consecutive threads store
 // consecutive rows of a bin (coalesced runs of tile_rows / fanout rows).
 //   SECOND = false: input columns -> first-level bins
 //   SECOND = true : first-level bin in buffer A -> final bins in buffer B
 template <bool SECOND>
-__global__ void __launch_bounds__(RPA_SCATTER_BLOCK)
+__global__ void __launch_bounds__(RPA_SCATTER_BLOCK) __attribute__((amdgpu_waves_per_eu(6)))
   k_rpa_scatter(gb_params const* __restrict__ Pp, rpa_params const* __restrict__ Rp, int64_t n)
 {
   extern __shared__ __attribute__((aligned(16))) uint8_t stage[];
@@ -191,6 +242,7 @@ __global__ void __launch_bounds__(RPA_SCATTER_BLOCK)
     uint32_t rank[RPA_MAX_RPT];
     uint64_t kb[RPA_MAX_RPT];
     uint8_t kf[RPA_MAX_RPT];
+    // all loads of the tile's keys first (validity and key together), then the ranks
 #pragma unroll
     for (int k = 0; k < RPA_MAX_RPT; k++) {
       bin[k] = -1;
@@ -204,13 +256,28 @@ __global__ void __launch_bounds__(RPA_SCATTER_BLOCK)
         h     = (kf[k] & RPA_KEY_NULL) ? null_key_hash() : inline_key_hash(P.key_storage, kb[k]);
         bin[k] = int(top_bits(h, R.log_bins) & uint32_t(fan - 1));
       } else {
-        bool kn;
-        if (!input_key(P, row, kb[k], kn, h)) continue;
-        kf[k]  = kn ? RPA_KEY_NULL : 0;
+        dcol const& kc       = P.keys[0];
+        bool const kvalid    = dcol_valid(kc, row);
+        uint64_t const bits  = load_fixed_bits(kc, row);
+        if (!kvalid && P.skip_null_keys) continue;
+        kb[k]  = kvalid ? bits : 0;
+        kf[k]  = kvalid ? 0 : RPA_KEY_NULL;
+        h      = kvalid ? inline_key_hash(kc.storage, bits) : null_key_hash();
         bin[k] = int(top_bits(h, R.log_f1));
       }
-      rank[k] = __hip_atomic_fetch_add(&cnt[bin[k]], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     }
+    // the first value columns' loads go out with the keys' (their latency
+    // overlaps the ranking, the scan and the range reservation)
+    int const ncols = R.plan_ncols;
+    uint64_t praw[RPA_PREFETCH_COLS][RPA_MAX_RPT];
+    bool pvv[RPA_PREFETCH_COLS][RPA_MAX_RPT];
+#pragma unroll
+    for (int i = 0; i < RPA_PREFETCH_COLS; i++)
+      if (i < ncols) load_vals<SECOND>(P, R, R.plan_col[i], r0, bin, praw[i], pvv[i]);
+#pragma unroll
+    for (int k = 0; k < RPA_MAX_RPT; k++)
+      if (bin[k] >= 0)
+        rank[k] = __hip_atomic_fetch_add(&cnt[bin[k]], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     __syncthreads();
     scan_counts(cnt, toff, fan);
     for (int i = threadIdx.x; i < fan; i += RPA_SCATTER_BLOCK) {
@@ -230,41 +297,16 @@ __global__ void __launch_bounds__(RPA_SCATTER_BLOCK)
       skeys[lp[k]] = kb[k];
       sbins[lp[k]] = uint16_t(bin[k]);
     }
-    // values column by column, the RPA_MAX_RPT loads of a column in flight together
-    for (int v = 0; v < P.num_vals; v++) {
-      if (P.val_field_begin[v] == P.val_field_begin[v + 1]) continue;
-      int const w              = storage_width(P.vals[v].storage);
-      auto const src = gp(static_cast<uint8_t const*>(SECOND ? R.valsA[v] : P.vals[v].data));
-      uint8_t* const dst       = stage + R.stage_val_off[v];
+    // values: the prefetched columns, then any others column by column
+#pragma unroll
+    for (int i = 0; i < RPA_PREFETCH_COLS; i++)
+      if (i < ncols) stage_vals<SECOND>(P, R, stage, R.plan_col[i], lp, praw[i], pvv[i], kf);
+    for (int i = RPA_PREFETCH_COLS; i < ncols; i++) {
+      int const v = R.plan_col[i];
       uint64_t raw[RPA_MAX_RPT];
-#pragma unroll
-      for (int k = 0; k < RPA_MAX_RPT; k++) {
-        int64_t const row = r0 + int64_t(k) * RPA_SCATTER_BLOCK + threadIdx.x;
-        raw[k]            = 0;
-        if (lp[k] >= 0) {
-          switch (w) {
-            case 1: raw[k] = src[row]; break;
-            case 2: raw[k] = reinterpret_cast<gmem<uint16_t const>*>(src)[row]; break;
-            case 4: raw[k] = reinterpret_cast<gmem<uint32_t const>*>(src)[row]; break;
-            default: raw[k] = reinterpret_cast<gmem<uint64_t const>*>(src)[row]; break;
-          }
-        }
-      }
-#pragma unroll
-      for (int k = 0; k < RPA_MAX_RPT; k++) {
-        if (lp[k] < 0) continue;
-        switch (w) {
-          case 1: dst[lp[k]] = uint8_t(raw[k]); break;
-          case 2: reinterpret_cast<uint16_t*>(dst)[lp[k]] = uint16_t(raw[k]); break;
-          case 4: reinterpret_cast<uint32_t*>(dst)[lp[k]] = uint32_t(raw[k]); break;
-          default: reinterpret_cast<uint64_t*>(dst)[lp[k]] = raw[k]; break;
-        }
-        if constexpr (!SECOND) {
-          int const fb = R.val_flag_bit[v];
-          int64_t const row = r0 + int64_t(k) * RPA_SCATTER_BLOCK + threadIdx.x;
-          if (fb >= 0 && dcol_valid(P.vals[v], row)) kf[k] |= uint8_t(1u << fb);
-        }
-      }
+      bool vv[RPA_MAX_RPT];
+      load_vals<SECOND>(P, R, v, r0, bin, raw, vv);
+      stage_vals<SECOND>(P, R, stage, v, lp, raw, vv, kf);
     }
 #pragma unroll
     for (int k = 0; k < RPA_MAX_RPT; k++)
@@ -277,8 +319,8 @@ __global__ void __launch_bounds__(RPA_SCATTER_BLOCK)
       int64_t const dst = int64_t(base[b]) + (lp - int(toff[b]));
       gp(okeys)[dst]    = skeys[lp];
       gp(oflags)[dst]   = sflags[lp];
-      for (int v = 0; v < P.num_vals; v++) {
-        if (P.val_field_begin[v] == P.val_field_begin[v + 1]) continue;
+      for (int i = 0; i < ncols; i++) {
+        int const v       = R.plan_col[i];
         uint8_t* const ov = (SECOND || !R.two_level) ? R.valsB[v] : R.valsA[v];
         copy_width(ov, dst, stage + R.stage_val_off[v], lp, storage_width(P.vals[v].storage));
       }
@@ -292,14 +334,11 @@ static void allow_large_lds(void const* fn, int bytes) { allow_dynamic_lds(fn, b
 
 // Workgroups of a scatter kernel that are resident at once: the tiles are
 // strided over the grid, so a grid larger than what fits would leave its last
-// workgroups to run after the others (LDS: 160 KB per CU; 4 x 512 threads).
+// workgroups to run after the others.
 static int64_t resident_grid(void const* fn, int dyn_lds, int num_cus)
 {
-  hipFuncAttributes fa{};
-  MIG_HIP_TRY(hipFuncGetAttributes(&fa, fn));
-  int64_t const per_wg = int64_t(dyn_lds) + int64_t(fa.sharedSizeBytes);
-  int64_t per_cu       = per_wg > 0 ? (160 * 1024) / per_wg : 4;
-  if (per_cu > 4) per_cu = 4;
+  int per_cu = 0;  // registers and LDS both limit it
+  MIG_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, RPA_SCATTER_BLOCK, size_t(dyn_lds)));
   if (per_cu < 1) per_cu = 1;
   return int64_t(num_cus) * per_cu;
 }
