@@ -446,7 +446,7 @@ owned_column make_owned_fixed(device_allocator& A, int32_t type_id, int64_t n, b
   oc.col.data       = n > 0 ? A.allocate(oc.data_bytes) : nullptr;
   oc.col.data_bytes = int64_t(oc.data_bytes);
   if (with_mask) {
-    oc.mask_bytes    = std::max<size_t>(4, size_t((n + 31) / 32) * 4);
+    oc.mask_bytes    = std::max<size_t>(8, size_t((n + 63) / 64) * 8);  // whole 64-bit words
     oc.col.null_mask = static_cast<uint32_t*>(A.allocate(oc.mask_bytes));
     if (n == 0) MIG_HIP_TRY(hipMemsetAsync(oc.col.null_mask, 0, oc.mask_bytes, A.stream));
   }

@@ -162,22 +162,18 @@ __device__ __forceinline__ uint32_t lds_count(gb_params const& P, rpa_params con
                  : *lds_at<uint32_t>(lds, P.fields[f].lds_off, id);
 }
 
-// Validity bits of the 64 output rows [p0, p0 + 64) of one wave, clipped to
-// the workgroup's rows [.., end): words wholly inside are stored, words shared
-// with a neighbouring workgroup are OR-ed in (the mask starts zeroed).
-__device__ __forceinline__ void write_valid_bits(uint32_t* mask, uint64_t p0, uint64_t end, bool valid)
+// Validity bits of the 64 output rows [c0, c0 + 64) (c0 a multiple of 64)
+// of a wave whose workgroup owns the rows [base, end): one 64-bit word,
+// stored when the workgroup owns all of it, else OR-ed in.  Words at or past
+// `cap` (the rows allocated) are left alone.
+__device__ __forceinline__ void store_valid_word(uint32_t* mask, uint64_t c0, uint64_t base, uint64_t end,
+                                                 uint64_t cap, bool valid)
 {
   uint64_t const bal = __ballot(valid);
-  if ((threadIdx.x & 63) != 0) return;
-  uint64_t const last = (p0 + 64 < end) ? p0 + 64 : end;
-  if (last <= p0) return;
-  for (uint64_t w = p0 >> 5; w <= (last - 1) >> 5; w++) {
-    int64_t const shift = int64_t(w * 32) - int64_t(p0);
-    uint32_t bits       = shift >= 0 ? uint32_t(bal >> shift) : uint32_t(bal << (-shift));
-    bool const whole    = (w * 32 >= p0) && (w * 32 + 32 <= last);
-    if (whole) gp(mask)[w] = bits;
-    else if (bits) __hip_atomic_fetch_or(gp(mask + w), bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
+  if ((threadIdx.x & 63) != 0 || c0 >= cap) return;
+  auto const w = gp(reinterpret_cast<uint64_t*>(mask)) + (c0 >> 6);
+  if (c0 >= base && c0 + 64 <= end) *w = bal;
+  else if (bal) __hip_atomic_fetch_or(w, bal, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 struct rpa_misc {
@@ -609,24 +605,28 @@ __global__ void __launch_bounds__(RPA_AGG_BLOCK) __attribute__((amdgpu_waves_per
   }
   __syncthreads();
 
-  // ---- outputs: output row base + j for j in [0, g_out)
-  uint32_t const g_out           = misc->g_out;
-  unsigned long long const base  = misc->base;
-  uint64_t const end             = base + g_out;
-  int const kw                   = storage_width(P.key_storage);
-  for (uint32_t j0 = uint32_t(tid) & ~63u; j0 < g_out; j0 += RPA_AGG_BLOCK) {
-    uint32_t const jj   = j0 + lane;
-    bool const live     = jj < g_out;
-    uint64_t const pos  = base + jj;
-    int const slot      = live ? int(slot_of_j[jj]) : 0;
-    bool const in_range = live && pos < R.out_capacity;
+  // ---- outputs: group j of the workgroup goes to output row base + j.  The
+  // waves take 64-row chunks aligned to 64 output rows, so a data store
+  // covers whole lines and the validity bits of a chunk are one 64-bit word
+  // (stored, or OR-ed in where a neighbouring workgroup shares it; the masks
+  // start zeroed).
+  uint32_t const g_out          = misc->g_out;
+  unsigned long long const base = misc->base;
+  uint64_t const end            = base + g_out;
+  uint64_t const cap            = R.out_capacity;
+  int const kw                  = storage_width(P.key_storage);
+  for (uint64_t c0 = (base & ~63ull) + 64 * uint64_t(wave); c0 < end; c0 += RPA_AGG_BLOCK) {
+    uint64_t const pos  = c0 + lane;
+    bool const live     = pos >= base && pos < end;
+    int const slot      = live ? int(slot_of_j[pos - base]) : 0;
+    bool const in_range = live && pos < cap;
     bool const kvalid   = slot != S;
     if (live && !in_range) astore(R.overflow, 1);
     if (in_range) {
       uint64_t const key = slot < S ? tkeys[slot] : (slot == S ? 0ull : EMPTY_KEY);
       store_width(P.out_keys[0], int64_t(pos), kw, key);
     }
-    if (P.out_key_mask[0]) write_valid_bits(P.out_key_mask[0], base + j0, end, in_range && kvalid);
+    if (P.out_key_mask[0]) store_valid_word(P.out_key_mask[0], c0, base, end, cap, in_range && kvalid);
     // outputs in batches: the LDS reads of a batch are issued together, then
     // the batch's values are converted and stored
     for (int o0 = 0; o0 < P.num_outs; o0 += RPA_OUT_BATCH) {
@@ -683,7 +683,7 @@ __global__ void __launch_bounds__(RPA_AGG_BLOCK) __attribute__((amdgpu_waves_per
             default: break;
           }
         }
-        if (O.mask) write_valid_bits(O.mask, base + j0, end, in_range && valid);
+        if (O.mask) store_valid_word(O.mask, c0, base, end, cap, in_range && valid);
       }
     }
   }
