@@ -100,6 +100,7 @@ class BatchOptions(ctypes.Structure):
         ("block_size", ctypes.c_int32),
         ("tile_rows", ctypes.c_int32),
         ("use_graph", ctypes.c_int32),
+        ("partition_min_rows", ctypes.c_int32),
     ]
 
 

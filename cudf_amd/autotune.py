@@ -27,6 +27,7 @@ GRID = {
     "lds_max_groups": [512, -1, 128, 768],
     "block_size": [256, 512],
     "tile_rows": [4096, 2048, 8192, 16384],
+    "partition_min_rows": [0, -1],
 }
 
 _cache: Dict[tuple, BatchOptions] = {}

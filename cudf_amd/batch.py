@@ -30,10 +30,12 @@ class BatchOptions:
     block_size: int = 0         # 256 or 512 (default 256)
     tile_rows: int = 0          # rows per work item (default 4096)
     use_graph: int = 0          # -1 = launch kernels one by one instead of replaying the graph
+    partition_min_rows: int = 0  # large high-cardinality instances take the single-call
+                                 # partitioned path (default 1 << 20 rows, -1 never)
 
     def to_c(self) -> _lib.BatchOptions:
         return _lib.BatchOptions(float(self.load_factor), int(self.lds_max_groups), int(self.block_size),
-                                 int(self.tile_rows), int(self.use_graph))
+                                 int(self.tile_rows), int(self.use_graph), int(self.partition_min_rows))
 
 
 def _stream_handle(stream: Optional[torch.cuda.Stream]):

@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "batch_host.hpp"
+#include "groupby.hpp"
 #include "host_plan.hpp"
 
 struct mig_batch_plan {
@@ -37,6 +38,10 @@ struct mig_batch_plan {
   std::vector<mig_column_view> val_views;          // [n_inst * num_vals]
   std::vector<uint8_t> val_nulls;                  // [n_inst * num_vals]
   std::vector<int32_t> agg_begin;                  // [num_requests + 1]
+  std::vector<int32_t> vcol_of;                    // [num_requests] value column of each request
+  std::vector<mig_aggregation> aggs;               // the requests' aggregations, flattened
+  std::vector<uint8_t> solo;                       // [n_inst] runs the single-call path
+  std::vector<std::unique_ptr<mig_groupby_result>> solo_res;
   std::vector<int32_t> out_of_agg;                 // flattened (request, agg) -> output
   std::vector<out_info> outs;
   std::vector<int32_t> kinds;                      // [n_inst * num_fields]
@@ -94,6 +99,12 @@ namespace mig {
 namespace {
 
 constexpr int32_t BATCH_SAMPLE = 4096;  // sampled rows per instance
+// Instances this large with this many estimated groups run the single-call
+// path: its radix-partitioned aggregation (LDS atomics per bin) beats the
+// per-row device-wide atomics the batched kernels fall back to when an
+// instance's groups exceed what LDS pre-aggregation takes.
+constexpr int32_t BATCH_SOLO_ROWS   = 1 << 20;
+constexpr double BATCH_SOLO_GROUPS  = 65536.0;
 
 int32_t default_i(int32_t v, int32_t d) { return v == 0 ? d : v; }
 
@@ -344,6 +355,8 @@ mig_batch_plan* batch_plan_create(mig_batch_instance const* instances, int32_t n
     }
   }
   p.num_vals = int32_t(req_of_vcol.size());
+  p.vcol_of  = vcol_of;
+  p.aggs.assign(aggregations, aggregations + p.agg_begin[size_t(num_requests)]);
   p.val_views.resize(size_t(num_instances) * size_t(p.num_vals));
   for (int32_t v = 0; v < p.num_vals; v++)
     for (int32_t i = 0; i < num_instances; i++) {
@@ -565,6 +578,14 @@ mig_batch_plan* batch_plan_create(mig_batch_instance const* instances, int32_t n
 
   // ---- estimates, tables, layout --------------------------------------------------
   estimate(p, stream);
+  int32_t const solo_rows = p.opt.partition_min_rows == 0 ? BATCH_SOLO_ROWS : p.opt.partition_min_rows;
+  p.solo.assign(size_t(num_instances), 0);
+  p.solo_res.resize(size_t(num_instances));
+  for (int32_t i = 0; i < num_instances; i++)
+    if (p.opt.partition_min_rows >= 0 && p.hinst[i].rows >= solo_rows && p.est[i] >= BATCH_SOLO_GROUPS) {
+      p.solo[i]       = 1;
+      p.hinst[i].rows = 0;  // no rows in the batched kernels
+    }
   p.cap.resize(size_t(num_instances));
   for (int32_t i = 0; i < num_instances; i++) {
     p.cap[i]       = table_slots(p.est[i], p.hinst[i].rows, p.opt.load_factor);
@@ -585,15 +606,43 @@ mig_batch_plan* batch_plan_create(mig_batch_instance const* instances, int32_t n
   return plan.release();
 }
 
+namespace {
+
+void launch_batched(mig_batch_plan& p, hipStream_t stream)
+{
+  if (p.exec) MIG_HIP_TRY(hipGraphLaunch(p.exec, stream));
+  else enqueue(p, stream);
+}
+
+// The single-call instances, one after the other on `stream` (each call
+// returns once its group count is known).
+void run_solo(mig_batch_plan& p, hipStream_t stream)
+{
+  std::vector<mig_aggregation_request> reqs(size_t(p.num_requests));
+  for (int32_t i = 0; i < p.n_inst; i++) {
+    if (!p.solo[size_t(i)]) continue;
+    p.solo_res[size_t(i)].reset();
+    for (int32_t r = 0; r < p.num_requests; r++) {
+      reqs[r].values           = p.val_views[size_t(i) * size_t(p.num_vals) + size_t(p.vcol_of[r])];
+      reqs[r].aggregations     = p.aggs.data() + p.agg_begin[r];
+      reqs[r].num_aggregations = p.agg_begin[r + 1] - p.agg_begin[r];
+    }
+    p.solo_res[size_t(i)] =
+      groupby_aggregate(&p.key_views[size_t(i)], 1, p.null_policy, reqs.data(), p.num_requests, stream, nullptr);
+  }
+}
+
+}  // namespace
+
 void batch_execute(mig_batch_plan* plan, hipStream_t stream)
 {
   MIG_EXPECTS(plan != nullptr, "null batch plan");
   mig_batch_plan& p = *plan;
-  if (p.exec) MIG_HIP_TRY(hipGraphLaunch(p.exec, stream));
-  else enqueue(p, stream);
+  launch_batched(p, stream);
   p.executed    = true;
   p.synced      = false;
   p.last_stream = stream;
+  run_solo(p, stream);
 }
 
 void batch_synchronize(mig_batch_plan* plan, hipStream_t stream)
@@ -622,7 +671,7 @@ void batch_synchronize(mig_batch_plan* plan, hipStream_t stream)
     MIG_EXPECTS(attempt < 16, "batched group-by: tables kept overflowing");
     p.retries++;
     layout(p, stream);
-    batch_execute(plan, stream);
+    launch_batched(p, stream);
   }
   p.groups.assign(p.pinned, p.pinned + n);
   p.valid.assign(p.pinned + 2 * n, p.pinned + 2 * n + nv);
@@ -632,6 +681,7 @@ void batch_synchronize(mig_batch_plan* plan, hipStream_t stream)
 int32_t batch_num_groups(mig_batch_plan const* plan, int32_t instance)
 {
   if (!plan || !plan->synced || instance < 0 || instance >= plan->n_inst) return -1;
+  if (plan->solo[size_t(instance)]) return plan->solo_res[size_t(instance)]->num_groups;
   return int32_t(plan->groups[size_t(instance)]);
 }
 
@@ -641,6 +691,7 @@ mig_column batch_result_key(mig_batch_plan const* plan, int32_t instance)
   mig_batch_plan const& p = *plan;
   MIG_EXPECTS(p.synced, "results read before mig_batch_synchronize");
   MIG_EXPECTS(instance >= 0 && instance < p.n_inst, "instance index out of range");
+  if (p.solo[size_t(instance)]) return p.solo_res[size_t(instance)]->keys[0].col;
   batch_inst const& I = p.hinst[size_t(instance)];
   uint32_t const G    = p.groups[size_t(instance)];
   int32_t const tid   = p.key_views[size_t(instance)].type_id;
@@ -668,6 +719,7 @@ mig_column batch_result_column(mig_batch_plan const* plan, int32_t instance, int
   MIG_EXPECTS(request >= 0 && request < p.num_requests, "request index out of range");
   MIG_EXPECTS(agg >= 0 && agg < p.agg_begin[size_t(request) + 1] - p.agg_begin[size_t(request)],
               "aggregation index out of range");
+  if (p.solo[size_t(instance)]) return p.solo_res[size_t(instance)]->results[size_t(request)][size_t(agg)].col;
   int32_t const o     = p.out_of_agg[size_t(p.agg_begin[size_t(request)] + agg)];
   auto const& oi      = p.outs[size_t(o)];
   batch_inst const& I = p.hinst[size_t(instance)];

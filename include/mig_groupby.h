@@ -179,6 +179,10 @@ typedef struct mig_batch_options {
   int32_t tile_rows;         /* rows per work item (default 4096)               */
   int32_t use_graph;         /* -1 = launch the kernels one by one instead of
                                 replaying the captured hipGraph (default on)    */
+  int32_t partition_min_rows; /* instances with at least this many rows and an
+                                estimated 65536 or more groups run the single-
+                                call path (radix-partitioned aggregation) inside
+                                mig_batch_execute (default 1 << 20; -1 never)   */
 } mig_batch_options;
 
 typedef struct mig_batch_plan mig_batch_plan;

@@ -162,6 +162,23 @@ def test_batch_nullability_matches_single_call():
                 assert b.null_count == s.null_count == int((~b.validity()).sum())
 
 
+@pytest.mark.parametrize("null_policy", [0, 1])
+def test_batch_partitioned_instances(null_policy):
+    """A large high-cardinality instance runs the single-call partitioned
+    path inside the plan; every instance still matches the oracle, also on a
+    second execution, and with the path switched off."""
+    from cudf_amd import BatchOptions
+    from cudf_amd._lib import last_stats
+    inst = make_instances(23, 6, 5_000, big=[(1_200_000, 500_000)])
+    plan = run_batch(inst, [REQ0, REQ1], null_policy)
+    assert last_stats()["strategy"] == 2  # the big instance's call was partitioned
+    check_all(plan, inst, [REQ0, REQ1], null_policy)
+    plan.run()
+    check_all(plan, inst, [REQ0, REQ1], null_policy)
+    off = run_batch(inst, [REQ0, REQ1], null_policy, BatchOptions(partition_min_rows=-1))
+    check_all(off, inst, [REQ0, REQ1], null_policy)
+
+
 def test_batch_empty_instances():
     inst = [(host_col(np.zeros(0, np.int32)), [host_col(np.zeros(0, np.int64)),
                                                 host_col(np.zeros(0, np.float32))]) for _ in range(3)]
