@@ -960,9 +960,9 @@ std::unique_ptr<mig_groupby_result> groupby_aggregate(mig_column_view const* key
       T.mark(10);
       launch_rpa_scatter1(dP, dR, R, N, dp.num_cus, stream);
       T.mark(11);
-      if (R.two_level) launch_rpa_scatter2(dP, dR, R, dp.num_cus, stream);
+      if (R.two_level) launch_rpa_scatter2(dP, dR, R, dp.num_cus, 0, R.num_tiles1, stream);
       T.mark(12);
-      launch_rpa_aggregate(dP, dR, R, stream);
+      launch_rpa_aggregate(dP, dR, R, 0, B, stream);
       T.mark(13);
       MIG_HIP_TRY(hipGetLastError());
       auto* h = reinterpret_cast<uint64_t*>(pin + 2 * sizeof(gb_params));

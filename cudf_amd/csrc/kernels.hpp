@@ -242,9 +242,11 @@ void launch_rpa_hist(gb_params const* dP, rpa_params const* dR, rpa_params const
                      int num_cus, hipStream_t s);
 void launch_rpa_scatter1(gb_params const* dP, rpa_params const* dR, rpa_params const& hR, int64_t n,
                          int num_cus, hipStream_t s);
-void launch_rpa_scatter2(gb_params const* dP, rpa_params const* dR, rpa_params const& hR,
-                         int num_cus, hipStream_t s);
-void launch_rpa_aggregate(gb_params const* dP, rpa_params const* dR, rpa_params const& hR,
-                          hipStream_t s);
+// second pass over the tiles [tile_begin, tile_end) of the first-level bins
+void launch_rpa_scatter2(gb_params const* dP, rpa_params const* dR, rpa_params const& hR, int num_cus,
+                         int64_t tile_begin, int64_t tile_end, hipStream_t s);
+// final bins [bin_begin, bin_end)
+void launch_rpa_aggregate(gb_params const* dP, rpa_params const* dR, rpa_params const& hR, int64_t bin_begin,
+                          int64_t bin_end, hipStream_t s);
 
 }  // namespace mig

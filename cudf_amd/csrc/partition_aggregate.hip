@@ -477,12 +477,12 @@ __device__ __forceinline__ bool m2_from_sums(field_desc const& SUMF, int32_t st,
 }
 
 __global__ void __launch_bounds__(RPA_AGG_BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
-  k_rpa_aggregate(gb_params const* __restrict__ Pp, rpa_params const* __restrict__ Rp)
+  k_rpa_aggregate(gb_params const* __restrict__ Pp, rpa_params const* __restrict__ Rp, int64_t bin_begin)
 {
   extern __shared__ __attribute__((aligned(16))) uint8_t lds[];
   gb_params const& P = *Pp;
   rpa_params const& R = *Rp;
-  int const b         = blockIdx.x;
+  int64_t const b     = bin_begin + blockIdx.x;
   int64_t const s     = R.bin_start[b];
   int64_t const e     = R.bin_start[b + 1];
   if (s == e) return;
@@ -689,11 +689,13 @@ __global__ void __launch_bounds__(RPA_AGG_BLOCK) __attribute__((amdgpu_waves_per
   }
 }
 
-void launch_rpa_aggregate(gb_params const* dP, rpa_params const* dR, rpa_params const& hR, hipStream_t s)
+void launch_rpa_aggregate(gb_params const* dP, rpa_params const* dR, rpa_params const& hR, int64_t bin_begin,
+                          int64_t bin_end, hipStream_t s)
 {
-  int64_t const B = int64_t(1) << hR.log_bins;
+  if (bin_end <= bin_begin) return;
   allow_dynamic_lds(reinterpret_cast<void const*>(&k_rpa_aggregate), hR.lds_bytes);
-  hipLaunchKernelGGL(k_rpa_aggregate, dim3(B), dim3(RPA_AGG_BLOCK), hR.lds_bytes, s, dP, dR);
+  hipLaunchKernelGGL(k_rpa_aggregate, dim3(bin_end - bin_begin), dim3(RPA_AGG_BLOCK), hR.lds_bytes, s, dP, dR,
+                     bin_begin);
 }
 
 }  // namespace mig

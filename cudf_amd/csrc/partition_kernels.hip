@@ -187,7 +187,7 @@ __device__ __forceinline__ void stage_vals(gb_params const& P, rpa_params const&
 //   SECOND = true : first-level bin in buffer A -> final bins in buffer B
 template <bool SECOND>
 __global__ void __launch_bounds__(RPA_SCATTER_BLOCK) __attribute__((amdgpu_waves_per_eu(6)))
-  k_rpa_scatter(gb_params const* __restrict__ Pp, rpa_params const* __restrict__ Rp, int64_t n)
+  k_rpa_scatter(gb_params const* __restrict__ Pp, rpa_params const* __restrict__ Rp, int64_t n, int64_t t_begin)
 {
   extern __shared__ __attribute__((aligned(16))) uint8_t stage[];
   __shared__ uint32_t cnt[RPA_MAX_FANOUT];
@@ -211,11 +211,11 @@ __global__ void __launch_bounds__(RPA_SCATTER_BLOCK) __attribute__((amdgpu_waves
     for (int i = threadIdx.x; i <= F1; i += RPA_SCATTER_BLOCK)
       tstart[i] = gp(static_cast<int64_t const*>(R.tile_start1))[i];
     __syncthreads();
-    ntiles = R.num_tiles1;
+    ntiles = n;  // tiles [t_begin, n) of the first-level bins
   } else {
     ntiles = (n + T - 1) / T;
   }
-  for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+  for (int64_t t = t_begin + blockIdx.x; t < ntiles; t += gridDim.x) {
     // rows [r0, r1) of this tile; for the second pass `b1` is the first-level bin
     int64_t r0, r1;
     int b1 = 0;
@@ -352,18 +352,19 @@ void launch_rpa_scatter1(gb_params const* dP, rpa_params const* dR, rpa_params c
   if (blocks > tiles) blocks = tiles;
   if (blocks < 1) blocks = 1;
   allow_large_lds(fn, hR.stage_bytes);
-  hipLaunchKernelGGL(k_rpa_scatter<false>, dim3(blocks), dim3(RPA_SCATTER_BLOCK), hR.stage_bytes, s, dP, dR, n);
+  hipLaunchKernelGGL(k_rpa_scatter<false>, dim3(blocks), dim3(RPA_SCATTER_BLOCK), hR.stage_bytes, s, dP, dR, n,
+                     int64_t(0));
 }
 
-void launch_rpa_scatter2(gb_params const* dP, rpa_params const* dR, rpa_params const& hR,
-                         int num_cus, hipStream_t s)
+void launch_rpa_scatter2(gb_params const* dP, rpa_params const* dR, rpa_params const& hR, int num_cus,
+                         int64_t tile_begin, int64_t tile_end, hipStream_t s)
 {
+  if (tile_end <= tile_begin) return;
   int64_t blocks = resident_grid(reinterpret_cast<void const*>(&k_rpa_scatter<true>), hR.stage_bytes, num_cus);
-  if (blocks > hR.num_tiles1) blocks = hR.num_tiles1;
-  if (blocks < 1) blocks = 1;
+  if (blocks > tile_end - tile_begin) blocks = tile_end - tile_begin;
   allow_large_lds(reinterpret_cast<void const*>(&k_rpa_scatter<true>), hR.stage_bytes);
   hipLaunchKernelGGL(k_rpa_scatter<true>, dim3(blocks), dim3(RPA_SCATTER_BLOCK), hR.stage_bytes, s, dP, dR,
-                     int64_t(0));
+                     tile_end, tile_begin);
 }
 
 }  // namespace mig
