@@ -189,5 +189,7 @@ def last_stats() -> dict:
     return {STAT_NAMES[i] if i < len(STAT_NAMES) else f"stat{i}": buf[i] for i in range(n)}
 
 
-def set_phase_timing(on: bool) -> None:
-    lib().mig_set_phase_timing(1 if on else 0)
+def set_phase_timing(on) -> None:
+    """Phase timing of later calls: False/0 off, True/1 every phase, 2 the
+    main aggregation kernel only (two events per call)."""
+    lib().mig_set_phase_timing(int(on))

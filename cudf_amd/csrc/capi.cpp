@@ -172,7 +172,7 @@ int32_t mig_last_stats(double* values, int32_t n)
   return mig::STAT_COUNT;
 }
 
-void mig_set_phase_timing(int32_t enable) { mig::set_phase_timing(enable != 0); }
+void mig_set_phase_timing(int32_t enable) { mig::set_phase_timing(enable == 2 ? 2 : (enable != 0 ? 1 : 0)); }
 
 void mig_set_strategy(int32_t strategy)
 {

@@ -34,7 +34,7 @@ double* thread_stats()
 }
 
 namespace {
-std::atomic<bool> g_phase_timing{false};
+std::atomic<int> g_phase_timing{0};  // 0 off, 1 every phase, 2 the main aggregation kernel only
 std::atomic<int> g_strategy{MIG_STRATEGY_AUTO};
 
 // Partitioned path: LDS hash slots and byte budget of the aggregating
@@ -53,11 +53,12 @@ constexpr int32_t LDS_PREAGG_BUDGET     = 96 * 1024;
 // hipEvents marking phase boundaries on the call's stream.
 struct phase_timer {
   static constexpr int N = 16;
+  int level;
   bool on;
   hipStream_t stream;
   hipEvent_t ev[N];
   bool recorded[N] = {false};
-  explicit phase_timer(hipStream_t s) : on(g_phase_timing.load()), stream(s)
+  explicit phase_timer(hipStream_t s) : level(g_phase_timing.load()), on(level > 0), stream(s)
   {
     if (!on) return;
     static thread_local std::unordered_map<int, std::vector<hipEvent_t>> pool;
@@ -72,7 +73,8 @@ struct phase_timer {
   }
   void mark(int i)
   {
-    if (!on) return;
+    // level 2 brackets only the accumulate kernel of either strategy (marks 2, 3 and 12, 13)
+    if (!on || (level == 2 && i != 2 && i != 3 && i != 12 && i != 13)) return;
     MIG_HIP_TRY(hipEventRecord(ev[i], stream));
     recorded[i] = true;
   }
@@ -86,7 +88,7 @@ struct phase_timer {
 };
 }  // namespace
 
-void set_phase_timing(bool on) { g_phase_timing.store(on); }
+void set_phase_timing(int level) { g_phase_timing.store(level); }
 void set_strategy(int s) { g_strategy.store(s); }
 
 // ---------------------------------------------------------------------------
@@ -1139,11 +1141,9 @@ std::unique_ptr<mig_groupby_result> groupby_aggregate(mig_column_view const* key
   uint32_t* d_valid = masked.empty() ? nullptr : S.get<uint32_t>(masked.size());
   if (d_valid) {
     MIG_HIP_TRY(hipMemsetAsync(d_valid, 0, 4 * masked.size(), stream));
-    for (size_t m = 0; m < masked.size(); m++) {
-      dcol c{};
-      c.mask = masked[m]->null_mask;
-      launch_count_nulls(c, G, d_valid + m, stream);
-    }
+    std::vector<uint32_t const*> masks(masked.size());
+    for (size_t m = 0; m < masked.size(); m++) masks[m] = masked[m]->null_mask;
+    launch_count_valid_list(masks.data(), int(masks.size()), G, d_valid, stream);
     std::vector<uint32_t> hv(masked.size());
     MIG_HIP_TRY(hipMemcpyAsync(hv.data(), d_valid, 4 * masked.size(), hipMemcpyDeviceToHost, stream));
     MIG_HIP_TRY(hipStreamSynchronize(stream));

@@ -59,7 +59,7 @@ enum stat_index : int {
   STAT_COUNT
 };
 double* thread_stats();
-void set_phase_timing(bool on);
+void set_phase_timing(int level);
 void set_strategy(int s);
 
 std::unique_ptr<mig_groupby_result> groupby_aggregate(mig_column_view const* keys, int32_t num_keys,

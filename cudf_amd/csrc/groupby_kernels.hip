@@ -866,6 +866,41 @@ __global__ void __launch_bounds__(BLOCK) k_count_valid(dcol c, int64_t n, uint32
   if (threadIdx.x == 0 && cnt != 0) aadd(out, cnt);
 }
 
+// Set bits among the first n bits of each of up to MASK_LIST masks (word
+// aligned, as result masks are): blockIdx.y picks the mask.
+constexpr int MASK_LIST = 32;
+struct mask_list {
+  uint32_t const* mask[MASK_LIST];
+};
+
+__global__ void __launch_bounds__(BLOCK) k_count_valid_list(mask_list L, int64_t n, uint32_t* out)
+{
+  __shared__ uint32_t red[BLOCK / 64];
+  auto const mask    = gp(L.mask[blockIdx.y]);
+  int64_t const w1   = (n + 31) >> 5;
+  uint32_t cnt       = 0;
+  for (int64_t w = int64_t(blockIdx.x) * BLOCK + threadIdx.x; w < w1; w += int64_t(gridDim.x) * BLOCK) {
+    uint32_t bits = mask[w];
+    if (w == w1 - 1 && (n & 31) != 0) bits &= (1u << (n & 31)) - 1u;
+    cnt += __popc(bits);
+  }
+  cnt = block_sum_u32(cnt, red);
+  if (threadIdx.x == 0 && cnt != 0) aadd(out + blockIdx.y, cnt);
+}
+
+void launch_count_valid_list(uint32_t const* const* masks, int count, int64_t n, uint32_t* d_counts, hipStream_t s)
+{
+  if (n <= 0) return;
+  int64_t blocks = ((n + 31) / 32 + BLOCK - 1) / BLOCK;
+  if (blocks > 256) blocks = 256;
+  for (int i0 = 0; i0 < count; i0 += MASK_LIST) {
+    int const k = count - i0 < MASK_LIST ? count - i0 : MASK_LIST;
+    mask_list L{};
+    for (int i = 0; i < k; i++) L.mask[i] = masks[i0 + i];
+    hipLaunchKernelGGL(k_count_valid_list, dim3(blocks, k), dim3(BLOCK), 0, s, L, n, d_counts + i0);
+  }
+}
+
 void launch_count_nulls(dcol const& c, int64_t n, uint32_t* d_count, hipStream_t s)
 {
   if (n <= 0) return;

@@ -236,6 +236,9 @@ void launch_exclusive_scan_i32(int32_t* data, int64_t n, int32_t* total, hipStre
 void launch_gather_string_chars(dcol const& src, uint32_t const* rep_rows, int64_t G,
                                 int32_t const* out_offsets, uint8_t* out_chars, hipStream_t s);
 void launch_count_nulls(dcol const& c, int64_t n, uint32_t* d_count, hipStream_t s);
+// valid rows among the first n of each word-aligned mask, added into d_counts[i]
+void launch_count_valid_list(uint32_t const* const* masks, int count, int64_t n, uint32_t* d_counts,
+                             hipStream_t s);
 
 // ---- radix-partitioned aggregation (partition_kernels.hip) ------------------
 void launch_rpa_hist(gb_params const* dP, rpa_params const* dR, rpa_params const& hR, int64_t n,

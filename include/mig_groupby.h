@@ -287,7 +287,8 @@ void mig_result_free(mig_groupby_result* r);
 int32_t mig_last_stats(double* values, int32_t n);
 
 /* Turns hipEvent timing of each phase of mig_groupby_aggregate on (1) or off
- * (0) for the calling process; events are recorded on the call's stream.
+ * (0) for the calling process, or (2) times only the main aggregation kernel
+ * (two events per call); events are recorded on the call's stream.
  * Instrumentation only, with no counterpart in the reference (which marks
  * the call with an NVTX range, cpp/src/groupby/groupby.cu:209). */
 void mig_set_phase_timing(int32_t enable);
