@@ -136,8 +136,9 @@ __device__ __forceinline__ bool elem_equal(dcol const& c, int64_t a, int64_t b)
     str_span(c, a, ab, ae);
     str_span(c, b, bb, be);
     if (ae - ab != be - bb) return false;
+    auto const d = gp(c.data);
     for (int64_t i = 0; i < ae - ab; i++)
-      if (c.data[ab + i] != c.data[bb + i]) return false;
+      if (d[ab + i] != d[bb + i]) return false;
     return true;
   }
   return load_fixed_bits(c, a) == load_fixed_bits(c, b);

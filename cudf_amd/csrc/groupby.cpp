@@ -40,7 +40,7 @@ std::atomic<int> g_strategy{MIG_STRATEGY_AUTO};
 // Partitioned path: LDS hash slots and byte budget of the aggregating
 // workgroup (two 512-thread workgroups fit in a CU's 160 KB of LDS).
 constexpr int32_t RPA_LDS_SLOTS      = 4096;
-constexpr int32_t RPA_LDS_BUDGET     = 76 * 1024;
+constexpr int32_t RPA_LDS_BUDGET     = 80 * 1024;  // two aggregating workgroups per CU (160 KB)
 constexpr int RPA_MAX_LOG_BINS       = 15;
 constexpr int RPA_MAX_LOG_FANOUT     = 8;
 constexpr int32_t RPA_STAGE_BUDGET   = 56 * 1024;  // LDS staging of a scatter tile (3 workgroups per CU)
@@ -786,11 +786,43 @@ std::unique_ptr<mig_groupby_result> groupby_aggregate(mig_column_view const* key
     res->results.clear();
   };
 
+  // Key columns of general keys from each group's representative input row:
+  // fixed-width keys and validity by k_gather_keys, string keys by sizes, a
+  // scan of the offsets and a character gather.
+  auto gather_general_keys = [&](uint32_t const* rep_rows, int64_t G) {
+    int32_t* str_sizes = n_str_keys > 0 ? S.get<int32_t>(int64_t(n_str_keys) * G + 1) : nullptr;
+    launch_gather_keys(dP, P, rep_rows, G, str_sizes, stream);
+    int si = 0;
+    for (int i = 0; i < num_keys; i++) {
+      if (storage_of(keys[i].type_id) != ST_STRING) continue;
+      owned_column& oc = res->keys[i];
+      auto* offs       = static_cast<int32_t*>(oc.col.offsets);
+      if (G == 0) {
+        MIG_HIP_TRY(hipMemsetAsync(offs, 0, sizeof(int32_t), stream));
+        si++;
+        continue;
+      }
+      MIG_HIP_TRY(hipMemcpyAsync(offs, str_sizes + int64_t(si) * G, sizeof(int32_t) * G,
+                                 hipMemcpyDeviceToDevice, stream));
+      launch_exclusive_scan_i32(offs, G, offs + G, stream);
+      auto* h = reinterpret_cast<int32_t*>(pin + 2 * sizeof(gb_params));
+      MIG_HIP_TRY(hipMemcpyAsync(h, offs + G, 4, hipMemcpyDeviceToHost, stream));
+      MIG_HIP_TRY(hipStreamSynchronize(stream));
+      int32_t const total_chars = *h;
+      MIG_EXPECTS(total_chars >= 0, "unique string keys exceed 2^31 characters");
+      oc.data_bytes     = size_t(total_chars);
+      oc.col.data       = A.allocate(oc.data_bytes);
+      oc.col.data_bytes = total_chars;
+      launch_gather_string_chars(P.keys[i], rep_rows, G, offs, static_cast<uint8_t*>(oc.col.data), stream);
+      si++;
+    }
+  };
+
   int64_t G = -1;
 
   // ---- strategy: radix-partitioned LDS aggregation for high cardinality -----
   int const forced = g_strategy.load();
-  bool rpa_ok      = P.key_mode == KM_INLINE;
+  bool rpa_ok      = P.key_mode == KM_INLINE || P.key_mode == KM_GENERAL;
   int n_flagged    = 0;
   for (int v = 0; v < P.num_vals && rpa_ok; v++) {
     if (P.val_field_begin[v] == P.val_field_begin[v + 1]) continue;
@@ -825,7 +857,9 @@ std::unique_ptr<mig_groupby_result> groupby_aggregate(mig_column_view const* key
   int32_t const max_fill = int32_t(0.85 * S_slots);
   bool use_rpa = false;
   if (forced == MIG_STRATEGY_PARTITIONED) use_rpa = rpa_ok;
-  else if (forced == MIG_STRATEGY_AUTO) use_rpa = rpa_ok && N >= (1 << 20) && est >= 65536;
+  else if (forced == MIG_STRATEGY_AUTO)  // and the estimate fits the largest partition
+    use_rpa = rpa_ok && N >= (1 << 20) && est >= 65536 &&
+              est <= double(int64_t(1) << RPA_MAX_LOG_BINS) * max_fill;
 
   if (use_rpa) {
     // LDS offsets (8-byte aligned)
@@ -955,8 +989,9 @@ std::unique_ptr<mig_groupby_result> groupby_aggregate(mig_column_view const* key
         }
       }
       stats[STAT_STATE_BYTES] = double(part_bytes);
-      alloc_outputs(rows_cap, 0, true);
+      alloc_outputs(rows_cap, P.key_mode == KM_GENERAL ? rows_cap : 0, true);
       R.out_capacity = uint64_t(rows_cap);
+      R.out_rep      = P.key_mode == KM_GENERAL ? S.get<uint32_t>(std::max<int64_t>(1, rows_cap)) : nullptr;
       upload(0);
       MIG_HIP_TRY(hipMemcpyAsync(dR, &R, sizeof(rpa_params), hipMemcpyHostToDevice, stream));
       T.mark(10);
@@ -964,7 +999,7 @@ std::unique_ptr<mig_groupby_result> groupby_aggregate(mig_column_view const* key
       T.mark(11);
       if (R.two_level) launch_rpa_scatter2(dP, dR, R, dp.num_cus, 0, R.num_tiles1, stream);
       T.mark(12);
-      launch_rpa_aggregate(dP, dR, R, 0, B, stream);
+      launch_rpa_aggregate(dP, dR, R, P.key_mode == KM_GENERAL, 0, B, stream);
       T.mark(13);
       MIG_HIP_TRY(hipGetLastError());
       auto* h = reinterpret_cast<uint64_t*>(pin + 2 * sizeof(gb_params));
@@ -976,6 +1011,7 @@ std::unique_ptr<mig_groupby_result> groupby_aggregate(mig_column_view const* key
         for (auto& k : res->keys) k.col.size = int32_t(G);
         for (auto& rc : res->results)
           for (auto& c : rc) c.col.size = int32_t(G);
+        if (P.key_mode == KM_GENERAL) gather_general_keys(R.out_rep, G);
         stats[STAT_STRATEGY] = MIG_STRATEGY_PARTITIONED;
         stats[STAT_CAPACITY] = double(B);
         stats[STAT_RETRIES]  = attempt;
@@ -1097,34 +1133,8 @@ std::unique_ptr<mig_groupby_result> groupby_aggregate(mig_column_view const* key
     uint32_t* rep_rows = P.key_mode == KM_GENERAL ? S.get<uint32_t>(G) : nullptr;
     upload(1);  // stream-ordered after the kernels that read the previous block
     launch_outputs(dP, P, map, G, rep_rows, stream);
-    if (P.key_mode == KM_GENERAL) {
-      int32_t* str_sizes = n_str_keys > 0 ? S.get<int32_t>(int64_t(n_str_keys) * G + 1) : nullptr;
-      launch_gather_keys(dP, P, rep_rows, G, str_sizes, stream);
-      int si = 0;
-      for (int i = 0; i < num_keys; i++) {
-        if (storage_of(keys[i].type_id) != ST_STRING) continue;
-        owned_column& oc = res->keys[i];
-        auto* offs       = static_cast<int32_t*>(oc.col.offsets);
-        if (G == 0) {
-          MIG_HIP_TRY(hipMemsetAsync(offs, 0, sizeof(int32_t), stream));
-          si++;
-          continue;
-        }
-        MIG_HIP_TRY(hipMemcpyAsync(offs, str_sizes + int64_t(si) * G, sizeof(int32_t) * G,
-                                   hipMemcpyDeviceToDevice, stream));
-        launch_exclusive_scan_i32(offs, G, offs + G, stream);
-        auto* h = reinterpret_cast<int32_t*>(pin + 2 * sizeof(gb_params));
-        MIG_HIP_TRY(hipMemcpyAsync(h, offs + G, 4, hipMemcpyDeviceToHost, stream));
-        MIG_HIP_TRY(hipStreamSynchronize(stream));
-        int32_t const total_chars = *h;
-        MIG_EXPECTS(total_chars >= 0, "unique string keys exceed 2^31 characters");
-        oc.data_bytes     = size_t(total_chars);
-        oc.col.data       = A.allocate(oc.data_bytes);
-        oc.col.data_bytes = total_chars;
-        launch_gather_string_chars(P.keys[i], rep_rows, G, offs, static_cast<uint8_t*>(oc.col.data), stream);
-        si++;
-      }
-    }
+    if (P.key_mode == KM_GENERAL) gather_general_keys(rep_rows, G);
+
   }
   MIG_HIP_TRY(hipGetLastError());
   stats[STAT_KEY_MODE] = P.key_mode;

@@ -185,6 +185,7 @@ struct rpa_params {
   int32_t* overflow;                 // a bin had more groups than max_ids
   int32_t pad1;
   uint64_t out_capacity;             // rows allocated in every output column
+  uint32_t* out_rep;                 // general keys: representative input row of each output row
   // Pass-A plan of the aggregate kernel, read into registers once per
   // workgroup: the accumulators of value column plan_col[i] are the fields
   // [plan_fb[i], plan_fb[i + 1]), field f described by plan_fd[f] =
@@ -248,8 +249,8 @@ void launch_rpa_scatter1(gb_params const* dP, rpa_params const* dR, rpa_params c
 // second pass over the tiles [tile_begin, tile_end) of the first-level bins
 void launch_rpa_scatter2(gb_params const* dP, rpa_params const* dR, rpa_params const& hR, int num_cus,
                          int64_t tile_begin, int64_t tile_end, hipStream_t s);
-// final bins [bin_begin, bin_end)
-void launch_rpa_aggregate(gb_params const* dP, rpa_params const* dR, rpa_params const& hR, int64_t bin_begin,
-                          int64_t bin_end, hipStream_t s);
+// final bins [bin_begin, bin_end); `general`: general keys (row hash, row) words
+void launch_rpa_aggregate(gb_params const* dP, rpa_params const* dR, rpa_params const& hR, bool general,
+                          int64_t bin_begin, int64_t bin_end, hipStream_t s);
 
 }  // namespace mig

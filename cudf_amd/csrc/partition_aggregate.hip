@@ -25,6 +25,11 @@ constexpr int WG_SCOPE = __HIP_MEMORY_SCOPE_WORKGROUP;
 
 constexpr int RPA_CACHED = 6;     // rows per thread per chunk; their slots stay in registers
 constexpr int RPA_OUT_BATCH = 4;  // output columns whose LDS reads are issued together
+// Probes before a row's key counts as not fitting: the table is filled to at
+// most ~85%, where a double-hashing probe longer than this has probability
+// below 1e-18, so reaching it means the bin has more groups than the table
+// holds (the call is retried with more bins).
+constexpr int RPA_PROBE_LIMIT = 256;
 
 // Apply `f(k)` to the rows k of a chunk whose bit is set in `ok` (the loop is
 // unrolled; the caller's switch on the accumulator kind stays outside it).
@@ -299,9 +304,18 @@ __device__ __forceinline__ uint32_t lane_value(uint32_t v, int i)
 // sequences short at the table's ~60% fill where linear probing clusters.
 // Inserted keys are counted once per chunk; a bin whose count passes
 // max_fill is retried with more bins.
-template <bool INSERT>
-__device__ __forceinline__ void chunk_slots(acc_plan const& A, uint64_t* tkeys, rpa_misc* misc, int S,
-                                            int64_t r0, int64_t e, int (&slot)[RPA_CACHED], uint64_t& flp)
+// Whether two general-key table words (row hash, row) are the same group:
+// equal hashes and equal key rows (nulls equal, NaN equal NaN, -0.0 equal
+// 0.0, strings byte-wise).
+__device__ __forceinline__ bool same_general(gb_params const& P, uint64_t a, uint64_t b)
+{
+  return (a >> 32) == (b >> 32) && rows_equal(P, int64_t(uint32_t(a)), int64_t(uint32_t(b)));
+}
+
+template <bool INSERT, bool GENERAL>
+__device__ __forceinline__ void chunk_slots(gb_params const& P, acc_plan const& A, uint64_t* tkeys,
+                                            rpa_misc* misc, int S, int64_t r0, int64_t e,
+                                            int (&slot)[RPA_CACHED], uint64_t& flp)
 {
   uint64_t key[RPA_CACHED];
   uint8_t fl[RPA_CACHED];
@@ -328,7 +342,7 @@ __device__ __forceinline__ void chunk_slots(acc_plan const& A, uint64_t* tkeys, 
       if (INSERT) misc->present1 = 1;
       slot[k] = S + 1;
     } else {
-      uint32_t const h = inline_key_hash(A.key_storage, key[k]);
+      uint32_t const h = GENERAL ? uint32_t(key[k] >> 32) : inline_key_hash(A.key_storage, key[k]);
       pos[k]           = h & uint32_t(S - 1);
       step[k]          = ((h >> 9) | 1u) & uint32_t(S - 1);
       pending |= 1u << k;
@@ -338,14 +352,16 @@ __device__ __forceinline__ void chunk_slots(acc_plan const& A, uint64_t* tkeys, 
 #pragma unroll
   for (int k = 0; k < RPA_CACHED; k++) {
     if (!((pending >> k) & 1u)) continue;
-    uint32_t p = pos[k];
-    int i      = 0;
-    for (; i < S; i++) {
+    uint32_t p      = pos[k];
+    int const limit = S < RPA_PROBE_LIMIT ? S : RPA_PROBE_LIMIT;
+    int i           = 0;
+    for (; i < limit; i++) {
       uint64_t const cur = __hip_atomic_load(tkeys + p, __ATOMIC_RELAXED, WG_SCOPE);
       if (cur == key[k]) break;
+      if (GENERAL && cur != EMPTY_KEY && same_general(P, cur, key[k])) break;
       if (cur == EMPTY_KEY) {
         if (!INSERT) {
-          i = S;
+          i = limit;
           break;
         }
         uint64_t expected = EMPTY_KEY;
@@ -354,11 +370,11 @@ __device__ __forceinline__ void chunk_slots(acc_plan const& A, uint64_t* tkeys, 
           inserted++;
           break;
         }
-        if (expected == key[k]) break;
+        if (expected == key[k] || (GENERAL && same_general(P, expected, key[k]))) break;
       }
       p = (p + step[k]) & uint32_t(S - 1);
     }
-    if (i < S) slot[k] = int(p);
+    if (i < limit) slot[k] = int(p);
     else if (INSERT) __hip_atomic_store(&misc->overflow, 1, __ATOMIC_RELAXED, WG_SCOPE);  // table full
   }
   if (INSERT && inserted) __hip_atomic_fetch_add(&misc->fill, inserted, __ATOMIC_RELAXED, WG_SCOPE);
@@ -476,6 +492,7 @@ __device__ __forceinline__ bool m2_from_sums(field_desc const& SUMF, int32_t st,
   return true;
 }
 
+template <bool GENERAL>
 __global__ void __launch_bounds__(RPA_AGG_BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
   k_rpa_aggregate(gb_params const* __restrict__ Pp, rpa_params const* __restrict__ Rp, int64_t bin_begin)
 {
@@ -522,7 +539,7 @@ __global__ void __launch_bounds__(RPA_AGG_BLOCK) __attribute__((amdgpu_waves_per
       break;
     int slot[RPA_CACHED];
     uint64_t fl;
-    chunk_slots<true>(A, tkeys, misc, S, c + tid, e, slot, fl);
+    chunk_slots<true, GENERAL>(P, A, tkeys, misc, S, c + tid, e, slot, fl);
     chunk_accumulate(A, lds, c + tid, e, slot, fl);
   }
   __syncthreads();
@@ -561,7 +578,7 @@ __global__ void __launch_bounds__(RPA_AGG_BLOCK) __attribute__((amdgpu_waves_per
     for (int64_t c = s; c < e; c += CHUNK) {
       int slot[RPA_CACHED];
       uint64_t fl;
-      chunk_slots<false>(A, tkeys, misc, S, c + tid, e, slot, fl);
+      chunk_slots<false, GENERAL>(P, A, tkeys, misc, S, c + tid, e, slot, fl);
       chunk_deviations(P, R, lds, c + tid, e, slot, fl);
     }
     __syncthreads();
@@ -622,11 +639,15 @@ __global__ void __launch_bounds__(RPA_AGG_BLOCK) __attribute__((amdgpu_waves_per
     bool const in_range = live && pos < cap;
     bool const kvalid   = slot != S;
     if (live && !in_range) astore(R.overflow, 1);
-    if (in_range) {
-      uint64_t const key = slot < S ? tkeys[slot] : (slot == S ? 0ull : EMPTY_KEY);
-      store_width(P.out_keys[0], int64_t(pos), kw, key);
+    if (GENERAL) {  // keys are gathered from the representative rows afterwards
+      if (in_range) gp(R.out_rep)[pos] = uint32_t(tkeys[slot]);
+    } else {
+      if (in_range) {
+        uint64_t const key = slot < S ? tkeys[slot] : (slot == S ? 0ull : EMPTY_KEY);
+        store_width(P.out_keys[0], int64_t(pos), kw, key);
+      }
+      if (P.out_key_mask[0]) store_valid_word(P.out_key_mask[0], c0, base, end, cap, in_range && kvalid);
     }
-    if (P.out_key_mask[0]) store_valid_word(P.out_key_mask[0], c0, base, end, cap, in_range && kvalid);
     // outputs in batches: the LDS reads of a batch are issued together, then
     // the batch's values are converted and stored
     for (int o0 = 0; o0 < P.num_outs; o0 += RPA_OUT_BATCH) {
@@ -689,13 +710,19 @@ __global__ void __launch_bounds__(RPA_AGG_BLOCK) __attribute__((amdgpu_waves_per
   }
 }
 
-void launch_rpa_aggregate(gb_params const* dP, rpa_params const* dR, rpa_params const& hR, int64_t bin_begin,
-                          int64_t bin_end, hipStream_t s)
+void launch_rpa_aggregate(gb_params const* dP, rpa_params const* dR, rpa_params const& hR, bool general,
+                          int64_t bin_begin, int64_t bin_end, hipStream_t s)
 {
   if (bin_end <= bin_begin) return;
-  allow_dynamic_lds(reinterpret_cast<void const*>(&k_rpa_aggregate), hR.lds_bytes);
-  hipLaunchKernelGGL(k_rpa_aggregate, dim3(bin_end - bin_begin), dim3(RPA_AGG_BLOCK), hR.lds_bytes, s, dP, dR,
-                     bin_begin);
+  if (general) {
+    allow_dynamic_lds(reinterpret_cast<void const*>(&k_rpa_aggregate<true>), hR.lds_bytes);
+    hipLaunchKernelGGL(k_rpa_aggregate<true>, dim3(bin_end - bin_begin), dim3(RPA_AGG_BLOCK), hR.lds_bytes, s,
+                       dP, dR, bin_begin);
+  } else {
+    allow_dynamic_lds(reinterpret_cast<void const*>(&k_rpa_aggregate<false>), hR.lds_bytes);
+    hipLaunchKernelGGL(k_rpa_aggregate<false>, dim3(bin_end - bin_begin), dim3(RPA_AGG_BLOCK), hR.lds_bytes, s,
+                       dP, dR, bin_begin);
+  }
 }
 
 }  // namespace mig

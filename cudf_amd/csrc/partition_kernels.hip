@@ -253,8 +253,13 @@ __global__ void __launch_bounds__(RPA_SCATTER_BLOCK) __attribute__((amdgpu_waves
       if constexpr (SECOND) {
         kf[k] = gp(static_cast<uint8_t const*>(R.flagsA))[row];
         kb[k] = gp(static_cast<uint64_t const*>(R.keysA))[row];
-        h     = (kf[k] & RPA_KEY_NULL) ? null_key_hash() : inline_key_hash(P.key_storage, kb[k]);
+        h     = record_hash(P, kb[k], kf[k]);
         bin[k] = int(top_bits(h, R.log_bins) & uint32_t(fan - 1));
+      } else if (P.key_mode == KM_GENERAL) {
+        bool kn;
+        if (!input_key(P, row, kb[k], kn, h)) continue;
+        kf[k]  = 0;
+        bin[k] = int(top_bits(h, R.log_f1));
       } else {
         dcol const& kc       = P.keys[0];
         bool const kvalid    = dcol_valid(kc, row);

@@ -175,3 +175,52 @@ def test_variance_one_pass_and_exact_fallback(case):
             continue
         assert ok
         assert abs(got - w) <= 1e-10 * abs(w) + (0.0 if w != 0 else 1e-300), (case, key, got, w)
+
+
+def _string_pool(rng, size, max_len=24):
+    return [bytes(rng.integers(97, 123, int(rng.integers(0, max_len)), dtype=np.uint8)) for _ in range(size)]
+
+
+@pytest.mark.parametrize("policy", [0, 1])
+def test_general_keys_partitioned(policy):
+    """Several key columns (int32, string, float64 with NaN and -0.0), nulls
+    in keys and values: the partitioned path compares key rows instead of
+    key bits."""
+    from tests.gbutil import orc
+    rng = np.random.default_rng(31)
+    n = 80_000
+    k1 = rng.integers(0, 300, n).astype(np.int32)
+    pool = _string_pool(rng, 200)
+    k2 = orc.HostColumn.from_strings([pool[i] for i in rng.integers(0, len(pool), n)], rng.random(n) > 0.03)
+    k3 = (rng.integers(0, 3, n) * 0.5).astype(np.float64)
+    k3[rng.random(n) < 0.02] = np.nan
+    k3[rng.random(n) < 0.02] = -0.0
+    v = rng.standard_normal(n)
+    assert_parity([host_col(k1, rng.random(n) > 0.03), k2, host_col(k3)],
+                  [(host_col(v, rng.random(n) > 0.1), ALL_NUMERIC_AGGS)], policy)
+    assert _stats()["strategy"] == 2 and _stats()["key_mode"] == 1
+
+
+@pytest.mark.parametrize("policy", [0, 1])
+def test_single_string_key_two_level(policy):
+    """One string key with enough groups for a two-level partition."""
+    from tests.gbutil import orc
+    rng = np.random.default_rng(32)
+    n = 300_000
+    pool = [b"k%07d" % i + bytes(rng.integers(97, 123, int(rng.integers(0, 9)), dtype=np.uint8))
+            for i in range(250_000)]
+    keys = orc.HostColumn.from_strings([pool[i] for i in rng.integers(0, len(pool), n)], rng.random(n) > 0.02)
+    v = rng.integers(-1000, 1000, n).astype(np.int64)
+    assert_parity([keys], [(host_col(v, rng.random(n) > 0.05), C2_AGGS)], policy)
+    assert _stats()["strategy"] == 2 and _stats()["capacity"] > 256
+
+
+@pytest.mark.parametrize("policy", [0, 1])
+def test_c3_shape_partitioned(policy):
+    """The skewed three-column key of the C3 workload (int32, Zipf int64,
+    string) at 200k rows."""
+    from tests.gbutil import c3_data
+    k1, k2, k3, f32, i64 = c3_data(n=200_000, seed=5)
+    aggs = [(AGG["SUM"], 1), (AGG["COUNT_VALID"], 1), (AGG["MAX"], 1)]
+    assert_parity([k1, k2, k3], [(f32, aggs), (i64, aggs)], policy)
+    assert _stats()["strategy"] == 2
