@@ -571,7 +571,8 @@ mig_batch_plan* batch_plan_create(mig_batch_instance const* instances, int32_t n
     off += ((BATCH_LDS_SLOTS + 2) * 4 + 7) & ~7;
   }
   P.lds_bytes = std::max(off, m2_off);
-  if (P.lds_bytes > 160 * 1024 || p.opt.lds_max_groups < 0) {
+  // (the LDS path holds its field plan one field per lane: at most 64 fields)
+  if (P.lds_bytes > 160 * 1024 || p.opt.lds_max_groups < 0 || nf > 64) {
     p.opt.lds_max_groups = -1;  // off, or does not fit: direct updates only
     P.lds_bytes          = 0;
   }

@@ -17,6 +17,7 @@ namespace mig {
 constexpr int BATCH_LDS_SLOTS = 1024;  // LDS table of the pre-aggregating path
 constexpr int BATCH_LDS_FILL  = 768;   // keys a tile may insert before rows bypass LDS
 constexpr int BATCH_ELEM      = 8;     // bytes per accumulator / output element slot
+constexpr int BATCH_CHUNK     = 4;     // rows per thread handled together in an LDS tile
 constexpr int BATCH_MAX_PARTS = 16;    // hash parts of an LDS-aggregated instance (more: direct updates)
 
 // One instance.  Its table occupies slots [slot_base, slot_base + cap + 2) of

@@ -323,42 +323,89 @@ __global__ void __launch_bounds__(NT) k_batch_aggregate(batch_params const* __re
     }
     __syncthreads();
     uint32_t const parts = uint32_t(I.lds);
-    for (int64_t row = r0 + threadIdx.x; row < r1; row += NT) {
-      int id        = -1;
-      uint64_t bits = 0;
-      uint32_t h    = 0;
-      if (!dcol_valid(I.key, row)) {
-        if (B.skip_null_keys) continue;
-        if (parts > 1 && part_of(hash_combine(0u, NULL_HASH), parts) != part) continue;
-        id            = S + 1;
-        present_sh[1] = 1;
-      } else {
-        bits = load_fixed_bits(I.key, row);
-        h    = b_key_hash(I.key.storage, bits);
-        if (parts > 1 && part_of(h, parts) != part) continue;  // another item's part
-        if (bits == EMPTY_KEY) {
-          id            = S;
-          present_sh[0] = 1;
+    // The item's field plan, one field per lane of a VGPR (kind | (LDS offset
+    // + 1) << 8), fetched with v_readlane: reading descriptors from memory
+    // inside the row loop would wait for the LDS atomics in flight each time.
+    int const lane         = int(threadIdx.x & 63);
+    uint32_t const fd_lane = lane < B.num_fields
+                               ? uint32_t(kinds_sh[lane]) | (uint32_t(B.lds_field_off[lane] + 1) << 8)
+                               : 0u;
+    // BATCH_CHUNK rows per thread at a time: their key loads, then their
+    // LDS slots, then each value column's loads, in flight together
+    for (int64_t c0 = r0 + threadIdx.x; c0 < r1; c0 += int64_t(BATCH_CHUNK) * NT) {
+      uint64_t bits[BATCH_CHUNK];
+      bool kv[BATCH_CHUNK];
+      int id[BATCH_CHUNK];
+      uint32_t hh[BATCH_CHUNK];
+#pragma unroll
+      for (int k = 0; k < BATCH_CHUNK; k++) {  // rows past the tile read its last row
+        int64_t const row = c0 + int64_t(k) * NT;
+        int64_t const rc  = row < r1 ? row : r1 - 1;
+        kv[k]             = dcol_valid(I.key, rc);
+        bits[k]           = load_fixed_bits(I.key, rc);
+      }
+#pragma unroll
+      for (int k = 0; k < BATCH_CHUNK; k++) {
+        int64_t const row = c0 + int64_t(k) * NT;
+        id[k]             = -2;  // no part in this item
+        hh[k]             = 0;
+        if (row >= r1) continue;
+        if (!kv[k]) {
+          if (B.skip_null_keys) continue;
+          if (parts > 1 && part_of(hash_combine(0u, NULL_HASH), parts) != part) continue;
+          id[k]         = S + 1;
+          present_sh[1] = 1;
         } else {
-          id = lds_insert(tkeys, &fill_sh, bits, h, EMPTY_KEY, S, BATCH_LDS_FILL);
+          uint32_t const h = b_key_hash(I.key.storage, bits[k]);
+          if (parts > 1 && part_of(h, parts) != part) continue;  // another item's part
+          hh[k] = h;
+          if (bits[k] == EMPTY_KEY) {
+            id[k]         = S;
+            present_sh[0] = 1;
+          } else {
+            id[k] = lds_insert(tkeys, &fill_sh, bits[k], h, EMPTY_KEY, S, BATCH_LDS_FILL);
+          }
         }
       }
-      if (id < 0) {
-        // the LDS table is full: this row goes straight to the instance table
-        int64_t const slot = b_locate(B, I, inst, bits, h);
-        if (slot >= 0) b_update_global(B, kinds_sh, inst, slot, row);
-        continue;
+      uint32_t live = 0;
+#pragma unroll
+      for (int k = 0; k < BATCH_CHUNK; k++) {
+        live |= uint32_t(id[k] >= 0) << k;
+        if (id[k] == -1) {  // the LDS table is full: this row goes straight to the instance table
+          int64_t const slot = b_locate(B, I, inst, bits[k], hh[k]);
+          if (slot >= 0) b_update_global(B, kinds_sh, inst, slot, c0 + int64_t(k) * NT);
+        }
       }
-      if (B.lds_count_off >= 0)
-        __hip_atomic_fetch_add(lds_at<uint32_t>(lds, B.lds_count_off, id), 1u, __ATOMIC_RELAXED, WG);
+      if (live == 0) continue;
+      if (B.lds_count_off >= 0) {
+#pragma unroll
+        for (int k = 0; k < BATCH_CHUNK; k++)
+          if ((live >> k) & 1u)
+            __hip_atomic_fetch_add(lds_at<uint32_t>(lds, B.lds_count_off, id[k]), 1u, __ATOMIC_RELAXED, WG);
+      }
       for (int v = 0; v < B.num_vals; v++) {
         int const fb = B.val_field_begin[v], fe = B.val_field_begin[v + 1];
         if (fb == fe) continue;
-        dcol const& c = B.vals[int64_t(inst) * B.num_vals + v];
-        if (!dcol_valid(c, row)) continue;
-        loaded_value const val = load_value(c, row);
-        for (int f = fb; f < fe; f++)
-          if (B.lds_field_off[f] >= 0) lds_apply(kinds_sh[f], lds_at<uint64_t>(lds, B.lds_field_off[f], id), val);
+        dcol const cv = B.vals[int64_t(inst) * B.num_vals + v];
+        loaded_value val[BATCH_CHUNK];
+        uint32_t ok = live;
+#pragma unroll
+        for (int k = 0; k < BATCH_CHUNK; k++) {
+          val[k] = loaded_value{0, 0.0, 0.0f, 0};
+          if (!((live >> k) & 1u)) continue;
+          int64_t const row = c0 + int64_t(k) * NT;
+          if (dcol_valid(cv, row)) val[k] = load_value(cv, row);
+          else ok &= ~(1u << k);
+        }
+        for (int f = fb; f < fe; f++) {
+          uint32_t const d = uint32_t(__builtin_amdgcn_readlane(int(fd_lane), f));
+          int const off    = int(d >> 8) - 1;
+          if (off < 0) continue;
+          int32_t const kind = int32_t(d & 0xffu);
+#pragma unroll
+          for (int k = 0; k < BATCH_CHUNK; k++)
+            if ((ok >> k) & 1u) lds_apply(kind, lds_at<uint64_t>(lds, off, id[k]), val[k]);
+        }
       }
     }
     __syncthreads();
