@@ -291,16 +291,57 @@ __global__ void __launch_bounds__(NT) k_batch_aggregate(batch_params const* __re
     int64_t const r1    = (r0 + B.tile_rows < I.rows) ? r0 + B.tile_rows : I.rows;
 
     if (!I.lds) {
-      for (int64_t row = r0 + threadIdx.x; row < r1; row += NT) {
-        int64_t slot;
-        if (!dcol_valid(I.key, row)) {
-          if (B.skip_null_keys) continue;
-          slot = b_null_slot(B, I, inst);
-        } else {
-          uint64_t const bits = load_fixed_bits(I.key, row);
-          slot                = b_locate(B, I, inst, bits, b_key_hash(I.key.storage, bits));
+      // direct updates of the instance table, BATCH_CHUNK rows per thread at
+      // a time (their key loads in flight together, then their slots, then
+      // their updates)
+      for (int64_t c0 = r0 + threadIdx.x; c0 < r1; c0 += int64_t(BATCH_CHUNK) * NT) {
+        uint64_t bits[BATCH_CHUNK];
+        bool kv[BATCH_CHUNK];
+#pragma unroll
+        for (int k = 0; k < BATCH_CHUNK; k++) {  // rows past the tile read its last row
+          int64_t const row = c0 + int64_t(k) * NT;
+          int64_t const rc  = row < r1 ? row : r1 - 1;
+          kv[k]             = dcol_valid(I.key, rc);
+          bits[k]           = load_fixed_bits(I.key, rc);
         }
-        if (slot >= 0) b_update_global(B, kinds_sh, inst, slot, row);
+        int64_t slot[BATCH_CHUNK];
+#pragma unroll
+        for (int k = 0; k < BATCH_CHUNK; k++) {
+          slot[k] = -1;
+          if (c0 + int64_t(k) * NT >= r1) continue;
+          if (!kv[k]) {
+            if (!B.skip_null_keys) slot[k] = b_null_slot(B, I, inst);
+          } else {
+            slot[k] = b_locate(B, I, inst, bits[k], b_key_hash(I.key.storage, bits[k]));
+          }
+        }
+        if (B.need_count_all) {
+#pragma unroll
+          for (int k = 0; k < BATCH_CHUNK; k++)
+            if (slot[k] >= 0) aadd(B.slot_count + slot[k], 1u);
+        }
+        for (int v = 0; v < B.num_vals; v++) {
+          int const fb = B.val_field_begin[v], fe = B.val_field_begin[v + 1];
+          if (fb == fe) continue;
+          dcol const cv = B.vals[int64_t(inst) * B.num_vals + v];
+          loaded_value val[BATCH_CHUNK];
+          uint32_t ok = 0;
+#pragma unroll
+          for (int k = 0; k < BATCH_CHUNK; k++) {
+            val[k]            = loaded_value{0, 0.0, 0.0f, 0};
+            int64_t const row = c0 + int64_t(k) * NT;
+            if (slot[k] >= 0 && dcol_valid(cv, row)) {
+              val[k] = load_value(cv, row);
+              ok |= 1u << k;
+            }
+          }
+          for (int f = fb; f < fe; f++) {
+            int32_t const kind = kinds_sh[f];
+#pragma unroll
+            for (int k = 0; k < BATCH_CHUNK; k++)
+              if ((ok >> k) & 1u) acc_apply(kind, fa(B.fields[f], slot[k]), val[k]);
+          }
+        }
       }
       __syncthreads();
       continue;
