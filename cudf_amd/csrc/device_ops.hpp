@@ -135,10 +135,18 @@ __device__ __forceinline__ bool elem_equal(dcol const& c, int64_t a, int64_t b)
     int64_t ab, ae, bb, be;
     str_span(c, a, ab, ae);
     str_span(c, b, bb, be);
-    if (ae - ab != be - bb) return false;
+    int64_t const len = ae - ab;
+    if (len != be - bb) return false;
+    // 8 bytes of each string per step, their loads in flight together (a
+    // byte-by-byte loop with an exit per byte waits for every load in turn)
     auto const d = gp(c.data);
-    for (int64_t i = 0; i < ae - ab; i++)
-      if (d[ab + i] != d[bb + i]) return false;
+    for (int64_t i = 0; i < len; i += 8) {
+      uint32_t diff = 0;
+#pragma unroll
+      for (int j = 0; j < 8; j++)
+        if (i + j < len) diff |= uint32_t(d[ab + i + j] ^ d[bb + i + j]);
+      if (diff) return false;
+    }
     return true;
   }
   return load_fixed_bits(c, a) == load_fixed_bits(c, b);
