@@ -224,3 +224,38 @@ def test_c3_shape_partitioned(policy):
     aggs = [(AGG["SUM"], 1), (AGG["COUNT_VALID"], 1), (AGG["MAX"], 1)]
     assert_parity([k1, k2, k3], [(f32, aggs), (i64, aggs)], policy)
     assert _stats()["strategy"] == 2
+
+
+@pytest.mark.parametrize("policy", [0, 1])
+def test_general_keys_string_edge_cases(policy):
+    """Two string key columns with empty strings, long strings (over 64
+    bytes, lengths not multiples of 8) and strings that differ only in the
+    last byte, plus an int64 key."""
+    from tests.gbutil import orc
+    rng = np.random.default_rng(33)
+    n = 60_000
+    base = [b"", b"a", b"ab", b"x" * 7, b"x" * 8, b"x" * 9, b"y" * 65, b"y" * 64 + b"z", b"q" * 130]
+    pool = base + [b"p" * int(rng.integers(0, 40)) + bytes([97 + i % 26]) for i in range(400)]
+    s1 = orc.HostColumn.from_strings([pool[i] for i in rng.integers(0, len(pool), n)], rng.random(n) > 0.05)
+    s2 = orc.HostColumn.from_strings([pool[i] for i in rng.integers(0, 20, n)])
+    k = rng.integers(0, 5, n).astype(np.int64)
+    v = rng.integers(-50, 50, n).astype(np.int32)
+    assert_parity([s1, host_col(k, rng.random(n) > 0.02), s2],
+                  [(host_col(v, rng.random(n) > 0.1), ALL_NUMERIC_AGGS)], policy)
+    assert _stats()["strategy"] == 2
+
+
+def test_general_keys_underestimated_fall_back():
+    """99% of rows share one (int, string) key and the last 1% are all
+    distinct: the strided sample sees few distinct keys, the bins overflow,
+    and the call must still return the exact result (more bins, then the
+    global table)."""
+    from tests.gbutil import orc
+    n = 1_000_000
+    tail = n // 100
+    k1 = np.zeros(n, dtype=np.int32)
+    k1[-tail:] = np.arange(1, tail + 1)
+    s = orc.HostColumn.from_strings([b"same"] * (n - tail) + [b"t%d" % (i % 7) for i in range(tail)])
+    v = np.arange(n, dtype=np.int64) % 1000
+    ng = assert_parity([host_col(k1), s], [(host_col(v), [(AGG["SUM"], 1), (AGG["COUNT_ALL"], 1)])], 0)
+    assert ng == tail + 1
