@@ -943,18 +943,22 @@ std::unique_ptr<mig_groupby_result> groupby_aggregate(mig_column_view const* key
       T.mark(8);
       launch_rpa_hist(dP, dR, R, N, dp.num_cus, stream);
       T.mark(9);
+      // The output columns are allocated while the histogram kernel runs: a
+      // bin yields at most min(its rows, max_fill + 2) groups, so
+      // B·(max_fill + 2) rows, and at most N, bound the groups before the bin
+      // sizes are known.
+      int64_t const rows_cap = std::min<int64_t>(N, B * int64_t(max_fill + 2));
+      alloc_outputs(rows_cap, P.key_mode == KM_GENERAL ? rows_cap : 0, true);
       std::vector<uint32_t> hist(B);
       MIG_HIP_TRY(hipMemcpyAsync(hist.data(), R.hist, sizeof(uint32_t) * B, hipMemcpyDeviceToHost, stream));
       MIG_HIP_TRY(hipStreamSynchronize(stream));
       // bin layout, cursors and the second-level tile table
       std::vector<int64_t> start(B + 1), t1(F1 + 1);
       std::vector<unsigned long long> c1(F1), c2(B);
-      int64_t rows_cap = 0;
       start[0] = 0;
       for (int64_t b = 0; b < B; b++) {
         start[b + 1] = start[b] + hist[b];
         c2[b]        = (unsigned long long)start[b];
-        rows_cap += std::min<int64_t>(hist[b], max_fill + 2);
       }
       int64_t const nrows = start[B];
       t1[0] = 0;
@@ -989,7 +993,6 @@ std::unique_ptr<mig_groupby_result> groupby_aggregate(mig_column_view const* key
         }
       }
       stats[STAT_STATE_BYTES] = double(part_bytes);
-      alloc_outputs(rows_cap, P.key_mode == KM_GENERAL ? rows_cap : 0, true);
       R.out_capacity = uint64_t(rows_cap);
       R.out_rep      = P.key_mode == KM_GENERAL ? S.get<uint32_t>(std::max<int64_t>(1, rows_cap)) : nullptr;
       upload(0);
