@@ -924,7 +924,6 @@ std::unique_ptr<mig_groupby_result> groupby_aggregate(mig_column_view const* key
       R.log_f1    = R.two_level ? (log_bins + 1) / 2 : log_bins;
       int64_t const B  = int64_t(1) << log_bins;
       int64_t const F1 = int64_t(1) << R.log_f1;
-      int const lf2    = log_bins - R.log_f1;
       R.hist        = S.get<uint32_t>(B);
       R.hist_partial = S.get<uint32_t>(size_t(R.hist_blocks) * B);
       R.bin_start   = S.get<int64_t>(B + 1);
@@ -949,29 +948,10 @@ std::unique_ptr<mig_groupby_result> groupby_aggregate(mig_column_view const* key
       // sizes are known.
       int64_t const rows_cap = std::min<int64_t>(N, B * int64_t(max_fill + 2));
       alloc_outputs(rows_cap, P.key_mode == KM_GENERAL ? rows_cap : 0, true);
-      std::vector<uint32_t> hist(B);
-      MIG_HIP_TRY(hipMemcpyAsync(hist.data(), R.hist, sizeof(uint32_t) * B, hipMemcpyDeviceToHost, stream));
-      MIG_HIP_TRY(hipStreamSynchronize(stream));
-      // bin layout, cursors and the second-level tile table
-      std::vector<int64_t> start(B + 1), t1(F1 + 1);
-      std::vector<unsigned long long> c1(F1), c2(B);
-      start[0] = 0;
-      for (int64_t b = 0; b < B; b++) {
-        start[b + 1] = start[b] + hist[b];
-        c2[b]        = (unsigned long long)start[b];
-      }
-      int64_t const nrows = start[B];
-      t1[0] = 0;
-      for (int64_t b1 = 0; b1 < F1; b1++) {
-        c1[b1]      = (unsigned long long)start[b1 << lf2];
-        int64_t len = start[(b1 + 1) << lf2] - start[b1 << lf2];
-        t1[b1 + 1]  = t1[b1] + (len + R.tile_rows - 1) / R.tile_rows;
-      }
-      R.num_tiles1 = t1[F1];
-      MIG_HIP_TRY(hipMemcpyAsync(R.bin_start, start.data(), sizeof(int64_t) * (B + 1), hipMemcpyHostToDevice, stream));
-      MIG_HIP_TRY(hipMemcpyAsync(R.cursor1, c1.data(), sizeof(unsigned long long) * F1, hipMemcpyHostToDevice, stream));
-      MIG_HIP_TRY(hipMemcpyAsync(R.cursor2, c2.data(), sizeof(unsigned long long) * B, hipMemcpyHostToDevice, stream));
-      MIG_HIP_TRY(hipMemcpyAsync(R.tile_start1, t1.data(), sizeof(int64_t) * (F1 + 1), hipMemcpyHostToDevice, stream));
+      // bin layout (starts, cursors, second-pass tiles) computed on the device
+      // from the histogram; the partition buffers are sized for every row
+      launch_rpa_layout(dR, stream);
+      int64_t const nrows = N;
       // partition buffers
       size_t part_bytes = 0;
       R.keysB  = S.get<uint64_t>(nrows);
@@ -1000,7 +980,7 @@ std::unique_ptr<mig_groupby_result> groupby_aggregate(mig_column_view const* key
       T.mark(10);
       launch_rpa_scatter1(dP, dR, R, N, dp.num_cus, stream);
       T.mark(11);
-      if (R.two_level) launch_rpa_scatter2(dP, dR, R, dp.num_cus, 0, R.num_tiles1, stream);
+      if (R.two_level) launch_rpa_scatter2(dP, dR, R, dp.num_cus, 0, -1, stream);
       T.mark(12);
       launch_rpa_aggregate(dP, dR, R, P.key_mode == KM_GENERAL, 0, B, stream);
       T.mark(13);

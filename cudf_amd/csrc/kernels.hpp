@@ -173,7 +173,6 @@ struct rpa_params {
   unsigned long long* cursor1;       // [F1] next free row of each first-level bin (buffer A)
   unsigned long long* cursor2;       // [B] next free row of each final bin (buffer B)
   int64_t* tile_start1;              // [F1 + 1] scatter-2 tiles before each first-level bin
-  int64_t num_tiles1;
   uint64_t* keysA;
   uint64_t* keysB;
   uint8_t* flagsA;
@@ -246,7 +245,10 @@ void launch_rpa_hist(gb_params const* dP, rpa_params const* dR, rpa_params const
                      int num_cus, hipStream_t s);
 void launch_rpa_scatter1(gb_params const* dP, rpa_params const* dR, rpa_params const& hR, int64_t n,
                          int num_cus, hipStream_t s);
+// bin layout (bin_start, cursors, second-pass tiles) from the histogram
+void launch_rpa_layout(rpa_params const* dR, hipStream_t s);
 // second pass over the tiles [tile_begin, tile_end) of the first-level bins
+// (tile_end < 0: to the last tile, counted on the device)
 void launch_rpa_scatter2(gb_params const* dP, rpa_params const* dR, rpa_params const& hR, int num_cus,
                          int64_t tile_begin, int64_t tile_end, hipStream_t s);
 // final bins [bin_begin, bin_end); `general`: general keys (row hash, row) words

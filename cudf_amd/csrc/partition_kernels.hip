@@ -76,6 +76,61 @@ __global__ void __launch_bounds__(RPA_SMALL_BLOCK) k_rpa_hist_reduce(rpa_params 
   if (s) aadd(R.hist + b, s);
 }
 
+// Bin layout from the histogram, on the device: bin_start (exclusive prefix
+// sums of the rows per final bin, rows < 2^31), the scatter cursors (each
+// bin's start) and, per first-level bin, its first second-pass tile.  One
+// workgroup; each thread scans a contiguous run of bins.
+constexpr int RPA_LAYOUT_BLOCK = 1024;
+
+__global__ void __launch_bounds__(RPA_LAYOUT_BLOCK) k_rpa_layout(rpa_params const* __restrict__ Rp)
+{
+  __shared__ uint32_t wave_sum[RPA_LAYOUT_BLOCK / 64];
+  rpa_params const& R = *Rp;
+  int const B         = 1 << R.log_bins;
+  int const F1        = 1 << R.log_f1;
+  int const lf2       = R.log_bins - R.log_f1;
+  int const tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  auto const hist      = gp(static_cast<uint32_t const*>(R.hist));
+  auto const bin_start = gp(R.bin_start);
+  int const per        = (B + RPA_LAYOUT_BLOCK - 1) / RPA_LAYOUT_BLOCK;
+  int const i0         = tid * per < B ? tid * per : B;
+  int const i1         = i0 + per < B ? i0 + per : B;
+  uint32_t local       = 0;
+  for (int i = i0; i < i1; i++) local += hist[i];
+  uint32_t incl = local;  // inclusive scan over the block
+  for (int o = 1; o < 64; o <<= 1) {
+    uint32_t const y = __shfl_up(incl, o, 64);
+    if (lane >= o) incl += y;
+  }
+  if (lane == 63) wave_sum[wave] = incl;
+  __syncthreads();
+  uint32_t before = 0;
+  for (int w = 0; w < wave; w++) before += wave_sum[w];
+  uint64_t run = uint64_t(before) + incl - local;
+  for (int i = i0; i < i1; i++) {
+    bin_start[i]       = int64_t(run);
+    gp(R.cursor2)[i]   = run;
+    run += hist[i];
+  }
+  if (i1 == B && i0 < B) bin_start[B] = int64_t(run);
+  __syncthreads();
+  if (tid == 0) {
+    int64_t t = 0;
+    for (int b1 = 0; b1 < F1; b1++) {
+      int64_t const lo = bin_start[int64_t(b1) << lf2], hi = bin_start[int64_t(b1 + 1) << lf2];
+      gp(R.cursor1)[b1]     = uint64_t(lo);
+      gp(R.tile_start1)[b1] = t;
+      t += (hi - lo + R.tile_rows - 1) / R.tile_rows;
+    }
+    gp(R.tile_start1)[F1] = t;
+  }
+}
+
+void launch_rpa_layout(rpa_params const* dR, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_rpa_layout, dim3(1), dim3(RPA_LAYOUT_BLOCK), 0, s, dR);
+}
+
 void launch_rpa_hist(gb_params const* dP, rpa_params const* dR, rpa_params const& hR, int64_t n,
                      int num_cus, hipStream_t s)
 {
@@ -211,7 +266,7 @@ __global__ void __launch_bounds__(RPA_SCATTER_BLOCK) __attribute__((amdgpu_waves
     for (int i = threadIdx.x; i <= F1; i += RPA_SCATTER_BLOCK)
       tstart[i] = gp(static_cast<int64_t const*>(R.tile_start1))[i];
     __syncthreads();
-    ntiles = n;  // tiles [t_begin, n) of the first-level bins
+    ntiles = n < 0 ? tstart[F1] : n;  // tiles [t_begin, n) of the first-level bins (n < 0: all)
   } else {
     ntiles = (n + T - 1) / T;
   }
@@ -364,9 +419,9 @@ void launch_rpa_scatter1(gb_params const* dP, rpa_params const* dR, rpa_params c
 void launch_rpa_scatter2(gb_params const* dP, rpa_params const* dR, rpa_params const& hR, int num_cus,
                          int64_t tile_begin, int64_t tile_end, hipStream_t s)
 {
-  if (tile_end <= tile_begin) return;
+  if (tile_end >= 0 && tile_end <= tile_begin) return;
   int64_t blocks = resident_grid(reinterpret_cast<void const*>(&k_rpa_scatter<true>), hR.stage_bytes, num_cus);
-  if (blocks > tile_end - tile_begin) blocks = tile_end - tile_begin;
+  if (tile_end >= 0 && blocks > tile_end - tile_begin) blocks = tile_end - tile_begin;
   allow_large_lds(reinterpret_cast<void const*>(&k_rpa_scatter<true>), hR.stage_bytes);
   hipLaunchKernelGGL(k_rpa_scatter<true>, dim3(blocks), dim3(RPA_SCATTER_BLOCK), hR.stage_bytes, s, dP, dR,
                      tile_end, tile_begin);
