@@ -717,7 +717,36 @@ void launch_outputs(gb_params const* dP, gb_params const& hP, uint32_t const* ma
 }
 
 // ---------------------------------------------------------------------------
-// key gather for the general table (fixed-width keys and string lengths)
+// key gather for the general table (fixed-width keys and string lengths).
+// With up to GATHER_KEYS_AT_ONCE key columns every column's validity word
+// and value (or string offsets) is loaded before the first store, so a
+// group costs one round of random reads instead of one or two per column.
+constexpr int GATHER_KEYS_AT_ONCE = 4;
+
+__device__ __forceinline__ void gather_key_column(gb_params const& P, int c, int64_t g, int64_t G, bool live,
+                                                  bool v, uint64_t x, uint64_t e, int32_t* str_sizes, int& nstr)
+{
+  dcol const& kc = P.keys[c];
+  if (kc.storage == ST_STRING) {
+    if (live) str_sizes[int64_t(nstr) * G + g] = v ? int32_t(e - x) : 0;
+    nstr++;
+  } else if (live) {
+    store_width(P.out_keys[c], g, storage_width(kc.storage), x);
+  }
+  if (P.out_key_mask[c]) store_valid_bits(P.out_key_mask[c], g, G, live && v);
+}
+
+__device__ __forceinline__ uint64_t raw_key(dcol const& kc, int64_t r)
+{
+  auto const d = gp(kc.data);
+  switch (storage_width(kc.storage)) {
+    case 1: return d[r];
+    case 2: return reinterpret_cast<gmem<uint16_t const>*>(d)[r];
+    case 4: return reinterpret_cast<gmem<uint32_t const>*>(d)[r];
+    default: return reinterpret_cast<gmem<uint64_t const>*>(d)[r];
+  }
+}
+
 __global__ void __launch_bounds__(BLOCK)
   k_gather_keys(gb_params const* __restrict__ Pp, uint32_t const* __restrict__ rep_rows, int64_t G,
                 int32_t* str_sizes)
@@ -727,28 +756,46 @@ __global__ void __launch_bounds__(BLOCK)
   bool const live    = g < G;
   int64_t const r    = live ? int64_t(rep_rows[g]) : 0;
   int nstr           = 0;
+  if (P.num_keys <= GATHER_KEYS_AT_ONCE) {
+    bool v[GATHER_KEYS_AT_ONCE];
+    uint64_t x[GATHER_KEYS_AT_ONCE], e[GATHER_KEYS_AT_ONCE];  // value, or string begin / end
+#pragma unroll
+    for (int c = 0; c < GATHER_KEYS_AT_ONCE; c++) {
+      v[c] = false;
+      x[c] = e[c] = 0;
+      if (c >= P.num_keys) continue;
+      dcol const& kc = P.keys[c];
+      v[c]           = live && dcol_valid(kc, r);
+      if (!live) continue;
+      if (kc.storage == ST_STRING) {
+        int64_t b0, e0;
+        str_span(kc, r, b0, e0);
+        x[c] = uint64_t(b0);
+        e[c] = uint64_t(e0);
+      } else {
+        x[c] = raw_key(kc, r);
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < GATHER_KEYS_AT_ONCE; c++)
+      if (c < P.num_keys) gather_key_column(P, c, g, G, live, v[c], x[c], e[c], str_sizes, nstr);
+    return;
+  }
   for (int c = 0; c < P.num_keys; c++) {
     dcol const& kc = P.keys[c];
     bool const v   = live && dcol_valid(kc, r);
-    if (kc.storage == ST_STRING) {
-      if (live) {
-        int64_t b = 0, e = 0;
-        if (v) str_span(kc, r, b, e);
-        str_sizes[int64_t(nstr) * G + g] = int32_t(e - b);
+    uint64_t x = 0, e = 0;
+    if (live) {
+      if (kc.storage == ST_STRING) {
+        int64_t b0, e0;
+        str_span(kc, r, b0, e0);
+        x = uint64_t(b0);
+        e = uint64_t(e0);
+      } else {
+        x = raw_key(kc, r);
       }
-      nstr++;
-    } else if (live) {
-      int const w = storage_width(kc.storage);
-      uint64_t bits;
-      switch (w) {
-        case 1: bits = kc.data[r]; break;
-        case 2: bits = reinterpret_cast<uint16_t const*>(kc.data)[r]; break;
-        case 4: bits = reinterpret_cast<uint32_t const*>(kc.data)[r]; break;
-        default: bits = reinterpret_cast<uint64_t const*>(kc.data)[r]; break;
-      }
-      store_width(P.out_keys[c], g, w, bits);
     }
-    if (P.out_key_mask[c]) store_valid_bits(P.out_key_mask[c], g, G, v);
+    gather_key_column(P, c, g, G, live, v, x, e, str_sizes, nstr);
   }
 }
 
@@ -831,7 +878,7 @@ __global__ void __launch_bounds__(BLOCK)
   if (len == 0) return;
   int64_t b, e;
   str_span(src, int64_t(rep_rows[g]), b, e);
-  for (int64_t i = lane; i < len; i += GATHER_LANES) out_chars[o + i] = src.data[b + i];
+  for (int64_t i = lane; i < len; i += GATHER_LANES) gp(out_chars)[o + i] = gp(src.data)[b + i];
 }
 
 void launch_gather_string_chars(dcol const& src, uint32_t const* rep_rows, int64_t G,

@@ -114,16 +114,26 @@ __global__ void __launch_bounds__(RPA_LAYOUT_BLOCK) k_rpa_layout(rpa_params cons
   }
   if (i1 == B && i0 < B) bin_start[B] = int64_t(run);
   __syncthreads();
-  if (tid == 0) {
-    int64_t t = 0;
-    for (int b1 = 0; b1 < F1; b1++) {
-      int64_t const lo = bin_start[int64_t(b1) << lf2], hi = bin_start[int64_t(b1 + 1) << lf2];
-      gp(R.cursor1)[b1]     = uint64_t(lo);
-      gp(R.tile_start1)[b1] = t;
-      t += (hi - lo + R.tile_rows - 1) / R.tile_rows;
-    }
-    gp(R.tile_start1)[F1] = t;
+  // first-level bins (F1 <= 256): thread b1 takes bin b1, a block scan
+  // numbers the second-pass tiles
+  uint32_t tiles = 0;
+  if (tid < F1) {
+    int64_t const lo = bin_start[int64_t(tid) << lf2], hi = bin_start[int64_t(tid + 1) << lf2];
+    gp(R.cursor1)[tid] = uint64_t(lo);
+    tiles              = uint32_t((hi - lo + R.tile_rows - 1) / R.tile_rows);
   }
+  uint32_t tincl = tiles;
+  for (int o = 1; o < 64; o <<= 1) {
+    uint32_t const y = __shfl_up(tincl, o, 64);
+    if (lane >= o) tincl += y;
+  }
+  __syncthreads();  // wave_sum is reused
+  if (lane == 63) wave_sum[wave] = tincl;
+  __syncthreads();
+  uint32_t tbefore = 0;
+  for (int w = 0; w < wave; w++) tbefore += wave_sum[w];
+  if (tid < F1) gp(R.tile_start1)[tid] = int64_t(tbefore + tincl - tiles);
+  if (tid == F1 - 1) gp(R.tile_start1)[F1] = int64_t(tbefore + tincl);
 }
 
 void launch_rpa_layout(rpa_params const* dR, hipStream_t s)
