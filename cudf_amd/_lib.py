@@ -12,7 +12,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "lib", "libmig_groupby.so")
 
 # status codes (mig_status)
-MIG_OK, MIG_ERROR_LOGIC, MIG_ERROR_HIP, MIG_ERROR_DATA_TYPE, MIG_ERROR_ALLOC, MIG_ERROR_OVERFLOW = range(6)
+(MIG_OK, MIG_ERROR_LOGIC, MIG_ERROR_HIP, MIG_ERROR_DATA_TYPE, MIG_ERROR_ALLOC, MIG_ERROR_OVERFLOW,
+ MIG_ERROR_OUT_OF_RANGE) = range(7)
 
 
 class LogicError(RuntimeError):
@@ -36,7 +37,8 @@ _ERRORS = {
     MIG_ERROR_HIP: HipError,
     MIG_ERROR_DATA_TYPE: DataTypeError,
     MIG_ERROR_ALLOC: AllocationError,
-    MIG_ERROR_OVERFLOW: LogicError,
+    MIG_ERROR_OVERFLOW: OverflowError,
+    MIG_ERROR_OUT_OF_RANGE: IndexError,
 }
 
 
@@ -142,6 +144,22 @@ SIGNATURES = {
     "mig_batch_result_column": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32,
                                                ctypes.c_int32, ctypes.POINTER(OutColumn)]),
     "mig_batch_plan_free": (None, [ctypes.c_void_p]),
+    "mig_table_num_rows": (ctypes.c_int32, [ctypes.c_void_p]),
+    "mig_table_num_columns": (ctypes.c_int32, [ctypes.c_void_p]),
+    "mig_table_column": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(OutColumn)]),
+    "mig_table_release_column": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(OutColumn)]),
+    "mig_table_free": (None, [ctypes.c_void_p]),
+    "mig_hash_partition": (ctypes.c_int, [ctypes.POINTER(ColumnView), ctypes.c_int32,
+                                          ctypes.POINTER(ctypes.c_int32), ctypes.c_int32, ctypes.c_int32,
+                                          ctypes.c_int32, ctypes.c_uint32, ctypes.c_void_p,
+                                          ctypes.POINTER(MemoryResource), ctypes.POINTER(ctypes.c_void_p),
+                                          ctypes.POINTER(ctypes.c_int32)]),
+    "mig_partition": (ctypes.c_int, [ctypes.POINTER(ColumnView), ctypes.c_int32, ctypes.POINTER(ColumnView),
+                                     ctypes.c_int32, ctypes.c_void_p, ctypes.POINTER(MemoryResource),
+                                     ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_int32)]),
+    "mig_concatenate": (ctypes.c_int, [ctypes.POINTER(ColumnView), ctypes.c_int32, ctypes.c_int32,
+                                       ctypes.c_void_p, ctypes.POINTER(MemoryResource),
+                                       ctypes.POINTER(ctypes.c_void_p)]),
 }
 
 STAT_NAMES = ["strategy", "key_mode", "capacity", "est_groups", "groups", "retries", "state_bytes",

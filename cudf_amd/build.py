@@ -31,7 +31,7 @@ COMMON_FLAGS = [
 ]
 
 SOURCES = ["groupby_kernels.hip", "partition_kernels.hip", "partition_aggregate.hip", "batch_kernels.hip",
-           "groupby.cpp", "batch.cpp", "capi.cpp"]
+           "partitioning_kernels.hip", "groupby.cpp", "batch.cpp", "partitioning.cpp", "capi.cpp"]
 
 
 def _compile(src: str) -> str:

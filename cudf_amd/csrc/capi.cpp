@@ -9,6 +9,7 @@
 
 #include "batch_host.hpp"
 #include "groupby.hpp"
+#include "partitioning.hpp"
 
 namespace {
 
@@ -65,6 +66,13 @@ mig::owned_column* key_col(mig_groupby_result const* r, int32_t i)
   if (!r) mig::fail(MIG_ERROR_LOGIC, "null result handle");
   if (i < 0 || i >= int32_t(r->keys.size())) mig::fail(MIG_ERROR_LOGIC, "key index out of range");
   return &const_cast<mig_groupby_result*>(r)->keys[i];
+}
+
+mig::owned_column* table_col(mig_table const* t, int32_t i)
+{
+  if (!t) mig::fail(MIG_ERROR_LOGIC, "null table handle");
+  if (i < 0 || i >= int32_t(t->columns.size())) mig::fail(MIG_ERROR_OUT_OF_RANGE, "column index out of range");
+  return &const_cast<mig_table*>(t)->columns[i];
 }
 
 }  // namespace
@@ -226,5 +234,73 @@ mig_status mig_batch_result_column(const mig_batch_plan* plan, int32_t instance,
 }
 
 void mig_batch_plan_free(mig_batch_plan* plan) { mig::batch_plan_free(plan); }
+
+int32_t mig_table_num_rows(const mig_table* t) { return t ? t->num_rows : 0; }
+
+int32_t mig_table_num_columns(const mig_table* t) { return t ? int32_t(t->columns.size()) : 0; }
+
+mig_status mig_table_column(const mig_table* t, int32_t i, mig_column* out)
+{
+  return guarded([&] {
+    if (!out) mig::fail(MIG_ERROR_LOGIC, "null output column");
+    *out = exported(*table_col(t, i));
+  });
+}
+
+mig_status mig_table_release_column(mig_table* t, int32_t i, mig_column* out)
+{
+  return guarded([&] {
+    if (!out) mig::fail(MIG_ERROR_LOGIC, "null output column");
+    auto* c = table_col(t, i);
+    *out    = exported(*c);
+    *c      = mig::owned_column{};
+  });
+}
+
+void mig_table_free(mig_table* t) { delete t; }
+
+mig_status mig_hash_partition(const mig_column_view* input, int32_t num_columns, const int32_t* columns_to_hash,
+                              int32_t num_columns_to_hash, int32_t num_partitions, int32_t hash_function,
+                              uint32_t seed, void* stream, const mig_memory_resource* mr, mig_table** out,
+                              int32_t* partition_offsets)
+{
+  return guarded([&] {
+    if (!out) mig::fail(MIG_ERROR_LOGIC, "null output handle");
+    *out = nullptr;
+    if (num_partitions > 0 && !partition_offsets) mig::fail(MIG_ERROR_LOGIC, "null offsets array");
+    std::vector<int32_t> offsets;
+    auto t = mig::hash_partition(input, num_columns, columns_to_hash, num_columns_to_hash, num_partitions,
+                                 hash_function, seed, static_cast<hipStream_t>(stream), mr, offsets);
+    if (!offsets.empty()) std::memcpy(partition_offsets, offsets.data(), sizeof(int32_t) * offsets.size());
+    *out = t.release();
+  });
+}
+
+mig_status mig_partition(const mig_column_view* input, int32_t num_columns, const mig_column_view* partition_map,
+                         int32_t num_partitions, void* stream, const mig_memory_resource* mr, mig_table** out,
+                         int32_t* partition_offsets)
+{
+  return guarded([&] {
+    if (!out) mig::fail(MIG_ERROR_LOGIC, "null output handle");
+    *out = nullptr;
+    if (!partition_map) mig::fail(MIG_ERROR_LOGIC, "null partition map");
+    if (!partition_offsets) mig::fail(MIG_ERROR_LOGIC, "null offsets array");
+    std::vector<int32_t> offsets;
+    auto t = mig::partition_by_map(input, num_columns, *partition_map, num_partitions,
+                                   static_cast<hipStream_t>(stream), mr, offsets);
+    std::memcpy(partition_offsets, offsets.data(), sizeof(int32_t) * offsets.size());
+    *out = t.release();
+  });
+}
+
+mig_status mig_concatenate(const mig_column_view* tables, int32_t num_tables, int32_t num_columns, void* stream,
+                           const mig_memory_resource* mr, mig_table** out)
+{
+  return guarded([&] {
+    if (!out) mig::fail(MIG_ERROR_LOGIC, "null output handle");
+    *out = nullptr;
+    *out = mig::concatenate(tables, num_tables, num_columns, static_cast<hipStream_t>(stream), mr).release();
+  });
+}
 
 }  // extern "C"

@@ -37,7 +37,9 @@ typedef enum mig_status {
   MIG_ERROR_HIP       = 2, /* cudf::cuda_error equivalent: a HIP call failed  */
   MIG_ERROR_DATA_TYPE = 3, /* cudf::data_type_error: unsupported type combo   */
   MIG_ERROR_ALLOC     = 4, /* device allocation failed                        */
-  MIG_ERROR_OVERFLOW  = 5  /* a fixed-capacity buffer would be exceeded       */
+  MIG_ERROR_OVERFLOW  = 5, /* a fixed-capacity buffer would be exceeded, or a
+                              result would exceed 2^31-1 rows (std::overflow_error) */
+  MIG_ERROR_OUT_OF_RANGE = 6 /* std::out_of_range: a column index outside the table */
 } mig_status;
 
 /* ---- type ids: same numbering as cudf::type_id (cudf/types.hpp:203-235) -- */
@@ -292,6 +294,76 @@ int32_t mig_last_stats(double* values, int32_t n);
  * Instrumentation only, with no counterpart in the reference (which marks
  * the call with an NVTX range, cpp/src/groupby/groupby.cu:209). */
 void mig_set_phase_timing(int32_t enable);
+
+/* ---- partitioning and concatenation ----------------------------------------
+ * The building blocks of a multi-device group-by: each device aggregates its
+ * rows, hash-partitions the partial results by key, exchanges partitions, and
+ * concatenates what it receives before aggregating again (dask_cudf's
+ * shuffle, python/dask_cudf/dask_cudf/groupby.py:367-432). */
+
+/* cudf::hash_id (cpp/include/cudf/partitioning.hpp:41-44). */
+enum { MIG_HASH_IDENTITY = 0, MIG_HASH_MURMUR3 = 1 };
+
+/* Columns owned by a handle (a cudf::table). */
+typedef struct mig_table mig_table;
+int32_t mig_table_num_rows(const mig_table* t);
+int32_t mig_table_num_columns(const mig_table* t);
+/* Borrow column `i`. */
+mig_status mig_table_column(const mig_table* t, int32_t i, mig_column* out);
+/* Take ownership of column `i` (freed with the resource it came from). */
+mig_status mig_table_release_column(mig_table* t, int32_t i, mig_column* out);
+void mig_table_free(mig_table* t);
+
+/* Replaces cudf::hash_partition(input, columns_to_hash, num_partitions,
+ * hash_function, seed, stream, mr) (cpp/include/cudf/partitioning.hpp:105-112;
+ * src/partitioning/partitioning.cu:759-830).  Row r goes to partition
+ * row_hash(r) % num_partitions, where row_hash combines the MurmurHash3_x86_32
+ * (or IdentityHash) of the hashed columns from `seed` with hash_combine, as
+ * mig_hash_murmurhash3_x86_32 does.  `*out` holds the rows regrouped so that
+ * each partition is contiguous; inside a partition rows keep their input
+ * order.  `partition_offsets` (num_partitions entries, host memory) receives
+ * the first row of each partition.  Errors: MIG_ERROR_OUT_OF_RANGE for a bad
+ * column index; MIG_ERROR_LOGIC for an unknown hash function or IdentityHash
+ * on a non-numeric column.  No partitions, rows or hashed columns give empty
+ * columns of the input's types and zero offsets. */
+mig_status mig_hash_partition(const mig_column_view* input,
+                              int32_t num_columns,
+                              const int32_t* columns_to_hash,
+                              int32_t num_columns_to_hash,
+                              int32_t num_partitions,
+                              int32_t hash_function,
+                              uint32_t seed,
+                              void* stream,
+                              const mig_memory_resource* mr,
+                              mig_table** out,
+                              int32_t* partition_offsets);
+
+/* Replaces cudf::partition(t, partition_map, num_partitions, mr)
+ * (cpp/include/cudf/partitioning.hpp:79-83; partitioning.cu:782-802): row r
+ * goes to partition partition_map[r] (a non-null integer column).
+ * `partition_offsets` receives num_partitions + 1 entries, the last being the
+ * row count.  A map value outside [0, num_partitions) is an error. */
+mig_status mig_partition(const mig_column_view* input,
+                         int32_t num_columns,
+                         const mig_column_view* partition_map,
+                         int32_t num_partitions,
+                         void* stream,
+                         const mig_memory_resource* mr,
+                         mig_table** out,
+                         int32_t* partition_offsets);
+
+/* Replaces cudf::concatenate(tables_to_concat, stream, mr)
+ * (cpp/include/cudf/concatenate.hpp:95-98; src/copying/concatenate.cu):
+ * `tables` holds num_tables * num_columns views, table-major
+ * (tables[t * num_columns + c]).  Column types must match across tables
+ * (MIG_ERROR_DATA_TYPE); more than 2^31-1 rows is MIG_ERROR_OVERFLOW.  An
+ * output column has a validity mask when some input piece has nulls. */
+mig_status mig_concatenate(const mig_column_view* tables,
+                           int32_t num_tables,
+                           int32_t num_columns,
+                           void* stream,
+                           const mig_memory_resource* mr,
+                           mig_table** out);
 
 /* Aggregation strategies.  AUTO picks per call from the key layout and a
  * sampled estimate of the number of groups; the others force one (used by
