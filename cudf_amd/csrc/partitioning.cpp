@@ -101,26 +101,73 @@ std::unique_ptr<mig_table> empty_like(mig_column_view const* input, int32_t num_
 
 // Regroups every column of `input` by the partition numbers in `pid` (one
 // per row, below num_partitions) with a stable LSD counting sort; writes the
-// first row of each partition to `offsets`.
+// first row of each partition to `offsets`.  The last counting pass moves up
+// to HP_MAX_MOVE_COLS fixed-width columns itself (rows read in order, values
+// written in runs per partition); validity, strings and further columns are
+// gathered afterwards through the row map.
 std::unique_ptr<mig_table> regroup(mig_column_view const* input, int32_t num_columns, int64_t n, uint32_t* pid,
                                    uint32_t num_partitions, call_scratch& S, hipStream_t stream,
                                    mig_memory_resource const* mr, std::vector<int32_t>& offsets)
 {
+  auto t = std::make_unique<mig_table>();
+  init_table(*t, stream, mr);
+  t->num_rows = int32_t(n);
+  t->columns.resize(num_columns);
+  std::vector<hp_move_col> moves;
+  std::vector<int32_t> gathered;  // fixed-width columns beyond the moved ones
+  std::vector<int32_t> str_cols;
+  bool need_map = false;
+  for (int32_t c = 0; c < num_columns; c++) {
+    mig_column_view const& v = input[c];
+    owned_column& oc         = t->columns[c];
+    oc.col.type_id           = v.type_id;
+    oc.col.size              = int32_t(n);
+    if (v.null_mask) {
+      oc.mask_bytes    = mask_bytes_for(n);
+      oc.col.null_mask = static_cast<uint32_t*>(t->alloc.allocate(oc.mask_bytes));
+      need_map         = true;
+    }
+    if (v.type_id == MIG_TYPE_STRING) {
+      oc.col.offsets_type_id = MIG_TYPE_INT32;
+      oc.offsets_bytes       = sizeof(int32_t) * size_t(n + 1);
+      oc.col.offsets         = t->alloc.allocate(oc.offsets_bytes);
+      str_cols.push_back(c);
+      need_map = true;
+      continue;
+    }
+    int32_t const w = element_width(v.type_id);
+    oc.data_bytes   = size_t(w) * size_t(n);
+    oc.col.data     = t->alloc.allocate(oc.data_bytes);
+    if (int(moves.size()) < HP_MAX_MOVE_COLS) {
+      moves.push_back(hp_move_col{static_cast<uint8_t const*>(v.data) + int64_t(w) * v.offset, oc.col.data, w, 0});
+    } else {
+      gathered.push_back(c);
+      need_map = true;
+    }
+  }
+  hp_move_col* d_moves = S.get<hp_move_col>(std::max<size_t>(1, moves.size()));
+  if (!moves.empty())
+    MIG_HIP_TRY(hipMemcpyAsync(d_moves, moves.data(), sizeof(hp_move_col) * moves.size(), hipMemcpyHostToDevice,
+                               stream));
+
   int total_bits = 1;
   while (total_bits < 32 && (uint64_t(1) << total_bits) < num_partitions) total_bits++;
-  int const passes = (total_bits + HP_DIGIT_BITS - 1) / HP_DIGIT_BITS;
+  int const passes        = (total_bits + HP_DIGIT_BITS - 1) / HP_DIGIT_BITS;
   int const bits_per_pass = (total_bits + passes - 1) / passes;
 
-  uint32_t* pid_alt = S.get<uint32_t>(n);
-  uint32_t* rows_a  = S.get<uint32_t>(n);
-  uint32_t* rows_b  = S.get<uint32_t>(n);
-  uint32_t* pid_cur = pid;
-  uint32_t* rows_cur = nullptr;  // identity before the first pass
-  uint32_t* pid_next = pid_alt;
+  uint32_t* pid_alt   = passes > 1 ? S.get<uint32_t>(n) : nullptr;
+  uint32_t* rows_a    = (passes > 1 || need_map) ? S.get<uint32_t>(n) : nullptr;
+  uint32_t* rows_b    = passes > 1 ? S.get<uint32_t>(n) : nullptr;
+  uint32_t* pid_cur   = pid;
+  uint32_t* rows_cur  = nullptr;  // identity before the first pass
+  uint32_t* pid_next  = pid_alt;
   uint32_t* rows_next = rows_a;
-  int shift = 0;
+  int shift           = 0;
+  uint32_t const* last_hist = nullptr;
+  int64_t last_W            = 0;
   for (int p = 0; p < passes; p++) {
-    int const bits = std::min(bits_per_pass, total_bits - shift);
+    bool const last  = p == passes - 1;
+    int const bits   = std::min(bits_per_pass, total_bits - shift);
     int64_t const nd = int64_t(1) << bits;
     // waves: enough to fill the device, few enough that the digit-major
     // histogram stays within 2^23 counters
@@ -132,8 +179,15 @@ std::unique_ptr<mig_table> regroup(mig_column_view const* input, int32_t num_col
     launch_hp_count(pid_cur, n, per, W, shift, bits, hist, stream);
     launch_exclusive_scan_i32(reinterpret_cast<int32_t*>(hist), nd * W, reinterpret_cast<int32_t*>(hist) + nd * W,
                               stream);
-    launch_hp_scatter(pid_cur, rows_cur, n, per, W, shift, bits, hist, pid_next, rows_next, stream);
+    // partition numbers go out for a later pass or for the offsets of a
+    // multi-pass sort (a single pass reads the offsets off its histogram);
+    // the row map for a later pass or for the gathers
+    launch_hp_scatter(pid_cur, rows_cur, n, per, W, shift, bits, hist, passes == 1 ? nullptr : pid_next,
+                      (!last || need_map) ? rows_next : nullptr, last ? d_moves : nullptr,
+                      last ? int(moves.size()) : 0, stream);
     MIG_HIP_TRY(hipGetLastError());
+    last_hist = hist;
+    last_W    = W;
     std::swap(pid_cur, pid_next);
     rows_cur  = rows_next;
     rows_next = rows_cur == rows_a ? rows_b : rows_a;
@@ -142,42 +196,28 @@ std::unique_ptr<mig_table> regroup(mig_column_view const* input, int32_t num_col
   uint32_t const* map = rows_cur;
 
   int32_t* d_offsets = S.get<int32_t>(num_partitions);
-  launch_hp_offsets(pid_cur, n, num_partitions, d_offsets, stream);
+  if (passes == 1) launch_hp_offsets_from_hist(last_hist, last_W, num_partitions, d_offsets, stream);
+  else launch_hp_offsets(pid_cur, n, num_partitions, d_offsets, stream);
 
-  auto t = std::make_unique<mig_table>();
-  init_table(*t, stream, mr);
-  t->num_rows = int32_t(n);
-  t->columns.resize(num_columns);
   uint32_t* d_nulls = S.get<uint32_t>(std::max(1, num_columns));
   MIG_HIP_TRY(hipMemsetAsync(d_nulls, 0, sizeof(uint32_t) * std::max(1, num_columns), stream));
-  std::vector<int32_t> str_cols;
   for (int32_t c = 0; c < num_columns; c++) {
     mig_column_view const& v = input[c];
     owned_column& oc         = t->columns[c];
-    oc.col.type_id           = v.type_id;
-    oc.col.size              = int32_t(n);
-    if (v.null_mask) {
-      oc.mask_bytes    = mask_bytes_for(n);
-      oc.col.null_mask = static_cast<uint32_t*>(t->alloc.allocate(oc.mask_bytes));
-      launch_gather_mask(v.null_mask, v.offset, map, n, oc.col.null_mask, d_nulls + c, stream);
-    }
-    if (v.type_id == MIG_TYPE_STRING) {
-      oc.col.offsets_type_id = MIG_TYPE_INT32;
-      oc.offsets_bytes       = sizeof(int32_t) * size_t(n + 1);
-      oc.col.offsets         = t->alloc.allocate(oc.offsets_bytes);
-      auto* offs             = static_cast<int32_t*>(oc.col.offsets);
-      launch_gather_str_sizes(make_dcol(v, false), map, n, offs, stream);
-      launch_exclusive_scan_i32(offs, n, offs + n, stream);
-      str_cols.push_back(c);
-    } else {
-      int32_t const w = element_width(v.type_id);
-      oc.data_bytes   = size_t(w) * size_t(n);
-      oc.col.data     = t->alloc.allocate(oc.data_bytes);
-      launch_gather_fixed(static_cast<uint8_t const*>(v.data) + int64_t(w) * v.offset, w, map, n, oc.col.data,
-                          stream);
-    }
-    MIG_HIP_TRY(hipGetLastError());
+    if (v.null_mask) launch_gather_mask(v.null_mask, v.offset, map, n, oc.col.null_mask, d_nulls + c, stream);
   }
+  for (int32_t c : gathered) {
+    mig_column_view const& v = input[c];
+    int32_t const w          = element_width(v.type_id);
+    launch_gather_fixed(static_cast<uint8_t const*>(v.data) + int64_t(w) * v.offset, w, map, n,
+                        t->columns[c].col.data, stream);
+  }
+  for (int32_t c : str_cols) {
+    auto* offs = static_cast<int32_t*>(t->columns[c].col.offsets);
+    launch_gather_str_sizes(make_dcol(input[c], false), map, n, offs, stream);
+    launch_exclusive_scan_i32(offs, n, offs + n, stream);
+  }
+  MIG_HIP_TRY(hipGetLastError());
   // chars of the string columns (sizes known once the scans are done)
   std::vector<int32_t> totals(str_cols.size());
   for (size_t k = 0; k < str_cols.size(); k++) {
@@ -196,9 +236,8 @@ std::unique_ptr<mig_table> regroup(mig_column_view const* input, int32_t num_col
     MIG_EXPECTS(totals[k] >= 0, "partitioned string column exceeds 2^31 characters");
     oc.data_bytes = size_t(totals[k]);
     oc.col.data   = t->alloc.allocate(oc.data_bytes);
-    if (n > 0)
-      launch_gather_string_chars(make_dcol(v, false), map, n, static_cast<int32_t const*>(oc.col.offsets),
-                                 static_cast<uint8_t*>(oc.col.data), stream);
+    launch_gather_string_chars(make_dcol(v, false), map, n, static_cast<int32_t const*>(oc.col.offsets),
+                               static_cast<uint8_t*>(oc.col.data), stream);
   }
   for (int32_t c = 0; c < num_columns; c++)
     if (t->columns[c].col.null_mask) t->columns[c].col.null_count = int32_t(nulls[c]);

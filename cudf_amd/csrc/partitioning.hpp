@@ -39,11 +39,23 @@ void launch_map_partition_ids(void const* map, int32_t map_storage, int64_t n, u
                               uint32_t* pid, uint32_t* bad, hipStream_t s);
 void launch_hp_count(uint32_t const* pid, int64_t n, int64_t per, int64_t W, int shift, int bits, uint32_t* hist,
                      hipStream_t s);
+// Fixed-width column moved by the last counting pass: dst[pos] = src[row].
+struct hp_move_col {
+  const void* src;  // element 0 of the view
+  void* dst;
+  int32_t width;    // 1, 2, 4, 8 or 16 bytes
+  int32_t pad;
+};
+constexpr int HP_MAX_MOVE_COLS = 16;
+
+// pid_out / rows_out may be null when nothing reads them later.
 void launch_hp_scatter(uint32_t const* pid_in, uint32_t const* rows_in, int64_t n, int64_t per, int64_t W,
                        int shift, int bits, uint32_t const* start, uint32_t* pid_out, uint32_t* rows_out,
-                       hipStream_t s);
+                       hp_move_col const* move, int nmove, hipStream_t s);
 void launch_hp_offsets(uint32_t const* pid_sorted, int64_t n, uint32_t num_partitions, int32_t* offsets,
                        hipStream_t s);
+void launch_hp_offsets_from_hist(uint32_t const* start, int64_t W, uint32_t num_partitions, int32_t* offsets,
+                                 hipStream_t s);
 void launch_gather_fixed(void const* src, int width, uint32_t const* map, int64_t n, void* dst, hipStream_t s);
 void launch_gather_mask(uint32_t const* mask, int64_t mask_offset, uint32_t const* map, int64_t n, uint32_t* out,
                         uint32_t* null_count, hipStream_t s);

@@ -126,7 +126,7 @@ __global__ void __launch_bounds__(HP_BLOCK)
 __global__ void __launch_bounds__(HP_BLOCK)
   k_hp_scatter(uint32_t const* __restrict__ pid_in, uint32_t const* __restrict__ rows_in, int64_t n, int64_t per,
                int64_t W, int shift, int bits, uint32_t const* __restrict__ start, uint32_t* __restrict__ pid_out,
-               uint32_t* __restrict__ rows_out)
+               uint32_t* __restrict__ rows_out, hp_move_col const* __restrict__ move, int nmove)
 {
   extern __shared__ uint32_t hp_lds[];
   int const wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -157,8 +157,24 @@ __global__ void __launch_bounds__(HP_BLOCK)
     __builtin_amdgcn_wave_barrier();
     if (valid) {
       uint32_t const pos = base + uint32_t(__popcll(peers & below));
-      gp(pid_out)[pos]  = p;
-      gp(rows_out)[pos] = row;
+      if (pid_out) gp(pid_out)[pos] = p;  // needed by a later pass or the offsets kernel only
+      if (rows_out) gp(rows_out)[pos] = row;
+      // the last pass moves fixed-width values straight to their place
+      for (int c = 0; c < nmove; c++) {
+        hp_move_col const mc = move[c];
+        switch (mc.width) {
+          case 1: gp(static_cast<uint8_t*>(mc.dst))[pos] = gp(static_cast<uint8_t const*>(mc.src))[row]; break;
+          case 2: gp(static_cast<uint16_t*>(mc.dst))[pos] = gp(static_cast<uint16_t const*>(mc.src))[row]; break;
+          case 4: gp(static_cast<uint32_t*>(mc.dst))[pos] = gp(static_cast<uint32_t const*>(mc.src))[row]; break;
+          case 8: gp(static_cast<uint64_t*>(mc.dst))[pos] = gp(static_cast<uint64_t const*>(mc.src))[row]; break;
+          default: {
+            uint64_t const lo = gp(static_cast<uint64_t const*>(mc.src))[2 * int64_t(row)];
+            uint64_t const hi = gp(static_cast<uint64_t const*>(mc.src))[2 * int64_t(row) + 1];
+            gp(static_cast<uint64_t*>(mc.dst))[2 * int64_t(pos)]     = lo;
+            gp(static_cast<uint64_t*>(mc.dst))[2 * int64_t(pos) + 1] = hi;
+          }
+        }
+      }
       if ((peers & below) == 0) next[d] = base + uint32_t(__popcll(peers));
     }
     __builtin_amdgcn_wave_barrier();
@@ -175,6 +191,16 @@ __global__ void __launch_bounds__(HP_BLOCK)
   int64_t const prev = i == 0 ? -1 : int64_t(gp(pid_sorted)[i - 1]);
   int64_t const cur  = i == n ? int64_t(num_partitions) - 1 : int64_t(gp(pid_sorted)[i]);
   for (int64_t q = prev + 1; q <= cur; q++) gp(offsets)[q] = int32_t(i);
+}
+
+// One counting pass: the first row of partition q is the scanned histogram
+// entry of digit q for wave 0.
+__global__ void __launch_bounds__(HP_BLOCK)
+  k_hp_offsets_from_hist(uint32_t const* __restrict__ start, int64_t W, uint32_t num_partitions,
+                         int32_t* __restrict__ offsets)
+{
+  int64_t const q = int64_t(blockIdx.x) * HP_BLOCK + threadIdx.x;
+  if (q < num_partitions) gp(offsets)[q] = int32_t(gp(start)[q * W]);
 }
 
 template <typename T>
@@ -341,12 +367,12 @@ void launch_hp_count(uint32_t const* pid, int64_t n, int64_t per, int64_t W, int
 
 void launch_hp_scatter(uint32_t const* pid_in, uint32_t const* rows_in, int64_t n, int64_t per, int64_t W,
                        int shift, int bits, uint32_t const* start, uint32_t* pid_out, uint32_t* rows_out,
-                       hipStream_t s)
+                       hp_move_col const* move, int nmove, hipStream_t s)
 {
   size_t const lds = size_t(HP_WAVES) * (size_t(1) << bits) * sizeof(uint32_t);
   unsigned const blocks = unsigned((W + HP_WAVES - 1) / HP_WAVES);
   hipLaunchKernelGGL(k_hp_scatter, dim3(blocks), dim3(HP_BLOCK), lds, s, pid_in, rows_in, n, per, W, shift,
-                     bits, start, pid_out, rows_out);
+                     bits, start, pid_out, rows_out, move, nmove);
 }
 
 void launch_hp_offsets(uint32_t const* pid_sorted, int64_t n, uint32_t num_partitions, int32_t* offsets,
@@ -354,6 +380,13 @@ void launch_hp_offsets(uint32_t const* pid_sorted, int64_t n, uint32_t num_parti
 {
   hipLaunchKernelGGL(k_hp_offsets, dim3(grid_of(n + 1)), dim3(HP_BLOCK), 0, s, pid_sorted, n, num_partitions,
                      offsets);
+}
+
+void launch_hp_offsets_from_hist(uint32_t const* start, int64_t W, uint32_t num_partitions, int32_t* offsets,
+                                 hipStream_t s)
+{
+  hipLaunchKernelGGL(k_hp_offsets_from_hist, dim3(grid_of(num_partitions)), dim3(HP_BLOCK), 0, s, start, W,
+                     num_partitions, offsets);
 }
 
 void launch_gather_fixed(void const* src, int width, uint32_t const* map, int64_t n, void* dst, hipStream_t s)

@@ -95,6 +95,17 @@ def test_fixed_width(hash_function, dtype, shape):
     _check_hash_partition(cols, list(range(ncols)), nparts, hash_function)
 
 
+@pytest.mark.parametrize("nparts", [6, 3000])
+def test_many_columns(nparts):
+    # the last counting pass moves up to 16 fixed-width columns; the rest are gathered
+    rng = np.random.default_rng(nparts)
+    n = 4000
+    dts = [np.int8, np.int16, np.int32, np.int64, np.float32, np.float64]
+    cols = [host_col(rng.integers(-100, 100, n).astype(dts[i % len(dts)]), (rng.random(n) > 0.3) if i % 5 == 0
+                     else None) for i in range(20)]
+    _check_hash_partition(cols, [0, 3, 19], nparts)
+
+
 def test_nullable_without_nulls_and_strings_with_nulls():
     strs = orc.HostColumn.from_strings([b"a", b"bb", b"ccc", b"d"], np.ones(4, bool))
     out, _ = _check_hash_partition([strs], [0], 3)
