@@ -354,6 +354,54 @@ def bench_c3(n=50_000_000, steps=3):
             "last_call": {kk: v for kk, v in st.items()}}
 
 
+def bench_partition(n=100_000_000, card=10_000_000, reps=5, shards=8):
+    """Hash partitioning of a 100M-row table (int64 key over 10M values, float64
+    value) into 8 and 256 partitions; then the C2 group-by run the way a
+    multi-device group-by would (8 row shards aggregated separately, partials
+    hash-partitioned by key into 8 parts and combined per part), on one device."""
+    import torch
+    import cudf_amd as ca
+    from cudf_amd.dask_groupby import groupby_agg
+    from cudf_amd.partitioning import concatenate, hash_partition, split
+    g = torch.Generator(device="cuda").manual_seed(1)
+    keys = torch.randint(0, card, (n,), device="cuda", dtype=torch.int64, generator=g)
+    vals = torch.rand(n, device="cuda", dtype=torch.float64, generator=g)
+    t = ca.Table([ca.Column(ca.TypeId.INT64, n, keys.view(torch.uint8)),
+                  ca.Column(ca.TypeId.FLOAT64, n, vals.view(torch.uint8))])
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            r = fn()
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t0)
+            del r
+        return sorted(ts)[len(ts) // 2] * 1e3
+
+    out = {"workload": f"{n} rows: int64 key over {card} values, float64 value (16 B/row)"}
+    for nparts in (8, 256):
+        ms = timed(lambda: hash_partition(t, [0], nparts))
+        out[f"hash_partition_{nparts}"] = {"ms": ms, "rows_per_s": n / (ms * 1e-3),
+                                           "alg_GBps": n * 32 / (ms * 1e-3) / 1e9}
+    part, offs = hash_partition(t, [0], 8)
+    pieces = split(part, offs)
+    ms = timed(lambda: concatenate(pieces))
+    out["concatenate_8"] = {"ms": ms, "alg_GBps": n * 32 / (ms * 1e-3) / 1e9}
+    del part, pieces, t, keys, vals
+    cols, keep = make_c2(n, card)
+    step = n // shards
+    parts = [(ca.Table([cols[0].slice(i * step, step)]),
+              {"f64": cols[1].slice(i * step, step), "i32": cols[2].slice(i * step, step)}) for i in range(shards)]
+    aggs = {"f64": ["sum", "count", "mean"], "i32": ["max"]}
+    ms = timed(lambda: groupby_agg(parts, aggs, split_out=shards))
+    out["sharded_c2"] = {"shards": shards, "split_out": shards, "ms": ms, "rows_per_s": n / (ms * 1e-3),
+                         "aggs": aggs}
+    return out
+
+
 def bench_served(n_req, loads=(0.5, 1.0, 2.0, 4.0), workers=4):
     """Plain kernel run serially over the stream, then the continuous-batching
     server at Poisson arrival rates that are multiples of the serial rate and
@@ -450,6 +498,8 @@ def main():
                     help="requests of the served workload (0 skips it)")
     ap.add_argument("--served-workers", type=int, default=4)
     ap.add_argument("--only-served", action="store_true", help="run the served workload alone")
+    ap.add_argument("--only-partition", action="store_true",
+                    help="run hash partitioning and the sharded (partial / combine) group-by alone")
     ap.add_argument("--only-c3", action="store_true",
                     help="run the 50M-row multi-column (string) key workload alone")
     args = ap.parse_args()
@@ -470,6 +520,9 @@ def main():
         return
     if args.only_c3:
         print(json.dumps({"c3": bench_c3()}), flush=True)
+        return
+    if args.only_partition:
+        print(json.dumps({"partition": bench_partition()}), flush=True)
         return
     if args.only_served:
         print(json.dumps({"served": bench_served(args.served_requests, workers=args.served_workers)}),

@@ -289,7 +289,9 @@ std::unique_ptr<mig_table> hash_partition(mig_column_view const* input, int32_t 
   std::vector<dcol> h(num_columns_to_hash);
   for (int32_t k = 0; k < num_columns_to_hash; k++) {
     mig_column_view const& v = input[columns_to_hash[k]];
-    h[k]                     = make_dcol(v, null_count_of(v, stream) > 0);
+    // honouring a mask that has no nulls hashes the same, so an unknown null
+    // count needs no device count here
+    h[k] = make_dcol(v, v.null_mask != nullptr && v.null_count != 0);
   }
   dcol* d_cols = S.get<dcol>(h.size());
   MIG_HIP_TRY(hipMemcpyAsync(d_cols, h.data(), sizeof(dcol) * h.size(), hipMemcpyHostToDevice, stream));
@@ -380,7 +382,9 @@ std::unique_ptr<mig_table> concatenate(mig_column_view const* columns, int32_t n
       mig_column_view const& v = at(tb, c);
       concat_piece& P          = pieces[tb];
       P                        = concat_piece{};
-      P.mask                   = null_count_of(v, stream) > 0 ? v.null_mask : nullptr;
+      // a piece with an unknown null count is read through its mask; a
+      // result mask without nulls is dropped at the end
+      P.mask                   = (v.null_mask != nullptr && v.null_count != 0) ? v.null_mask : nullptr;
       P.offsets                = v.offsets;
       P.offsets64              = v.offsets_type_id == MIG_TYPE_INT64;
       P.row_start              = row;
@@ -445,8 +449,16 @@ std::unique_ptr<mig_table> concatenate(mig_column_view const* columns, int32_t n
   std::vector<uint32_t> nulls(std::max(1, num_columns));
   MIG_HIP_TRY(hipMemcpyAsync(nulls.data(), d_nulls, sizeof(uint32_t) * nulls.size(), hipMemcpyDeviceToHost, stream));
   MIG_HIP_TRY(hipStreamSynchronize(stream));
-  for (int32_t c = 0; c < num_columns; c++)
-    if (t->columns[c].col.null_mask) t->columns[c].col.null_count = int32_t(nulls[c]);
+  for (int32_t c = 0; c < num_columns; c++) {
+    owned_column& oc = t->columns[c];
+    if (!oc.col.null_mask) continue;
+    oc.col.null_count = int32_t(nulls[c]);
+    if (nulls[c] == 0) {  // no piece had nulls: no mask, as for inputs without nulls
+      t->alloc.deallocate(oc.col.null_mask, oc.mask_bytes);
+      oc.col.null_mask = nullptr;
+      oc.mask_bytes    = 0;
+    }
+  }
   return t;
 }
 

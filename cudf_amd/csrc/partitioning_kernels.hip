@@ -224,26 +224,45 @@ __global__ void __launch_bounds__(HP_BLOCK)
   gp(dst)[2 * i + 1] = hi;
 }
 
-// Output validity, 64 rows per wave: two mask words per wave, nulls counted.
+// Null count of a workgroup added once: the per-wave counts are summed in
+// LDS first (one atomic per row would serialise on a single address).
+__device__ __forceinline__ void add_block_nulls(uint32_t wave_nulls, uint32_t* null_count)
+{
+  __shared__ uint32_t part[HP_WAVES];
+  int const wave = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) part[wave] = wave_nulls;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+    for (int w = 0; w < HP_WAVES; w++) t += part[w];
+    if (t) atomicAdd(null_count, t);
+  }
+}
+
+// Output validity: each wave builds two mask words per 64 rows (grid-stride
+// over the rows), nulls counted per workgroup.
 __global__ void __launch_bounds__(HP_BLOCK)
   k_gather_mask(uint32_t const* __restrict__ mask, int64_t mask_offset, uint32_t const* __restrict__ map,
                 int64_t n, uint32_t* __restrict__ out, uint32_t* __restrict__ null_count)
 {
-  int64_t const i = int64_t(blockIdx.x) * HP_BLOCK + threadIdx.x;
-  bool v = false;
-  if (i < n) {
-    int64_t const r = int64_t(gp(map)[i]) + mask_offset;
-    v               = (gp(mask)[r >> 5] >> (r & 31)) & 1u;
+  uint32_t nulls = 0;
+  for (int64_t base = int64_t(blockIdx.x) * HP_BLOCK; base < n; base += int64_t(gridDim.x) * HP_BLOCK) {
+    int64_t const i = base + threadIdx.x;
+    bool v = false;
+    if (i < n) {
+      int64_t const r = int64_t(gp(map)[i]) + mask_offset;
+      v               = (gp(mask)[r >> 5] >> (r & 31)) & 1u;
+    }
+    uint64_t const bal = __ballot(v);
+    uint64_t const in  = __ballot(i < n);
+    if ((threadIdx.x & 63) == 0 && in) {
+      int64_t const word = (i >> 5);
+      gp(out)[word]      = uint32_t(bal);
+      gp(out)[word + 1]  = uint32_t(bal >> 32);
+      nulls += uint32_t(__popcll(in & ~bal));
+    }
   }
-  uint64_t const bal = __ballot(v);
-  uint64_t const in  = __ballot(i < n);
-  if ((threadIdx.x & 63) == 0 && in) {
-    int64_t const word = (i >> 5);
-    gp(out)[word]      = uint32_t(bal);
-    gp(out)[word + 1]  = uint32_t(bal >> 32);
-    uint32_t const nulls = uint32_t(__popcll(in & ~bal));
-    if (nulls) atomicAdd(null_count, nulls);
-  }
+  add_block_nulls(nulls, null_count);
 }
 
 __global__ void __launch_bounds__(HP_BLOCK)
@@ -272,25 +291,28 @@ __global__ void __launch_bounds__(HP_BLOCK)
   k_concat_mask(concat_piece const* __restrict__ pieces, int np, int64_t n, uint32_t* __restrict__ out,
                 uint32_t* __restrict__ null_count)
 {
-  int64_t const i = int64_t(blockIdx.x) * HP_BLOCK + threadIdx.x;
-  bool v = false;
-  if (i < n) {
-    concat_piece const& P = pieces[find_piece(pieces, np, i)];
-    if (P.mask == nullptr) v = true;
-    else {
-      int64_t const r = i - P.row_start + P.offset;
-      v               = (gp(P.mask)[r >> 5] >> (r & 31)) & 1u;
+  uint32_t nulls = 0;
+  for (int64_t base = int64_t(blockIdx.x) * HP_BLOCK; base < n; base += int64_t(gridDim.x) * HP_BLOCK) {
+    int64_t const i = base + threadIdx.x;
+    bool v = false;
+    if (i < n) {
+      concat_piece const& P = pieces[find_piece(pieces, np, i)];
+      if (P.mask == nullptr) v = true;
+      else {
+        int64_t const r = i - P.row_start + P.offset;
+        v               = (gp(P.mask)[r >> 5] >> (r & 31)) & 1u;
+      }
+    }
+    uint64_t const bal = __ballot(v);
+    uint64_t const in  = __ballot(i < n);
+    if ((threadIdx.x & 63) == 0 && in) {
+      int64_t const word = (i >> 5);
+      gp(out)[word]      = uint32_t(bal);
+      gp(out)[word + 1]  = uint32_t(bal >> 32);
+      nulls += uint32_t(__popcll(in & ~bal));
     }
   }
-  uint64_t const bal = __ballot(v);
-  uint64_t const in  = __ballot(i < n);
-  if ((threadIdx.x & 63) == 0 && in) {
-    int64_t const word = (i >> 5);
-    gp(out)[word]      = uint32_t(bal);
-    gp(out)[word + 1]  = uint32_t(bal >> 32);
-    uint32_t const nulls = uint32_t(__popcll(in & ~bal));
-    if (nulls) atomicAdd(null_count, nulls);
-  }
+  add_block_nulls(nulls, null_count);
 }
 
 // String offsets of the concatenation: each piece's offsets rebased to where
@@ -337,6 +359,8 @@ __global__ void k_concat_char_ranges(concat_piece const* __restrict__ pieces, in
 }
 
 inline unsigned grid_of(int64_t n) { return unsigned((n + HP_BLOCK - 1) / HP_BLOCK); }
+// mask kernels: enough workgroups to fill the device, each looping over rows
+inline unsigned mask_grid(int64_t n) { return unsigned(std::min<int64_t>(grid_of(n), 4096)); }
 
 }  // namespace
 
@@ -406,7 +430,7 @@ void launch_gather_mask(uint32_t const* mask, int64_t mask_offset, uint32_t cons
                         uint32_t* null_count, hipStream_t s)
 {
   if (n == 0) return;
-  hipLaunchKernelGGL(k_gather_mask, dim3(grid_of(n)), dim3(HP_BLOCK), 0, s, mask, mask_offset, map, n, out,
+  hipLaunchKernelGGL(k_gather_mask, dim3(mask_grid(n)), dim3(HP_BLOCK), 0, s, mask, mask_offset, map, n, out,
                      null_count);
 }
 
@@ -420,7 +444,7 @@ void launch_concat_mask(concat_piece const* pieces, int np, int64_t n, uint32_t*
                         hipStream_t s)
 {
   if (n == 0) return;
-  hipLaunchKernelGGL(k_concat_mask, dim3(grid_of(n)), dim3(HP_BLOCK), 0, s, pieces, np, n, out, null_count);
+  hipLaunchKernelGGL(k_concat_mask, dim3(mask_grid(n)), dim3(HP_BLOCK), 0, s, pieces, np, n, out, null_count);
 }
 
 void launch_concat_char_ranges(concat_piece const* pieces, int np, int64_t* ranges, hipStream_t s)
