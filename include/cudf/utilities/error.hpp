@@ -36,6 +36,8 @@ inline void throw_on_error(mig_status s)
     case MIG_ERROR_HIP: throw cuda_error(msg);
     case MIG_ERROR_DATA_TYPE: throw data_type_error(msg);
     case MIG_ERROR_ALLOC: throw std::bad_alloc();
+    case MIG_ERROR_OUT_OF_RANGE: throw std::out_of_range(msg);
+    case MIG_ERROR_OVERFLOW: throw std::overflow_error(msg);
     default: throw logic_error(msg);
   }
 }

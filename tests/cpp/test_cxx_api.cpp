@@ -12,8 +12,10 @@
 #include <string>
 #include <vector>
 
+#include <cudf/concatenate.hpp>
 #include <cudf/groupby.hpp>
 #include <cudf/hashing.hpp>
+#include <cudf/partitioning.hpp>
 
 namespace {
 
@@ -207,6 +209,46 @@ int main()
     CHECK(hv[0] == -1930516747 && hv[1] == 422619251 && hv[2] == -941520876, "murmur3 row hashes %d %d %d",
           hv[0], hv[1], hv[2]);
     std::printf("%s murmurhash3_x86_32\n", failures ? "..." : "ok ");
+  }
+
+  // hash_partition (cpp/tests/partitioning/hash_partition_test.cpp:155-179, 53-64)
+  {
+    std::vector<std::unique_ptr<dev_buf>> keep;
+    auto to_hash = make_col(std::vector<int>{1, 2, 3, 4, 5, 6}, {}, cudf::type_id::INT32, keep);
+    auto first   = make_col(std::vector<int>{7, 8, 9, 10, 11, 12}, {}, cudf::type_id::INT32, keep);
+    auto second  = make_col(std::vector<int>{13, 14, 15, 16, 17, 18}, {}, cudf::type_id::INT32, keep);
+    auto [r1, o1] = cudf::hash_partition(cudf::table_view{{to_hash, first}}, {0}, 3);
+    auto [r2, o2] = cudf::hash_partition(cudf::table_view{{to_hash, second}}, {0}, 3);
+    CHECK(o1.size() == 3 && o1 == o2, "hash_partition offsets differ");
+    auto a = to_host<int32_t>(r1->view().column(0).head(), 6);
+    auto b = to_host<int32_t>(r2->view().column(0).head(), 6);
+    CHECK(a == b, "hash_partition hashed columns differ");
+    // every row sits in the partition its row hash names, rows in input order inside it
+    auto h  = cudf::hashing::murmurhash3_x86_32(cudf::table_view{{r1->view().column(0)}});
+    auto hv = to_host<uint32_t>(h->view().head(), 6);
+    auto f  = to_host<int32_t>(r1->view().column(1).head(), 6);
+    for (int i = 0; i < 6; i++) {
+      int p = 0;
+      while (p + 1 < 3 && o1[p + 1] <= i) p++;
+      CHECK(int(hv[i] % 3) == p, "row %d in partition %d, hash says %u", i, p, hv[i] % 3);
+      if (i > 0 && int(hv[i - 1] % 3) == p) CHECK(f[i - 1] < f[i], "rows out of input order in partition %d", p);
+    }
+    bool threw = false;
+    try {
+      cudf::hash_partition(cudf::table_view{{to_hash}}, {-1}, 3);
+    } catch (std::out_of_range const&) {
+      threw = true;
+    }
+    CHECK(threw, "invalid column index did not throw std::out_of_range");
+    auto pmap = make_col(std::vector<int>{2, 0, 1, 2, 0, 0}, {}, cudf::type_id::INT32, keep);
+    auto [r3, o3] = cudf::partition(cudf::table_view{{first}}, pmap, 3);
+    auto g = to_host<int32_t>(r3->view().column(0).head(), 6);
+    CHECK((o3 == std::vector<int>{0, 3, 4, 6}) && (g == std::vector<int>{8, 11, 12, 9, 7, 10}),
+          "partition by map");
+    auto cat = cudf::concatenate({cudf::table_view{{to_hash, first}}, cudf::table_view{{second, to_hash}}});
+    auto c0  = to_host<int32_t>(cat->view().column(0).head(), 12);
+    CHECK(cat->num_rows() == 12 && c0[5] == 6 && c0[6] == 13 && c0[11] == 18, "concatenate");
+    std::printf("%s hash_partition / partition / concatenate\n", failures ? "..." : "ok ");
   }
 
   std::printf("%d failure(s)\n", failures);
