@@ -123,6 +123,22 @@ mig_status mig_groupby_aggregate(const mig_column_view* keys, int32_t num_keys, 
   });
 }
 
+mig_status mig_groupby_aggregate_sorted(const mig_column_view* keys, int32_t num_keys, int32_t null_policy,
+                                        const mig_aggregation_request* requests, int32_t num_requests,
+                                        void* stream, const mig_memory_resource* mr,
+                                        mig_groupby_result** out)
+{
+  return guarded([&] {
+    if (!out) mig::fail(MIG_ERROR_LOGIC, "null output handle");
+    *out = nullptr;
+    if (num_keys > 0 && !keys) mig::fail(MIG_ERROR_LOGIC, "null key array");
+    if (num_requests > 0 && !requests) mig::fail(MIG_ERROR_LOGIC, "null request array");
+    auto r = mig::groupby_aggregate(keys, num_keys, null_policy, requests, num_requests,
+                                    static_cast<hipStream_t>(stream), mr, true);
+    *out = r.release();
+  });
+}
+
 int32_t mig_result_num_groups(const mig_groupby_result* r) { return r ? r->num_groups : 0; }
 int32_t mig_result_num_keys(const mig_groupby_result* r) { return r ? int32_t(r->keys.size()) : 0; }
 int32_t mig_result_num_requests(const mig_groupby_result* r)

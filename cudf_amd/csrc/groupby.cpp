@@ -500,7 +500,7 @@ std::unique_ptr<mig_groupby_result> groupby_aggregate(mig_column_view const* key
                                                       int32_t null_policy,
                                                       mig_aggregation_request const* requests,
                                                       int32_t num_requests, hipStream_t stream,
-                                                      mig_memory_resource const* mr)
+                                                      mig_memory_resource const* mr, bool keys_sorted)
 {
   double* stats = thread_stats();
   std::fill(stats, stats + STAT_COUNT, 0.0);
@@ -655,7 +655,7 @@ std::unique_ptr<mig_groupby_result> groupby_aggregate(mig_column_view const* key
   T.mark(0);
   uint64_t const max_cap = next_pow2(uint64_t(2 * N + 2));
   double est = double(N);
-  if (N > 4096) {
+  if (N > 4096 && !keys_sorted) {
     int64_t const ns = std::min<int64_t>(N, SAMPLE_ROWS);
     uint32_t const tslots = uint32_t(next_pow2(uint64_t(2 * ns)));
     auto* table    = S.get<uint64_t>(tslots);
@@ -820,9 +820,61 @@ std::unique_ptr<mig_groupby_result> groupby_aggregate(mig_column_view const* key
 
   int64_t G = -1;
 
+  // ---- pre-sorted keys: groups are the runs of equal adjacent key rows -----
+  // (the reference's sort path with keys_pre_sorted, sort/sort_helper.cu:55-72,
+  // 133-181); groups come out in input order.
+  if (keys_sorted) {
+    P.key_mode            = KM_GENERAL;  // keys are gathered from each group's first row
+    int64_t const ntiles  = sorted_tiles(N);
+    uint32_t* tile_counts = S.get<uint32_t>(ntiles + 1);
+    upload(0);
+    launch_sorted_count(dP, N, tile_counts, stream);
+    launch_scan_blocks(tile_counts, ntiles, tile_counts + ntiles, stream);
+    auto* h_groups = reinterpret_cast<uint32_t*>(pin + 2 * sizeof(gb_params));
+    MIG_HIP_TRY(hipMemcpyAsync(h_groups, tile_counts + ntiles, 4, hipMemcpyDeviceToHost, stream));
+    MIG_HIP_TRY(hipStreamSynchronize(stream));
+    G = int64_t(*h_groups);
+    // accumulators of G groups (slots 0..G-1; the two special slots unused)
+    int32_t* d_flags = S.get<int32_t>(4);
+    P.cap            = uint64_t(G);
+    P.cap_mask       = 0;
+    P.overflow       = d_flags;
+    P.special        = reinterpret_cast<uint32_t*>(d_flags + 1);
+    uint64_t const total = uint64_t(G) + 2;
+    P.slot_rows      = S.get<uint64_t>(total);
+    P.slot_count     = P.need_count_all ? S.get<uint32_t>(total) : nullptr;
+    P.count_stride   = 4;
+    for (int f = 0; f < nf; f++) {
+      P.fields[f].ptr    = P.fields[f].width == 8 ? static_cast<void*>(S.get<uint64_t>(total))
+                                                  : static_cast<void*>(S.get<uint32_t>(total));
+      P.fields[f].stride = P.fields[f].width;
+    }
+    upload(0);
+    launch_init_state(dP, P, stream);
+    uint32_t* gid = S.get<uint32_t>(N);
+    launch_sorted_assign(dP, N, tile_counts, gid, stream);
+    T.mark(2);
+    launch_sorted_aggregate(dP, N, gid, dp.num_cus, stream);
+    T.mark(3);
+    T.mark(4);
+    if (P.need_second_pass) launch_sorted_second_pass(dP, N, gid, dp.num_cus, stream);
+    T.mark(5);
+    uint32_t* map = S.get<uint32_t>(std::max<int64_t>(G, 1));
+    launch_iota_u32(map, G, stream);
+    T.mark(6);
+    alloc_outputs(G, G, false);
+    uint32_t* rep_rows = S.get<uint32_t>(std::max<int64_t>(G, 1));
+    upload(1);
+    launch_outputs(dP, P, map, G, rep_rows, stream);
+    gather_general_keys(rep_rows, G);
+    MIG_HIP_TRY(hipGetLastError());
+    stats[STAT_STRATEGY] = 3;  // runs of pre-sorted keys
+    stats[STAT_CAPACITY] = double(G);
+  }
+
   // ---- strategy: radix-partitioned LDS aggregation for high cardinality -----
   int const forced = g_strategy.load();
-  bool rpa_ok      = P.key_mode == KM_INLINE || P.key_mode == KM_GENERAL;
+  bool rpa_ok      = G < 0 && (P.key_mode == KM_INLINE || P.key_mode == KM_GENERAL);
   int n_flagged    = 0;
   for (int v = 0; v < P.num_vals && rpa_ok; v++) {
     if (P.val_field_begin[v] == P.val_field_begin[v + 1]) continue;

@@ -66,7 +66,7 @@ std::unique_ptr<mig_groupby_result> groupby_aggregate(mig_column_view const* key
                                                       int32_t null_policy,
                                                       mig_aggregation_request const* requests,
                                                       int32_t num_requests, hipStream_t stream,
-                                                      mig_memory_resource const* mr);
+                                                      mig_memory_resource const* mr, bool keys_sorted = false);
 
 void hash_rows(mig_column_view const* cols, int32_t ncols, uint32_t seed, uint32_t* out,
                hipStream_t stream);

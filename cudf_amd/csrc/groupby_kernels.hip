@@ -957,4 +957,254 @@ void launch_count_nulls(dcol const& c, int64_t n, uint32_t* d_count, hipStream_t
   hipLaunchKernelGGL(k_count_valid, dim3(blocks), dim3(BLOCK), 0, s, c, n, d_count);
 }
 
+// ---------------------------------------------------------------------------
+// Pre-sorted keys (groupby(keys, null_policy, sorted::YES)): the groups are
+// the runs of equal adjacent key rows, numbered in input order, as the
+// reference's sort path finds them with keys_pre_sorted (sort/sort_helper.cu:
+// 55-72, 133-181).  k_sorted_count counts run heads per tile, a scan gives
+// each tile its first group number, k_sorted_assign writes every row's group
+// (NO_GROUP for rows the null policy drops) and each group's first row, and
+// the aggregation kernels reduce each 64-row chunk with a segmented scan
+// across the wave, so a run costs one atomic per wave and field.
+constexpr int SORTED_ROWS = 16;                    // rows per thread per tile
+constexpr int SORTED_TILE = BLOCK * SORTED_ROWS;   // rows per tile
+constexpr uint32_t NO_GROUP = 0xFFFFFFFFu;
+
+__device__ __forceinline__ bool sorted_included(gb_params const& P, int64_t row)
+{
+  return !(P.skip_null_keys && any_key_null(P, row));
+}
+
+__device__ __forceinline__ bool sorted_head(gb_params const& P, int64_t row)
+{
+  if (!sorted_included(P, row)) return false;
+  if (row == 0 || !sorted_included(P, row - 1)) return true;
+  return !rows_equal(P, row, row - 1);
+}
+
+__global__ void __launch_bounds__(BLOCK) k_sorted_count(gb_params const* __restrict__ Pp, uint32_t* block_counts)
+{
+  __shared__ uint32_t red[BLOCK / 64];
+  gb_params const& P = *Pp;
+  int64_t const base = int64_t(blockIdx.x) * SORTED_TILE;
+  uint32_t heads = 0;
+  for (int j = 0; j < SORTED_ROWS; j++) {
+    int64_t const row = base + int64_t(j) * BLOCK + threadIdx.x;
+    if (row < P.num_rows && sorted_head(P, row)) heads++;
+  }
+  uint32_t const t = block_sum_u32<BLOCK>(heads, red);
+  if (threadIdx.x == 0) block_counts[blockIdx.x] = t;
+}
+
+__global__ void __launch_bounds__(BLOCK)
+  k_sorted_assign(gb_params const* __restrict__ Pp, uint32_t const* __restrict__ block_offsets,
+                  uint32_t* __restrict__ gid)
+{
+  __shared__ uint32_t wave_tot[BLOCK / 64];
+  gb_params const& P = *Pp;
+  int const wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  uint64_t const below = (1ull << lane) - 1ull;
+  int64_t const base   = int64_t(blockIdx.x) * SORTED_TILE;
+  uint32_t running     = block_offsets[blockIdx.x];  // heads before this iteration's rows
+  for (int j = 0; j < SORTED_ROWS; j++) {
+    int64_t const row = base + int64_t(j) * BLOCK + threadIdx.x;
+    bool const live   = row < P.num_rows;
+    bool const incl   = live && sorted_included(P, row);
+    bool const head   = incl && sorted_head(P, row);
+    uint64_t const bal = __ballot(head);
+    if (lane == 0) wave_tot[wave] = uint32_t(__popcll(bal));
+    __syncthreads();
+    uint32_t before = 0, all = 0;
+    for (int w = 0; w < BLOCK / 64; w++) {
+      if (w < wave) before += wave_tot[w];
+      all += wave_tot[w];
+    }
+    uint32_t const heads_before = running + before + uint32_t(__popcll(bal & below));
+    if (live) {
+      uint32_t const g = !incl ? NO_GROUP : (head ? heads_before : heads_before - 1u);
+      gid[row] = g;
+      if (head) P.slot_rows[g] = uint64_t(row);
+    }
+    running += all;
+    __syncthreads();
+  }
+}
+
+// Value of one row in the accumulator encoding of field kind `kind`.
+__device__ __forceinline__ uint64_t sorted_part(int32_t kind, loaded_value const& v)
+{
+  switch (kind) {
+    case FK_SUM_I64: return uint64_t(v.i);
+    case FK_SUM_F32: return __float_as_uint(v.f);
+    case FK_SUM_F64: return uint64_t(__double_as_longlong(v.d));
+    case FK_SUMSQ_I64: return uint64_t(v.i) * uint64_t(v.i);
+    case FK_SUMSQ_F32: return __float_as_uint(v.f * v.f);
+    case FK_SUMSQ_F64: return uint64_t(__double_as_longlong(v.d * v.d));
+    case FK_MIN_S32:
+    case FK_MAX_S32: return uint32_t(int32_t(v.i));
+    case FK_MIN_S64:
+    case FK_MAX_S64:
+    case FK_MIN_U64:
+    case FK_MAX_U64: return uint64_t(v.i);
+    case FK_MIN_U32:
+    case FK_MAX_U32: return uint32_t(v.i);
+    case FK_MIN_F32:
+    case FK_MAX_F32: return isnan(v.f) ? field_identity(kind) : ord32(uint32_t(v.b));
+    case FK_MIN_F64:
+    case FK_MAX_F64: return isnan(v.d) ? field_identity(kind) : ord64(v.b);
+    case FK_COUNT: return 1;
+    default: return 0;
+  }
+}
+
+// Combine two partials in the accumulator encoding.
+__device__ __forceinline__ uint64_t sorted_combine(int32_t kind, uint64_t a, uint64_t b)
+{
+  switch (kind) {
+    case FK_SUM_I64:
+    case FK_SUMSQ_I64:
+    case FK_COUNT: return a + b;
+    case FK_SUM_F32:
+    case FK_SUMSQ_F32: return __float_as_uint(__uint_as_float(uint32_t(a)) + __uint_as_float(uint32_t(b)));
+    case FK_SUM_F64:
+    case FK_SUMSQ_F64:
+    case FK_M2:
+      return uint64_t(__double_as_longlong(__longlong_as_double(int64_t(a)) + __longlong_as_double(int64_t(b))));
+    case FK_MIN_S32: return uint32_t(min(int32_t(uint32_t(a)), int32_t(uint32_t(b))));
+    case FK_MAX_S32: return uint32_t(max(int32_t(uint32_t(a)), int32_t(uint32_t(b))));
+    case FK_MIN_S64: return uint64_t(min(int64_t(a), int64_t(b)));
+    case FK_MAX_S64: return uint64_t(max(int64_t(a), int64_t(b)));
+    case FK_MIN_U32:
+    case FK_MIN_F32: return min(uint32_t(a), uint32_t(b));
+    case FK_MAX_U32:
+    case FK_MAX_F32: return max(uint32_t(a), uint32_t(b));
+    case FK_MIN_U64:
+    case FK_MIN_F64: return min(a, b);
+    case FK_MAX_U64:
+    case FK_MAX_F64: return max(a, b);
+    default: return a;
+  }
+}
+
+// Inclusive scan of `part` over the lanes of each run of equal `g` (runs
+// are contiguous lanes because group numbers never decrease along rows).
+__device__ __forceinline__ uint64_t segmented_scan(int32_t kind, uint64_t part, uint32_t g, int lane)
+{
+  for (int o = 1; o < 64; o <<= 1) {
+    uint64_t const other = __shfl_up(part, o, 64);
+    uint32_t const og    = __shfl_up(g, o, 64);
+    if (lane >= o && og == g) part = sorted_combine(kind, part, other);
+  }
+  return part;
+}
+
+__global__ void __launch_bounds__(BLOCK)
+  k_sorted_aggregate(gb_params const* __restrict__ Pp, uint32_t const* __restrict__ gid)
+{
+  gb_params const& P = *Pp;
+  int const lane       = threadIdx.x & 63;
+  int64_t const nwaves = int64_t(gridDim.x) * (BLOCK / 64);
+  for (int64_t chunk = int64_t(blockIdx.x) * (BLOCK / 64) + (threadIdx.x >> 6); chunk * 64 < P.num_rows;
+       chunk += nwaves) {
+    int64_t const row = chunk * 64 + lane;
+    bool const live   = row < P.num_rows;
+    uint32_t const g  = live ? gid[row] : NO_GROUP;
+    uint32_t const gn = __shfl_down(g, 1, 64);
+    bool const last   = g != NO_GROUP && (lane == 63 || gn != g);
+    if (P.need_count_all) {
+      uint64_t const part = segmented_scan(FK_COUNT, g != NO_GROUP ? 1u : 0u, g, lane);
+      if (last) aadd(count_at(P, g), uint32_t(part));
+    }
+    for (int vc = 0; vc < P.num_vals; vc++) {
+      int const fb = P.val_field_begin[vc], fe = P.val_field_begin[vc + 1];
+      if (fb == fe) continue;
+      dcol const& c    = P.vals[vc];
+      bool const valid = g != NO_GROUP && dcol_valid(c, row);
+      loaded_value v{0, 0.0, 0.0f, 0};
+      if (valid) v = load_value(c, row);
+      for (int f = fb; f < fe; f++) {
+        field_desc const& F = P.fields[f];
+        if (F.kind == FK_M2) continue;
+        uint64_t const part = segmented_scan(F.kind, valid ? sorted_part(F.kind, v) : field_identity(F.kind), g, lane);
+        if (last) acc_combine(F.kind, field_at(F, g), part);
+      }
+    }
+  }
+}
+
+__global__ void __launch_bounds__(BLOCK)
+  k_sorted_second_pass(gb_params const* __restrict__ Pp, uint32_t const* __restrict__ gid)
+{
+  gb_params const& P = *Pp;
+  int const lane       = threadIdx.x & 63;
+  int64_t const nwaves = int64_t(gridDim.x) * (BLOCK / 64);
+  for (int64_t chunk = int64_t(blockIdx.x) * (BLOCK / 64) + (threadIdx.x >> 6); chunk * 64 < P.num_rows;
+       chunk += nwaves) {
+    int64_t const row = chunk * 64 + lane;
+    bool const live   = row < P.num_rows;
+    uint32_t const g  = live ? gid[row] : NO_GROUP;
+    uint32_t const gn = __shfl_down(g, 1, 64);
+    bool const last   = g != NO_GROUP && (lane == 63 || gn != g);
+    for (int vc = 0; vc < P.num_vals; vc++) {
+      int const fb = P.val_field_begin[vc], fe = P.val_field_begin[vc + 1];
+      bool has_m2  = false;
+      for (int f = fb; f < fe; f++) has_m2 |= (P.fields[f].kind == FK_M2);
+      if (!has_m2) continue;
+      dcol const& c    = P.vals[vc];
+      bool const valid = g != NO_GROUP && dcol_valid(c, row);
+      loaded_value v{0, 0.0, 0.0f, 0};
+      if (valid) v = load_value(c, row);
+      for (int f = fb; f < fe; f++) {
+        field_desc const& F = P.fields[f];
+        if (F.kind != FK_M2) continue;
+        double sq = 0.0;
+        if (valid) {
+          double const cnt  = double(count_of(P, F.aux_count, g));
+          double const mean = field_as_double(P.fields[F.aux_sum], g) / cnt;
+          double const d    = v.d - mean;
+          sq                = d * d;
+        }
+        uint64_t const part = segmented_scan(FK_M2, uint64_t(__double_as_longlong(sq)), g, lane);
+        if (last) gadd_f64(reinterpret_cast<double*>(field_at(F, g)), __longlong_as_double(int64_t(part)));
+      }
+    }
+  }
+}
+
+__global__ void __launch_bounds__(BLOCK) k_iota_u32(uint32_t* out, int64_t n)
+{
+  int64_t const i = int64_t(blockIdx.x) * BLOCK + threadIdx.x;
+  if (i < n) out[i] = uint32_t(i);
+}
+
+void launch_sorted_count(gb_params const* dP, int64_t n, uint32_t* block_counts, hipStream_t s)
+{
+  int64_t const nb = (n + SORTED_TILE - 1) / SORTED_TILE;
+  hipLaunchKernelGGL(k_sorted_count, dim3(nb), dim3(BLOCK), 0, s, dP, block_counts);
+}
+
+void launch_sorted_assign(gb_params const* dP, int64_t n, uint32_t const* block_offsets, uint32_t* gid,
+                          hipStream_t s)
+{
+  int64_t const nb = (n + SORTED_TILE - 1) / SORTED_TILE;
+  hipLaunchKernelGGL(k_sorted_assign, dim3(nb), dim3(BLOCK), 0, s, dP, block_offsets, gid);
+}
+
+int64_t sorted_tiles(int64_t n) { return (n + SORTED_TILE - 1) / SORTED_TILE; }
+
+void launch_sorted_aggregate(gb_params const* dP, int64_t n, uint32_t const* gid, int num_cus, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_sorted_aggregate, dim3(row_grid(n, num_cus, 16)), dim3(BLOCK), 0, s, dP, gid);
+}
+
+void launch_sorted_second_pass(gb_params const* dP, int64_t n, uint32_t const* gid, int num_cus, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_sorted_second_pass, dim3(row_grid(n, num_cus, 16)), dim3(BLOCK), 0, s, dP, gid);
+}
+
+void launch_iota_u32(uint32_t* out, int64_t n, hipStream_t s)
+{
+  if (n > 0) hipLaunchKernelGGL(k_iota_u32, dim3((n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, s, out, n);
+}
+
 }  // namespace mig

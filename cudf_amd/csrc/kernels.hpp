@@ -240,6 +240,17 @@ void launch_count_nulls(dcol const& c, int64_t n, uint32_t* d_count, hipStream_t
 void launch_count_valid_list(uint32_t const* const* masks, int count, int64_t n, uint32_t* d_counts,
                              hipStream_t s);
 
+// ---- pre-sorted keys: groups are runs of equal adjacent key rows ------------
+int64_t sorted_tiles(int64_t n);
+// run heads per tile of rows
+void launch_sorted_count(gb_params const* dP, int64_t n, uint32_t* block_counts, hipStream_t s);
+// group of every row (0xFFFFFFFF when dropped) and slot_rows[g] = first row of group g
+void launch_sorted_assign(gb_params const* dP, int64_t n, uint32_t const* block_offsets, uint32_t* gid,
+                          hipStream_t s);
+void launch_sorted_aggregate(gb_params const* dP, int64_t n, uint32_t const* gid, int num_cus, hipStream_t s);
+void launch_sorted_second_pass(gb_params const* dP, int64_t n, uint32_t const* gid, int num_cus, hipStream_t s);
+void launch_iota_u32(uint32_t* out, int64_t n, hipStream_t s);
+
 // ---- radix-partitioned aggregation (partition_kernels.hip) ------------------
 void launch_rpa_hist(gb_params const* dP, rpa_params const* dR, rpa_params const& hR, int64_t n,
                      int num_cus, hipStream_t s);

@@ -59,9 +59,8 @@ class GroupBy:
                  keys_are_sorted: Sorted = Sorted.NO, column_order=None, null_precedence=None):
         self._keys = keys
         self._null_handling = NullPolicy(null_handling)
-        # Sorted keys take the sort-based path in the reference
-        # (groupby.cu:78-83); the hash path computes the same groups, so the
-        # flag only documents the caller's promise here.
+        # Sorted keys take the sort-based path (groupby.cu:66-84): groups are
+        # the runs of equal adjacent key rows, returned in input order.
         self._keys_are_sorted = Sorted(keys_are_sorted)
 
     def aggregate(self, requests: List[GroupByRequest],
@@ -82,9 +81,9 @@ class GroupBy:
         mr = TorchMemoryResource()
         handle = ctypes.c_void_p()
         L = _lib.lib()
-        _lib.check(L.mig_groupby_aggregate(keys, self._keys.num_columns(), int(self._null_handling),
-                                           reqs, nreq, _stream_handle(stream), mr.pointer(),
-                                           ctypes.byref(handle)))
+        entry = L.mig_groupby_aggregate_sorted if self._keys_are_sorted == Sorted.YES else L.mig_groupby_aggregate
+        _lib.check(entry(keys, self._keys.num_columns(), int(self._null_handling), reqs, nreq,
+                         _stream_handle(stream), mr.pointer(), ctypes.byref(handle)))
         try:
             out = _lib.OutColumn()
             key_cols = []

@@ -50,10 +50,11 @@ inline std::unique_ptr<column> adopt(mig_column const& c, device_memory_resource
 
 }  // namespace detail
 
-/// Hash group-by over `keys` (groupby.hpp:109-200).  Keys are not copied; the
-/// caller keeps them alive.  Only the hash path exists here: `keys_are_sorted`
-/// is accepted and does not change the result, which is the same set of
-/// groups in unspecified order.
+/// Group-by over `keys` (groupby.hpp:109-200).  Keys are not copied; the
+/// caller keeps them alive.  Unsorted keys take the hash path (groups in
+/// unspecified order); with `keys_are_sorted == sorted::YES` the groups are
+/// the runs of equal adjacent key rows, in input order (the sort path with
+/// pre-sorted keys).
 class groupby {
  public:
   groupby() = delete;
@@ -85,7 +86,8 @@ class groupby {
     }
     mig_memory_resource cmr = mr->c_resource();
     mig_groupby_result* handle = nullptr;
-    cudf::detail::throw_on_error(mig_groupby_aggregate(
+    auto const entry = _keys_are_sorted == sorted::YES ? mig_groupby_aggregate_sorted : mig_groupby_aggregate;
+    cudf::detail::throw_on_error(entry(
       keys.data(), int32_t(keys.size()), _include_null_keys == null_policy::INCLUDE ? MIG_NULL_INCLUDE : MIG_NULL_EXCLUDE,
       reqs.data(), int32_t(reqs.size()), stream.value(), &cmr, &handle));
     struct closer {

@@ -259,6 +259,24 @@ mig_status mig_groupby_aggregate(const mig_column_view* keys,
                                  const mig_memory_resource* mr,
                                  mig_groupby_result** out);
 
+/* Replaces cudf::groupby::groupby(keys, null_policy, sorted::YES).aggregate(
+ * requests, stream, mr): keys the caller promises are sorted take the
+ * reference's sort path with keys_pre_sorted (cpp/src/groupby/groupby.cu:66-84;
+ * sort/sort_helper.cu:55-72, 133-181).  Groups are the runs of equal adjacent
+ * key rows (null keys equal each other), returned in input order, so sorted
+ * keys come back sorted; unsorted keys give one group per run.  Rows with a
+ * null key are dropped under MIG_NULL_EXCLUDE (the reference re-sorts such
+ * keys first, which gives the same groups when the keys are sorted).  Same
+ * aggregations, results and errors as mig_groupby_aggregate. */
+mig_status mig_groupby_aggregate_sorted(const mig_column_view* keys,
+                                        int32_t num_keys,
+                                        int32_t null_policy,
+                                        const mig_aggregation_request* requests,
+                                        int32_t num_requests,
+                                        void* stream,
+                                        const mig_memory_resource* mr,
+                                        mig_groupby_result** out);
+
 int32_t mig_result_num_groups(const mig_groupby_result* r);
 int32_t mig_result_num_keys(const mig_groupby_result* r);
 int32_t mig_result_num_requests(const mig_groupby_result* r);

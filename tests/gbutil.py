@@ -37,16 +37,18 @@ def to_device(c: orc.HostColumn):
     return Column.from_numpy(c.data, c.valid, type_id=c.type_id)
 
 
-def gpu_groupby(keys: Sequence[orc.HostColumn], requests, null_policy: int = 0):
+def gpu_groupby(keys: Sequence[orc.HostColumn], requests, null_policy: int = 0, keys_sorted: bool = False):
     """Run the GPU group-by; returns (key columns, results) as host data:
-    key columns are (values, valid) pairs, results[r][a] = (values, valid)."""
-    from cudf_amd import GroupBy, GroupByRequest, NullPolicy, Table
+    key columns are (values, valid) pairs, results[r][a] = (values, valid).
+    ``keys_sorted`` passes keys_are_sorted=YES (groups = runs of equal rows)."""
+    from cudf_amd import GroupBy, GroupByRequest, NullPolicy, Sorted, Table
     from cudf_amd.aggregation import Aggregation
     dkeys = Table([to_device(k) for k in keys])
     reqs = []
     for col, aggs in requests:
         reqs.append(GroupByRequest(to_device(col), [Aggregation(k, d) for k, d in aggs]))
-    out_keys, out_res = GroupBy(dkeys, NullPolicy(null_policy)).aggregate(reqs)
+    gb = GroupBy(dkeys, NullPolicy(null_policy), Sorted.YES if keys_sorted else Sorted.NO)
+    out_keys, out_res = gb.aggregate(reqs)
     kcols = [c.to_numpy() for c in out_keys.columns()]
     res = [[c.to_numpy() for c in t.columns()] for t in out_res]
     return kcols, res

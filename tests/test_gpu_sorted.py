@@ -1,0 +1,134 @@
+"""GPU parity of the pre-sorted-keys path (groupby(keys, null_policy,
+sorted::YES)): groups are the runs of equal adjacent key rows, in input order
+(cpp/src/groupby/sort/sort_helper.cu:55-72, 133-181).
+
+Sorted inputs: the oracle's group-by (first-occurrence order, which for sorted
+keys is key order) must match the GPU result in order, and every value to the
+parity bar of tests/gbutil.py.  Unsorted inputs: one group per run, checked
+against the oracle grouped by a run-number column.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+from tests.gbutil import AGG, assert_keys_equal, assert_parity, gpu_groupby, host_col, oracle_groupby
+
+pytestmark = pytest.mark.gpu
+
+ALL_AGGS = [(AGG["SUM"], 1), (AGG["MIN"], 1), (AGG["MAX"], 1), (AGG["COUNT_VALID"], 1), (AGG["COUNT_ALL"], 1),
+            (AGG["MEAN"], 1), (AGG["VARIANCE"], 1), (AGG["STD"], 0), (AGG["SUM_OF_SQUARES"], 1)]
+
+
+def _check_sorted(keys, requests, policy):
+    g_keys, g_res = gpu_groupby(keys, requests, policy, keys_sorted=True)
+    o_keys, _ = oracle_groupby(keys, requests, policy)
+    n = len(o_keys[0][0])
+    ident = np.arange(n)
+    assert len(g_keys[0][0]) == n
+    assert_keys_equal(g_keys, o_keys, ident, ident)  # same groups in the same (input) order
+    return assert_parity(keys, requests, policy, gpu=(g_keys, g_res))
+
+
+@pytest.mark.parametrize("policy", [0, 1])
+@pytest.mark.parametrize("kdt", [np.int32, np.int64, np.int8, np.uint16, np.float64])
+def test_sorted_fixed_keys(policy, kdt):
+    rng = np.random.default_rng(int(np.dtype(kdt).itemsize) * 7 + policy)
+    n = 200_000
+    k = np.sort(rng.integers(0, 100 if kdt == np.int8 else 20_000, n).astype(kdt))
+    kvalid = np.ones(n, bool)
+    kvalid[-3000:] = False  # nulls last, as a sort with nulls after puts them
+    vals = [host_col(rng.integers(-10**6, 10**6, n).astype(np.int64), rng.random(n) > 0.1),
+            host_col(rng.standard_normal(n), rng.random(n) > 0.05),
+            host_col(rng.standard_normal(n).astype(np.float32)),
+            host_col(rng.integers(0, 200, n).astype(np.uint8))]
+    requests = [(v, ALL_AGGS) for v in vals]
+    _check_sorted([host_col(k, kvalid)], requests, policy)
+
+
+def test_sorted_float_keys_nan_and_signed_zero():
+    rng = np.random.default_rng(2)
+    base = np.concatenate([rng.standard_normal(5000).round(1), [0.0, -0.0] * 50, [np.nan] * 70])
+    k = np.sort(base)  # NaN last; -0.0 and 0.0 adjacent (equal keys)
+    v = host_col(rng.integers(0, 100, len(k)).astype(np.int32))
+    _check_sorted([host_col(k)], [(v, [(AGG["SUM"], 1), (AGG["COUNT_ALL"], 1), (AGG["MAX"], 1)])], 0)
+
+
+def test_sorted_multicolumn_and_string_keys():
+    rng = np.random.default_rng(3)
+    n = 60_000
+    a = rng.integers(0, 30, n).astype(np.int32)
+    words = np.array([b"w%03d" % i for i in range(200)], dtype=object)
+    b = words[rng.integers(0, 200, n)]
+    order = np.lexsort((b.astype(bytes), a))
+    a, b = a[order], list(b[order])
+    bvalid = np.ones(n, bool)
+    v = host_col(rng.standard_normal(n), rng.random(n) > 0.2)
+    keys = [host_col(a), orc.HostColumn.from_strings(b, bvalid)]
+    for policy in (0, 1):
+        _check_sorted(keys, [(v, ALL_AGGS)], policy)
+
+
+def test_sorted_single_group_and_all_groups_distinct():
+    n = 300_000
+    rng = np.random.default_rng(4)
+    v = host_col(rng.standard_normal(n), rng.random(n) > 0.3)
+    _check_sorted([host_col(np.zeros(n, np.int64))], [(v, ALL_AGGS)], 0)  # one run over every wave
+    _check_sorted([host_col(np.arange(n, dtype=np.int64))], [(v, ALL_AGGS)], 0)  # one row per group
+    # all keys null: EXCLUDE gives no groups, INCLUDE one
+    nk = host_col(np.zeros(1000, np.int32), np.zeros(1000, bool))
+    g_keys, _ = gpu_groupby([nk], [(host_col(np.arange(1000.0)), [(AGG["SUM"], 1)])], 0, True)
+    assert len(g_keys[0][0]) == 0
+    _check_sorted([nk], [(host_col(np.arange(1000.0)), [(AGG["SUM"], 1), (AGG["COUNT_ALL"], 1)])], 1)
+
+
+def test_unsorted_keys_give_one_group_per_run():
+    rng = np.random.default_rng(5)
+    n = 50_000
+    k = rng.integers(0, 4, n).astype(np.int32)  # many short runs, keys repeat across runs
+    kvalid = rng.random(n) > 0.1
+    v = host_col(rng.integers(-100, 100, n).astype(np.int64), rng.random(n) > 0.2)
+    for policy in (0, 1):
+        incl = kvalid if policy == 0 else np.ones(n, bool)
+        # run heads: included rows whose previous row is dropped or differs (nulls equal nulls)
+        prev_same = np.zeros(n, bool)
+        prev_same[1:] = incl[:-1] & ((kvalid[1:] == kvalid[:-1]) & ((k[1:] == k[:-1]) | ~kvalid[1:]))
+        head = incl & ~prev_same
+        run = np.cumsum(head) - 1
+        reqs = [(v, [(AGG["SUM"], 1), (AGG["COUNT_VALID"], 1), (AGG["MIN"], 1), (AGG["MEAN"], 1)])]
+        g_keys, g_res = gpu_groupby([host_col(k, kvalid)], reqs, policy, keys_sorted=True)
+        nruns = int(head.sum())
+        assert len(g_keys[0][0]) == nruns
+        first = np.flatnonzero(head)
+        gv, gvalid = g_keys[0]
+        gvalid = np.ones(nruns, bool) if gvalid is None else gvalid
+        np.testing.assert_array_equal(gvalid, kvalid[first])
+        np.testing.assert_array_equal(gv[gvalid], k[first][kvalid[first]])
+        # values: the oracle grouped by run number (rows outside every run dropped as null keys)
+        rk = host_col(np.where(incl, run, 0).astype(np.int64), incl)
+        assert_parity([rk], reqs, 0, gpu=([(np.arange(nruns, dtype=np.int64), None)], g_res))
+
+
+def test_sorted_c2_shape_properties():
+    """10M rows over 1M sorted keys, C2-like nulls: sorted output, counts sum to
+    the non-null rows, sums and min/max against torch on the device."""
+    import torch
+    import cudf_amd as ca
+    from cudf_amd import GroupBy, GroupByRequest, NullPolicy, Sorted
+    from cudf_amd.aggregation import Aggregation, Kind
+    n = 10_000_000
+    g = torch.Generator(device="cuda").manual_seed(6)
+    key = torch.sort(torch.randint(0, 1_000_000, (n,), device="cuda", dtype=torch.int64, generator=g)).values
+    val = torch.randint(-1000, 1000, (n,), device="cuda", dtype=torch.int64, generator=g)
+    kc = ca.Column(ca.TypeId.INT64, n, key.view(torch.uint8))
+    vc = ca.Column(ca.TypeId.INT64, n, val.view(torch.uint8))
+    out_keys, (res,) = GroupBy(ca.Table([kc]), NullPolicy.EXCLUDE, Sorted.YES).aggregate(
+        [GroupByRequest(vc, [Aggregation(Kind.SUM), Aggregation(Kind.COUNT_VALID), Aggregation(Kind.MAX)])])
+    uk, counts = torch.unique_consecutive(key, return_counts=True)
+    ok = out_keys.columns()[0].data.view(torch.int64)[:len(uk)]
+    assert out_keys.num_rows() == len(uk) and torch.equal(ok, uk)
+    seg = torch.repeat_interleave(torch.arange(len(uk), device="cuda"), counts)
+    sums = torch.zeros(len(uk), dtype=torch.int64, device="cuda").index_add_(0, seg, val)
+    assert torch.equal(res.columns()[0].data.view(torch.int64)[:len(uk)], sums)
+    assert torch.equal(res.columns()[1].data.view(torch.int32)[:len(uk)].to(torch.int64), counts)
+    mx = torch.full((len(uk),), -2**63, dtype=torch.int64, device="cuda").scatter_reduce_(0, seg, val, "amax")
+    assert torch.equal(res.columns()[2].data.view(torch.int64)[:len(uk)], mx)
