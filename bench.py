@@ -402,6 +402,49 @@ def bench_partition(n=100_000_000, card=10_000_000, reps=5, shards=8):
     return out
 
 
+def bench_sorted(n=100_000_000, card=10_000_000, reps=5):
+    """The C2 request (float64 and int32 values, 5% nulls, sum/mean/count/min/
+    max/var) over an int64 key that is sorted (10M distinct values): the
+    pre-sorted path (keys_are_sorted=YES, groups = runs) against the hash path
+    on the same data."""
+    import torch
+    import cudf_amd as ca
+    from cudf_amd import Column, TypeId
+    from cudf_amd.aggregation import Aggregation, Kind
+    dev = torch.device("cuda", 0)
+    g = torch.Generator(device=dev).manual_seed(7)
+    key = torch.sort(torch.randint(0, card, (n,), device=dev, dtype=torch.int64, generator=g)).values
+    f64 = torch.randn(n, device=dev, dtype=torch.float64, generator=g)
+    fvalid = torch.rand(n, device=dev, generator=g) >= 0.05
+    i32 = torch.randint(0, 10_001, (n,), device=dev, dtype=torch.int32, generator=g)
+    ivalid = torch.rand(n, device=dev, generator=g) >= 0.05
+    kcol = Column(TypeId.INT64, n, key.view(torch.uint8))
+    fcol = Column(TypeId.FLOAT64, n, f64.view(torch.uint8), pack_bits(fvalid), int((~fvalid).sum().item()))
+    icol = Column(TypeId.INT32, n, i32.view(torch.uint8), pack_bits(ivalid), int((~ivalid).sum().item()))
+    del fvalid, ivalid
+    aggs = lambda: [Aggregation(Kind.SUM), Aggregation(Kind.MEAN), Aggregation(Kind.COUNT_VALID),
+                    Aggregation(Kind.MIN), Aggregation(Kind.MAX), Aggregation(Kind.VARIANCE, 1)]
+    reqs = [ca.GroupByRequest(fcol, aggs()), ca.GroupByRequest(icol, aggs())]
+    out = {"workload": f"{n} rows, sorted int64 key over {card} values, float64 and int32 values "
+                       f"(5% nulls), sum/mean/count/min/max/var"}
+    for name, flag in (("sorted_path", ca.Sorted.YES), ("hash_path", ca.Sorted.NO)):
+        gb = ca.GroupBy(ca.Table([kcol]), ca.NullPolicy.EXCLUDE, flag)
+        r = gb.aggregate(reqs)
+        groups = r[0].num_rows()
+        del r
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            r = gb.aggregate(reqs)
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t0)
+            del r
+        ms = sorted(ts)[len(ts) // 2] * 1e3
+        out[name] = {"ms": ms, "rows_per_s": n / (ms * 1e-3), "groups": groups}
+    return out
+
+
 def bench_served(n_req, loads=(0.5, 1.0, 2.0, 4.0), workers=4):
     """Plain kernel run serially over the stream, then the continuous-batching
     server at Poisson arrival rates that are multiples of the serial rate and
@@ -498,6 +541,8 @@ def main():
                     help="requests of the served workload (0 skips it)")
     ap.add_argument("--served-workers", type=int, default=4)
     ap.add_argument("--only-served", action="store_true", help="run the served workload alone")
+    ap.add_argument("--only-sorted", action="store_true",
+                    help="run the pre-sorted-keys path against the hash path on sorted C2-shaped data")
     ap.add_argument("--only-partition", action="store_true",
                     help="run hash partitioning and the sharded (partial / combine) group-by alone")
     ap.add_argument("--only-c3", action="store_true",
@@ -523,6 +568,9 @@ def main():
         return
     if args.only_partition:
         print(json.dumps({"partition": bench_partition()}), flush=True)
+        return
+    if args.only_sorted:
+        print(json.dumps({"sorted": bench_sorted()}), flush=True)
         return
     if args.only_served:
         print(json.dumps({"served": bench_served(args.served_requests, workers=args.served_workers)}),

@@ -854,10 +854,10 @@ std::unique_ptr<mig_groupby_result> groupby_aggregate(mig_column_view const* key
     uint32_t* gid = S.get<uint32_t>(N);
     launch_sorted_assign(dP, N, tile_counts, gid, stream);
     T.mark(2);
-    launch_sorted_aggregate(dP, N, gid, dp.num_cus, stream);
+    launch_sorted_aggregate(dP, P, N, gid, dp.num_cus, stream);
     T.mark(3);
     T.mark(4);
-    if (P.need_second_pass) launch_sorted_second_pass(dP, N, gid, dp.num_cus, stream);
+    if (P.need_second_pass) launch_sorted_second_pass(dP, P, N, gid, dp.num_cus, stream);
     T.mark(5);
     uint32_t* map = S.get<uint32_t>(std::max<int64_t>(G, 1));
     launch_iota_u32(map, G, stream);

@@ -35,6 +35,30 @@ void launch_hash_rows(dcol const* d_cols, int ncols, int64_t n, uint32_t seed, u
 
 // ---------------------------------------------------------------------------
 // cardinality sample: row hashes of `nsample` evenly strided rows
+// A pseudo-random permutation of [0, n): a 4-round Feistel network on the
+// smallest even number of bits covering n (round function: MurmurHash3's
+// finaliser), walking the cycle until the value falls below n.
+__device__ __forceinline__ int64_t sample_row(int64_t i, int64_t n)
+{
+  int bits = 2;
+  while ((int64_t(1) << bits) < n) bits += 2;
+  int const half      = bits / 2;
+  uint64_t const mask = (uint64_t(1) << half) - 1;
+  uint64_t x          = uint64_t(i);
+  for (int walk = 0; walk < 256; walk++) {
+    uint64_t l = x >> half, r = x & mask;
+    for (uint32_t round = 0; round < 4; round++) {
+      uint64_t const f = mm_fmix32(uint32_t(r) * 0x9E3779B9u + round * 0x85EBCA6Bu + uint32_t(r >> 32));
+      uint64_t const t = l ^ (f & mask);
+      l                = r;
+      r                = t;
+    }
+    x = (l << half) | r;
+    if (x < uint64_t(n)) return int64_t(x);
+  }
+  return (i * 7919) % n;  // not reached in practice: the cycle through i < n returns below n
+}
+
 __global__ void __launch_bounds__(BLOCK)
   k_sample_hash(gb_params const* __restrict__ Pp, int64_t n, int64_t nsample, uint64_t* table,
                 uint32_t table_mask, uint32_t* counters)
@@ -42,7 +66,13 @@ __global__ void __launch_bounds__(BLOCK)
   gb_params const& P = *Pp;
   int64_t i = int64_t(blockIdx.x) * BLOCK + threadIdx.x;
   if (i >= nsample) return;
-  int64_t row = (i * n) / nsample;
+  // every row when the sample covers the input; otherwise a uniformly random
+  // subset (sample index -> row through a pseudo-random permutation of
+  // [0, n)): evenly strided rows would all fall in different runs of sorted
+  // or clustered keys and look like all-distinct keys, and rows drawn with
+  // replacement would repeat rows and look like repeated keys
+  int64_t row = i;
+  if (nsample < n) row = sample_row(i, n);
   // rows the group-by drops (null keys under EXCLUDE) are not part of the
   // population whose distinct keys are estimated
   if (P.skip_null_keys && any_key_null(P, row)) return;
@@ -1098,75 +1128,109 @@ __device__ __forceinline__ uint64_t segmented_scan(int32_t kind, uint64_t part, 
   return part;
 }
 
-__global__ void __launch_bounds__(BLOCK)
-  k_sorted_aggregate(gb_params const* __restrict__ Pp, uint32_t const* __restrict__ gid)
+// Accumulators one pass over the rows reduces: up to SORTED_FPP fields
+// (index -1: the group row count).
+constexpr int SORTED_FPP = 8;
+struct sorted_pass {
+  int32_t n;
+  int32_t field[SORTED_FPP];
+};
+
+__device__ __forceinline__ void sorted_item(gb_params const& P, int32_t f, int64_t g, int32_t& kind, int32_t& width,
+                                            void*& ptr)
 {
-  gb_params const& P = *Pp;
-  int const lane       = threadIdx.x & 63;
-  int64_t const nwaves = int64_t(gridDim.x) * (BLOCK / 64);
-  for (int64_t chunk = int64_t(blockIdx.x) * (BLOCK / 64) + (threadIdx.x >> 6); chunk * 64 < P.num_rows;
-       chunk += nwaves) {
-    int64_t const row = chunk * 64 + lane;
-    bool const live   = row < P.num_rows;
-    uint32_t const g  = live ? gid[row] : NO_GROUP;
-    uint32_t const gn = __shfl_down(g, 1, 64);
-    bool const last   = g != NO_GROUP && (lane == 63 || gn != g);
-    if (P.need_count_all) {
-      uint64_t const part = segmented_scan(FK_COUNT, g != NO_GROUP ? 1u : 0u, g, lane);
-      if (last) aadd(count_at(P, g), uint32_t(part));
-    }
-    for (int vc = 0; vc < P.num_vals; vc++) {
-      int const fb = P.val_field_begin[vc], fe = P.val_field_begin[vc + 1];
-      if (fb == fe) continue;
-      dcol const& c    = P.vals[vc];
-      bool const valid = g != NO_GROUP && dcol_valid(c, row);
-      loaded_value v{0, 0.0, 0.0f, 0};
-      if (valid) v = load_value(c, row);
-      for (int f = fb; f < fe; f++) {
-        field_desc const& F = P.fields[f];
-        if (F.kind == FK_M2) continue;
-        uint64_t const part = segmented_scan(F.kind, valid ? sorted_part(F.kind, v) : field_identity(F.kind), g, lane);
-        if (last) acc_combine(F.kind, field_at(F, g), part);
-      }
-    }
+  if (f < 0) {
+    kind  = FK_COUNT;
+    width = 4;
+    ptr   = count_at(P, g);
+  } else {
+    field_desc const& F = P.fields[f];
+    kind  = F.kind;
+    width = F.width;
+    ptr   = field_at(F, g);
   }
 }
 
+// A finished group total: stored when no other wave holds rows of the group,
+// combined atomically otherwise (the accumulators start at the identity).
+__device__ __forceinline__ void sorted_flush(gb_params const& P, int32_t f, uint32_t g, uint64_t part, bool shared)
+{
+  int32_t kind, width;
+  void* ptr;
+  sorted_item(P, f, g, kind, width, ptr);
+  if (shared) {
+    if (kind == FK_M2) gadd_f64(static_cast<double*>(ptr), __longlong_as_double(int64_t(part)));
+    else acc_combine(kind, ptr, part);
+  } else if (width == 8) {
+    *gp(static_cast<uint64_t*>(ptr)) = part;
+  } else {
+    *gp(static_cast<uint32_t*>(ptr)) = uint32_t(part);
+  }
+}
+
+// Row contribution of accumulator `f` (the value's square deviation from the
+// group mean for FK_M2).
+__device__ __forceinline__ uint64_t sorted_row_part(gb_params const& P, int32_t f, uint32_t g, int64_t row)
+{
+  if (f < 0) return g != NO_GROUP ? 1u : 0u;
+  field_desc const& F = P.fields[f];
+  dcol const& c       = P.vals[F.vcol];
+  if (g == NO_GROUP || !dcol_valid(c, row)) return F.kind == FK_M2 ? 0ull : field_identity(F.kind);
+  loaded_value const v = load_value(c, row);
+  if (F.kind != FK_M2) return sorted_part(F.kind, v);
+  double const cnt  = double(count_of(P, F.aux_count, g));
+  double const mean = field_as_double(P.fields[F.aux_sum], g) / cnt;
+  double const d    = v.d - mean;
+  return uint64_t(__double_as_longlong(d * d));
+}
+
+// Each wave reduces a contiguous range of rows in 64-row chunks; the group
+// still open at a chunk's last lane is carried into the next chunk.  Groups
+// that start and end inside the range are stored once; the range's first and
+// last groups may have rows in the neighbouring ranges and are combined
+// atomically.
 __global__ void __launch_bounds__(BLOCK)
-  k_sorted_second_pass(gb_params const* __restrict__ Pp, uint32_t const* __restrict__ gid)
+  k_sorted_reduce(gb_params const* __restrict__ Pp, uint32_t const* __restrict__ gid, int64_t per, int64_t W,
+                  sorted_pass pass)
 {
   gb_params const& P = *Pp;
-  int const lane       = threadIdx.x & 63;
-  int64_t const nwaves = int64_t(gridDim.x) * (BLOCK / 64);
-  for (int64_t chunk = int64_t(blockIdx.x) * (BLOCK / 64) + (threadIdx.x >> 6); chunk * 64 < P.num_rows;
-       chunk += nwaves) {
-    int64_t const row = chunk * 64 + lane;
-    bool const live   = row < P.num_rows;
+  int const lane     = threadIdx.x & 63;
+  int64_t const w    = int64_t(blockIdx.x) * (BLOCK / 64) + (threadIdx.x >> 6);
+  if (w >= W) return;
+  int64_t const b = w * per;
+  int64_t const e = b + per < P.num_rows ? b + per : P.num_rows;
+  uint32_t const first_g = gid[b];  // may continue from the previous range
+  uint64_t carry[SORTED_FPP];
+#pragma unroll
+  for (int k = 0; k < SORTED_FPP; k++) carry[k] = 0;
+  uint32_t carry_g = NO_GROUP;
+  for (int64_t i0 = b; i0 < e; i0 += 64) {
+    int64_t const row = i0 + lane;
+    bool const live   = row < e;
     uint32_t const g  = live ? gid[row] : NO_GROUP;
     uint32_t const gn = __shfl_down(g, 1, 64);
-    bool const last   = g != NO_GROUP && (lane == 63 || gn != g);
-    for (int vc = 0; vc < P.num_vals; vc++) {
-      int const fb = P.val_field_begin[vc], fe = P.val_field_begin[vc + 1];
-      bool has_m2  = false;
-      for (int f = fb; f < fe; f++) has_m2 |= (P.fields[f].kind == FK_M2);
-      if (!has_m2) continue;
-      dcol const& c    = P.vals[vc];
-      bool const valid = g != NO_GROUP && dcol_valid(c, row);
-      loaded_value v{0, 0.0, 0.0f, 0};
-      if (valid) v = load_value(c, row);
-      for (int f = fb; f < fe; f++) {
-        field_desc const& F = P.fields[f];
-        if (F.kind != FK_M2) continue;
-        double sq = 0.0;
-        if (valid) {
-          double const cnt  = double(count_of(P, F.aux_count, g));
-          double const mean = field_as_double(P.fields[F.aux_sum], g) / cnt;
-          double const d    = v.d - mean;
-          sq                = d * d;
-        }
-        uint64_t const part = segmented_scan(FK_M2, uint64_t(__double_as_longlong(sq)), g, lane);
-        if (last) gadd_f64(reinterpret_cast<double*>(field_at(F, g)), __longlong_as_double(int64_t(part)));
-      }
+    uint32_t const g0 = __shfl(g, 0, 64);
+    bool const carry_ends = carry_g != NO_GROUP && g0 != carry_g;
+    bool const ends_here  = g != NO_GROUP && lane < 63 && gn != g;
+#pragma unroll
+    for (int k = 0; k < SORTED_FPP; k++) {
+      if (k >= pass.n) break;
+      int32_t const f    = pass.field[k];
+      int32_t const kind = f < 0 ? int32_t(FK_COUNT) : P.fields[f].kind;
+      if (carry_ends && lane == 0) sorted_flush(P, f, carry_g, carry[k], carry_g == first_g);
+      uint64_t part = sorted_row_part(P, f, g, row);
+      if (lane == 0 && carry_g != NO_GROUP && g == carry_g) part = sorted_combine(kind, part, carry[k]);
+      part = segmented_scan(kind, part, g, lane);
+      if (ends_here) sorted_flush(P, f, g, part, g == first_g);
+      carry[k] = __shfl(part, 63, 64);
+    }
+    carry_g = __shfl(g, 63, 64);
+  }
+  if (carry_g != NO_GROUP && lane == 0) {
+#pragma unroll
+    for (int k = 0; k < SORTED_FPP; k++) {
+      if (k >= pass.n) break;
+      sorted_flush(P, pass.field[k], carry_g, carry[k], true);  // may continue in the next range
     }
   }
 }
@@ -1192,14 +1256,43 @@ void launch_sorted_assign(gb_params const* dP, int64_t n, uint32_t const* block_
 
 int64_t sorted_tiles(int64_t n) { return (n + SORTED_TILE - 1) / SORTED_TILE; }
 
-void launch_sorted_aggregate(gb_params const* dP, int64_t n, uint32_t const* gid, int num_cus, hipStream_t s)
+// Wave ranges: enough waves to fill the device, at least 64 rows each.
+static void sorted_ranges(int64_t n, int num_cus, int64_t& per, int64_t& W)
 {
-  hipLaunchKernelGGL(k_sorted_aggregate, dim3(row_grid(n, num_cus, 16)), dim3(BLOCK), 0, s, dP, gid);
+  int64_t const target = int64_t(num_cus) * 64;
+  per = ((n + target - 1) / target + 63) / 64 * 64;
+  if (per < 64) per = 64;
+  W = (n + per - 1) / per;
 }
 
-void launch_sorted_second_pass(gb_params const* dP, int64_t n, uint32_t const* gid, int num_cus, hipStream_t s)
+static void launch_sorted_passes(gb_params const* dP, gb_params const& hP, int64_t n, uint32_t const* gid,
+                                 int num_cus, bool m2, hipStream_t s)
 {
-  hipLaunchKernelGGL(k_sorted_second_pass, dim3(row_grid(n, num_cus, 16)), dim3(BLOCK), 0, s, dP, gid);
+  std::vector<int32_t> items;
+  if (!m2 && hP.need_count_all) items.push_back(-1);
+  for (int f = 0; f < hP.num_fields; f++)
+    if ((hP.fields[f].kind == FK_M2) == m2) items.push_back(f);
+  int64_t per, W;
+  sorted_ranges(n, num_cus, per, W);
+  unsigned const blocks = unsigned((W + BLOCK / 64 - 1) / (BLOCK / 64));
+  for (size_t i = 0; i < items.size(); i += SORTED_FPP) {
+    sorted_pass pass{};
+    pass.n = int32_t(std::min<size_t>(SORTED_FPP, items.size() - i));
+    for (int k = 0; k < pass.n; k++) pass.field[k] = items[i + k];
+    hipLaunchKernelGGL(k_sorted_reduce, dim3(blocks), dim3(BLOCK), 0, s, dP, gid, per, W, pass);
+  }
+}
+
+void launch_sorted_aggregate(gb_params const* dP, gb_params const& hP, int64_t n, uint32_t const* gid, int num_cus,
+                             hipStream_t s)
+{
+  launch_sorted_passes(dP, hP, n, gid, num_cus, false, s);
+}
+
+void launch_sorted_second_pass(gb_params const* dP, gb_params const& hP, int64_t n, uint32_t const* gid,
+                               int num_cus, hipStream_t s)
+{
+  launch_sorted_passes(dP, hP, n, gid, num_cus, true, s);
 }
 
 void launch_iota_u32(uint32_t* out, int64_t n, hipStream_t s)

@@ -247,8 +247,11 @@ void launch_sorted_count(gb_params const* dP, int64_t n, uint32_t* block_counts,
 // group of every row (0xFFFFFFFF when dropped) and slot_rows[g] = first row of group g
 void launch_sorted_assign(gb_params const* dP, int64_t n, uint32_t const* block_offsets, uint32_t* gid,
                           hipStream_t s);
-void launch_sorted_aggregate(gb_params const* dP, int64_t n, uint32_t const* gid, int num_cus, hipStream_t s);
-void launch_sorted_second_pass(gb_params const* dP, int64_t n, uint32_t const* gid, int num_cus, hipStream_t s);
+// wave-range reductions with plain stores for groups inside one range
+void launch_sorted_aggregate(gb_params const* dP, gb_params const& hP, int64_t n, uint32_t const* gid, int num_cus,
+                             hipStream_t s);
+void launch_sorted_second_pass(gb_params const* dP, gb_params const& hP, int64_t n, uint32_t const* gid,
+                               int num_cus, hipStream_t s);
 void launch_iota_u32(uint32_t* out, int64_t n, hipStream_t s);
 
 // ---- radix-partitioned aggregation (partition_kernels.hip) ------------------

@@ -311,11 +311,41 @@ def test_lds_preaggregation_variance_nulls_sentinel(policy):
 
 
 def test_lds_preaggregation_table_full_falls_back_to_global():
-    """The strided sample sees one key while the input has ~20k: the LDS
-    tables fill and the remaining keys go straight to the global table."""
+    """The sampled rows all hold one key while the input has ~20k: the LDS
+    tables fill and the remaining keys go straight to the global table.  The
+    sampled rows are the estimator's: sample index -> row through a 4-round
+    Feistel permutation of [0, n) (round function MurmurHash3's fmix32)."""
     n, ns = 400_000, 65536
     k = (np.arange(n, dtype=np.int64) % 20_000) + 1
-    k[(np.arange(ns, dtype=np.int64) * n) // ns] = 0
+    M = 0xFFFFFFFF
+
+    def fmix(h):
+        h ^= h >> 16
+        h = (h * 0x85EBCA6B) & M
+        h ^= h >> 13
+        h = (h * 0xC2B2AE35) & M
+        return h ^ (h >> 16)
+
+    bits = 2
+    while (1 << bits) < n:
+        bits += 2
+    half = bits // 2
+    mask = (1 << half) - 1
+
+    def row_of(i):
+        x = i
+        while True:
+            left, r = x >> half, x & mask
+            for rnd in range(4):
+                f = fmix((r * 0x9E3779B9 + rnd * 0x85EBCA6B + (r >> 32)) & M)
+                left, r = r, left ^ (f & mask)
+            x = (left << half) | r
+            if x < n:
+                return x
+
+    rows = np.array([row_of(i) for i in range(ns)], dtype=np.int64)
+    assert len(np.unique(rows)) == ns  # a permutation: no row twice
+    k[rows] = 0
     rng = np.random.default_rng(67)
     v = host_col(rng.integers(-100, 100, n).astype(np.int64))
     f = host_col(rng.standard_normal(n))
