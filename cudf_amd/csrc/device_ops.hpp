@@ -167,7 +167,7 @@ __device__ __forceinline__ bool any_key_null(gb_params const& P, int64_t row)
 }
 
 
-__device__ __forceinline__ uint64_t field_identity(int32_t kind)
+__host__ __device__ inline uint64_t field_identity(int32_t kind)
 {
   switch (kind) {
     case FK_MIN_S32: return 0x7fffffffull;

@@ -1116,25 +1116,154 @@ __device__ __forceinline__ uint64_t sorted_combine(int32_t kind, uint64_t a, uin
   }
 }
 
-// Inclusive scan of `part` over the lanes of each run of equal `g` (runs
-// are contiguous lanes because group numbers never decrease along rows).
-__device__ __forceinline__ uint64_t segmented_scan(int32_t kind, uint64_t part, uint32_t g, int lane)
+// Combining operations of the accumulator kinds (sums and counts add in their
+// type; min / max compare signed or unsigned: float min / max accumulators
+// hold order-preserving unsigned encodings).
+enum seg_op : int32_t {
+  OP_ADD_I64 = 0, OP_ADD_F32, OP_ADD_F64, OP_MIN_S32, OP_MAX_S32, OP_MIN_S64, OP_MAX_S64,
+  OP_MIN_U32, OP_MAX_U32, OP_MIN_U64, OP_MAX_U64
+};
+
+__host__ __device__ inline int32_t seg_op_of(int32_t kind)
 {
-  for (int o = 1; o < 64; o <<= 1) {
-    uint64_t const other = __shfl_up(part, o, 64);
-    uint32_t const og    = __shfl_up(g, o, 64);
-    if (lane >= o && og == g) part = sorted_combine(kind, part, other);
+  switch (kind) {
+    case FK_SUM_F32:
+    case FK_SUMSQ_F32: return OP_ADD_F32;
+    case FK_SUM_F64:
+    case FK_SUMSQ_F64:
+    case FK_M2: return OP_ADD_F64;
+    case FK_MIN_S32: return OP_MIN_S32;
+    case FK_MAX_S32: return OP_MAX_S32;
+    case FK_MIN_S64: return OP_MIN_S64;
+    case FK_MAX_S64: return OP_MAX_S64;
+    case FK_MIN_U32:
+    case FK_MIN_F32: return OP_MIN_U32;
+    case FK_MAX_U32:
+    case FK_MAX_F32: return OP_MAX_U32;
+    case FK_MIN_U64:
+    case FK_MIN_F64: return OP_MIN_U64;
+    case FK_MAX_U64:
+    case FK_MAX_F64: return OP_MAX_U64;
+    default: return OP_ADD_I64;  // integer sums, sums of squares, counts
   }
+}
+
+template <int OP>
+__device__ __forceinline__ uint64_t op_combine(uint64_t a, uint64_t b)
+{
+  if constexpr (OP == OP_ADD_I64) return a + b;
+  if constexpr (OP == OP_ADD_F32) return __float_as_uint(__uint_as_float(uint32_t(a)) + __uint_as_float(uint32_t(b)));
+  if constexpr (OP == OP_ADD_F64)
+    return uint64_t(__double_as_longlong(__longlong_as_double(int64_t(a)) + __longlong_as_double(int64_t(b))));
+  if constexpr (OP == OP_MIN_S32) return uint32_t(min(int32_t(uint32_t(a)), int32_t(uint32_t(b))));
+  if constexpr (OP == OP_MAX_S32) return uint32_t(max(int32_t(uint32_t(a)), int32_t(uint32_t(b))));
+  if constexpr (OP == OP_MIN_S64) return uint64_t(min(int64_t(a), int64_t(b)));
+  if constexpr (OP == OP_MAX_S64) return uint64_t(max(int64_t(a), int64_t(b)));
+  if constexpr (OP == OP_MIN_U32) return min(uint32_t(a), uint32_t(b));
+  if constexpr (OP == OP_MAX_U32) return max(uint32_t(a), uint32_t(b));
+  if constexpr (OP == OP_MIN_U64) return min(a, b);
+  if constexpr (OP == OP_MAX_U64) return max(a, b);
+  return a;
+}
+
+// One step of the segmented scan: lanes take the partial of the lane the DPP
+// control names (row shift within 16-lane rows, or a broadcast of a row's last
+// lane to the next rows) and combine it when that lane holds the same group.
+// Lanes without a source keep `old`, whose group value matches no group.
+template <int OP, int CTRL, int ROW_MASK>
+__device__ __forceinline__ void seg_step(uint64_t& part, uint32_t g)
+{
+  constexpr int NOT_A_GROUP = int(0xFFFFFFFEu);
+  uint32_t const olo = uint32_t(__builtin_amdgcn_update_dpp(0, int(uint32_t(part)), CTRL, ROW_MASK, 0xF, false));
+  uint32_t const ohi = uint32_t(__builtin_amdgcn_update_dpp(0, int(uint32_t(part >> 32)), CTRL, ROW_MASK, 0xF, false));
+  uint32_t const og  = uint32_t(__builtin_amdgcn_update_dpp(NOT_A_GROUP, int(g), CTRL, ROW_MASK, 0xF, false));
+  if (og == g) part = op_combine<OP>(part, (uint64_t(ohi) << 32) | olo);
+}
+
+// Inclusive scan of `part` over the lanes of each run of equal `g` (runs
+// are contiguous lanes because group numbers never decrease along rows):
+// row shifts by 1, 2, 4, 8 inside each 16-lane row, then row 0's and row 2's
+// last lanes into rows 1 and 3, then lane 31 into rows 2 and 3.  A lane only
+// combines a partial of its own group, and groups are contiguous, so every
+// lane between the source and the lane belongs to the same group.
+template <int OP>
+__device__ __forceinline__ uint64_t seg_scan(uint64_t part, uint32_t g)
+{
+  seg_step<OP, 0x111, 0xF>(part, g);  // row_shr:1
+  seg_step<OP, 0x112, 0xF>(part, g);  // row_shr:2
+  seg_step<OP, 0x114, 0xF>(part, g);  // row_shr:4
+  seg_step<OP, 0x118, 0xF>(part, g);  // row_shr:8
+  seg_step<OP, 0x142, 0xA>(part, g);  // row_bcast:15
+  seg_step<OP, 0x143, 0xC>(part, g);  // row_bcast:31
   return part;
 }
 
+// One dispatch on the combining operation per scan.
+__device__ __forceinline__ uint64_t segmented_scan(int32_t kind, uint64_t part, uint32_t g, int)
+{
+  switch (seg_op_of(kind)) {
+    case OP_ADD_F32: return seg_scan<OP_ADD_F32>(part, g);
+    case OP_ADD_F64: return seg_scan<OP_ADD_F64>(part, g);
+    case OP_MIN_S32: return seg_scan<OP_MIN_S32>(part, g);
+    case OP_MAX_S32: return seg_scan<OP_MAX_S32>(part, g);
+    case OP_MIN_S64: return seg_scan<OP_MIN_S64>(part, g);
+    case OP_MAX_S64: return seg_scan<OP_MAX_S64>(part, g);
+    case OP_MIN_U32: return seg_scan<OP_MIN_U32>(part, g);
+    case OP_MAX_U32: return seg_scan<OP_MAX_U32>(part, g);
+    case OP_MIN_U64: return seg_scan<OP_MIN_U64>(part, g);
+    case OP_MAX_U64: return seg_scan<OP_MAX_U64>(part, g);
+    default: return seg_scan<OP_ADD_I64>(part, g);
+  }
+}
+
 // Accumulators one pass over the rows reduces: up to SORTED_FPP fields
-// (index -1: the group row count).
-constexpr int SORTED_FPP = 8;
+// (index -1: the group row count) reading up to SORTED_COLS value columns.
+constexpr int SORTED_FPP  = 8;
+constexpr int SORTED_COLS = 4;
 struct sorted_pass {
   int32_t n;
+  int32_t ncols;
   int32_t field[SORTED_FPP];
+  int32_t col_of[SORTED_FPP];   // loaded column slot of each field (-1: the row count)
+  int32_t kind[SORTED_FPP];     // accumulator kind (FK_COUNT for the row count)
+  int32_t op[SORTED_FPP];       // combining operation (seg_op)
+  uint64_t ident[SORTED_FPP];   // identity in the accumulator encoding
+  int32_t col[SORTED_COLS];     // value column (index into P.vals) of each slot
 };
+
+// Element bits of a fixed-width column as stored (no float normalisation).
+__device__ __forceinline__ uint64_t raw_bits(dcol const& c, int64_t row)
+{
+  auto const d = gp(c.data);
+  switch (storage_width(c.storage)) {
+    case 1: return d[row];
+    case 2: return reinterpret_cast<gmem<uint16_t const>*>(d)[row];
+    case 4: return reinterpret_cast<gmem<uint32_t const>*>(d)[row];
+    default: return reinterpret_cast<gmem<uint64_t const>*>(d)[row];
+  }
+}
+
+// The loaded_value of element bits (as load_value builds it from memory).
+__device__ __forceinline__ loaded_value value_of_bits(int32_t st, uint64_t u)
+{
+  loaded_value v{0, 0.0, 0.0f, 0};
+  switch (st) {
+    case ST_I8: v.i = int8_t(u); break;
+    case ST_I16: v.i = int16_t(u); break;
+    case ST_I32: v.i = int32_t(u); break;
+    case ST_I64: v.i = int64_t(u); break;
+    case ST_BOOL: v.i = (u & 0xff) != 0; break;
+    case ST_U8: v.i = uint8_t(u); break;
+    case ST_U16: v.i = uint16_t(u); break;
+    case ST_U32: v.i = uint32_t(u); break;
+    case ST_U64: v.i = int64_t(u); v.d = double(u); return v;
+    case ST_F32: v.b = uint32_t(u); v.f = __uint_as_float(uint32_t(u)); v.d = double(v.f); return v;
+    case ST_F64: v.b = u; v.d = __longlong_as_double(int64_t(u)); return v;
+    default: return v;
+  }
+  v.d = double(v.i);
+  return v;
+}
 
 __device__ __forceinline__ void sorted_item(gb_params const& P, int32_t f, int64_t g, int32_t& kind, int32_t& width,
                                             void*& ptr)
@@ -1168,20 +1297,38 @@ __device__ __forceinline__ void sorted_flush(gb_params const& P, int32_t f, uint
   }
 }
 
-// Row contribution of accumulator `f` (the value's square deviation from the
-// group mean for FK_M2).
-__device__ __forceinline__ uint64_t sorted_row_part(gb_params const& P, int32_t f, uint32_t g, int64_t row)
+// Row contribution of accumulator `f` (kind `kind`, identity `ident`) from
+// the row's loaded value (the value's square deviation from the group mean
+// for FK_M2).
+__device__ __forceinline__ uint64_t sorted_row_part(gb_params const& P, int32_t f, int32_t kind, uint64_t ident,
+                                                    uint32_t g, bool valid, loaded_value const& v)
 {
   if (f < 0) return g != NO_GROUP ? 1u : 0u;
+  if (g == NO_GROUP || !valid) return ident;
+  if (kind != FK_M2) return sorted_part(kind, v);
   field_desc const& F = P.fields[f];
-  dcol const& c       = P.vals[F.vcol];
-  if (g == NO_GROUP || !dcol_valid(c, row)) return F.kind == FK_M2 ? 0ull : field_identity(F.kind);
-  loaded_value const v = load_value(c, row);
-  if (F.kind != FK_M2) return sorted_part(F.kind, v);
-  double const cnt  = double(count_of(P, F.aux_count, g));
-  double const mean = field_as_double(P.fields[F.aux_sum], g) / cnt;
-  double const d    = v.d - mean;
+  double const cnt    = double(count_of(P, F.aux_count, g));
+  double const mean   = field_as_double(P.fields[F.aux_sum], g) / cnt;
+  double const d      = v.d - mean;
   return uint64_t(__double_as_longlong(d * d));
+}
+
+// Segmented scan with the combining operation chosen once per call.
+__device__ __forceinline__ uint64_t segmented_scan_op(int32_t op, uint64_t part, uint32_t g)
+{
+  switch (op) {
+    case OP_ADD_F32: return seg_scan<OP_ADD_F32>(part, g);
+    case OP_ADD_F64: return seg_scan<OP_ADD_F64>(part, g);
+    case OP_MIN_S32: return seg_scan<OP_MIN_S32>(part, g);
+    case OP_MAX_S32: return seg_scan<OP_MAX_S32>(part, g);
+    case OP_MIN_S64: return seg_scan<OP_MIN_S64>(part, g);
+    case OP_MAX_S64: return seg_scan<OP_MAX_S64>(part, g);
+    case OP_MIN_U32: return seg_scan<OP_MIN_U32>(part, g);
+    case OP_MAX_U32: return seg_scan<OP_MAX_U32>(part, g);
+    case OP_MIN_U64: return seg_scan<OP_MIN_U64>(part, g);
+    case OP_MAX_U64: return seg_scan<OP_MAX_U64>(part, g);
+    default: return seg_scan<OP_ADD_I64>(part, g);
+  }
 }
 
 // Each wave reduces a contiguous range of rows in 64-row chunks; the group
@@ -1193,45 +1340,70 @@ __global__ void __launch_bounds__(BLOCK)
   k_sorted_reduce(gb_params const* __restrict__ Pp, uint32_t const* __restrict__ gid, int64_t per, int64_t W,
                   sorted_pass pass)
 {
+  // the partials carried between chunks live in LDS (one row per wave): the
+  // field loop stays rolled, which keeps the kernel small enough for the
+  // instruction cache
+  __shared__ uint64_t carry[BLOCK / 64][SORTED_FPP];
   gb_params const& P = *Pp;
   int const lane     = threadIdx.x & 63;
-  int64_t const w    = int64_t(blockIdx.x) * (BLOCK / 64) + (threadIdx.x >> 6);
+  int const wave     = threadIdx.x >> 6;
+  int64_t const w    = int64_t(blockIdx.x) * (BLOCK / 64) + wave;
   if (w >= W) return;
   int64_t const b = w * per;
   int64_t const e = b + per < P.num_rows ? b + per : P.num_rows;
   uint32_t const first_g = gid[b];  // may continue from the previous range
-  uint64_t carry[SORTED_FPP];
-#pragma unroll
-  for (int k = 0; k < SORTED_FPP; k++) carry[k] = 0;
-  uint32_t carry_g = NO_GROUP;
+  uint32_t carry_g       = NO_GROUP;
   for (int64_t i0 = b; i0 < e; i0 += 64) {
     int64_t const row = i0 + lane;
     bool const live   = row < e;
-    uint32_t const g  = live ? gid[row] : NO_GROUP;
-    uint32_t const gn = __shfl_down(g, 1, 64);
-    uint32_t const g0 = __shfl(g, 0, 64);
+    int64_t const rc  = live ? row : e - 1;  // every load of the chunk issues before any use
+    uint32_t const gl = gid[rc];
+    uint64_t bits[SORTED_COLS];
+    bool vld[SORTED_COLS];
+#pragma unroll
+    for (int c = 0; c < SORTED_COLS; c++) {
+      bits[c] = 0;
+      vld[c]  = false;
+      if (c < pass.ncols) {
+        dcol const& d = P.vals[pass.col[c]];
+        bits[c]       = raw_bits(d, rc);
+        vld[c]        = dcol_valid(d, rc);
+      }
+    }
+    // each column's value decoded once per chunk
+    loaded_value lv[SORTED_COLS];
+#pragma unroll
+    for (int c = 0; c < SORTED_COLS; c++)
+      lv[c] = c < pass.ncols ? value_of_bits(P.vals[pass.col[c]].storage, bits[c]) : loaded_value{0, 0.0, 0.0f, 0};
+    uint32_t const g  = live ? gl : NO_GROUP;
+    // group of the next lane (wave_shl:1; the last lane gets no group)
+    uint32_t const gn = uint32_t(__builtin_amdgcn_update_dpp(int(NO_GROUP), int(g), 0x130, 0xF, 0xF, false));
+    uint32_t const g0 = uint32_t(__builtin_amdgcn_readlane(int(g), 0));
     bool const carry_ends = carry_g != NO_GROUP && g0 != carry_g;
     bool const ends_here  = g != NO_GROUP && lane < 63 && gn != g;
-#pragma unroll
-    for (int k = 0; k < SORTED_FPP; k++) {
-      if (k >= pass.n) break;
+#pragma unroll 1
+    for (int k = 0; k < pass.n; k++) {
       int32_t const f    = pass.field[k];
-      int32_t const kind = f < 0 ? int32_t(FK_COUNT) : P.fields[f].kind;
-      if (carry_ends && lane == 0) sorted_flush(P, f, carry_g, carry[k], carry_g == first_g);
-      uint64_t part = sorted_row_part(P, f, g, row);
-      if (lane == 0 && carry_g != NO_GROUP && g == carry_g) part = sorted_combine(kind, part, carry[k]);
-      part = segmented_scan(kind, part, g, lane);
+      int32_t const cs   = pass.col_of[k];
+      int32_t const kind = pass.kind[k];
+      uint64_t const c   = carry[wave][k];
+      if (carry_ends && lane == 0) sorted_flush(P, f, carry_g, c, carry_g == first_g);
+      // the field's column among the loaded ones (uniform select, no indexing)
+      loaded_value const v  = cs <= 0 ? lv[0] : cs == 1 ? lv[1] : cs == 2 ? lv[2] : lv[3];
+      bool const vv         = cs <= 0 ? vld[0] : cs == 1 ? vld[1] : cs == 2 ? vld[2] : vld[3];
+      uint64_t part         = sorted_row_part(P, f, kind, pass.ident[k], g, vv, v);
+      if (lane == 0 && carry_g != NO_GROUP && g == carry_g) part = sorted_combine(kind, part, c);
+      part = segmented_scan_op(pass.op[k], part, g);
       if (ends_here) sorted_flush(P, f, g, part, g == first_g);
-      carry[k] = __shfl(part, 63, 64);
+      uint64_t const last = (uint64_t(uint32_t(__builtin_amdgcn_readlane(int(uint32_t(part >> 32)), 63))) << 32) |
+                            uint32_t(__builtin_amdgcn_readlane(int(uint32_t(part)), 63));
+      if (lane == 0) carry[wave][k] = last;
     }
-    carry_g = __shfl(g, 63, 64);
+    carry_g = uint32_t(__builtin_amdgcn_readlane(int(g), 63));
   }
   if (carry_g != NO_GROUP && lane == 0) {
-#pragma unroll
-    for (int k = 0; k < SORTED_FPP; k++) {
-      if (k >= pass.n) break;
-      sorted_flush(P, pass.field[k], carry_g, carry[k], true);  // may continue in the next range
-    }
+    for (int k = 0; k < pass.n; k++)
+      sorted_flush(P, pass.field[k], carry_g, carry[wave][k], true);  // may continue in the next range
   }
 }
 
@@ -1275,10 +1447,31 @@ static void launch_sorted_passes(gb_params const* dP, gb_params const& hP, int64
   int64_t per, W;
   sorted_ranges(n, num_cus, per, W);
   unsigned const blocks = unsigned((W + BLOCK / 64 - 1) / (BLOCK / 64));
-  for (size_t i = 0; i < items.size(); i += SORTED_FPP) {
+  size_t i = 0;
+  while (i < items.size()) {
     sorted_pass pass{};
-    pass.n = int32_t(std::min<size_t>(SORTED_FPP, items.size() - i));
-    for (int k = 0; k < pass.n; k++) pass.field[k] = items[i + k];
+    while (i < items.size() && pass.n < SORTED_FPP) {
+      int32_t const f = items[i];
+      int32_t slot    = -1;
+      if (f >= 0) {
+        int32_t const vc = hP.fields[f].vcol;
+        for (int c = 0; c < pass.ncols; c++)
+          if (pass.col[c] == vc) slot = c;
+        if (slot < 0) {
+          if (pass.ncols == SORTED_COLS) break;  // next pass
+          slot                  = pass.ncols;
+          pass.col[pass.ncols++] = vc;
+        }
+      }
+      int32_t const kind  = f < 0 ? int32_t(FK_COUNT) : hP.fields[f].kind;
+      pass.field[pass.n]  = f;
+      pass.col_of[pass.n] = slot;
+      pass.kind[pass.n]   = kind;
+      pass.op[pass.n]     = seg_op_of(kind);
+      pass.ident[pass.n]  = kind == FK_M2 ? 0ull : field_identity(kind);
+      pass.n++;
+      i++;
+    }
     hipLaunchKernelGGL(k_sorted_reduce, dim3(blocks), dim3(BLOCK), 0, s, dP, gid, per, W, pass);
   }
 }
