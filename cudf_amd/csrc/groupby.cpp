@@ -967,7 +967,10 @@ std::unique_ptr<mig_groupby_result> groupby_aggregate(mig_column_view const* key
     int flag_bit = 0;
     for (int v = 0; v < P.num_vals; v++)
       R.val_flag_bit[v] = (P.val_field_begin[v] < P.val_field_begin[v + 1] && vals[v].has_nulls) ? flag_bit++ : -1;
-    double const per_bin = 0.6 * double(S_slots);
+    // bins average 0.6·S groups; the extra 10% absorbs the sampling error of
+    // the estimate (about 7% at 10M groups from 65,536 sampled rows), so an
+    // estimate a little high does not double the bins
+    double const per_bin = 0.66 * double(S_slots);
     int log_bins = 1;
     while (log_bins < RPA_MAX_LOG_BINS && double(int64_t(1) << log_bins) * per_bin < est) log_bins++;
     for (int attempt = 0; attempt < 3 && G < 0; attempt++) {
