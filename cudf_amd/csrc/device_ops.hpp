@@ -180,8 +180,15 @@ __host__ __device__ inline uint64_t field_identity(int32_t kind)
     case FK_MAX_F32: return 0x007fffffull;              // ord32(-inf)
     case FK_MIN_F64: return 0xfff0000000000000ull;      // ord64(+inf)
     case FK_MAX_F64: return 0x000fffffffffffffull;      // ord64(-inf)
+    case FK_ARGMIN:
+    case FK_ARGMAX: return 0xffffffffull;               // no row yet
     default: return 0;                                  // sums, counts, max of unsigned
   }
+}
+
+__device__ __forceinline__ bool is_unsigned_storage_dev(int32_t st)
+{
+  return st == ST_U8 || st == ST_U16 || st == ST_U32 || st == ST_U64 || st == ST_BOOL;
 }
 
 struct loaded_value {

@@ -322,7 +322,9 @@ int32_t target_type_of(int32_t src, int32_t kind)
     case MIG_AGG_MIN:
     case MIG_AGG_MAX: return src;
     case MIG_AGG_COUNT_VALID:
-    case MIG_AGG_COUNT_ALL: return MIG_TYPE_INT32;
+    case MIG_AGG_COUNT_ALL:
+    case MIG_AGG_ARGMIN:  // size_type row index (aggregation.hpp:1346-1357)
+    case MIG_AGG_ARGMAX: return MIG_TYPE_INT32;
     case MIG_AGG_MEAN:
     case MIG_AGG_VARIANCE:
     case MIG_AGG_STD: return MIG_TYPE_FLOAT64;
@@ -343,7 +345,9 @@ void check_supported(int32_t src, int32_t kind)
     case MIG_AGG_COUNT_VALID:
     case MIG_AGG_COUNT_ALL: ok = src != MIG_TYPE_EMPTY; break;
     case MIG_AGG_MIN:
-    case MIG_AGG_MAX: ok = is_numeric_type(src) || is_chrono_type(src); break;
+    case MIG_AGG_MAX:
+    case MIG_AGG_ARGMIN:
+    case MIG_AGG_ARGMAX: ok = is_numeric_type(src) || is_chrono_type(src); break;
     case MIG_AGG_SUM:
     case MIG_AGG_SUM_OF_SQUARES:
     case MIG_AGG_MEAN:
@@ -364,8 +368,9 @@ struct value_plan {
   int32_t storage   = ST_NONE;
   bool has_nulls    = false;
   bool want_sum = false, want_sumsq = false, want_min = false, want_max = false;
-  bool want_count = false, want_m2 = false;
+  bool want_count = false, want_m2 = false, want_argmin = false, want_argmax = false;
   int32_t f_sum = -1, f_sumsq = -1, f_min = -1, f_max = -1, f_count = -1, f_m2 = -1;
+  int32_t f_argmin = -1, f_argmax = -1;
 };
 
 int32_t sum_field_kind(int32_t st)
@@ -407,7 +412,9 @@ int32_t field_width(int32_t fk)
     case FK_MAX_U32:
     case FK_MIN_F32:
     case FK_MAX_F32:
-    case FK_COUNT: return 4;
+    case FK_COUNT:
+    case FK_ARGMIN:
+    case FK_ARGMAX: return 4;
     default: return 8;
   }
 }
@@ -596,6 +603,8 @@ std::unique_ptr<mig_groupby_result> groupby_aggregate(mig_column_view const* key
         case MIG_AGG_MEAN: vp.want_sum = vp.want_count = true; break;
         case MIG_AGG_VARIANCE:
         case MIG_AGG_STD: vp.want_sum = vp.want_count = vp.want_m2 = true; break;
+        case MIG_AGG_ARGMIN: vp.want_argmin = true; break;
+        case MIG_AGG_ARGMAX: vp.want_argmax = true; break;
         default: break;
       }
     }
@@ -631,6 +640,8 @@ std::unique_ptr<mig_groupby_result> groupby_aggregate(mig_column_view const* key
         P.need_count_all = 1;
       }
     }
+    if (vp.want_argmin) vp.f_argmin = add_field(int(j), FK_ARGMIN);
+    if (vp.want_argmax) vp.f_argmax = add_field(int(j), FK_ARGMAX);
     if (vp.want_m2) {
       vp.f_m2                   = add_field(int(j), FK_M2);
       P.fields[vp.f_m2].aux_sum   = vp.f_sum;
@@ -641,6 +652,10 @@ std::unique_ptr<mig_groupby_result> groupby_aggregate(mig_column_view const* key
   P.val_field_begin[vals.size()] = nf;
   P.num_vals   = int32_t(vals.size());
   P.num_fields = nf;
+  // ARGMIN / ARGMAX update a row index by compare-and-swap against the
+  // values column: only the device-wide table path implements them
+  bool any_arg = false;
+  for (int f = 0; f < nf; f++) any_arg |= P.fields[f].kind == FK_ARGMIN || P.fields[f].kind == FK_ARGMAX;
 
   // ---- cardinality estimate ------------------------------------------------
   scratch S(stream);
@@ -753,6 +768,12 @@ std::unique_ptr<mig_groupby_result> groupby_aggregate(mig_column_view const* key
             O.count_field = vp.f_count;
             with_mask     = vp.has_nulls;
             break;
+          case MIG_AGG_ARGMIN:
+          case MIG_AGG_ARGMAX:
+            O.kind    = OK_ARG;
+            O.field   = kind == MIG_AGG_ARGMIN ? vp.f_argmin : vp.f_argmax;
+            with_mask = vp.has_nulls;
+            break;
           case MIG_AGG_VARIANCE:
           case MIG_AGG_STD:
             O.kind        = kind == MIG_AGG_VARIANCE ? OK_VAR : OK_STD;
@@ -824,6 +845,7 @@ std::unique_ptr<mig_groupby_result> groupby_aggregate(mig_column_view const* key
   // (the reference's sort path with keys_pre_sorted, sort/sort_helper.cu:55-72,
   // 133-181); groups come out in input order.
   if (keys_sorted) {
+    MIG_EXPECTS(!any_arg, "ARGMIN / ARGMAX are not supported with pre-sorted keys");
     P.key_mode            = KM_GENERAL;  // keys are gathered from each group's first row
     int64_t const ntiles  = sorted_tiles(N);
     uint32_t* tile_counts = S.get<uint32_t>(ntiles + 1);
@@ -874,7 +896,7 @@ std::unique_ptr<mig_groupby_result> groupby_aggregate(mig_column_view const* key
 
   // ---- strategy: radix-partitioned LDS aggregation for high cardinality -----
   int const forced = g_strategy.load();
-  bool rpa_ok      = G < 0 && (P.key_mode == KM_INLINE || P.key_mode == KM_GENERAL);
+  bool rpa_ok      = G < 0 && !any_arg && (P.key_mode == KM_INLINE || P.key_mode == KM_GENERAL);
   int n_flagged    = 0;
   for (int v = 0; v < P.num_vals && rpa_ok; v++) {
     if (P.val_field_begin[v] == P.val_field_begin[v + 1]) continue;
@@ -1067,7 +1089,8 @@ std::unique_ptr<mig_groupby_result> groupby_aggregate(mig_column_view const* key
     // group and field instead of one per row and field).
     int64_t lds_blocks = 0;
     P.lds_bytes        = 0;
-    if (P.key_mode == KM_INLINE && est <= double(LDS_PREAGG_MAX_GROUPS) && forced != MIG_STRATEGY_PARTITIONED) {
+    if (P.key_mode == KM_INLINE && !any_arg && est <= double(LDS_PREAGG_MAX_GROUPS) &&
+        forced != MIG_STRATEGY_PARTITIONED) {
       int32_t S_l = 256;
       while (S_l < LDS_PREAGG_SLOTS && double(S_l) * 0.75 < est * 1.5) S_l *= 2;
       int32_t off = S_l * 8, m2_off = S_l * 8;

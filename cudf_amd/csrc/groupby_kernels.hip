@@ -242,6 +242,41 @@ __device__ __forceinline__ int64_t locate(gb_params const& P, int64_t row)
 // ---------------------------------------------------------------------------
 // accumulate one row into the group's accumulators (apply_field: device_ops.hpp)
 
+// ARGMIN / ARGMAX: is row r's value a better choice than row o's?  The
+// reference keeps the first of equal values to reach the target
+// (aggregation.cuh:440-490); here equal values take the smaller row, and a
+// NaN only wins when both are NaN (NaN compares false in the reference, so a
+// number there is never replaced by a NaN), which makes the result
+// deterministic.
+__device__ __forceinline__ bool arg_better(dcol const& c, int32_t kind, int64_t r, int64_t o)
+{
+  loaded_value const a = load_value(c, r), b = load_value(c, o);
+  int cmp = 0;
+  if (c.storage == ST_F32 || c.storage == ST_F64) {
+    bool const an = isnan(a.d), bn = isnan(b.d);
+    if (an != bn) return bn;  // a number beats a NaN
+    if (!an) cmp = a.d < b.d ? -1 : (a.d > b.d ? 1 : 0);
+  } else if (is_unsigned_storage_dev(c.storage)) {
+    cmp = uint64_t(a.i) < uint64_t(b.i) ? -1 : (uint64_t(a.i) > uint64_t(b.i) ? 1 : 0);
+  } else {
+    cmp = a.i < b.i ? -1 : (a.i > b.i ? 1 : 0);
+  }
+  if (cmp != 0) return kind == FK_ARGMIN ? cmp < 0 : cmp > 0;
+  return r < o;
+}
+
+// Compare-and-swap loop on the group's row index (aggregation.cuh:440-490).
+__device__ __forceinline__ void arg_update(field_desc const& F, int64_t slot, dcol const& c, int64_t row)
+{
+  auto* a      = reinterpret_cast<unsigned int*>(field_at(F, slot));
+  uint32_t cur = __hip_atomic_load(a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  while (cur == 0xFFFFFFFFu || arg_better(c, F.kind, row, int64_t(cur))) {
+    uint32_t const prev = atomicCAS(a, cur, uint32_t(row));
+    if (prev == cur) break;
+    cur = prev;
+  }
+}
+
 __device__ __forceinline__ void update_row(gb_params const& P, int64_t slot, int64_t row)
 {
   if (P.need_count_all) aadd(count_at(P, slot), 1u);
@@ -251,7 +286,11 @@ __device__ __forceinline__ void update_row(gb_params const& P, int64_t slot, int
     dcol const& c = P.vals[vc];
     if (!dcol_valid(c, row)) continue;
     loaded_value const v = load_value(c, row);
-    for (int f = fb; f < fe; f++) apply_field(P.fields[f], slot, v);
+    for (int f = fb; f < fe; f++) {
+      field_desc const& F = P.fields[f];
+      if (F.kind == FK_ARGMIN || F.kind == FK_ARGMAX) arg_update(F, slot, c, row);
+      else apply_field(F, slot, v);
+    }
   }
 }
 
@@ -717,6 +756,12 @@ __global__ void __launch_bounds__(BLOCK)
         case OK_MEAN: {
           double const sum = field_as_double(P.fields[O.field], slot);
           reinterpret_cast<double*>(O.data)[g] = valid ? sum / double(cnt) : 0.0;
+          break;
+        }
+        case OK_ARG: {
+          uint32_t const r = *reinterpret_cast<uint32_t const*>(field_at(P.fields[O.field], slot));
+          valid            = r != 0xFFFFFFFFu;
+          reinterpret_cast<int32_t*>(O.data)[g] = valid ? int32_t(r) : 0;
           break;
         }
         case OK_VAR:

@@ -48,7 +48,9 @@ enum field_kind : int32_t {
   FK_MIN_F64,
   FK_MAX_F64,
   FK_COUNT,  // valid values (u32)
-  FK_M2      // sum of squared deviations from the group mean (f64, second pass)
+  FK_M2,     // sum of squared deviations from the group mean (f64, second pass)
+  FK_ARGMIN, // row of the smallest valid value (u32, 0xFFFFFFFF = none)
+  FK_ARGMAX  // row of the largest valid value
 };
 
 struct field_desc {
@@ -72,7 +74,8 @@ enum out_kind : int32_t {
   OK_MINMAX,     // decode a min/max field into the source type
   OK_MEAN,       // double(sum) / count
   OK_VAR,        // m2 / (count - ddof)
-  OK_STD         // sqrt(var)
+  OK_STD,        // sqrt(var)
+  OK_ARG         // int32 row index from an ARGMIN / ARGMAX field, null when none
 };
 
 struct out_desc {

@@ -79,6 +79,16 @@ std::unique_ptr<Base> make_sum_of_squares_aggregation()
   return std::make_unique<Base>(aggregation::SUM_OF_SQUARES);
 }
 template <typename Base = aggregation>
+std::unique_ptr<Base> make_argmax_aggregation()
+{
+  return std::make_unique<Base>(aggregation::ARGMAX);
+}
+template <typename Base = aggregation>
+std::unique_ptr<Base> make_argmin_aggregation()
+{
+  return std::make_unique<Base>(aggregation::ARGMIN);
+}
+template <typename Base = aggregation>
 std::unique_ptr<Base> make_mean_aggregation()
 {
   return std::make_unique<Base>(aggregation::MEAN);

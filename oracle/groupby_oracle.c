@@ -30,7 +30,7 @@ enum {
 /* aggregation kinds as in cudf/aggregation.hpp:86-122 */
 enum {
   A_SUM = 0, A_PRODUCT = 1, A_MIN = 2, A_MAX = 3, A_COUNT_VALID = 4, A_COUNT_ALL = 5,
-  A_SUM_OF_SQUARES = 8, A_MEAN = 9, A_VARIANCE = 11, A_STD = 12
+  A_SUM_OF_SQUARES = 8, A_MEAN = 9, A_VARIANCE = 11, A_STD = 12, A_ARGMAX = 15, A_ARGMIN = 16
 };
 
 typedef struct orc_col {
@@ -220,6 +220,7 @@ int32_t orc_target_type(int32_t src, int32_t kind)
   switch (kind) {
     case A_MIN: case A_MAX: return src;
     case A_COUNT_VALID: case A_COUNT_ALL: return T_INT32;
+    case A_ARGMIN: case A_ARGMAX: return T_INT32; /* size_type row index, aggregation.hpp:1346-1357 */
     case A_MEAN: case A_VARIANCE: case A_STD: return T_FLOAT64;
     case A_SUM: case A_SUM_OF_SQUARES: case A_PRODUCT:
       if (is_float(src)) return src;
@@ -231,6 +232,35 @@ int32_t orc_target_type(int32_t src, int32_t kind)
       return -1;
   }
   return -1;
+}
+
+/* Order of two valid values of a column for ARGMIN / ARGMAX: <0, 0, >0.
+ * Integers by value, strings bytewise (then by length), floats by value with
+ * NaN after every number (a NaN row never replaces a number). */
+static int value_cmp(const orc_col* c, int64_t a, int64_t b)
+{
+  int t = c->type_id;
+  if (t == T_STRING) {
+    int64_t la = c->offsets[a + 1] - c->offsets[a], lb = c->offsets[b + 1] - c->offsets[b];
+    const uint8_t* pa = (const uint8_t*)c->data + c->offsets[a];
+    const uint8_t* pb = (const uint8_t*)c->data + c->offsets[b];
+    int64_t m = la < lb ? la : lb;
+    for (int64_t i = 0; i < m; i++)
+      if (pa[i] != pb[i]) return pa[i] < pb[i] ? -1 : 1;
+    return la < lb ? -1 : (la > lb ? 1 : 0);
+  }
+  if (is_float(t)) {
+    long double x = as_ld(c, a), y = as_ld(c, b);
+    int nx = isnan(x), ny = isnan(y);
+    if (nx || ny) return nx == ny ? 0 : (nx ? 1 : -1);
+    return x < y ? -1 : (x > y ? 1 : 0);
+  }
+  if (is_unsigned_int(t)) {
+    uint64_t x = (uint64_t)as_i64(c, a), y = (uint64_t)as_i64(c, b);
+    return x < y ? -1 : (x > y ? 1 : 0);
+  }
+  int64_t x = as_i64(c, a), y = as_i64(c, b);
+  return x < y ? -1 : (x > y ? 1 : 0);
 }
 
 /* ---- open-addressing map row -> group, first-occurrence group order ---- */
@@ -252,7 +282,13 @@ typedef struct { int64_t* slots; int64_t cap; } omap;
  *   - MEAN = double(sum) / count (hash/groupby.cu:287-306, binary_ops DIV in
  *     double);
  *   - VARIANCE = sum((x - mean)^2) / (n - ddof), null when n - ddof <= 0,
- *     two passes in long double (multi_pass_kernels.cuh:74-95).
+ *     two passes in long double (multi_pass_kernels.cuh:74-95);
+ *   - ARGMIN / ARGMAX = row index of the smallest / largest valid value,
+ *     null when the group has none (aggregation.cuh:440-490).  The reference
+ *     keeps whichever of equal values reaches the target first; this
+ *     restatement takes the first row in input order, and for floats a NaN
+ *     only when every value of the group is NaN (NaN compares false, so a
+ *     number is never replaced by a NaN).
  * Outputs: `rep_rows[g]` = first row of group g; `outs[a]` = array of G
  * elements of the target type; `out_valid[a]` = G validity bytes.
  * Returns G, or -1 on an unsupported (type, kind). */
@@ -300,7 +336,31 @@ int64_t orc_groupby(const orc_col* keys, int32_t nkeys, int32_t null_policy,
     int k = kinds[a];
     int src = v->type_id;
     int tgt = orc_target_type(src, k);
-    if (tgt < 0 || (src == T_STRING && k != A_COUNT_VALID && k != A_COUNT_ALL)) { ok = 0; break; }
+    if (tgt < 0 || (src == T_STRING && k != A_COUNT_VALID && k != A_COUNT_ALL && k != A_ARGMIN &&
+                    k != A_ARGMAX)) { ok = 0; break; }
+    if (k == A_ARGMIN || k == A_ARGMAX) {
+      int64_t* best = (int64_t*)malloc(sizeof(int64_t) * (G > 0 ? G : 1));
+      for (int64_t g = 0; g < G; g++) best[g] = -1;
+      for (int64_t r = 0; r < n; r++) {
+        int64_t g = group_of_row[r];
+        if (g < 0 || !row_valid(v, r)) continue;
+        if (best[g] < 0) { best[g] = r; continue; }
+        int c = value_cmp(v, r, best[g]);
+        if (k == A_ARGMIN ? c < 0 : c > 0) {
+          /* for ARGMAX a NaN orders after numbers but must not win: */
+          if (k == A_ARGMAX && is_float(src) && isnan(as_ld(v, r))) continue;
+          best[g] = r;
+        } else if (k == A_ARGMAX && is_float(src) && isnan(as_ld(v, best[g])) && !isnan(as_ld(v, r))) {
+          best[g] = r;
+        }
+      }
+      for (int64_t g = 0; g < G; g++) {
+        ((int32_t*)outs[a])[g] = best[g] < 0 ? 0 : (int32_t)best[g];
+        out_valid[a][g] = (uint8_t)(best[g] >= 0);
+      }
+      free(best);
+      continue;
+    }
     uint8_t* ov = out_valid[a];
     for (int64_t g = 0; g < G; g++) { acc[g] = 0; iacc[g] = 0; cnt[g] = 0; cnt_all[g] = 0; }
     int fl = is_float(src);

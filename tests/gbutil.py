@@ -13,7 +13,7 @@ from oracle import oracle as orc
 
 AGG = {
     "SUM": 0, "PRODUCT": 1, "MIN": 2, "MAX": 3, "COUNT_VALID": 4, "COUNT_ALL": 5,
-    "SUM_OF_SQUARES": 8, "MEAN": 9, "VARIANCE": 11, "STD": 12,
+    "SUM_OF_SQUARES": 8, "MEAN": 9, "VARIANCE": 11, "STD": 12, "ARGMAX": 15, "ARGMIN": 16,
 }
 
 # Tolerances of the parity bar.  Integer, index, key, count, min and max
@@ -151,7 +151,8 @@ def assert_parity(keys, requests, null_policy=0, gpu=None):
             gs, os_ = gv[og][gvalid_s], ov[oo][ovalid_s]
             assert gs.dtype == os_.dtype, f"dtype {gs.dtype} vs {os_.dtype} (agg {kind})"
             is_float_src = col.type_id in (orc.FLOAT32, orc.FLOAT64)
-            if kind in (AGG["COUNT_VALID"], AGG["COUNT_ALL"], AGG["MIN"], AGG["MAX"]) or (
+            if kind in (AGG["COUNT_VALID"], AGG["COUNT_ALL"], AGG["MIN"], AGG["MAX"], AGG["ARGMIN"],
+                        AGG["ARGMAX"]) or (
                     kind in (AGG["SUM"], AGG["SUM_OF_SQUARES"]) and not is_float_src):
                 if gs.dtype.kind == "f":
                     same = (gs == os_) | (np.isnan(gs) & np.isnan(os_))

@@ -242,6 +242,8 @@ def test_batch_errors():
     inst = make_instances(23, 3, 1_000)
     with pytest.raises(_lib.LogicError):
         run_batch(inst, [[(AGG["PRODUCT"], 1)], REQ1], 0)
+    with pytest.raises(_lib.LogicError):
+        run_batch(inst, [[(AGG["ARGMIN"], 1)], REQ1], 0)
     bad = list(inst)
     k, vs = bad[2]
     bad[2] = (k, [host_col(vs[0].data[:-1]), vs[1]])

@@ -354,3 +354,43 @@ def test_lds_preaggregation_table_full_falls_back_to_global():
     assert ng == len(np.unique(k))
     st = _last_stats()
     assert st["lds_preagg"] == 1.0 and st["est_groups"] < 100
+
+
+# ---------------------------------------------------------- ARGMIN / ARGMAX --
+@pytest.mark.parametrize("val_dtype", [np.int8, np.int32, np.int64, np.uint16, np.uint64, np.float32, np.float64])
+@pytest.mark.parametrize("card", [7, 5_000, 150_000])
+def test_argmin_argmax(val_dtype, card):
+    """Ties (small value range) take the first row; NaN only wins in an all-NaN
+    group; nulls skipped; mixed with other aggregations on the same column."""
+    rng = np.random.default_rng(card + np.dtype(val_dtype).itemsize)
+    n = 300_000
+    k = rng.integers(0, card, n).astype(np.int64)
+    kvalid = rng.random(n) > 0.02
+    if np.dtype(val_dtype).kind == "f":
+        v = rng.integers(-20, 20, n).astype(val_dtype)
+        v[rng.random(n) < 0.05] = np.nan
+    else:
+        info = np.iinfo(val_dtype)
+        v = rng.integers(max(info.min, -20), min(info.max, 40), n).astype(val_dtype)
+    vvalid = rng.random(n) > 0.1
+    aggs = [(AGG["ARGMIN"], 1), (AGG["ARGMAX"], 1), (AGG["MIN"], 1), (AGG["COUNT_VALID"], 1)]
+    if np.dtype(val_dtype).kind != "f":
+        aggs.append((AGG["SUM"], 1))  # float sums of groups with NaN are NaN on both sides
+    for policy in (0, 1):
+        assert_parity([host_col(k, kvalid)], [(host_col(v, vvalid), aggs)], policy)
+    # no nulls in the values: no result mask
+    g_keys, g_res = gpu_groupby([host_col(k)], [(host_col(v), [(AGG["ARGMAX"], 1)])])
+    assert g_res[0][0][1] is None or g_res[0][0][1].all()
+
+
+def test_argmin_argmax_all_nan_groups_and_errors():
+    from cudf_amd import _lib
+    k = np.repeat(np.arange(50, dtype=np.int32), 20)
+    v = np.full(len(k), np.nan)
+    v[k % 2 == 0] = np.arange((k % 2 == 0).sum(), dtype=np.float64) % 7
+    assert_parity([host_col(k)], [(host_col(v), [(AGG["ARGMIN"], 1), (AGG["ARGMAX"], 1)])])
+    s = orc.HostColumn.from_strings([b"x%d" % i for i in range(len(k))])
+    with pytest.raises(_lib.DataTypeError):
+        gpu_groupby([host_col(k)], [(s, [(AGG["ARGMIN"], 1)])])
+    with pytest.raises(_lib.LogicError):
+        gpu_groupby([host_col(np.sort(k))], [(host_col(v), [(AGG["ARGMIN"], 1)])], keys_sorted=True)

@@ -188,6 +188,42 @@ def test_oracle_known_answers(case):
         _run_kat(case, vt, oracle_groupby)
 
 
+@pytest.mark.parametrize("case", KATS["string_value_arg_cases"], ids=lambda c: c["name"])
+def test_oracle_string_value_argmin_argmax(case):
+    keys = host_col(np.array(case["keys"], np.int32))
+    vals = orc.HostColumn.from_strings([s.encode() for s in case["vals"]])
+    kcols, res = oracle_groupby([keys], [(vals, [(AGG[case["agg"]], 1)])])
+    got = {int(kcols[0][0][i]): int(res[0][0][0][i]) for i in range(len(kcols[0][0]))}
+    assert got == dict(zip(case["expect_keys"], case["expect_vals"]))
+
+
+def test_oracle_argmin_argmax_brute_force():
+    """Ties take the first row; a NaN wins only in an all-NaN group."""
+    rng = np.random.default_rng(12)
+    for trial in range(20):
+        n = 300
+        k = rng.integers(0, 15, n).astype(np.int32)
+        v = rng.integers(-5, 5, n).astype(np.float64)
+        v[rng.random(n) < 0.1] = np.nan
+        valid = rng.random(n) > 0.15
+        kcols, res = oracle_groupby([host_col(k)], [(host_col(v, valid), [(AGG["ARGMIN"], 1), (AGG["ARGMAX"], 1)])])
+        for gi in range(len(kcols[0][0])):
+            key = int(kcols[0][0][gi])
+            rows = [r for r in range(n) if k[r] == key and valid[r]]
+            for a, pick in ((0, min), (1, max)):
+                got, ok = res[0][a][0][gi], res[0][a][1][gi]
+                if not rows:
+                    assert not ok
+                    continue
+                nums = [r for r in rows if not np.isnan(v[r])]
+                if nums:
+                    best = pick(v[r] for r in nums)
+                    want = next(r for r in nums if v[r] == best)
+                else:
+                    want = rows[0]
+                assert ok and got == want, (trial, key, a, got, want)
+
+
 def test_oracle_string_keys_known_answer():
     case = KATS["string_key_cases"][0]
     keys = orc.HostColumn.from_strings([s.encode() for s in case["keys"]])
