@@ -182,6 +182,9 @@ __host__ __device__ inline uint64_t field_identity(int32_t kind)
     case FK_MAX_F64: return 0x000fffffffffffffull;      // ord64(-inf)
     case FK_ARGMIN:
     case FK_ARGMAX: return 0xffffffffull;               // no row yet
+    case FK_PROD_I64: return 1ull;
+    case FK_PROD_F32: return 0x3f800000ull;             // 1.0f
+    case FK_PROD_F64: return 0x3ff0000000000000ull;     // 1.0
     default: return 0;                                  // sums, counts, max of unsigned
   }
 }
@@ -267,6 +270,33 @@ __device__ __forceinline__ void acc_apply(int32_t kind, void* a, loaded_value co
       if (!isnan(v.d)) amax(static_cast<uint64_t*>(a), ord64(v.b));
       break;
     case FK_COUNT: aadd(static_cast<uint32_t*>(a), 1u); break;
+    case FK_PROD_I64:
+    case FK_PROD_F64: {
+      // no atomic multiply: compare-and-swap loop (DeviceProduct through
+      // atomicCAS in the reference's device_atomics)
+      auto* p      = static_cast<uint64_t*>(a);
+      uint64_t cur = __hip_atomic_load(gp(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      for (;;) {
+        uint64_t const next =
+          kind == FK_PROD_I64 ? cur * uint64_t(v.i)
+                              : uint64_t(__double_as_longlong(__longlong_as_double(int64_t(cur)) * v.d));
+        uint64_t const seen = acas64(p, cur, next);
+        if (seen == cur) break;
+        cur = seen;
+      }
+      break;
+    }
+    case FK_PROD_F32: {
+      auto* p      = static_cast<unsigned int*>(a);
+      uint32_t cur = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      for (;;) {
+        uint32_t const next = __float_as_uint(__uint_as_float(cur) * v.f);
+        uint32_t const seen = atomicCAS(p, cur, next);
+        if (seen == cur) break;
+        cur = seen;
+      }
+      break;
+    }
     default: break;
   }
 }

@@ -330,6 +330,7 @@ int32_t target_type_of(int32_t src, int32_t kind)
     case MIG_AGG_STD: return MIG_TYPE_FLOAT64;
     case MIG_AGG_SUM:
     case MIG_AGG_SUM_OF_SQUARES:
+    case MIG_AGG_PRODUCT:  // aggregation.hpp:1276-1306
       if (src == MIG_TYPE_FLOAT32 || src == MIG_TYPE_FLOAT64) return src;
       return MIG_TYPE_INT64;
     default: return MIG_TYPE_EMPTY;
@@ -350,6 +351,7 @@ void check_supported(int32_t src, int32_t kind)
     case MIG_AGG_ARGMAX: ok = is_numeric_type(src) || is_chrono_type(src); break;
     case MIG_AGG_SUM:
     case MIG_AGG_SUM_OF_SQUARES:
+    case MIG_AGG_PRODUCT:
     case MIG_AGG_MEAN:
     case MIG_AGG_VARIANCE:
     case MIG_AGG_STD: ok = is_numeric_type(src); break;
@@ -368,9 +370,9 @@ struct value_plan {
   int32_t storage   = ST_NONE;
   bool has_nulls    = false;
   bool want_sum = false, want_sumsq = false, want_min = false, want_max = false;
-  bool want_count = false, want_m2 = false, want_argmin = false, want_argmax = false;
+  bool want_count = false, want_m2 = false, want_argmin = false, want_argmax = false, want_prod = false;
   int32_t f_sum = -1, f_sumsq = -1, f_min = -1, f_max = -1, f_count = -1, f_m2 = -1;
-  int32_t f_argmin = -1, f_argmax = -1;
+  int32_t f_argmin = -1, f_argmax = -1, f_prod = -1;
 };
 
 int32_t sum_field_kind(int32_t st)
@@ -414,7 +416,8 @@ int32_t field_width(int32_t fk)
     case FK_MAX_F32:
     case FK_COUNT:
     case FK_ARGMIN:
-    case FK_ARGMAX: return 4;
+    case FK_ARGMAX:
+    case FK_PROD_F32: return 4;
     default: return 8;
   }
 }
@@ -603,6 +606,7 @@ std::unique_ptr<mig_groupby_result> groupby_aggregate(mig_column_view const* key
         case MIG_AGG_MEAN: vp.want_sum = vp.want_count = true; break;
         case MIG_AGG_VARIANCE:
         case MIG_AGG_STD: vp.want_sum = vp.want_count = vp.want_m2 = true; break;
+        case MIG_AGG_PRODUCT: vp.want_prod = true; vp.want_count |= vp.has_nulls; break;
         case MIG_AGG_ARGMIN: vp.want_argmin = true; break;
         case MIG_AGG_ARGMAX: vp.want_argmax = true; break;
         default: break;
@@ -640,6 +644,8 @@ std::unique_ptr<mig_groupby_result> groupby_aggregate(mig_column_view const* key
         P.need_count_all = 1;
       }
     }
+    if (vp.want_prod)
+      vp.f_prod = add_field(int(j), vp.storage == ST_F32 ? FK_PROD_F32 : (vp.storage == ST_F64 ? FK_PROD_F64 : FK_PROD_I64));
     if (vp.want_argmin) vp.f_argmin = add_field(int(j), FK_ARGMIN);
     if (vp.want_argmax) vp.f_argmax = add_field(int(j), FK_ARGMAX);
     if (vp.want_m2) {
@@ -652,10 +658,14 @@ std::unique_ptr<mig_groupby_result> groupby_aggregate(mig_column_view const* key
   P.val_field_begin[vals.size()] = nf;
   P.num_vals   = int32_t(vals.size());
   P.num_fields = nf;
-  // ARGMIN / ARGMAX update a row index by compare-and-swap against the
-  // values column: only the device-wide table path implements them
+  // ARGMIN / ARGMAX (a row index updated by compare-and-swap against the
+  // values column) and PRODUCT (compare-and-swap multiply): only the
+  // device-wide table path implements them
   bool any_arg = false;
-  for (int f = 0; f < nf; f++) any_arg |= P.fields[f].kind == FK_ARGMIN || P.fields[f].kind == FK_ARGMAX;
+  for (int f = 0; f < nf; f++) {
+    int32_t const k = P.fields[f].kind;
+    any_arg |= k == FK_ARGMIN || k == FK_ARGMAX || k == FK_PROD_I64 || k == FK_PROD_F32 || k == FK_PROD_F64;
+  }
 
   // ---- cardinality estimate ------------------------------------------------
   scratch S(stream);
@@ -747,7 +757,8 @@ std::unique_ptr<mig_groupby_result> groupby_aggregate(mig_column_view const* key
         switch (kind) {
           case MIG_AGG_SUM:
           case MIG_AGG_SUM_OF_SQUARES:
-            O.field = kind == MIG_AGG_SUM ? vp.f_sum : vp.f_sumsq;
+          case MIG_AGG_PRODUCT:
+            O.field = kind == MIG_AGG_SUM ? vp.f_sum : (kind == MIG_AGG_PRODUCT ? vp.f_prod : vp.f_sumsq);
             O.kind  = tt == MIG_TYPE_FLOAT32 ? OK_COPY_F32 : (tt == MIG_TYPE_FLOAT64 ? OK_COPY_F64 : OK_COPY_I64);
             if (vp.has_nulls) { O.count_field = vp.f_count; with_mask = true; }
             break;
@@ -845,7 +856,7 @@ std::unique_ptr<mig_groupby_result> groupby_aggregate(mig_column_view const* key
   // (the reference's sort path with keys_pre_sorted, sort/sort_helper.cu:55-72,
   // 133-181); groups come out in input order.
   if (keys_sorted) {
-    MIG_EXPECTS(!any_arg, "ARGMIN / ARGMAX are not supported with pre-sorted keys");
+    MIG_EXPECTS(!any_arg, "PRODUCT / ARGMIN / ARGMAX are not supported with pre-sorted keys");
     P.key_mode            = KM_GENERAL;  // keys are gathered from each group's first row
     int64_t const ntiles  = sorted_tiles(N);
     uint32_t* tile_counts = S.get<uint32_t>(ntiles + 1);

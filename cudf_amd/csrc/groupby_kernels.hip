@@ -331,6 +331,9 @@ void launch_aggregate(gb_params const* dP, gb_params const& hP, int num_cus, hip
 __device__ __forceinline__ double field_as_double(field_desc const& F, int64_t slot)
 {
   switch (F.kind) {
+    case FK_PROD_I64: return double(*reinterpret_cast<int64_t const*>(field_at(F, slot)));
+    case FK_PROD_F32: return double(*reinterpret_cast<float const*>(field_at(F, slot)));
+    case FK_PROD_F64: return *reinterpret_cast<double const*>(field_at(F, slot));
     case FK_SUM_I64: return double(*reinterpret_cast<int64_t const*>(field_at(F, slot)));
     case FK_SUM_F32: return double(*reinterpret_cast<float const*>(field_at(F, slot)));
     case FK_SUM_F64: return *reinterpret_cast<double const*>(field_at(F, slot));

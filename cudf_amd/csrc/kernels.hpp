@@ -50,7 +50,10 @@ enum field_kind : int32_t {
   FK_COUNT,  // valid values (u32)
   FK_M2,     // sum of squared deviations from the group mean (f64, second pass)
   FK_ARGMIN, // row of the smallest valid value (u32, 0xFFFFFFFF = none)
-  FK_ARGMAX  // row of the largest valid value
+  FK_ARGMAX, // row of the largest valid value
+  FK_PROD_I64,  // products (compare-and-swap loops)
+  FK_PROD_F32,
+  FK_PROD_F64
 };
 
 struct field_desc {

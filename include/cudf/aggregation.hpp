@@ -79,6 +79,11 @@ std::unique_ptr<Base> make_sum_of_squares_aggregation()
   return std::make_unique<Base>(aggregation::SUM_OF_SQUARES);
 }
 template <typename Base = aggregation>
+std::unique_ptr<Base> make_product_aggregation()
+{
+  return std::make_unique<Base>(aggregation::PRODUCT);
+}
+template <typename Base = aggregation>
 std::unique_ptr<Base> make_argmax_aggregation()
 {
   return std::make_unique<Base>(aggregation::ARGMAX);

@@ -283,6 +283,9 @@ typedef struct { int64_t* slots; int64_t cap; } omap;
  *     double);
  *   - VARIANCE = sum((x - mean)^2) / (n - ddof), null when n - ddof <= 0,
  *     two passes in long double (multi_pass_kernels.cuh:74-95);
+ *   - PRODUCT multiplies the valid values (int64 for integers, wrapping;
+ *     floats exact and rounded once), null when none is valid
+ *     (aggregation.cuh update_target_element PRODUCT, DeviceProduct);
  *   - ARGMIN / ARGMAX = row index of the smallest / largest valid value,
  *     null when the group has none (aggregation.cuh:440-490).  The reference
  *     keeps whichever of equal values reaches the target first; this
@@ -367,6 +370,9 @@ int64_t orc_groupby(const orc_col* keys, int32_t nkeys, int32_t null_policy,
     if (k == A_MIN || k == A_MAX) {
       for (int64_t g = 0; g < G; g++) acc[g] = (k == A_MIN) ? INFINITY : -INFINITY;
     }
+    if (k == A_PRODUCT) {
+      for (int64_t g = 0; g < G; g++) { acc[g] = 1; iacc[g] = 1; }
+    }
     /* first pass */
     for (int64_t r = 0; r < n; r++) {
       int64_t g = group_of_row[r];
@@ -378,6 +384,11 @@ int64_t orc_groupby(const orc_col* keys, int32_t nkeys, int32_t null_policy,
       switch (k) {
         case A_SUM: case A_MEAN: case A_VARIANCE: case A_STD:
           if (fl) acc[g] += x; else iacc[g] += (uint64_t)as_i64(v, r);
+          break;
+        case A_PRODUCT:
+          /* integers multiply in int64 with wrap-around; floats exactly
+             (long double) and round once, as for SUM */
+          if (fl) acc[g] *= x; else iacc[g] *= (uint64_t)as_i64(v, r);
           break;
         case A_SUM_OF_SQUARES:
           if (fl) {
@@ -406,7 +417,7 @@ int64_t orc_groupby(const orc_col* keys, int32_t nkeys, int32_t null_policy,
       switch (k) {
         case A_COUNT_VALID: ((int32_t*)outs[a])[g] = (int32_t)cnt[g]; valid = 1; break;
         case A_COUNT_ALL: ((int32_t*)outs[a])[g] = (int32_t)cnt_all[g]; valid = 1; break;
-        case A_SUM: case A_SUM_OF_SQUARES:
+        case A_SUM: case A_SUM_OF_SQUARES: case A_PRODUCT:
           if (tgt == T_FLOAT32) ((float*)outs[a])[g] = (float)acc[g];
           else if (tgt == T_FLOAT64) ((double*)outs[a])[g] = (double)acc[g];
           else if (type_width(tgt) == 8) ((int64_t*)outs[a])[g] = (int64_t)iacc[g];

@@ -153,7 +153,7 @@ def assert_parity(keys, requests, null_policy=0, gpu=None):
             is_float_src = col.type_id in (orc.FLOAT32, orc.FLOAT64)
             if kind in (AGG["COUNT_VALID"], AGG["COUNT_ALL"], AGG["MIN"], AGG["MAX"], AGG["ARGMIN"],
                         AGG["ARGMAX"]) or (
-                    kind in (AGG["SUM"], AGG["SUM_OF_SQUARES"]) and not is_float_src):
+                    kind in (AGG["SUM"], AGG["SUM_OF_SQUARES"], AGG["PRODUCT"]) and not is_float_src):
                 if gs.dtype.kind == "f":
                     same = (gs == os_) | (np.isnan(gs) & np.isnan(os_))
                     assert same.all(), f"agg {kind} not bit-exact: {gs[~same][:5]} vs {os_[~same][:5]}"
@@ -192,6 +192,11 @@ def assert_parity(keys, requests, null_policy=0, gpu=None):
                         floor = np.sqrt(floor)
                     bound = bound + floor
                 assert (err <= bound).all(), f"VAR/STD error {err.max()} above bound"
+            elif kind == AGG["PRODUCT"]:
+                # unordered products of at most ~80 factors: |error| <= 80 * eps * |product|
+                tol = TOL_F32_SUM if col.type_id == orc.FLOAT32 else TOL_F64_SUM
+                err = np.abs(gs.astype(np.float64) - os_.astype(np.float64))
+                assert (err <= tol * np.abs(os_.astype(np.float64)) + 1e-300).all(), f"PRODUCT error {err.max()}"
             elif kind == AGG["SUM_OF_SQUARES"]:
                 tol = TOL_F32_SUM if col.type_id == orc.FLOAT32 else TOL_F64_SUM
                 err = np.abs(gs.astype(np.float64) - os_.astype(np.float64))

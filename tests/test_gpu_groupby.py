@@ -394,3 +394,24 @@ def test_argmin_argmax_all_nan_groups_and_errors():
         gpu_groupby([host_col(k)], [(s, [(AGG["ARGMIN"], 1)])])
     with pytest.raises(_lib.LogicError):
         gpu_groupby([host_col(np.sort(k))], [(host_col(v), [(AGG["ARGMIN"], 1)])], keys_sorted=True)
+
+
+# ------------------------------------------------------------------ PRODUCT --
+@pytest.mark.parametrize("val_dtype", [np.int8, np.int32, np.int64, np.uint32, np.float32, np.float64])
+def test_product(val_dtype):
+    """Groups of ~40 values: integers in [-3, 3] (int64 products, wrapping
+    exactly as the oracle), floats in [0.9, 1.1]."""
+    rng = np.random.default_rng(np.dtype(val_dtype).itemsize * 3)
+    n = 200_000
+    k = rng.integers(0, 5_000, n).astype(np.int32)
+    if np.dtype(val_dtype).kind == "f":
+        v = (0.9 + 0.2 * rng.random(n)).astype(val_dtype)
+    elif np.dtype(val_dtype).kind == "u":
+        v = rng.integers(0, 4, n).astype(val_dtype)
+    else:
+        v = rng.integers(-3, 4, n).astype(val_dtype)
+    vvalid = rng.random(n) > 0.1
+    for policy in (0, 1):
+        assert_parity([host_col(k, rng.random(n) > 0.02)],
+                      [(host_col(v, vvalid), [(AGG["PRODUCT"], 1), (AGG["SUM"], 1), (AGG["ARGMAX"], 1)])], policy)
+    assert_parity([host_col(k)], [(host_col(v), [(AGG["PRODUCT"], 1)])])

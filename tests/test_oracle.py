@@ -267,6 +267,17 @@ def _brute(keys, vals, valid_k, valid_v, kind, ddof, policy):
                 s = sum(int(x) for x in xs)
                 s = (s + 2**63) % 2**64 - 2**63
                 out[k] = (s, True)
+        elif kind == "PRODUCT":
+            if isinstance(xs[0], float):
+                pr = Fraction(1)
+                for x in xs:
+                    pr *= Fraction(x)
+                out[k] = (float(pr), True)
+            else:
+                pr = 1
+                for x in xs:
+                    pr = (pr * int(x)) % 2**64
+                out[k] = ((pr + 2**63) % 2**64 - 2**63, True)
         elif kind == "MIN":
             out[k] = (min(xs), True)
         elif kind == "MAX":
@@ -295,12 +306,14 @@ def test_oracle_matches_exact_brute_force(seed):
     vals = rng.standard_normal(n) * 100 if is_float else rng.integers(-10**12, 10**12, n)
     valid_v = rng.random(n) > 0.2
     policy = seed % 3 == 0
-    for kind in ["SUM", "MIN", "MAX", "COUNT_VALID", "COUNT_ALL", "MEAN", "VARIANCE"]:
+    for kind in ["SUM", "MIN", "MAX", "COUNT_VALID", "COUNT_ALL", "MEAN", "VARIANCE", "PRODUCT"]:
         ddof = int(rng.integers(0, 3)) if kind == "VARIANCE" else 1
         kc = host_col(keys, valid_k)
-        vc = host_col(vals, valid_v)
+        # products of a few factors near 1 (floats) or wrapping int64 (integers)
+        pv = (1 + vals / 1e4) if (kind == "PRODUCT" and is_float) else vals
+        vc = host_col(pv, valid_v)
         kcols, res = oracle_groupby([kc], [(vc, [(AGG[kind], ddof)])], int(policy))
-        expect = _brute(list(keys), list(vals), valid_k, valid_v, kind, ddof, int(policy))
+        expect = _brute(list(keys), list(pv), valid_k, valid_v, kind, ddof, int(policy))
         kv, kvalid = kcols[0]
         got = {}
         for i in range(len(kv)):
@@ -314,6 +327,8 @@ def test_oracle_matches_exact_brute_force(seed):
                 continue
             if kind in ("MEAN", "VARIANCE"):
                 assert math.isclose(float(g), e, rel_tol=1e-14, abs_tol=1e-300), (kind, g, e)
+            elif kind == "PRODUCT" and isinstance(e, float):
+                assert float(g) == e, (kind, g, e)  # exact product rounded once
             elif isinstance(e, float):
                 assert float(g) == e, (kind, g, e)
             else:
