@@ -348,7 +348,7 @@ void check_supported(int32_t src, int32_t kind)
     case MIG_AGG_MIN:
     case MIG_AGG_MAX:
     case MIG_AGG_ARGMIN:
-    case MIG_AGG_ARGMAX: ok = is_numeric_type(src) || is_chrono_type(src); break;
+    case MIG_AGG_ARGMAX: ok = is_numeric_type(src) || is_chrono_type(src) || src == MIG_TYPE_STRING; break;
     case MIG_AGG_SUM:
     case MIG_AGG_SUM_OF_SQUARES:
     case MIG_AGG_PRODUCT:
@@ -599,8 +599,15 @@ std::unique_ptr<mig_groupby_result> groupby_aggregate(mig_column_view const* key
       switch (requests[r].aggregations[a].kind) {
         case MIG_AGG_SUM: vp.want_sum = true; vp.want_count |= vp.has_nulls; break;
         case MIG_AGG_SUM_OF_SQUARES: vp.want_sumsq = true; vp.want_count |= vp.has_nulls; break;
-        case MIG_AGG_MIN: vp.want_min = true; vp.want_count |= vp.has_nulls; break;
-        case MIG_AGG_MAX: vp.want_max = true; vp.want_count |= vp.has_nulls; break;
+        // string MIN / MAX through ARGMIN / ARGMAX and a gather (hash/groupby.cu:113-181)
+        case MIG_AGG_MIN:
+          if (vp.storage == ST_STRING) vp.want_argmin = true;
+          else { vp.want_min = true; vp.want_count |= vp.has_nulls; }
+          break;
+        case MIG_AGG_MAX:
+          if (vp.storage == ST_STRING) vp.want_argmax = true;
+          else { vp.want_max = true; vp.want_count |= vp.has_nulls; }
+          break;
         case MIG_AGG_COUNT_VALID: vp.want_count = true; break;
         case MIG_AGG_COUNT_ALL: P.need_count_all = 1; break;
         case MIG_AGG_MEAN: vp.want_sum = vp.want_count = true; break;
@@ -705,9 +712,16 @@ std::unique_ptr<mig_groupby_result> groupby_aggregate(mig_column_view const* key
   // partitioned path allocates before the exact group count is known).  Masks
   // are zeroed when `zero_masks` (they are then set bit by bit).
   int n_str_keys = 0;
+  // string MIN / MAX results: (request, aggregation, value column, group rows)
+  struct string_out {
+    int r, a, vc;
+    int32_t* rows;
+  };
+  std::vector<string_out> string_outs;
   auto alloc_outputs = [&](int64_t rows_cap, int64_t G, bool zero_masks) {
     res->keys.clear();
     res->results.clear();
+    string_outs.clear();
     n_str_keys = 0;
     auto fixed = [&](int32_t type_id, bool with_mask) {
       owned_column oc = make_owned_fixed(A, type_id, rows_cap, with_mask);
@@ -764,6 +778,27 @@ std::unique_ptr<mig_groupby_result> groupby_aggregate(mig_column_view const* key
             break;
           case MIG_AGG_MIN:
           case MIG_AGG_MAX:
+            if (vp.storage == ST_STRING) {
+              // the group's ARGMIN / ARGMAX rows into scratch; the strings are
+              // gathered from them once the rows are known
+              O.kind    = OK_ARG;
+              O.field   = kind == MIG_AGG_MIN ? vp.f_argmin : vp.f_argmax;
+              O.data    = S.get<int32_t>(std::max<int64_t>(rows_cap, 1));
+              O.mask    = nullptr;
+              owned_column oc;
+              oc.col.type_id         = MIG_TYPE_STRING;
+              oc.col.size            = int32_t(G);
+              oc.col.offsets_type_id = MIG_TYPE_INT32;
+              oc.offsets_bytes       = sizeof(int32_t) * size_t(G + 1);
+              oc.col.offsets         = A.allocate(oc.offsets_bytes);
+              if (vp.has_nulls) {
+                oc.mask_bytes    = std::max<size_t>(8, size_t((G + 63) / 64) * 8);
+                oc.col.null_mask = static_cast<uint32_t*>(A.allocate(oc.mask_bytes));
+              }
+              string_outs.push_back({r, int(cols.size()), vcol_of_request[r], static_cast<int32_t*>(O.data)});
+              cols.push_back(oc);
+              continue;
+            }
             O.field = kind == MIG_AGG_MIN ? vp.f_min : vp.f_max;
             O.kind  = OK_MINMAX;
             if (vp.has_nulls) { O.count_field = vp.f_count; with_mask = true; }
@@ -1206,7 +1241,27 @@ std::unique_ptr<mig_groupby_result> groupby_aggregate(mig_column_view const* key
     upload(1);  // stream-ordered after the kernels that read the previous block
     launch_outputs(dP, P, map, G, rep_rows, stream);
     if (P.key_mode == KM_GENERAL) gather_general_keys(rep_rows, G);
-
+    // string MIN / MAX: lengths and validity, offsets, then the characters
+    for (auto const& so : string_outs) {
+      owned_column& oc = res->results[so.r][so.a];
+      auto* offs       = static_cast<int32_t*>(oc.col.offsets);
+      launch_str_arg_sizes(P.vals[so.vc], so.rows, G, offs, oc.col.null_mask, stream);
+      if (G == 0) {
+        MIG_HIP_TRY(hipMemsetAsync(offs, 0, sizeof(int32_t), stream));
+        continue;
+      }
+      launch_exclusive_scan_i32(offs, G, offs + G, stream);
+      auto* h = reinterpret_cast<int32_t*>(pin + 2 * sizeof(gb_params));
+      MIG_HIP_TRY(hipMemcpyAsync(h, offs + G, 4, hipMemcpyDeviceToHost, stream));
+      MIG_HIP_TRY(hipStreamSynchronize(stream));
+      int32_t const total = *h;
+      MIG_EXPECTS(total >= 0, "string MIN / MAX results exceed 2^31 characters");
+      oc.data_bytes     = size_t(total);
+      oc.col.data       = A.allocate(oc.data_bytes);
+      oc.col.data_bytes = total;
+      launch_gather_string_chars(P.vals[so.vc], reinterpret_cast<uint32_t const*>(so.rows), G, offs,
+                                 static_cast<uint8_t*>(oc.col.data), stream);
+    }
   }
   MIG_HIP_TRY(hipGetLastError());
   stats[STAT_KEY_MODE] = P.key_mode;

@@ -250,8 +250,22 @@ __device__ __forceinline__ int64_t locate(gb_params const& P, int64_t row)
 // deterministic.
 __device__ __forceinline__ bool arg_better(dcol const& c, int32_t kind, int64_t r, int64_t o)
 {
-  loaded_value const a = load_value(c, r), b = load_value(c, o);
   int cmp = 0;
+  if (c.storage == ST_STRING) {  // bytewise, then by length (string_view::compare)
+    int64_t rb, re, ob, oe;
+    str_span(c, r, rb, re);
+    str_span(c, o, ob, oe);
+    int64_t const lr = re - rb, lo = oe - ob, m = lr < lo ? lr : lo;
+    auto const d     = gp(c.data);
+    for (int64_t i = 0; i < m && cmp == 0; i++) {
+      uint8_t const x = d[rb + i], y = d[ob + i];
+      cmp             = x < y ? -1 : (x > y ? 1 : 0);
+    }
+    if (cmp == 0) cmp = lr < lo ? -1 : (lr > lo ? 1 : 0);
+    if (cmp != 0) return kind == FK_ARGMIN ? cmp < 0 : cmp > 0;
+    return r < o;
+  }
+  loaded_value const a = load_value(c, r), b = load_value(c, o);
   if (c.storage == ST_F32 || c.storage == ST_F64) {
     bool const an = isnan(a.d), bn = isnan(b.d);
     if (an != bn) return bn;  // a number beats a NaN
@@ -764,7 +778,7 @@ __global__ void __launch_bounds__(BLOCK)
         case OK_ARG: {
           uint32_t const r = *reinterpret_cast<uint32_t const*>(field_at(P.fields[O.field], slot));
           valid            = r != 0xFFFFFFFFu;
-          reinterpret_cast<int32_t*>(O.data)[g] = valid ? int32_t(r) : 0;
+          reinterpret_cast<int32_t*>(O.data)[g] = valid ? int32_t(r) : -1;
           break;
         }
         case OK_VAR:
@@ -1534,6 +1548,31 @@ void launch_sorted_second_pass(gb_params const* dP, gb_params const& hP, int64_t
                                int num_cus, hipStream_t s)
 {
   launch_sorted_passes(dP, hP, n, gid, num_cus, true, s);
+}
+
+// String MIN / MAX from the groups' ARGMIN / ARGMAX rows: lengths (0 for
+// groups without a valid value) and validity words.
+__global__ void __launch_bounds__(BLOCK)
+  k_str_arg_sizes(dcol src, int32_t const* __restrict__ rows, int64_t G, int32_t* __restrict__ sizes,
+                  uint32_t* __restrict__ mask)
+{
+  int64_t const g = int64_t(blockIdx.x) * BLOCK + threadIdx.x;
+  bool valid      = false;
+  if (g < G) {
+    int32_t const r = rows[g];
+    valid           = r != -1;
+    int64_t b = 0, e = 0;
+    if (valid) str_span(src, r, b, e);
+    sizes[g] = int32_t(e - b);
+  }
+  if (mask) store_valid_bits(mask, g, G, valid);
+}
+
+void launch_str_arg_sizes(dcol const& src, int32_t const* rows, int64_t G, int32_t* sizes, uint32_t* mask,
+                          hipStream_t s)
+{
+  if (G > 0)
+    hipLaunchKernelGGL(k_str_arg_sizes, dim3((G + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, s, src, rows, G, sizes, mask);
 }
 
 void launch_iota_u32(uint32_t* out, int64_t n, hipStream_t s)

@@ -259,6 +259,9 @@ void launch_sorted_aggregate(gb_params const* dP, gb_params const& hP, int64_t n
 void launch_sorted_second_pass(gb_params const* dP, gb_params const& hP, int64_t n, uint32_t const* gid,
                                int num_cus, hipStream_t s);
 void launch_iota_u32(uint32_t* out, int64_t n, hipStream_t s);
+// string MIN / MAX: lengths and validity of the values at the ARGMIN / ARGMAX rows (-1 = none)
+void launch_str_arg_sizes(dcol const& src, int32_t const* rows, int64_t G, int32_t* sizes, uint32_t* mask,
+                          hipStream_t s);
 
 // ---- radix-partitioned aggregation (partition_kernels.hip) ------------------
 void launch_rpa_hist(gb_params const* dP, rpa_params const* dR, rpa_params const& hR, int64_t n,

@@ -391,7 +391,7 @@ def test_argmin_argmax_all_nan_groups_and_errors():
     assert_parity([host_col(k)], [(host_col(v), [(AGG["ARGMIN"], 1), (AGG["ARGMAX"], 1)])])
     s = orc.HostColumn.from_strings([b"x%d" % i for i in range(len(k))])
     with pytest.raises(_lib.DataTypeError):
-        gpu_groupby([host_col(k)], [(s, [(AGG["ARGMIN"], 1)])])
+        gpu_groupby([host_col(k)], [(s, [(AGG["SUM"], 1)])])
     with pytest.raises(_lib.LogicError):
         gpu_groupby([host_col(np.sort(k))], [(host_col(v), [(AGG["ARGMIN"], 1)])], keys_sorted=True)
 
@@ -415,3 +415,46 @@ def test_product(val_dtype):
         assert_parity([host_col(k, rng.random(n) > 0.02)],
                       [(host_col(v, vvalid), [(AGG["PRODUCT"], 1), (AGG["SUM"], 1), (AGG["ARGMAX"], 1)])], policy)
     assert_parity([host_col(k)], [(host_col(v), [(AGG["PRODUCT"], 1)])])
+
+
+# ------------------------------------------------- string MIN / MAX / ARG* --
+@pytest.mark.parametrize("case", KATS["string_value_arg_cases"], ids=lambda c: c["name"])
+def test_string_value_argmin_argmax_known_answers(case):
+    keys = host_col(np.array(case["keys"], np.int32))
+    vals = orc.HostColumn.from_strings([s.encode() for s in case["vals"]])
+    kcols, res = gpu_groupby([keys], [(vals, [(AGG[case["agg"]], 1)])])
+    got = {int(kcols[0][0][i]): int(res[0][0][0][i]) for i in range(len(kcols[0][0]))}
+    assert got == dict(zip(case["expect_keys"], case["expect_vals"]))
+
+
+@pytest.mark.parametrize("policy", [0, 1])
+def test_string_min_max(policy):
+    """MIN / MAX of string values are the strings at the ARGMIN / ARGMAX rows
+    (hash/groupby.cu:113-181); ARGMIN / ARGMAX match the oracle row for row."""
+    rng = np.random.default_rng(41 + policy)
+    n = 60_000
+    pool = [bytes(rng.integers(97, 100, int(rng.integers(0, 6)), dtype=np.uint8)) for _ in range(300)]
+    k = rng.integers(0, 3_000, n).astype(np.int32)
+    kvalid = rng.random(n) > 0.03
+    strs = [pool[i] for i in rng.integers(0, len(pool), n)]
+    svalid = rng.random(n) > 0.2
+    vals = orc.HostColumn.from_strings(strs, svalid)
+    aggs = [(AGG["MIN"], 1), (AGG["MAX"], 1), (AGG["ARGMIN"], 1), (AGG["ARGMAX"], 1), (AGG["COUNT_VALID"], 1)]
+    from tests.gbutil import oracle_groupby
+    g_keys, g_res = gpu_groupby([host_col(k, kvalid)], [(vals, aggs)], policy)
+    o_keys, o_res = oracle_groupby([host_col(k, kvalid)], [(vals, [(AGG["ARGMIN"], 1), (AGG["ARGMAX"], 1),
+                                                                   (AGG["COUNT_VALID"], 1)])], policy)
+    og, oo = sort_order(g_keys), sort_order(o_keys)
+    assert len(og) == len(oo)
+    for j, (gi, oi) in enumerate(zip(og, oo)):
+        for a_str, a_arg, o_a in ((0, 2, 0), (1, 3, 1)):
+            o_row, o_ok = o_res[0][o_a][0][oi], o_res[0][o_a][1][oi]
+            gs, gsv = g_res[0][a_str]
+            ga, gav = g_res[0][a_arg]
+            gsv = True if gsv is None else bool(gsv[gi])
+            gav = True if gav is None else bool(gav[gi])
+            assert gsv == bool(o_ok) and gav == bool(o_ok), (j, a_str)
+            if o_ok:
+                assert int(ga[gi]) == int(o_row), (j, a_arg)
+                assert bytes(gs[gi]) == strs[int(o_row)], (j, a_str)
+        assert int(g_res[0][4][0][gi]) == int(o_res[0][2][0][oi])
