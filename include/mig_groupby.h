@@ -77,10 +77,10 @@ enum {
 };
 
 /* ---- aggregation kinds: same numbering as cudf::aggregation::Kind
- *      (cudf/aggregation.hpp:86-122).  The hash path accepts the kinds
- *      listed in hash/groupby.cu:73-84 that this library implements:
- *      SUM, MIN, MAX, COUNT_VALID, COUNT_ALL, SUM_OF_SQUARES, MEAN,
- *      VARIANCE, STD. ---------------------------------------------------- */
+ *      (cudf/aggregation.hpp:86-122).  The hash path accepts the 12 kinds
+ *      listed in hash/groupby.cu:73-84, all implemented here: SUM, PRODUCT,
+ *      MIN, MAX, COUNT_VALID, COUNT_ALL, SUM_OF_SQUARES, MEAN, VARIANCE, STD,
+ *      ARGMIN, ARGMAX (string values: COUNT_*, MIN, MAX, ARGMIN, ARGMAX). --- */
 enum {
   MIG_AGG_SUM            = 0,
   MIG_AGG_PRODUCT        = 1,
