@@ -27,22 +27,27 @@ import numpy as np
 from .aggregation import Aggregation, Kind
 
 _KINDS = {"sum": Kind.SUM, "count": Kind.COUNT_VALID, "min": Kind.MIN, "max": Kind.MAX,
-          "mean": Kind.MEAN, "var": Kind.VARIANCE, "std": Kind.STD, "size": Kind.COUNT_ALL}
+          "mean": Kind.MEAN, "var": Kind.VARIANCE, "std": Kind.STD, "size": Kind.COUNT_ALL,
+          "prod": Kind.PRODUCT}
+# aggregations the GPU takes on string values
+_STRING_VALUE_AGGS = {"count", "size", "min", "max"}
 
 
 class Unsupported(Exception):
     """The GPU path does not take this frame or aggregation."""
 
 
-def _series_to_column(s, for_key: bool):
+def _series_to_column(s, for_key: bool, strings_ok: bool = False):
     """Device column of a pandas Series (NaN and pandas NA become nulls)."""
     import pandas as pd
     from .column import Column
     if isinstance(s.dtype, pd.CategoricalDtype):
         raise Unsupported("categorical columns")
     if s.dtype == object or isinstance(s.dtype, pd.StringDtype):
-        if not for_key:
+        if not (for_key or strings_ok):
             raise Unsupported("string values")
+        if not all(isinstance(v, str) for v in s.dropna().tolist()):
+            raise Unsupported("object column of non-strings")
         valid = ~s.isna().to_numpy()
         vals = [b"" if not ok else str(v).encode() for v, ok in zip(s.tolist(), valid)]
         return Column.from_strings(vals, valid if not valid.all() else None)
@@ -75,7 +80,7 @@ def _gpu(df, by: List[str], spec: Dict[str, List[str]], flat: bool, sort: bool, 
     keys = Table([_series_to_column(df[k], True) for k in by])
     reqs, order = [], []
     for c, aggs in spec.items():
-        col = _series_to_column(df[c], False)
+        col = _series_to_column(df[c], False, strings_ok=set(aggs) <= _STRING_VALUE_AGGS)
         reqs.append(GroupByRequest(col, [Aggregation(_KINDS[a], 1) for a in aggs]))
         order.append((c, aggs))
     rownum = _series_to_column(pd.Series(np.arange(len(df), dtype=np.int64)), False)
@@ -108,11 +113,19 @@ def _gpu(df, by: List[str], spec: Dict[str, List[str]], flat: bool, sort: bool, 
         src = df[c]
         for a, rc in zip(aggs, table.columns()):
             vals, valid = rc.to_numpy()
+            if isinstance(vals, list):  # strings come back as a list of bytes
+                vals = np.array(vals + [b""], dtype=object)[:-1]
             vals = vals[perm]
             valid = None if valid is None else valid[perm]
-            if a == "sum":
+            if rc.type_id.name == "STRING":  # min / max of strings
+                arr = np.array([v.decode() for v in vals], dtype=object)
                 if valid is not None:
-                    vals = np.where(valid, vals, 0).astype(vals.dtype)
+                    arr[~valid] = np.nan
+                vals = arr
+            elif a in ("sum", "prod"):
+                # pandas: an all-missing group sums to 0 and multiplies to 1 (min_count=0)
+                if valid is not None:
+                    vals = np.where(valid, vals, 0 if a == "sum" else 1).astype(vals.dtype)
                 if isinstance(src.dtype, pd.api.extensions.ExtensionDtype):
                     vals = pd.array(vals, dtype=pd.Int64Dtype() if vals.dtype.kind == "i" else src.dtype)
             elif a in ("count", "size"):

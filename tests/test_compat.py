@@ -86,6 +86,35 @@ def test_gpu_path_matches_pandas(by, sort, dropna):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("by", ["k", ["k", "ks"]])
+def test_gpu_prod_and_string_min_max_match_pandas(by):
+    from cudf_amd.pandas_compat import groupby_agg
+    rng = np.random.default_rng(9)
+    df = frame(11)
+    df["p"] = 0.9 + 0.2 * rng.random(len(df))
+    df.loc[rng.random(len(df)) < 0.05, "p"] = np.nan
+    df["q"] = rng.integers(-2, 3, len(df)).astype(np.int64)
+    names = np.array([f"n{v:03d}" for v in rng.integers(0, 500, len(df))], dtype=object)
+    names[rng.random(len(df)) < 0.1] = None
+    df["name"] = pd.array(names, dtype="string")  # pandas' min / max skip <NA> on string dtype
+    spec = {"p": ["prod"], "q": ["prod", "sum"], "name": ["min", "max", "count"]}
+    got, path = groupby_agg(df, by, spec, sort=True, dropna=True, fallback=False)
+    assert path == "gpu"
+    want = df.groupby(by, sort=True, dropna=True).agg(spec)
+    assert list(got.columns) == list(want.columns)
+    for c in want.columns:
+        g, w = got[c].to_numpy(dtype=object), want[c].to_numpy(dtype=object)
+        assert len(g) == len(w)
+        for x, y in zip(g, w):
+            if pd.isna(y):
+                assert pd.isna(x), (c, x, y)
+            elif isinstance(y, str):
+                assert x == y, (c, x, y)
+            else:
+                assert abs(float(x) - float(y)) <= 1e-9 * max(1.0, abs(float(y))), (c, x, y)
+
+
+@pytest.mark.gpu
 def test_arrow_round_trip_and_groupby():
     import pyarrow as pa
     from cudf_amd import GroupBy, GroupByRequest, Table
