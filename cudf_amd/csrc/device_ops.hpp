@@ -340,6 +340,14 @@ __device__ __forceinline__ void acc_combine(int32_t kind, void* a, uint64_t part
     case FK_MAX_U64:
     case FK_MAX_F64: amax(static_cast<uint64_t*>(a), part); break;
     case FK_COUNT: aadd(static_cast<uint32_t*>(a), uint32_t(part)); break;
+    case FK_PROD_I64:
+    case FK_PROD_F32:
+    case FK_PROD_F64: {
+      // multiply by the partial with the row update's compare-and-swap loop
+      loaded_value v{int64_t(part), __longlong_as_double(int64_t(part)), __uint_as_float(uint32_t(part)), part};
+      acc_apply(kind, a, v);
+      break;
+    }
     default: break;
   }
 }

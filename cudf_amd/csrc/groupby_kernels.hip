@@ -1145,6 +1145,9 @@ __device__ __forceinline__ uint64_t sorted_part(int32_t kind, loaded_value const
     case FK_MIN_F64:
     case FK_MAX_F64: return isnan(v.d) ? field_identity(kind) : ord64(v.b);
     case FK_COUNT: return 1;
+    case FK_PROD_I64: return uint64_t(v.i);
+    case FK_PROD_F32: return __float_as_uint(v.f);
+    case FK_PROD_F64: return uint64_t(__double_as_longlong(v.d));
     default: return 0;
   }
 }
@@ -1174,6 +1177,10 @@ __device__ __forceinline__ uint64_t sorted_combine(int32_t kind, uint64_t a, uin
     case FK_MIN_F64: return min(a, b);
     case FK_MAX_U64:
     case FK_MAX_F64: return max(a, b);
+    case FK_PROD_I64: return a * b;
+    case FK_PROD_F32: return __float_as_uint(__uint_as_float(uint32_t(a)) * __uint_as_float(uint32_t(b)));
+    case FK_PROD_F64:
+      return uint64_t(__double_as_longlong(__longlong_as_double(int64_t(a)) * __longlong_as_double(int64_t(b))));
     default: return a;
   }
 }
@@ -1183,7 +1190,7 @@ __device__ __forceinline__ uint64_t sorted_combine(int32_t kind, uint64_t a, uin
 // hold order-preserving unsigned encodings).
 enum seg_op : int32_t {
   OP_ADD_I64 = 0, OP_ADD_F32, OP_ADD_F64, OP_MIN_S32, OP_MAX_S32, OP_MIN_S64, OP_MAX_S64,
-  OP_MIN_U32, OP_MAX_U32, OP_MIN_U64, OP_MAX_U64
+  OP_MIN_U32, OP_MAX_U32, OP_MIN_U64, OP_MAX_U64, OP_MUL_I64, OP_MUL_F32, OP_MUL_F64
 };
 
 __host__ __device__ inline int32_t seg_op_of(int32_t kind)
@@ -1206,6 +1213,9 @@ __host__ __device__ inline int32_t seg_op_of(int32_t kind)
     case FK_MIN_F64: return OP_MIN_U64;
     case FK_MAX_U64:
     case FK_MAX_F64: return OP_MAX_U64;
+    case FK_PROD_I64: return OP_MUL_I64;
+    case FK_PROD_F32: return OP_MUL_F32;
+    case FK_PROD_F64: return OP_MUL_F64;
     default: return OP_ADD_I64;  // integer sums, sums of squares, counts
   }
 }
@@ -1225,6 +1235,10 @@ __device__ __forceinline__ uint64_t op_combine(uint64_t a, uint64_t b)
   if constexpr (OP == OP_MAX_U32) return max(uint32_t(a), uint32_t(b));
   if constexpr (OP == OP_MIN_U64) return min(a, b);
   if constexpr (OP == OP_MAX_U64) return max(a, b);
+  if constexpr (OP == OP_MUL_I64) return a * b;
+  if constexpr (OP == OP_MUL_F32) return __float_as_uint(__uint_as_float(uint32_t(a)) * __uint_as_float(uint32_t(b)));
+  if constexpr (OP == OP_MUL_F64)
+    return uint64_t(__double_as_longlong(__longlong_as_double(int64_t(a)) * __longlong_as_double(int64_t(b))));
   return a;
 }
 
@@ -1274,6 +1288,9 @@ __device__ __forceinline__ uint64_t segmented_scan(int32_t kind, uint64_t part, 
     case OP_MAX_U32: return seg_scan<OP_MAX_U32>(part, g);
     case OP_MIN_U64: return seg_scan<OP_MIN_U64>(part, g);
     case OP_MAX_U64: return seg_scan<OP_MAX_U64>(part, g);
+    case OP_MUL_I64: return seg_scan<OP_MUL_I64>(part, g);
+    case OP_MUL_F32: return seg_scan<OP_MUL_F32>(part, g);
+    case OP_MUL_F64: return seg_scan<OP_MUL_F64>(part, g);
     default: return seg_scan<OP_ADD_I64>(part, g);
   }
 }
@@ -1389,6 +1406,9 @@ __device__ __forceinline__ uint64_t segmented_scan_op(int32_t op, uint64_t part,
     case OP_MAX_U32: return seg_scan<OP_MAX_U32>(part, g);
     case OP_MIN_U64: return seg_scan<OP_MIN_U64>(part, g);
     case OP_MAX_U64: return seg_scan<OP_MAX_U64>(part, g);
+    case OP_MUL_I64: return seg_scan<OP_MUL_I64>(part, g);
+    case OP_MUL_F32: return seg_scan<OP_MUL_F32>(part, g);
+    case OP_MUL_F64: return seg_scan<OP_MUL_F64>(part, g);
     default: return seg_scan<OP_ADD_I64>(part, g);
   }
 }

@@ -132,3 +132,18 @@ def test_sorted_c2_shape_properties():
     assert torch.equal(res.columns()[1].data.view(torch.int32)[:len(uk)].to(torch.int64), counts)
     mx = torch.full((len(uk),), -2**63, dtype=torch.int64, device="cuda").scatter_reduce_(0, seg, val, "amax")
     assert torch.equal(res.columns()[2].data.view(torch.int64)[:len(uk)], mx)
+
+
+@pytest.mark.parametrize("vdt", [np.int32, np.int64, np.float32, np.float64])
+def test_sorted_product(vdt):
+    """PRODUCT on runs (groups of ~20 rows): integers wrap in int64 exactly as
+    the oracle; floats near 1 within the unordered-product bound."""
+    rng = np.random.default_rng(np.dtype(vdt).itemsize + 70)
+    n = 200_000
+    k = np.sort(rng.integers(0, 10_000, n).astype(np.int64))
+    if np.dtype(vdt).kind == "f":
+        v = (0.9 + 0.2 * rng.random(n)).astype(vdt)
+    else:
+        v = rng.integers(-3, 4, n).astype(vdt)
+    vvalid = rng.random(n) > 0.1
+    _check_sorted([host_col(k)], [(host_col(v, vvalid), [(AGG["PRODUCT"], 1), (AGG["SUM"], 1)])], 0)
