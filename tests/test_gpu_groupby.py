@@ -458,3 +458,38 @@ def test_string_min_max(policy):
                 assert int(ga[gi]) == int(o_row), (j, a_arg)
                 assert bytes(gs[gi]) == strs[int(o_row)], (j, a_str)
         assert int(g_res[0][4][0][gi]) == int(o_res[0][2][0][oi])
+
+
+def test_argmin_argmax_large_properties():
+    """20M rows, 2M groups, on the device-wide table: the ARGMIN / ARGMAX row
+    holds the group's MIN / MAX and the group's key, and is the first such row."""
+    import torch
+    import cudf_amd as ca
+    from cudf_amd import GroupBy, GroupByRequest
+    from cudf_amd.aggregation import Aggregation, Kind
+    n = 20_000_000
+    g = torch.Generator(device="cuda").manual_seed(17)
+    key = torch.randint(0, 2_000_000, (n,), device="cuda", dtype=torch.int64, generator=g)
+    val = torch.randint(-50, 50, (n,), device="cuda", dtype=torch.int32, generator=g)
+    kc = ca.Column(ca.TypeId.INT64, n, key.view(torch.uint8))
+    vc = ca.Column(ca.TypeId.INT32, n, val.view(torch.uint8))
+    aggs = [Aggregation(Kind.ARGMIN), Aggregation(Kind.ARGMAX), Aggregation(Kind.MIN), Aggregation(Kind.MAX)]
+    out_keys, (res,) = GroupBy(ca.Table([kc])).aggregate([GroupByRequest(vc, aggs)])
+    G = out_keys.num_rows()
+    uk = out_keys.columns()[0].data.view(torch.int64)[:G]
+    amin = res.columns()[0].data.view(torch.int32)[:G].to(torch.int64)
+    amax = res.columns()[1].data.view(torch.int32)[:G].to(torch.int64)
+    mn = res.columns()[2].data.view(torch.int32)[:G]
+    mx = res.columns()[3].data.view(torch.int32)[:G]
+    assert G == torch.unique(key).numel()
+    assert torch.equal(key[amin], uk) and torch.equal(key[amax], uk)
+    assert torch.equal(val[amin], mn) and torch.equal(val[amax], mx)
+    # first row attaining the extreme: no earlier row of the group holds it
+    order = torch.argsort(uk)
+    pos = torch.searchsorted(uk[order], key)
+    grp = order[pos]  # output group of every row
+    first_min = torch.full((G,), n, dtype=torch.int64, device="cuda")
+    rows = torch.arange(n, device="cuda")
+    hit = val == mn[grp]
+    first_min.scatter_reduce_(0, grp[hit], rows[hit], "amin")
+    assert torch.equal(first_min, amin)
