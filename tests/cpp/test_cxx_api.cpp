@@ -139,6 +139,19 @@ int main()
   // sum_tests.cpp:38-54, 110-132
   run_case<int64_t>("sum basic", K, {}, V, {}, cudf::make_sum_aggregation<gba>(),
                     {{1, {9, true}}, {2, {19, true}}, {3, {17, true}}});
+  // argmin_tests.cpp:94-118, argmax_tests.cpp:93-116, product_tests.cpp:115-139 (int32 values)
+  run_case<int32_t>("argmin null keys and values", {1, 2, 3, 1, 2, 2, 1, 3, 3, 2, 4},
+                    {1, 1, 1, 1, 1, 1, 1, 0, 1, 1, 1}, {9, 8, 7, 6, 5, 4, 3, 2, 1, 0, 4},
+                    {1, 1, 1, 1, 1, 0, 0, 1, 1, 1, 0}, cudf::make_argmin_aggregation<gba>(),
+                    {{1, {3, true}}, {2, {9, true}}, {3, {8, true}}, {4, {0, false}}});
+  run_case<int32_t>("argmax null keys and values", {1, 2, 3, 1, 2, 2, 1, 3, 3, 2, 4},
+                    {1, 1, 0, 1, 1, 1, 1, 1, 1, 1, 1}, {9, 8, 7, 6, 5, 4, 3, 2, 1, 0, 4},
+                    {0, 0, 1, 1, 1, 1, 1, 1, 1, 1, 0}, cudf::make_argmax_aggregation<gba>(),
+                    {{1, {3, true}}, {2, {4, true}}, {3, {7, true}}, {4, {0, false}}});
+  run_case<int64_t>("product null keys and values", {1, 2, 3, 1, 2, 2, 1, 3, 3, 2, 4},
+                    {1, 1, 1, 1, 1, 1, 1, 0, 1, 1, 1}, {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 3},
+                    {0, 1, 1, 1, 1, 0, 1, 1, 1, 1, 0}, cudf::make_product_aggregation<gba>(),
+                    {{1, {18, true}}, {2, {36, true}}, {3, {16, true}}, {4, {0, false}}});
   run_case<int64_t>("sum null keys and values", KN, KNV, VN, VNV, cudf::make_sum_aggregation<gba>(),
                     {{1, {9, true}}, {2, {14, true}}, {3, {10, true}}, {4, {0, false}}});
   // count_tests.cpp:32-51, 114-140
