@@ -668,10 +668,9 @@ std::unique_ptr<mig_groupby_result> groupby_aggregate(mig_column_view const* key
   // ARGMIN / ARGMAX (a row index updated by compare-and-swap against the
   // values column) and PRODUCT (compare-and-swap multiply): only the
   // device-wide table path implements them
-  bool any_arg = false, any_row_index = false;
+  bool any_arg = false;
   for (int f = 0; f < nf; f++) {
     int32_t const k = P.fields[f].kind;
-    any_row_index |= k == FK_ARGMIN || k == FK_ARGMAX;
     any_arg |= k == FK_ARGMIN || k == FK_ARGMAX || k == FK_PROD_I64 || k == FK_PROD_F32 || k == FK_PROD_F64;
   }
 
@@ -892,7 +891,9 @@ std::unique_ptr<mig_groupby_result> groupby_aggregate(mig_column_view const* key
   // (the reference's sort path with keys_pre_sorted, sort/sort_helper.cu:55-72,
   // 133-181); groups come out in input order.
   if (keys_sorted) {
-    MIG_EXPECTS(!any_row_index, "ARGMIN / ARGMAX (and string MIN / MAX) are not supported with pre-sorted keys");
+    for (size_t j = 0; j < vals.size(); j++)
+      MIG_EXPECTS(!(vals[j].storage == ST_STRING && (vals[j].want_argmin || vals[j].want_argmax)),
+                  "string MIN / MAX / ARGMIN / ARGMAX are not supported with pre-sorted keys");
     P.key_mode            = KM_GENERAL;  // keys are gathered from each group's first row
     int64_t const ntiles  = sorted_tiles(N);
     uint32_t* tile_counts = S.get<uint32_t>(ntiles + 1);

@@ -1413,6 +1413,65 @@ __device__ __forceinline__ uint64_t segmented_scan_op(int32_t op, uint64_t part,
   }
 }
 
+// ARGMIN / ARGMAX on runs: a row's partial is (order key of its value, row),
+// reduced lexicographically (smallest / largest key, then the smallest row),
+// which is the device-wide path's rule (first row among equal values, NaN
+// only in an all-NaN group; -0.0 equals 0.0).  Rows without a valid value
+// carry the identity key and row 0xFFFFFFFF.
+template <bool IS_MIN>
+__device__ __forceinline__ void arg_pick(uint64_t& key, uint32_t& row, uint64_t okey, uint32_t orow)
+{
+  bool const take = IS_MIN ? (okey < key || (okey == key && orow < row)) : (okey > key || (okey == key && orow < row));
+  if (take) {
+    key = okey;
+    row = orow;
+  }
+}
+
+__device__ __forceinline__ uint64_t arg_order_key(bool is_min, int32_t st, loaded_value const& v)
+{
+  if (st == ST_F32 || st == ST_F64) {
+    if (isnan(v.d)) return is_min ? ~0ull : 0ull;  // a NaN never beats a number
+    double const d = v.d == 0.0 ? 0.0 : v.d;        // -0.0 equals 0.0
+    return ord64(uint64_t(__double_as_longlong(d)));
+  }
+  if (is_unsigned_storage_dev(st)) return uint64_t(v.i);
+  return uint64_t(v.i) ^ 0x8000000000000000ull;
+}
+
+template <bool IS_MIN, int CTRL, int ROW_MASK>
+__device__ __forceinline__ void arg_step(uint64_t& key, uint32_t& row, uint32_t g)
+{
+  constexpr int NOT_A_GROUP = int(0xFFFFFFFEu);
+  constexpr int ID_WORD     = IS_MIN ? -1 : 0;
+  uint32_t const klo = uint32_t(__builtin_amdgcn_update_dpp(ID_WORD, int(uint32_t(key)), CTRL, ROW_MASK, 0xF, false));
+  uint32_t const khi = uint32_t(__builtin_amdgcn_update_dpp(ID_WORD, int(uint32_t(key >> 32)), CTRL, ROW_MASK, 0xF, false));
+  uint32_t const orow = uint32_t(__builtin_amdgcn_update_dpp(-1, int(row), CTRL, ROW_MASK, 0xF, false));
+  uint32_t const og   = uint32_t(__builtin_amdgcn_update_dpp(NOT_A_GROUP, int(g), CTRL, ROW_MASK, 0xF, false));
+  if (og == g) arg_pick<IS_MIN>(key, row, (uint64_t(khi) << 32) | klo, orow);
+}
+
+template <bool IS_MIN>
+__device__ __forceinline__ void arg_scan(uint64_t& key, uint32_t& row, uint32_t g)
+{
+  arg_step<IS_MIN, 0x111, 0xF>(key, row, g);  // row_shr:1
+  arg_step<IS_MIN, 0x112, 0xF>(key, row, g);  // row_shr:2
+  arg_step<IS_MIN, 0x114, 0xF>(key, row, g);  // row_shr:4
+  arg_step<IS_MIN, 0x118, 0xF>(key, row, g);  // row_shr:8
+  arg_step<IS_MIN, 0x142, 0xA>(key, row, g);  // row_bcast:15
+  arg_step<IS_MIN, 0x143, 0xC>(key, row, g);  // row_bcast:31
+}
+
+// A finished group's row: stored when no other wave holds rows of the group,
+// otherwise merged by the device-wide path's compare-and-swap on the values.
+__device__ __forceinline__ void arg_flush(gb_params const& P, int32_t f, uint32_t g, uint32_t row, bool shared)
+{
+  if (row == 0xFFFFFFFFu) return;  // no valid value: the accumulator keeps "none"
+  field_desc const& F = P.fields[f];
+  if (shared) arg_update(F, g, P.vals[F.vcol], int64_t(row));
+  else *gp(reinterpret_cast<uint32_t*>(field_at(F, g))) = row;
+}
+
 // Each wave reduces a contiguous range of rows in 64-row chunks; the group
 // still open at a chunk's last lane is carried into the next chunk.  Groups
 // that start and end inside the range are stored once; the range's first and
@@ -1426,6 +1485,7 @@ __global__ void __launch_bounds__(BLOCK)
   // field loop stays rolled, which keeps the kernel small enough for the
   // instruction cache
   __shared__ uint64_t carry[BLOCK / 64][SORTED_FPP];
+  __shared__ uint32_t carry_row[BLOCK / 64][SORTED_FPP];  // ARGMIN / ARGMAX partials
   gb_params const& P = *Pp;
   int const lane     = threadIdx.x & 63;
   int const wave     = threadIdx.x >> 6;
@@ -1469,10 +1529,33 @@ __global__ void __launch_bounds__(BLOCK)
       int32_t const cs   = pass.col_of[k];
       int32_t const kind = pass.kind[k];
       uint64_t const c   = carry[wave][k];
-      if (carry_ends && lane == 0) sorted_flush(P, f, carry_g, c, carry_g == first_g);
       // the field's column among the loaded ones (uniform select, no indexing)
       loaded_value const v  = cs <= 0 ? lv[0] : cs == 1 ? lv[1] : cs == 2 ? lv[2] : lv[3];
       bool const vv         = cs <= 0 ? vld[0] : cs == 1 ? vld[1] : cs == 2 ? vld[2] : vld[3];
+      if (kind == FK_ARGMIN || kind == FK_ARGMAX) {
+        bool const is_min  = kind == FK_ARGMIN;
+        uint32_t const crow = carry_row[wave][k];
+        if (carry_ends && lane == 0) arg_flush(P, f, carry_g, crow, carry_g == first_g);
+        bool const valid = g != NO_GROUP && vv;
+        uint64_t key     = valid ? arg_order_key(is_min, P.vals[pass.col[cs]].storage, v) : (is_min ? ~0ull : 0ull);
+        uint32_t r       = valid ? uint32_t(row) : 0xFFFFFFFFu;
+        if (lane == 0 && carry_g != NO_GROUP && g == carry_g) {
+          if (is_min) arg_pick<true>(key, r, c, crow);
+          else arg_pick<false>(key, r, c, crow);
+        }
+        if (is_min) arg_scan<true>(key, r, g);
+        else arg_scan<false>(key, r, g);
+        if (ends_here) arg_flush(P, f, g, r, g == first_g);
+        uint64_t const klast = (uint64_t(uint32_t(__builtin_amdgcn_readlane(int(uint32_t(key >> 32)), 63))) << 32) |
+                               uint32_t(__builtin_amdgcn_readlane(int(uint32_t(key)), 63));
+        uint32_t const rlast = uint32_t(__builtin_amdgcn_readlane(int(r), 63));
+        if (lane == 0) {
+          carry[wave][k]     = klast;
+          carry_row[wave][k] = rlast;
+        }
+        continue;
+      }
+      if (carry_ends && lane == 0) sorted_flush(P, f, carry_g, c, carry_g == first_g);
       uint64_t part         = sorted_row_part(P, f, kind, pass.ident[k], g, vv, v);
       if (lane == 0 && carry_g != NO_GROUP && g == carry_g) part = sorted_combine(kind, part, c);
       part = segmented_scan_op(pass.op[k], part, g);
@@ -1484,8 +1567,12 @@ __global__ void __launch_bounds__(BLOCK)
     carry_g = uint32_t(__builtin_amdgcn_readlane(int(g), 63));
   }
   if (carry_g != NO_GROUP && lane == 0) {
-    for (int k = 0; k < pass.n; k++)
-      sorted_flush(P, pass.field[k], carry_g, carry[wave][k], true);  // may continue in the next range
+    for (int k = 0; k < pass.n; k++) {  // may continue in the next range
+      if (pass.kind[k] == FK_ARGMIN || pass.kind[k] == FK_ARGMAX)
+        arg_flush(P, pass.field[k], carry_g, carry_row[wave][k], true);
+      else
+        sorted_flush(P, pass.field[k], carry_g, carry[wave][k], true);
+    }
   }
 }
 

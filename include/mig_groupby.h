@@ -267,7 +267,8 @@ mig_status mig_groupby_aggregate(const mig_column_view* keys,
  * keys come back sorted; unsorted keys give one group per run.  Rows with a
  * null key are dropped under MIG_NULL_EXCLUDE (the reference re-sorts such
  * keys first, which gives the same groups when the keys are sorted).  Same
- * aggregations, results and errors as mig_groupby_aggregate. */
+ * aggregations, results and errors as mig_groupby_aggregate, except that
+ * MIN / MAX / ARGMIN / ARGMAX of string values return MIG_ERROR_LOGIC. */
 mig_status mig_groupby_aggregate_sorted(const mig_column_view* keys,
                                         int32_t num_keys,
                                         int32_t null_policy,

@@ -147,3 +147,30 @@ def test_sorted_product(vdt):
         v = rng.integers(-3, 4, n).astype(vdt)
     vvalid = rng.random(n) > 0.1
     _check_sorted([host_col(k)], [(host_col(v, vvalid), [(AGG["PRODUCT"], 1), (AGG["SUM"], 1)])], 0)
+
+
+@pytest.mark.parametrize("vdt", [np.int8, np.int32, np.int64, np.uint64, np.float32, np.float64])
+def test_sorted_argmin_argmax(vdt):
+    """ARGMIN / ARGMAX on runs: row indices bit-exact against the oracle, with
+    many equal values per group (ties take the smaller row), NaN, -0.0 / 0.0,
+    the type's extremes, null values and groups long enough to span waves."""
+    rng = np.random.default_rng(np.dtype(vdt).itemsize + 90)
+    n = 300_000
+    k = np.sort(np.concatenate([rng.integers(0, 8_000, n - 20_000), np.full(20_000, 9_000)]).astype(np.int64))
+    if np.dtype(vdt).kind == "f":
+        v = rng.integers(-5, 6, n).astype(vdt)
+        v[rng.random(n) < 0.02] = np.nan
+        v[rng.random(n) < 0.02] = -0.0
+        v[k == 77] = np.nan  # an all-NaN group
+    else:
+        info = np.iinfo(vdt)
+        v = rng.integers(max(info.min, -5), 6, n).astype(vdt)
+        v[rng.random(n) < 0.01] = info.min
+        v[rng.random(n) < 0.01] = info.max
+    vvalid = rng.random(n) > 0.1
+    vvalid[k == 55] = False  # a group without a valid value
+    kvalid = np.ones(n, bool)
+    kvalid[-500:] = False
+    for policy in (0, 1):
+        _check_sorted([host_col(k, kvalid)], [(host_col(v, vvalid), [(AGG["ARGMIN"], 1), (AGG["ARGMAX"], 1),
+                                                                  (AGG["COUNT_VALID"], 1)])], policy)
