@@ -30,9 +30,10 @@ class Kind(enum.IntEnum):
 
 
 class Aggregation:
-    def __init__(self, kind: Kind, ddof: int = 1):
+    def __init__(self, kind: Kind, ddof: int = 1, null_handling: NullPolicy = NullPolicy.EXCLUDE):
         self._kind = Kind(kind)
         self.ddof = int(ddof)
+        self.null_handling = NullPolicy(null_handling)  # NUNIQUE only
 
     def kind(self) -> Kind:
         return self._kind
@@ -77,6 +78,12 @@ def variance(ddof: int = 1) -> Aggregation:
 
 def std(ddof: int = 1) -> Aggregation:
     return Aggregation(Kind.STD, ddof)
+
+
+def nunique(null_handling: NullPolicy = NullPolicy.EXCLUDE) -> Aggregation:
+    """Distinct values per group; INCLUDE counts nulls as one value
+    (aggregation.hpp make_nunique_aggregation).  Needs keys_are_sorted=YES."""
+    return Aggregation(Kind.NUNIQUE, null_handling=null_handling)
 
 
 def argmax() -> Aggregation:

@@ -311,6 +311,7 @@ char const* kind_name(int32_t k)
     case MIG_AGG_STD: return "STD";
     case MIG_AGG_ARGMIN: return "ARGMIN";
     case MIG_AGG_ARGMAX: return "ARGMAX";
+    case MIG_AGG_NUNIQUE: return "NUNIQUE";
     default: return "UNKNOWN";
   }
 }
@@ -324,7 +325,8 @@ int32_t target_type_of(int32_t src, int32_t kind)
     case MIG_AGG_COUNT_VALID:
     case MIG_AGG_COUNT_ALL:
     case MIG_AGG_ARGMIN:  // size_type row index (aggregation.hpp:1346-1357)
-    case MIG_AGG_ARGMAX: return MIG_TYPE_INT32;
+    case MIG_AGG_ARGMAX:
+    case MIG_AGG_NUNIQUE: return MIG_TYPE_INT32;  // size_type count (group_nunique.cu:90-91)
     case MIG_AGG_MEAN:
     case MIG_AGG_VARIANCE:
     case MIG_AGG_STD: return MIG_TYPE_FLOAT64;
@@ -355,6 +357,10 @@ void check_supported(int32_t src, int32_t kind)
     case MIG_AGG_MEAN:
     case MIG_AGG_VARIANCE:
     case MIG_AGG_STD: ok = is_numeric_type(src); break;
+    case MIG_AGG_NUNIQUE:
+      if (src == MIG_TYPE_STRING) fail(MIG_ERROR_LOGIC, "NUNIQUE of string values is not supported");
+      ok = is_numeric_type(src) || is_chrono_type(src);
+      break;
     default:
       fail(MIG_ERROR_LOGIC, std::string("aggregation ") + kind_name(kind) +
                               " is not supported by the hash group-by");
@@ -373,6 +379,8 @@ struct value_plan {
   bool want_count = false, want_m2 = false, want_argmin = false, want_argmax = false, want_prod = false;
   int32_t f_sum = -1, f_sumsq = -1, f_min = -1, f_max = -1, f_count = -1, f_m2 = -1;
   int32_t f_argmin = -1, f_argmax = -1, f_prod = -1;
+  bool want_nunique[2] = {false, false};  // [null_handling == INCLUDE]
+  int32_t f_nunique[2] = {-1, -1};
 };
 
 int32_t sum_field_kind(int32_t st)
@@ -417,6 +425,8 @@ int32_t field_width(int32_t fk)
     case FK_COUNT:
     case FK_ARGMIN:
     case FK_ARGMAX:
+    case FK_NUNIQUE:
+    case FK_NUNIQUE_NULLS:
     case FK_PROD_F32: return 4;
     default: return 8;
   }
@@ -531,8 +541,12 @@ std::unique_ptr<mig_groupby_result> groupby_aggregate(mig_column_view const* key
     MIG_EXPECTS(requests[r].values.size == N,
                 "Size mismatch between request values and groupby keys.");
     MIG_EXPECTS(requests[r].num_aggregations >= 0, "negative number of aggregations");
-    for (int a = 0; a < requests[r].num_aggregations; a++)
+    for (int a = 0; a < requests[r].num_aggregations; a++) {
       check_supported(requests[r].values.type_id, requests[r].aggregations[a].kind);
+      // a sort-based aggregation (groupby.cu:78-83): taken on runs of pre-sorted keys only
+      if (requests[r].aggregations[a].kind == MIG_AGG_NUNIQUE && !keys_sorted)
+        fail(MIG_ERROR_LOGIC, "NUNIQUE needs keys_are_sorted == YES (the hash group-by does not take it)");
+    }
   }
 
   auto res = std::make_unique<mig_groupby_result>();
@@ -616,6 +630,9 @@ std::unique_ptr<mig_groupby_result> groupby_aggregate(mig_column_view const* key
         case MIG_AGG_PRODUCT: vp.want_prod = true; vp.want_count |= vp.has_nulls; break;
         case MIG_AGG_ARGMIN: vp.want_argmin = true; break;
         case MIG_AGG_ARGMAX: vp.want_argmax = true; break;
+        case MIG_AGG_NUNIQUE:
+          vp.want_nunique[requests[r].aggregations[a].null_handling == MIG_NULL_INCLUDE] = true;
+          break;
         default: break;
       }
     }
@@ -644,6 +661,8 @@ std::unique_ptr<mig_groupby_result> groupby_aggregate(mig_column_view const* key
     if (vp.want_sumsq) vp.f_sumsq = add_field(int(j), sumsq_field_kind(vp.storage));
     if (vp.want_min) vp.f_min = add_field(int(j), minmax_field_kind(vp.storage, true));
     if (vp.want_max) vp.f_max = add_field(int(j), minmax_field_kind(vp.storage, false));
+    for (int i = 0; i < 2; i++)
+      if (vp.want_nunique[i]) vp.f_nunique[i] = add_field(int(j), i ? FK_NUNIQUE_NULLS : FK_NUNIQUE);
     if (vp.want_count) {
       if (vp.has_nulls) vp.f_count = add_field(int(j), FK_COUNT);
       else {
@@ -820,6 +839,10 @@ std::unique_ptr<mig_groupby_result> groupby_aggregate(mig_column_view const* key
             O.field   = kind == MIG_AGG_ARGMIN ? vp.f_argmin : vp.f_argmax;
             with_mask = vp.has_nulls;
             break;
+          case MIG_AGG_NUNIQUE:
+            O.kind  = OK_COUNT;
+            O.field = vp.f_nunique[requests[r].aggregations[a].null_handling == MIG_NULL_INCLUDE];
+            break;
           case MIG_AGG_VARIANCE:
           case MIG_AGG_STD:
             O.kind        = kind == MIG_AGG_VARIANCE ? OK_VAR : OK_STD;
@@ -928,6 +951,11 @@ std::unique_ptr<mig_groupby_result> groupby_aggregate(mig_column_view const* key
     T.mark(3);
     T.mark(4);
     if (P.need_second_pass) launch_sorted_second_pass(dP, P, N, gid, dp.num_cus, stream);
+    for (int f = 0; f < nf; f++) {
+      if (P.fields[f].kind != FK_NUNIQUE && P.fields[f].kind != FK_NUNIQUE_NULLS) continue;
+      uint64_t const cap = next_pow2(std::max<uint64_t>(2 * uint64_t(N), 64));
+      launch_sorted_nunique(dP, f, N, gid, S.get<uint32_t>(int64_t(cap)), cap, stream);
+    }
     T.mark(5);
     uint32_t* map = S.get<uint32_t>(std::max<int64_t>(G, 1));
     launch_iota_u32(map, G, stream);

@@ -1611,8 +1611,10 @@ static void launch_sorted_passes(gb_params const* dP, gb_params const& hP, int64
 {
   std::vector<int32_t> items;
   if (!m2 && hP.need_count_all) items.push_back(-1);
-  for (int f = 0; f < hP.num_fields; f++)
+  for (int f = 0; f < hP.num_fields; f++) {
+    if (hP.fields[f].kind == FK_NUNIQUE || hP.fields[f].kind == FK_NUNIQUE_NULLS) continue;  // own kernel
     if ((hP.fields[f].kind == FK_M2) == m2) items.push_back(f);
+  }
   int64_t per, W;
   sorted_ranges(n, num_cus, per, W);
   unsigned const blocks = unsigned((W + BLOCK / 64 - 1) / (BLOCK / 64));
@@ -1655,6 +1657,80 @@ void launch_sorted_second_pass(gb_params const* dP, gb_params const& hP, int64_t
                                int num_cus, hipStream_t s)
 {
   launch_sorted_passes(dP, hP, n, gid, num_cus, true, s);
+}
+
+// NUNIQUE on runs.  The reference sorts each group's values and counts the
+// heads of equal runs (sort/group_nunique.cu:44-64); here every counted row
+// inserts (group, value) into an open-addressing set of row indices, and the
+// row that claims an empty slot adds one distinct value to its group.  Rows
+// are equal when they share the group and both are null (counted only with
+// nulls included) or hold the same value, -0.0 equal to 0.0 and NaN equal to
+// NaN (the row comparator's defaults).  The set has at least twice as many
+// slots as rows, so every probe sequence reaches an empty slot or a match.
+__device__ __forceinline__ uint64_t nunique_bits(dcol const& c, int64_t row)
+{
+  uint64_t b = raw_bits(c, row);
+  if (c.storage == ST_F32) {
+    float const x = __uint_as_float(uint32_t(b));
+    b             = isnan(x) ? 0x7fc00000ull : (x == 0.0f ? 0ull : b);
+  } else if (c.storage == ST_F64) {
+    double const x = __longlong_as_double(int64_t(b));
+    b              = isnan(x) ? 0x7ff8000000000000ull : (x == 0.0 ? 0ull : b);
+  } else if (c.storage == ST_BOOL) {
+    b = b != 0;
+  }
+  return b;
+}
+
+__global__ void __launch_bounds__(BLOCK) k_fill_empty(uint32_t* __restrict__ out, int64_t n)
+{
+  for (int64_t i = int64_t(blockIdx.x) * BLOCK + threadIdx.x; i < n; i += int64_t(gridDim.x) * BLOCK)
+    out[i] = 0xFFFFFFFFu;
+}
+
+__global__ void __launch_bounds__(BLOCK)
+  k_sorted_nunique(gb_params const* __restrict__ Pp, int32_t f, int64_t n, uint32_t const* __restrict__ gid,
+                   uint32_t* __restrict__ table, uint64_t cap_mask)
+{
+  constexpr uint32_t EMPTY = 0xFFFFFFFFu;
+  gb_params const& P   = *Pp;
+  field_desc const& F  = P.fields[f];
+  dcol const& c        = P.vals[F.vcol];
+  bool const with_null = F.kind == FK_NUNIQUE_NULLS;
+  for (int64_t row = int64_t(blockIdx.x) * BLOCK + threadIdx.x; row < n; row += int64_t(gridDim.x) * BLOCK) {
+    uint32_t const g = gid[row];
+    if (g == NO_GROUP) continue;
+    bool const valid = dcol_valid(c, row);
+    if (!valid && !with_null) continue;
+    uint64_t const b = valid ? nunique_bits(c, row) : 0ull;
+    uint32_t const h = mm_fmix32(uint32_t(b) ^ mm_fmix32(uint32_t(b >> 32) ^ mm_fmix32(g * 2u + uint32_t(valid))));
+    uint64_t slot    = h & cap_mask;
+    for (;;) {
+      uint32_t cur = __hip_atomic_load(&table[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (cur == EMPTY) {
+        uint32_t const prev = atomicCAS(&table[slot], EMPTY, uint32_t(row));
+        if (prev == EMPTY) {
+          atomicAdd(reinterpret_cast<unsigned int*>(field_at(F, g)), 1u);
+          break;
+        }
+        cur = prev;
+      }
+      if (gid[cur] == g) {
+        bool const cv = dcol_valid(c, cur);
+        if (cv == valid && (!valid || nunique_bits(c, cur) == b)) break;  // already counted
+      }
+      slot = (slot + 1) & cap_mask;
+    }
+  }
+}
+
+void launch_sorted_nunique(gb_params const* dP, int32_t f, int64_t n, uint32_t const* gid, uint32_t* table,
+                           uint64_t cap, hipStream_t s)
+{
+  if (n <= 0) return;
+  auto const grid = [](int64_t m) { return dim3(unsigned(std::min<int64_t>((m + BLOCK - 1) / BLOCK, 1 << 20))); };
+  hipLaunchKernelGGL(k_fill_empty, grid(int64_t(cap)), dim3(BLOCK), 0, s, table, int64_t(cap));
+  hipLaunchKernelGGL(k_sorted_nunique, grid(n), dim3(BLOCK), 0, s, dP, f, n, gid, table, cap - 1);
 }
 
 // String MIN / MAX from the groups' ARGMIN / ARGMAX rows: lengths (0 for

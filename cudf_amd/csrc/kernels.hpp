@@ -53,7 +53,9 @@ enum field_kind : int32_t {
   FK_ARGMAX, // row of the largest valid value
   FK_PROD_I64,  // products (compare-and-swap loops)
   FK_PROD_F32,
-  FK_PROD_F64
+  FK_PROD_F64,
+  FK_NUNIQUE,        // distinct valid values (u32; filled by the pre-sorted path's set insert)
+  FK_NUNIQUE_NULLS   // distinct values, nulls counted as one value
 };
 
 struct field_desc {
@@ -258,6 +260,11 @@ void launch_sorted_aggregate(gb_params const* dP, gb_params const& hP, int64_t n
                              hipStream_t s);
 void launch_sorted_second_pass(gb_params const* dP, gb_params const& hP, int64_t n, uint32_t const* gid,
                                int num_cus, hipStream_t s);
+// NUNIQUE of field `f` on runs (`gid` = group of every row, NO_GROUP when
+// dropped): a set of (group, value) in `table` (`cap` u32 slots, a power of two
+// of at least 2n; cleared here) counts each group's distinct values.
+void launch_sorted_nunique(gb_params const* dP, int32_t f, int64_t n, uint32_t const* gid, uint32_t* table,
+                           uint64_t cap, hipStream_t s);
 void launch_iota_u32(uint32_t* out, int64_t n, hipStream_t s);
 // string MIN / MAX: lengths and validity of the values at the ARGMIN / ARGMAX rows (-1 = none)
 void launch_str_arg_sizes(dcol const& src, int32_t const* rows, int64_t G, int32_t* sizes, uint32_t* mask,

@@ -31,7 +31,10 @@ class aggregation {
     ARGMIN,
     NUNIQUE
   };
-  explicit aggregation(Kind k, size_type ddof = 1) : kind{k}, _ddof{ddof} {}
+  explicit aggregation(Kind k, size_type ddof = 1, null_policy nulls = null_policy::EXCLUDE)
+    : kind{k}, _ddof{ddof}, _null_handling{nulls}
+  {
+  }
   virtual ~aggregation() = default;
   [[nodiscard]] virtual std::unique_ptr<aggregation> clone() const
   {
@@ -39,12 +42,16 @@ class aggregation {
   }
   Kind kind;
   size_type _ddof;  // VARIANCE / STD
+  null_policy _null_handling;  // NUNIQUE
 };
 
 /// Aggregations valid in a group-by (aggregation.hpp groupby_aggregation).
 class groupby_aggregation : public virtual aggregation {
  public:
-  explicit groupby_aggregation(Kind k, size_type ddof = 1) : aggregation(k, ddof) {}
+  explicit groupby_aggregation(Kind k, size_type ddof = 1, null_policy nulls = null_policy::EXCLUDE)
+    : aggregation(k, ddof, nulls)
+  {
+  }
   [[nodiscard]] std::unique_ptr<aggregation> clone() const override
   {
     return std::make_unique<groupby_aggregation>(*this);
@@ -97,6 +104,13 @@ template <typename Base = aggregation>
 std::unique_ptr<Base> make_mean_aggregation()
 {
   return std::make_unique<Base>(aggregation::MEAN);
+}
+/// Distinct values per group (aggregation.hpp make_nunique_aggregation);
+/// taken with keys_are_sorted == sorted::YES.
+template <typename Base = aggregation>
+std::unique_ptr<Base> make_nunique_aggregation(null_policy null_handling = null_policy::EXCLUDE)
+{
+  return std::make_unique<Base>(aggregation::NUNIQUE, 1, null_handling);
 }
 template <typename Base = aggregation>
 std::unique_ptr<Base> make_variance_aggregation(size_type ddof = 1)

@@ -79,7 +79,8 @@ class groupby {
     std::vector<mig_aggregation_request> reqs(requests.size());
     for (size_t r = 0; r < requests.size(); r++) {
       for (auto const& a : requests[r].aggregations)
-        aggs[r].push_back(mig_aggregation{static_cast<int32_t>(a->kind), a->_ddof});
+        aggs[r].push_back(mig_aggregation{static_cast<int32_t>(a->kind), a->_ddof,
+                                          static_cast<int32_t>(a->_null_handling)});
       reqs[r].values           = requests[r].values.c_view();
       reqs[r].aggregations     = aggs[r].data();
       reqs[r].num_aggregations = int32_t(aggs[r].size());

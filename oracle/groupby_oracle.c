@@ -14,7 +14,9 @@
  *   orc_groupby          the known-answer tests of cpp/tests/groupby/
  *                        {sum,count,min,max,mean,var,keys}_tests.cpp, brute
  *                        force over tiny inputs, and pandas group-by on the
- *                        same seeded data (tests/test_oracle.py)
+ *                        same seeded data (tests/test_oracle.py); NUNIQUE
+ *                        by cpp/tests/groupby/nunique_tests.cpp and a
+ *                        set-based brute force
  */
 #include <math.h>
 #include <stdint.h>
@@ -30,7 +32,8 @@ enum {
 /* aggregation kinds as in cudf/aggregation.hpp:86-122 */
 enum {
   A_SUM = 0, A_PRODUCT = 1, A_MIN = 2, A_MAX = 3, A_COUNT_VALID = 4, A_COUNT_ALL = 5,
-  A_SUM_OF_SQUARES = 8, A_MEAN = 9, A_VARIANCE = 11, A_STD = 12, A_ARGMAX = 15, A_ARGMIN = 16
+  A_SUM_OF_SQUARES = 8, A_MEAN = 9, A_VARIANCE = 11, A_STD = 12, A_ARGMAX = 15, A_ARGMIN = 16,
+  A_NUNIQUE = 17
 };
 
 typedef struct orc_col {
@@ -221,6 +224,7 @@ int32_t orc_target_type(int32_t src, int32_t kind)
     case A_MIN: case A_MAX: return src;
     case A_COUNT_VALID: case A_COUNT_ALL: return T_INT32;
     case A_ARGMIN: case A_ARGMAX: return T_INT32; /* size_type row index, aggregation.hpp:1346-1357 */
+    case A_NUNIQUE: return T_INT32;               /* size_type count, group_nunique.cu:90-91 */
     case A_MEAN: case A_VARIANCE: case A_STD: return T_FLOAT64;
     case A_SUM: case A_SUM_OF_SQUARES: case A_PRODUCT:
       if (is_float(src)) return src;
@@ -261,6 +265,40 @@ static int value_cmp(const orc_col* c, int64_t a, int64_t b)
   }
   int64_t x = as_i64(c, a), y = as_i64(c, b);
   return x < y ? -1 : (x > y ? 1 : 0);
+}
+
+/* NUNIQUE (cpp/src/groupby/sort/group_nunique.cu:44-64): the reference sorts
+ * each group's values and counts a row when it is countable (valid, or any
+ * row under null_policy INCLUDE) and it opens the group or differs from its
+ * predecessor; nulls equal each other, NaN equals NaN and -0.0 equals 0.0
+ * (the row comparator's defaults).  Restated the same way: the group's rows
+ * sorted by (group, null first, value) and the heads of equal runs counted. */
+static const orc_col* g_nu_col;
+static const int64_t* g_nu_group;
+static int nunique_cmp(const void* pa, const void* pb)
+{
+  int64_t a = *(const int64_t*)pa, b = *(const int64_t*)pb;
+  if (g_nu_group[a] != g_nu_group[b]) return g_nu_group[a] < g_nu_group[b] ? -1 : 1;
+  int va = row_valid(g_nu_col, a), vb = row_valid(g_nu_col, b);
+  if (va != vb) return va < vb ? -1 : 1;
+  if (!va) return 0;
+  return value_cmp(g_nu_col, a, b);
+}
+
+static void nunique_counts(const orc_col* v, const int64_t* group_of_row, int64_t n, int include_nulls,
+                           int64_t G, int32_t* out)
+{
+  int64_t* rows = (int64_t*)malloc(sizeof(int64_t) * (n > 0 ? n : 1));
+  int64_t m = 0;
+  for (int64_t r = 0; r < n; r++)
+    if (group_of_row[r] >= 0 && (include_nulls || row_valid(v, r))) rows[m++] = r;
+  g_nu_col = v;
+  g_nu_group = group_of_row;
+  qsort(rows, (size_t)m, sizeof(int64_t), nunique_cmp);
+  for (int64_t g = 0; g < G; g++) out[g] = 0;
+  for (int64_t i = 0; i < m; i++)
+    if (i == 0 || nunique_cmp(&rows[i - 1], &rows[i]) != 0) out[group_of_row[rows[i]]]++;
+  free(rows);
 }
 
 /* ---- open-addressing map row -> group, first-occurrence group order ---- */
@@ -340,7 +378,12 @@ int64_t orc_groupby(const orc_col* keys, int32_t nkeys, int32_t null_policy,
     int src = v->type_id;
     int tgt = orc_target_type(src, k);
     if (tgt < 0 || (src == T_STRING && k != A_COUNT_VALID && k != A_COUNT_ALL && k != A_ARGMIN &&
-                    k != A_ARGMAX)) { ok = 0; break; }
+                    k != A_ARGMAX && k != A_NUNIQUE)) { ok = 0; break; }
+    if (k == A_NUNIQUE) {  /* ddofs[a] holds the null policy: 0 EXCLUDE, 1 INCLUDE */
+      nunique_counts(v, group_of_row, n, ddofs[a] != 0, G, (int32_t*)outs[a]);
+      for (int64_t g = 0; g < G; g++) out_valid[a][g] = 1;
+      continue;
+    }
     if (k == A_ARGMIN || k == A_ARGMAX) {
       int64_t* best = (int64_t*)malloc(sizeof(int64_t) * (G > 0 ? G : 1));
       for (int64_t g = 0; g < G; g++) best[g] = -1;

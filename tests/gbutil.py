@@ -13,7 +13,7 @@ from oracle import oracle as orc
 
 AGG = {
     "SUM": 0, "PRODUCT": 1, "MIN": 2, "MAX": 3, "COUNT_VALID": 4, "COUNT_ALL": 5,
-    "SUM_OF_SQUARES": 8, "MEAN": 9, "VARIANCE": 11, "STD": 12, "ARGMAX": 15, "ARGMIN": 16,
+    "SUM_OF_SQUARES": 8, "MEAN": 9, "VARIANCE": 11, "STD": 12, "ARGMAX": 15, "ARGMIN": 16, "NUNIQUE": 17,
 }
 
 # Tolerances of the parity bar.  Integer, index, key, count, min and max
@@ -46,7 +46,10 @@ def gpu_groupby(keys: Sequence[orc.HostColumn], requests, null_policy: int = 0, 
     dkeys = Table([to_device(k) for k in keys])
     reqs = []
     for col, aggs in requests:
-        reqs.append(GroupByRequest(to_device(col), [Aggregation(k, d) for k, d in aggs]))
+        # NUNIQUE's second element is its null policy, every other kind's is ddof
+        reqs.append(GroupByRequest(to_device(col), [
+            Aggregation(k, null_handling=NullPolicy(d)) if k == AGG["NUNIQUE"] else Aggregation(k, d)
+            for k, d in aggs]))
     gb = GroupBy(dkeys, NullPolicy(null_policy), Sorted.YES if keys_sorted else Sorted.NO)
     out_keys, out_res = gb.aggregate(reqs)
     kcols = [c.to_numpy() for c in out_keys.columns()]
@@ -152,7 +155,7 @@ def assert_parity(keys, requests, null_policy=0, gpu=None):
             assert gs.dtype == os_.dtype, f"dtype {gs.dtype} vs {os_.dtype} (agg {kind})"
             is_float_src = col.type_id in (orc.FLOAT32, orc.FLOAT64)
             if kind in (AGG["COUNT_VALID"], AGG["COUNT_ALL"], AGG["MIN"], AGG["MAX"], AGG["ARGMIN"],
-                        AGG["ARGMAX"]) or (
+                        AGG["ARGMAX"], AGG["NUNIQUE"]) or (
                     kind in (AGG["SUM"], AGG["SUM_OF_SQUARES"], AGG["PRODUCT"]) and not is_float_src):
                 if gs.dtype.kind == "f":
                     same = (gs == os_) | (np.isnan(gs) & np.isnan(os_))

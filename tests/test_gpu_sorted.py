@@ -174,3 +174,62 @@ def test_sorted_argmin_argmax(vdt):
     for policy in (0, 1):
         _check_sorted([host_col(k, kvalid)], [(host_col(v, vvalid), [(AGG["ARGMIN"], 1), (AGG["ARGMAX"], 1),
                                                                   (AGG["COUNT_VALID"], 1)])], policy)
+
+
+# ---------------------------------------------------------------- NUNIQUE --
+@pytest.mark.parametrize("vdt", [np.int8, np.int32, np.int64, np.uint32, np.float32, np.float64, np.bool_])
+@pytest.mark.parametrize("name", ["basic", "basic_duplicates", "zero_valid_values", "null_keys_and_values",
+                                  "null_keys_and_values_with_duplicates", "include_nulls"])
+def test_nunique_reference_kats(name, vdt):
+    """cpp/tests/groupby/nunique_tests.cpp with the rows sorted by key (null
+    keys last; a distinct count does not depend on row order)."""
+    from tests.test_oracle import nunique_kat_columns
+    kcol, vcol, policy, ek, ev = nunique_kat_columns(name, vdt)
+    kvalid = np.ones(kcol.size, bool) if kcol.valid is None else kcol.valid
+    order = np.lexsort((kcol.data, ~kvalid))
+    ks = host_col(kcol.data[order], None if kcol.valid is None else kvalid[order])
+    vs = host_col(vcol.data[order], None if vcol.valid is None else vcol.valid[order])
+    g_keys, g_res = gpu_groupby([ks], [(vs, [(AGG["NUNIQUE"], policy)])], 0, keys_sorted=True)
+    assert list(g_keys[0][0]) == ek
+    got, valid = g_res[0][0]
+    assert got.dtype == np.int32 and (valid is None or valid.all())
+    assert list(got) == ev
+
+
+@pytest.mark.parametrize("vdt", [np.int16, np.int64, np.uint64, np.float32, np.float64])
+def test_sorted_nunique_parity(vdt):
+    """Random runs against the oracle under both key null policies and both
+    NUNIQUE null policies: NaN, -0.0 / 0.0 and null values, a group spanning
+    many waves, a group of only nulls."""
+    rng = np.random.default_rng(np.dtype(vdt).itemsize + 110)
+    n = 400_000
+    k = np.sort(np.concatenate([rng.integers(0, 20_000, n - 30_000), np.full(30_000, 30_000)]).astype(np.int32))
+    if np.dtype(vdt).kind == "f":
+        v = rng.integers(-50, 50, n).astype(vdt)
+        v[rng.random(n) < 0.03] = np.nan
+        v[rng.random(n) < 0.03] = -0.0
+    else:
+        v = rng.integers(0, 60, n).astype(vdt)
+    vvalid = rng.random(n) > 0.1
+    vvalid[k == 123] = False
+    kvalid = np.ones(n, bool)
+    kvalid[:700] = False  # null keys first
+    for policy in (0, 1):
+        _check_sorted([host_col(k, kvalid)], [(host_col(v, vvalid), [(AGG["NUNIQUE"], 0), (AGG["NUNIQUE"], 1),
+                                                                  (AGG["COUNT_VALID"], 1)])], policy)
+
+
+def test_sorted_nunique_large_and_errors():
+    """3M rows over 100k sorted keys against the oracle; NUNIQUE without
+    keys_are_sorted and NUNIQUE of strings raise logic errors."""
+    from cudf_amd import _lib
+    rng = np.random.default_rng(120)
+    n = 3_000_000
+    k = np.sort(rng.integers(0, 100_000, n)).astype(np.int64)
+    v = rng.integers(0, 40, n).astype(np.int32)
+    _check_sorted([host_col(k)], [(host_col(v, rng.random(n) > 0.2), [(AGG["NUNIQUE"], 0), (AGG["NUNIQUE"], 1)])], 0)
+    with pytest.raises(_lib.LogicError):
+        gpu_groupby([host_col(k[:100])], [(host_col(v[:100]), [(AGG["NUNIQUE"], 0)])])
+    s = orc.HostColumn.from_strings([b"x%d" % (i % 3) for i in range(100)])
+    with pytest.raises(_lib.LogicError):
+        gpu_groupby([host_col(k[:100])], [(s, [(AGG["NUNIQUE"], 0)])], keys_sorted=True)

@@ -403,3 +403,105 @@ def test_oracle_vs_pandas_nullable_c2_shape():
                 assert np.allclose(got, exn, rtol=1e-10, atol=0), (col, a)
             else:
                 assert np.allclose(got, exn, rtol=1e-12, atol=1e-12), (col, a)
+
+
+# --------------------------------------------------------------- NUNIQUE --
+# cpp/tests/groupby/nunique_tests.cpp: (keys, key validity, values, value
+# validity, null policy, expected keys, expected counts, expected for bool8)
+NUNIQUE_KATS = {
+    "basic": ([1, 2, 3, 1, 2, 2, 1, 3, 3, 2], None, [0, 1, 2, 3, 4, 5, 6, 7, 8, 9], None, 0,
+              [1, 2, 3], [3, 4, 3], [2, 1, 1]),                                               # :31-52
+    "basic_duplicates": ([1, 2, 3, 1, 2, 2, 1, 3, 3, 2], None, [0, 1, 2, 3, 4, 5, 3, 2, 2, 9], None, 0,
+                         [1, 2, 3], [2, 4, 1], [2, 1, 1]),                                    # :70-88
+    "zero_valid_values": ([1, 1, 1], None, [0, 0, 0], [0, 0, 0], 0, [1], [0], [0]),           # :106-120
+    "null_keys_and_values": ([1, 2, 3, 1, 2, 2, 1, 3, 3, 2, 4], [1, 1, 1, 1, 1, 1, 1, 0, 1, 1, 1],
+                             [0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 4], [0, 1, 1, 1, 1, 0, 1, 1, 1, 1, 0], 0,
+                             [1, 2, 3, 4], [2, 3, 2, 0], [1, 1, 1, 0]),                       # :122-146
+    "null_keys_and_values_with_duplicates": (
+        [1, 2, 3, 3, 1, 2, 2, 1, 3, 3, 2, 4, 4, 2], [1, 1, 1, 1, 1, 1, 1, 1, 0, 1, 1, 1, 1, 1],
+        [0, 1, 2, 2, 3, 4, 5, 6, 7, 8, 9, 4, 4, 2], [0, 1, 1, 1, 1, 1, 0, 1, 1, 1, 1, 0, 0, 0], 0,
+        [1, 2, 3, 4], [2, 3, 2, 0], [1, 1, 1, 0]),                                            # :148-173
+    "include_nulls": (
+        [1, 2, 3, 3, 1, 2, 2, 1, 3, 3, 2, 4, 4, 2], [1, 1, 1, 1, 1, 1, 1, 1, 0, 1, 1, 1, 1, 1],
+        [0, 1, 2, 2, 3, 4, 5, 6, 7, 8, 9, 4, 4, 2], [0, 1, 1, 1, 1, 1, 0, 1, 1, 1, 1, 0, 0, 0], 1,
+        [1, 2, 3, 4], [3, 4, 2, 1], [2, 2, 1, 1]),                                            # :175-200
+}
+
+
+def nunique_kat_columns(name, vdt):
+    k, kv, v, vv, policy, ek, ev, evb = NUNIQUE_KATS[name]
+    kcol = orc.HostColumn.from_numpy(np.array(k, np.int32), None if kv is None else np.array(kv, bool))
+    varr = np.array(v).astype(vdt)
+    vcol = orc.HostColumn.from_numpy(varr, None if vv is None else np.array(vv, bool))
+    return kcol, vcol, policy, ek, (evb if vdt == np.bool_ else ev)
+
+
+@pytest.mark.parametrize("vdt", [np.int8, np.int32, np.int64, np.uint32, np.float32, np.float64, np.bool_])
+@pytest.mark.parametrize("name", sorted(NUNIQUE_KATS))
+def test_nunique_reference_kats(name, vdt):
+    kcol, vcol, policy, ek, ev = nunique_kat_columns(name, vdt)
+    rep, res = orc.groupby([kcol], [(vcol, [(orc.NUNIQUE, policy)])], 0)
+    order = np.argsort(kcol.data[rep])
+    assert list(kcol.data[rep][order]) == ek
+    got, valid = res[0][0]
+    assert got.dtype == np.int32 and valid.all()
+    assert list(got[order]) == ev
+
+
+def _nunique_brute(keys, kvalid, vals, vvalid, include_nulls):
+    """len(set(...)) per group, NaN as one value, -0.0 as 0.0, null as one value."""
+    groups = {}
+    for i in range(len(keys)):
+        if not kvalid[i]:
+            continue
+        s = groups.setdefault(keys[i], set())
+        if not vvalid[i]:
+            if include_nulls:
+                s.add(("null",))
+            continue
+        x = vals[i]
+        if isinstance(x, float) and x != x:
+            s.add(("nan",))
+        else:
+            s.add(x + 0 if not isinstance(x, bytes) else x)  # -0.0 + 0 == 0.0
+    return groups
+
+
+@pytest.mark.parametrize("vdt", [np.float64, np.int16, "str"])
+@pytest.mark.parametrize("include_nulls", [0, 1])
+def test_nunique_brute_force(vdt, include_nulls):
+    rng = np.random.default_rng(17 + include_nulls)
+    n = 3000
+    k = rng.integers(0, 60, n).astype(np.int64)
+    kvalid = rng.random(n) > 0.05
+    vvalid = rng.random(n) > 0.15
+    if vdt == "str":
+        pool = [b"", b"a", b"ab", b"b", b"abc", b"ba"]
+        strs = [pool[i] for i in rng.integers(0, len(pool), n)]
+        vcol = orc.HostColumn.from_strings(strs, vvalid)
+        py_vals = strs
+    elif vdt == np.float64:
+        v = rng.integers(-4, 5, n).astype(np.float64)
+        v[rng.random(n) < 0.1] = np.nan
+        v[rng.random(n) < 0.1] = -0.0
+        vcol = orc.HostColumn.from_numpy(v, vvalid)
+        py_vals = [float(x) for x in v]
+    else:
+        v = rng.integers(-20, 20, n).astype(vdt)
+        vcol = orc.HostColumn.from_numpy(v, vvalid)
+        py_vals = [int(x) for x in v]
+    kcol = orc.HostColumn.from_numpy(k, kvalid)
+    rep, res = orc.groupby([kcol], [(vcol, [(orc.NUNIQUE, include_nulls)])], 0)
+    want = _nunique_brute([int(x) for x in k], kvalid, py_vals, vvalid, include_nulls)
+    got = {int(k[r]): int(c) for r, c in zip(rep, res[0][0][0])}
+    assert got == {g: len(s) for g, s in want.items()}
+    # a planted mistake (-0.0 distinct from 0.0) changes the brute force's answer
+    if vdt == np.float64:
+        def planted(x):
+            return ("negzero",) if x == 0 and np.signbit(x) else x
+        bad = {}
+        for i in range(n):
+            if kvalid[i] and vvalid[i] and py_vals[i] == py_vals[i]:
+                bad.setdefault(int(k[i]), set()).add(planted(py_vals[i]))
+        assert any(len(bad.get(g, ())) != len([x for x in s if x not in (("null",), ("nan",))])
+                   for g, s in want.items())
