@@ -57,7 +57,8 @@ class ColumnView(ctypes.Structure):
 
 
 class Aggregation(ctypes.Structure):
-    _fields_ = [("kind", ctypes.c_int32), ("ddof", ctypes.c_int32), ("null_handling", ctypes.c_int32)]
+    _fields_ = [("kind", ctypes.c_int32), ("ddof", ctypes.c_int32), ("null_handling", ctypes.c_int32),
+                ("n", ctypes.c_int32)]
 
 
 class AggregationRequest(ctypes.Structure):

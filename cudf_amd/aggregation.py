@@ -27,13 +27,15 @@ class Kind(enum.IntEnum):
     ARGMAX = 15
     ARGMIN = 16
     NUNIQUE = 17
+    NTH_ELEMENT = 18
 
 
 class Aggregation:
-    def __init__(self, kind: Kind, ddof: int = 1, null_handling: NullPolicy = NullPolicy.EXCLUDE):
+    def __init__(self, kind: Kind, ddof: int = 1, null_handling: NullPolicy = NullPolicy.EXCLUDE, n: int = 0):
         self._kind = Kind(kind)
         self.ddof = int(ddof)
-        self.null_handling = NullPolicy(null_handling)  # NUNIQUE only
+        self.null_handling = NullPolicy(null_handling)  # NUNIQUE, NTH_ELEMENT
+        self.n = int(n)  # NTH_ELEMENT
 
     def kind(self) -> Kind:
         return self._kind
@@ -84,6 +86,13 @@ def nunique(null_handling: NullPolicy = NullPolicy.EXCLUDE) -> Aggregation:
     """Distinct values per group; INCLUDE counts nulls as one value
     (aggregation.hpp make_nunique_aggregation).  Needs keys_are_sorted=YES."""
     return Aggregation(Kind.NUNIQUE, null_handling=null_handling)
+
+
+def nth_element(n: int, null_handling: NullPolicy = NullPolicy.INCLUDE) -> Aggregation:
+    """The n-th value of each group in row order (from the end when n < 0);
+    EXCLUDE counts valid values only (aggregation.hpp
+    make_nth_element_aggregation).  Needs keys_are_sorted=YES."""
+    return Aggregation(Kind.NTH_ELEMENT, null_handling=null_handling, n=n)
 
 
 def argmax() -> Aggregation:

@@ -369,7 +369,8 @@ mig_batch_plan* batch_plan_create(mig_batch_instance const* instances, int32_t n
   for (int32_t r = 0; r < num_requests; r++)
     for (int32_t a = 0; a < num_aggregations[r]; a++) {
       int32_t const kind = aggregations[p.agg_begin[r] + a].kind;
-      if (kind == MIG_AGG_ARGMIN || kind == MIG_AGG_ARGMAX || kind == MIG_AGG_PRODUCT || kind == MIG_AGG_NUNIQUE)
+      if (kind == MIG_AGG_ARGMIN || kind == MIG_AGG_ARGMAX || kind == MIG_AGG_PRODUCT || kind == MIG_AGG_NUNIQUE ||
+          kind == MIG_AGG_NTH_ELEMENT)
         fail(MIG_ERROR_LOGIC, std::string("aggregation ") + kind_name(kind) + " is not supported by the batched form");
       for (int32_t i = 0; i < num_instances; i++)
         check_supported(p.val_views[size_t(i) * p.num_vals + vcol_of[r]].type_id, kind);

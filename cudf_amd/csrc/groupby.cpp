@@ -15,6 +15,7 @@
 #include "groupby.hpp"
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
 #include <atomic>
@@ -312,6 +313,7 @@ char const* kind_name(int32_t k)
     case MIG_AGG_ARGMIN: return "ARGMIN";
     case MIG_AGG_ARGMAX: return "ARGMAX";
     case MIG_AGG_NUNIQUE: return "NUNIQUE";
+    case MIG_AGG_NTH_ELEMENT: return "NTH_ELEMENT";
     default: return "UNKNOWN";
   }
 }
@@ -321,7 +323,8 @@ int32_t target_type_of(int32_t src, int32_t kind)
 {
   switch (kind) {
     case MIG_AGG_MIN:
-    case MIG_AGG_MAX: return src;
+    case MIG_AGG_MAX:
+    case MIG_AGG_NTH_ELEMENT: return src;
     case MIG_AGG_COUNT_VALID:
     case MIG_AGG_COUNT_ALL:
     case MIG_AGG_ARGMIN:  // size_type row index (aggregation.hpp:1346-1357)
@@ -358,7 +361,9 @@ void check_supported(int32_t src, int32_t kind)
     case MIG_AGG_VARIANCE:
     case MIG_AGG_STD: ok = is_numeric_type(src); break;
     case MIG_AGG_NUNIQUE:
-      if (src == MIG_TYPE_STRING) fail(MIG_ERROR_LOGIC, "NUNIQUE of string values is not supported");
+    case MIG_AGG_NTH_ELEMENT:
+      if (src == MIG_TYPE_STRING)
+        fail(MIG_ERROR_LOGIC, std::string(kind_name(kind)) + " of string values is not supported");
       ok = is_numeric_type(src) || is_chrono_type(src);
       break;
     default:
@@ -381,6 +386,13 @@ struct value_plan {
   int32_t f_argmin = -1, f_argmax = -1, f_prod = -1;
   bool want_nunique[2] = {false, false};  // [null_handling == INCLUDE]
   int32_t f_nunique[2] = {-1, -1};
+  std::vector<std::array<int32_t, 3>> nth;  // NTH_ELEMENT: (n, include nulls, field)
+  int32_t nth_field(int32_t n, bool include) const
+  {
+    for (auto const& e : nth)
+      if (e[0] == n && e[1] == int32_t(include)) return e[2];
+    return -1;
+  }
 };
 
 int32_t sum_field_kind(int32_t st)
@@ -427,6 +439,7 @@ int32_t field_width(int32_t fk)
     case FK_ARGMAX:
     case FK_NUNIQUE:
     case FK_NUNIQUE_NULLS:
+    case FK_NTH:
     case FK_PROD_F32: return 4;
     default: return 8;
   }
@@ -544,8 +557,10 @@ std::unique_ptr<mig_groupby_result> groupby_aggregate(mig_column_view const* key
     for (int a = 0; a < requests[r].num_aggregations; a++) {
       check_supported(requests[r].values.type_id, requests[r].aggregations[a].kind);
       // a sort-based aggregation (groupby.cu:78-83): taken on runs of pre-sorted keys only
-      if (requests[r].aggregations[a].kind == MIG_AGG_NUNIQUE && !keys_sorted)
-        fail(MIG_ERROR_LOGIC, "NUNIQUE needs keys_are_sorted == YES (the hash group-by does not take it)");
+      int32_t const k = requests[r].aggregations[a].kind;
+      if ((k == MIG_AGG_NUNIQUE || k == MIG_AGG_NTH_ELEMENT) && !keys_sorted)
+        fail(MIG_ERROR_LOGIC, std::string(kind_name(k)) +
+                                " needs keys_are_sorted == YES (the hash group-by does not take it)");
     }
   }
 
@@ -633,6 +648,14 @@ std::unique_ptr<mig_groupby_result> groupby_aggregate(mig_column_view const* key
         case MIG_AGG_NUNIQUE:
           vp.want_nunique[requests[r].aggregations[a].null_handling == MIG_NULL_INCLUDE] = true;
           break;
+        case MIG_AGG_NTH_ELEMENT: {
+          int32_t const n    = requests[r].aggregations[a].n;
+          bool const include = requests[r].aggregations[a].null_handling == MIG_NULL_INCLUDE;
+          if (std::none_of(vp.nth.begin(), vp.nth.end(),
+                           [&](auto const& e) { return e[0] == n && e[1] == int32_t(include); }))
+            vp.nth.push_back({n, int32_t(include), -1});
+          break;
+        }
         default: break;
       }
     }
@@ -663,6 +686,11 @@ std::unique_ptr<mig_groupby_result> groupby_aggregate(mig_column_view const* key
     if (vp.want_max) vp.f_max = add_field(int(j), minmax_field_kind(vp.storage, false));
     for (int i = 0; i < 2; i++)
       if (vp.want_nunique[i]) vp.f_nunique[i] = add_field(int(j), i ? FK_NUNIQUE_NULLS : FK_NUNIQUE);
+    for (auto& e : vp.nth) {  // the field carries n and the null policy in its aux slots
+      e[2]                   = add_field(int(j), FK_NTH);
+      P.fields[e[2]].aux_sum   = e[0];
+      P.fields[e[2]].aux_count = e[1];
+    }
     if (vp.want_count) {
       if (vp.has_nulls) vp.f_count = add_field(int(j), FK_COUNT);
       else {
@@ -839,6 +867,12 @@ std::unique_ptr<mig_groupby_result> groupby_aggregate(mig_column_view const* key
             O.field   = kind == MIG_AGG_ARGMIN ? vp.f_argmin : vp.f_argmax;
             with_mask = vp.has_nulls;
             break;
+          case MIG_AGG_NTH_ELEMENT:
+            O.kind    = OK_NTH;
+            O.field   = vp.nth_field(requests[r].aggregations[a].n,
+                                     requests[r].aggregations[a].null_handling == MIG_NULL_INCLUDE);
+            with_mask = true;
+            break;
           case MIG_AGG_NUNIQUE:
             O.kind  = OK_COUNT;
             O.field = vp.f_nunique[requests[r].aggregations[a].null_handling == MIG_NULL_INCLUDE];
@@ -956,6 +990,8 @@ std::unique_ptr<mig_groupby_result> groupby_aggregate(mig_column_view const* key
       uint64_t const cap = next_pow2(std::max<uint64_t>(2 * uint64_t(N), 64));
       launch_sorted_nunique(dP, f, N, gid, S.get<uint32_t>(int64_t(cap)), cap, stream);
     }
+    for (int f = 0; f < nf; f++)
+      if (P.fields[f].kind == FK_NTH) launch_sorted_nth(dP, f, N, G, gid, stream);
     T.mark(5);
     uint32_t* map = S.get<uint32_t>(std::max<int64_t>(G, 1));
     launch_iota_u32(map, G, stream);

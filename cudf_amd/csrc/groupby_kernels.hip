@@ -781,6 +781,24 @@ __global__ void __launch_bounds__(BLOCK)
           reinterpret_cast<int32_t*>(O.data)[g] = valid ? int32_t(r) : -1;
           break;
         }
+        case OK_NTH: {  // gather of the value at the group's n-th row
+          field_desc const& F = P.fields[O.field];
+          dcol const& c       = P.vals[F.vcol];
+          uint32_t const r    = *reinterpret_cast<uint32_t const*>(field_at(F, slot));
+          valid               = r != 0xFFFFFFFFu && dcol_valid(c, r);
+          uint64_t b          = 0;
+          if (valid) {
+            auto const d = gp(c.data);
+            switch (O.out_width) {
+              case 1: b = d[r]; break;
+              case 2: b = reinterpret_cast<gmem<uint16_t const>*>(d)[r]; break;
+              case 4: b = reinterpret_cast<gmem<uint32_t const>*>(d)[r]; break;
+              default: b = reinterpret_cast<gmem<uint64_t const>*>(d)[r]; break;
+            }
+          }
+          store_width(O.data, g, O.out_width, b);
+          break;
+        }
         case OK_VAR:
         case OK_STD: {
           int64_t const dn = int64_t(cnt) - O.ddof;
@@ -1612,7 +1630,8 @@ static void launch_sorted_passes(gb_params const* dP, gb_params const& hP, int64
   std::vector<int32_t> items;
   if (!m2 && hP.need_count_all) items.push_back(-1);
   for (int f = 0; f < hP.num_fields; f++) {
-    if (hP.fields[f].kind == FK_NUNIQUE || hP.fields[f].kind == FK_NUNIQUE_NULLS) continue;  // own kernel
+    int32_t const k = hP.fields[f].kind;
+    if (k == FK_NUNIQUE || k == FK_NUNIQUE_NULLS || k == FK_NTH) continue;  // kernels of their own
     if ((hP.fields[f].kind == FK_M2) == m2) items.push_back(f);
   }
   int64_t per, W;
@@ -1722,6 +1741,62 @@ __global__ void __launch_bounds__(BLOCK)
       slot = (slot + 1) & cap_mask;
     }
   }
+}
+
+// NTH_ELEMENT on runs (sort/group_nth_element.cu:50-140): one thread per
+// group walks the group's rows in row order (from its first row forward for
+// n >= 0, from the next group's first row backward for n < 0), counting every
+// row under INCLUDE and rows with a valid value under EXCLUDE, and keeps the
+// row where the count reaches the n-th.  A group's rows are contiguous (a
+// dropped null-key row ends a run), so the forward walk ends at the first row
+// of another group or a dropped row, and the backward walk skips the dropped
+// rows between the group's last row and the next group.  Cost per group:
+// O(|n|) rows plus rows without a counted value.
+__global__ void __launch_bounds__(BLOCK)
+  k_sorted_nth(gb_params const* __restrict__ Pp, int32_t f, int64_t n_rows, int64_t G,
+               uint32_t const* __restrict__ gid)
+{
+  int64_t const g = int64_t(blockIdx.x) * BLOCK + threadIdx.x;
+  if (g >= G) return;
+  gb_params const& P  = *Pp;
+  field_desc const& F = P.fields[f];
+  dcol const& c       = P.vals[F.vcol];
+  int64_t const nth   = F.aux_sum;
+  bool const include  = F.aux_count != 0;
+  int64_t const first = int64_t(P.slot_rows[g]);
+  uint32_t found      = 0xFFFFFFFFu;
+  int64_t seen        = 0;
+  if (nth >= 0) {
+    for (int64_t r = first; r < n_rows; r++) {
+      uint32_t const rg = gid[r];
+      if (rg == NO_GROUP) continue;
+      if (rg != uint32_t(g)) break;
+      if (include || dcol_valid(c, r)) {
+        if (seen == nth) {
+          found = uint32_t(r);
+          break;
+        }
+        seen++;
+      }
+    }
+  } else {
+    int64_t const end = g + 1 < G ? int64_t(P.slot_rows[g + 1]) : n_rows;
+    for (int64_t r = end - 1; r >= first; r--) {
+      if (gid[r] != uint32_t(g)) continue;  // a dropped row between the group's rows
+      if (include || dcol_valid(c, r)) {
+        if (++seen == -nth) {
+          found = uint32_t(r);
+          break;
+        }
+      }
+    }
+  }
+  *reinterpret_cast<uint32_t*>(field_at(F, g)) = found;
+}
+
+void launch_sorted_nth(gb_params const* dP, int32_t f, int64_t n, int64_t G, uint32_t const* gid, hipStream_t s)
+{
+  if (G > 0) hipLaunchKernelGGL(k_sorted_nth, dim3(unsigned((G + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0, s, dP, f, n, G, gid);
 }
 
 void launch_sorted_nunique(gb_params const* dP, int32_t f, int64_t n, uint32_t const* gid, uint32_t* table,

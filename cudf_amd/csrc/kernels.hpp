@@ -55,7 +55,8 @@ enum field_kind : int32_t {
   FK_PROD_F32,
   FK_PROD_F64,
   FK_NUNIQUE,        // distinct valid values (u32; filled by the pre-sorted path's set insert)
-  FK_NUNIQUE_NULLS   // distinct values, nulls counted as one value
+  FK_NUNIQUE_NULLS,  // distinct values, nulls counted as one value
+  FK_NTH             // row of the group's n-th value (u32, 0xFFFFFFFF = none); aux_sum = n, aux_count = include
 };
 
 struct field_desc {
@@ -80,7 +81,8 @@ enum out_kind : int32_t {
   OK_MEAN,       // double(sum) / count
   OK_VAR,        // m2 / (count - ddof)
   OK_STD,        // sqrt(var)
-  OK_ARG         // int32 row index from an ARGMIN / ARGMAX field, null when none
+  OK_ARG,        // int32 row index from an ARGMIN / ARGMAX field, null when none
+  OK_NTH         // the value at an FK_NTH field's row, null when none or null
 };
 
 struct out_desc {
@@ -265,6 +267,8 @@ void launch_sorted_second_pass(gb_params const* dP, gb_params const& hP, int64_t
 // of at least 2n; cleared here) counts each group's distinct values.
 void launch_sorted_nunique(gb_params const* dP, int32_t f, int64_t n, uint32_t const* gid, uint32_t* table,
                            uint64_t cap, hipStream_t s);
+// NTH_ELEMENT of field `f` on runs: the row of each group's n-th counted value.
+void launch_sorted_nth(gb_params const* dP, int32_t f, int64_t n, int64_t G, uint32_t const* gid, hipStream_t s);
 void launch_iota_u32(uint32_t* out, int64_t n, hipStream_t s);
 // string MIN / MAX: lengths and validity of the values at the ARGMIN / ARGMAX rows (-1 = none)
 void launch_str_arg_sizes(dcol const& src, int32_t const* rows, int64_t G, int32_t* sizes, uint32_t* mask,

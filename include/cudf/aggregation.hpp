@@ -29,10 +29,11 @@ class aggregation {
     QUANTILE,
     ARGMAX,
     ARGMIN,
-    NUNIQUE
+    NUNIQUE,
+    NTH_ELEMENT
   };
-  explicit aggregation(Kind k, size_type ddof = 1, null_policy nulls = null_policy::EXCLUDE)
-    : kind{k}, _ddof{ddof}, _null_handling{nulls}
+  explicit aggregation(Kind k, size_type ddof = 1, null_policy nulls = null_policy::EXCLUDE, size_type n = 0)
+    : kind{k}, _ddof{ddof}, _null_handling{nulls}, _n{n}
   {
   }
   virtual ~aggregation() = default;
@@ -42,14 +43,16 @@ class aggregation {
   }
   Kind kind;
   size_type _ddof;  // VARIANCE / STD
-  null_policy _null_handling;  // NUNIQUE
+  null_policy _null_handling;  // NUNIQUE, NTH_ELEMENT
+  size_type _n;                // NTH_ELEMENT
 };
 
 /// Aggregations valid in a group-by (aggregation.hpp groupby_aggregation).
 class groupby_aggregation : public virtual aggregation {
  public:
-  explicit groupby_aggregation(Kind k, size_type ddof = 1, null_policy nulls = null_policy::EXCLUDE)
-    : aggregation(k, ddof, nulls)
+  explicit groupby_aggregation(Kind k, size_type ddof = 1, null_policy nulls = null_policy::EXCLUDE,
+                               size_type n = 0)
+    : aggregation(k, ddof, nulls, n)
   {
   }
   [[nodiscard]] std::unique_ptr<aggregation> clone() const override
@@ -111,6 +114,13 @@ template <typename Base = aggregation>
 std::unique_ptr<Base> make_nunique_aggregation(null_policy null_handling = null_policy::EXCLUDE)
 {
   return std::make_unique<Base>(aggregation::NUNIQUE, 1, null_handling);
+}
+/// The n-th value of each group (aggregation.hpp make_nth_element_aggregation);
+/// taken with keys_are_sorted == sorted::YES.
+template <typename Base = aggregation>
+std::unique_ptr<Base> make_nth_element_aggregation(size_type n, null_policy null_handling = null_policy::INCLUDE)
+{
+  return std::make_unique<Base>(aggregation::NTH_ELEMENT, 1, null_handling, n);
 }
 template <typename Base = aggregation>
 std::unique_ptr<Base> make_variance_aggregation(size_type ddof = 1)

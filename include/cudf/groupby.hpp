@@ -80,7 +80,7 @@ class groupby {
     for (size_t r = 0; r < requests.size(); r++) {
       for (auto const& a : requests[r].aggregations)
         aggs[r].push_back(mig_aggregation{static_cast<int32_t>(a->kind), a->_ddof,
-                                          static_cast<int32_t>(a->_null_handling)});
+                                          static_cast<int32_t>(a->_null_handling), a->_n});
       reqs[r].values           = requests[r].values.c_view();
       reqs[r].aggregations     = aggs[r].data();
       reqs[r].num_aggregations = int32_t(aggs[r].size());

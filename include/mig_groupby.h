@@ -81,10 +81,10 @@ enum {
  *      listed in hash/groupby.cu:73-84, all implemented here: SUM, PRODUCT,
  *      MIN, MAX, COUNT_VALID, COUNT_ALL, SUM_OF_SQUARES, MEAN, VARIANCE, STD,
  *      ARGMIN, ARGMAX (string values: COUNT_*, MIN, MAX, ARGMIN, ARGMAX).
- *      NUNIQUE, a sort-path aggregation in the reference (groupby.cu:78-83,
- *      sort/group_nunique.cu), is taken by mig_groupby_aggregate_sorted for
- *      fixed-width values; mig_groupby_aggregate rejects it with
- *      MIG_ERROR_LOGIC. --- */
+ *      NUNIQUE and NTH_ELEMENT, sort-path aggregations in the reference
+ *      (groupby.cu:78-83, sort/group_nunique.cu, sort/group_nth_element.cu),
+ *      are taken by mig_groupby_aggregate_sorted for fixed-width values;
+ *      mig_groupby_aggregate rejects them with MIG_ERROR_LOGIC. --- */
 enum {
   MIG_AGG_SUM            = 0,
   MIG_AGG_PRODUCT        = 1,
@@ -101,7 +101,8 @@ enum {
   MIG_AGG_STD            = 12,
   MIG_AGG_ARGMAX         = 15,
   MIG_AGG_ARGMIN         = 16,
-  MIG_AGG_NUNIQUE        = 17
+  MIG_AGG_NUNIQUE        = 17,
+  MIG_AGG_NTH_ELEMENT    = 18
 };
 
 /* cudf::null_policy (cudf/types.hpp:126-129). */
@@ -127,11 +128,15 @@ typedef struct mig_column_view {
 /* One aggregation of a request (cudf::groupby_aggregation).  `ddof` is used
  * by VARIANCE and STD (cudf/aggregation.hpp make_variance_aggregation),
  * `null_handling` by NUNIQUE (make_nunique_aggregation: MIG_NULL_EXCLUDE
- * does not count nulls, MIG_NULL_INCLUDE counts them as one value). */
+ * does not count nulls, MIG_NULL_INCLUDE counts them as one value) and by
+ * NTH_ELEMENT (make_nth_element_aggregation, whose default is INCLUDE:
+ * EXCLUDE counts valid values only), `n` by NTH_ELEMENT (the group's n-th
+ * row, counted from the end when negative). */
 typedef struct mig_aggregation {
   int32_t kind; /* MIG_AGG_* */
   int32_t ddof;
   int32_t null_handling;
+  int32_t n;
 } mig_aggregation;
 
 /* cudf::groupby::aggregation_request (cudf/groupby.hpp:61-64). */

@@ -16,7 +16,9 @@
  *                        force over tiny inputs, and pandas group-by on the
  *                        same seeded data (tests/test_oracle.py); NUNIQUE
  *                        by cpp/tests/groupby/nunique_tests.cpp and a
- *                        set-based brute force
+ *                        set-based brute force; NTH_ELEMENT by
+ *                        cpp/tests/groupby/nth_element_tests.cpp and a
+ *                        per-group list brute force
  */
 #include <math.h>
 #include <stdint.h>
@@ -33,7 +35,7 @@ enum {
 enum {
   A_SUM = 0, A_PRODUCT = 1, A_MIN = 2, A_MAX = 3, A_COUNT_VALID = 4, A_COUNT_ALL = 5,
   A_SUM_OF_SQUARES = 8, A_MEAN = 9, A_VARIANCE = 11, A_STD = 12, A_ARGMAX = 15, A_ARGMIN = 16,
-  A_NUNIQUE = 17
+  A_NUNIQUE = 17, A_NTH_ELEMENT = 18
 };
 
 typedef struct orc_col {
@@ -222,6 +224,7 @@ int32_t orc_target_type(int32_t src, int32_t kind)
 {
   switch (kind) {
     case A_MIN: case A_MAX: return src;
+    case A_NTH_ELEMENT: return src == T_STRING ? -1 : src;
     case A_COUNT_VALID: case A_COUNT_ALL: return T_INT32;
     case A_ARGMIN: case A_ARGMAX: return T_INT32; /* size_type row index, aggregation.hpp:1346-1357 */
     case A_NUNIQUE: return T_INT32;               /* size_type count, group_nunique.cu:90-91 */
@@ -379,6 +382,35 @@ int64_t orc_groupby(const orc_col* keys, int32_t nkeys, int32_t null_policy,
     int tgt = orc_target_type(src, k);
     if (tgt < 0 || (src == T_STRING && k != A_COUNT_VALID && k != A_COUNT_ALL && k != A_ARGMIN &&
                     k != A_ARGMAX && k != A_NUNIQUE)) { ok = 0; break; }
+    if (k == A_NTH_ELEMENT) {
+      /* group_nth_element.cu:50-140: the n-th row of the group (from the end
+       * when n < 0) in row order, counting every row under INCLUDE and valid
+       * rows under EXCLUDE; null when the group has too few such rows, and
+       * the value's own null under INCLUDE.  ddofs[a] = 2 * n + include. */
+      int include = ddofs[a] & 1;
+      int64_t nth = (int64_t)(ddofs[a] - include) / 2;
+      int64_t* total = (int64_t*)calloc((size_t)(G > 0 ? G : 1), sizeof(int64_t));
+      int64_t* seen = (int64_t*)calloc((size_t)(G > 0 ? G : 1), sizeof(int64_t));
+      int64_t* pick = (int64_t*)malloc(sizeof(int64_t) * (G > 0 ? G : 1));
+      for (int64_t g = 0; g < G; g++) pick[g] = -1;
+      for (int64_t r = 0; r < n; r++)
+        if (group_of_row[r] >= 0 && (include || row_valid(v, r))) total[group_of_row[r]]++;
+      for (int64_t r = 0; r < n; r++) {
+        int64_t g = group_of_row[r];
+        if (g < 0 || !(include || row_valid(v, r))) continue;
+        int64_t want = nth >= 0 ? nth : total[g] + nth;
+        if (seen[g]++ == want) pick[g] = r;
+      }
+      int w = type_width(src);
+      for (int64_t g = 0; g < G; g++) {
+        int ok_g = pick[g] >= 0 && row_valid(v, pick[g]);
+        if (ok_g) memcpy((uint8_t*)outs[a] + g * w, (const uint8_t*)v->data + pick[g] * w, (size_t)w);
+        else memset((uint8_t*)outs[a] + g * w, 0, (size_t)w);
+        out_valid[a][g] = (uint8_t)ok_g;
+      }
+      free(total); free(seen); free(pick);
+      continue;
+    }
     if (k == A_NUNIQUE) {  /* ddofs[a] holds the null policy: 0 EXCLUDE, 1 INCLUDE */
       nunique_counts(v, group_of_row, n, ddofs[a] != 0, G, (int32_t*)outs[a]);
       for (int64_t g = 0; g < G; g++) out_valid[a][g] = 1;

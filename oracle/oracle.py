@@ -31,6 +31,7 @@ SUM, PRODUCT, MIN, MAX, COUNT_VALID, COUNT_ALL = 0, 1, 2, 3, 4, 5
 SUM_OF_SQUARES, MEAN, VARIANCE, STD = 8, 9, 11, 12
 ARGMAX, ARGMIN = 15, 16
 NUNIQUE = 17  # the request's second element is the null policy (0 EXCLUDE, 1 INCLUDE)
+NTH_ELEMENT = 18  # the request's second element is 2 * n + (1 if nulls are included)
 
 NP_OF_TYPE = {
     INT8: np.int8, INT16: np.int16, INT32: np.int32, INT64: np.int64,

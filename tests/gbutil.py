@@ -13,7 +13,7 @@ from oracle import oracle as orc
 
 AGG = {
     "SUM": 0, "PRODUCT": 1, "MIN": 2, "MAX": 3, "COUNT_VALID": 4, "COUNT_ALL": 5,
-    "SUM_OF_SQUARES": 8, "MEAN": 9, "VARIANCE": 11, "STD": 12, "ARGMAX": 15, "ARGMIN": 16, "NUNIQUE": 17,
+    "SUM_OF_SQUARES": 8, "MEAN": 9, "VARIANCE": 11, "STD": 12, "ARGMAX": 15, "ARGMIN": 16, "NUNIQUE": 17, "NTH_ELEMENT": 18,
 }
 
 # Tolerances of the parity bar.  Integer, index, key, count, min and max
@@ -46,15 +46,24 @@ def gpu_groupby(keys: Sequence[orc.HostColumn], requests, null_policy: int = 0, 
     dkeys = Table([to_device(k) for k in keys])
     reqs = []
     for col, aggs in requests:
-        # NUNIQUE's second element is its null policy, every other kind's is ddof
-        reqs.append(GroupByRequest(to_device(col), [
-            Aggregation(k, null_handling=NullPolicy(d)) if k == AGG["NUNIQUE"] else Aggregation(k, d)
-            for k, d in aggs]))
+        reqs.append(GroupByRequest(to_device(col), [_aggregation(k, d) for k, d in aggs]))
     gb = GroupBy(dkeys, NullPolicy(null_policy), Sorted.YES if keys_sorted else Sorted.NO)
     out_keys, out_res = gb.aggregate(reqs)
     kcols = [c.to_numpy() for c in out_keys.columns()]
     res = [[c.to_numpy() for c in t.columns()] for t in out_res]
     return kcols, res
+
+
+def _aggregation(kind, param):
+    """The second element of a request pair: ddof; NUNIQUE's null policy;
+    NTH_ELEMENT's 2 * n + (1 when nulls are included)."""
+    from cudf_amd import NullPolicy
+    from cudf_amd.aggregation import Aggregation
+    if kind == AGG["NUNIQUE"]:
+        return Aggregation(kind, null_handling=NullPolicy(param))
+    if kind == AGG["NTH_ELEMENT"]:
+        return Aggregation(kind, null_handling=NullPolicy(param & 1), n=param >> 1)
+    return Aggregation(kind, param)
 
 
 def oracle_groupby(keys: Sequence[orc.HostColumn], requests, null_policy: int = 0):
@@ -155,7 +164,7 @@ def assert_parity(keys, requests, null_policy=0, gpu=None):
             assert gs.dtype == os_.dtype, f"dtype {gs.dtype} vs {os_.dtype} (agg {kind})"
             is_float_src = col.type_id in (orc.FLOAT32, orc.FLOAT64)
             if kind in (AGG["COUNT_VALID"], AGG["COUNT_ALL"], AGG["MIN"], AGG["MAX"], AGG["ARGMIN"],
-                        AGG["ARGMAX"], AGG["NUNIQUE"]) or (
+                        AGG["ARGMAX"], AGG["NUNIQUE"], AGG["NTH_ELEMENT"]) or (
                     kind in (AGG["SUM"], AGG["SUM_OF_SQUARES"], AGG["PRODUCT"]) and not is_float_src):
                 if gs.dtype.kind == "f":
                     same = (gs == os_) | (np.isnan(gs) & np.isnan(os_))

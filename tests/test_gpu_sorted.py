@@ -233,3 +233,84 @@ def test_sorted_nunique_large_and_errors():
     s = orc.HostColumn.from_strings([b"x%d" % (i % 3) for i in range(100)])
     with pytest.raises(_lib.LogicError):
         gpu_groupby([host_col(k[:100])], [(s, [(AGG["NUNIQUE"], 0)])], keys_sorted=True)
+
+
+# ------------------------------------------------------------ NTH_ELEMENT --
+@pytest.mark.parametrize("vdt", [np.int8, np.int32, np.int64, np.uint16, np.float32, np.float64])
+def test_nth_element_reference_kats(vdt):
+    """cpp/tests/groupby/nth_element_tests.cpp with the rows stably sorted by
+    key (null keys last): the order inside each group is kept, so the n-th
+    row of every group is the reference's."""
+    from tests.test_oracle import NTH_KATS, nth_kat_columns
+    for name in sorted(NTH_KATS):
+        kcol, vcol, param, ek, ev = nth_kat_columns(name, vdt)
+        kvalid = np.ones(kcol.size, bool) if kcol.valid is None else kcol.valid
+        order = np.lexsort((kcol.data, ~kvalid))
+        ks = host_col(kcol.data[order], None if kcol.valid is None else kvalid[order])
+        vs = host_col(vcol.data[order], None if vcol.valid is None else vcol.valid[order])
+        g_keys, g_res = gpu_groupby([ks], [(vs, [(AGG["NTH_ELEMENT"], param)])], 0, keys_sorted=True)
+        assert list(g_keys[0][0]) == ek, name
+        got, valid = g_res[0][0]
+        valid = np.ones(len(got), bool) if valid is None else valid
+        assert got.dtype == np.dtype(vdt)
+        assert [(None if not ok else int(x)) for x, ok in zip(got, valid)] == ev, name
+
+
+@pytest.mark.parametrize("vdt", [np.int16, np.int64, np.float32, np.float64])
+def test_sorted_nth_element_parity(vdt):
+    """Random runs against the oracle: n in {0, 1, 5, -1, -3}, both null
+    policies, null values and keys, NaN values, a group spanning many waves;
+    also unsorted keys with null keys between a group's rows."""
+    rng = np.random.default_rng(np.dtype(vdt).itemsize + 130)
+    n = 300_000
+    k = np.sort(np.concatenate([rng.integers(0, 30_000, n - 20_000), np.full(20_000, 40_000)]).astype(np.int64))
+    v = (rng.standard_normal(n) * 100).astype(vdt) if np.dtype(vdt).kind == "f" else \
+        rng.integers(-500, 500, n).astype(vdt)
+    if np.dtype(vdt).kind == "f":
+        v[rng.random(n) < 0.02] = np.nan
+    vvalid = rng.random(n) > 0.25
+    kvalid = np.ones(n, bool)
+    kvalid[-400:] = False
+    aggs = [(AGG["NTH_ELEMENT"], 2 * m + inc) for m in (0, 1, 5, -1, -3) for inc in (0, 1)]
+    for policy in (0, 1):
+        _check_sorted([host_col(k, kvalid)], [(host_col(v, vvalid), aggs)], policy)
+    # runs of unsorted keys with null keys among them (dropped under EXCLUDE)
+    ku = np.repeat(rng.integers(0, 50, 4000), rng.integers(1, 9, 4000))[:20_000].astype(np.int32)
+    m_ = len(ku)
+    kuv = rng.random(m_) > 0.1
+    vu = rng.integers(0, 100, m_).astype(np.int32)
+    vuv = rng.random(m_) > 0.3
+    g_keys, g_res = gpu_groupby([host_col(ku, kuv)], [(host_col(vu, vuv), aggs)], 0, keys_sorted=True)
+    # expected: runs of kept rows on the host (a dropped null-key row ends a run)
+    runs, cur = [], None
+    for i in range(m_):
+        if not kuv[i]:
+            cur = None
+        elif cur is None or ku[i] != ku[cur[-1]]:
+            cur = [i]
+            runs.append(cur)
+        else:
+            cur.append(i)
+    assert len(g_keys[0][0]) == len(runs)
+    for a, (_, param) in enumerate(aggs):
+        m, inc = param >> 1, param & 1
+        got, valid = g_res[0][a]
+        valid = np.ones(len(got), bool) if valid is None else valid
+        for gi, rows in enumerate(runs):
+            cnt = [r for r in rows if inc or vuv[r]]
+            want = cnt[m] if -len(cnt) <= m < len(cnt) else None
+            ok = want is not None and vuv[want]
+            assert valid[gi] == ok, (param, gi)
+            if ok:
+                assert got[gi] == vu[want]
+
+
+def test_nth_element_errors():
+    from cudf_amd import _lib
+    k = np.arange(100, dtype=np.int64)
+    v = np.arange(100, dtype=np.int32)
+    with pytest.raises(_lib.LogicError):
+        gpu_groupby([host_col(k)], [(host_col(v), [(AGG["NTH_ELEMENT"], 1)])])
+    s = orc.HostColumn.from_strings([b"x%d" % i for i in range(100)])
+    with pytest.raises(_lib.LogicError):
+        gpu_groupby([host_col(k)], [(s, [(AGG["NTH_ELEMENT"], 1)])], keys_sorted=True)

@@ -505,3 +505,85 @@ def test_nunique_brute_force(vdt, include_nulls):
                 bad.setdefault(int(k[i]), set()).add(planted(py_vals[i]))
         assert any(len(bad.get(g, ())) != len([x for x in s if x not in (("null",), ("nan",))])
                    for g, s in want.items())
+
+
+# ----------------------------------------------------------- NTH_ELEMENT --
+_K10 = [1, 2, 3, 1, 2, 2, 1, 3, 3, 2]
+_K11 = ([1, 2, 3, 1, 2, 2, 1, 3, 3, 2, 4], [1, 1, 1, 1, 1, 1, 1, 0, 1, 1, 1])
+_V11 = ([0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 4], [0, 1, 1, 1, 1, 0, 1, 1, 1, 1, 0])
+_K14 = ([1, 2, 3, 3, 1, 2, 2, 1, 3, 3, 2, 4, 4, 2], [1, 1, 1, 1, 1, 1, 1, 1, 0, 1, 1, 1, 1, 1])
+_V14 = ([0, 1, 2, 2, 3, 4, 5, 6, 7, 8, 9, 4, 4, 2], [0, 1, 1, 1, 1, 1, 0, 1, 1, 1, 1, 1, 0, 0])
+_ALL10 = list(range(10))
+# cpp/tests/groupby/nth_element_tests.cpp: (keys, key validity, values, value
+# validity, n, include nulls, expected keys, expected values (None = null))
+NTH_KATS = {
+    "basic_0": (_K10, None, _ALL10, None, 0, 1, [1, 2, 3], [0, 1, 2]),                      # :35-59
+    "basic_1": (_K10, None, _ALL10, None, 1, 1, [1, 2, 3], [3, 4, 7]),
+    "basic_2": (_K10, None, _ALL10, None, 2, 1, [1, 2, 3], [6, 5, 8]),
+    "out_of_bounds": (_K10, None, [0, 1, 2, 3, 4, 5, 3, 2, 2, 9], None, 3, 1, [1, 2, 3],
+                      [None, 9, None]),                                                         # :78-92
+    "negative_1": (_K10, None, _ALL10, None, -1, 1, [1, 2, 3], [6, 9, 8]),                   # :94-119
+    "negative_2": (_K10, None, _ALL10, None, -2, 1, [1, 2, 3], [3, 5, 7]),
+    "negative_3": (_K10, None, _ALL10, None, -3, 1, [1, 2, 3], [0, 4, 2]),
+    "negative_out_of_bounds": (_K10, None, [0, 1, 2, 3, 4, 5, 3, 2, 2, 9], None, -4, 1, [1, 2, 3],
+                               [None, 1, None]),                                                # :121-135
+    "zero_valid_values": ([1, 1, 1], None, [3, 4, 5], [0, 0, 0], 0, 1, [1], [None]),          # :153-167
+    "null_keys_and_values": (_K11[0], _K11[1], _V11[0], _V11[1], 0, 1, [1, 2, 3, 4],
+                             [None, 1, 2, None]),                                               # :169-187
+    "null_keys_and_values_out_of_bounds": (_K11[0], _K11[1], _V11[0], _V11[1], 2, 1, [1, 2, 3, 4],
+                                           [6, None, None, None]),                              # :189-207
+    "include_nulls_0": (_K14[0], _K14[1], _V14[0], _V14[1], 0, 1, [1, 2, 3, 4], [None, 1, 2, 4]),  # :209-249
+    "include_nulls_1": (_K14[0], _K14[1], _V14[0], _V14[1], 1, 1, [1, 2, 3, 4], [3, 4, 2, None]),
+    "include_nulls_2": (_K14[0], _K14[1], _V14[0], _V14[1], 2, 1, [1, 2, 3, 4], [6, None, 8, None]),
+    "exclude_nulls_0": (_K14[0], _K14[1], _V14[0], _V14[1], 0, 0, [1, 2, 3, 4], [3, 1, 2, 4]),
+    "exclude_nulls_1": (_K14[0], _K14[1], _V14[0], _V14[1], 1, 0, [1, 2, 3, 4], [6, 4, 2, None]),
+    "exclude_nulls_2": (_K14[0], _K14[1], _V14[0], _V14[1], 2, 0, [1, 2, 3, 4], [None, 9, 8, None]),
+}
+
+
+def nth_kat_columns(name, vdt):
+    k, kv, v, vv, n, include, ek, ev = NTH_KATS[name]
+    kcol = orc.HostColumn.from_numpy(np.array(k, np.int32), None if kv is None else np.array(kv, bool))
+    vcol = orc.HostColumn.from_numpy(np.array(v).astype(vdt), None if vv is None else np.array(vv, bool))
+    return kcol, vcol, 2 * n + include, ek, ev
+
+
+@pytest.mark.parametrize("vdt", [np.int8, np.int32, np.int64, np.uint16, np.float32, np.float64])
+@pytest.mark.parametrize("name", sorted(NTH_KATS))
+def test_nth_element_reference_kats(name, vdt):
+    kcol, vcol, param, ek, ev = nth_kat_columns(name, vdt)
+    rep, res = orc.groupby([kcol], [(vcol, [(orc.NTH_ELEMENT, param)])], 0)
+    order = np.argsort(kcol.data[rep])
+    assert list(kcol.data[rep][order]) == ek
+    got, valid = res[0][0]
+    assert got.dtype == np.dtype(vdt)
+    assert [(None if not ok else int(x)) for x, ok in zip(got[order], valid[order])] == ev
+
+
+@pytest.mark.parametrize("n", [0, 1, 3, -1, -2, -5])
+@pytest.mark.parametrize("include", [0, 1])
+def test_nth_element_brute_force(n, include):
+    """Per-group lists of counted rows in row order, indexed with Python's own
+    negative indexing."""
+    rng = np.random.default_rng(40 + n * 2 + include)
+    m = 2000
+    k = rng.integers(0, 80, m).astype(np.int32)
+    kvalid = rng.random(m) > 0.05
+    v = rng.integers(-1000, 1000, m).astype(np.int64)
+    vvalid = rng.random(m) > 0.3
+    rep, res = orc.groupby([orc.HostColumn.from_numpy(k, kvalid)],
+                           [(orc.HostColumn.from_numpy(v, vvalid), [(orc.NTH_ELEMENT, 2 * n + include)])], 0)
+    lists = {}
+    for i in range(m):
+        if kvalid[i]:
+            lists.setdefault(int(k[i]), [])
+            if include or vvalid[i]:
+                lists[int(k[i])].append(i)
+    got, valid = res[0][0]
+    for gi, r in enumerate(rep):
+        rows = lists[int(k[r])]
+        want = rows[n] if -len(rows) <= n < len(rows) else None
+        if want is None or not vvalid[want]:
+            assert not valid[gi]
+        else:
+            assert valid[gi] and got[gi] == v[want]
