@@ -442,6 +442,24 @@ def bench_sorted(n=100_000_000, card=10_000_000, reps=5):
             del r
         ms = sorted(ts)[len(ts) // 2] * 1e3
         out[name] = {"ms": ms, "rows_per_s": n / (ms * 1e-3), "groups": groups}
+    # sort-only aggregations on the same runs: NUNIQUE of the int32 values and
+    # their first / last valid value (NTH_ELEMENT 0 and -1, nulls excluded)
+    sort_only = [ca.GroupByRequest(icol, [Aggregation(Kind.NUNIQUE),
+                                          Aggregation(Kind.NTH_ELEMENT, null_handling=ca.NullPolicy.EXCLUDE, n=0),
+                                          Aggregation(Kind.NTH_ELEMENT, null_handling=ca.NullPolicy.EXCLUDE, n=-1)])]
+    gb = ca.GroupBy(ca.Table([kcol]), ca.NullPolicy.EXCLUDE, ca.Sorted.YES)
+    r = gb.aggregate(sort_only)
+    del r
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        r = gb.aggregate(sort_only)
+        torch.cuda.synchronize()
+        ts.append(time.perf_counter() - t0)
+        del r
+    ms = sorted(ts)[len(ts) // 2] * 1e3
+    out["sorted_nunique_first_last"] = {"ms": ms, "rows_per_s": n / (ms * 1e-3)}
     return out
 
 

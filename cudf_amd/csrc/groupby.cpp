@@ -987,8 +987,7 @@ std::unique_ptr<mig_groupby_result> groupby_aggregate(mig_column_view const* key
     if (P.need_second_pass) launch_sorted_second_pass(dP, P, N, gid, dp.num_cus, stream);
     for (int f = 0; f < nf; f++) {
       if (P.fields[f].kind != FK_NUNIQUE && P.fields[f].kind != FK_NUNIQUE_NULLS) continue;
-      uint64_t const cap = next_pow2(std::max<uint64_t>(2 * uint64_t(N), 64));
-      launch_sorted_nunique(dP, f, N, gid, S.get<uint32_t>(int64_t(cap)), cap, stream);
+      launch_sorted_nunique(dP, f, N, G, gid, S.get<uint32_t>(2 * N), stream);
     }
     for (int f = 0; f < nf; f++)
       if (P.fields[f].kind == FK_NTH) launch_sorted_nth(dP, f, N, G, gid, stream);

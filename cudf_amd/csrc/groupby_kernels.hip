@@ -1494,7 +1494,9 @@ __device__ __forceinline__ void arg_flush(gb_params const& P, int32_t f, uint32_
 // still open at a chunk's last lane is carried into the next chunk.  Groups
 // that start and end inside the range are stored once; the range's first and
 // last groups may have rows in the neighbouring ranges and are combined
-// atomically.
+// atomically.  HAS_ARG: the pass holds ARGMIN / ARGMAX accumulators (their
+// branch is compiled only into that variant, which keeps the common one lean).
+template <bool HAS_ARG>
 __global__ void __launch_bounds__(BLOCK)
   k_sorted_reduce(gb_params const* __restrict__ Pp, uint32_t const* __restrict__ gid, int64_t per, int64_t W,
                   sorted_pass pass)
@@ -1503,7 +1505,7 @@ __global__ void __launch_bounds__(BLOCK)
   // field loop stays rolled, which keeps the kernel small enough for the
   // instruction cache
   __shared__ uint64_t carry[BLOCK / 64][SORTED_FPP];
-  __shared__ uint32_t carry_row[BLOCK / 64][SORTED_FPP];  // ARGMIN / ARGMAX partials
+  __shared__ uint32_t carry_row[HAS_ARG ? BLOCK / 64 : 1][SORTED_FPP];  // ARGMIN / ARGMAX partials
   gb_params const& P = *Pp;
   int const lane     = threadIdx.x & 63;
   int const wave     = threadIdx.x >> 6;
@@ -1550,7 +1552,7 @@ __global__ void __launch_bounds__(BLOCK)
       // the field's column among the loaded ones (uniform select, no indexing)
       loaded_value const v  = cs <= 0 ? lv[0] : cs == 1 ? lv[1] : cs == 2 ? lv[2] : lv[3];
       bool const vv         = cs <= 0 ? vld[0] : cs == 1 ? vld[1] : cs == 2 ? vld[2] : vld[3];
-      if (kind == FK_ARGMIN || kind == FK_ARGMAX) {
+      if (HAS_ARG && (kind == FK_ARGMIN || kind == FK_ARGMAX)) {
         bool const is_min  = kind == FK_ARGMIN;
         uint32_t const crow = carry_row[wave][k];
         if (carry_ends && lane == 0) arg_flush(P, f, carry_g, crow, carry_g == first_g);
@@ -1586,7 +1588,7 @@ __global__ void __launch_bounds__(BLOCK)
   }
   if (carry_g != NO_GROUP && lane == 0) {
     for (int k = 0; k < pass.n; k++) {  // may continue in the next range
-      if (pass.kind[k] == FK_ARGMIN || pass.kind[k] == FK_ARGMAX)
+      if (HAS_ARG && (pass.kind[k] == FK_ARGMIN || pass.kind[k] == FK_ARGMAX))
         arg_flush(P, pass.field[k], carry_g, carry_row[wave][k], true);
       else
         sorted_flush(P, pass.field[k], carry_g, carry[wave][k], true);
@@ -1662,7 +1664,10 @@ static void launch_sorted_passes(gb_params const* dP, gb_params const& hP, int64
       pass.n++;
       i++;
     }
-    hipLaunchKernelGGL(k_sorted_reduce, dim3(blocks), dim3(BLOCK), 0, s, dP, gid, per, W, pass);
+    bool has_arg = false;
+    for (int k = 0; k < pass.n; k++) has_arg |= pass.kind[k] == FK_ARGMIN || pass.kind[k] == FK_ARGMAX;
+    if (has_arg) hipLaunchKernelGGL(k_sorted_reduce<true>, dim3(blocks), dim3(BLOCK), 0, s, dP, gid, per, W, pass);
+    else hipLaunchKernelGGL(k_sorted_reduce<false>, dim3(blocks), dim3(BLOCK), 0, s, dP, gid, per, W, pass);
   }
 }
 
@@ -1684,8 +1689,12 @@ void launch_sorted_second_pass(gb_params const* dP, gb_params const& hP, int64_t
 // row that claims an empty slot adds one distinct value to its group.  Rows
 // are equal when they share the group and both are null (counted only with
 // nulls included) or hold the same value, -0.0 equal to 0.0 and NaN equal to
-// NaN (the row comparator's defaults).  The set has at least twice as many
-// slots as rows, so every probe sequence reaches an empty slot or a match.
+// NaN (the row comparator's defaults).  Group g probes only its own region
+// of the set, slots [2 * first, 2 * end) where [first, end) are the rows from
+// its first row to the next group's: the region has at least twice as many
+// slots as the group has rows, so every probe sequence reaches an empty slot
+// or a match, and a group's probes stay within a few cache lines (the groups
+// of sorted keys are contiguous rows).
 __device__ __forceinline__ uint64_t nunique_bits(dcol const& c, int64_t row)
 {
   uint64_t b = raw_bits(c, row);
@@ -1708,37 +1717,58 @@ __global__ void __launch_bounds__(BLOCK) k_fill_empty(uint32_t* __restrict__ out
 }
 
 __global__ void __launch_bounds__(BLOCK)
-  k_sorted_nunique(gb_params const* __restrict__ Pp, int32_t f, int64_t n, uint32_t const* __restrict__ gid,
-                   uint32_t* __restrict__ table, uint64_t cap_mask)
+  k_sorted_nunique(gb_params const* __restrict__ Pp, int32_t f, int64_t n, int64_t G,
+                   uint32_t const* __restrict__ gid, uint32_t* __restrict__ table)
 {
   constexpr uint32_t EMPTY = 0xFFFFFFFFu;
   gb_params const& P   = *Pp;
   field_desc const& F  = P.fields[f];
   dcol const& c        = P.vals[F.vcol];
   bool const with_null = F.kind == FK_NUNIQUE_NULLS;
-  for (int64_t row = int64_t(blockIdx.x) * BLOCK + threadIdx.x; row < n; row += int64_t(gridDim.x) * BLOCK) {
-    uint32_t const g = gid[row];
-    if (g == NO_GROUP) continue;
-    bool const valid = dcol_valid(c, row);
-    if (!valid && !with_null) continue;
-    uint64_t const b = valid ? nunique_bits(c, row) : 0ull;
-    uint32_t const h = mm_fmix32(uint32_t(b) ^ mm_fmix32(uint32_t(b >> 32) ^ mm_fmix32(g * 2u + uint32_t(valid))));
-    uint64_t slot    = h & cap_mask;
-    for (;;) {
-      uint32_t cur = __hip_atomic_load(&table[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      if (cur == EMPTY) {
-        uint32_t const prev = atomicCAS(&table[slot], EMPTY, uint32_t(row));
-        if (prev == EMPTY) {
-          atomicAdd(reinterpret_cast<unsigned int*>(field_at(F, g)), 1u);
-          break;
+  int const lane       = threadIdx.x & 63;
+  // each wave takes 64 consecutive rows per step (the loop bound is uniform
+  // across the wave, so every lane reaches the ballots below)
+  for (int64_t base0 = int64_t(blockIdx.x) * BLOCK + (threadIdx.x & ~63); base0 < n;
+       base0 += int64_t(gridDim.x) * BLOCK) {
+    int64_t const row = base0 + lane;
+    uint32_t const g  = row < n ? gid[row] : NO_GROUP;
+    bool const valid  = g != NO_GROUP && dcol_valid(c, row);
+    bool inserted     = false;
+    if (g != NO_GROUP && (valid || with_null)) {
+      uint64_t const b     = valid ? nunique_bits(c, row) : 0ull;
+      uint32_t const h     = mm_fmix32(uint32_t(b) ^ mm_fmix32(uint32_t(b >> 32) ^ uint32_t(valid)));
+      uint64_t const first = P.slot_rows[g];
+      uint64_t const end   = int64_t(g) + 1 < G ? P.slot_rows[g + 1] : uint64_t(n);
+      uint64_t const base  = 2 * first, len = 2 * (end - first);
+      uint64_t off         = (uint64_t(h) * len) >> 32;  // in [0, len)
+      for (;;) {
+        uint64_t const slot = base + off;
+        uint32_t cur = __hip_atomic_load(&table[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (cur == EMPTY) {
+          uint32_t const prev = atomicCAS(&table[slot], EMPTY, uint32_t(row));
+          if (prev == EMPTY) {
+            inserted = true;
+            break;
+          }
+          cur = prev;
         }
-        cur = prev;
-      }
-      if (gid[cur] == g) {
+        // the region holds rows of group g only
         bool const cv = dcol_valid(c, cur);
         if (cv == valid && (!valid || nunique_bits(c, cur) == b)) break;  // already counted
+        off = off + 1 == len ? 0 : off + 1;
       }
-      slot = (slot + 1) & cap_mask;
+    }
+    // one add per run of equal groups in the wave (group numbers do not
+    // decrease along the rows): the run's first lane adds the run's insertions
+    uint32_t const gp_  = uint32_t(__shfl(int(g), lane == 0 ? 0 : lane - 1));
+    bool const head     = lane == 0 || gp_ != g;
+    uint64_t const heads = __ballot(head);
+    uint64_t const ins   = __ballot(inserted);
+    if (head && g != NO_GROUP) {
+      uint64_t const later = lane == 63 ? 0ull : heads & ~((2ull << lane) - 1ull);  // heads above this lane
+      uint64_t const upto  = later ? ((later & (~later + 1ull)) - 1ull) : ~0ull;     // lanes below the next head
+      uint32_t const cnt   = uint32_t(__popcll(ins & upto & ~((1ull << lane) - 1ull)));
+      if (cnt) atomicAdd(reinterpret_cast<unsigned int*>(field_at(F, g)), cnt);
     }
   }
 }
@@ -1799,13 +1829,13 @@ void launch_sorted_nth(gb_params const* dP, int32_t f, int64_t n, int64_t G, uin
   if (G > 0) hipLaunchKernelGGL(k_sorted_nth, dim3(unsigned((G + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0, s, dP, f, n, G, gid);
 }
 
-void launch_sorted_nunique(gb_params const* dP, int32_t f, int64_t n, uint32_t const* gid, uint32_t* table,
-                           uint64_t cap, hipStream_t s)
+void launch_sorted_nunique(gb_params const* dP, int32_t f, int64_t n, int64_t G, uint32_t const* gid,
+                           uint32_t* table, hipStream_t s)
 {
-  if (n <= 0) return;
+  if (n <= 0 || G <= 0) return;
   auto const grid = [](int64_t m) { return dim3(unsigned(std::min<int64_t>((m + BLOCK - 1) / BLOCK, 1 << 20))); };
-  hipLaunchKernelGGL(k_fill_empty, grid(int64_t(cap)), dim3(BLOCK), 0, s, table, int64_t(cap));
-  hipLaunchKernelGGL(k_sorted_nunique, grid(n), dim3(BLOCK), 0, s, dP, f, n, gid, table, cap - 1);
+  hipLaunchKernelGGL(k_fill_empty, grid(2 * n), dim3(BLOCK), 0, s, table, 2 * n);
+  hipLaunchKernelGGL(k_sorted_nunique, grid(n), dim3(BLOCK), 0, s, dP, f, n, G, gid, table);
 }
 
 // String MIN / MAX from the groups' ARGMIN / ARGMAX rows: lengths (0 for

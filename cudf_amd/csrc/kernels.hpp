@@ -263,10 +263,10 @@ void launch_sorted_aggregate(gb_params const* dP, gb_params const& hP, int64_t n
 void launch_sorted_second_pass(gb_params const* dP, gb_params const& hP, int64_t n, uint32_t const* gid,
                                int num_cus, hipStream_t s);
 // NUNIQUE of field `f` on runs (`gid` = group of every row, NO_GROUP when
-// dropped): a set of (group, value) in `table` (`cap` u32 slots, a power of two
-// of at least 2n; cleared here) counts each group's distinct values.
-void launch_sorted_nunique(gb_params const* dP, int32_t f, int64_t n, uint32_t const* gid, uint32_t* table,
-                           uint64_t cap, hipStream_t s);
+// dropped; G groups): a set of (group, value) in `table` (2n u32 slots,
+// cleared here) counts each group's distinct values.
+void launch_sorted_nunique(gb_params const* dP, int32_t f, int64_t n, int64_t G, uint32_t const* gid,
+                           uint32_t* table, hipStream_t s);
 // NTH_ELEMENT of field `f` on runs: the row of each group's n-th counted value.
 void launch_sorted_nth(gb_params const* dP, int32_t f, int64_t n, int64_t G, uint32_t const* gid, hipStream_t s);
 void launch_iota_u32(uint32_t* out, int64_t n, hipStream_t s);
