@@ -94,7 +94,8 @@ std::map<int, std::pair<R, bool>> result_by_key(cudf::table& keys, cudf::column 
 template <typename R>
 void run_case(char const* name, std::vector<int> keys, std::vector<int> key_valid, std::vector<int> vals,
               std::vector<int> val_valid, std::unique_ptr<cudf::groupby_aggregation> agg,
-              std::map<int, std::pair<R, bool>> expect, cudf::null_policy policy = cudf::null_policy::EXCLUDE)
+              std::map<int, std::pair<R, bool>> expect, cudf::null_policy policy = cudf::null_policy::EXCLUDE,
+              cudf::sorted keys_sorted = cudf::sorted::NO)
 {
   std::vector<std::unique_ptr<dev_buf>> keep;
   auto k = make_col(keys, key_valid, cudf::type_id::INT32, keep);
@@ -102,7 +103,7 @@ void run_case(char const* name, std::vector<int> keys, std::vector<int> key_vali
   std::vector<cudf::groupby::aggregation_request> requests(1);
   requests[0].values = v;
   requests[0].aggregations.push_back(std::move(agg));
-  cudf::groupby::groupby gb(cudf::table_view{{k}}, policy);
+  cudf::groupby::groupby gb(cudf::table_view{{k}}, policy, keys_sorted);
   auto [out_keys, results] = gb.aggregate(requests);
   CHECK(out_keys->num_rows() == int(expect.size()), "%s: %d groups, expected %zu", name,
         out_keys->num_rows(), expect.size());
@@ -136,6 +137,29 @@ int main()
   std::vector<int> const VNV{0, 1, 1, 1, 1, 0, 1, 1, 1, 1, 0};
   using gba = cudf::groupby_aggregation;
 
+  // sort-only kinds on pre-sorted keys: nunique_tests.cpp:70-88, 175-200 and
+  // nth_element_tests.cpp:35-59, 94-119 with the rows stably sorted by key (null keys last)
+  std::vector<int> const KS{1, 1, 1, 2, 2, 2, 2, 3, 3, 3};
+  auto const SORTED = cudf::sorted::YES;
+  auto const EXCL   = cudf::null_policy::EXCLUDE;
+  run_case<int32_t>("nunique sorted duplicates", KS, {}, {0, 3, 3, 1, 4, 5, 9, 2, 2, 2}, {},
+                    cudf::make_nunique_aggregation<gba>(), {{1, {2, true}}, {2, {4, true}}, {3, {1, true}}},
+                    EXCL, SORTED);
+  std::vector<int> const KS14{1, 1, 1, 2, 2, 2, 2, 2, 3, 3, 3, 4, 4, 3};
+  std::vector<int> const KS14V{1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 0};
+  std::vector<int> const VS14{0, 3, 6, 1, 4, 5, 9, 2, 2, 2, 8, 4, 4, 7};
+  std::vector<int> const VS14V{0, 1, 1, 1, 1, 0, 1, 0, 1, 1, 1, 0, 0, 1};
+  run_case<int32_t>("nunique sorted include nulls", KS14, KS14V, VS14, VS14V,
+                    cudf::make_nunique_aggregation<gba>(cudf::null_policy::INCLUDE),
+                    {{1, {3, true}}, {2, {4, true}}, {3, {2, true}}, {4, {1, true}}}, EXCL, SORTED);
+  std::vector<int> const VS{0, 3, 6, 1, 4, 5, 9, 2, 7, 8};
+  run_case<int32_t>("nth_element 1 sorted", KS, {}, VS, {}, cudf::make_nth_element_aggregation<gba>(1),
+                    {{1, {3, true}}, {2, {4, true}}, {3, {7, true}}}, EXCL, SORTED);
+  run_case<int32_t>("nth_element -1 sorted", KS, {}, VS, {}, cudf::make_nth_element_aggregation<gba>(-1),
+                    {{1, {6, true}}, {2, {9, true}}, {3, {8, true}}}, EXCL, SORTED);
+  run_case<int32_t>("nth_element 1 exclude nulls sorted", KS14, KS14V, VS14, VS14V,
+                    cudf::make_nth_element_aggregation<gba>(1, cudf::null_policy::EXCLUDE),
+                    {{1, {6, true}}, {2, {4, true}}, {3, {2, true}}, {4, {0, false}}}, EXCL, SORTED);
   // sum_tests.cpp:38-54, 110-132
   run_case<int64_t>("sum basic", K, {}, V, {}, cudf::make_sum_aggregation<gba>(),
                     {{1, {9, true}}, {2, {19, true}}, {3, {17, true}}});
