@@ -16,7 +16,12 @@ The adapter between the two semantics (the reference's parity tests,
 * pandas ``count`` is int64 where COUNT is int32; ``mean``/``var``/``std``
   are float64 with NaN for null, float32 ``mean`` stays float32;
 * ``sort=False`` orders groups by first appearance: the GPU path computes
-  each group's first row (MIN of the row number) and orders by it.
+  each group's first row (MIN of the row number) and orders by it;
+* ``nunique``, ``first`` and ``last`` (sort-only aggregations: NUNIQUE and
+  NTH_ELEMENT 0 / -1 with nulls excluded, as pandas skips missing values) run
+  on the GPU when the frame is grouped by one key column that is already
+  sorted ascending without missing keys (the pre-sorted path, whose runs are
+  then the groups in key order); otherwise pandas runs them.
 """
 from __future__ import annotations
 
@@ -31,6 +36,10 @@ _KINDS = {"sum": Kind.SUM, "count": Kind.COUNT_VALID, "min": Kind.MIN, "max": Ki
           "prod": Kind.PRODUCT}
 # aggregations the GPU takes on string values
 _STRING_VALUE_AGGS = {"count", "size", "min", "max"}
+# sort-only aggregations: taken with one key column sorted ascending
+_SORT_ONLY = {"nunique": lambda: Aggregation(Kind.NUNIQUE),
+              "first": lambda: Aggregation(Kind.NTH_ELEMENT, n=0),
+              "last": lambda: Aggregation(Kind.NTH_ELEMENT, n=-1)}
 
 
 class Unsupported(Exception):
@@ -72,20 +81,28 @@ def _normalise_spec(spec) -> Tuple[Dict[str, List[str]], bool]:
 
 def _gpu(df, by: List[str], spec: Dict[str, List[str]], flat: bool, sort: bool, dropna: bool):
     import pandas as pd
-    from . import GroupBy, GroupByRequest, NullPolicy, Table
+    from . import GroupBy, GroupByRequest, NullPolicy, Sorted, Table
+    sort_only = False
     for aggs in spec.values():
         for a in aggs:
-            if a not in _KINDS:
+            if a in _SORT_ONLY:
+                sort_only = True
+            elif a not in _KINDS:
                 raise Unsupported(f"aggregation {a}")
+    if sort_only:
+        if len(by) != 1 or df[by[0]].isna().any() or not df[by[0]].is_monotonic_increasing:
+            raise Unsupported("nunique / first / last need one sorted key column without missing keys")
     keys = Table([_series_to_column(df[k], True) for k in by])
     reqs, order = [], []
     for c, aggs in spec.items():
         col = _series_to_column(df[c], False, strings_ok=set(aggs) <= _STRING_VALUE_AGGS)
-        reqs.append(GroupByRequest(col, [Aggregation(_KINDS[a], 1) for a in aggs]))
+        reqs.append(GroupByRequest(col, [_SORT_ONLY[a]() if a in _SORT_ONLY else Aggregation(_KINDS[a], 1)
+                                         for a in aggs]))
         order.append((c, aggs))
     rownum = _series_to_column(pd.Series(np.arange(len(df), dtype=np.int64)), False)
     reqs.append(GroupByRequest(rownum, [Aggregation(Kind.MIN)]))
-    out_keys, out = GroupBy(keys, NullPolicy.EXCLUDE if dropna else NullPolicy.INCLUDE).aggregate(reqs)
+    out_keys, out = GroupBy(keys, NullPolicy.EXCLUDE if dropna else NullPolicy.INCLUDE,
+                            Sorted.YES if sort_only else Sorted.NO).aggregate(reqs)
     first = out[-1].columns()[0].to_numpy()[0]
     # key index
     key_arrays = []
@@ -128,9 +145,9 @@ def _gpu(df, by: List[str], spec: Dict[str, List[str]], flat: bool, sort: bool, 
                     vals = np.where(valid, vals, 0 if a == "sum" else 1).astype(vals.dtype)
                 if isinstance(src.dtype, pd.api.extensions.ExtensionDtype):
                     vals = pd.array(vals, dtype=pd.Int64Dtype() if vals.dtype.kind == "i" else src.dtype)
-            elif a in ("count", "size"):
+            elif a in ("count", "size", "nunique"):
                 vals = vals.astype(np.int64)
-            elif a in ("min", "max"):
+            elif a in ("min", "max", "first", "last"):
                 if isinstance(src.dtype, pd.api.extensions.ExtensionDtype):
                     vals = pd.array(vals, dtype=src.dtype)
                     if valid is not None:

@@ -146,3 +146,22 @@ def test_arrow_round_trip_and_groupby():
     assert len(got) == len(want)
     for k, v in want.items():
         assert (got[k] is None and pd.isna(v)) or got[k] == v
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("sort", [False, True])
+def test_gpu_nunique_first_last_on_sorted_keys(sort):
+    """nunique / first / last on a frame sorted by its key run on the
+    pre-sorted path and match pandas; on unsorted keys they fall back."""
+    from cudf_amd.pandas_compat import groupby_agg
+    df = frame(21)
+    df = df.dropna(subset=["k"]).sort_values("k", kind="stable").reset_index(drop=True)
+    df["r"] = np.round(df["f"] / 25)  # few distinct values per group (and NaN)
+    spec = {"r": ["nunique", "first", "last", "count"], "j": ["nunique", "first", "last"],
+            "i": ["first", "last"]}
+    got, path = groupby_agg(df, "k", spec, sort=sort, dropna=True, fallback=False)
+    assert path == "gpu"
+    compare(got, df.groupby("k", sort=sort, dropna=True).agg(spec))
+    shuffled = df.sample(frac=1.0, random_state=3)
+    got, path = groupby_agg(shuffled, "k", spec, sort=sort)
+    assert path == "pandas"
