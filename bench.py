@@ -120,7 +120,7 @@ def c4_shapes(n_inst, seed=4):
 def make_c4(n_inst, seed=4):
     """The batched workload on the device: per instance an int32 key uniform
     over [0, card), an int64 value uniform in [-1e9, 1e9] and a float32 value
-    uniform in [0, 1); instances are views into three concatenated buffers."""
+    uniform in [0, 1); instances are slices of three concatenated buffers."""
     import torch
     from cudf_amd import Column, TypeId
     rows, card = c4_shapes(n_inst, seed)
@@ -133,12 +133,14 @@ def make_c4(n_inst, seed=4):
         keys[s:s + r].random_(0, int(c), generator=g)
     v0 = torch.randint(-1_000_000_000, 1_000_000_001, (total,), dtype=torch.int64, device=dev, generator=g)
     v1 = torch.rand(total, dtype=torch.float32, device=dev, generator=g)
-    kb, v0b, v1b = keys.view(torch.uint8), v0.view(torch.uint8), v1.view(torch.uint8)
+    # each instance's columns start at their own data pointer (row offset 0):
+    # the concatenated buffers hold ~2e9 rows, beyond cudf's int32 row offsets
     inst = []
     for s, r in zip(starts, rows):
-        inst.append((Column(TypeId.INT32, int(r), kb, None, 0, int(s)),
-                     [Column(TypeId.INT64, int(r), v0b, None, 0, int(s)),
-                      Column(TypeId.FLOAT32, int(r), v1b, None, 0, int(s))]))
+        s, r = int(s), int(r)
+        inst.append((Column(TypeId.INT32, r, keys[s:s + r].view(torch.uint8), None, 0, 0),
+                     [Column(TypeId.INT64, r, v0[s:s + r].view(torch.uint8), None, 0, 0),
+                      Column(TypeId.FLOAT32, r, v1[s:s + r].view(torch.uint8), None, 0, 0)]))
     return inst, rows, card, (keys, v0, v1)
 
 

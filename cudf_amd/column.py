@@ -17,6 +17,9 @@ from . import _lib
 from .types import TypeId, numpy_dtype, type_id_of
 
 
+_INT32_MAX = 2**31 - 1
+
+
 def _device() -> torch.device:
     return torch.device("cuda", torch.cuda.current_device())
 
@@ -93,6 +96,11 @@ class Column:
 
     # -- C ABI ----------------------------------------------------------------
     def view(self) -> _lib.ColumnView:
+        """The C ABI view.  Its size and offset are cudf ``size_type`` (int32):
+        values outside [0, 2^31) raise instead of wrapping."""
+        for name, x in (("size", self.size), ("offset", self.offset)):
+            if not 0 <= x <= _INT32_MAX:
+                raise OverflowError(f"column {name} {x} is outside [0, 2^31 - 1] (cudf size_type)")
         v = _lib.ColumnView()
         v.type_id = int(self.type_id)
         v.scale = 0

@@ -325,7 +325,7 @@ mig_batch_plan* batch_plan_create(mig_batch_instance const* instances, int32_t n
     mig_column_view const& k = instances[i].keys[0];
     if (!fixed_width(k.type_id))
       fail(MIG_ERROR_DATA_TYPE, "the batched group-by takes one fixed-width key column per instance");
-    MIG_EXPECTS(k.size >= 0, "negative column size");
+    MIG_EXPECTS(k.size >= 0 && k.offset >= 0, "negative column size or offset");
     p.key_views[i]  = k;
     p.hinst[i].rows = k.size;
     p.hinst[i].key  = make_dcol(k, k.null_mask != nullptr);
@@ -364,6 +364,7 @@ mig_batch_plan* batch_plan_create(mig_batch_instance const* instances, int32_t n
       if (!fixed_width(c.type_id))
         fail(MIG_ERROR_DATA_TYPE, "the batched group-by takes fixed-width value columns");
       MIG_EXPECTS(c.size == p.key_views[i].size, "Size mismatch between request values and groupby keys.");
+      MIG_EXPECTS(c.offset >= 0, "negative column offset");
       p.val_views[size_t(i) * p.num_vals + v] = c;
     }
   for (int32_t r = 0; r < num_requests; r++)

@@ -512,6 +512,7 @@ void hash_rows(mig_column_view const* cols, int32_t ncols, uint32_t seed, uint32
   int64_t const n = cols[0].size;
   for (int i = 0; i < ncols; i++) {
     MIG_EXPECTS(cols[i].size == n, "all columns must have the same number of rows");
+    MIG_EXPECTS(cols[i].size >= 0 && cols[i].offset >= 0, "negative column size or offset");
     if (storage_of(cols[i].type_id) == ST_NONE)
       fail(MIG_ERROR_DATA_TYPE, "unsupported column type " + std::to_string(cols[i].type_id) + " for hashing");
   }
@@ -544,6 +545,7 @@ std::unique_ptr<mig_groupby_result> groupby_aggregate(mig_column_view const* key
   MIG_EXPECTS(null_policy == MIG_NULL_EXCLUDE || null_policy == MIG_NULL_INCLUDE, "invalid null policy");
   int64_t const N = num_keys > 0 ? keys[0].size : 0;
   for (int i = 0; i < num_keys; i++) {
+    MIG_EXPECTS(keys[i].size >= 0 && keys[i].offset >= 0, "negative column size or offset");
     MIG_EXPECTS(keys[i].size == N, "key columns must have the same number of rows");
     int32_t st = storage_of(keys[i].type_id);
     if (st == ST_NONE)
@@ -553,6 +555,7 @@ std::unique_ptr<mig_groupby_result> groupby_aggregate(mig_column_view const* key
   for (int r = 0; r < num_requests; r++) {
     MIG_EXPECTS(requests[r].values.size == N,
                 "Size mismatch between request values and groupby keys.");
+    MIG_EXPECTS(requests[r].values.offset >= 0, "negative column offset");
     MIG_EXPECTS(requests[r].num_aggregations >= 0, "negative number of aggregations");
     for (int a = 0; a < requests[r].num_aggregations; a++) {
       check_supported(requests[r].values.type_id, requests[r].aggregations[a].kind);

@@ -1329,6 +1329,8 @@ struct sorted_pass {
 };
 
 // Element bits of a fixed-width column as stored (no float normalisation).
+// A string column (width 0) is in a pass only for its validity (COUNT_VALID):
+// its chars buffer is not indexed by row, so nothing is loaded.
 __device__ __forceinline__ uint64_t raw_bits(dcol const& c, int64_t row)
 {
   auto const d = gp(c.data);
@@ -1336,7 +1338,8 @@ __device__ __forceinline__ uint64_t raw_bits(dcol const& c, int64_t row)
     case 1: return d[row];
     case 2: return reinterpret_cast<gmem<uint16_t const>*>(d)[row];
     case 4: return reinterpret_cast<gmem<uint32_t const>*>(d)[row];
-    default: return reinterpret_cast<gmem<uint64_t const>*>(d)[row];
+    case 8: return reinterpret_cast<gmem<uint64_t const>*>(d)[row];
+    default: return 0;
   }
 }
 

@@ -310,6 +310,7 @@ std::unique_ptr<mig_table> partition_by_map(mig_column_view const* input, int32_
   check_table(input, num_columns);
   int64_t const n = num_columns > 0 ? input[0].size : 0;
   MIG_EXPECTS(partition_map.size == n, "Size mismatch between table and partition map.");
+  MIG_EXPECTS(partition_map.offset >= 0, "negative column offset");
   MIG_EXPECTS(null_count_of(partition_map, stream) == 0, "Unexpected null values in partition_map.");
   int32_t const st = storage_of(partition_map.type_id);
   bool const integral = st == ST_I8 || st == ST_I16 || st == ST_I32 || st == ST_I64 || st == ST_U8 ||

@@ -4,7 +4,6 @@ cpp/tests/groupby/groupby_test_util.cpp:34-98: output order is unspecified,
 so both sides are sorted by key before comparing)."""
 from __future__ import annotations
 
-import math
 from typing import List, Optional, Sequence
 
 import numpy as np
@@ -30,11 +29,33 @@ def host_col(arr, valid=None) -> orc.HostColumn:
 
 
 def to_device(c: orc.HostColumn):
-    from cudf_amd import Column
+    import torch
+    from cudf_amd import Column, TypeId
+    from cudf_amd.column import pack_validity
     if c.type_id == orc.STRING:
-        strs = [bytes(c.data[c.offsets[i]:c.offsets[i + 1]]) for i in range(c.size)]
-        return Column.from_strings(strs, c.valid)
+        # offsets and chars copied as arrays (int32 offsets when they fit)
+        offs = np.asarray(c.offsets, dtype=np.int64)
+        large = offs[-1] >= 2**31
+        offs = offs if large else offs.astype(np.int32)
+        chars = np.asarray(c.data, dtype=np.uint8)
+        dev = torch.device("cuda", torch.cuda.current_device())
+        data = torch.from_numpy(chars.copy() if chars.size else np.zeros(1, np.uint8)).to(dev)
+        mask, nulls = None, 0
+        if c.valid is not None:
+            mask = torch.from_numpy(pack_validity(c.valid).view(np.int32).copy()).to(dev)
+            nulls = int(len(c.valid) - np.count_nonzero(c.valid))
+        return Column(TypeId.STRING, c.size, data, mask, nulls, offsets=torch.from_numpy(offs.copy()).to(dev))
     return Column.from_numpy(c.data, c.valid, type_id=c.type_id)
+
+
+def host_keys(col):
+    """(values, valid) of a device key column; strings as ``StrKeys``."""
+    from cudf_amd import TypeId
+    if col.type_id == TypeId.STRING:
+        offs = col.offsets.cpu().numpy().astype(np.int64)[col.offset: col.offset + col.size + 1]
+        chars = col.data.cpu().numpy().view(np.uint8) if col.data is not None else np.zeros(0, np.uint8)
+        return StrKeys(offs, chars), col.validity()
+    return col.to_numpy()
 
 
 def gpu_groupby(keys: Sequence[orc.HostColumn], requests, null_policy: int = 0, keys_sorted: bool = False):
@@ -49,7 +70,7 @@ def gpu_groupby(keys: Sequence[orc.HostColumn], requests, null_policy: int = 0, 
         reqs.append(GroupByRequest(to_device(col), [_aggregation(k, d) for k, d in aggs]))
     gb = GroupBy(dkeys, NullPolicy(null_policy), Sorted.YES if keys_sorted else Sorted.NO)
     out_keys, out_res = gb.aggregate(reqs)
-    kcols = [c.to_numpy() for c in out_keys.columns()]
+    kcols = [host_keys(c) for c in out_keys.columns()]
     res = [[c.to_numpy() for c in t.columns()] for t in out_res]
     return kcols, res
 
@@ -66,12 +87,58 @@ def _aggregation(kind, param):
     return Aggregation(kind, param)
 
 
+def take(col: orc.HostColumn, rows: np.ndarray) -> orc.HostColumn:
+    """Rows ``rows`` of a host column (strings re-packed)."""
+    valid = None if col.valid is None else col.valid[rows]
+    if col.type_id == orc.STRING:
+        sk = StrKeys(col.offsets, col.data).gather(rows)
+        chars = sk.chars if len(sk.chars) else np.zeros(1, np.uint8)
+        return orc.HostColumn(orc.STRING, chars, valid, sk.offsets)
+    return orc.HostColumn(col.type_id, col.data[rows], valid)
+
+
+# Rows per shard below which the oracle runs in one call.
+_SHARD_MIN_ROWS = 4_000_000
+
+
+def _oracle_sharded(keys, requests, null_policy, shards):
+    """The oracle on key-disjoint shards in parallel threads (ctypes releases
+    the GIL): rows go to shard row_hash % shards, so every group lies in one
+    shard and the union of the shards' groups is the full result.  Returns
+    what ``orc.groupby`` returns, with rep rows of the whole input."""
+    from concurrent.futures import ThreadPoolExecutor
+    h = orc.hash_rows(list(keys))
+    sid = (h % np.uint32(shards)).astype(np.uint8)
+    order = np.argsort(sid, kind="stable")
+    bounds = np.searchsorted(sid[order], np.arange(shards + 1))
+
+    def run(i):
+        rows = order[bounds[i]:bounds[i + 1]]
+        sk = [take(k, rows) for k in keys]
+        sr = [(take(c, rows), aggs) for c, aggs in requests]
+        rep, res = orc.groupby(sk, sr, null_policy)
+        return rows[rep], res
+
+    with ThreadPoolExecutor(max_workers=shards) as ex:
+        parts = list(ex.map(run, range(shards)))
+    rep = np.concatenate([p[0] for p in parts])
+    results = []
+    for r, (_, aggs) in enumerate(requests):
+        results.append([(np.concatenate([p[1][r][a][0] for p in parts]),
+                         np.concatenate([p[1][r][a][1] for p in parts])) for a in range(len(aggs))])
+    return rep, results
+
+
 def oracle_groupby(keys: Sequence[orc.HostColumn], requests, null_policy: int = 0):
-    rep, res = orc.groupby(keys, requests, null_policy)
+    n = keys[0].size if keys else 0
+    if n >= 2 * _SHARD_MIN_ROWS:
+        rep, res = _oracle_sharded(keys, requests, null_policy, int(min(16, n // _SHARD_MIN_ROWS)))
+    else:
+        rep, res = orc.groupby(keys, requests, null_policy)
     kcols = []
     for k in keys:
         if k.type_id == orc.STRING:
-            vals = [bytes(k.data[k.offsets[r]:k.offsets[r + 1]]) for r in rep]
+            vals = StrKeys(k.offsets, k.data).gather(rep)
         else:
             vals = k.data[rep]
         valid = None if k.valid is None else k.valid[rep]
@@ -79,90 +146,218 @@ def oracle_groupby(keys: Sequence[orc.HostColumn], requests, null_policy: int = 
     return kcols, res
 
 
-def _key_sort_token(v, valid):
-    if not valid:
-        return (1, 0)
-    if isinstance(v, (bytes, bytearray)):
-        return (0, bytes(v))
-    f = float(v) if isinstance(v, (float, np.floating)) else None
-    if f is not None:
-        if math.isnan(f):
-            return (0, (1, 0.0))
-        return (0, (0, f + 0.0))
-    return (0, int(v))
+class StrKeys:
+    """A string key column as arrays: ``offsets`` (int64, size + 1, absolute
+    into ``chars``) and ``chars`` (uint8).  Indexing gives ``bytes``."""
+
+    def __init__(self, offsets, chars):
+        self.offsets = np.asarray(offsets, dtype=np.int64)
+        self.chars = np.asarray(chars, dtype=np.uint8)
+
+    @staticmethod
+    def from_list(strs) -> "StrKeys":
+        lens = np.array([len(x) for x in strs], dtype=np.int64)
+        offs = np.zeros(len(strs) + 1, np.int64)
+        np.cumsum(lens, out=offs[1:])
+        return StrKeys(offs, np.frombuffer(b"".join(bytes(x) for x in strs), np.uint8))
+
+    def __len__(self):
+        return len(self.offsets) - 1
+
+    def __getitem__(self, i):
+        return bytes(self.chars[self.offsets[i]:self.offsets[i + 1]])
+
+    def gather(self, rows) -> "StrKeys":
+        rows = np.asarray(rows, dtype=np.int64)
+        lens = (self.offsets[1:] - self.offsets[:-1])[rows]
+        offs = np.zeros(len(rows) + 1, np.int64)
+        np.cumsum(lens, out=offs[1:])
+        src = np.repeat(self.offsets[:-1][rows] - offs[:-1], lens) + np.arange(offs[-1])
+        return StrKeys(offs, self.chars[src] if offs[-1] else np.zeros(0, np.uint8))
+
+    def __iter__(self):
+        return (self[i] for i in range(len(self)))
+
+    def words(self):
+        """uint64 arrays whose lexicographic order is the strings' byte-wise
+        order: the bytes zero-padded, big-endian in 8-byte words, then the
+        length."""
+        lens = self.offsets[1:] - self.offsets[:-1]
+        n = len(lens)
+        nw = (int(lens.max()) + 7) // 8 if n else 0
+        mat = np.zeros((n, nw * 8), np.uint8)
+        if n and self.offsets[-1] > self.offsets[0]:
+            rows = np.repeat(np.arange(n), lens)
+            cols = np.arange(self.offsets[-1] - self.offsets[0]) - np.repeat(self.offsets[:-1] - self.offsets[0], lens)
+            mat[rows, cols] = self.chars[self.offsets[0]:self.offsets[-1]]
+        w = mat.view(">u8").astype(np.uint64) if nw else np.zeros((n, 0), np.uint64)
+        return [w[:, j] for j in range(nw)] + [lens.astype(np.uint64)]
+
+    def fingerprint(self) -> np.ndarray:
+        if getattr(self, "_fp", None) is None:
+            self._fp = self._fingerprint()
+        return self._fp
+
+    def _fingerprint(self) -> np.ndarray:
+        """64-bit fingerprint of every string (a position-keyed multiply-add
+        hash of the bytes and the length): equal strings give equal values,
+        so it orders both sides of a comparison the same way; equality itself
+        is checked on the bytes."""
+        lens = self.offsets[1:] - self.offsets[:-1]
+        n = len(lens)
+        h = lens.astype(np.uint64) * np.uint64(0x9E3779B97F4A7C15)
+        start, end = int(self.offsets[0]), int(self.offsets[-1])
+        if n == 0 or end == start:
+            return h
+        pos = np.arange(end - start) - np.repeat(self.offsets[:-1] - start, lens)
+        mult = _position_multipliers(int(lens.max()))
+        prod = self.chars[start:end].astype(np.uint64) * mult[pos]
+        idx = np.minimum(self.offsets[:-1] - start, end - start - 1)
+        sums = np.add.reduceat(prod, idx)
+        return h ^ np.where(lens > 0, sums, np.uint64(0))
 
 
-def sort_order(kcols) -> np.ndarray:
-    n = len(kcols[0][0]) if kcols else 0
-    toks = []
-    for i in range(n):
-        t = []
+_MULTS = np.zeros(0, np.uint64)
+
+
+def _position_multipliers(n):
+    global _MULTS
+    if len(_MULTS) < n:
+        rng = np.random.default_rng(12345)
+        _MULTS = rng.integers(0, 2**63, max(n, 64), dtype=np.uint64) * np.uint64(2) + np.uint64(1)
+    return _MULTS
+
+
+def _as_keys(vals):
+    if isinstance(vals, (list, tuple)):
+        return StrKeys.from_list(vals)
+    return vals
+
+
+def _key_arrays(vals, valid):
+    """Arrays (most significant first) that order one key column and decide
+    key equality under the group-by's rules: nulls equal each other and sort
+    last, NaN equals NaN, -0.0 equals 0.0, strings byte-wise."""
+    vals = _as_keys(vals)
+    n = len(vals)
+    v = np.ones(n, bool) if valid is None else np.asarray(valid, bool)
+    if isinstance(vals, StrKeys):
+        arrs = [np.where(v, vals.fingerprint(), np.uint64(0))]
+    else:
+        a = np.asarray(vals)
+        if a.dtype.kind == "f":
+            a = a.astype(np.float64)
+            nan = np.isnan(a)
+            arrs = [nan.astype(np.uint8), np.where(nan | ~v, 0.0, a) + 0.0]
+        elif a.dtype.kind == "b":
+            arrs = [np.where(v, a.astype(np.uint8), 0)]
+        else:
+            arrs = [np.where(v, a, 0)]
+    return [(~v).astype(np.uint8)] + arrs
+
+
+def sort_order(kcols, exact: bool = True) -> np.ndarray:
+    """Order of the rows of key columns ``[(values, valid), ...]``.  With
+    ``exact`` the rows are sorted by key (strings byte-wise); otherwise by a
+    64-bit hash of the key (the same order for the same set of keys on either
+    side of a comparison, much faster for millions of groups), falling back
+    to the exact sort if two different keys share a hash."""
+    if not kcols:
+        return np.zeros(0, np.int64)
+    if exact:
+        arrays = []
         for vals, valid in kcols:
-            t.append(_key_sort_token(vals[i], True if valid is None else bool(valid[i])))
-        toks.append(tuple(t))
-    return np.array(sorted(range(n), key=lambda i: toks[i]), dtype=np.int64)
-
-
-def _keys_equal(a, av, b, bv) -> bool:
-    if av != bv:
-        return False
-    if not av:
-        return True
-    if isinstance(a, (bytes, bytearray)):
-        return bytes(a) == bytes(b)
-    if isinstance(a, (float, np.floating)):
-        if math.isnan(a) and math.isnan(b):
-            return True
-        return a == b
-    return a == b
+            vals = _as_keys(vals)
+            if isinstance(vals, StrKeys):
+                v = np.ones(len(vals), bool) if valid is None else np.asarray(valid, bool)
+                arrays += [(~v).astype(np.uint8)] + [np.where(v, w, np.uint64(0)) for w in vals.words()]
+            else:
+                arrays += _key_arrays(vals, valid)
+        return np.lexsort(arrays[::-1]).astype(np.int64)
+    arrays = []
+    for vals, valid in kcols:
+        arrays += _key_arrays(vals, valid)
+    h = np.zeros(len(arrays[0]), np.uint64)
+    for a in arrays:
+        x = np.ascontiguousarray(a)
+        x = x.view(np.uint64) if x.dtype.itemsize == 8 else x.astype(np.uint64)
+        h = (h ^ x) * np.uint64(0x9E3779B97F4A7C15)
+        h ^= h >> np.uint64(29)
+    order = np.argsort(h, kind="stable")
+    hs = h[order]
+    if np.any(hs[1:] == hs[:-1]):
+        return sort_order(kcols, exact=True)
+    return order.astype(np.int64)
 
 
 def assert_keys_equal(gk, ok, og, oo):
-    """gk/ok: key columns; og/oo: sort orders."""
+    """gk/ok: key columns; og/oo: sort orders.  Keys must be equal row by row
+    (nulls equal, NaN equal NaN, -0.0 equal 0.0, strings byte-wise)."""
     assert len(og) == len(oo), f"group count differs: gpu {len(og)} vs oracle {len(oo)}"
     for c, ((gv, gval), (ov, oval)) in enumerate(zip(gk, ok)):
-        for i in range(len(og)):
-            a, b = gv[og[i]], ov[oo[i]]
-            av = True if gval is None else bool(gval[og[i]])
-            bv = True if oval is None else bool(oval[oo[i]])
-            assert _keys_equal(a, av, b, bv), f"key column {c} row {i}: gpu {a!r}/{av} vs oracle {b!r}/{bv}"
+        gv, ov = _as_keys(gv), _as_keys(ov)
+        for x, y in zip(_key_arrays(gv, gval), _key_arrays(ov, oval)):
+            bad = np.flatnonzero(x[og] != y[oo])
+            assert bad.size == 0, f"key column {c} sorted row {bad[0]}: gpu {gv[og[bad[0]]]!r} vs oracle {ov[oo[bad[0]]]!r}"
+        if isinstance(gv, StrKeys):
+            m = np.ones(len(og), bool) if gval is None else np.asarray(gval, bool)[og]
+            a, b = gv.gather(og[m]), ov.gather(oo[m])
+            assert np.array_equal(np.diff(a.offsets), np.diff(b.offsets)), f"key column {c}: string lengths differ"
+            bad = np.flatnonzero(a.chars != b.chars)
+            assert bad.size == 0, f"key column {c}: string bytes differ at char {bad[0]}"
 
 
-def group_abs_sums(keys, col: orc.HostColumn, null_policy: int) -> np.ndarray:
-    """Per-group sum of |x| (first-occurrence order of the oracle), used as the
-    scale of floating-point sum tolerances."""
-    absdata = np.abs(col.data.astype(np.float64))
-    absc = orc.HostColumn(orc.FLOAT64, absdata, col.valid)
-    _, res = orc.groupby(keys, [(absc, [(AGG["SUM"], 1)])], null_policy)
-    return res[0][0][0]
+_FP_KINDS = ("SUM", "MEAN", "VARIANCE", "STD", "PRODUCT", "SUM_OF_SQUARES")
 
 
-def group_sq_sums(keys, col: orc.HostColumn, null_policy: int) -> np.ndarray:
-    sq = col.data.astype(np.float64) ** 2
-    c = orc.HostColumn(orc.FLOAT64, sq, col.valid)
-    _, res = orc.groupby(keys, [(c, [(AGG["SUM"], 1)])], null_policy)
-    return res[0][0][0]
+def _scale_requests(requests):
+    """Extra oracle requests giving the scales of the floating-point bounds:
+    per floating-point column with such aggregations, sum |x|, sum x^2 and
+    the valid count per group.  Returns (requests, index of each column's
+    first extra request or None)."""
+    extra, where = [], []
+    for col, aggs in requests:
+        is_float = col.type_id in (orc.FLOAT32, orc.FLOAT64)
+        if is_float and any(k in [AGG[n] for n in _FP_KINDS] for k, _ in aggs):
+            x = col.data.astype(np.float64)
+            where.append(len(requests) + len(extra))
+            extra.append((orc.HostColumn(orc.FLOAT64, np.abs(x), col.valid), [(AGG["SUM"], 1)]))
+            extra.append((orc.HostColumn(orc.FLOAT64, x * x, col.valid), [(AGG["SUM"], 1)]))
+            extra.append((col, [(AGG["COUNT_VALID"], 1)]))
+        else:
+            where.append(None)
+    return list(requests) + extra, where
 
 
-def assert_parity(keys, requests, null_policy=0, gpu=None):
+def assert_parity(keys, requests, null_policy=0, gpu=None, stats=None):
     """Run both sides (unless ``gpu`` results are given) and compare under the
-    parity bar.  Returns the number of groups."""
+    parity bar, after sorting both by key.  Returns the number of groups.
+    ``stats`` (a dict), when given, receives per FP64 SUM / MEAN result the
+    largest relative error and the number of groups whose relative error
+    exceeds 1e-12 (a report; the bound asserted is the normwise one)."""
     g_keys, g_res = gpu if gpu is not None else gpu_groupby(keys, requests, null_policy)
-    o_keys, o_res = oracle_groupby(keys, requests, null_policy)
-    og, oo = sort_order(g_keys), sort_order(o_keys)
+    all_reqs, where = _scale_requests(requests)
+    o_keys, o_res = oracle_groupby(keys, all_reqs, null_policy)
+    exact = len(o_keys[0][0]) < 1_000_000 if o_keys else True
+    og, oo = sort_order(g_keys, exact), sort_order(o_keys, exact)
     assert_keys_equal(g_keys, o_keys, og, oo)
     for r, (col, aggs) in enumerate(requests):
-        scale = None
+        is_float_src = col.type_id in (orc.FLOAT32, orc.FLOAT64)
+        scale_abs = scale_sq = n_valid = None
+        if where[r] is not None:
+            w = where[r]
+            scale_abs, scale_sq, n_valid = (o_res[w][0][0][oo], o_res[w + 1][0][0][oo],
+                                            o_res[w + 2][0][0][oo].astype(np.float64))
         for a, (kind, ddof) in enumerate(aggs):
             gv, gval = g_res[r][a]
             ov, oval = o_res[r][a]
             assert len(gv) == len(ov)
             gvalid = np.ones(len(gv), bool) if gval is None else gval
             gvalid_s, ovalid_s = gvalid[og], oval[oo]
-            assert np.array_equal(gvalid_s, ovalid_s), f"validity differs for request {r} agg {kind}"
+            bad = np.flatnonzero(gvalid_s != ovalid_s)
+            assert bad.size == 0, f"validity differs for request {r} agg {kind} at sorted group {bad[0]}"
             gs, os_ = gv[og][gvalid_s], ov[oo][ovalid_s]
             assert gs.dtype == os_.dtype, f"dtype {gs.dtype} vs {os_.dtype} (agg {kind})"
-            is_float_src = col.type_id in (orc.FLOAT32, orc.FLOAT64)
             if kind in (AGG["COUNT_VALID"], AGG["COUNT_ALL"], AGG["MIN"], AGG["MAX"], AGG["ARGMIN"],
                         AGG["ARGMAX"], AGG["NUNIQUE"], AGG["NTH_ELEMENT"]) or (
                     kind in (AGG["SUM"], AGG["SUM_OF_SQUARES"], AGG["PRODUCT"]) and not is_float_src):
@@ -172,47 +367,41 @@ def assert_parity(keys, requests, null_policy=0, gpu=None):
                 else:
                     assert np.array_equal(gs, os_), f"agg {kind} not exact: {gs[gs != os_][:5]} vs {os_[gs != os_][:5]}"
                 continue
-            if scale is None:
-                scale = group_abs_sums(keys, col, null_policy)[oo][ovalid_s] if is_float_src else None
+            tol = TOL_F32_SUM if col.type_id == orc.FLOAT32 else TOL_F64_SUM
+            g64, o64 = gs.astype(np.float64), os_.astype(np.float64)
+            err = np.abs(g64 - o64)
+            if stats is not None and col.type_id == orc.FLOAT64 and kind in (AGG["SUM"], AGG["MEAN"]):
+                rel = err / np.maximum(np.abs(o64), 1e-300)
+                stats[(r, kind)] = {"groups": int(len(rel)), "max_rel": float(rel.max(initial=0.0)),
+                                    "over_1e-12": int((rel > 1e-12).sum()),
+                                    "max_normwise": float((err / np.maximum(scale_abs[ovalid_s], 1e-300)).max(initial=0.0))}
             if kind == AGG["SUM"]:
-                tol = TOL_F32_SUM if col.type_id == orc.FLOAT32 else TOL_F64_SUM
-                err = np.abs(gs.astype(np.float64) - os_.astype(np.float64))
-                bound = tol * scale + np.finfo(gs.dtype).tiny
+                if is_float_src:
+                    bound = tol * scale_abs[ovalid_s] + np.finfo(gs.dtype).tiny
+                else:
+                    bound = np.zeros_like(err)
                 assert (err <= bound).all(), f"SUM error {err.max()} above bound"
             elif kind == AGG["MEAN"]:
-                tol = TOL_F32_SUM if col.type_id == orc.FLOAT32 else TOL_F64_SUM
-                cnt_valid = None
                 if is_float_src:
                     # mean = sum / n: the sum's normwise bound divided by n
-                    _, cres = orc.groupby(keys, [(col, [(AGG["COUNT_VALID"], 1)])], null_policy)
-                    cnt_valid = cres[0][0][0][oo][ovalid_s].astype(np.float64)
-                    bound = tol * scale / cnt_valid + 1e-300
+                    bound = tol * scale_abs[ovalid_s] / n_valid[ovalid_s] + 1e-300
                 else:
-                    bound = 1e-15 * np.abs(os_) + 1e-300
-                err = np.abs(gs - os_)
+                    bound = 1e-15 * np.abs(o64) + 1e-300
                 assert (err <= bound).all(), f"MEAN error {err.max()} above bound"
             elif kind in (AGG["VARIANCE"], AGG["STD"]):
-                err = np.abs(gs - os_)
-                bound = TOL_VAR * np.abs(os_) + 1e-300
+                bound = TOL_VAR * np.abs(o64) + 1e-300
                 if is_float_src:
                     # absolute floor from the data scale: tol * sum(x^2)/(n-ddof)
-                    sq = group_sq_sums(keys, col, null_policy)[oo][ovalid_s]
-                    _, cres = orc.groupby(keys, [(col, [(AGG["COUNT_VALID"], 1)])], null_policy)
-                    n = cres[0][0][0][oo][ovalid_s].astype(np.float64)
-                    floor = 1e-13 * sq / np.maximum(n - ddof, 1)
+                    floor = 1e-13 * scale_sq[ovalid_s] / np.maximum(n_valid[ovalid_s] - ddof, 1)
                     if kind == AGG["STD"]:
                         floor = np.sqrt(floor)
                     bound = bound + floor
                 assert (err <= bound).all(), f"VAR/STD error {err.max()} above bound"
             elif kind == AGG["PRODUCT"]:
                 # unordered products of at most ~80 factors: |error| <= 80 * eps * |product|
-                tol = TOL_F32_SUM if col.type_id == orc.FLOAT32 else TOL_F64_SUM
-                err = np.abs(gs.astype(np.float64) - os_.astype(np.float64))
-                assert (err <= tol * np.abs(os_.astype(np.float64)) + 1e-300).all(), f"PRODUCT error {err.max()}"
+                assert (err <= tol * np.abs(o64) + 1e-300).all(), f"PRODUCT error {err.max()}"
             elif kind == AGG["SUM_OF_SQUARES"]:
-                tol = TOL_F32_SUM if col.type_id == orc.FLOAT32 else TOL_F64_SUM
-                err = np.abs(gs.astype(np.float64) - os_.astype(np.float64))
-                assert (err <= tol * np.abs(os_.astype(np.float64)) + 1e-300).all()
+                assert (err <= tol * np.abs(o64) + 1e-300).all()
             else:
                 raise AssertionError(f"unhandled kind {kind}")
     return len(og)

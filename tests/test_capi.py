@@ -90,3 +90,46 @@ def test_unsupported_kind_and_type_rejected():
     assert call(3, 13) == _lib.MIG_ERROR_LOGIC          # MEDIAN: not a hash aggregation
     assert call(23, 0) == _lib.MIG_ERROR_DATA_TYPE      # SUM of strings
     assert call(12, 9) == _lib.MIG_ERROR_DATA_TYPE      # MEAN of timestamps
+
+
+def test_negative_offset_or_size_rejected_before_device_work():
+    """A view with a negative row offset or size is a logic error raised on
+    the host (a wrapped int32 offset would otherwise read before the buffer)."""
+    from cudf_amd import _lib
+    lib = _lib.lib()
+    aggs = (_lib.Aggregation * 1)(_lib.Aggregation(0, 1))
+
+    def call(key_offset, val_offset, size=10):
+        keys = (_lib.ColumnView * 1)()
+        keys[0].type_id = 3
+        keys[0].size = size
+        keys[0].offset = key_offset
+        reqs = (_lib.AggregationRequest * 1)()
+        reqs[0].values.type_id = 3
+        reqs[0].values.size = size
+        reqs[0].values.offset = val_offset
+        reqs[0].aggregations = ctypes.cast(aggs, ctypes.POINTER(_lib.Aggregation))
+        reqs[0].num_aggregations = 1
+        h = ctypes.c_void_p()
+        return lib.mig_groupby_aggregate(keys, 1, 0, reqs, 1, None, None, ctypes.byref(h))
+
+    assert call(-5, 0) == _lib.MIG_ERROR_LOGIC
+    assert b"negative" in lib.mig_last_error()
+    assert call(0, -(2**31)) == _lib.MIG_ERROR_LOGIC
+    assert call(0, 0, size=-3) == _lib.MIG_ERROR_LOGIC
+
+
+def test_column_view_rejects_offsets_beyond_int32():
+    """cudf size_type is int32: a Column whose row offset or size does not
+    fit raises instead of wrapping in the ctypes view."""
+    import torch
+    from cudf_amd import Column, TypeId
+    buf = torch.zeros(16, dtype=torch.uint8)
+    with pytest.raises(OverflowError):
+        Column(TypeId.INT32, 2, buf, None, 0, 2**31).view()
+    with pytest.raises(OverflowError):
+        Column(TypeId.INT32, 2**31, buf).view()
+    with pytest.raises(OverflowError):
+        Column(TypeId.INT32, 2, buf, None, 0, -1).view()
+    v = Column(TypeId.INT32, 2, buf, None, 0, 2**31 - 1).view()
+    assert v.offset == 2**31 - 1
