@@ -314,3 +314,33 @@ def test_nth_element_errors():
     s = orc.HostColumn.from_strings([b"x%d" % i for i in range(100)])
     with pytest.raises(_lib.LogicError):
         gpu_groupby([host_col(k)], [(s, [(AGG["NTH_ELEMENT"], 1)])], keys_sorted=True)
+
+
+@pytest.mark.parametrize("policy", [0, 1])
+def test_sorted_count_valid_of_nullable_strings(policy):
+    """COUNT_VALID of nullable string values on pre-sorted keys reads only the
+    strings' validity: short strings (fewer chars than 8 per row), a column
+    whose strings are all empty, and one whose chars buffer is absent, beside
+    a fixed-width value column in the same pass; against the oracle."""
+    from cudf_amd import Column, GroupBy, GroupByRequest, NullPolicy, Sorted, Table, TypeId
+    from cudf_amd.aggregation import Aggregation
+    from tests.gbutil import host_keys, to_device
+    rng = np.random.default_rng(91)
+    n = 40_000
+    k = np.sort(rng.integers(0, 3000, n).astype(np.int64))
+    kvalid = np.ones(n, bool)
+    kvalid[-500:] = False
+    short = orc.HostColumn.from_strings([b"x" * int(i) for i in rng.integers(0, 3, n)], rng.random(n) > 0.2)
+    empty = orc.HostColumn.from_strings([b""] * n, rng.random(n) > 0.3)
+    v = host_col(rng.integers(-100, 100, n).astype(np.int32), rng.random(n) > 0.1)
+    cv = [(AGG["COUNT_VALID"], 1)]
+    requests = [(short, cv), (empty, cv + [(AGG["COUNT_ALL"], 1)]), (v, [(AGG["SUM"], 1), (AGG["COUNT_VALID"], 1)])]
+    _check_sorted([host_col(k, kvalid)], requests, policy)
+    # the all-empty column again with no chars buffer at all (a null data pointer)
+    de = to_device(empty)
+    nochars = Column(TypeId.STRING, n, None, de.mask, de.null_count, offsets=de.offsets)
+    gb = GroupBy(Table([to_device(host_col(k, kvalid))]), NullPolicy(policy), Sorted.YES)
+    out_keys, out_res = gb.aggregate([GroupByRequest(nochars, [Aggregation(AGG["COUNT_VALID"], 1)])])
+    g_keys = [host_keys(c) for c in out_keys.columns()]
+    g_res = [[c.to_numpy() for c in t.columns()] for t in out_res]
+    assert_parity([host_col(k, kvalid)], [(empty, cv)], policy, gpu=(g_keys, g_res))
