@@ -7,11 +7,14 @@ U[0,1e4] value columns, each 5% null; sum, mean, count, min, max, var
 HBM; a step is one `GroupBy(keys).aggregate(requests)` call including the
 materialised output columns.
 
-Prints one JSON line.  `roofline` is for the dominant kernel (the accumulate
-pass), timed with hipEvents the library records on the stream it launches
-on; `pipeline` gives the same ratio for the whole step.  `cpu_baseline` times
-the CPU oracle on a bounded sample; `pandas_baseline` times pandas (the
-cudf.pandas CPU fallback) on the same sample.
+Prints one JSON line.  Every timed step starts with cold caches (a 512 MB
+write between steps, outside the timing).  `roofline` divides the
+operation's algorithmic bytes (each input buffer read once, each output
+buffer written once) by the dominant kernel's time, measured with hipEvents
+the library records on the stream it launches on; `pipeline` gives the same
+ratio for the whole step.  `cpu_baseline` times pandas group-by (the
+cudf.pandas CPU fallback) on a bounded sample of the same distribution,
+median of 3; `oracle_baseline` times the C oracle on that sample.
 """
 from __future__ import annotations
 
@@ -77,19 +80,16 @@ def output_bytes(key_t, res):
     return total
 
 
-def cpu_baselines(sample_rows, card, seed=1):
-    """Time the CPU oracle (one thread) and pandas on a bounded sample of the
-    same distribution."""
+def cpu_baselines(sample_rows, card, seed=1, reps=3):
+    """The CPU baselines on a bounded sample of the same distribution (same
+    generator, same aggregations): pandas group-by -- the CPU fallback the
+    reference's cudf.pandas layer runs -- with nullable Int64 / Float64 /
+    Int32 columns, one warm-up and the median of `reps` runs; and the C
+    oracle (one thread, one run) as a secondary figure."""
     from oracle import oracle as orc
     from tests.gbutil import C2_AGGS, c2_data
     key, f64, i32 = c2_data(n=sample_rows, card=card, seed=seed)
-    t0 = time.perf_counter()
-    orc.groupby([key], [(f64, C2_AGGS), (i32, C2_AGGS)], 0)
-    t_oracle = time.perf_counter() - t0
-    out = {"cpu_baseline": {"value": sample_rows / t_oracle, "unit": "rows/s", "cores": 1, "kind": "port",
-                            "sample": f"{sample_rows} rows, {card} distinct keys, same distribution and "
-                                      f"aggregations, C oracle (oracle/groupby_oracle.c), 1 thread, "
-                                      f"{t_oracle:.2f} s"}}
+    out = {}
     try:
         import pandas as pd
         df = pd.DataFrame({"k": pd.arrays.IntegerArray(key.data, ~key.valid),
@@ -97,15 +97,29 @@ def cpu_baselines(sample_rows, card, seed=1):
                            "i": pd.arrays.IntegerArray(i32.data, ~i32.valid)})
         aggs = ["sum", "mean", "count", "min", "max", "var"]
         df.groupby("k", sort=False, dropna=True).agg({"f": aggs, "i": aggs})  # warm-up
-        t0 = time.perf_counter()
-        df.groupby("k", sort=False, dropna=True).agg({"f": aggs, "i": aggs})
-        t_pd = time.perf_counter() - t0
-        out["pandas_baseline"] = {"value": sample_rows / t_pd, "unit": "rows/s", "cores": 1,
-                                  "version": pd.__version__, "cpu_count": os.cpu_count(),
-                                  "sample": f"{sample_rows} rows, nullable Int64/Float64/Int32, "
-                                            f"groupby(sort=False, dropna=True).agg, {t_pd:.2f} s"}
+        times = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            df.groupby("k", sort=False, dropna=True).agg({"f": aggs, "i": aggs})
+            times.append(time.perf_counter() - t0)
+        t_pd = float(np.median(times))
+        out["cpu_baseline"] = {"value": sample_rows / t_pd, "unit": "rows/s", "cores": 1, "kind": "reference",
+                               "library": f"pandas {pd.__version__} (the cudf.pandas CPU fallback; "
+                                          f"group-by is single-threaded)",
+                               "cpu_count": os.cpu_count(),
+                               "sample": f"{sample_rows} rows, {card} distinct keys, C2's distributions and "
+                                         f"aggregations, nullable Int64/Float64/Int32, groupby(sort=False, "
+                                         f"dropna=True).agg; median of {reps} runs after one warm-up: "
+                                         f"{t_pd:.2f} s"}
+        del df
     except ImportError:
         pass
+    t0 = time.perf_counter()
+    orc.groupby([key], [(f64, C2_AGGS), (i32, C2_AGGS)], 0)
+    t_oracle = time.perf_counter() - t0
+    out["oracle_baseline"] = {"value": sample_rows / t_oracle, "unit": "rows/s", "cores": 1, "kind": "port",
+                              "sample": f"the same {sample_rows} rows, C oracle (oracle/groupby_oracle.c), "
+                                        f"1 thread, one run: {t_oracle:.2f} s"}
     return out
 
 
@@ -465,6 +479,43 @@ def bench_sorted(n=100_000_000, card=10_000_000, reps=5):
     return out
 
 
+def result_digest(query, out):
+    """Order-independent digest of a result's exact columns: per column, the
+    wrapping sum over groups of a mix of the group's key row hash
+    (MurmurHash3 of the key columns) with the element's bits and validity.
+    Exact columns are COUNT, MIN, MAX and integer SUM; floating-point sums,
+    means and variances are left out (their rounding depends on the order of
+    accumulation and is checked against the oracle by the tests)."""
+    import torch
+    from cudf_amd import murmurhash3_x86_32
+    from cudf_amd.aggregation import Kind
+    from cudf_amd.types import numpy_dtype
+    keys_t, res = out
+    n = keys_t.num_rows()
+    rh = murmurhash3_x86_32(keys_t).data[:4 * n].view(torch.int32).to(torch.int64) & 0xFFFFFFFF
+    digests = []
+    for req, t in zip(query.requests, res):
+        for agg, c in zip(req.aggregations, t.columns()):
+            dt = np.dtype(numpy_dtype(c.type_id))
+            if not (agg.kind() in (Kind.COUNT_VALID, Kind.COUNT_ALL, Kind.MIN, Kind.MAX) or
+                    (agg.kind() == Kind.SUM and dt.kind in "iu")):
+                continue
+            raw = c.data[:dt.itemsize * n]
+            bits = (raw.view(torch.int64) if dt.itemsize == 8 else
+                    raw.view(torch.int32).to(torch.int64) if dt.itemsize == 4 else
+                    raw.view(torch.int16).to(torch.int64) if dt.itemsize == 2 else raw.to(torch.int64))
+            valid = torch.ones(n, dtype=torch.int64, device=bits.device)
+            if c.mask is not None:
+                words = c.mask.view(torch.int32).to(torch.int64) & 0xFFFFFFFF
+                idx = torch.arange(n, device=bits.device)
+                valid = (words[idx >> 5] >> (idx & 31)) & 1
+            bits = bits * valid  # a null's bits are not part of the result
+            # odd 64-bit multipliers as signed int64 (torch wraps on overflow)
+            h = (rh * (0x9E3779B97F4A7C15 - (1 << 64)) + bits) * (0xBF58476D1CE4E5B9 - (1 << 64)) + valid
+            digests.append(int(h.sum().item()))
+    return (n, tuple(digests))
+
+
 def bench_served(n_req, loads=(0.5, 1.0, 2.0, 4.0), workers=4):
     """Plain kernel run serially over the stream, then the continuous-batching
     server at Poisson arrival rates that are multiples of the serial rate and
@@ -484,10 +535,11 @@ def bench_served(n_req, loads=(0.5, 1.0, 2.0, 4.0), workers=4):
     for f in factories:
         q = f()
         t0 = time.perf_counter()
-        k, _ = GroupBy(q.keys).aggregate(q.requests)
+        r = GroupBy(q.keys).aggregate(q.requests)
         torch.cuda.synchronize()
         t_serial.append(time.perf_counter() - t0)
-        groups_serial.append(k.num_rows())
+        groups_serial.append(result_digest(q, r))
+        del r
     serial_qps = n_req / sum(t_serial)
     lat_serial = np.array(t_serial) * 1e3
     out = {"workload": f"{n_req} requests from the three templates (1M-row int32 key/int64 value sum,count,"
@@ -523,7 +575,7 @@ def bench_served(n_req, loads=(0.5, 1.0, 2.0, 4.0), workers=4):
                 srv.submit(q)
         res = [q.future.result(timeout=600) for q in queries]
         srv.stop()
-        mismatches += sum(int(r[0].num_rows() != g) for r, g in zip(res, groups_serial))
+        mismatches += sum(int(result_digest(q, r) != g) for q, r, g in zip(queries, res, groups_serial))
         st = latency_stats(queries)
         st["offered_qps"] = None if load == "burst" else serial_qps * load
         st["load"] = load
@@ -534,7 +586,9 @@ def bench_served(n_req, loads=(0.5, 1.0, 2.0, 4.0), workers=4):
     best = max(s["queries_per_s"] for s in out["sweep"])
     out["max_queries_per_s"] = best
     out["speedup_vs_plain_serial"] = best / serial_qps
-    out["group_count_mismatches_vs_serial"] = mismatches
+    out["exact_result_mismatches_vs_serial"] = mismatches
+    out["exact_check"] = ("per query: group count and an order-independent digest of every exact column "
+                          "(COUNT, MIN, MAX, integer SUM) with the group's key hash, served vs serial")
     return out
 
 
@@ -545,7 +599,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--rows", type=int, default=N_ROWS)
     ap.add_argument("--card", type=int, default=CARD)
-    ap.add_argument("--cpu-sample", type=int, default=4_000_000)
+    ap.add_argument("--cpu-sample", type=int, default=20_000_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--strategy", choices=["global", "partitioned"], default=None,
                     help="force an aggregation strategy (default: chosen per call)")
@@ -616,20 +670,30 @@ def main():
         out = gb.aggregate(reqs)
         del out
     torch.cuda.synchronize()
-    # timed steps: hipEvents bracket only the main aggregation kernel (its
-    # duration for the roofline); the other kernels' times come from two
-    # untimed steps with every phase timed
+    # timed steps: each step starts with cold caches (a 512 MB buffer, twice
+    # the 256 MB Infinity Cache, is written before it, outside its timing)
+    # and is bracketed by device synchronisations; hipEvents the library
+    # records on its stream bracket the main aggregation kernel (its duration
+    # for the roofline).  The other kernels' times come from two untimed
+    # steps with every phase timed.
+    flush = torch.empty(512 << 20, dtype=torch.uint8, device="cuda")
     _lib.set_phase_timing(2)
     dom_ms = {}
-    t0 = time.perf_counter()
+    step_ms = []
     for _ in range(args.steps):
+        flush.fill_(1)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
         keys_t, res = gb.aggregate(reqs)
+        torch.cuda.synchronize()
+        step_ms.append((time.perf_counter() - t0) * 1e3)
         st = _lib.last_stats()
         for k in ("t_rpa_aggregate_ms", "t_accumulate_ms"):
             dom_ms[k] = dom_ms.get(k, 0.0) + st[k] / args.steps
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    ms_per_step = (t1 - t0) * 1e3 / args.steps
+        if _ < args.steps - 1:
+            del keys_t, res
+    del flush
+    ms_per_step = float(np.mean(step_ms))
     _lib.set_phase_timing(True)
     phase = {}
     for _ in range(2):
@@ -657,13 +721,18 @@ def main():
         kernels = {"k_aggregate_global": (phase.get("t_accumulate_ms", 0), in_b)}
     dom = max(kernels, key=lambda k: kernels[k][0])
     t_dom = kernels[dom][0] * 1e-3
-    dom_bytes = kernels[dom][1]
-    achieved = dom_bytes / t_dom if t_dom > 0 else 0.0
-    traffic = None
+    # the roofline's bytes are the operation's algorithmic bytes (every input
+    # buffer read once, every output buffer written once: 29.3 B/row here)
+    # over the dominant kernel's time; the bytes that kernel must move by
+    # its design are reported beside them
+    bytes_alg = in_b + out_b
+    achieved = bytes_alg / t_dom if t_dom > 0 else 0.0
+    traffic, traffic_src = None, None
     tf = os.path.join(ROOT, "profiles", f"traffic_{dom}_c2.json")
     if os.path.exists(tf):
         with open(tf) as f:
             traffic = json.load(f).get("bytes_per_launch")
+        traffic_src = f"profiles/traffic_{dom}_c2.json (rocprofv3 FETCH_SIZE + WRITE_SIZE, not measured in this run)"
     line = {
         "metric": METRIC,
         "value": args.rows / (ms_per_step * 1e-3),
@@ -672,6 +741,7 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": ms_per_step,
+        "ms_per_step_median": float(np.median(step_ms)),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -684,8 +754,9 @@ def main():
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": dom,
                      "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK, "traffic": traffic,
-                     "algorithmic_bytes": dom_bytes, "avg_ms": t_dom * 1e3},
+                     "frac": achieved / HBM_PEAK, "traffic": traffic, "traffic_source": traffic_src,
+                     "algorithmic_bytes": bytes_alg, "kernel_design_bytes": kernels[dom][1],
+                     "avg_ms": t_dom * 1e3},
         "pipeline": {"strategy": {1: "global", 2: "partitioned"}.get(int(st["strategy"]), "?"),
                      "bytes_alg": in_b + out_b, "bytes_per_row": (in_b + out_b) / args.rows,
                      "achieved_GBps": (in_b + out_b) / (ms_per_step * 1e-3) / 1e9,

@@ -58,7 +58,8 @@ class ColumnView(ctypes.Structure):
 
 class Aggregation(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_int32), ("ddof", ctypes.c_int32), ("null_handling", ctypes.c_int32),
-                ("n", ctypes.c_int32)]
+                ("n", ctypes.c_int32), ("interpolation", ctypes.c_int32), ("num_quantiles", ctypes.c_int32),
+                ("quantiles", ctypes.POINTER(ctypes.c_double))]
 
 
 class AggregationRequest(ctypes.Structure):
@@ -170,7 +171,7 @@ SIGNATURES = {
 STAT_NAMES = ["strategy", "key_mode", "capacity", "est_groups", "groups", "retries", "state_bytes",
               "t_estimate_ms", "t_init_ms", "t_accumulate_ms", "t_second_pass_ms", "t_compact_ms",
               "t_outputs_ms", "t_rpa_hist_ms", "t_rpa_scatter1_ms", "t_rpa_scatter2_ms",
-              "t_rpa_aggregate_ms", "lds_preagg"]
+              "t_rpa_aggregate_ms", "lds_preagg", "rpa_speculative"]
 
 STRATEGY_AUTO, STRATEGY_GLOBAL, STRATEGY_PARTITIONED = 0, 1, 2
 

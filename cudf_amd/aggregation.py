@@ -30,12 +30,36 @@ class Kind(enum.IntEnum):
     NTH_ELEMENT = 18
 
 
+class Interpolation(enum.IntEnum):
+    """cudf::interpolation (cudf/types.hpp:192-198)."""
+    LINEAR = 0
+    LOWER = 1
+    HIGHER = 2
+    MIDPOINT = 3
+    NEAREST = 4
+
+
 class Aggregation:
-    def __init__(self, kind: Kind, ddof: int = 1, null_handling: NullPolicy = NullPolicy.EXCLUDE, n: int = 0):
+    def __init__(self, kind: Kind, ddof: int = 1, null_handling: NullPolicy = NullPolicy.EXCLUDE, n: int = 0,
+                 quantiles=(), interpolation: Interpolation = Interpolation.LINEAR):
         self._kind = Kind(kind)
         self.ddof = int(ddof)
         self.null_handling = NullPolicy(null_handling)  # NUNIQUE, NTH_ELEMENT
         self.n = int(n)  # NTH_ELEMENT
+        self.quantiles = tuple(float(q) for q in quantiles)  # QUANTILE
+        self.interpolation = Interpolation(interpolation)
+
+    def to_c(self, keep: list):
+        """The C ABI struct; `keep` holds buffers that must outlive the call."""
+        from . import _lib
+        import ctypes
+        a = _lib.Aggregation(int(self._kind), self.ddof, int(self.null_handling), self.n, int(self.interpolation),
+                             len(self.quantiles))
+        if self.quantiles:
+            qs = (ctypes.c_double * len(self.quantiles))(*self.quantiles)
+            keep.append(qs)
+            a.quantiles = ctypes.cast(qs, ctypes.POINTER(ctypes.c_double))
+        return a
 
     def kind(self) -> Kind:
         return self._kind
@@ -84,15 +108,28 @@ def std(ddof: int = 1) -> Aggregation:
 
 def nunique(null_handling: NullPolicy = NullPolicy.EXCLUDE) -> Aggregation:
     """Distinct values per group; INCLUDE counts nulls as one value
-    (aggregation.hpp make_nunique_aggregation).  Needs keys_are_sorted=YES."""
+    (aggregation.hpp make_nunique_aggregation).  A sort-based aggregation."""
     return Aggregation(Kind.NUNIQUE, null_handling=null_handling)
 
 
 def nth_element(n: int, null_handling: NullPolicy = NullPolicy.INCLUDE) -> Aggregation:
     """The n-th value of each group in row order (from the end when n < 0);
     EXCLUDE counts valid values only (aggregation.hpp
-    make_nth_element_aggregation).  Needs keys_are_sorted=YES."""
+    make_nth_element_aggregation).  A sort-based aggregation."""
     return Aggregation(Kind.NTH_ELEMENT, null_handling=null_handling, n=n)
+
+
+def median() -> Aggregation:
+    """The 0.5 quantile with linear interpolation, as double (aggregation.hpp
+    make_median_aggregation)."""
+    return Aggregation(Kind.MEDIAN)
+
+
+def quantile(quantiles, interp: Interpolation = Interpolation.LINEAR) -> Aggregation:
+    """Quantiles of each group's valid values, as double; the result holds
+    num_groups * len(quantiles) rows, group-major (aggregation.hpp
+    make_quantile_aggregation)."""
+    return Aggregation(Kind.QUANTILE, quantiles=quantiles, interpolation=interp)
 
 
 def argmax() -> Aggregation:

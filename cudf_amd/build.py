@@ -31,7 +31,8 @@ COMMON_FLAGS = [
 ]
 
 SOURCES = ["groupby_kernels.hip", "partition_kernels.hip", "partition_aggregate.hip", "batch_kernels.hip",
-           "partitioning_kernels.hip", "groupby.cpp", "batch.cpp", "partitioning.cpp", "capi.cpp"]
+           "partitioning_kernels.hip", "groupby.cpp", "batch.cpp", "partitioning.cpp", "capi.cpp",
+           "sort_kernels.hip", "sort_groupby.cpp"]
 
 
 def _compile(src: str) -> str:

@@ -248,6 +248,7 @@ pinned_host& thread_pinned()
 // Planning helpers shared with the batched driver (declared in host_plan.hpp).
 
 void* scratch_take(int dev, size_t bytes) { return scratch_cache::get().take(dev, bytes); }
+int num_cus_of_device() { return props().num_cus; }
 void scratch_give(int dev, void* p, size_t bytes) { scratch_cache::get().give(dev, p, bytes); }
 
 bool same_view(mig_column_view const& a, mig_column_view const& b)
@@ -314,6 +315,8 @@ char const* kind_name(int32_t k)
     case MIG_AGG_ARGMAX: return "ARGMAX";
     case MIG_AGG_NUNIQUE: return "NUNIQUE";
     case MIG_AGG_NTH_ELEMENT: return "NTH_ELEMENT";
+    case MIG_AGG_MEDIAN: return "MEDIAN";
+    case MIG_AGG_QUANTILE: return "QUANTILE";
     default: return "UNKNOWN";
   }
 }
@@ -332,7 +335,9 @@ int32_t target_type_of(int32_t src, int32_t kind)
     case MIG_AGG_NUNIQUE: return MIG_TYPE_INT32;  // size_type count (group_nunique.cu:90-91)
     case MIG_AGG_MEAN:
     case MIG_AGG_VARIANCE:
-    case MIG_AGG_STD: return MIG_TYPE_FLOAT64;
+    case MIG_AGG_STD:
+    case MIG_AGG_MEDIAN:    // always double (aggregation.hpp:1334-1344)
+    case MIG_AGG_QUANTILE: return MIG_TYPE_FLOAT64;
     case MIG_AGG_SUM:
     case MIG_AGG_SUM_OF_SQUARES:
     case MIG_AGG_PRODUCT:  // aggregation.hpp:1276-1306
@@ -366,6 +371,9 @@ void check_supported(int32_t src, int32_t kind)
         fail(MIG_ERROR_LOGIC, std::string(kind_name(kind)) + " of string values is not supported");
       ok = is_numeric_type(src) || is_chrono_type(src);
       break;
+    // arithmetic values only (group_quantiles.cu quantiles_functor)
+    case MIG_AGG_MEDIAN:
+    case MIG_AGG_QUANTILE: ok = is_numeric_type(src); break;
     default:
       fail(MIG_ERROR_LOGIC, std::string("aggregation ") + kind_name(kind) +
                               " is not supported by the hash group-by");
@@ -552,6 +560,7 @@ std::unique_ptr<mig_groupby_result> groupby_aggregate(mig_column_view const* key
       fail(MIG_ERROR_DATA_TYPE, "unsupported key column type " + std::to_string(keys[i].type_id));
   }
   // groupby.cu:210-214 and :175-192
+  bool needs_sort = false;
   for (int r = 0; r < num_requests; r++) {
     MIG_EXPECTS(requests[r].values.size == N,
                 "Size mismatch between request values and groupby keys.");
@@ -559,11 +568,19 @@ std::unique_ptr<mig_groupby_result> groupby_aggregate(mig_column_view const* key
     MIG_EXPECTS(requests[r].num_aggregations >= 0, "negative number of aggregations");
     for (int a = 0; a < requests[r].num_aggregations; a++) {
       check_supported(requests[r].values.type_id, requests[r].aggregations[a].kind);
-      // a sort-based aggregation (groupby.cu:78-83): taken on runs of pre-sorted keys only
+      // a sort-based aggregation (groupby.cu:78-83): NUNIQUE and NTH_ELEMENT
+      // run on the runs of pre-sorted keys, or else (and MEDIAN / QUANTILE
+      // always) on the sort-based group-by
       int32_t const k = requests[r].aggregations[a].kind;
-      if ((k == MIG_AGG_NUNIQUE || k == MIG_AGG_NTH_ELEMENT) && !keys_sorted)
-        fail(MIG_ERROR_LOGIC, std::string(kind_name(k)) +
-                                " needs keys_are_sorted == YES (the hash group-by does not take it)");
+      if (k == MIG_AGG_QUANTILE) {
+        mig_aggregation const& ag = requests[r].aggregations[a];
+        MIG_EXPECTS(ag.num_quantiles >= 0 && (ag.num_quantiles == 0 || ag.quantiles != nullptr),
+                    "invalid quantile list");
+        if (ag.interpolation < MIG_INTERP_LINEAR || ag.interpolation > MIG_INTERP_NEAREST)
+          fail(MIG_ERROR_LOGIC, "Invalid interpolation operation for quantiles.");
+      }
+      needs_sort |= k == MIG_AGG_MEDIAN || k == MIG_AGG_QUANTILE ||
+                    ((k == MIG_AGG_NUNIQUE || k == MIG_AGG_NTH_ELEMENT) && !keys_sorted);
     }
   }
 
@@ -587,6 +604,9 @@ std::unique_ptr<mig_groupby_result> groupby_aggregate(mig_column_view const* key
     }
     MIG_HIP_TRY(hipStreamSynchronize(stream));
     return res;
+  }
+  if (needs_sort) {
+    return sort_groupby(keys, num_keys, null_policy, requests, num_requests, stream, mr, keys_sorted);
   }
 
   device_props const& dp = props();
@@ -737,21 +757,24 @@ std::unique_ptr<mig_groupby_result> groupby_aggregate(mig_column_view const* key
   T.mark(0);
   uint64_t const max_cap = next_pow2(uint64_t(2 * N + 2));
   double est = double(N);
+  double pop = double(N);       // rows the group-by keeps (estimated)
+  uint32_t max_mult = 0;        // most sampled rows sharing one key hash
   if (N > 4096 && !keys_sorted) {
     int64_t const ns = std::min<int64_t>(N, SAMPLE_ROWS);
     uint32_t const tslots = uint32_t(next_pow2(uint64_t(2 * ns)));
     auto* table    = S.get<uint64_t>(tslots);
-    auto* counters = S.get<uint32_t>(2);
+    auto* counters = S.get<uint32_t>(3);
     MIG_HIP_TRY(hipMemsetAsync(table, 0, sizeof(uint64_t) * tslots, stream));
-    MIG_HIP_TRY(hipMemsetAsync(counters, 0, sizeof(uint32_t) * 2, stream));
+    MIG_HIP_TRY(hipMemsetAsync(counters, 0, sizeof(uint32_t) * 3, stream));
     upload(0);
     launch_sample_hash(dP, N, ns, table, tslots - 1, counters, stream);
     auto* h = reinterpret_cast<uint32_t*>(pin + 2 * sizeof(gb_params));
-    MIG_HIP_TRY(hipMemcpyAsync(h, counters, 8, hipMemcpyDeviceToHost, stream));
+    MIG_HIP_TRY(hipMemcpyAsync(h, counters, 12, hipMemcpyDeviceToHost, stream));
     MIG_HIP_TRY(hipStreamSynchronize(stream));
     double const sampled = double(h[0]), d = double(h[1]);
+    max_mult = h[2];
     // population: rows that are not dropped, scaled from the sample
-    double const pop = double(N) * (sampled / double(ns));
+    pop = double(N) * (sampled / double(ns));
     est = ns == N ? d : (sampled > 0 ? estimate_distinct(sampled, d, std::max(pop, d)) : 1.0);
   }
   stats[STAT_EST_GROUPS] = est;
@@ -1109,58 +1132,81 @@ std::unique_ptr<mig_groupby_result> groupby_aggregate(mig_column_view const* key
     double const per_bin = 0.66 * double(S_slots);
     int log_bins = 1;
     while (log_bins < RPA_MAX_LOG_BINS && double(int64_t(1) << log_bins) * per_bin < est) log_bins++;
-    for (int attempt = 0; attempt < 3 && G < 0; attempt++) {
+    // Bins are laid out either exactly, from a histogram pass over the keys,
+    // or speculatively, at fixed strides sized from the sampled estimate so
+    // that no histogram pass is needed.  Rows of one key all land in one
+    // bin, so a key heavy enough to fill a bin defeats the speculative
+    // layout: it is used only when no key repeats 8 times in the sample
+    // (then the heaviest key holds under ~2e-4 of the rows with high
+    // probability); a bin that still fills up is flagged and the call is
+    // redone with the exact layout.
+    bool speculative = max_mult < 8 && pop >= double(1 << 20) && est >= 65536;
+    int bin_attempts = 0;
+    while (G < 0 && bin_attempts < 3) {
       R.log_bins  = log_bins;
       R.two_level = log_bins > RPA_MAX_LOG_FANOUT;
       R.log_f1    = R.two_level ? (log_bins + 1) / 2 : log_bins;
       int64_t const B  = int64_t(1) << log_bins;
       int64_t const F1 = int64_t(1) << R.log_f1;
-      R.hist        = S.get<uint32_t>(B);
-      R.hist_partial = S.get<uint32_t>(size_t(R.hist_blocks) * B);
+      int64_t const HUGE_CAP = int64_t(1) << 62;
+      R.cap1 = R.cap2 = HUGE_CAP;
+      int64_t rowsA = N, rowsB = N;  // records each partition buffer holds
+      if (speculative) {
+        // rows per bin: mean mu, variance ~ mu * (1 + rows per group)
+        double const m  = pop / std::max(est, 1.0);
+        double const mu2 = pop / double(B), mu1 = pop / double(F1);
+        R.cap2 = int64_t(std::ceil(1.2 * mu2 + 6.0 * std::sqrt(mu2 * (1.0 + m)) + 256.0));
+        R.cap1 = R.two_level ? int64_t(std::ceil(1.1 * mu1 + 6.0 * std::sqrt(mu1 * (1.0 + m)) + 4096.0)) : R.cap2;
+        rowsB = B * R.cap2;
+        rowsA = R.two_level ? F1 * R.cap1 : 0;
+      }
+      R.hist        = speculative ? nullptr : S.get<uint32_t>(B);
+      R.hist_partial = speculative ? nullptr : S.get<uint32_t>(size_t(R.hist_blocks) * B);
       R.bin_start   = S.get<int64_t>(B + 1);
       R.cursor1     = S.get<unsigned long long>(F1);
       R.cursor2     = S.get<unsigned long long>(B);
       R.tile_start1 = S.get<int64_t>(F1 + 1);
+      R.first_start = S.get<int64_t>(F1);
       int32_t* d_small = S.get<int32_t>(4);
       R.overflow      = d_small;
+      R.cap_overflow  = d_small + 1;
       R.group_counter = reinterpret_cast<unsigned long long*>(S.get<uint64_t>(1));
       rpa_params* dR  = S.get<rpa_params>(1);
-      MIG_HIP_TRY(hipMemsetAsync(R.hist, 0, sizeof(uint32_t) * B, stream));
+      if (!speculative) MIG_HIP_TRY(hipMemsetAsync(R.hist, 0, sizeof(uint32_t) * B, stream));
       MIG_HIP_TRY(hipMemsetAsync(d_small, 0, sizeof(int32_t) * 4, stream));
       MIG_HIP_TRY(hipMemsetAsync(R.group_counter, 0, sizeof(uint64_t), stream));
       upload(0);
       MIG_HIP_TRY(hipMemcpyAsync(dR, &R, sizeof(rpa_params), hipMemcpyHostToDevice, stream));
       T.mark(8);
-      launch_rpa_hist(dP, dR, R, N, dp.num_cus, stream);
+      if (!speculative) launch_rpa_hist(dP, dR, R, N, dp.num_cus, stream);
       T.mark(9);
-      // The output columns are allocated while the histogram kernel runs: a
-      // bin yields at most min(its rows, max_fill + 2) groups, so
+      // The output columns are allocated while the first kernels run: a bin
+      // yields at most min(its rows, max_fill + 2) groups, so
       // B·(max_fill + 2) rows, and at most N, bound the groups before the bin
       // sizes are known.
       int64_t const rows_cap = std::min<int64_t>(N, B * int64_t(max_fill + 2));
       alloc_outputs(rows_cap, P.key_mode == KM_GENERAL ? rows_cap : 0, true);
-      // bin layout (starts, cursors, second-pass tiles) computed on the device
-      // from the histogram; the partition buffers are sized for every row
-      launch_rpa_layout(dR, stream);
-      int64_t const nrows = N;
+      // bin layout (starts, cursors) computed on the device
+      if (speculative) launch_rpa_spec_layout(dR, R, stream);
+      else launch_rpa_layout(dR, stream);
       // partition buffers
       size_t part_bytes = 0;
-      R.keysB  = S.get<uint64_t>(nrows);
-      R.flagsB = S.get<uint8_t>(nrows);
-      part_bytes += 9 * size_t(nrows);
+      R.keysB  = S.get<uint64_t>(rowsB);
+      R.flagsB = S.get<uint8_t>(rowsB);
+      part_bytes += 9 * size_t(rowsB);
       if (R.two_level) {
-        R.keysA  = S.get<uint64_t>(nrows);
-        R.flagsA = S.get<uint8_t>(nrows);
-        part_bytes += 9 * size_t(nrows);
+        R.keysA  = S.get<uint64_t>(rowsA);
+        R.flagsA = S.get<uint8_t>(rowsA);
+        part_bytes += 9 * size_t(rowsA);
       }
       for (int v = 0; v < P.num_vals; v++) {
         if (P.val_field_begin[v] == P.val_field_begin[v + 1]) continue;
         size_t w  = size_t(storage_width(vals[v].storage));
-        R.valsB[v] = S.get<uint8_t>(w * nrows);
-        part_bytes += w * nrows;
+        R.valsB[v] = S.get<uint8_t>(w * rowsB);
+        part_bytes += w * rowsB;
         if (R.two_level) {
-          R.valsA[v] = S.get<uint8_t>(w * nrows);
-          part_bytes += w * nrows;
+          R.valsA[v] = S.get<uint8_t>(w * rowsA);
+          part_bytes += w * rowsA;
         }
       }
       stats[STAT_STATE_BYTES] = double(part_bytes);
@@ -1171,16 +1217,24 @@ std::unique_ptr<mig_groupby_result> groupby_aggregate(mig_column_view const* key
       T.mark(10);
       launch_rpa_scatter1(dP, dR, R, N, dp.num_cus, stream);
       T.mark(11);
-      if (R.two_level) launch_rpa_scatter2(dP, dR, R, dp.num_cus, 0, -1, stream);
+      if (R.two_level) {
+        launch_rpa_tiles(dR, stream);
+        launch_rpa_scatter2(dP, dR, R, dp.num_cus, 0, -1, stream);
+      }
       T.mark(12);
       launch_rpa_aggregate(dP, dR, R, P.key_mode == KM_GENERAL, 0, B, stream);
       T.mark(13);
       MIG_HIP_TRY(hipGetLastError());
       auto* h = reinterpret_cast<uint64_t*>(pin + 2 * sizeof(gb_params));
       MIG_HIP_TRY(hipMemcpyAsync(h, R.group_counter, 8, hipMemcpyDeviceToHost, stream));
-      MIG_HIP_TRY(hipMemcpyAsync(h + 1, d_small, 4, hipMemcpyDeviceToHost, stream));
+      MIG_HIP_TRY(hipMemcpyAsync(h + 1, d_small, 8, hipMemcpyDeviceToHost, stream));
       MIG_HIP_TRY(hipStreamSynchronize(stream));
-      if (uint32_t(h[1]) == 0) {
+      uint32_t const groups_overflow = uint32_t(h[1]), bins_overflow = uint32_t(h[1] >> 32);
+      if (bins_overflow) {  // a speculative bin filled up: the same bins, laid out exactly
+        release_outputs();
+        speculative = false;
+        stats[STAT_RETRIES] += 1;
+      } else if (groups_overflow == 0) {
         G = int64_t(h[0]);
         for (auto& k : res->keys) k.col.size = int32_t(G);
         for (auto& rc : res->results)
@@ -1188,10 +1242,12 @@ std::unique_ptr<mig_groupby_result> groupby_aggregate(mig_column_view const* key
         if (P.key_mode == KM_GENERAL) gather_general_keys(R.out_rep, G);
         stats[STAT_STRATEGY] = MIG_STRATEGY_PARTITIONED;
         stats[STAT_CAPACITY] = double(B);
-        stats[STAT_RETRIES]  = attempt;
+        stats[STAT_RETRIES] += bin_attempts;
+        stats[STAT_RPA_SPECULATIVE] = speculative ? 1.0 : 0.0;
       } else {
         release_outputs();
         log_bins = std::min(RPA_MAX_LOG_BINS, log_bins + 2);
+        bin_attempts++;
       }
     }
   }

@@ -24,6 +24,7 @@ struct device_allocator {
   void deallocate(void* p, size_t bytes);
 };
 
+
 }  // namespace mig
 
 struct mig_groupby_result {
@@ -56,6 +57,7 @@ enum stat_index : int {
   STAT_T_RPA_SCATTER2,
   STAT_T_RPA_AGGREGATE,
   STAT_LDS_PREAGG,       // 1 when the global path pre-aggregated in LDS
+  STAT_RPA_SPECULATIVE,  // 1 when the partitioned path's bins were sized without a histogram
   STAT_COUNT
 };
 double* thread_stats();
@@ -67,6 +69,13 @@ std::unique_ptr<mig_groupby_result> groupby_aggregate(mig_column_view const* key
                                                       mig_aggregation_request const* requests,
                                                       int32_t num_requests, hipStream_t stream,
                                                       mig_memory_resource const* mr, bool keys_sorted = false);
+
+// The sort-based group-by (sort_groupby.cpp): MEDIAN and QUANTILE, and
+// NUNIQUE / NTH_ELEMENT of unsorted keys, with the other requested kinds.
+std::unique_ptr<mig_groupby_result> sort_groupby(mig_column_view const* keys, int32_t num_keys, int32_t null_policy,
+                                                 mig_aggregation_request const* requests, int32_t num_requests,
+                                                 hipStream_t stream, mig_memory_resource const* mr, bool keys_sorted);
+int num_cus_of_device();
 
 void hash_rows(mig_column_view const* cols, int32_t ncols, uint32_t seed, uint32_t* out,
                hipStream_t stream);

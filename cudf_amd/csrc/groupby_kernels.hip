@@ -77,19 +77,25 @@ __global__ void __launch_bounds__(BLOCK)
   // population whose distinct keys are estimated
   if (P.skip_null_keys && any_key_null(P, row)) return;
   aadd(counters, 1u);
-  uint32_t const h   = row_hash(P.keys, P.num_keys, row, 0);
-  uint64_t const tag = (1ull << 32) | h;
-  uint32_t s         = mm_fmix32(h) & table_mask;
+  // a slot holds (rows sharing the hash << 32) | hash; 0 = empty
+  uint32_t const h = row_hash(P.keys, P.num_keys, row, 0);
+  uint32_t s       = mm_fmix32(h) & table_mask;
   for (uint32_t k = 0; k <= table_mask; k++) {
     uint64_t cur = aload(table + s);
-    if (cur == tag) return;
     if (cur == 0) {
-      uint64_t old = acas64(table + s, 0ull, tag);
+      uint64_t const old = acas64(table + s, 0ull, (1ull << 32) | h);
       if (old == 0) {
         aadd(counters + 1, 1u);
+        amax(counters + 2, 1u);
         return;
       }
-      if (old == tag) return;
+      cur = old;
+    }
+    if (uint32_t(cur) == h) {
+      uint64_t const before = __hip_atomic_fetch_add(table + s, 1ull << 32, __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_AGENT);
+      amax(counters + 2, uint32_t(before >> 32) + 1u);
+      return;
     }
     s = (s + 1) & table_mask;
   }
@@ -338,6 +344,57 @@ void launch_aggregate(gb_params const* dP, gb_params const& hP, int num_cus, hip
     hipLaunchKernelGGL(k_aggregate_global<KM_INLINE>, dim3(blocks), dim3(BLOCK), 0, s, dP);
   else
     hipLaunchKernelGGL(k_aggregate_global<KM_GENERAL>, dim3(blocks), dim3(BLOCK), 0, s, dP);
+}
+
+// ---------------------------------------------------------------------------
+// Group labels for the sort-based path: every row's slot in the device-wide
+// table (UINT32_MAX for a dropped row), then each slot's rank among the
+// occupied slots (the compaction order), then every row's group number.
+template <int KM>
+__global__ void __launch_bounds__(BLOCK) k_label_rows(gb_params const* __restrict__ Pp, uint32_t* __restrict__ slot_of_row)
+{
+  gb_params const& P = *Pp;
+  int64_t const n    = P.num_rows;
+  for (int64_t row = int64_t(blockIdx.x) * BLOCK + threadIdx.x; row < n; row += int64_t(gridDim.x) * BLOCK) {
+    int64_t const slot = locate<KM, true>(P, row);
+    gp(slot_of_row)[row] = slot < 0 ? 0xFFFFFFFFu : uint32_t(slot);
+  }
+}
+
+void launch_label_rows(gb_params const* dP, gb_params const& hP, int num_cus, uint32_t* slot_of_row, hipStream_t s)
+{
+  if (hP.num_rows <= 0) return;
+  int64_t blocks = row_grid(hP.num_rows, num_cus, 16);
+  if (hP.key_mode == KM_INLINE)
+    hipLaunchKernelGGL(k_label_rows<KM_INLINE>, dim3(blocks), dim3(BLOCK), 0, s, dP, slot_of_row);
+  else
+    hipLaunchKernelGGL(k_label_rows<KM_GENERAL>, dim3(blocks), dim3(BLOCK), 0, s, dP, slot_of_row);
+}
+
+__global__ void __launch_bounds__(BLOCK)
+  k_rank_slots(uint32_t const* __restrict__ map, int64_t G, uint32_t* __restrict__ rank_of_slot)
+{
+  int64_t const j = int64_t(blockIdx.x) * BLOCK + threadIdx.x;
+  if (j < G) gp(rank_of_slot)[gp(map)[j]] = uint32_t(j);
+}
+
+__global__ void __launch_bounds__(BLOCK)
+  k_gid_of_rows(uint32_t const* __restrict__ slot_of_row, uint32_t const* __restrict__ rank_of_slot, int64_t n,
+                uint32_t dropped, uint32_t* __restrict__ gid)
+{
+  int64_t const r = int64_t(blockIdx.x) * BLOCK + threadIdx.x;
+  if (r >= n) return;
+  uint32_t const s = gp(slot_of_row)[r];
+  gp(gid)[r]       = s == 0xFFFFFFFFu ? dropped : gp(rank_of_slot)[s];
+}
+
+void launch_group_numbers(uint32_t const* map, int64_t G, uint32_t const* slot_of_row, int64_t n,
+                          uint32_t* rank_of_slot, uint32_t* gid, hipStream_t s)
+{
+  if (G > 0) hipLaunchKernelGGL(k_rank_slots, dim3((G + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, s, map, G, rank_of_slot);
+  if (n > 0)
+    hipLaunchKernelGGL(k_gid_of_rows, dim3((n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, s, slot_of_row, rank_of_slot, n,
+                       uint32_t(G), gid);
 }
 
 // ---------------------------------------------------------------------------

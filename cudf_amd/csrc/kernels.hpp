@@ -183,6 +183,14 @@ struct rpa_params {
   unsigned long long* cursor1;       // [F1] next free row of each first-level bin (buffer A)
   unsigned long long* cursor2;       // [B] next free row of each final bin (buffer B)
   int64_t* tile_start1;              // [F1 + 1] scatter-2 tiles before each first-level bin
+  int64_t* first_start;              // [F1] first record of each first-level bin (buffer A)
+  // Rows each first-level / final bin may hold.  Exact layouts (from the
+  // histogram) never reach them; speculative layouts (sized from the sampled
+  // estimate, no histogram pass) flag a bin that would pass its capacity in
+  // cap_overflow and write none of the rows that do not fit.
+  int64_t cap1;
+  int64_t cap2;
+  int32_t* cap_overflow;
   uint64_t* keysA;
   uint64_t* keysB;
   uint8_t* flagsA;
@@ -212,8 +220,9 @@ struct launch_cfg {
 
 void launch_hash_rows(dcol const* d_cols, int ncols, int64_t n, uint32_t seed, uint32_t* out,
                       hipStream_t s);
-// Distinct row hashes among `nsample` evenly strided rows: counters[0] =
-// rows sampled (dropped rows excluded), counters[1] = distinct hashes.
+// Distinct row hashes among `nsample` sampled rows: counters[0] = rows
+// sampled (dropped rows excluded), counters[1] = distinct hashes,
+// counters[2] = the largest number of sampled rows sharing one hash.
 // `table` has table_mask + 1 zeroed slots.
 void launch_sample_hash(gb_params const* dP, int64_t n, int64_t nsample, uint64_t* table,
                         uint32_t table_mask, uint32_t* counters, hipStream_t s);
@@ -270,6 +279,31 @@ void launch_sorted_nunique(gb_params const* dP, int32_t f, int64_t n, int64_t G,
 // NTH_ELEMENT of field `f` on runs: the row of each group's n-th counted value.
 void launch_sorted_nth(gb_params const* dP, int32_t f, int64_t n, int64_t G, uint32_t const* gid, hipStream_t s);
 void launch_iota_u32(uint32_t* out, int64_t n, hipStream_t s);
+
+// ---- sort-based aggregations (sort_kernels.hip, label kernels in
+// groupby_kernels.hip) ---------------------------------------------------------
+// Every row's slot in the device-wide table (inserting its key), UINT32_MAX
+// for rows the null policy drops.
+void launch_label_rows(gb_params const* dP, gb_params const& hP, int num_cus, uint32_t* slot_of_row, hipStream_t s);
+// gid[r] = rank of row r's slot among the G occupied slots (map[j] = slot of
+// group j), G for dropped rows; rank_of_slot has cap + 2 entries.
+void launch_group_numbers(uint32_t const* map, int64_t G, uint32_t const* slot_of_row, int64_t n,
+                          uint32_t* rank_of_slot, uint32_t* gid, hipStream_t s);
+// Sort keys of a value column for quantiles: the order-preserving bits of
+// each element split into lo / hi 32-bit words (hi only for 8-byte values)
+// and the null flag (1 = null, so nulls sort after the values of a group).
+// NaN orders after every number; -0.0 is taken as 0.0.
+void launch_quantile_keys(dcol const& v, int64_t n, uint32_t* lo, uint32_t* hi, uint32_t* null_flag, hipStream_t s);
+// Quantiles of every group: the group's rows are [off[g], off[g + 1]) of the
+// order keys (ord_lo/ord_hi) and null flags sorted by (group, null, value);
+// out[g * nq + j] and its validity bit (the mask is zeroed by the caller).
+void launch_group_quantiles(uint32_t const* ord_lo, uint32_t const* ord_hi, uint32_t const* null_flag,
+                            int32_t const* off, int64_t G, int32_t storage, double const* q, int32_t nq,
+                            int32_t interp, double* out, uint32_t* out_mask, hipStream_t s);
+// Group numbers above G (rows the null policy dropped) become G.
+void launch_clamp_groups(uint32_t* gid, int64_t n, uint32_t G, hipStream_t s);
+// rep[g] = map[off[g]]: the first input row of each group after a row sort.
+void launch_first_rows(uint32_t const* map, int32_t const* off, int64_t G, uint32_t* rep, hipStream_t s);
 // string MIN / MAX: lengths and validity of the values at the ARGMIN / ARGMAX rows (-1 = none)
 void launch_str_arg_sizes(dcol const& src, int32_t const* rows, int64_t G, int32_t* sizes, uint32_t* mask,
                           hipStream_t s);
@@ -281,6 +315,11 @@ void launch_rpa_scatter1(gb_params const* dP, rpa_params const* dR, rpa_params c
                          int num_cus, hipStream_t s);
 // bin layout (bin_start, cursors, second-pass tiles) from the histogram
 void launch_rpa_layout(rpa_params const* dR, hipStream_t s);
+// Speculative layout: every first-level bin b1 starts at b1 * cap1, every
+// final bin b at b * cap2 (no histogram).
+void launch_rpa_spec_layout(rpa_params const* dR, rpa_params const& hR, hipStream_t s);
+// Second-pass tiles of each first-level bin from the rows scatter 1 wrote.
+void launch_rpa_tiles(rpa_params const* dR, hipStream_t s);
 // second pass over the tiles [tile_begin, tile_end) of the first-level bins
 // (tile_end < 0: to the last tile, counted on the device)
 void launch_rpa_scatter2(gb_params const* dP, rpa_params const* dR, rpa_params const& hR, int num_cus,

@@ -500,8 +500,12 @@ __global__ void __launch_bounds__(RPA_AGG_BLOCK) __attribute__((amdgpu_waves_per
   gb_params const& P = *Pp;
   rpa_params const& R = *Rp;
   int64_t const b     = bin_begin + blockIdx.x;
+  // the bin's records: from its start to where the scatter's cursor stopped
+  // (within its capacity; a bin that overflowed a speculative layout is
+  // redone with an exact one)
   int64_t const s     = R.bin_start[b];
-  int64_t const e     = R.bin_start[b + 1];
+  int64_t e           = int64_t(gp(R.two_level ? R.cursor2 : R.cursor1)[b]);
+  if (e - s > R.cap2) e = s + R.cap2;
   if (s == e) return;
   int const S       = R.lds_slots;
   int const NS      = S + 2;

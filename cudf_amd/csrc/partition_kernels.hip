@@ -78,7 +78,7 @@ __global__ void __launch_bounds__(RPA_SMALL_BLOCK) k_rpa_hist_reduce(rpa_params 
 
 // Bin layout from the histogram, on the device: bin_start (exclusive prefix
 // sums of the rows per final bin, rows < 2^31), the scatter cursors (each
-// bin's start) and, per first-level bin, its first second-pass tile.  One
+// bin's start) and the first record of each first-level bin.  One
 // workgroup; each thread scans a contiguous run of bins.
 constexpr int RPA_LAYOUT_BLOCK = 1024;
 
@@ -114,26 +114,76 @@ __global__ void __launch_bounds__(RPA_LAYOUT_BLOCK) k_rpa_layout(rpa_params cons
   }
   if (i1 == B && i0 < B) bin_start[B] = int64_t(run);
   __syncthreads();
-  // first-level bins (F1 <= 256): thread b1 takes bin b1, a block scan
-  // numbers the second-pass tiles
-  uint32_t tiles = 0;
+  // first-level bins (F1 <= 256): their first records and scatter cursors
   if (tid < F1) {
-    int64_t const lo = bin_start[int64_t(tid) << lf2], hi = bin_start[int64_t(tid + 1) << lf2];
-    gp(R.cursor1)[tid] = uint64_t(lo);
-    tiles              = uint32_t((hi - lo + R.tile_rows - 1) / R.tile_rows);
+    int64_t const lo         = bin_start[int64_t(tid) << lf2];
+    gp(R.first_start)[tid]   = lo;
+    gp(R.cursor1)[tid]       = uint64_t(lo);
   }
+}
+
+// Speculative layout: bins at fixed strides of their capacities.
+__global__ void __launch_bounds__(RPA_SMALL_BLOCK) k_rpa_spec_layout(rpa_params const* __restrict__ Rp)
+{
+  rpa_params const& R = *Rp;
+  int64_t const B     = int64_t(1) << R.log_bins;
+  int64_t const F1    = int64_t(1) << R.log_f1;
+  for (int64_t b = int64_t(blockIdx.x) * RPA_SMALL_BLOCK + threadIdx.x; b <= B; b += int64_t(gridDim.x) * RPA_SMALL_BLOCK) {
+    int64_t const st = b * R.cap2;
+    gp(R.bin_start)[b] = st;
+    if (b < B) gp(R.cursor2)[b] = uint64_t(st);
+    if (b < F1) {
+      // one level: the first-level bins are the final bins (buffer B, cap2)
+      int64_t const fs      = R.two_level ? b * R.cap1 : st;
+      gp(R.first_start)[b] = fs;
+      gp(R.cursor1)[b]     = uint64_t(fs);
+    }
+  }
+}
+
+void launch_rpa_spec_layout(rpa_params const* dR, rpa_params const& hR, hipStream_t s)
+{
+  int64_t const B = (int64_t(1) << hR.log_bins) + 1;
+  int64_t blocks  = (B + RPA_SMALL_BLOCK - 1) / RPA_SMALL_BLOCK;
+  if (blocks > 1024) blocks = 1024;
+  hipLaunchKernelGGL(k_rpa_spec_layout, dim3(blocks), dim3(RPA_SMALL_BLOCK), 0, s, dR);
+}
+
+// Rows of first-level bin b1 after scatter 1 (clamped to its capacity).
+__device__ __forceinline__ int64_t first_rows(rpa_params const& R, int b1)
+{
+  int64_t const lo = gp(static_cast<int64_t const*>(R.first_start))[b1];
+  int64_t hi       = int64_t(gp(static_cast<unsigned long long const*>(R.cursor1))[b1]);
+  if (hi - lo > R.cap1) hi = lo + R.cap1;
+  return hi - lo;
+}
+
+// Second-pass tiles: thread b1 takes first-level bin b1, a block scan numbers
+// the tiles (F1 <= 256).
+__global__ void __launch_bounds__(RPA_SMALL_BLOCK) k_rpa_tiles(rpa_params const* __restrict__ Rp)
+{
+  __shared__ uint32_t wave_sum[RPA_SMALL_BLOCK / 64];
+  rpa_params const& R = *Rp;
+  int const F1        = 1 << R.log_f1;
+  int const tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  uint32_t tiles = 0;
+  if (tid < F1) tiles = uint32_t((first_rows(R, tid) + R.tile_rows - 1) / R.tile_rows);
   uint32_t tincl = tiles;
   for (int o = 1; o < 64; o <<= 1) {
     uint32_t const y = __shfl_up(tincl, o, 64);
     if (lane >= o) tincl += y;
   }
-  __syncthreads();  // wave_sum is reused
   if (lane == 63) wave_sum[wave] = tincl;
   __syncthreads();
   uint32_t tbefore = 0;
   for (int w = 0; w < wave; w++) tbefore += wave_sum[w];
   if (tid < F1) gp(R.tile_start1)[tid] = int64_t(tbefore + tincl - tiles);
   if (tid == F1 - 1) gp(R.tile_start1)[F1] = int64_t(tbefore + tincl);
+}
+
+void launch_rpa_tiles(rpa_params const* dR, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_rpa_tiles, dim3(1), dim3(RPA_SMALL_BLOCK), 0, s, dR);
 }
 
 void launch_rpa_layout(rpa_params const* dR, hipStream_t s)
@@ -164,6 +214,7 @@ __device__ __forceinline__ void copy_width(uint8_t* dst, int64_t di, uint8_t con
 }
 
 constexpr int RPA_MAX_RPT    = RPA_TILE / RPA_SCATTER_BLOCK;  // rows per thread per tile (at most)
+constexpr unsigned long long NO_BASE = ~0ull;                // a bin whose rows are not written
 constexpr int RPA_MAX_FANOUT = 256;
 
 // Exclusive scan of cnt[0, fan) into off[0, fan] by wave 0 (fan <= 256).
@@ -292,8 +343,8 @@ __global__ void __launch_bounds__(RPA_SCATTER_BLOCK) __attribute__((amdgpu_waves
         else hi = mid - 1;
       }
       b1                 = lo;
-      int64_t const bs   = gp(static_cast<int64_t const*>(R.bin_start))[int64_t(b1) << lf2];
-      int64_t const be   = gp(static_cast<int64_t const*>(R.bin_start))[int64_t(b1 + 1) << lf2];
+      int64_t const bs   = gp(static_cast<int64_t const*>(R.first_start))[b1];
+      int64_t const be   = bs + first_rows(R, b1);
       r0                 = bs + (t - tstart[b1]) * T;
       r1                 = (r0 + T < be) ? r0 + T : be;
     } else {
@@ -352,9 +403,21 @@ __global__ void __launch_bounds__(RPA_SCATTER_BLOCK) __attribute__((amdgpu_waves
     scan_counts(cnt, toff, fan);
     for (int i = threadIdx.x; i < fan; i += RPA_SCATTER_BLOCK) {
       if (cnt[i]) {
-        unsigned long long* cur = SECOND ? R.cursor2 + ((int64_t(b1) << lf2) + i) : R.cursor1 + i;
-        base[i] = __hip_atomic_fetch_add(gp(cur), (unsigned long long)cnt[i], __ATOMIC_RELAXED,
-                                         __HIP_MEMORY_SCOPE_AGENT);
+        int64_t const fb        = (int64_t(b1) << lf2) + i;
+        unsigned long long* cur = SECOND ? R.cursor2 + fb : R.cursor1 + i;
+        unsigned long long const b0 = __hip_atomic_fetch_add(gp(cur), (unsigned long long)cnt[i], __ATOMIC_RELAXED,
+                                                             __HIP_MEMORY_SCOPE_AGENT);
+        // a bin of a speculative layout may fill up: its rows that do not
+        // fit are not written, and the call is redone with an exact layout
+        int64_t const start = SECOND ? gp(static_cast<int64_t const*>(R.bin_start))[fb]
+                                     : gp(static_cast<int64_t const*>(R.first_start))[i];
+        int64_t const cap   = (SECOND || !R.two_level) ? R.cap2 : R.cap1;
+        if (int64_t(b0) + int64_t(cnt[i]) > start + cap) {
+          astore(R.cap_overflow, 1);
+          base[i] = NO_BASE;
+        } else {
+          base[i] = b0;
+        }
       }
     }
     __syncthreads();
@@ -386,6 +449,7 @@ __global__ void __launch_bounds__(RPA_SCATTER_BLOCK) __attribute__((amdgpu_waves
     int const total = int(toff[fan]);
     for (int lp = threadIdx.x; lp < total; lp += RPA_SCATTER_BLOCK) {
       int const b       = sbins[lp];
+      if (base[b] == NO_BASE) continue;
       int64_t const dst = int64_t(base[b]) + (lp - int(toff[b]));
       gp(okeys)[dst]    = skeys[lp];
       gp(oflags)[dst]   = sflags[lp];

@@ -246,6 +246,58 @@ std::unique_ptr<mig_table> regroup(mig_column_view const* input, int32_t num_col
   return t;
 }
 
+}  // namespace
+
+void lsd_sort_rows(lsd_key const* keys, int nkeys, int64_t n, uint32_t* map, uint32_t* last_sorted, hipStream_t s)
+{
+  MIG_EXPECTS(n >= 0 && n < (int64_t(1) << 32), "row count out of range for a row sort");
+  if (n == 0) return;
+  call_scratch S(s);
+  uint32_t* pid_a    = S.get<uint32_t>(n);
+  uint32_t* pid_b    = S.get<uint32_t>(n);
+  uint32_t* rows_a   = S.get<uint32_t>(n);
+  uint32_t* rows_b   = S.get<uint32_t>(n);
+  uint32_t* rows_cur = nullptr;  // identity before the first pass
+  for (int k = 0; k < nkeys; k++) {
+    int const total_bits = std::max(1, std::min(32, keys[k].bits));
+    // the key's values in the current order
+    uint32_t* pid_cur = pid_a;
+    if (rows_cur == nullptr)
+      MIG_HIP_TRY(hipMemcpyAsync(pid_cur, keys[k].data, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, s));
+    else
+      launch_gather_fixed(keys[k].data, 4, rows_cur, n, pid_cur, s);
+    uint32_t* pid_next      = pid_b;
+    int const passes        = (total_bits + HP_DIGIT_BITS - 1) / HP_DIGIT_BITS;
+    int const bits_per_pass = (total_bits + passes - 1) / passes;
+    int shift               = 0;
+    for (int p = 0; p < passes; p++) {
+      int const bits   = std::min(bits_per_pass, total_bits - shift);
+      int64_t const nd = int64_t(1) << bits;
+      int64_t const w_cap = std::max<int64_t>(64, std::min<int64_t>(16384, (int64_t(1) << 23) / nd));
+      int64_t W           = std::min<int64_t>(w_cap, std::max<int64_t>(1, (n + 255) / 256));
+      int64_t const per   = (n + W - 1) / W;
+      W                   = (n + per - 1) / per;
+      uint32_t* hist      = S.get<uint32_t>(size_t(nd * W + 1));
+      launch_hp_count(pid_cur, n, per, W, shift, bits, hist, s);
+      launch_exclusive_scan_i32(reinterpret_cast<int32_t*>(hist), nd * W, reinterpret_cast<int32_t*>(hist) + nd * W,
+                                s);
+      uint32_t* rows_next = (rows_cur == rows_a) ? rows_b : rows_a;
+      bool const final_pass = k == nkeys - 1 && p == passes - 1;
+      if (final_pass) rows_next = map;
+      bool const keep_pid = p < passes - 1 || (final_pass && last_sorted);
+      uint32_t* pid_out   = final_pass && last_sorted ? last_sorted : pid_next;
+      launch_hp_scatter(pid_cur, rows_cur, n, per, W, shift, bits, hist, keep_pid ? pid_out : nullptr, rows_next,
+                        nullptr, 0, s);
+      MIG_HIP_TRY(hipGetLastError());
+      rows_cur = rows_next;
+      std::swap(pid_cur, pid_next);
+      shift += bits;
+    }
+  }
+}
+
+namespace {
+
 void check_table(mig_column_view const* input, int32_t num_columns)
 {
   MIG_EXPECTS(num_columns >= 0 && (num_columns == 0 || input != nullptr), "invalid table");

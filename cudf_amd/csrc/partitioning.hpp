@@ -66,6 +66,19 @@ void launch_concat_char_ranges(concat_piece const* pieces, int np, int64_t* rang
 void launch_concat_str_offsets(concat_piece const* pieces, int np, int64_t n, int64_t total_chars, int32_t* out,
                                hipStream_t s);
 
+// One key of a stable LSD sort of rows: a 32-bit value per input row (in
+// input row order) of which the low `bits` bits are significant.
+struct lsd_key {
+  uint32_t const* data;
+  int bits;
+};
+// Stable sort of the rows 0..n-1 by `keys`, least significant key first:
+// map[i] = the input row at sorted position i.  `last_sorted`, when given,
+// receives the most significant key's values in sorted order.  Counting-sort
+// passes of at most HP_DIGIT_BITS bits per key; enqueued on `s` (scratch is
+// released after synchronising `s`).
+void lsd_sort_rows(lsd_key const* keys, int nkeys, int64_t n, uint32_t* map, uint32_t* last_sorted, hipStream_t s);
+
 // cudf::hash_partition / cudf::partition: the regrouped table and the first
 // row of every partition (`offsets`, num_partitions entries).
 std::unique_ptr<mig_table> hash_partition(mig_column_view const* input, int32_t num_columns,

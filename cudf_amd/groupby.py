@@ -73,7 +73,7 @@ class GroupBy:
         for i, r in enumerate(requests):
             aggs = (_lib.Aggregation * max(1, len(r.aggregations)))()
             for j, a in enumerate(r.aggregations):
-                aggs[j] = _lib.Aggregation(int(a.kind()), a.ddof, int(a.null_handling), a.n)
+                aggs[j] = a.to_c(keep)
             keep.append(aggs)
             reqs[i].values = r.values.view()
             reqs[i].aggregations = ctypes.cast(aggs, ctypes.POINTER(_lib.Aggregation))

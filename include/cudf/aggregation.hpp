@@ -4,6 +4,7 @@
 #pragma once
 
 #include <memory>
+#include <vector>
 
 #include "types.hpp"
 
@@ -45,6 +46,8 @@ class aggregation {
   size_type _ddof;  // VARIANCE / STD
   null_policy _null_handling;  // NUNIQUE, NTH_ELEMENT
   size_type _n;                // NTH_ELEMENT
+  std::vector<double> _quantiles;                     // QUANTILE
+  interpolation _interpolation = interpolation::LINEAR;  // QUANTILE
 };
 
 /// Aggregations valid in a group-by (aggregation.hpp groupby_aggregation).
@@ -121,6 +124,24 @@ template <typename Base = aggregation>
 std::unique_ptr<Base> make_nth_element_aggregation(size_type n, null_policy null_handling = null_policy::INCLUDE)
 {
   return std::make_unique<Base>(aggregation::NTH_ELEMENT, 1, null_handling, n);
+}
+/// The 0.5 quantile, linear interpolation (aggregation.hpp make_median_aggregation).
+template <typename Base = aggregation>
+std::unique_ptr<Base> make_median_aggregation()
+{
+  return std::make_unique<Base>(aggregation::MEDIAN);
+}
+/// Quantiles of each group's valid values; the result column holds
+/// num_groups * quantiles.size() doubles, group-major
+/// (aggregation.hpp make_quantile_aggregation).
+template <typename Base = aggregation>
+std::unique_ptr<Base> make_quantile_aggregation(std::vector<double> const& quantiles,
+                                                interpolation interp = interpolation::LINEAR)
+{
+  auto a            = std::make_unique<Base>(aggregation::QUANTILE);
+  a->_quantiles     = quantiles;
+  a->_interpolation = interp;
+  return a;
 }
 template <typename Base = aggregation>
 std::unique_ptr<Base> make_variance_aggregation(size_type ddof = 1)

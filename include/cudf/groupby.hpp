@@ -80,7 +80,9 @@ class groupby {
     for (size_t r = 0; r < requests.size(); r++) {
       for (auto const& a : requests[r].aggregations)
         aggs[r].push_back(mig_aggregation{static_cast<int32_t>(a->kind), a->_ddof,
-                                          static_cast<int32_t>(a->_null_handling), a->_n});
+                                          static_cast<int32_t>(a->_null_handling), a->_n,
+                                          static_cast<int32_t>(a->_interpolation),
+                                          static_cast<int32_t>(a->_quantiles.size()), a->_quantiles.data()});
       reqs[r].values           = requests[r].values.c_view();
       reqs[r].aggregations     = aggs[r].data();
       reqs[r].num_aggregations = int32_t(aggs[r].size());

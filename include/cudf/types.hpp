@@ -69,6 +69,8 @@ enum class null_equality : bool { EQUAL, UNEQUAL };  // types.hpp:151-154
 enum class sorted : bool { NO, YES };                // types.hpp:167
 enum class order : bool { ASCENDING, DESCENDING };
 enum class null_order : bool { AFTER, BEFORE };
+// types.hpp:192-198
+enum class interpolation : int32_t { LINEAR, LOWER, HIGHER, MIDPOINT, NEAREST };
 
 /// Element width in bytes of a fixed-width type, 0 otherwise.
 constexpr std::size_t size_of(data_type t)

@@ -81,10 +81,11 @@ enum {
  *      listed in hash/groupby.cu:73-84, all implemented here: SUM, PRODUCT,
  *      MIN, MAX, COUNT_VALID, COUNT_ALL, SUM_OF_SQUARES, MEAN, VARIANCE, STD,
  *      ARGMIN, ARGMAX (string values: COUNT_*, MIN, MAX, ARGMIN, ARGMAX).
- *      NUNIQUE and NTH_ELEMENT, sort-path aggregations in the reference
- *      (groupby.cu:78-83, sort/group_nunique.cu, sort/group_nth_element.cu),
- *      are taken by mig_groupby_aggregate_sorted for fixed-width values;
- *      mig_groupby_aggregate rejects them with MIG_ERROR_LOGIC. --- */
+ *      MEDIAN, QUANTILE, NUNIQUE and NTH_ELEMENT are sort-path aggregations
+ *      in the reference (groupby.cu:78-83, sort/aggregate.cpp:323-410); a
+ *      request that holds one runs the sort-based group-by, for fixed-width
+ *      values (a request there with string values returns
+ *      MIG_ERROR_DATA_TYPE). --- */
 enum {
   MIG_AGG_SUM            = 0,
   MIG_AGG_PRODUCT        = 1,
@@ -99,6 +100,8 @@ enum {
   MIG_AGG_M2             = 10,
   MIG_AGG_VARIANCE       = 11,
   MIG_AGG_STD            = 12,
+  MIG_AGG_MEDIAN         = 13,
+  MIG_AGG_QUANTILE       = 14,
   MIG_AGG_ARGMAX         = 15,
   MIG_AGG_ARGMIN         = 16,
   MIG_AGG_NUNIQUE        = 17,
@@ -107,6 +110,15 @@ enum {
 
 /* cudf::null_policy (cudf/types.hpp:126-129). */
 enum { MIG_NULL_EXCLUDE = 0, MIG_NULL_INCLUDE = 1 };
+
+/* cudf::interpolation (cudf/types.hpp:192-198), used by QUANTILE. */
+enum {
+  MIG_INTERP_LINEAR   = 0,
+  MIG_INTERP_LOWER    = 1,
+  MIG_INTERP_HIGHER   = 2,
+  MIG_INTERP_MIDPOINT = 3,
+  MIG_INTERP_NEAREST  = 4
+};
 
 /* Non-owning view of one column; mirrors cudf::column_view_base fields
  * (cudf/column/column_view.hpp:240-247).  For STRING columns `data` is the
@@ -131,12 +143,18 @@ typedef struct mig_column_view {
  * does not count nulls, MIG_NULL_INCLUDE counts them as one value) and by
  * NTH_ELEMENT (make_nth_element_aggregation, whose default is INCLUDE:
  * EXCLUDE counts valid values only), `n` by NTH_ELEMENT (the group's n-th
- * row, counted from the end when negative). */
+ * row, counted from the end when negative), `quantiles` /
+ * `num_quantiles` / `interpolation` by QUANTILE
+ * (make_quantile_aggregation(quantiles, interp); host memory, read during
+ * the call).  Fields a kind does not use may be left zero. */
 typedef struct mig_aggregation {
   int32_t kind; /* MIG_AGG_* */
   int32_t ddof;
   int32_t null_handling;
   int32_t n;
+  int32_t interpolation; /* MIG_INTERP_* */
+  int32_t num_quantiles;
+  const double* quantiles;
 } mig_aggregation;
 
 /* cudf::groupby::aggregation_request (cudf/groupby.hpp:61-64). */

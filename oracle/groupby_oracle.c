@@ -336,16 +336,15 @@ typedef struct { int64_t* slots; int64_t cap; } omap;
  * Outputs: `rep_rows[g]` = first row of group g; `outs[a]` = array of G
  * elements of the target type; `out_valid[a]` = G validity bytes.
  * Returns G, or -1 on an unsupported (type, kind). */
-int64_t orc_groupby(const orc_col* keys, int32_t nkeys, int32_t null_policy,
-                    const orc_col* values, const int32_t* value_index,
-                    const int32_t* kinds, const int32_t* ddofs, int32_t naggs,
-                    int64_t* rep_rows, void** outs, uint8_t** out_valid)
+/* Groups of the rows in first-occurrence order: group_of_row[r] (-1 for a
+ * dropped row), rep_rows[g] = first row of group g; returns G. */
+static int64_t assign_groups(const orc_col* keys, int32_t nkeys, int32_t null_policy, int64_t* rep_rows,
+                             int64_t* group_of_row)
 {
   int64_t n = nkeys > 0 ? keys[0].size : 0;
   int64_t cap = 16;
   while (cap < 2 * n + 16) cap <<= 1;
   int64_t* slots = (int64_t*)malloc(sizeof(int64_t) * cap);
-  int64_t* group_of_row = (int64_t*)malloc(sizeof(int64_t) * (n > 0 ? n : 1));
   int64_t* slot_group = (int64_t*)malloc(sizeof(int64_t) * cap);
   for (int64_t i = 0; i < cap; i++) slots[i] = -1;
   int64_t G = 0;
@@ -369,6 +368,17 @@ int64_t orc_groupby(const orc_col* keys, int32_t nkeys, int32_t null_policy,
   }
   free(slots);
   free(slot_group);
+  return G;
+}
+
+int64_t orc_groupby(const orc_col* keys, int32_t nkeys, int32_t null_policy,
+                    const orc_col* values, const int32_t* value_index,
+                    const int32_t* kinds, const int32_t* ddofs, int32_t naggs,
+                    int64_t* rep_rows, void** outs, uint8_t** out_valid)
+{
+  int64_t n = nkeys > 0 ? keys[0].size : 0;
+  int64_t* group_of_row = (int64_t*)malloc(sizeof(int64_t) * (n > 0 ? n : 1));
+  int64_t G = assign_groups(keys, nkeys, null_policy, rep_rows, group_of_row);
 
   long double* acc = (long double*)malloc(sizeof(long double) * (G > 0 ? G : 1));
   uint64_t* iacc = (uint64_t*)malloc(sizeof(uint64_t) * (G > 0 ? G : 1));
@@ -547,4 +557,98 @@ int64_t orc_groupby(const orc_col* keys, int32_t nkeys, int32_t null_policy,
   }
   free(acc); free(iacc); free(cnt); free(cnt_all); free(group_of_row);
   return ok ? G : -1;
+}
+
+/* ---- QUANTILE / MEDIAN --------------------------------------------------
+ * sort/group_quantiles.cu:30-80 and quantiles/quantiles_util.hpp: each
+ * group's valid values sorted ascending (NaN after every number); with
+ * n = valid count, q clamped to [0, 1] and v = q * (n - 1):
+ * lower = floor(v), higher = ceil(v), nearest = nearbyint(v) (ties to even),
+ * fraction = v - lower;  LINEAR (1 - f) * x[lower] + f * x[higher] in
+ * double;  MIDPOINT x[lower] / 2 + x[higher] / 2 in double, except for
+ * int64 values: half = a / 2 + b / 2, rest = a % 2 + b % 2 (C division),
+ * result = half + rest * 0.5;  LOWER / HIGHER / NEAREST the element.  Every
+ * result is double; null when the group has no valid value.  MEDIAN is
+ * QUANTILE {0.5} LINEAR (sort/aggregate.cpp:344-362).  Outputs are group-
+ * major: out[g * nq + j] for quantile j of group g. */
+enum { I_LINEAR = 0, I_LOWER, I_HIGHER, I_MIDPOINT, I_NEAREST };
+
+static const orc_col* g_q_col;
+static int q_cmp(const void* pa, const void* pb)
+{
+  int64_t a = *(const int64_t*)pa, b = *(const int64_t*)pb;
+  const orc_col* c = g_q_col;
+  if (is_float(c->type_id)) {
+    long double x = as_ld(c, a), y = as_ld(c, b);
+    int nx = isnan(x), ny = isnan(y);
+    if (nx || ny) return nx - ny;
+    return (x > y) - (x < y);
+  }
+  if (c->type_id == T_UINT64) {
+    uint64_t x = (uint64_t)as_i64(c, a), y = (uint64_t)as_i64(c, b);
+    return (x > y) - (x < y);
+  }
+  int64_t x = as_i64(c, a), y = as_i64(c, b);
+  return (x > y) - (x < y);
+}
+
+static double q_elem(const orc_col* c, int64_t row)
+{
+  return (double)as_ld(c, row);
+}
+
+int64_t orc_group_quantiles(const orc_col* keys, int32_t nkeys, int32_t null_policy, const orc_col* values,
+                            const double* q, int32_t nq, int32_t interp, int64_t* rep_rows, double* out,
+                            uint8_t* out_valid)
+{
+  int t = values->type_id;
+  if (!(is_float(t) || is_signed_int(t) || (is_unsigned_int(t) && t != T_BOOL8))) return -1;
+  if (interp < I_LINEAR || interp > I_NEAREST) return -1;
+  int64_t n = nkeys > 0 ? keys[0].size : 0;
+  int64_t* group_of_row = (int64_t*)malloc(sizeof(int64_t) * (n > 0 ? n : 1));
+  int64_t G = assign_groups(keys, nkeys, null_policy, rep_rows, group_of_row);
+  /* rows of each group's valid values, grouped by a counting sort */
+  int64_t* start = (int64_t*)calloc((size_t)G + 1, sizeof(int64_t));
+  for (int64_t r = 0; r < n; r++)
+    if (group_of_row[r] >= 0 && row_valid(values, r)) start[group_of_row[r] + 1]++;
+  for (int64_t g = 0; g < G; g++) start[g + 1] += start[g];
+  int64_t* fill = (int64_t*)malloc(sizeof(int64_t) * ((size_t)G + 1));
+  memcpy(fill, start, sizeof(int64_t) * ((size_t)G + 1));
+  int64_t* rows = (int64_t*)malloc(sizeof(int64_t) * (size_t)(start[G] > 0 ? start[G] : 1));
+  for (int64_t r = 0; r < n; r++)
+    if (group_of_row[r] >= 0 && row_valid(values, r)) rows[fill[group_of_row[r]]++] = r;
+  g_q_col = values;
+  for (int64_t g = 0; g < G; g++) {
+    int64_t m = start[g + 1] - start[g];
+    int64_t* x = rows + start[g];
+    qsort(x, (size_t)m, sizeof(int64_t), q_cmp);
+    for (int32_t j = 0; j < nq; j++) {
+      double r = 0.0;
+      if (m > 0) {
+        double qq = q[j] < 0.0 ? 0.0 : (q[j] > 1.0 ? 1.0 : q[j]);
+        double v = qq * (double)(m - 1);
+        int64_t lo = (int64_t)floor(v), hi = (int64_t)ceil(v), ne = (int64_t)nearbyint(v);
+        double f = v - (double)lo;
+        switch (interp) {
+          case I_LINEAR: r = (1.0 - f) * q_elem(values, x[lo]) + f * q_elem(values, x[hi]); break;
+          case I_LOWER: r = q_elem(values, x[lo]); break;
+          case I_HIGHER: r = q_elem(values, x[hi]); break;
+          case I_NEAREST: r = q_elem(values, x[ne]); break;
+          case I_MIDPOINT:
+            if (t == T_INT64) {
+              int64_t a = as_i64(values, x[lo]), b = as_i64(values, x[hi]);
+              int64_t half = a / 2 + b / 2, rest = a % 2 + b % 2;
+              r = (double)half + (double)rest * 0.5;
+            } else {
+              r = q_elem(values, x[lo]) / 2 + q_elem(values, x[hi]) / 2;
+            }
+            break;
+        }
+      }
+      out[g * nq + j] = r;
+      out_valid[g * nq + j] = (uint8_t)(m > 0);
+    }
+  }
+  free(start); free(fill); free(rows); free(group_of_row);
+  return G;
 }

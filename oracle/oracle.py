@@ -29,7 +29,9 @@ STRING = 23
 # cudf::aggregation::Kind numbering (cudf/aggregation.hpp:86-122)
 SUM, PRODUCT, MIN, MAX, COUNT_VALID, COUNT_ALL = 0, 1, 2, 3, 4, 5
 SUM_OF_SQUARES, MEAN, VARIANCE, STD = 8, 9, 11, 12
+MEDIAN, QUANTILE = 13, 14  # QUANTILE's second element is (quantiles, interpolation)
 ARGMAX, ARGMIN = 15, 16
+LINEAR, LOWER, HIGHER, MIDPOINT, NEAREST = 0, 1, 2, 3, 4  # cudf::interpolation
 NUNIQUE = 17  # the request's second element is the null policy (0 EXCLUDE, 1 INCLUDE)
 NTH_ELEMENT = 18  # the request's second element is 2 * n + (1 if nulls are included)
 
@@ -113,6 +115,10 @@ def _load():
                                       ctypes.c_uint32, ctypes.c_void_p]
         lib.orc_target_type.restype = ctypes.c_int32
         lib.orc_target_type.argtypes = [ctypes.c_int32, ctypes.c_int32]
+        lib.orc_group_quantiles.restype = ctypes.c_int64
+        lib.orc_group_quantiles.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p,
+                                            ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p,
+                                            ctypes.c_void_p, ctypes.c_void_p]
         lib.orc_groupby.restype = ctypes.c_int64
         lib.orc_groupby.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32,
                                     ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
@@ -202,3 +208,25 @@ def groupby(keys: Sequence[HostColumn], requests: Sequence[tuple], null_policy: 
             a += 1
         results.append(res)
     return rep[:g].copy(), results
+
+
+def group_quantiles(keys: Sequence[HostColumn], values: HostColumn, quantiles, interpolation: int = LINEAR,
+                    null_policy: int = 0):
+    """QUANTILE of every group (first-occurrence group order): returns
+    ``(rep_rows, out, valid)`` with ``out`` of shape (G, len(quantiles)),
+    float64, group-major as the reference lays the result column out."""
+    n = keys[0].size if keys else 0
+    q = np.ascontiguousarray(np.asarray(quantiles, dtype=np.float64).reshape(-1))
+    nq = len(q)
+    rep = np.zeros(max(1, n), dtype=np.int64)
+    out = np.zeros(max(1, n * max(nq, 1)), dtype=np.float64)
+    valid = np.zeros(max(1, n * max(nq, 1)), dtype=np.uint8)
+    karr, kk = _cols(keys)
+    varr, vk = _cols([values])
+    qb = q if nq else np.zeros(1, np.float64)
+    g = _load().orc_group_quantiles(ctypes.addressof(karr), len(keys), null_policy, ctypes.addressof(varr),
+                                    qb.ctypes.data, nq, interpolation, rep.ctypes.data, out.ctypes.data,
+                                    valid.ctypes.data)
+    if g < 0:
+        raise TypeError("unsupported quantile request")
+    return rep[:g].copy(), out[:g * nq].reshape(g, nq).copy(), valid[:g * nq].reshape(g, nq).astype(bool)

@@ -12,8 +12,15 @@ from oracle import oracle as orc
 
 AGG = {
     "SUM": 0, "PRODUCT": 1, "MIN": 2, "MAX": 3, "COUNT_VALID": 4, "COUNT_ALL": 5,
-    "SUM_OF_SQUARES": 8, "MEAN": 9, "VARIANCE": 11, "STD": 12, "ARGMAX": 15, "ARGMIN": 16, "NUNIQUE": 17, "NTH_ELEMENT": 18,
+    "SUM_OF_SQUARES": 8, "MEAN": 9, "VARIANCE": 11, "STD": 12, "MEDIAN": 13, "QUANTILE": 14,
+    "ARGMAX": 15, "ARGMIN": 16, "NUNIQUE": 17, "NTH_ELEMENT": 18,
 }
+QUANTILE_KINDS = (13, 14)  # a request pair (QUANTILE, (quantiles, interpolation)); (MEDIAN, anything)
+
+
+def n_quantiles(kind, param) -> int:
+    """Result rows per group of an aggregation."""
+    return len(param[0]) if kind == AGG["QUANTILE"] else 1
 
 # Tolerances of the parity bar.  Integer, index, key, count, min and max
 # results are compared bit-exactly.  Floating-point sums carry the error of
@@ -84,6 +91,10 @@ def _aggregation(kind, param):
         return Aggregation(kind, null_handling=NullPolicy(param))
     if kind == AGG["NTH_ELEMENT"]:
         return Aggregation(kind, null_handling=NullPolicy(param & 1), n=param >> 1)
+    if kind == AGG["QUANTILE"]:
+        return Aggregation(kind, quantiles=param[0], interpolation=param[1])
+    if kind == AGG["MEDIAN"]:
+        return Aggregation(kind)
     return Aggregation(kind, param)
 
 
@@ -95,6 +106,29 @@ def take(col: orc.HostColumn, rows: np.ndarray) -> orc.HostColumn:
         chars = sk.chars if len(sk.chars) else np.zeros(1, np.uint8)
         return orc.HostColumn(orc.STRING, chars, valid, sk.offsets)
     return orc.HostColumn(col.type_id, col.data[rows], valid)
+
+
+def _orc_groupby_any(keys, requests, null_policy):
+    """orc.groupby with MEDIAN / QUANTILE answered by orc.group_quantiles
+    (both number the groups in first-occurrence order)."""
+    plain = []
+    for col, aggs in requests:
+        plain.append((col, [a for a in aggs if a[0] not in QUANTILE_KINDS] or [(AGG["COUNT_ALL"], 1)]))
+    rep, pres = orc.groupby(keys, plain, null_policy)
+    results = []
+    for r, (col, aggs) in enumerate(requests):
+        out, j = [], 0
+        for kind, param in aggs:
+            if kind in QUANTILE_KINDS:
+                qs, interp = ((0.5,), orc.LINEAR) if kind == AGG["MEDIAN"] else param
+                qrep, v, valid = orc.group_quantiles(keys, col, qs, interp, null_policy)
+                assert np.array_equal(qrep, rep)
+                out.append((v.reshape(-1), valid.reshape(-1)))
+            else:
+                out.append(pres[r][j])
+                j += 1
+        results.append(out)
+    return rep, results
 
 
 # Rows per shard below which the oracle runs in one call.
@@ -116,7 +150,7 @@ def _oracle_sharded(keys, requests, null_policy, shards):
         rows = order[bounds[i]:bounds[i + 1]]
         sk = [take(k, rows) for k in keys]
         sr = [(take(c, rows), aggs) for c, aggs in requests]
-        rep, res = orc.groupby(sk, sr, null_policy)
+        rep, res = _orc_groupby_any(sk, sr, null_policy)
         return rows[rep], res
 
     with ThreadPoolExecutor(max_workers=shards) as ex:
@@ -134,7 +168,7 @@ def oracle_groupby(keys: Sequence[orc.HostColumn], requests, null_policy: int = 
     if n >= 2 * _SHARD_MIN_ROWS:
         rep, res = _oracle_sharded(keys, requests, null_policy, int(min(16, n // _SHARD_MIN_ROWS)))
     else:
-        rep, res = orc.groupby(keys, requests, null_policy)
+        rep, res = _orc_groupby_any(keys, requests, null_policy)
     kcols = []
     for k in keys:
         if k.type_id == orc.STRING:
@@ -352,6 +386,17 @@ def assert_parity(keys, requests, null_policy=0, gpu=None, stats=None):
             gv, gval = g_res[r][a]
             ov, oval = o_res[r][a]
             assert len(gv) == len(ov)
+            if kind in QUANTILE_KINDS:
+                # G * nq rows, group-major: both sides in key order, bit-exact
+                nq = n_quantiles(kind, ddof)
+                gq = (og[:, None] * nq + np.arange(nq)).reshape(-1)
+                oq = (oo[:, None] * nq + np.arange(nq)).reshape(-1)
+                gm = np.ones(len(gv), bool) if gval is None else gval
+                assert np.array_equal(gm[gq], oval[oq]), f"quantile validity differs (request {r} agg {a})"
+                x, y = gv[gq][oval[oq]], ov[oq][oval[oq]]
+                same = (x == y) | (np.isnan(x) & np.isnan(y))
+                assert same.all(), f"quantile not bit-exact: {x[~same][:5]} vs {y[~same][:5]}"
+                continue
             gvalid = np.ones(len(gv), bool) if gval is None else gval
             gvalid_s, ovalid_s = gvalid[og], oval[oo]
             bad = np.flatnonzero(gvalid_s != ovalid_s)

@@ -87,7 +87,8 @@ def test_unsupported_kind_and_type_rejected():
         h = ctypes.c_void_p()
         return lib.mig_groupby_aggregate(keys, 1, 0, reqs, 1, None, None, ctypes.byref(h))
 
-    assert call(3, 13) == _lib.MIG_ERROR_LOGIC          # MEDIAN: not a hash aggregation
+    assert call(3, 6) == _lib.MIG_ERROR_LOGIC           # ANY: not a group-by aggregation here
+    assert call(23, 13) == _lib.MIG_ERROR_DATA_TYPE     # MEDIAN of strings
     assert call(23, 0) == _lib.MIG_ERROR_DATA_TYPE      # SUM of strings
     assert call(12, 9) == _lib.MIG_ERROR_DATA_TYPE      # MEAN of timestamps
 

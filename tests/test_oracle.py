@@ -587,3 +587,84 @@ def test_nth_element_brute_force(n, include):
             assert not valid[gi]
         else:
             assert valid[gi] and got[gi] == v[want]
+
+
+# ---------------------------------------------------------------- quantiles --
+QKATS = load("quantile_kats.json")
+INTERP = {"LINEAR": orc.LINEAR, "LOWER": orc.LOWER, "HIGHER": orc.HIGHER, "MIDPOINT": orc.MIDPOINT,
+          "NEAREST": orc.NEAREST}
+
+
+def quantile_request(case):
+    if case["agg"] == "MEDIAN":
+        return (AGG["MEDIAN"], 1)
+    return (AGG["QUANTILE"], (tuple(case["quantiles"]), INTERP[case["interpolation"]]))
+
+
+def run_quantile_kat(case, vt, groupby_fn):
+    """One MEDIAN / QUANTILE known answer: keys and the len(quantiles) values
+    of each group after sorting by key."""
+    keys = host_col(np.array(case["keys"], np.int32),
+                    None if case["key_valid"] is None else np.array(case["key_valid"], bool))
+    vals = host_col(np.array(case["vals"], np.dtype(vt)),
+                    None if case["val_valid"] is None else np.array(case["val_valid"], bool))
+    kcols, res = groupby_fn([keys], [(vals, [quantile_request(case)])], 0)
+    order = sort_order(kcols)
+    assert [int(kcols[0][0][i]) for i in order] == case["expect_keys"], case["name"]
+    nq = len(case["quantiles"]) if case["agg"] == "QUANTILE" else 1
+    v, valid = res[0][0]
+    valid = np.ones(len(v), bool) if valid is None else valid
+    assert v.dtype == np.float64 and len(v) == nq * len(order)
+    rows = (order[:, None] * nq + np.arange(nq)).reshape(-1)
+    exp_valid = case["expect_val_valid"] or [1] * len(case["expect_vals"])
+    for j, i in enumerate(rows):
+        assert bool(valid[i]) == bool(exp_valid[j]), (case["name"], vt, j)
+        if exp_valid[j]:  # the reference compares floating columns to a few ulps
+            assert math.isclose(v[i], case["expect_vals"][j], rel_tol=4e-16 * 4), (case["name"], vt, v[i],
+                                                                                  case["expect_vals"][j])
+
+
+@pytest.mark.parametrize("case", QKATS["cases"], ids=[c["name"] for c in QKATS["cases"]])
+def test_oracle_quantile_known_answers(case):
+    for vt in QKATS["types"]:
+        run_quantile_kat(case, vt, oracle_groupby)
+
+
+@pytest.mark.parametrize("interp", ["LINEAR", "LOWER", "HIGHER", "NEAREST", "MIDPOINT"])
+def test_oracle_quantiles_against_numpy(interp):
+    """numpy.quantile's methods of the same names (q * (n - 1) positions,
+    round-half-to-even for nearest) on random groups without NaN; MIDPOINT
+    against the exact (a + b) / 2 of each pair, rounded once."""
+    rng = np.random.default_rng(41)
+    n = 4000
+    k = rng.integers(0, 60, n).astype(np.int32)
+    for dt in (np.float64, np.float32, np.int32, np.int64, np.uint16):
+        v = (rng.standard_normal(n) * 1000).astype(dt) if np.dtype(dt).kind == "f" else \
+            rng.integers(-10**6 if np.dtype(dt).kind == "i" else 0, 10**6 if np.dtype(dt).kind == "i" else 60000,
+                         n).astype(dt)
+        valid = rng.random(n) > 0.1
+        qs = (0.0, 0.1, 0.25, 0.5, 0.77, 0.9, 1.0)
+        rep, out, ok = orc.group_quantiles([host_col(k)], host_col(v, valid), qs, INTERP[interp])
+        for g, r in enumerate(rep):
+            x = np.sort(v[(k == k[r]) & valid].astype(np.float64))
+            assert ok[g].all() == (len(x) > 0)
+            if not len(x):
+                continue
+            for j, q in enumerate(qs):
+                if interp == "MIDPOINT":
+                    pos = q * (len(x) - 1)
+                    a, b = x[int(math.floor(pos))], x[int(math.ceil(pos))]
+                    want = float((Fraction(float(a)) + Fraction(float(b))) / 2)
+                else:
+                    want = np.quantile(x, q, method=interp.lower())
+                assert math.isclose(out[g, j], want, rel_tol=1e-15, abs_tol=1e-300), (interp, dt, g, q, out[g, j], want)
+
+
+def test_oracle_quantiles_nan_sorts_last():
+    """cudf orders NaN after every number: a group {1, 2, NaN} has median 2,
+    {NaN, NaN, 5} median NaN, and LOWER of the 0.0 quantile is the minimum."""
+    k = host_col(np.array([1, 1, 1, 2, 2, 2], np.int32))
+    v = host_col(np.array([np.nan, 2.0, 1.0, np.nan, 5.0, np.nan]))
+    rep, out, ok = orc.group_quantiles([k], v, (0.0, 0.5, 1.0), orc.LOWER)
+    assert out[0].tolist()[:2] == [1.0, 2.0] and math.isnan(out[0][2])
+    assert out[1][0] == 5.0 and math.isnan(out[1][1]) and math.isnan(out[1][2])
