@@ -18,7 +18,8 @@ struct owned_column {
 
 struct device_allocator {
   mig_memory_resource mr{};
-  bool has_mr = false;
+  bool has_mr   = false;
+  bool borrowed = false;  // columns are the caller's buffers: allocate fails, deallocate does nothing
   hipStream_t stream = nullptr;
   void* allocate(size_t bytes);
   void deallocate(void* p, size_t bytes);
@@ -69,6 +70,21 @@ std::unique_ptr<mig_groupby_result> groupby_aggregate(mig_column_view const* key
                                                       mig_aggregation_request const* requests,
                                                       int32_t num_requests, hipStream_t stream,
                                                       mig_memory_resource const* mr, bool keys_sorted = false);
+
+// Memory of an allocation-free call: device scratch carved from a fixed
+// arena, and the caller's output buffers (num_keys key columns, then one
+// column per aggregation in request order; see mig_groupby_aggregate_into).
+struct call_memory {
+  uint8_t* arena     = nullptr;
+  size_t arena_bytes = 0;
+  mig_column* out_keys    = nullptr;
+  mig_column* out_results = nullptr;
+};
+// The group-by on the device-wide table with no allocation: returns the
+// number of groups and fills the caller's column structs.
+int32_t groupby_aggregate_into(mig_column_view const* keys, int32_t num_keys, int32_t null_policy,
+                               mig_aggregation_request const* requests, int32_t num_requests, hipStream_t stream,
+                               call_memory const& mem);
 
 // The sort-based group-by (sort_groupby.cpp): MEDIAN and QUANTILE, and
 // NUNIQUE / NTH_ELEMENT of unsorted keys, with the other requested kinds.

@@ -276,8 +276,8 @@ static int value_cmp(const orc_col* c, int64_t a, int64_t b)
  * predecessor; nulls equal each other, NaN equals NaN and -0.0 equals 0.0
  * (the row comparator's defaults).  Restated the same way: the group's rows
  * sorted by (group, null first, value) and the heads of equal runs counted. */
-static const orc_col* g_nu_col;
-static const int64_t* g_nu_group;
+static _Thread_local const orc_col* g_nu_col;  /* per thread: calls run concurrently */
+static _Thread_local const int64_t* g_nu_group;
 static int nunique_cmp(const void* pa, const void* pb)
 {
   int64_t a = *(const int64_t*)pa, b = *(const int64_t*)pb;
@@ -573,7 +573,9 @@ int64_t orc_groupby(const orc_col* keys, int32_t nkeys, int32_t null_policy,
  * major: out[g * nq + j] for quantile j of group g. */
 enum { I_LINEAR = 0, I_LOWER, I_HIGHER, I_MIDPOINT, I_NEAREST };
 
-static const orc_col* g_q_col;
+/* the column q_cmp orders by: one per thread, since the sharded oracle runs
+ * several calls at once */
+static _Thread_local const orc_col* g_q_col;
 static int q_cmp(const void* pa, const void* pb)
 {
   int64_t a = *(const int64_t*)pa, b = *(const int64_t*)pb;

@@ -414,7 +414,12 @@ def assert_parity(keys, requests, null_policy=0, gpu=None, stats=None):
                 continue
             tol = TOL_F32_SUM if col.type_id == orc.FLOAT32 else TOL_F64_SUM
             g64, o64 = gs.astype(np.float64), os_.astype(np.float64)
-            err = np.abs(g64 - o64)
+            # NaN (a NaN among the group's values) and infinities must match
+            # exactly; equal results pass whatever their bound
+            gnan, onan = np.isnan(g64), np.isnan(o64)
+            assert np.array_equal(gnan, onan), f"agg {kind}: NaN results differ at {np.flatnonzero(gnan != onan)[:5]}"
+            same = (g64 == o64) | gnan
+            err = np.where(same, 0.0, np.abs(np.where(same, 0.0, g64 - o64)))
             if stats is not None and col.type_id == orc.FLOAT64 and kind in (AGG["SUM"], AGG["MEAN"]):
                 rel = err / np.maximum(np.abs(o64), 1e-300)
                 stats[(r, kind)] = {"groups": int(len(rel)), "max_rel": float(rel.max(initial=0.0)),
@@ -425,14 +430,14 @@ def assert_parity(keys, requests, null_policy=0, gpu=None, stats=None):
                     bound = tol * scale_abs[ovalid_s] + np.finfo(gs.dtype).tiny
                 else:
                     bound = np.zeros_like(err)
-                assert (err <= bound).all(), f"SUM error {err.max()} above bound"
+                assert (same | (err <= bound)).all(), f"SUM error {err.max()} above bound"
             elif kind == AGG["MEAN"]:
                 if is_float_src:
                     # mean = sum / n: the sum's normwise bound divided by n
                     bound = tol * scale_abs[ovalid_s] / n_valid[ovalid_s] + 1e-300
                 else:
                     bound = 1e-15 * np.abs(o64) + 1e-300
-                assert (err <= bound).all(), f"MEAN error {err.max()} above bound"
+                assert (same | (err <= bound)).all(), f"MEAN error {err.max()} above bound"
             elif kind in (AGG["VARIANCE"], AGG["STD"]):
                 bound = TOL_VAR * np.abs(o64) + 1e-300
                 if is_float_src:
@@ -441,12 +446,12 @@ def assert_parity(keys, requests, null_policy=0, gpu=None, stats=None):
                     if kind == AGG["STD"]:
                         floor = np.sqrt(floor)
                     bound = bound + floor
-                assert (err <= bound).all(), f"VAR/STD error {err.max()} above bound"
+                assert (same | (err <= bound)).all(), f"VAR/STD error {err.max()} above bound"
             elif kind == AGG["PRODUCT"]:
                 # unordered products of at most ~80 factors: |error| <= 80 * eps * |product|
-                assert (err <= tol * np.abs(o64) + 1e-300).all(), f"PRODUCT error {err.max()}"
+                assert (same | (err <= tol * np.abs(o64) + 1e-300)).all(), f"PRODUCT error {err.max()}"
             elif kind == AGG["SUM_OF_SQUARES"]:
-                assert (err <= tol * np.abs(o64) + 1e-300).all()
+                assert (same | (err <= tol * np.abs(o64) + 1e-300)).all()
             else:
                 raise AssertionError(f"unhandled kind {kind}")
     return len(og)

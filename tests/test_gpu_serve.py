@@ -132,7 +132,7 @@ def test_served_query_error_is_its_own():
     stream = make_stream(53, 6, 20_000)
     queries = [to_query(*s) for s in stream]
     bad = to_query(*stream[0])
-    bad.requests = [GroupByRequest(bad.requests[0].values, [Aggregation(13)])]  # MEDIAN: not a hash aggregation
+    bad.requests = [GroupByRequest(bad.requests[0].values, [Aggregation(6)])]  # ANY: not a group-by aggregation
     with GroupByServer(workers=2) as srv:
         futs = [srv.submit(q) for q in queries[:3]] + [srv.submit(bad)] + [srv.submit(q) for q in queries[3:]]
         with pytest.raises(_lib.LogicError):

@@ -668,3 +668,32 @@ def test_oracle_quantiles_nan_sorts_last():
     rep, out, ok = orc.group_quantiles([k], v, (0.0, 0.5, 1.0), orc.LOWER)
     assert out[0].tolist()[:2] == [1.0, 2.0] and math.isnan(out[0][2])
     assert out[1][0] == 5.0 and math.isnan(out[1][1]) and math.isnan(out[1][2])
+
+
+def test_oracle_sort_kinds_thread_safe():
+    """The sharded parity harness calls the oracle from several threads: the
+    quantile and NUNIQUE sorts must not share state between concurrent calls
+    (each call's results equal its serial result)."""
+    import concurrent.futures as cf
+    rng = np.random.default_rng(12)
+    shards = []
+    for i in range(6):
+        n = 60_000 + 10_000 * i
+        k = orc.HostColumn.from_numpy(rng.integers(0, 6_000, n).astype(np.int64))
+        v = orc.HostColumn.from_numpy(rng.standard_normal(n))
+        w = orc.HostColumn.from_numpy(rng.integers(0, 40, n).astype(np.int32))
+        shards.append((k, v, w))
+
+    def run(kvw):
+        k, v, w = kvw
+        q = orc.group_quantiles([k], v, (0.1, 0.5, 0.9), orc.NEAREST)
+        g = orc.groupby([k], [(w, [(AGG["NUNIQUE"], 0)])])
+        return q, g
+
+    serial = [run(s) for s in shards]
+    for _ in range(3):
+        with cf.ThreadPoolExecutor(6) as ex:
+            conc = list(ex.map(run, shards))
+        for (q1, g1), (q2, g2) in zip(serial, conc):
+            assert np.array_equal(q1[1], q2[1]) and np.array_equal(q1[0], q2[0])
+            assert np.array_equal(g1[1][0][0][0], g2[1][0][0][0])
