@@ -246,9 +246,12 @@ class StrKeys:
         pos = np.arange(end - start) - np.repeat(self.offsets[:-1] - start, lens)
         mult = _position_multipliers(int(lens.max()))
         prod = self.chars[start:end].astype(np.uint64) * mult[pos]
-        idx = np.minimum(self.offsets[:-1] - start, end - start - 1)
-        sums = np.add.reduceat(prod, idx)
-        return h ^ np.where(lens > 0, sums, np.uint64(0))
+        # per-string sums as differences of a prefix sum (mod 2^64), so a
+        # string's value does not depend on its neighbours
+        csum = np.zeros(end - start + 1, np.uint64)
+        np.cumsum(prod, out=csum[1:])
+        sums = csum[self.offsets[1:] - start] - csum[self.offsets[:-1] - start]
+        return h ^ sums
 
 
 _MULTS = np.zeros(0, np.uint64)

@@ -697,3 +697,18 @@ def test_oracle_sort_kinds_thread_safe():
         for (q1, g1), (q2, g2) in zip(serial, conc):
             assert np.array_equal(q1[1], q2[1]) and np.array_equal(q1[0], q2[0])
             assert np.array_equal(g1[1][0][0][0], g2[1][0][0][0])
+
+
+def test_string_key_fingerprint_ignores_position():
+    """The parity harness orders string keys by a fingerprint: equal strings
+    must get equal values wherever they sit (before trailing empty strings,
+    first, last)."""
+    from tests.gbutil import StrKeys
+    rng = np.random.default_rng(3)
+    words = [b"", b"a", b"xxxxxxxx", b"y" * 65, b"q" * 9]
+    for _ in range(50):
+        strs = [words[i] for i in rng.integers(0, len(words), 7)]
+        perm = rng.permutation(7)
+        fa = StrKeys.from_list(strs).fingerprint()
+        fb = StrKeys.from_list([strs[i] for i in perm]).fingerprint()
+        assert np.array_equal(fa[perm], fb)
