@@ -124,6 +124,11 @@ SIGNATURES = {
                                                     ctypes.POINTER(AggregationRequest), ctypes.c_int32,
                                                     ctypes.c_void_p, ctypes.POINTER(MemoryResource),
                                                     ctypes.POINTER(ctypes.c_void_p)]),
+    "mig_groupby_aggregate_into": (ctypes.c_int, [ctypes.POINTER(ColumnView), ctypes.c_int32, ctypes.c_int32,
+                                                  ctypes.POINTER(AggregationRequest), ctypes.c_int32,
+                                                  ctypes.POINTER(OutColumn), ctypes.POINTER(OutColumn),
+                                                  ctypes.POINTER(ctypes.c_int32), ctypes.c_void_p]),
+    "mig_groupby_into_scratch_bytes": (ctypes.c_size_t, []),
     "mig_result_num_groups": (ctypes.c_int32, [ctypes.c_void_p]),
     "mig_result_num_keys": (ctypes.c_int32, [ctypes.c_void_p]),
     "mig_result_num_requests": (ctypes.c_int32, [ctypes.c_void_p]),

@@ -32,7 +32,7 @@ COMMON_FLAGS = [
 
 SOURCES = ["groupby_kernels.hip", "partition_kernels.hip", "partition_aggregate.hip", "batch_kernels.hip",
            "partitioning_kernels.hip", "groupby.cpp", "batch.cpp", "partitioning.cpp", "capi.cpp",
-           "sort_kernels.hip", "sort_groupby.cpp"]
+           "sort_kernels.hip", "sort_groupby.cpp", "groupby_noalloc.hip"]
 
 
 def _compile(src: str) -> str:

@@ -123,6 +123,26 @@ mig_status mig_groupby_aggregate(const mig_column_view* keys, int32_t num_keys, 
   });
 }
 
+mig_status mig_groupby_aggregate_into(const mig_column_view* keys, int32_t num_keys, int32_t null_policy,
+                                      const mig_aggregation_request* requests, int32_t num_requests,
+                                      mig_column* out_keys, mig_column* out_results, int32_t* num_groups,
+                                      void* stream)
+{
+  return guarded([&] {
+    if (!num_groups) mig::fail(MIG_ERROR_LOGIC, "null group count");
+    *num_groups = 0;
+    if (num_keys > 0 && (!keys || !out_keys)) mig::fail(MIG_ERROR_LOGIC, "null key or key output array");
+    if (num_requests > 0 && !requests) mig::fail(MIG_ERROR_LOGIC, "null request array");
+    int64_t outs = 0;
+    for (int32_t r = 0; r < num_requests; r++) outs += requests[r].num_aggregations > 0 ? requests[r].num_aggregations : 0;
+    if (outs > 0 && !out_results) mig::fail(MIG_ERROR_LOGIC, "null result column array");
+    *num_groups = mig::groupby_aggregate_noalloc(keys, num_keys, null_policy, requests, num_requests, out_keys,
+                                                 out_results, static_cast<hipStream_t>(stream));
+  });
+}
+
+size_t mig_groupby_into_scratch_bytes(void) { return mig::noalloc_scratch_bytes(); }
+
 mig_status mig_groupby_aggregate_sorted(const mig_column_view* keys, int32_t num_keys, int32_t null_policy,
                                         const mig_aggregation_request* requests, int32_t num_requests,
                                         void* stream, const mig_memory_resource* mr,

@@ -86,6 +86,13 @@ int32_t groupby_aggregate_into(mig_column_view const* keys, int32_t num_keys, in
                                mig_aggregation_request const* requests, int32_t num_requests, hipStream_t stream,
                                call_memory const& mem);
 
+// mig_groupby_aggregate_into (groupby_noalloc.hip): the same on the
+// library's fixed per-device scratch array of noalloc_scratch_bytes().
+size_t noalloc_scratch_bytes();
+int32_t groupby_aggregate_noalloc(mig_column_view const* keys, int32_t num_keys, int32_t null_policy,
+                                  mig_aggregation_request const* requests, int32_t num_requests,
+                                  mig_column* out_keys, mig_column* out_results, hipStream_t stream);
+
 // The sort-based group-by (sort_groupby.cpp): MEDIAN and QUANTILE, and
 // NUNIQUE / NTH_ELEMENT of unsorted keys, with the other requested kinds.
 std::unique_ptr<mig_groupby_result> sort_groupby(mig_column_view const* keys, int32_t num_keys, int32_t null_policy,

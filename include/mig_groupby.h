@@ -309,6 +309,42 @@ mig_status mig_groupby_aggregate_sorted(const mig_column_view* keys,
                                         const mig_memory_resource* mr,
                                         mig_groupby_result** out);
 
+/* The hash group-by with no device allocation: no counterpart in the
+ * reference (whose aggregate() allocates through its memory resource,
+ * cpp/src/groupby/hash/groupby.cu:404, 478).  For callers that cannot
+ * allocate device memory: code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc.  Same requests, results and errors as
+ * mig_groupby_aggregate on its device-wide hash table path, for fixed-width
+ * key and value columns and the hash aggregation kinds (sort-based kinds,
+ * strings and pre-sorted keys return MIG_ERROR_LOGIC).
+ *   - Scratch: the library's fixed device array of
+ *     mig_groupby_into_scratch_bytes() bytes (128 MiB) per device, held until
+ *     the process ends.  A call that needs more (a hash table for very many
+ *     groups) returns MIG_ERROR_OVERFLOW without touching the outputs.  Every
+ *     call on one device shares that array: calls that overlap on a device
+ *     (several streams or threads at once) race and must not be made.
+ *   - Outputs: the caller's buffers.  `out_keys` holds num_keys columns and
+ *     `out_results` one column per aggregation, in request order (requests
+ *     flattened).  On entry each column's `data` / `data_bytes` give a buffer
+ *     for the result rows and `null_mask` / `mask_bytes` one for whole 64-bit
+ *     validity words (8 * ceil(rows / 64) bytes); a buffer too small for the
+ *     groups found returns MIG_ERROR_OVERFLOW (rows = the key count bounds
+ *     the groups).  On success every column's type_id, size (= *num_groups)
+ *     and null_count are set, and null_mask is set to NULL when the result has
+ *     no validity mask (the caller's buffer is left alone).
+ * Blocks until the results are complete on `stream`. */
+mig_status mig_groupby_aggregate_into(const mig_column_view* keys,
+                                      int32_t num_keys,
+                                      int32_t null_policy,
+                                      const mig_aggregation_request* requests,
+                                      int32_t num_requests,
+                                      mig_column* out_keys,
+                                      mig_column* out_results,
+                                      int32_t* num_groups,
+                                      void* stream);
+/* Bytes of the fixed device scratch mig_groupby_aggregate_into uses. */
+size_t mig_groupby_into_scratch_bytes(void);
+
 int32_t mig_result_num_groups(const mig_groupby_result* r);
 int32_t mig_result_num_keys(const mig_groupby_result* r);
 int32_t mig_result_num_requests(const mig_groupby_result* r);
