@@ -17,6 +17,7 @@
 #include <algorithm>
 #include <array>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <atomic>
 #include <mutex>
@@ -472,27 +473,7 @@ int32_t minmax_field_kind(int32_t st, bool is_min)
     default: return is_min ? FK_MIN_F64 : FK_MAX_F64;
   }
 }
-int32_t field_width(int32_t fk)
-{
-  switch (fk) {
-    case FK_SUM_F32:
-    case FK_SUMSQ_F32:
-    case FK_MIN_S32:
-    case FK_MAX_S32:
-    case FK_MIN_U32:
-    case FK_MAX_U32:
-    case FK_MIN_F32:
-    case FK_MAX_F32:
-    case FK_COUNT:
-    case FK_ARGMIN:
-    case FK_ARGMAX:
-    case FK_NUNIQUE:
-    case FK_NUNIQUE_NULLS:
-    case FK_NTH:
-    case FK_PROD_F32: return 4;
-    default: return 8;
-  }
-}
+int32_t field_width(int32_t fk) { return field_kind_width(fk); }
 
 uint64_t next_pow2(uint64_t x)
 {
@@ -1217,6 +1198,11 @@ int64_t groupby_call::aggregate_partitioned()
     if (vals_[v].has_nulls) n_flagged++;
   }
   rpa_ok = rpa_ok && n_flagged <= RPA_MAX_FLAGGED;
+  {  // the aggregate kernel keeps its output plan in registers
+    int n_outs = 0;
+    for (int r = 0; r < num_requests_; r++) n_outs += requests_[r].num_aggregations;
+    rpa_ok = rpa_ok && n_outs <= RPA_PLAN_OUTS;
+  }
   // the aggregate kernel keeps its accumulator plan in registers
   {
     int ncols = 0;
@@ -1226,16 +1212,22 @@ int64_t groupby_call::aggregate_partitioned()
   }
   // LDS of the aggregating workgroup for S hash slots: keys, S + 2 rows of
   // every accumulator, the compaction map and the shared counters.
-  auto agg_lds_bytes = [&](int32_t slots) {
+  auto agg_lds_bytes = [&](int32_t slots, bool wide) {
     int32_t bytes = slots * 8;
-    for (int f = 0; f < nf_; f++) bytes += ((slots + 2) * p.fields[f].width + 7) & ~7;
+    for (int f = 0; f < nf_; f++) bytes += ((slots + 2) * rpa_lds_width(p.fields[f].kind, wide) + 7) & ~7;
     if (p.need_count_all) bytes += ((slots + 2) * 4 + 7) & ~7;
     bytes += ((slots + 2) * 2 + 7) & ~7;
     return bytes + 128;
   };
-  int32_t S_slots = RPA_LDS_SLOTS;
-  while (S_slots > 64 && agg_lds_bytes(S_slots) > RPA_LDS_BUDGET) S_slots /= 2;
-  rpa_ok = rpa_ok && agg_lds_bytes(S_slots) <= RPA_LDS_BUDGET;
+  auto slots_for = [&](bool wide) {
+    int32_t slots = RPA_LDS_SLOTS;
+    while (slots > 64 && agg_lds_bytes(slots, wide) > RPA_LDS_BUDGET) slots /= 2;
+    return slots;
+  };
+  // float32 sums in float64 where that costs no hash slots
+  int32_t S_slots   = slots_for(false);
+  bool const wide   = slots_for(true) == S_slots;
+  rpa_ok = rpa_ok && agg_lds_bytes(S_slots, wide) <= RPA_LDS_BUDGET;
   int32_t const max_fill = int32_t(0.85 * S_slots);
   bool use_rpa = false;
   if (forced == MIG_STRATEGY_PARTITIONED) use_rpa = rpa_ok;
@@ -1248,9 +1240,10 @@ int64_t groupby_call::aggregate_partitioned()
   rpa_params R{};
   // LDS offsets (8-byte aligned)
   int32_t off = S_slots * 8;
+  R.wide_f32 = wide ? 1 : 0;
   for (int f = 0; f < nf_; f++) {
     p.fields[f].lds_off = off;
-    off += ((S_slots + 2) * p.fields[f].width + 7) & ~7;
+    off += ((S_slots + 2) * rpa_lds_width(p.fields[f].kind, wide) + 7) & ~7;
   }
   R.count_all_off = -1;
   if (p.need_count_all) {
@@ -1275,6 +1268,30 @@ int64_t groupby_call::aggregate_partitioned()
   }
   for (int f = 0; f < nf_; f++)
     R.plan_fd[f] = uint32_t(p.fields[f].kind) | (uint32_t(p.fields[f].lds_off) << 8);
+  // LDS offset of a count: a count field, or the group row counts (-2)
+  auto count_off = [&](int32_t f) { return f == -2 ? R.count_all_off : (f >= 0 ? p.fields[f].lds_off : -1); };
+  R.m2_n = 0;
+  for (int f = 0; f < nf_; f++) {
+    if (p.fields[f].kind != FK_M2) continue;
+    int const j      = R.m2_n++;
+    field_desc const& SF = p.fields[p.fields[f].aux_sum];
+    R.m2_off[j]     = p.fields[f].lds_off;
+    R.m2_sum[j]     = uint32_t(SF.kind) | (uint32_t(SF.lds_off) << 8);
+    R.m2_cnt_off[j] = count_off(p.fields[f].aux_count);
+    R.m2_storage[j] = p.vals[p.fields[f].vcol].storage;
+  }
+  // the output plan (fixed by the requests; the buffers come from P.outs)
+  auto fill_out_plan = [&]() {
+    R.out_n = p.num_outs;
+    for (int o = 0; o < p.num_outs; o++) {
+      out_desc const& O = p.outs[o];
+      int32_t const fk  = O.field >= 0 ? p.fields[O.field].kind : 0;
+      R.out_op[o]   = uint32_t(O.kind) | (uint32_t(fk) << 8) | (uint32_t(O.out_width) << 16);
+      R.out_foff[o] = O.kind == OK_COUNT_ALL ? R.count_all_off : (O.field >= 0 ? p.fields[O.field].lds_off : -1);
+      R.out_coff[o] = O.count_field == -1 ? -1 : count_off(O.count_field);
+      R.out_ddof[o] = O.ddof;
+    }
+  };
   // scatter staging layout: a tile of keys, copied values, flags and bins
   int32_t rec = 8 + 1 + 2;
   for (int v = 0; v < p.num_vals; v++)
@@ -1337,8 +1354,8 @@ int64_t groupby_call::aggregate_partitioned()
     R.hist         = speculative ? nullptr : S->get<uint32_t>(B);
     R.hist_partial = speculative ? nullptr : S->get<uint32_t>(size_t(R.hist_blocks) * B);
     R.bin_start    = S->get<int64_t>(B + 1);
-    R.cursor1      = S->get<unsigned long long>(F1);
-    R.cursor2      = S->get<unsigned long long>(B);
+    R.cursor1      = S->get<unsigned long long>(F1 * RPA_CURSOR_STRIDE);
+    R.cursor2      = S->get<unsigned long long>(B * RPA_CURSOR_STRIDE);
     R.tile_start1  = S->get<int64_t>(F1 + 1);
     R.first_start  = S->get<int64_t>(F1);
     int32_t* d_small = S->get<int32_t>(4);
@@ -1386,6 +1403,7 @@ int64_t groupby_call::aggregate_partitioned()
     stats_[STAT_STATE_BYTES] = double(part_bytes);
     R.out_capacity = uint64_t(rows_cap);
     R.out_rep      = p.key_mode == KM_GENERAL ? S->get<uint32_t>(std::max<int64_t>(1, rows_cap)) : nullptr;
+    fill_out_plan();
     upload(0);
     MIG_HIP_TRY(hipMemcpyAsync(dR, &R, sizeof(rpa_params), hipMemcpyHostToDevice, stream_));
     T->mark(10);

@@ -59,6 +59,43 @@ enum field_kind : int32_t {
   FK_NTH             // row of the group's n-th value (u32, 0xFFFFFFFF = none); aux_sum = n, aux_count = include
 };
 
+// Bytes of one accumulator of kind `fk`.
+__host__ __device__ inline int32_t field_kind_width(int32_t fk)
+{
+  switch (fk) {
+    case FK_SUM_F32:
+    case FK_SUMSQ_F32:
+    case FK_MIN_S32:
+    case FK_MAX_S32:
+    case FK_MIN_U32:
+    case FK_MAX_U32:
+    case FK_MIN_F32:
+    case FK_MAX_F32:
+    case FK_COUNT:
+    case FK_ARGMIN:
+    case FK_ARGMAX:
+    case FK_NUNIQUE:
+    case FK_NUNIQUE_NULLS:
+    case FK_NTH:
+    case FK_PROD_F32: return 4;
+    default: return 8;
+  }
+}
+
+// Bytes of an accumulator in the partitioned path's LDS table.  With
+// `wide_f32`, float32 sums and sums of squares accumulate in float64 there
+// (LDS float64 atomic adds run several times faster than float32 ones, and
+// the sum is rounded to float32 once at the end); everything else keeps its
+// own width.
+__host__ __device__ inline bool rpa_wide_sum(int32_t fk, bool wide_f32)
+{
+  return wide_f32 && (fk == FK_SUM_F32 || fk == FK_SUMSQ_F32);
+}
+__host__ __device__ inline int32_t rpa_lds_width(int32_t fk, bool wide_f32)
+{
+  return rpa_wide_sum(fk, wide_f32) ? 8 : field_kind_width(fk);
+}
+
 struct field_desc {
   void* ptr;        // global accumulator of slot 0 (cap + 2 slots)
   int32_t kind;     // field_kind
@@ -155,6 +192,8 @@ constexpr int RPA_HIST_BLOCK = 1024;    // histogram kernel
 constexpr int RPA_AGG_BLOCK = 512;
 constexpr int RPA_PLAN_COLS   = 4;   // value columns with accumulators (partitioned path)
 constexpr int RPA_PLAN_FIELDS = 16;  // accumulator fields (partitioned path)
+constexpr int RPA_PLAN_OUTS   = 64;  // result columns (partitioned path)
+constexpr int RPA_CURSOR_STRIDE = 16;  // u64 elements between scatter cursors (128 bytes)
 
 struct rpa_params {
   int32_t log_bins;   // final bins B = 1 << log_bins
@@ -180,8 +219,11 @@ struct rpa_params {
   uint32_t* hist_partial;            // [hist_blocks][B] per-block histograms
   uint32_t* hist;                    // [B] rows per final bin
   int64_t* bin_start;                // [B + 1]
-  unsigned long long* cursor1;       // [F1] next free row of each first-level bin (buffer A)
-  unsigned long long* cursor2;       // [B] next free row of each final bin (buffer B)
+  // next free row of each first-level bin (buffer A) / final bin (buffer
+  // B), one per RPA_CURSOR_STRIDE elements so that every cursor has a cache
+  // line of its own (each tile of a scatter reserves ranges on many of them)
+  unsigned long long* cursor1;       // [F1 * RPA_CURSOR_STRIDE]
+  unsigned long long* cursor2;       // [B * RPA_CURSOR_STRIDE]
   int64_t* tile_start1;              // [F1 + 1] scatter-2 tiles before each first-level bin
   int64_t* first_start;              // [F1] first record of each first-level bin (buffer A)
   // Rows each first-level / final bin may hold.  Exact layouts (from the
@@ -211,6 +253,24 @@ struct rpa_params {
   int32_t plan_col[RPA_PLAN_COLS];
   int32_t plan_fb[RPA_PLAN_COLS + 1];
   uint32_t plan_fd[RPA_PLAN_FIELDS];
+  // Variance finalisation plan: entry j describes M2 field j by its LDS
+  // offset, its column's sum field (kind | lds_off << 8), the LDS offset of
+  // its count (a count field or the group row counts) and the source storage.
+  int32_t wide_f32;  // float32 sums accumulate in float64 (rpa_lds_width)
+  int32_t m2_n;
+  int32_t m2_off[RPA_PLAN_COLS];
+  uint32_t m2_sum[RPA_PLAN_COLS];
+  int32_t m2_cnt_off[RPA_PLAN_COLS];
+  int32_t m2_storage[RPA_PLAN_COLS];
+  // Output plan, read into registers once per workgroup: result column o is
+  // out_op[o] = out kind | primary field kind << 8 | output width << 16, from
+  // the field at LDS byte offset out_foff[o] (-1: none) and the count at
+  // out_coff[o] (-1: always valid), VARIANCE/STD with out_ddof[o].
+  int32_t out_n;
+  uint32_t out_op[RPA_PLAN_OUTS];
+  int32_t out_foff[RPA_PLAN_OUTS];
+  int32_t out_coff[RPA_PLAN_OUTS];
+  int32_t out_ddof[RPA_PLAN_OUTS];
 };
 
 // ---- launchers (groupby_kernels.hip) --------------------------------------

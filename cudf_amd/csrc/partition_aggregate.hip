@@ -23,7 +23,7 @@ namespace mig {
 
 constexpr int WG_SCOPE = __HIP_MEMORY_SCOPE_WORKGROUP;
 
-constexpr int RPA_CACHED = 6;     // rows per thread per chunk; their slots stay in registers
+constexpr int RPA_CACHED = 4;     // rows per thread per chunk; their slots stay in registers
 constexpr int RPA_OUT_BATCH = 4;  // output columns whose LDS reads are issued together
 // Probes before a row's key counts as not fitting: the table is filled to at
 // most ~85%, where a double-hashing probe longer than this has probability
@@ -48,7 +48,7 @@ __device__ __forceinline__ double canon_double(int32_t st, uint64_t c);
 
 __device__ __forceinline__ void acc_rows_lds(int32_t kind, int32_t lds_off, int32_t st, uint8_t* lds,
                                              int const (&slot)[RPA_CACHED], uint32_t ok,
-                                             uint64_t const (&c)[RPA_CACHED])
+                                             uint64_t const (&c)[RPA_CACHED], bool wide_f32)
 {
   uint8_t* const base = lds + lds_off;
   auto u64 = [&](int k) { return reinterpret_cast<uint64_t*>(base) + slot[k]; };
@@ -58,8 +58,13 @@ __device__ __forceinline__ void acc_rows_lds(int32_t kind, int32_t lds_off, int3
   constexpr int R = __ATOMIC_RELAXED;
   switch (kind) {
     case FK_SUM_I64: for_rows(ok, [&](int k) { __hip_atomic_fetch_add(u64(k), c[k], R, WG_SCOPE); }); break;
-    case FK_SUM_F32:
-      for_rows(ok, [&](int k) { __hip_atomic_fetch_add(reinterpret_cast<float*>(u32(k)), fl(k), R, WG_SCOPE); });
+    case FK_SUM_F32:  // in float64 when wide (rounded to float32 at output)
+      if (wide_f32)
+        for_rows(ok, [&](int k) {
+          __hip_atomic_fetch_add(reinterpret_cast<double*>(u64(k)), double(fl(k)), R, WG_SCOPE);
+        });
+      else
+        for_rows(ok, [&](int k) { __hip_atomic_fetch_add(reinterpret_cast<float*>(u32(k)), fl(k), R, WG_SCOPE); });
       break;
     case FK_SUM_F64:
       for_rows(ok, [&](int k) { __hip_atomic_fetch_add(reinterpret_cast<double*>(u64(k)), db(k), R, WG_SCOPE); });
@@ -68,9 +73,15 @@ __device__ __forceinline__ void acc_rows_lds(int32_t kind, int32_t lds_off, int3
       for_rows(ok, [&](int k) { __hip_atomic_fetch_add(u64(k), c[k] * c[k], R, WG_SCOPE); });
       break;
     case FK_SUMSQ_F32:
-      for_rows(ok, [&](int k) {
-        __hip_atomic_fetch_add(reinterpret_cast<float*>(u32(k)), fl(k) * fl(k), R, WG_SCOPE);
-      });
+      if (wide_f32)
+        for_rows(ok, [&](int k) {
+          double const x = double(fl(k));
+          __hip_atomic_fetch_add(reinterpret_cast<double*>(u64(k)), x * x, R, WG_SCOPE);
+        });
+      else
+        for_rows(ok, [&](int k) {
+          __hip_atomic_fetch_add(reinterpret_cast<float*>(u32(k)), fl(k) * fl(k), R, WG_SCOPE);
+        });
       break;
     case FK_SUMSQ_F64:
       for_rows(ok, [&](int k) {
@@ -132,16 +143,11 @@ __device__ __forceinline__ void acc_rows_lds(int32_t kind, int32_t lds_off, int3
   }
 }
 
-__device__ __forceinline__ uint64_t lds_field_bits(field_desc const& F, uint8_t* lds, int id)
-{
-  return F.width == 8 ? *lds_at<uint64_t>(lds, F.lds_off, id) : *lds_at<uint32_t>(lds, F.lds_off, id);
-}
-
-__device__ __forceinline__ double lds_field_double(field_desc const& F, uint8_t* lds, int id)
+__device__ __forceinline__ double lds_field_double(field_desc const& F, uint8_t* lds, int id, bool wide_f32)
 {
   switch (F.kind) {
     case FK_SUM_I64: return double(*lds_at<int64_t>(lds, F.lds_off, id));
-    case FK_SUM_F32: return double(*lds_at<float>(lds, F.lds_off, id));
+    case FK_SUM_F32: return wide_f32 ? *lds_at<double>(lds, F.lds_off, id) : double(*lds_at<float>(lds, F.lds_off, id));
     case FK_SUM_F64: return *lds_at<double>(lds, F.lds_off, id);
     case FK_COUNT: return double(*lds_at<uint32_t>(lds, F.lds_off, id));
     default: return 0.0;
@@ -149,11 +155,11 @@ __device__ __forceinline__ double lds_field_double(field_desc const& F, uint8_t*
 }
 
 // A sum or count partial, given as its raw bits, as double.
-__device__ __forceinline__ double field_bits_double(int32_t kind, uint64_t bits)
+__device__ __forceinline__ double field_bits_double(int32_t kind, uint64_t bits, bool wide_f32)
 {
   switch (kind) {
     case FK_SUM_I64: return double(int64_t(bits));
-    case FK_SUM_F32: return double(__uint_as_float(uint32_t(bits)));
+    case FK_SUM_F32: return wide_f32 ? __longlong_as_double(int64_t(bits)) : double(__uint_as_float(uint32_t(bits)));
     case FK_SUM_F64: return __longlong_as_double(int64_t(bits));
     case FK_COUNT: return double(uint32_t(bits));
     default: return 0.0;
@@ -259,6 +265,8 @@ __device__ __forceinline__ double canon_double(int32_t st, uint64_t c)
 // v_readlane (a uniform index into a register array would go to memory).
 struct acc_plan {
   int32_t ncols;
+  int32_t nfields;
+  bool wide_f32;
   int32_t key_storage;
   int32_t count_all_off;
   uint32_t cd_lane;     // lane i < ncols: storage | (flag bit + 1) << 8 | first field << 16 | end << 24
@@ -273,6 +281,8 @@ __device__ __forceinline__ acc_plan load_plan(gb_params const& P, rpa_params con
 {
   acc_plan A;
   A.ncols         = R.plan_ncols;
+  A.nfields       = P.num_fields;
+  A.wide_f32      = R.wide_f32 != 0;
   A.key_storage   = P.key_storage;
   A.count_all_off = R.count_all_off;
   A.keys          = gp(static_cast<uint64_t const*>(R.keysB));
@@ -294,6 +304,39 @@ __device__ __forceinline__ uint32_t lane_value(uint32_t v, int i)
   return uint32_t(__builtin_amdgcn_readlane(int(v), i));
 }
 
+// Raw values of the first RPA_AGG_PREFETCH value columns of a chunk, loaded
+// together with the chunk's keys and flags so that the chunk waits for memory
+// once, not once per column.
+constexpr int RPA_AGG_PREFETCH = 1;
+struct chunk_prefetch {
+  uint64_t c[RPA_AGG_PREFETCH][RPA_CACHED];
+};
+
+// A chunk's records as loaded: keys, flags (byte k = row k) and the
+// prefetched value columns.
+struct chunk_in {
+  uint64_t key[RPA_CACHED];
+  uint64_t flp;
+  chunk_prefetch pf;
+};
+
+__device__ __forceinline__ gmem<uint8_t const>* plan_column(acc_plan const& A, int i)
+{
+  return reinterpret_cast<gmem<uint8_t const>*>((uint64_t(lane_value(A.cp_hi_lane, i)) << 32) |
+                                                lane_value(A.cp_lo_lane, i));
+}
+
+__device__ __forceinline__ void prefetch_values(acc_plan const& A, int64_t r0, int64_t e, chunk_prefetch& pf)
+{
+#pragma unroll
+  for (int i = 0; i < RPA_AGG_PREFETCH; i++) {
+    if (i >= A.ncols) break;
+    uint32_t const cd = lane_value(A.cd_lane, i);
+    load_chunk(plan_column(A, i), storage_width(int(cd & 0xffu)), r0, e, pf.c[i]);
+  }
+}
+
+
 // Slots of the rows [r0 + tid + k * block) of one chunk: S = null key,
 // S + 1 = key equal to EMPTY_KEY, else the row's LDS table slot (inserted
 // when INSERT); -1 when the row is past the bin or its key is not found.
@@ -312,19 +355,36 @@ __device__ __forceinline__ bool same_general(gb_params const& P, uint64_t a, uin
   return (a >> 32) == (b >> 32) && rows_equal(P, int64_t(uint32_t(a)), int64_t(uint32_t(b)));
 }
 
-template <bool INSERT, bool GENERAL>
-__device__ __forceinline__ void chunk_slots(gb_params const& P, acc_plan const& A, uint64_t* tkeys,
-                                            rpa_misc* misc, int S, int64_t r0, int64_t e,
-                                            int (&slot)[RPA_CACHED], uint64_t& flp)
+// Issues every load of a chunk (keys, flags and, with PREFETCH, the first
+// value columns) so that the chunk waits for memory once.
+template <bool PREFETCH>
+__device__ __forceinline__ void chunk_load(acc_plan const& A, int64_t r0, int64_t e, chunk_in& in)
 {
-  uint64_t key[RPA_CACHED];
   uint8_t fl[RPA_CACHED];
 #pragma unroll
   for (int k = 0; k < RPA_CACHED; k++) {  // rows past the bin read its last row (e > r0 - tid)
     int64_t const r  = r0 + int64_t(k) * RPA_AGG_BLOCK;
     int64_t const rc = r < e ? r : e - 1;
     fl[k]            = A.flags[rc];
-    key[k]           = A.keys[rc];
+    in.key[k]        = A.keys[rc];
+  }
+  if (PREFETCH) prefetch_values(A, r0, e, in.pf);
+  in.flp = 0;
+#pragma unroll
+  for (int k = 0; k < RPA_CACHED; k++) in.flp |= uint64_t(fl[k]) << (8 * k);
+}
+
+template <bool INSERT, bool GENERAL>
+__device__ __forceinline__ void chunk_slots(gb_params const& P, acc_plan const& A, uint64_t* tkeys,
+                                            rpa_misc* misc, int S, int64_t r0, int64_t e, chunk_in const& in,
+                                            int (&slot)[RPA_CACHED])
+{
+  uint64_t key[RPA_CACHED];
+  uint8_t fl[RPA_CACHED];
+#pragma unroll
+  for (int k = 0; k < RPA_CACHED; k++) {
+    key[k] = in.key[k];
+    fl[k]  = uint8_t(in.flp >> (8 * k));
   }
   uint32_t pending = 0;
   uint32_t pos[RPA_CACHED], step[RPA_CACHED];
@@ -378,16 +438,14 @@ __device__ __forceinline__ void chunk_slots(gb_params const& P, acc_plan const& 
     else if (INSERT) __hip_atomic_store(&misc->overflow, 1, __ATOMIC_RELAXED, WG_SCOPE);  // table full
   }
   if (INSERT && inserted) __hip_atomic_fetch_add(&misc->fill, inserted, __ATOMIC_RELAXED, WG_SCOPE);
-  flp = 0;
-#pragma unroll
-  for (int k = 0; k < RPA_CACHED; k++) flp |= uint64_t(fl[k]) << (8 * k);
 }
 
-// Pass A of one chunk: every accumulator, column by column, with the
-// chunk's values of a column loaded together and each accumulator applied to
-// all rows of the chunk at once.
+// Pass A of one chunk: every accumulator, column by column, each accumulator
+// applied to all rows of the chunk at once (the first columns' values come
+// prefetched, the others are loaded here).
 __device__ __forceinline__ void chunk_accumulate(acc_plan const& A, uint8_t* lds, int64_t r0, int64_t e,
-                                                 int const (&slot)[RPA_CACHED], uint64_t flp)
+                                                 int const (&slot)[RPA_CACHED], uint64_t flp,
+                                                 chunk_prefetch const& pf)
 {
   uint32_t live = 0;
 #pragma unroll
@@ -396,17 +454,12 @@ __device__ __forceinline__ void chunk_accumulate(acc_plan const& A, uint8_t* lds
     for_rows(live, [&](int k) {
       __hip_atomic_fetch_add(lds_at<uint32_t>(lds, A.count_all_off, slot[k]), 1u, __ATOMIC_RELAXED, WG_SCOPE);
     });
-  for (int i = 0; i < A.ncols; i++) {
+  auto column = [&](int i, uint64_t (&c)[RPA_CACHED]) {
     uint32_t const cd = lane_value(A.cd_lane, i);
-    auto const src    = reinterpret_cast<gmem<uint8_t const>*>(
-      (uint64_t(lane_value(A.cp_hi_lane, i)) << 32) | lane_value(A.cp_lo_lane, i));
-    int const st             = int(cd & 0xffu);
-    int const bit            = int((cd >> 8) & 0xffu) - 1;
-    int const fb             = int((cd >> 16) & 0xffu);
-    int const fe             = int(cd >> 24);
-    int const w              = storage_width(st);
-    uint64_t c[RPA_CACHED];
-    load_chunk(src, w, r0, e, c);
+    int const st      = int(cd & 0xffu);
+    int const bit     = int((cd >> 8) & 0xffu) - 1;
+    int const fb      = int((cd >> 16) & 0xffu);
+    int const fe      = int(cd >> 24);
     canon_rows(st, c);
     uint32_t ok = live;
     if (bit >= 0) {
@@ -415,8 +468,21 @@ __device__ __forceinline__ void chunk_accumulate(acc_plan const& A, uint8_t* lds
     }
     for (int f = fb; f < fe; f++) {
       uint32_t const d = lane_value(A.fd_lane, f);
-      acc_rows_lds(int32_t(d & 0xffu), int32_t(d >> 8), st, lds, slot, ok, c);
+      acc_rows_lds(int32_t(d & 0xffu), int32_t(d >> 8), st, lds, slot, ok, c, A.wide_f32);
     }
+  };
+#pragma unroll
+  for (int i = 0; i < RPA_AGG_PREFETCH; i++) {
+    if (i >= A.ncols) break;
+    uint64_t c[RPA_CACHED];
+#pragma unroll
+    for (int k = 0; k < RPA_CACHED; k++) c[k] = pf.c[i][k];
+    column(i, c);
+  }
+  for (int i = RPA_AGG_PREFETCH; i < A.ncols; i++) {
+    uint64_t c[RPA_CACHED];
+    load_chunk(plan_column(A, i), storage_width(int(lane_value(A.cd_lane, i) & 0xffu)), r0, e, c);
+    column(i, c);
   }
 }
 
@@ -454,7 +520,7 @@ __device__ __forceinline__ void chunk_deviations(gb_params const& P, rpa_params 
     for_rows(ok & flagged, [&](int k) {
       double const x    = canon_double(st, c[k]);
       double const cnt  = double(lds_count(P, R, lds, F.aux_count, slot[k]));
-      double const mean = lds_field_double(P.fields[F.aux_sum], lds, slot[k]) / cnt;
+      double const mean = lds_field_double(P.fields[F.aux_sum], lds, slot[k], R.wide_f32 != 0) / cnt;
       double const d    = x - mean;
       __hip_atomic_fetch_add(lds_at<double>(lds, F.lds_off, slot[k]), d * d, __ATOMIC_RELAXED, WG_SCOPE);
     });
@@ -468,8 +534,8 @@ __device__ __forceinline__ void chunk_deviations(gb_params const& P, rpa_params 
 // Σx² − (Σx)²/n when its rounding error, at most ~3·n·ε·Σx², is below 1e-11
 // of the result.  Anything else (float32 sums, large or ill-conditioned
 // groups) returns false: the exact second pass computes that group.
-__device__ __forceinline__ bool m2_from_sums(field_desc const& SUMF, int32_t st, uint8_t* lds, int i, uint32_t n,
-                                             double* m2)
+__device__ __forceinline__ bool m2_from_sums(int32_t sum_kind, int32_t sum_off, int32_t st, uint8_t* lds, int i,
+                                             uint32_t n, double* m2)
 {
   double const s2 = *m2;
   if (n <= 1) {  // no deviation from the mean of one value
@@ -479,14 +545,14 @@ __device__ __forceinline__ bool m2_from_sums(field_desc const& SUMF, int32_t st,
   if (st == ST_F32) return false;
   if (st == ST_F64) {
     if (n > 256) return false;
-    double const s1 = *lds_at<double>(lds, SUMF.lds_off, i);
+    double const s1 = *lds_at<double>(lds, sum_off, i);
     double const v  = s2 - s1 * (s1 / double(n));
     if (!(v > 0.0) || 3.0 * double(n) * 1.1102230246251565e-16 * s2 > 1e-11 * v) return v == 0.0 && s2 == 0.0;
     *m2 = v;
     return true;
   }
-  if (SUMF.kind != FK_SUM_I64 || !(s2 < 9007199254740992.0)) return false;
-  int64_t const s1        = *lds_at<int64_t>(lds, SUMF.lds_off, i);
+  if (sum_kind != FK_SUM_I64 || !(s2 < 9007199254740992.0)) return false;
+  int64_t const s1        = *lds_at<int64_t>(lds, sum_off, i);
   __int128 const num      = __int128(n) * __int128(uint64_t(s2)) - __int128(s1) * __int128(s1);
   *m2 = double(num) / double(n);
   return true;
@@ -499,14 +565,6 @@ __global__ void __launch_bounds__(RPA_AGG_BLOCK) __attribute__((amdgpu_waves_per
   extern __shared__ __attribute__((aligned(16))) uint8_t lds[];
   gb_params const& P = *Pp;
   rpa_params const& R = *Rp;
-  int64_t const b     = bin_begin + blockIdx.x;
-  // the bin's records: from its start to where the scatter's cursor stopped
-  // (within its capacity; a bin that overflowed a speculative layout is
-  // redone with an exact one)
-  int64_t const s     = R.bin_start[b];
-  int64_t e           = int64_t(gp(R.two_level ? R.cursor2 : R.cursor1)[b]);
-  if (e - s > R.cap2) e = s + R.cap2;
-  if (s == e) return;
   int const S       = R.lds_slots;
   int const NS      = S + 2;
   auto* tkeys       = reinterpret_cast<uint64_t*>(lds);
@@ -514,16 +572,33 @@ __global__ void __launch_bounds__(RPA_AGG_BLOCK) __attribute__((amdgpu_waves_per
   auto* misc        = reinterpret_cast<rpa_misc*>(lds + R.misc_off);
   int const tid     = threadIdx.x;
   acc_plan const A  = load_plan(P, R);
+  constexpr int64_t CHUNK = int64_t(RPA_CACHED) * RPA_AGG_BLOCK;
+  // One workgroup per bin.  The phases share only LDS, so the barriers wait
+  // for LDS operations alone, not for the output stores.
+  int64_t const b = bin_begin + blockIdx.x;
+  {
+  // the bin's records: from its start to where the scatter's cursor stopped
+  // (within its capacity; a bin that overflowed a speculative layout is
+  // redone with an exact one)
+  int64_t const s     = R.bin_start[b];
+  int64_t e           = int64_t(gp(R.two_level ? R.cursor2 : R.cursor1)[b * RPA_CURSOR_STRIDE]);
+  if (e - s > R.cap2) e = s + R.cap2;
+  if (s == e) return;
+  // the first chunk's loads are in flight while the table is initialised
+  chunk_in in;
+  chunk_load<true>(A, s + tid, e, in);
 
   // ---- init
   for (int i = tid; i < S; i += RPA_AGG_BLOCK) tkeys[i] = EMPTY_KEY;
-  for (int f = 0; f < P.num_fields; f++) {
-    field_desc const& F = P.fields[f];
-    uint64_t const idv  = field_identity(F.kind);
-    if (F.width == 8)
-      for (int i = tid; i < NS; i += RPA_AGG_BLOCK) *lds_at<uint64_t>(lds, F.lds_off, i) = idv;
+  for (int f = 0; f < A.nfields; f++) {  // identities, from the register plan
+    uint32_t const d    = lane_value(A.fd_lane, f);
+    int32_t const kind  = int32_t(d & 0xffu);
+    int32_t const off   = int32_t(d >> 8);
+    uint64_t const idv  = field_identity(kind);
+    if (rpa_lds_width(kind, A.wide_f32) == 8)
+      for (int i = tid; i < NS; i += RPA_AGG_BLOCK) *lds_at<uint64_t>(lds, off, i) = idv;
     else
-      for (int i = tid; i < NS; i += RPA_AGG_BLOCK) *lds_at<uint32_t>(lds, F.lds_off, i) = uint32_t(idv);
+      for (int i = tid; i < NS; i += RPA_AGG_BLOCK) *lds_at<uint32_t>(lds, off, i) = uint32_t(idv);
   }
   if (R.count_all_off >= 0)
     for (int i = tid; i < NS; i += RPA_AGG_BLOCK) *lds_at<uint32_t>(lds, R.count_all_off, i) = 0;
@@ -533,21 +608,20 @@ __global__ void __launch_bounds__(RPA_AGG_BLOCK) __attribute__((amdgpu_waves_per
     misc->present1 = 0;
     misc->overflow = 0;
   }
-  __syncthreads();
+  lds_barrier();
 
   // ---- pass A: find-or-insert and accumulate, chunk by chunk
-  constexpr int64_t CHUNK = int64_t(RPA_CACHED) * RPA_AGG_BLOCK;
   for (int64_t c = s; c < e; c += CHUNK) {
     if (__hip_atomic_load(&misc->overflow, __ATOMIC_RELAXED, WG_SCOPE) ||
         __hip_atomic_load(&misc->fill, __ATOMIC_RELAXED, WG_SCOPE) > uint32_t(R.max_fill))
       break;
+    if (c != s) chunk_load<true>(A, c + tid, e, in);
     int slot[RPA_CACHED];
-    uint64_t fl;
-    chunk_slots<true, GENERAL>(P, A, tkeys, misc, S, c + tid, e, slot, fl);
-    chunk_accumulate(A, lds, c + tid, e, slot, fl);
+    chunk_slots<true, GENERAL>(P, A, tkeys, misc, S, c + tid, e, in, slot);
+    chunk_accumulate(A, lds, c + tid, e, slot, in.flp, in.pf);
   }
-  __syncthreads();
-  if (misc->overflow || misc->fill > uint32_t(R.max_fill)) {
+  lds_barrier();
+  if (misc->overflow || misc->fill > uint32_t(R.max_fill)) {  // the call is redone with more bins
     if (tid == 0) astore(R.overflow, 1);
     return;
   }
@@ -556,36 +630,60 @@ __global__ void __launch_bounds__(RPA_AGG_BLOCK) __attribute__((amdgpu_waves_per
   // the other groups are flagged for the exact second pass
   if (P.need_second_pass) {
     uint8_t* m2_flag = lds + R.slot_of_j_off;  // the compaction map's bytes, free until then
+    int const nm2    = R.m2_n;
+    int const lane   = tid & 63;
+    int const jl     = lane < nm2 ? lane : 0;  // the plan, one M2 field per lane
+    int32_t const m2_off_l = R.m2_off[jl], m2_cnt_l = R.m2_cnt_off[jl], m2_st_l = R.m2_storage[jl];
+    uint32_t const m2_sum_l = R.m2_sum[jl];
     if (tid == 0) misc->m2_pending = 0;
-    __syncthreads();
+    lds_barrier();
     for (int i = tid; i < NS; i += RPA_AGG_BLOCK) {
       bool redo = false;
-      for (int f = 0; f < P.num_fields; f++) {
-        field_desc const& F = P.fields[f];
-        if (F.kind != FK_M2) continue;
-        double* m2       = lds_at<double>(lds, F.lds_off, i);
-        uint32_t const n = lds_count(P, R, lds, F.aux_count, i);
-        redo |= !m2_from_sums(P.fields[F.aux_sum], P.vals[F.vcol].storage, lds, i, n, m2);
+      for (int j = 0; j < nm2; j++) {
+        int32_t const off   = int32_t(lane_value(uint32_t(m2_off_l), j));
+        uint32_t const sum  = lane_value(m2_sum_l, j);
+        double* m2          = lds_at<double>(lds, off, i);
+        uint32_t const n    = *lds_at<uint32_t>(lds, int32_t(lane_value(uint32_t(m2_cnt_l), j)), i);
+        redo |= !m2_from_sums(int32_t(sum & 0xffu), int32_t(sum >> 8), int32_t(lane_value(uint32_t(m2_st_l), j)),
+                              lds, i, n, m2);
       }
       m2_flag[i] = redo ? 1 : 0;
       if (redo) {
-        for (int f = 0; f < P.num_fields; f++)
-          if (P.fields[f].kind == FK_M2) *lds_at<double>(lds, P.fields[f].lds_off, i) = 0.0;
+        for (int j = 0; j < nm2; j++) *lds_at<double>(lds, int32_t(lane_value(uint32_t(m2_off_l), j)), i) = 0.0;
         __hip_atomic_store(&misc->m2_pending, 1u, __ATOMIC_RELAXED, WG_SCOPE);
       }
     }
-    __syncthreads();
+    lds_barrier();
   }
 
   // ---- pass B: squared deviations from the group mean, for flagged groups
   if (P.need_second_pass && misc->m2_pending) {
     for (int64_t c = s; c < e; c += CHUNK) {
       int slot[RPA_CACHED];
-      uint64_t fl;
-      chunk_slots<false, GENERAL>(P, A, tkeys, misc, S, c + tid, e, slot, fl);
-      chunk_deviations(P, R, lds, c + tid, e, slot, fl);
+      chunk_load<false>(A, c + tid, e, in);
+      chunk_slots<false, GENERAL>(P, A, tkeys, misc, S, c + tid, e, in, slot);
+      chunk_deviations(P, R, lds, c + tid, e, slot, in.flp);
     }
-    __syncthreads();
+    lds_barrier();
+  }
+
+  // the output plan, one result column per lane (loads in flight during the
+  // compaction)
+  int const nouts = R.out_n;
+  uint32_t out_op_l, out_dlo_l, out_dhi_l, out_mlo_l, out_mhi_l;
+  int32_t out_foff_l, out_coff_l, out_ddof_l;
+  {
+    int const ol = (tid & 63) < nouts ? (tid & 63) : 0;
+    out_op_l     = R.out_op[ol];
+    out_foff_l   = R.out_foff[ol];
+    out_coff_l   = R.out_coff[ol];
+    out_ddof_l   = R.out_ddof[ol];
+    uint64_t const d = reinterpret_cast<uint64_t>(P.outs[ol].data);
+    uint64_t const m = reinterpret_cast<uint64_t>(P.outs[ol].mask);
+    out_dlo_l = uint32_t(d);
+    out_dhi_l = uint32_t(d >> 32);
+    out_mlo_l = uint32_t(m);
+    out_mhi_l = uint32_t(m >> 32);
   }
 
   // ---- compact occupied slots (null-key group first, EMPTY_KEY group next)
@@ -610,7 +708,7 @@ __global__ void __launch_bounds__(RPA_AGG_BLOCK) __attribute__((amdgpu_waves_per
     if (lane >= o) incl += y;
   }
   if (lane == 63) misc->wave_sum[wave] = incl;
-  __syncthreads();
+  lds_barrier();
   uint32_t before = 0, total = 0;
   for (int w = 0; w < RPA_AGG_BLOCK / 64; w++) {
     if (w < wave) before += misc->wave_sum[w];
@@ -624,7 +722,7 @@ __global__ void __launch_bounds__(RPA_AGG_BLOCK) __attribute__((amdgpu_waves_per
     misc->base  = __hip_atomic_fetch_add(gp(R.group_counter), (unsigned long long)total, __ATOMIC_RELAXED,
                                          __HIP_MEMORY_SCOPE_AGENT);
   }
-  __syncthreads();
+  lds_barrier();
 
   // ---- outputs: group j of the workgroup goes to output row base + j.  The
   // waves take 64-row chunks aligned to 64 output rows, so a data store
@@ -653,64 +751,79 @@ __global__ void __launch_bounds__(RPA_AGG_BLOCK) __attribute__((amdgpu_waves_per
       if (P.out_key_mask[0]) store_valid_word(P.out_key_mask[0], c0, base, end, cap, in_range && kvalid);
     }
     // outputs in batches: the LDS reads of a batch are issued together, then
-    // the batch's values are converted and stored
-    for (int o0 = 0; o0 < P.num_outs; o0 += RPA_OUT_BATCH) {
+    // the batch's values are converted and stored; descriptors come from the
+    // register plan
+    for (int o0 = 0; o0 < nouts; o0 += RPA_OUT_BATCH) {
       uint64_t raw[RPA_OUT_BATCH];
       uint32_t cnt[RPA_OUT_BATCH];
 #pragma unroll
       for (int u = 0; u < RPA_OUT_BATCH; u++) {
-        raw[u] = 0;
-        cnt[u] = 0;
+        raw[u]      = 0;
+        cnt[u]      = 0;
         int const o = o0 + u;
-        if (o >= P.num_outs || !in_range) continue;
-        out_desc const& O = P.outs[o];
-        if (O.kind == OK_COUNT_ALL) raw[u] = *lds_at<uint32_t>(lds, R.count_all_off, slot);
-        else raw[u] = lds_field_bits(P.fields[O.field], lds, slot);
-        if (O.count_field != -1) cnt[u] = lds_count(P, R, lds, O.count_field, slot);
+        if (o >= nouts || !in_range) continue;
+        uint32_t const op  = lane_value(out_op_l, o);
+        int32_t const foff = int32_t(lane_value(uint32_t(out_foff_l), o));
+        int32_t const coff = int32_t(lane_value(uint32_t(out_coff_l), o));
+        bool const w8      = (op & 0xffu) != OK_COUNT_ALL && rpa_lds_width(int32_t((op >> 8) & 0xffu), A.wide_f32) == 8;
+        raw[u] = w8 ? *lds_at<uint64_t>(lds, foff, slot) : uint64_t(*lds_at<uint32_t>(lds, foff, slot));
+        if (coff >= 0) cnt[u] = *lds_at<uint32_t>(lds, coff, slot);
       }
 #pragma unroll
       for (int u = 0; u < RPA_OUT_BATCH; u++) {
         int const o = o0 + u;
-        if (o >= P.num_outs) break;
-        out_desc const& O = P.outs[o];
-        bool valid        = O.count_field == -1 || cnt[u] > 0;
+        if (o >= nouts) break;
+        uint32_t const op   = lane_value(out_op_l, o);
+        int32_t const okind = int32_t(op & 0xffu);
+        int32_t const fk    = int32_t((op >> 8) & 0xffu);
+        int32_t const ow    = int32_t((op >> 16) & 0xffu);
+        bool const counted  = int32_t(lane_value(uint32_t(out_coff_l), o)) >= 0;
+        void* const data    = reinterpret_cast<void*>((uint64_t(lane_value(out_dhi_l, o)) << 32) | lane_value(out_dlo_l, o));
+        auto* const mask    = reinterpret_cast<uint32_t*>((uint64_t(lane_value(out_mhi_l, o)) << 32) |
+                                                       lane_value(out_mlo_l, o));
+        bool valid          = !counted || cnt[u] > 0;
         if (in_range) {
-          switch (O.kind) {
+          switch (okind) {
             case OK_COPY_I64:
-            case OK_COPY_F64: gp(reinterpret_cast<uint64_t*>(O.data))[pos] = raw[u]; break;
-            case OK_COPY_F32: gp(reinterpret_cast<uint32_t*>(O.data))[pos] = uint32_t(raw[u]); break;
+            case OK_COPY_F64: gp(reinterpret_cast<uint64_t*>(data))[pos] = raw[u]; break;
+            case OK_COPY_F32: {  // float32 sums come from float64 accumulators
+              float const f = rpa_wide_sum(fk, A.wide_f32) ? float(__longlong_as_double(int64_t(raw[u])))
+                                                           : __uint_as_float(uint32_t(raw[u]));
+              gp(reinterpret_cast<float*>(data))[pos] = f;
+              break;
+            }
             case OK_COUNT:
             case OK_COUNT_ALL:
-              gp(reinterpret_cast<int32_t*>(O.data))[pos] = int32_t(uint32_t(raw[u]));
+              gp(reinterpret_cast<int32_t*>(data))[pos] = int32_t(uint32_t(raw[u]));
               valid = true;
               break;
             case OK_MINMAX: {
-              int const fk  = P.fields[O.field].kind;
               uint64_t bits = raw[u];
               if (fk == FK_MIN_F32 || fk == FK_MAX_F32) bits = unord32(uint32_t(bits));
               else if (fk == FK_MIN_F64 || fk == FK_MAX_F64) bits = unord64(bits);
-              store_width(O.data, int64_t(pos), O.out_width, bits);
+              store_width(data, int64_t(pos), ow, bits);
               break;
             }
             case OK_MEAN: {
-              double const sum = field_bits_double(P.fields[O.field].kind, raw[u]);
-              gp(reinterpret_cast<double*>(O.data))[pos] = valid ? sum / double(cnt[u]) : 0.0;
+              double const sum = field_bits_double(fk, raw[u], A.wide_f32);
+              gp(reinterpret_cast<double*>(data))[pos] = valid ? sum / double(cnt[u]) : 0.0;
               break;
             }
             case OK_VAR:
             case OK_STD: {
-              int64_t const dn = int64_t(cnt[u]) - O.ddof;
+              int64_t const dn = int64_t(cnt[u]) - int32_t(lane_value(uint32_t(out_ddof_l), o));
               valid            = cnt[u] > 0 && dn > 0;
               double var       = valid ? __longlong_as_double(int64_t(raw[u])) / double(dn) : 0.0;
-              gp(reinterpret_cast<double*>(O.data))[pos] = (O.kind == OK_STD) ? sqrt(var) : var;
+              gp(reinterpret_cast<double*>(data))[pos] = (okind == OK_STD) ? sqrt(var) : var;
               break;
             }
             default: break;
           }
         }
-        if (O.mask) store_valid_word(O.mask, c0, base, end, cap, in_range && valid);
+        if (mask) store_valid_word(mask, c0, base, end, cap, in_range && valid);
       }
     }
+  }
   }
 }
 
@@ -718,15 +831,15 @@ void launch_rpa_aggregate(gb_params const* dP, rpa_params const* dR, rpa_params 
                           int64_t bin_begin, int64_t bin_end, hipStream_t s)
 {
   if (bin_end <= bin_begin) return;
-  if (general) {
-    allow_dynamic_lds(reinterpret_cast<void const*>(&k_rpa_aggregate<true>), hR.lds_bytes);
-    hipLaunchKernelGGL(k_rpa_aggregate<true>, dim3(bin_end - bin_begin), dim3(RPA_AGG_BLOCK), hR.lds_bytes, s,
-                       dP, dR, bin_begin);
-  } else {
-    allow_dynamic_lds(reinterpret_cast<void const*>(&k_rpa_aggregate<false>), hR.lds_bytes);
-    hipLaunchKernelGGL(k_rpa_aggregate<false>, dim3(bin_end - bin_begin), dim3(RPA_AGG_BLOCK), hR.lds_bytes, s,
-                       dP, dR, bin_begin);
-  }
+  void const* fn = general ? reinterpret_cast<void const*>(&k_rpa_aggregate<true>)
+                           : reinterpret_cast<void const*>(&k_rpa_aggregate<false>);
+  allow_dynamic_lds(fn, hR.lds_bytes);
+  if (general)
+    hipLaunchKernelGGL(k_rpa_aggregate<true>, dim3(bin_end - bin_begin), dim3(RPA_AGG_BLOCK), hR.lds_bytes, s, dP,
+                       dR, bin_begin);
+  else
+    hipLaunchKernelGGL(k_rpa_aggregate<false>, dim3(bin_end - bin_begin), dim3(RPA_AGG_BLOCK), hR.lds_bytes, s, dP,
+                       dR, bin_begin);
 }
 
 }  // namespace mig

@@ -5,6 +5,16 @@
 
 namespace mig {
 
+// Workgroup barrier for phases that communicate through LDS only: each wave
+// waits for its LDS operations, not for its global stores (a __syncthreads
+// would also wait until every store of the workgroup is acknowledged).
+__device__ __forceinline__ void lds_barrier()
+{
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 __device__ __forceinline__ uint32_t inline_key_hash(int32_t st, uint64_t bits)
 {
   return hash_combine(0u, hash_fixed_bits(st, bits, 0u));

@@ -109,7 +109,7 @@ __global__ void __launch_bounds__(RPA_LAYOUT_BLOCK) k_rpa_layout(rpa_params cons
   uint64_t run = uint64_t(before) + incl - local;
   for (int i = i0; i < i1; i++) {
     bin_start[i]       = int64_t(run);
-    gp(R.cursor2)[i]   = run;
+    gp(R.cursor2)[int64_t(i) * RPA_CURSOR_STRIDE] = run;
     run += hist[i];
   }
   if (i1 == B && i0 < B) bin_start[B] = int64_t(run);
@@ -118,7 +118,7 @@ __global__ void __launch_bounds__(RPA_LAYOUT_BLOCK) k_rpa_layout(rpa_params cons
   if (tid < F1) {
     int64_t const lo         = bin_start[int64_t(tid) << lf2];
     gp(R.first_start)[tid]   = lo;
-    gp(R.cursor1)[tid]       = uint64_t(lo);
+    gp(R.cursor1)[tid * RPA_CURSOR_STRIDE] = uint64_t(lo);
   }
 }
 
@@ -131,12 +131,12 @@ __global__ void __launch_bounds__(RPA_SMALL_BLOCK) k_rpa_spec_layout(rpa_params 
   for (int64_t b = int64_t(blockIdx.x) * RPA_SMALL_BLOCK + threadIdx.x; b <= B; b += int64_t(gridDim.x) * RPA_SMALL_BLOCK) {
     int64_t const st = b * R.cap2;
     gp(R.bin_start)[b] = st;
-    if (b < B) gp(R.cursor2)[b] = uint64_t(st);
+    if (b < B) gp(R.cursor2)[b * RPA_CURSOR_STRIDE] = uint64_t(st);
     if (b < F1) {
       // one level: the first-level bins are the final bins (buffer B, cap2)
       int64_t const fs      = R.two_level ? b * R.cap1 : st;
       gp(R.first_start)[b] = fs;
-      gp(R.cursor1)[b]     = uint64_t(fs);
+      gp(R.cursor1)[b * RPA_CURSOR_STRIDE] = uint64_t(fs);
     }
   }
 }
@@ -153,7 +153,7 @@ void launch_rpa_spec_layout(rpa_params const* dR, rpa_params const& hR, hipStrea
 __device__ __forceinline__ int64_t first_rows(rpa_params const& R, int b1)
 {
   int64_t const lo = gp(static_cast<int64_t const*>(R.first_start))[b1];
-  int64_t hi       = int64_t(gp(static_cast<unsigned long long const*>(R.cursor1))[b1]);
+  int64_t hi       = int64_t(gp(static_cast<unsigned long long const*>(R.cursor1))[int64_t(b1) * RPA_CURSOR_STRIDE]);
   if (hi - lo > R.cap1) hi = lo + R.cap1;
   return hi - lo;
 }
@@ -326,7 +326,7 @@ __global__ void __launch_bounds__(RPA_SCATTER_BLOCK) __attribute__((amdgpu_waves
   if constexpr (SECOND) {
     for (int i = threadIdx.x; i <= F1; i += RPA_SCATTER_BLOCK)
       tstart[i] = gp(static_cast<int64_t const*>(R.tile_start1))[i];
-    __syncthreads();
+    lds_barrier();
     ntiles = n < 0 ? tstart[F1] : n;  // tiles [t_begin, n) of the first-level bins (n < 0: all)
   } else {
     ntiles = (n + T - 1) / T;
@@ -352,7 +352,7 @@ __global__ void __launch_bounds__(RPA_SCATTER_BLOCK) __attribute__((amdgpu_waves
       r1 = (r0 + T < n) ? r0 + T : n;
     }
     for (int i = threadIdx.x; i < fan; i += RPA_SCATTER_BLOCK) cnt[i] = 0;
-    __syncthreads();
+    lds_barrier();
     // ---- rank rows per bin
     int bin[RPA_MAX_RPT];
     uint32_t rank[RPA_MAX_RPT];
@@ -399,12 +399,12 @@ __global__ void __launch_bounds__(RPA_SCATTER_BLOCK) __attribute__((amdgpu_waves
     for (int k = 0; k < RPA_MAX_RPT; k++)
       if (bin[k] >= 0)
         rank[k] = __hip_atomic_fetch_add(&cnt[bin[k]], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    __syncthreads();
+    lds_barrier();
     scan_counts(cnt, toff, fan);
     for (int i = threadIdx.x; i < fan; i += RPA_SCATTER_BLOCK) {
       if (cnt[i]) {
         int64_t const fb        = (int64_t(b1) << lf2) + i;
-        unsigned long long* cur = SECOND ? R.cursor2 + fb : R.cursor1 + i;
+        unsigned long long* cur = SECOND ? R.cursor2 + fb * RPA_CURSOR_STRIDE : R.cursor1 + int64_t(i) * RPA_CURSOR_STRIDE;
         unsigned long long const b0 = __hip_atomic_fetch_add(gp(cur), (unsigned long long)cnt[i], __ATOMIC_RELAXED,
                                                              __HIP_MEMORY_SCOPE_AGENT);
         // a bin of a speculative layout may fill up: its rows that do not
@@ -420,7 +420,7 @@ __global__ void __launch_bounds__(RPA_SCATTER_BLOCK) __attribute__((amdgpu_waves
         }
       }
     }
-    __syncthreads();
+    lds_barrier();
     // ---- stage the tile in bin order
     int lp[RPA_MAX_RPT];
 #pragma unroll
@@ -444,7 +444,7 @@ __global__ void __launch_bounds__(RPA_SCATTER_BLOCK) __attribute__((amdgpu_waves
 #pragma unroll
     for (int k = 0; k < RPA_MAX_RPT; k++)
       if (lp[k] >= 0) sflags[lp[k]] = kf[k];
-    __syncthreads();
+    lds_barrier();
     // ---- write bin runs
     int const total = int(toff[fan]);
     for (int lp = threadIdx.x; lp < total; lp += RPA_SCATTER_BLOCK) {
@@ -459,7 +459,7 @@ __global__ void __launch_bounds__(RPA_SCATTER_BLOCK) __attribute__((amdgpu_waves
         copy_width(ov, dst, stage + R.stage_val_off[v], lp, storage_width(P.vals[v].storage));
       }
     }
-    __syncthreads();
+    lds_barrier();
   }
 }
 
