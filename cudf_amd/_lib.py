@@ -176,9 +176,9 @@ SIGNATURES = {
 STAT_NAMES = ["strategy", "key_mode", "capacity", "est_groups", "groups", "retries", "state_bytes",
               "t_estimate_ms", "t_init_ms", "t_accumulate_ms", "t_second_pass_ms", "t_compact_ms",
               "t_outputs_ms", "t_rpa_hist_ms", "t_rpa_scatter1_ms", "t_rpa_scatter2_ms",
-              "t_rpa_aggregate_ms", "lds_preagg", "rpa_speculative"]
+              "t_rpa_aggregate_ms", "lds_preagg", "rpa_speculative", "verified_pairs"]
 
-STRATEGY_AUTO, STRATEGY_GLOBAL, STRATEGY_PARTITIONED = 0, 1, 2
+STRATEGY_AUTO, STRATEGY_GLOBAL, STRATEGY_PARTITIONED, STRATEGY_PARTITIONED_EXACT = 0, 1, 2, 3
 
 
 def set_strategy(strategy: int) -> None:

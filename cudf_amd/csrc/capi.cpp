@@ -220,7 +220,7 @@ void mig_set_phase_timing(int32_t enable) { mig::set_phase_timing(enable == 2 ? 
 
 void mig_set_strategy(int32_t strategy)
 {
-  if (strategy >= MIG_STRATEGY_AUTO && strategy <= MIG_STRATEGY_PARTITIONED) mig::set_strategy(strategy);
+  if (strategy >= MIG_STRATEGY_AUTO && strategy <= MIG_STRATEGY_PARTITIONED_EXACT) mig::set_strategy(strategy);
 }
 
 mig_status mig_batch_plan_create(const mig_batch_instance* instances, int32_t num_instances,

@@ -127,6 +127,32 @@ __device__ __forceinline__ uint32_t row_hash(dcol const* cols, int ncols, int64_
   return h;
 }
 
+// row_hash(cols, ncols, row, 0) and row_hash(cols, ncols, row, s2) together,
+// each element read once.
+__device__ __forceinline__ void row_hash2(dcol const* cols, int ncols, int64_t row, uint32_t s2, uint32_t& h1,
+                                          uint32_t& h2)
+{
+  h1 = 0u;
+  h2 = s2;
+  for (int c = 0; c < ncols; c++) {
+    dcol const& col = cols[c];
+    uint32_t e1 = NULL_HASH, e2 = NULL_HASH;
+    if (dcol_valid(col, row)) {
+      if (col.storage == ST_STRING) {
+        int64_t b, e;
+        str_span(col, row, b, e);
+        mm_hash_bytes2(col.data + b, e - b, 0u, s2, e1, e2);
+      } else {
+        uint64_t const bits = load_fixed_bits(col, row);
+        e1 = hash_fixed_bits(col.storage, bits, 0u);
+        e2 = hash_fixed_bits(col.storage, bits, s2);
+      }
+    }
+    h1 = hash_combine(h1, e1);
+    h2 = hash_combine(h2, e2);
+  }
+}
+
 __device__ __forceinline__ bool elem_equal(dcol const& c, int64_t a, int64_t b)
 {
   bool const va = dcol_valid(c, a), vb = dcol_valid(c, b);

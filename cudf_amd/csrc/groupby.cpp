@@ -1230,7 +1230,7 @@ int64_t groupby_call::aggregate_partitioned()
   rpa_ok = rpa_ok && agg_lds_bytes(S_slots, wide) <= RPA_LDS_BUDGET;
   int32_t const max_fill = int32_t(0.85 * S_slots);
   bool use_rpa = false;
-  if (forced == MIG_STRATEGY_PARTITIONED) use_rpa = rpa_ok;
+  if (forced == MIG_STRATEGY_PARTITIONED || forced == MIG_STRATEGY_PARTITIONED_EXACT) use_rpa = rpa_ok;
   else if (forced == MIG_STRATEGY_AUTO)  // and the estimate fits the largest partition
     use_rpa = rpa_ok && N >= (1 << 20) && est_ >= 65536 &&
               est_ <= double(int64_t(1) << RPA_MAX_LOG_BINS) * max_fill;
@@ -1292,8 +1292,13 @@ int64_t groupby_call::aggregate_partitioned()
       R.out_ddof[o] = O.ddof;
     }
   };
+  // General keys travel in hashed form (compared by 64-bit words, verified
+  // exactly afterwards); the form that compares key rows while probing is
+  // the fallback when a verification finds two different keys.
+  bool hashed = p.key_mode == KM_GENERAL && forced != MIG_STRATEGY_PARTITIONED_EXACT;
+  R.general_hashed = hashed ? 1 : 0;
   // scatter staging layout: a tile of keys, copied values, flags and bins
-  int32_t rec = 8 + 1 + 2;
+  int32_t rec = 8 + 1 + 2 + (hashed ? 4 : 0);
   for (int v = 0; v < p.num_vals; v++)
     if (p.val_field_begin[v] < p.val_field_begin[v + 1]) rec += storage_width(vals_[v].storage);
   // the largest multiple of the block whose staging fits the budget
@@ -1309,7 +1314,9 @@ int64_t groupby_call::aggregate_partitioned()
     R.stage_flags_off = so;
     so += (R.tile_rows + 7) & ~7;
     R.stage_bin_off = so;
-    so += R.tile_rows * 2;
+    so += (R.tile_rows * 2 + 7) & ~7;
+    R.stage_row_off = so;
+    if (hashed) so += R.tile_rows * 4;
     R.stage_bytes = so;
   }
   R.hist_blocks = num_cus * 2;
@@ -1390,6 +1397,28 @@ int64_t groupby_call::aggregate_partitioned()
       R.flagsA = S->get<uint8_t>(rowsA);
       part_bytes += 9 * size_t(rowsA);
     }
+    R.general_hashed = hashed ? 1 : 0;
+    R.rowsA = R.rowsB = nullptr;
+    R.rep_rows = nullptr;
+    R.rec_slot = nullptr;
+    R.rec_count = 0;
+    R.verified = nullptr;
+    R.verify_failed = nullptr;
+    R.rep_stride = 0;
+    if (hashed) {
+      R.rowsB = S->get<uint32_t>(rowsB);
+      if (R.two_level) R.rowsA = S->get<uint32_t>(rowsA);
+      R.rep_stride    = (int64_t(S_slots) + 2 + 31) & ~int64_t(31);  // whole lines per bin
+      R.rep_rows      = S->get<uint32_t>(size_t(B) * size_t(R.rep_stride));
+      R.rec_count     = rowsB;
+      R.rec_slot      = S->get<uint32_t>(size_t(rowsB));
+      R.verified      = S->get<unsigned long long>(1);
+      R.verify_failed = S->get<int32_t>(1);
+      MIG_HIP_TRY(hipMemsetAsync(R.rec_slot, 0xff, sizeof(uint32_t) * size_t(rowsB), stream_));
+      MIG_HIP_TRY(hipMemsetAsync(R.verified, 0, sizeof(unsigned long long), stream_));
+      MIG_HIP_TRY(hipMemsetAsync(R.verify_failed, 0, sizeof(int32_t), stream_));
+      part_bytes += 4 * size_t(rowsB) * 2 + (R.two_level ? 4 * size_t(rowsA) : 0) + 4 * size_t(B) * R.rep_stride;
+    }
     for (int v = 0; v < p.num_vals; v++) {
       if (p.val_field_begin[v] == p.val_field_begin[v + 1]) continue;
       size_t w   = size_t(storage_width(vals_[v].storage));
@@ -1414,15 +1443,31 @@ int64_t groupby_call::aggregate_partitioned()
       launch_rpa_scatter2(dP, dR, R, num_cus, 0, -1, stream_);
     }
     T->mark(12);
-    launch_rpa_aggregate(dP, dR, R, p.key_mode == KM_GENERAL, 0, B, stream_);
+    launch_rpa_aggregate(dP, dR, R, p.key_mode == KM_GENERAL ? (hashed ? 2 : 1) : 0, 0, B, stream_);
     T->mark(13);
     MIG_HIP_TRY(hipGetLastError());
+    // the hashed form compares every record with its representative before
+    // the results are read back (a no-op launch for the other forms)
+    if (hashed) launch_rpa_verify(dP, dR, R.rec_count, stream_);
+    MIG_HIP_TRY(hipGetLastError());
     auto* h = reinterpret_cast<uint64_t*>(pin + 2 * sizeof(gb_params));
+    h[2] = 0;
+    h[3] = 0;
     MIG_HIP_TRY(hipMemcpyAsync(h, R.group_counter, 8, hipMemcpyDeviceToHost, stream_));
     MIG_HIP_TRY(hipMemcpyAsync(h + 1, d_small, 8, hipMemcpyDeviceToHost, stream_));
+    if (hashed) {
+      MIG_HIP_TRY(hipMemcpyAsync(h + 2, R.verified, 8, hipMemcpyDeviceToHost, stream_));
+      MIG_HIP_TRY(hipMemcpyAsync(h + 3, R.verify_failed, 4, hipMemcpyDeviceToHost, stream_));
+    }
     MIG_HIP_TRY(hipStreamSynchronize(stream_));
     uint32_t const groups_overflow = uint32_t(h[1]), bins_overflow = uint32_t(h[1] >> 32);
-    if (bins_overflow) {  // a speculative bin filled up: the same bins, laid out exactly
+    bool const verify_failed = hashed && !bins_overflow && groups_overflow == 0 && uint32_t(h[3]) != 0;
+    stats_[STAT_VERIFIED_PAIRS] = double(h[2]);
+    if (verify_failed) {  // two different keys met with equal words: compare key rows instead
+      release_outputs();
+      hashed = false;
+      stats_[STAT_RETRIES] += 1;
+    } else if (bins_overflow) {  // a speculative bin filled up: the same bins, laid out exactly
       release_outputs();
       speculative = false;
       stats_[STAT_RETRIES] += 1;

@@ -59,6 +59,7 @@ enum stat_index : int {
   STAT_T_RPA_AGGREGATE,
   STAT_LDS_PREAGG,       // 1 when the global path pre-aggregated in LDS
   STAT_RPA_SPECULATIVE,  // 1 when the partitioned path's bins were sized without a histogram
+  STAT_VERIFIED_PAIRS,   // general keys, hashed form: rows compared with their group's representative
   STAT_COUNT
 };
 double* thread_stats();

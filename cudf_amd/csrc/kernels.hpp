@@ -245,6 +245,24 @@ struct rpa_params {
   int32_t pad1;
   uint64_t out_capacity;             // rows allocated in every output column
   uint32_t* out_rep;                 // general keys: representative input row of each output row
+  // General keys in hashed form (general_hashed = 1): the record word is
+  // (row hash << 32) | a second, independent row hash, the row index travels
+  // in rowsA / rowsB, and bins compare words instead of key rows.  A bin slot's
+  // representative row is rep_rows[bin * rep_stride + slot]; rec_slot[r]
+  // (buffer B record r; 0xFFFFFFFF where no record is) receives the slot of
+  // the record's group, and k_rpa_verify compares every record's key row with
+  // its representative's exactly afterwards (any difference reruns the call
+  // with general_hashed = 0, which compares key rows while probing).
+  int32_t general_hashed;
+  int32_t stage_row_off;
+  int64_t rep_stride;
+  uint32_t* rowsA;
+  uint32_t* rowsB;
+  uint32_t* rep_rows;
+  uint32_t* rec_slot;                // [records of buffer B]
+  int64_t rec_count;                 // records buffer B holds (its positions)
+  unsigned long long* verified;      // rows compared with a representative
+  int32_t* verify_failed;
   // Pass-A plan of the aggregate kernel, read into registers once per
   // workgroup: the accumulators of value column plan_col[i] are the fields
   // [plan_fb[i], plan_fb[i + 1]), field f described by plan_fd[f] =
@@ -384,8 +402,11 @@ void launch_rpa_tiles(rpa_params const* dR, hipStream_t s);
 // (tile_end < 0: to the last tile, counted on the device)
 void launch_rpa_scatter2(gb_params const* dP, rpa_params const* dR, rpa_params const& hR, int num_cus,
                          int64_t tile_begin, int64_t tile_end, hipStream_t s);
-// final bins [bin_begin, bin_end); `general`: general keys (row hash, row) words
-void launch_rpa_aggregate(gb_params const* dP, rpa_params const* dR, rpa_params const& hR, bool general,
+// final bins [bin_begin, bin_end); key_form: 0 one fixed-width key inline,
+// 1 general keys compared row by row, 2 general keys in hashed form
+void launch_rpa_aggregate(gb_params const* dP, rpa_params const* dR, rpa_params const& hR, int key_form,
                           int64_t bin_begin, int64_t bin_end, hipStream_t s);
+// exact key comparison of the (row, representative) pairs of the hashed form
+void launch_rpa_verify(gb_params const* dP, rpa_params const* dR, int64_t records, hipStream_t s);
 
 }  // namespace mig

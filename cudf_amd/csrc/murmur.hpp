@@ -99,6 +99,43 @@ __host__ __device__ inline uint32_t mm_hash_bytes(uint8_t const* p, int64_t len,
   return mm_fmix32(h ^ static_cast<uint32_t>(len));
 }
 
+// MurmurHash3_x86_32 of the same bytes under two seeds, in one pass: the
+// mixing of each input block is shared, each hash keeps its own state.
+__host__ __device__ inline void mm_hash_bytes2(uint8_t const* p, int64_t len, uint32_t s1, uint32_t s2, uint32_t& o1,
+                                               uint32_t& o2)
+{
+  uint32_t h1 = s1, h2 = s2;
+  int64_t const nblocks = len >> 2;
+  bool const aligned    = (reinterpret_cast<uintptr_t>(p) & 3) == 0;  // word loads when 4-byte aligned
+  for (int64_t i = 0; i < nblocks; i++) {
+    uint8_t const* b = p + 4 * i;
+    uint32_t k = aligned ? reinterpret_cast<uint32_t const*>(p)[i]
+                         : uint32_t(b[0]) | (uint32_t(b[1]) << 8) | (uint32_t(b[2]) << 16) | (uint32_t(b[3]) << 24);
+    k *= MM_C1;
+    k = rotl32(k, 15);
+    k *= MM_C2;
+    h1 ^= k;
+    h2 ^= k;
+    h1 = rotl32(h1, 13) * 5 + MM_C3;
+    h2 = rotl32(h2, 13) * 5 + MM_C3;
+  }
+  uint8_t const* t = p + 4 * nblocks;
+  int const rem = static_cast<int>(len & 3);
+  if (rem) {
+    uint32_t k = 0;
+    if (rem == 3) k ^= uint32_t(t[2]) << 16;
+    if (rem >= 2) k ^= uint32_t(t[1]) << 8;
+    k ^= t[0];
+    k *= MM_C1;
+    k = rotl32(k, 15);
+    k *= MM_C2;
+    h1 ^= k;
+    h2 ^= k;
+  }
+  o1 = mm_fmix32(h1 ^ static_cast<uint32_t>(len));
+  o2 = mm_fmix32(h2 ^ static_cast<uint32_t>(len));
+}
+
 __host__ __device__ __forceinline__ uint32_t hash_combine(uint32_t lhs, uint32_t rhs)
 {
   return lhs ^ (rhs + 0x9e3779b9u + (lhs << 6) + (lhs >> 2));

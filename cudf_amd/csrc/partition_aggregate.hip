@@ -194,6 +194,7 @@ struct rpa_misc {
   int32_t overflow;
   uint32_t g_out;
   uint32_t m2_pending;  // some group's M2 needs the exact second pass
+  uint32_t rep1;        // hashed general keys: row representing the EMPTY_KEY word group
   unsigned long long base;
   uint32_t wave_sum[RPA_AGG_BLOCK / 64];
 };
@@ -275,6 +276,7 @@ struct acc_plan {
   uint32_t fd_lane;     // lane f: kind | lds_off << 8 of field f
   gmem<uint64_t const>* keys;
   gmem<uint8_t const>* flags;
+  gmem<uint32_t const>* rows;
 };
 
 __device__ __forceinline__ acc_plan load_plan(gb_params const& P, rpa_params const& R)
@@ -287,6 +289,7 @@ __device__ __forceinline__ acc_plan load_plan(gb_params const& P, rpa_params con
   A.count_all_off = R.count_all_off;
   A.keys          = gp(static_cast<uint64_t const*>(R.keysB));
   A.flags         = gp(static_cast<uint8_t const*>(R.flagsB));
+  A.rows          = gp(static_cast<uint32_t const*>(R.rowsB));
   int const lane  = int(threadIdx.x & 63);
   int const i     = lane < A.ncols ? lane : 0;
   int const v     = R.plan_col[i];
@@ -317,8 +320,14 @@ struct chunk_prefetch {
 struct chunk_in {
   uint64_t key[RPA_CACHED];
   uint64_t flp;
+  uint32_t row[RPA_CACHED];  // input rows (hashed general keys)
   chunk_prefetch pf;
 };
+
+// Key forms of the aggregating kernel.
+constexpr int KF_INLINE = 0;   // one fixed-width key, normalised bits in the table
+constexpr int KF_GENERAL = 1;  // general keys: (row hash, row) words, key rows compared while probing
+constexpr int KF_HASHED = 2;   // general keys: (row hash, second hash) words, pairs verified afterwards
 
 __device__ __forceinline__ gmem<uint8_t const>* plan_column(acc_plan const& A, int i)
 {
@@ -357,7 +366,7 @@ __device__ __forceinline__ bool same_general(gb_params const& P, uint64_t a, uin
 
 // Issues every load of a chunk (keys, flags and, with PREFETCH, the first
 // value columns) so that the chunk waits for memory once.
-template <bool PREFETCH>
+template <bool PREFETCH, int KM>
 __device__ __forceinline__ void chunk_load(acc_plan const& A, int64_t r0, int64_t e, chunk_in& in)
 {
   uint8_t fl[RPA_CACHED];
@@ -367,6 +376,7 @@ __device__ __forceinline__ void chunk_load(acc_plan const& A, int64_t r0, int64_
     int64_t const rc = r < e ? r : e - 1;
     fl[k]            = A.flags[rc];
     in.key[k]        = A.keys[rc];
+    if (KM == KF_HASHED) in.row[k] = A.rows[rc];
   }
   if (PREFETCH) prefetch_values(A, r0, e, in.pf);
   in.flp = 0;
@@ -374,11 +384,12 @@ __device__ __forceinline__ void chunk_load(acc_plan const& A, int64_t r0, int64_
   for (int k = 0; k < RPA_CACHED; k++) in.flp |= uint64_t(fl[k]) << (8 * k);
 }
 
-template <bool INSERT, bool GENERAL>
-__device__ __forceinline__ void chunk_slots(gb_params const& P, acc_plan const& A, uint64_t* tkeys,
-                                            rpa_misc* misc, int S, int64_t r0, int64_t e, chunk_in const& in,
-                                            int (&slot)[RPA_CACHED])
+template <bool INSERT, int KM>
+__device__ __forceinline__ void chunk_slots(gb_params const& P, acc_plan const& A, rpa_params const& R,
+                                            uint64_t* tkeys, rpa_misc* misc, int S, int64_t r0, int64_t e,
+                                            chunk_in const& in, int (&slot)[RPA_CACHED], int64_t rbase)
 {
+  constexpr bool GENERAL = KM != KF_INLINE;
   uint64_t key[RPA_CACHED];
   uint8_t fl[RPA_CACHED];
 #pragma unroll
@@ -401,6 +412,12 @@ __device__ __forceinline__ void chunk_slots(gb_params const& P, acc_plan const& 
     } else if (key[k] == EMPTY_KEY) {
       if (INSERT) misc->present1 = 1;
       slot[k] = S + 1;
+      if (KM == KF_HASHED && INSERT) {  // the first such row represents the group
+        uint32_t expected = 0xffffffffu;
+        if (__hip_atomic_compare_exchange_strong(&misc->rep1, &expected, in.row[k], __ATOMIC_RELAXED,
+                                                 __ATOMIC_RELAXED, WG_SCOPE))
+          __hip_atomic_store(gp(R.rep_rows) + rbase + S + 1, in.row[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
     } else {
       uint32_t const h = GENERAL ? uint32_t(key[k] >> 32) : inline_key_hash(A.key_storage, key[k]);
       pos[k]           = h & uint32_t(S - 1);
@@ -418,7 +435,7 @@ __device__ __forceinline__ void chunk_slots(gb_params const& P, acc_plan const& 
     for (; i < limit; i++) {
       uint64_t const cur = __hip_atomic_load(tkeys + p, __ATOMIC_RELAXED, WG_SCOPE);
       if (cur == key[k]) break;
-      if (GENERAL && cur != EMPTY_KEY && same_general(P, cur, key[k])) break;
+      if (KM == KF_GENERAL && cur != EMPTY_KEY && same_general(P, cur, key[k])) break;
       if (cur == EMPTY_KEY) {
         if (!INSERT) {
           i = limit;
@@ -428,9 +445,12 @@ __device__ __forceinline__ void chunk_slots(gb_params const& P, acc_plan const& 
         if (__hip_atomic_compare_exchange_strong(tkeys + p, &expected, key[k], __ATOMIC_RELAXED,
                                                  __ATOMIC_RELAXED, WG_SCOPE)) {
           inserted++;
+          if (KM == KF_HASHED)  // this row represents the new group
+            __hip_atomic_store(gp(R.rep_rows) + rbase + p, in.row[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
           break;
         }
-        if (expected == key[k] || (GENERAL && same_general(P, expected, key[k]))) break;
+        if (expected == key[k]) break;
+        if (KM == KF_GENERAL && same_general(P, expected, key[k])) break;
       }
       p = (p + step[k]) & uint32_t(S - 1);
     }
@@ -438,6 +458,13 @@ __device__ __forceinline__ void chunk_slots(gb_params const& P, acc_plan const& 
     else if (INSERT) __hip_atomic_store(&misc->overflow, 1, __ATOMIC_RELAXED, WG_SCOPE);  // table full
   }
   if (INSERT && inserted) __hip_atomic_fetch_add(&misc->fill, inserted, __ATOMIC_RELAXED, WG_SCOPE);
+  if (KM == KF_HASHED && INSERT) {  // each record's group, for the verification (coalesced stores)
+#pragma unroll
+    for (int k = 0; k < RPA_CACHED; k++) {
+      int64_t const r = r0 + int64_t(k) * RPA_AGG_BLOCK;
+      if (r < e && slot[k] >= 0) gp(R.rec_slot)[r] = uint32_t(rbase + slot[k]);
+    }
+  }
 }
 
 // Pass A of one chunk: every accumulator, column by column, each accumulator
@@ -558,7 +585,7 @@ __device__ __forceinline__ bool m2_from_sums(int32_t sum_kind, int32_t sum_off, 
   return true;
 }
 
-template <bool GENERAL>
+template <int KM>
 __global__ void __launch_bounds__(RPA_AGG_BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
   k_rpa_aggregate(gb_params const* __restrict__ Pp, rpa_params const* __restrict__ Rp, int64_t bin_begin)
 {
@@ -586,7 +613,8 @@ __global__ void __launch_bounds__(RPA_AGG_BLOCK) __attribute__((amdgpu_waves_per
   if (s == e) return;
   // the first chunk's loads are in flight while the table is initialised
   chunk_in in;
-  chunk_load<true>(A, s + tid, e, in);
+  chunk_load<true, KM>(A, s + tid, e, in);
+  int64_t const rbase = b * R.rep_stride;  // this bin's representative rows (hashed general keys)
 
   // ---- init
   for (int i = tid; i < S; i += RPA_AGG_BLOCK) tkeys[i] = EMPTY_KEY;
@@ -607,6 +635,7 @@ __global__ void __launch_bounds__(RPA_AGG_BLOCK) __attribute__((amdgpu_waves_per
     misc->present0 = 0;
     misc->present1 = 0;
     misc->overflow = 0;
+    misc->rep1     = 0xffffffffu;
   }
   lds_barrier();
 
@@ -615,9 +644,9 @@ __global__ void __launch_bounds__(RPA_AGG_BLOCK) __attribute__((amdgpu_waves_per
     if (__hip_atomic_load(&misc->overflow, __ATOMIC_RELAXED, WG_SCOPE) ||
         __hip_atomic_load(&misc->fill, __ATOMIC_RELAXED, WG_SCOPE) > uint32_t(R.max_fill))
       break;
-    if (c != s) chunk_load<true>(A, c + tid, e, in);
+    if (c != s) chunk_load<true, KM>(A, c + tid, e, in);
     int slot[RPA_CACHED];
-    chunk_slots<true, GENERAL>(P, A, tkeys, misc, S, c + tid, e, in, slot);
+    chunk_slots<true, KM>(P, A, R, tkeys, misc, S, c + tid, e, in, slot, rbase);
     chunk_accumulate(A, lds, c + tid, e, slot, in.flp, in.pf);
   }
   lds_barrier();
@@ -660,8 +689,8 @@ __global__ void __launch_bounds__(RPA_AGG_BLOCK) __attribute__((amdgpu_waves_per
   if (P.need_second_pass && misc->m2_pending) {
     for (int64_t c = s; c < e; c += CHUNK) {
       int slot[RPA_CACHED];
-      chunk_load<false>(A, c + tid, e, in);
-      chunk_slots<false, GENERAL>(P, A, tkeys, misc, S, c + tid, e, in, slot);
+      chunk_load<false, KM>(A, c + tid, e, in);
+      chunk_slots<false, KM>(P, A, R, tkeys, misc, S, c + tid, e, in, slot, rbase);
       chunk_deviations(P, R, lds, c + tid, e, slot, in.flp);
     }
     lds_barrier();
@@ -741,7 +770,11 @@ __global__ void __launch_bounds__(RPA_AGG_BLOCK) __attribute__((amdgpu_waves_per
     bool const in_range = live && pos < cap;
     bool const kvalid   = slot != S;
     if (live && !in_range) astore(R.overflow, 1);
-    if (GENERAL) {  // keys are gathered from the representative rows afterwards
+    if (KM == KF_HASHED) {  // keys are gathered from the representative rows afterwards
+      if (in_range)
+        gp(R.out_rep)[pos] = __hip_atomic_load(gp(R.rep_rows) + rbase + slot, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_AGENT);
+    } else if (KM == KF_GENERAL) {
       if (in_range) gp(R.out_rep)[pos] = uint32_t(tkeys[slot]);
     } else {
       if (in_range) {
@@ -827,19 +860,62 @@ __global__ void __launch_bounds__(RPA_AGG_BLOCK) __attribute__((amdgpu_waves_per
   }
 }
 
-void launch_rpa_aggregate(gb_params const* dP, rpa_params const* dR, rpa_params const& hR, bool general,
+template <int KM>
+static void launch_aggregate_form(gb_params const* dP, rpa_params const* dR, rpa_params const& hR, int64_t bin_begin,
+                                  int64_t bin_end, hipStream_t s)
+{
+  allow_dynamic_lds(reinterpret_cast<void const*>(&k_rpa_aggregate<KM>), hR.lds_bytes);
+  hipLaunchKernelGGL(k_rpa_aggregate<KM>, dim3(bin_end - bin_begin), dim3(RPA_AGG_BLOCK), hR.lds_bytes, s, dP, dR,
+                     bin_begin);
+}
+
+void launch_rpa_aggregate(gb_params const* dP, rpa_params const* dR, rpa_params const& hR, int key_form,
                           int64_t bin_begin, int64_t bin_end, hipStream_t s)
 {
   if (bin_end <= bin_begin) return;
-  void const* fn = general ? reinterpret_cast<void const*>(&k_rpa_aggregate<true>)
-                           : reinterpret_cast<void const*>(&k_rpa_aggregate<false>);
-  allow_dynamic_lds(fn, hR.lds_bytes);
-  if (general)
-    hipLaunchKernelGGL(k_rpa_aggregate<true>, dim3(bin_end - bin_begin), dim3(RPA_AGG_BLOCK), hR.lds_bytes, s, dP,
-                       dR, bin_begin);
-  else
-    hipLaunchKernelGGL(k_rpa_aggregate<false>, dim3(bin_end - bin_begin), dim3(RPA_AGG_BLOCK), hR.lds_bytes, s, dP,
-                       dR, bin_begin);
+  if (key_form == KF_HASHED) launch_aggregate_form<KF_HASHED>(dP, dR, hR, bin_begin, bin_end, s);
+  else if (key_form == KF_GENERAL) launch_aggregate_form<KF_GENERAL>(dP, dR, hR, bin_begin, bin_end, s);
+  else launch_aggregate_form<KF_INLINE>(dP, dR, hR, bin_begin, bin_end, s);
+}
+
+// Exact comparison of every record of the hashed form with its group's
+// representative: rows whose group was found by equal 64-bit words must also
+// have equal keys (nulls equal, NaN equal NaN, -0.0 equal 0.0, strings
+// byte-wise), otherwise verify_failed is set and the call is redone comparing
+// key rows.  Records are read in order; the key loads of different records
+// are independent and in flight together.
+__global__ void __launch_bounds__(256) k_rpa_verify(gb_params const* __restrict__ Pp, rpa_params const* __restrict__ Rp,
+                                                    int64_t records)
+{
+  __shared__ uint32_t block_count;
+  gb_params const& P  = *Pp;
+  rpa_params const& R = *Rp;
+  if (threadIdx.x == 0) block_count = 0;
+  __syncthreads();
+  bool bad       = false;
+  uint32_t count = 0;
+  for (int64_t r = int64_t(blockIdx.x) * 256 + threadIdx.x; r < records; r += int64_t(gridDim.x) * 256) {
+    uint32_t const gs = gp(R.rec_slot)[r];
+    if (gs == 0xffffffffu) continue;  // no record here
+    uint32_t const row = gp(R.rowsB)[r];
+    uint32_t const rep = __hip_atomic_load(gp(R.rep_rows) + gs, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (rep == row) continue;  // the representative itself
+    count++;
+    bad |= !rows_equal(P, int64_t(row), int64_t(rep));
+  }
+  if (__ballot(bad) != 0 && (threadIdx.x & 63) == 0) astore(R.verify_failed, 1);
+  if (count) __hip_atomic_fetch_add(&block_count, count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  __syncthreads();
+  if (threadIdx.x == 0 && block_count)
+    __hip_atomic_fetch_add(gp(R.verified), (unsigned long long)block_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+void launch_rpa_verify(gb_params const* dP, rpa_params const* dR, int64_t records, hipStream_t s)
+{
+  if (records <= 0) return;
+  int64_t blocks = (records + 255) / 256;
+  if (blocks > 16384) blocks = 16384;
+  hipLaunchKernelGGL(k_rpa_verify, dim3(blocks), dim3(256), 0, s, dP, dR, records);
 }
 
 }  // namespace mig

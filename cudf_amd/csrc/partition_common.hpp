@@ -33,6 +33,9 @@ __device__ __forceinline__ uint32_t top_bits(uint32_t h, int nbits)
   return nbits == 0 ? 0u : (h >> (32 - nbits));
 }
 
+// Seed of the second, independent row hash of the hashed general-key form.
+constexpr uint32_t SECOND_ROW_HASH_SEED = 0x2f3b9a57u;
+
 // Partition word of a row under general keys (several columns, strings):
 // its row hash and its row index.  Equal keys are found by comparing rows.
 __device__ __forceinline__ uint64_t general_word(uint32_t h, int64_t row)

@@ -320,8 +320,11 @@ __global__ void __launch_bounds__(RPA_SCATTER_BLOCK) __attribute__((amdgpu_waves
   auto* skeys     = reinterpret_cast<uint64_t*>(stage);
   auto* sflags    = stage + R.stage_flags_off;
   auto* sbins     = reinterpret_cast<uint16_t*>(stage + R.stage_bin_off);
+  auto* srows     = reinterpret_cast<uint32_t*>(stage + R.stage_row_off);  // hashed general keys only
+  bool const hashed     = R.general_hashed != 0;
   uint64_t* const okeys = (SECOND || !R.two_level) ? R.keysB : R.keysA;
   uint8_t* const oflags = (SECOND || !R.two_level) ? R.flagsB : R.flagsA;
+  uint32_t* const orows = (SECOND || !R.two_level) ? R.rowsB : R.rowsA;
   int64_t ntiles;
   if constexpr (SECOND) {
     for (int i = threadIdx.x; i <= F1; i += RPA_SCATTER_BLOCK)
@@ -358,6 +361,7 @@ __global__ void __launch_bounds__(RPA_SCATTER_BLOCK) __attribute__((amdgpu_waves
     uint32_t rank[RPA_MAX_RPT];
     uint64_t kb[RPA_MAX_RPT];
     uint8_t kf[RPA_MAX_RPT];
+    uint32_t rw[RPA_MAX_RPT];  // input row of each record (hashed general keys)
     // all loads of the tile's keys first (validity and key together), then the ranks
 #pragma unroll
     for (int k = 0; k < RPA_MAX_RPT; k++) {
@@ -369,11 +373,20 @@ __global__ void __launch_bounds__(RPA_SCATTER_BLOCK) __attribute__((amdgpu_waves
       if constexpr (SECOND) {
         kf[k] = gp(static_cast<uint8_t const*>(R.flagsA))[row];
         kb[k] = gp(static_cast<uint64_t const*>(R.keysA))[row];
+        if (hashed) rw[k] = gp(static_cast<uint32_t const*>(R.rowsA))[row];
         h     = record_hash(P, kb[k], kf[k]);
         bin[k] = int(top_bits(h, R.log_bins) & uint32_t(fan - 1));
       } else if (P.key_mode == KM_GENERAL) {
-        bool kn;
-        if (!input_key(P, row, kb[k], kn, h)) continue;
+        if (hashed) {  // (row hash, second row hash); the row travels beside it
+          if (P.skip_null_keys && any_key_null(P, row)) continue;
+          uint32_t h2;
+          row_hash2(P.keys, P.num_keys, row, SECOND_ROW_HASH_SEED, h, h2);
+          kb[k] = (uint64_t(h) << 32) | h2;
+          rw[k] = uint32_t(row);
+        } else {
+          bool kn;
+          if (!input_key(P, row, kb[k], kn, h)) continue;
+        }
         kf[k]  = 0;
         bin[k] = int(top_bits(h, R.log_f1));
       } else {
@@ -429,6 +442,7 @@ __global__ void __launch_bounds__(RPA_SCATTER_BLOCK) __attribute__((amdgpu_waves
       if (lp[k] < 0) continue;
       skeys[lp[k]] = kb[k];
       sbins[lp[k]] = uint16_t(bin[k]);
+      if (hashed) srows[lp[k]] = rw[k];
     }
     // values: the prefetched columns, then any others column by column
 #pragma unroll
@@ -453,6 +467,7 @@ __global__ void __launch_bounds__(RPA_SCATTER_BLOCK) __attribute__((amdgpu_waves
       int64_t const dst = int64_t(base[b]) + (lp - int(toff[b]));
       gp(okeys)[dst]    = skeys[lp];
       gp(oflags)[dst]   = sflags[lp];
+      if (hashed) gp(orows)[dst] = srows[lp];
       for (int i = 0; i < ncols; i++) {
         int const v       = R.plan_col[i];
         uint8_t* const ov = (SECOND || !R.two_level) ? R.valsB[v] : R.valsA[v];

@@ -458,7 +458,11 @@ mig_status mig_concatenate(const mig_column_view* tables,
 enum {
   MIG_STRATEGY_AUTO        = 0,
   MIG_STRATEGY_GLOBAL      = 1, /* device-wide hash table, global atomics      */
-  MIG_STRATEGY_PARTITIONED = 2  /* radix partition into LDS-sized bins          */
+  MIG_STRATEGY_PARTITIONED = 2, /* radix partition into LDS-sized bins          */
+  MIG_STRATEGY_PARTITIONED_EXACT = 3 /* the same, general keys compared row by row
+                                        while probing (the form the partitioned
+                                        path falls back to when a verification
+                                        finds different keys with equal hashes) */
 };
 void mig_set_strategy(int32_t strategy);
 

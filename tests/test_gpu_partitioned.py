@@ -259,3 +259,32 @@ def test_general_keys_underestimated_fall_back():
     v = np.arange(n, dtype=np.int64) % 1000
     ng = assert_parity([host_col(k1), s], [(host_col(v), [(AGG["SUM"], 1), (AGG["COUNT_ALL"], 1)])], 0)
     assert ng == tail + 1
+
+
+@pytest.mark.parametrize("form", ["hashed", "exact"])
+@pytest.mark.parametrize("policy", [0, 1])
+def test_general_key_forms(form, policy):
+    """General keys (float64 with NaN and -0.0, nullable int32, nullable
+    strings with empty ones) in both forms of the partitioned path: 64-bit
+    words, with every row that joined a group compared with its
+    representative afterwards, and key rows compared while probing."""
+    from cudf_amd import _lib
+    from tests.gbutil import orc
+    _lib.set_strategy(_lib.STRATEGY_PARTITIONED if form == "hashed" else _lib.STRATEGY_PARTITIONED_EXACT)
+    rng = np.random.default_rng(71 + policy)
+    n = 120_000
+    f = (rng.integers(-200, 200, n) / 4.0)
+    f[rng.random(n) < 0.02] = np.nan
+    f[rng.random(n) < 0.02] = -0.0
+    i = rng.integers(0, 30, n).astype(np.int32)
+    words = [b"", b"a", b"bb", b"x" * 9, b"y" * 33] + [b"w%d" % j for j in range(40)]
+    s = orc.HostColumn.from_strings([words[j] for j in rng.integers(0, len(words), n)], rng.random(n) > 0.03)
+    v = rng.integers(-1000, 1000, n).astype(np.int64)
+    assert_parity([host_col(f), host_col(i, rng.random(n) > 0.04), s],
+                  [(host_col(v, rng.random(n) > 0.1), ALL_NUMERIC_AGGS)], policy)
+    st = _stats()
+    assert st["strategy"] == 2
+    if form == "hashed":
+        assert st["verified_pairs"] > 0
+    else:
+        assert st["verified_pairs"] == 0
