@@ -1155,7 +1155,9 @@ int64_t groupby_call::aggregate_sorted_runs()
   upload(0);
   launch_init_state(dP, p, stream_);
   uint32_t* gid = S->get<uint32_t>(N);
-  launch_sorted_assign(dP, N, tile_counts, gid, stream_);
+  bool const any_nth = std::any_of(p.fields, p.fields + nf_, [](field_desc const& F) { return F.kind == FK_NTH; });
+  uint32_t* group_last = any_nth ? S->get<uint32_t>(std::max<int64_t>(G, 1)) : nullptr;
+  launch_sorted_assign(dP, N, tile_counts, gid, group_last, stream_);
   T->mark(2);
   launch_sorted_aggregate(dP, p, N, gid, num_cus, stream_);
   T->mark(3);
@@ -1166,7 +1168,7 @@ int64_t groupby_call::aggregate_sorted_runs()
     launch_sorted_nunique(dP, f, N, G, gid, S->get<uint32_t>(2 * N), stream_);
   }
   for (int f = 0; f < nf_; f++)
-    if (p.fields[f].kind == FK_NTH) launch_sorted_nth(dP, f, N, G, gid, stream_);
+    if (p.fields[f].kind == FK_NTH) launch_sorted_nth(dP, p, f, G, group_last, stream_);
   T->mark(5);
   uint32_t* map = S->get<uint32_t>(std::max<int64_t>(G, 1));
   launch_iota_u32(map, G, stream_);

@@ -1165,7 +1165,7 @@ __global__ void __launch_bounds__(BLOCK) k_sorted_count(gb_params const* __restr
 
 __global__ void __launch_bounds__(BLOCK)
   k_sorted_assign(gb_params const* __restrict__ Pp, uint32_t const* __restrict__ block_offsets,
-                  uint32_t* __restrict__ gid)
+                  uint32_t* __restrict__ gid, uint32_t* __restrict__ group_last)
 {
   __shared__ uint32_t wave_tot[BLOCK / 64];
   gb_params const& P = *Pp;
@@ -1187,10 +1187,26 @@ __global__ void __launch_bounds__(BLOCK)
       all += wave_tot[w];
     }
     uint32_t const heads_before = running + before + uint32_t(__popcll(bal & below));
+    // A group's rows are one run with no dropped row inside (a dropped row
+    // ends a run), so its last row is the included row whose successor is
+    // dropped, a head or past the end.  Lanes 0..62 read the successor's
+    // flags from the next lane; lane 63's successor is in another wave.
+    bool ends = false;
+    if (group_last != nullptr) {
+      int const next_incl = __shfl_down(int(incl), 1);
+      int const next_head = __shfl_down(int(head), 1);
+      if (incl) {
+        int64_t const nr = row + 1;
+        bool const ni    = lane < 63 ? next_incl != 0 : nr < P.num_rows && sorted_included(P, nr);
+        bool const nh    = lane < 63 ? next_head != 0 : ni && sorted_head(P, nr);
+        ends             = !ni || nh;
+      }
+    }
     if (live) {
       uint32_t const g = !incl ? NO_GROUP : (head ? heads_before : heads_before - 1u);
       gid[row] = g;
       if (head) P.slot_rows[g] = uint64_t(row);
+      if (ends) group_last[g] = uint32_t(row);
     }
     running += all;
     __syncthreads();
@@ -1669,10 +1685,10 @@ void launch_sorted_count(gb_params const* dP, int64_t n, uint32_t* block_counts,
 }
 
 void launch_sorted_assign(gb_params const* dP, int64_t n, uint32_t const* block_offsets, uint32_t* gid,
-                          hipStream_t s)
+                          uint32_t* group_last, hipStream_t s)
 {
   int64_t const nb = (n + SORTED_TILE - 1) / SORTED_TILE;
-  hipLaunchKernelGGL(k_sorted_assign, dim3(nb), dim3(BLOCK), 0, s, dP, block_offsets, gid);
+  hipLaunchKernelGGL(k_sorted_assign, dim3(nb), dim3(BLOCK), 0, s, dP, block_offsets, gid, group_last);
 }
 
 int64_t sorted_tiles(int64_t n) { return (n + SORTED_TILE - 1) / SORTED_TILE; }
@@ -1842,51 +1858,103 @@ __global__ void __launch_bounds__(BLOCK)
 // of another group or a dropped row, and the backward walk skips the dropped
 // rows between the group's last row and the next group.  Cost per group:
 // O(|n|) rows plus rows without a counted value.
+// NTH_ELEMENT on runs.  Group g holds rows first..last (slot_rows[g],
+// group_last[g]) with no dropped row among them, so when every row counts
+// (nulls included, or a value column without nulls) the n-th row is found by
+// index arithmetic, one thread per group.
 __global__ void __launch_bounds__(BLOCK)
-  k_sorted_nth(gb_params const* __restrict__ Pp, int32_t f, int64_t n_rows, int64_t G,
-               uint32_t const* __restrict__ gid)
+  k_sorted_nth_direct(gb_params const* __restrict__ Pp, int32_t f, int64_t G, uint32_t const* __restrict__ group_last)
 {
   int64_t const g = int64_t(blockIdx.x) * BLOCK + threadIdx.x;
   if (g >= G) return;
   gb_params const& P  = *Pp;
   field_desc const& F = P.fields[f];
-  dcol const& c       = P.vals[F.vcol];
   int64_t const nth   = F.aux_sum;
-  bool const include  = F.aux_count != 0;
   int64_t const first = int64_t(P.slot_rows[g]);
+  int64_t const last  = min(int64_t(group_last[g]), P.num_rows - 1);
+  int64_t const size  = last - first + 1;
   uint32_t found      = 0xFFFFFFFFu;
-  int64_t seen        = 0;
-  if (nth >= 0) {
-    for (int64_t r = first; r < n_rows; r++) {
-      uint32_t const rg = gid[r];
-      if (rg == NO_GROUP) continue;
-      if (rg != uint32_t(g)) break;
-      if (include || dcol_valid(c, r)) {
-        if (seen == nth) {
-          found = uint32_t(r);
-          break;
-        }
-        seen++;
-      }
-    }
-  } else {
-    int64_t const end = g + 1 < G ? int64_t(P.slot_rows[g + 1]) : n_rows;
-    for (int64_t r = end - 1; r >= first; r--) {
-      if (gid[r] != uint32_t(g)) continue;  // a dropped row between the group's rows
-      if (include || dcol_valid(c, r)) {
-        if (++seen == -nth) {
-          found = uint32_t(r);
-          break;
-        }
-      }
-    }
-  }
+  if (nth >= 0 ? nth < size : -nth <= size) found = uint32_t(nth >= 0 ? first + nth : last + 1 + nth);
   *reinterpret_cast<uint32_t*>(field_at(F, g)) = found;
 }
 
-void launch_sorted_nth(gb_params const* dP, int32_t f, int64_t n, int64_t G, uint32_t const* gid, hipStream_t s)
+// The r-th (0-based) set bit of `bits`, counted from the low end or, when
+// `high`, from the high end; r < popcount(bits).
+__device__ __forceinline__ int select_bit(uint32_t bits, int r, bool high)
 {
-  if (G > 0) hipLaunchKernelGGL(k_sorted_nth, dim3(unsigned((G + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0, s, dP, f, n, G, gid);
+  if (high) {
+    for (int i = 0; i < r; i++) bits &= ~(0x80000000u >> __clz(bits));
+    return 31 - __clz(bits);
+  }
+  for (int i = 0; i < r; i++) bits &= bits - 1u;
+  return __ffs(bits) - 1;
+}
+
+// NTH_ELEMENT on runs with null values excluded: one wave per group counts
+// the valid bits of the value column's mask over the group's rows, 64 mask
+// words (2,048 rows) per step, from the front for n >= 0 and from the back
+// for n < 0, and stops at the step holding the n-th valid row.
+__global__ void __launch_bounds__(BLOCK)
+  k_sorted_nth_scan(gb_params const* __restrict__ Pp, int32_t f, int64_t G, uint32_t const* __restrict__ group_last)
+{
+  gb_params const& P  = *Pp;
+  field_desc const& F = P.fields[f];
+  dcol const& c       = P.vals[F.vcol];
+  auto const mask     = gp(c.mask);
+  int64_t const nth   = F.aux_sum;
+  bool const back     = nth < 0;
+  int64_t const want  = back ? -nth - 1 : nth;  // 0-based among the valid rows, from the chosen end
+  int const lane      = threadIdx.x & 63;
+  int64_t const waves = int64_t(gridDim.x) * (BLOCK / 64);
+  for (int64_t g = (int64_t(blockIdx.x) * BLOCK + threadIdx.x) >> 6; g < G; g += waves) {
+    int64_t const b0 = int64_t(P.slot_rows[g]) + c.mask_offset;  // the group's bits b0..b1 of the mask
+    int64_t const b1 = min(int64_t(group_last[g]), P.num_rows - 1) + c.mask_offset;
+    int64_t const w0 = b0 >> 5, w1 = b1 >> 5;
+    int64_t seen     = 0;
+    uint32_t found   = 0xFFFFFFFFu;
+    for (int64_t k = 0; k <= w1 - w0; k += 64) {
+      int64_t const w = back ? w1 - k - lane : w0 + k + lane;
+      uint32_t bits   = 0;
+      if (w >= w0 && w <= w1) {
+        bits = mask[w];
+        if (w == w0) bits &= ~0u << (b0 & 31);
+        if (w == w1 && (b1 & 31) != 31) bits &= (2u << (b1 & 31)) - 1u;
+      }
+      int const cnt = __popc(bits);
+      int upto      = cnt;  // inclusive prefix over the lanes
+#pragma unroll
+      for (int d = 1; d < 64; d <<= 1) {
+        int const t = __shfl_up(upto, d);
+        if (lane >= d) upto += t;
+      }
+      int const total = __shfl(upto, 63);
+      if (seen + total > want) {
+        int64_t const r = want - seen;  // < total
+        bool const mine = upto - cnt <= r && r < upto;
+        uint32_t row    = 0;
+        if (mine) row = uint32_t((w << 5) + select_bit(bits, int(r - (upto - cnt)), back) - c.mask_offset);
+        int const src = __ffsll(static_cast<unsigned long long>(__ballot(mine))) - 1;
+        found         = uint32_t(__shfl(int(row), src));
+        break;
+      }
+      seen += total;
+    }
+    if (lane == 0) *reinterpret_cast<uint32_t*>(field_at(F, g)) = found;
+  }
+}
+
+void launch_sorted_nth(gb_params const* dP, gb_params const& hP, int32_t f, int64_t G, uint32_t const* group_last,
+                       hipStream_t s)
+{
+  if (G <= 0) return;
+  field_desc const& F = hP.fields[f];
+  if (F.aux_count != 0 || hP.vals[F.vcol].mask == nullptr) {
+    hipLaunchKernelGGL(k_sorted_nth_direct, dim3(unsigned((G + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0, s, dP, f, G,
+                       group_last);
+  } else {
+    int64_t const blocks = std::min<int64_t>((G + BLOCK / 64 - 1) / (BLOCK / 64), int64_t(1) << 18);
+    hipLaunchKernelGGL(k_sorted_nth_scan, dim3(unsigned(blocks)), dim3(BLOCK), 0, s, dP, f, G, group_last);
+  }
 }
 
 void launch_sorted_nunique(gb_params const* dP, int32_t f, int64_t n, int64_t G, uint32_t const* gid,

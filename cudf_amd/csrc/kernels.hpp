@@ -341,9 +341,10 @@ void launch_count_valid_list(uint32_t const* const* masks, int count, int64_t n,
 int64_t sorted_tiles(int64_t n);
 // run heads per tile of rows
 void launch_sorted_count(gb_params const* dP, int64_t n, uint32_t* block_counts, hipStream_t s);
-// group of every row (0xFFFFFFFF when dropped) and slot_rows[g] = first row of group g
+// group of every row (0xFFFFFFFF when dropped), slot_rows[g] = first row of group g
+// and, when group_last is not null, group_last[g] = its last row
 void launch_sorted_assign(gb_params const* dP, int64_t n, uint32_t const* block_offsets, uint32_t* gid,
-                          hipStream_t s);
+                          uint32_t* group_last, hipStream_t s);
 // wave-range reductions with plain stores for groups inside one range
 void launch_sorted_aggregate(gb_params const* dP, gb_params const& hP, int64_t n, uint32_t const* gid, int num_cus,
                              hipStream_t s);
@@ -354,8 +355,10 @@ void launch_sorted_second_pass(gb_params const* dP, gb_params const& hP, int64_t
 // cleared here) counts each group's distinct values.
 void launch_sorted_nunique(gb_params const* dP, int32_t f, int64_t n, int64_t G, uint32_t const* gid,
                            uint32_t* table, hipStream_t s);
-// NTH_ELEMENT of field `f` on runs: the row of each group's n-th counted value.
-void launch_sorted_nth(gb_params const* dP, int32_t f, int64_t n, int64_t G, uint32_t const* gid, hipStream_t s);
+// NTH_ELEMENT of field `f` on runs: the row of each group's n-th counted value,
+// from the groups' first and last rows (launch_sorted_assign's group_last).
+void launch_sorted_nth(gb_params const* dP, gb_params const& hP, int32_t f, int64_t G, uint32_t const* group_last,
+                       hipStream_t s);
 void launch_iota_u32(uint32_t* out, int64_t n, hipStream_t s);
 
 // ---- sort-based aggregations (sort_kernels.hip, label kernels in

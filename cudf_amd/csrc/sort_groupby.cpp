@@ -121,7 +121,7 @@ std::unique_ptr<mig_groupby_result> sort_groupby(mig_column_view const* keys, in
     G = int64_t(hg);
     P.slot_rows = S.get<uint64_t>(G + 2);  // first row of each run (written by the assign kernel)
     upload();
-    launch_sorted_assign(dP, N, tile_counts, gid, stream);
+    launch_sorted_assign(dP, N, tile_counts, gid, nullptr, stream);
     launch_clamp_groups(gid, N, uint32_t(G), stream);  // dropped rows: G
   } else {
     // the device-wide table at load <= 2/3, so no probe sequence can fail

@@ -311,6 +311,66 @@ def test_sorted_nth_element_parity(vdt):
                 assert got[gi] == vu[want]
 
 
+def _nth_expected(first, last, vvalid, m, inc):
+    """Row of the m-th counted value of rows first..last, or None."""
+    rows = np.arange(first, last + 1) if inc else first + np.flatnonzero(vvalid[first:last + 1])
+    return int(rows[m]) if -len(rows) <= m < len(rows) else None
+
+
+@pytest.mark.timeout(300)
+def test_sorted_nth_trailing_null_keys_and_one_large_group():
+    """20M pre-sorted rows: a 3M-row group, 2,000 small groups, then 15M
+    null keys; value column sliced at row 13 (its mask starts mid-word) with
+    30% nulls; n from the front and the back, beyond the group sizes, nulls
+    included and excluded, under both null policies.  Values are row numbers,
+    so each result names the row chosen.  The call must not walk the trailing
+    null-key rows or the large group row by row: a bounded time."""
+    import time
+    import torch
+    from cudf_amd import Column, GroupBy, GroupByRequest, NullPolicy, Sorted, Table
+    from cudf_amd.aggregation import Aggregation
+    rng = np.random.default_rng(2718)
+    n_big, n_null, n_small = 3_000_000, 15_000_000, 2000
+    sizes = rng.integers(1, 200, n_small)
+    n = n_big + int(sizes.sum()) + n_null
+    k = np.concatenate([np.zeros(n_big, np.int64), np.repeat(np.arange(1, n_small + 1), sizes),
+                        np.full(n_null, n_small + 1)]).astype(np.int64)
+    kvalid = np.arange(n) < n - n_null
+    off = 13
+    vfull = (np.arange(n + off, dtype=np.int64) - off).astype(np.int32)  # value = row number
+    vvfull = rng.random(n + off) > 0.3
+    vvalid = vvfull[off:]
+    kcol = Column.from_numpy(k, kvalid)
+    vcol = Column.from_numpy(vfull, vvfull).slice(off, n)
+    ms = (0, 7, -1, -2, 1_000_000, -1_000_000, 2_100_000, -2_100_000, 3_000_000)
+    params = [(m, inc) for m in ms for inc in (0, 1)]
+    aggs = [Aggregation(AGG["NTH_ELEMENT"], null_handling=NullPolicy(inc), n=m) for m, inc in params]
+    starts = np.concatenate([[0], n_big + np.concatenate([[0], np.cumsum(sizes)])])
+    for policy in (0, 1):
+        gb = GroupBy(Table([kcol]), NullPolicy(policy), Sorted.YES)
+        gb.aggregate([GroupByRequest(vcol, aggs)])  # warm-up
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        out_keys, out_res = gb.aggregate([GroupByRequest(vcol, aggs)])
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        print(f"policy {policy}: {n} rows, {out_keys.columns()[0].size} groups, {len(aggs)} NTH in {dt * 1e3:.1f} ms",
+              flush=True)
+        bounds = list(starts) + ([n] if policy == 1 else [])  # the null-key group under INCLUDE
+        groups = [(bounds[i], bounds[i + 1] - 1) for i in range(len(bounds) - 1)]
+        assert out_keys.columns()[0].size == len(groups)
+        for a, (m, inc) in enumerate(params):
+            got, valid = out_res[0].columns()[a].to_numpy()
+            valid = np.ones(len(got), bool) if valid is None else valid
+            for gi, (first, last) in enumerate(groups):
+                want = _nth_expected(first, last, vvalid, m, inc)
+                ok = want is not None and bool(vvalid[want])
+                assert valid[gi] == ok, (policy, m, inc, gi, want)
+                if ok:
+                    assert int(got[gi]) == want, (policy, m, inc, gi)
+        assert dt < 0.25, f"{dt:.3f} s"
+
+
 def test_nth_element_errors():
     from cudf_amd import _lib
     k = np.arange(100, dtype=np.int64)
