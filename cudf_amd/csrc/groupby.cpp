@@ -498,6 +498,20 @@ double estimate_distinct(double n, double d, double population)
   return hi;
 }
 
+// The larger of the equal-frequency estimate and Chao's lower bound
+// d + f1 (f1 - 1) / (2 (f2 + 1)) from the values seen once (f1) and twice
+// (f2) in the sample (A. Chao, Scand. J. Statist. 11 (1984) 265-270).  The
+// two agree for equally frequent values; under skew (Zipf-like keys) the hot
+// values fill the sample and the equal-frequency model reads the tail as
+// absent, where the singletons still show it.
+double estimate_distinct_skewed(double n, double d, double f1, double f2, double population, bool* lower_bound)
+{
+  double const chao    = d + f1 * std::max(f1 - 1.0, 0.0) / (2.0 * (f2 + 1.0));
+  double const uniform = estimate_distinct(n, d, population);
+  if (lower_bound) *lower_bound = chao > 1.5 * uniform;  // skewed: the tail may hold many more
+  return std::min(population, std::max(uniform, chao));
+}
+
 namespace {
 
 owned_column make_owned_fixed(device_allocator& A, int32_t type_id, int64_t n, bool with_mask)
@@ -620,6 +634,17 @@ class groupby_call {
   double est_    = 0.0;  // estimated groups
   double pop_    = 0.0;  // rows the group-by keeps (estimated)
   uint32_t max_mult_ = 0;  // most sampled rows sharing one key hash
+  double sampled_    = 0.0;  // sampled rows the group-by keeps
+  bool est_low_      = false;  // the estimate is a lower bound (skewed keys)
+  // A key on at least 1/512 of the sampled rows among at most 2M groups: one
+  // bin of the partitioned path would hold that key's rows (a straggler
+  // workgroup), while the device-wide table of that many groups stays in the
+  // last-level cache and workgroup-local tables absorb the hot key.
+  bool heavy_hitters() const
+  {
+    return sampled_ > 0.0 && double(max_mult_) * 512.0 >= sampled_ && max_mult_ >= 16 &&
+           est_ <= double(1 << 21);
+  }
   int n_str_keys_ = 0;
   std::vector<string_out> string_outs_;
   int borrowed_next_ = 0;  // allocation-free calls: next caller output column
@@ -886,19 +911,22 @@ void groupby_call::estimate_groups()
     int64_t const ns = std::min<int64_t>(N, SAMPLE_ROWS);
     uint32_t const tslots = uint32_t(next_pow2(uint64_t(2 * ns)));
     auto* table    = S->get<uint64_t>(tslots);
-    auto* counters = S->get<uint32_t>(3);
+    auto* counters = S->get<uint32_t>(5);
     MIG_HIP_TRY(hipMemsetAsync(table, 0, sizeof(uint64_t) * tslots, stream_));
-    MIG_HIP_TRY(hipMemsetAsync(counters, 0, sizeof(uint32_t) * 3, stream_));
+    MIG_HIP_TRY(hipMemsetAsync(counters, 0, sizeof(uint32_t) * 5, stream_));
     upload(0);
     launch_sample_hash(dP, N, ns, table, tslots - 1, counters, stream_);
     auto* h = reinterpret_cast<uint32_t*>(pin + 2 * sizeof(gb_params));
-    MIG_HIP_TRY(hipMemcpyAsync(h, counters, 12, hipMemcpyDeviceToHost, stream_));
+    MIG_HIP_TRY(hipMemcpyAsync(h, counters, 20, hipMemcpyDeviceToHost, stream_));
     MIG_HIP_TRY(hipStreamSynchronize(stream_));
     double const sampled = double(h[0]), d = double(h[1]);
     max_mult_ = h[2];
+    sampled_  = sampled;
     // population: rows that are not dropped, scaled from the sample
     pop_ = double(N) * (sampled / double(ns));
-    est_ = ns == N ? d : (sampled > 0 ? estimate_distinct(sampled, d, std::max(pop_, d)) : 1.0);
+    if (ns == N) est_ = d;
+    else if (sampled <= 0) est_ = 1.0;
+    else est_ = estimate_distinct_skewed(sampled, d, h[3], h[4], std::max(pop_, d), &est_low_);
   }
   stats_[STAT_EST_GROUPS] = est_;
 }
@@ -1234,7 +1262,7 @@ int64_t groupby_call::aggregate_partitioned()
   bool use_rpa = false;
   if (forced == MIG_STRATEGY_PARTITIONED || forced == MIG_STRATEGY_PARTITIONED_EXACT) use_rpa = rpa_ok;
   else if (forced == MIG_STRATEGY_AUTO)  // and the estimate fits the largest partition
-    use_rpa = rpa_ok && N >= (1 << 20) && est_ >= 65536 &&
+    use_rpa = rpa_ok && N >= (1 << 20) && est_ >= 65536 && !heavy_hitters() &&
               est_ <= double(int64_t(1) << RPA_MAX_LOG_BINS) * max_fill;
   if (!use_rpa) return -1;
 
@@ -1505,7 +1533,10 @@ int64_t groupby_call::aggregate_global()
   uint64_t const max_cap = next_pow2(uint64_t(2 * N + 2));
   int64_t lds_blocks = 0;
   p.lds_bytes        = 0;
-  if (p.key_mode == KM_INLINE && !any_arg_ && est_ <= double(LDS_PREAGG_MAX_GROUPS) &&
+  // workgroup-local tables for few groups, or for hot keys among many
+  // groups (rows of keys that find no room go to the device-wide table)
+  bool const hot = heavy_hitters() && est_ > double(LDS_PREAGG_MAX_GROUPS);
+  if (p.key_mode == KM_INLINE && !any_arg_ && (est_ <= double(LDS_PREAGG_MAX_GROUPS) || hot) &&
       forced != MIG_STRATEGY_PARTITIONED) {
     int32_t S_l = 256;
     while (S_l < LDS_PREAGG_SLOTS && double(S_l) * 0.75 < est_ * 1.5) S_l *= 2;
@@ -1532,13 +1563,16 @@ int64_t groupby_call::aggregate_global()
       p.lds_bytes = bytes;
       int64_t const per_cu = std::max<int64_t>(1, std::min<int64_t>(8, (160 * 1024) / bytes));
       // at least ~16 rows per group and workgroup, so that merging pays off
-      int64_t const rows_per_wg = std::max<int64_t>(4096, int64_t(16.0 * std::max(est_, 1.0)));
+      // (hot keys: the tables hold the hot keys, the rest goes through)
+      int64_t const rows_per_wg = hot ? 4096 : std::max<int64_t>(4096, int64_t(16.0 * std::max(est_, 1.0)));
       lds_blocks = std::max<int64_t>(1, std::min<int64_t>(int64_t(num_cus) * per_cu,
                                                            (N + rows_per_wg - 1) / rows_per_wg));
     }
   }
   stats_[STAT_LDS_PREAGG] = lds_blocks > 0 ? 1.0 : 0.0;
-  uint64_t cap = std::max<uint64_t>(64, next_pow2(uint64_t(est_ * 2.6) + 1));
+  // a lower-bound estimate (skewed keys) starts 4x larger: a table that
+  // overflows costs the rows processed before the rebuild
+  uint64_t cap = std::max<uint64_t>(64, next_pow2(uint64_t(est_ * (est_low_ ? 10.4 : 2.6)) + 1));
   cap = std::min(cap, max_cap);
   int32_t* d_flags = S->get<int32_t>(4);  // [0] overflow, [1..2] special occupancy
   uint32_t* block_counts = nullptr;

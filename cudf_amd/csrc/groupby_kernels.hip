@@ -101,12 +101,35 @@ __global__ void __launch_bounds__(BLOCK)
   }
 }
 
+// Frequencies of frequencies of the sample: counters[3] = hashes seen once,
+// counters[4] = hashes seen twice.
+__global__ void __launch_bounds__(BLOCK)
+  k_sample_freqs(uint64_t const* __restrict__ table, uint32_t slots, uint32_t* counters)
+{
+  __shared__ uint32_t red[BLOCK / 64];
+  uint32_t once = 0, twice = 0;
+  for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < slots; i += gridDim.x * BLOCK) {
+    uint32_t const c = uint32_t(table[i] >> 32);
+    once += c == 1u;
+    twice += c == 2u;
+  }
+  uint32_t const t1 = block_sum_u32<BLOCK>(once, red);
+  uint32_t const t2 = block_sum_u32<BLOCK>(twice, red);
+  if (threadIdx.x == 0) {
+    if (t1) aadd(counters + 3, t1);
+    if (t2) aadd(counters + 4, t2);
+  }
+}
+
 void launch_sample_hash(gb_params const* dP, int64_t n, int64_t nsample, uint64_t* table,
                         uint32_t table_mask, uint32_t* counters, hipStream_t s)
 {
   int64_t blocks = (nsample + BLOCK - 1) / BLOCK;
   hipLaunchKernelGGL(k_sample_hash, dim3(blocks), dim3(BLOCK), 0, s, dP, n, nsample, table, table_mask,
                      counters);
+  uint32_t const slots = table_mask + 1u;
+  hipLaunchKernelGGL(k_sample_freqs, dim3(std::min<uint32_t>(256u, (slots + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0, s,
+                     table, slots, counters);
 }
 
 // ---------------------------------------------------------------------------
@@ -168,6 +191,7 @@ __device__ __forceinline__ int64_t insert_inline(gb_params const& P, uint64_t ke
   }
   uint64_t s = h & P.cap_mask;
   for (uint32_t i = 0; i < P.probe_limit; i++) {
+    if ((i & 63u) == 63u && aload(P.overflow) != 0) return -1;  // the table is being rebuilt
     uint64_t cur = aload(&P.slot_keys[s]);
     if (cur == key) return int64_t(s);
     if (cur == EMPTY_KEY) {
@@ -198,6 +222,7 @@ __device__ __forceinline__ int64_t insert_general(gb_params const& P, int64_t ro
   uint64_t const mine = (uint64_t(h) << 32) | uint32_t(row);
   uint64_t s = h & P.cap_mask;
   for (uint32_t i = 0; i < P.probe_limit; i++) {
+    if ((i & 63u) == 63u && aload(P.overflow) != 0) return -1;  // the table is being rebuilt
     uint64_t cur = aload(&P.slot_rows[s]);
     if (cur == EMPTY_ROW) {
       uint64_t old = acas64(&P.slot_rows[s], EMPTY_ROW, mine);
@@ -321,6 +346,7 @@ __global__ void __launch_bounds__(BLOCK) k_aggregate_global(gb_params const* __r
   int64_t const n = P.num_rows;
   for (int64_t row = int64_t(blockIdx.x) * BLOCK + threadIdx.x; row < n;
        row += int64_t(gridDim.x) * BLOCK) {
+    if (aload(P.overflow) != 0) break;  // the call rebuilds a larger table: stop early
     int64_t const slot = locate<KM, true>(P, row);
     if (slot < 0) continue;
     update_row(P, slot, row);
@@ -336,10 +362,38 @@ static int64_t row_grid(int64_t n, int num_cus, int per_cu)
   return blocks;
 }
 
+template <int KM>
+__global__ void __launch_bounds__(BLOCK) k_aggregate_global_combined(gb_params const* __restrict__ Pp);
+
+// Whether every accumulator of the plan combines partials (the wave-combined
+// form of k_aggregate_global) and slot numbers fit its 26-bit sort keys.
+static bool combinable(gb_params const& hP)
+{
+  if (hP.cap + 2 >= (uint64_t(1) << 26)) return false;
+  for (int f = 0; f < hP.num_fields; f++) {
+    switch (hP.fields[f].kind) {
+      case FK_ARGMIN:
+      case FK_ARGMAX:
+      case FK_NUNIQUE:
+      case FK_NUNIQUE_NULLS:
+      case FK_NTH: return false;
+      default: break;
+    }
+  }
+  return true;
+}
+
 void launch_aggregate(gb_params const* dP, gb_params const& hP, int num_cus, hipStream_t s)
 {
   if (hP.num_rows <= 0) return;
   int64_t blocks = row_grid(hP.num_rows, num_cus, 16);
+  if (combinable(hP)) {
+    if (hP.key_mode == KM_INLINE)
+      hipLaunchKernelGGL(k_aggregate_global_combined<KM_INLINE>, dim3(blocks), dim3(BLOCK), 0, s, dP);
+    else
+      hipLaunchKernelGGL(k_aggregate_global_combined<KM_GENERAL>, dim3(blocks), dim3(BLOCK), 0, s, dP);
+    return;
+  }
   if (hP.key_mode == KM_INLINE)
     hipLaunchKernelGGL(k_aggregate_global<KM_INLINE>, dim3(blocks), dim3(BLOCK), 0, s, dP);
   else
@@ -497,6 +551,7 @@ __global__ void __launch_bounds__(LBLOCK) k_aggregate_lds(gb_params const* __res
   dcol const& kc = P.keys[0];
   for (int64_t row = int64_t(blockIdx.x) * LBLOCK + threadIdx.x; row < P.num_rows;
        row += int64_t(gridDim.x) * LBLOCK) {
+    if (aload(P.overflow) != 0) break;  // the call rebuilds a larger table: stop early
     int id        = -1;
     uint64_t bits = 0;
     uint32_t h    = 0;
@@ -534,6 +589,7 @@ __global__ void __launch_bounds__(LBLOCK) k_aggregate_lds(gb_params const* __res
     }
   }
   __syncthreads();
+  if (aload(P.overflow) != 0) return;  // this table is discarded
   // merge the partials into the global table
   for (int i = threadIdx.x; i < S + 2; i += LBLOCK) {
     int64_t slot;
@@ -1383,6 +1439,69 @@ __device__ __forceinline__ uint64_t segmented_scan(int32_t kind, uint64_t part, 
     case OP_MUL_F32: return seg_scan<OP_MUL_F32>(part, g);
     case OP_MUL_F64: return seg_scan<OP_MUL_F64>(part, g);
     default: return seg_scan<OP_ADD_I64>(part, g);
+  }
+}
+
+// Ascending bitonic sort of one 32-bit key per lane across the wave.
+__device__ __forceinline__ uint32_t wave_sort_u32(uint32_t key)
+{
+  int const lane = threadIdx.x & 63;
+#pragma unroll
+  for (int k = 2; k <= 64; k <<= 1) {
+#pragma unroll
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      uint32_t const other = uint32_t(__shfl_xor(int(key), j));
+      bool const keep_min  = ((lane & j) == 0) == ((lane & k) == 0);
+      key                  = keep_min ? min(key, other) : max(key, other);
+    }
+  }
+  return key;
+}
+
+// The device-wide table with equal groups of a wave combined before the
+// atomics: each wave sorts its rows' (slot, lane) pairs, so rows of one group
+// sit in adjacent lanes; every accumulator's row contributions are reduced
+// over each run by the segmented scan, and the run's last lane applies the
+// total with one atomic.  Hot keys then cost one atomic per wave and field
+// instead of one per row.
+template <int KM>
+__global__ void __launch_bounds__(BLOCK) k_aggregate_global_combined(gb_params const* __restrict__ Pp)
+{
+  constexpr uint32_t NONE = (1u << 26) - 1u;  // sort key of a lane without a group
+  gb_params const& P = *Pp;
+  int64_t const n    = P.num_rows;
+  int const lane     = threadIdx.x & 63;
+  for (int64_t base = int64_t(blockIdx.x) * BLOCK; base < n; base += int64_t(gridDim.x) * BLOCK) {
+    // the call rebuilds a larger table: stop early (one load of the flag per
+    // wave, so the whole wave leaves together)
+    if (__builtin_amdgcn_readfirstlane(int(aload(P.overflow))) != 0) break;
+    int64_t const row  = base + threadIdx.x;
+    int64_t const slot = row < n ? locate<KM, true>(P, row) : -1;
+    uint32_t const key = wave_sort_u32(((slot < 0 ? NONE : uint32_t(slot)) << 6) | uint32_t(lane));
+    uint32_t const g   = key >> 6;
+    int64_t const srow = base + (threadIdx.x & ~63) + int64_t(key & 63u);  // the row this lane now holds
+    bool const has     = g != NONE;
+    uint32_t const gn  = uint32_t(__shfl_down(int(g), 1));
+    bool const tail    = has && (lane == 63 || gn != g);
+    if (P.need_count_all) {
+      uint64_t const part = seg_scan<OP_ADD_I64>(has ? 1u : 0u, g);
+      if (tail) aadd(count_at(P, g), uint32_t(part));
+    }
+    for (int vc = 0; vc < P.num_vals; vc++) {
+      int const fb = P.val_field_begin[vc], fe = P.val_field_begin[vc + 1];
+      if (fb == fe) continue;
+      dcol const& c    = P.vals[vc];
+      bool const valid = has && dcol_valid(c, srow);
+      loaded_value v{0, 0.0, 0.0f, 0};
+      if (valid) v = load_value(c, srow);
+      for (int f = fb; f < fe; f++) {
+        field_desc const& F = P.fields[f];
+        if (F.kind == FK_M2) continue;  // the second pass
+        uint64_t const ident = field_identity(F.kind);
+        uint64_t const part  = segmented_scan(F.kind, valid ? sorted_part(F.kind, v) : ident, g, 0);
+        if (tail && part != ident) acc_combine(F.kind, field_at(F, g), part);
+      }
+    }
   }
 }
 

@@ -31,5 +31,9 @@ int32_t field_width(int32_t fk);
 uint64_t next_pow2(uint64_t x);
 // Distinct values of a population from `d` distinct among `n` samples.
 double estimate_distinct(double n, double d, double population);
+// max(estimate_distinct, Chao's bound from the sample's singletons f1 and doubletons f2)
+// (*lower_bound, when given: Chao's bound decided, so the true count may be far larger)
+double estimate_distinct_skewed(double n, double d, double f1, double f2, double population,
+                                bool* lower_bound = nullptr);
 
 }  // namespace mig

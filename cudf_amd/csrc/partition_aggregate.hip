@@ -398,6 +398,7 @@ __device__ __forceinline__ void chunk_slots(gb_params const& P, acc_plan const& 
     fl[k]  = uint8_t(in.flp >> (8 * k));
   }
   uint32_t pending = 0;
+  uint32_t reps    = 0;  // records that became their group's representative
   uint32_t pos[RPA_CACHED], step[RPA_CACHED];
 #pragma unroll
   for (int k = 0; k < RPA_CACHED; k++) {
@@ -415,8 +416,10 @@ __device__ __forceinline__ void chunk_slots(gb_params const& P, acc_plan const& 
       if (KM == KF_HASHED && INSERT) {  // the first such row represents the group
         uint32_t expected = 0xffffffffu;
         if (__hip_atomic_compare_exchange_strong(&misc->rep1, &expected, in.row[k], __ATOMIC_RELAXED,
-                                                 __ATOMIC_RELAXED, WG_SCOPE))
+                                                 __ATOMIC_RELAXED, WG_SCOPE)) {
           __hip_atomic_store(gp(R.rep_rows) + rbase + S + 1, in.row[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          reps |= 1u << k;
+        }
       }
     } else {
       uint32_t const h = GENERAL ? uint32_t(key[k] >> 32) : inline_key_hash(A.key_storage, key[k]);
@@ -445,8 +448,10 @@ __device__ __forceinline__ void chunk_slots(gb_params const& P, acc_plan const& 
         if (__hip_atomic_compare_exchange_strong(tkeys + p, &expected, key[k], __ATOMIC_RELAXED,
                                                  __ATOMIC_RELAXED, WG_SCOPE)) {
           inserted++;
-          if (KM == KF_HASHED)  // this row represents the new group
+          if (KM == KF_HASHED) {  // this row represents the new group
             __hip_atomic_store(gp(R.rep_rows) + rbase + p, in.row[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            reps |= 1u << k;
+          }
           break;
         }
         if (expected == key[k]) break;
@@ -458,11 +463,13 @@ __device__ __forceinline__ void chunk_slots(gb_params const& P, acc_plan const& 
     else if (INSERT) __hip_atomic_store(&misc->overflow, 1, __ATOMIC_RELAXED, WG_SCOPE);  // table full
   }
   if (INSERT && inserted) __hip_atomic_fetch_add(&misc->fill, inserted, __ATOMIC_RELAXED, WG_SCOPE);
-  if (KM == KF_HASHED && INSERT) {  // each record's group, for the verification (coalesced stores)
+  if (KM == KF_HASHED && INSERT) {
+    // each record's group for the verification (coalesced stores); a group's
+    // representative is marked 0xFFFFFFFF, having nothing to compare
 #pragma unroll
     for (int k = 0; k < RPA_CACHED; k++) {
       int64_t const r = r0 + int64_t(k) * RPA_AGG_BLOCK;
-      if (r < e && slot[k] >= 0) gp(R.rec_slot)[r] = uint32_t(rbase + slot[k]);
+      if (r < e && slot[k] >= 0) gp(R.rec_slot)[r] = ((reps >> k) & 1u) ? 0xffffffffu : uint32_t(rbase + slot[k]);
     }
   }
 }
@@ -896,10 +903,10 @@ __global__ void __launch_bounds__(256) k_rpa_verify(gb_params const* __restrict_
   uint32_t count = 0;
   for (int64_t r = int64_t(blockIdx.x) * 256 + threadIdx.x; r < records; r += int64_t(gridDim.x) * 256) {
     uint32_t const gs = gp(R.rec_slot)[r];
-    if (gs == 0xffffffffu) continue;  // no record here
+    if (gs == 0xffffffffu) continue;  // no record here, or a representative
     uint32_t const row = gp(R.rowsB)[r];
     uint32_t const rep = __hip_atomic_load(gp(R.rep_rows) + gs, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (rep == row) continue;  // the representative itself
+    if (rep == row) continue;
     count++;
     bad |= !rows_equal(P, int64_t(row), int64_t(rep));
   }
@@ -914,7 +921,7 @@ void launch_rpa_verify(gb_params const* dP, rpa_params const* dR, int64_t record
 {
   if (records <= 0) return;
   int64_t blocks = (records + 255) / 256;
-  if (blocks > 16384) blocks = 16384;
+  if (blocks > 65536) blocks = 65536;
   hipLaunchKernelGGL(k_rpa_verify, dim3(blocks), dim3(256), 0, s, dP, dR, records);
 }
 

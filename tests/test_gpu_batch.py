@@ -171,7 +171,10 @@ def test_batch_partitioned_instances(null_policy):
     from cudf_amd._lib import last_stats
     inst = make_instances(23, 6, 5_000, big=[(1_200_000, 500_000)])
     plan = run_batch(inst, [REQ0, REQ1], null_policy)
-    assert last_stats()["strategy"] == 2  # the big instance's call was partitioned
+    # the big instance's call was partitioned; with null keys included, its
+    # null group holds 3% of the rows, a hot key that the device-wide table
+    # with workgroup-local tables takes instead
+    assert last_stats()["strategy"] == (2 if null_policy == 0 else 1)
     check_all(plan, inst, [REQ0, REQ1], null_policy)
     plan.run()
     check_all(plan, inst, [REQ0, REQ1], null_policy)
