@@ -947,11 +947,15 @@ void launch_outputs(gb_params const* dP, gb_params const& hP, uint32_t const* ma
 constexpr int GATHER_KEYS_AT_ONCE = 4;
 
 __device__ __forceinline__ void gather_key_column(gb_params const& P, int c, int64_t g, int64_t G, bool live,
-                                                  bool v, uint64_t x, uint64_t e, int32_t* str_sizes, int& nstr)
+                                                  bool v, uint64_t x, uint64_t e, int32_t* str_sizes,
+                                                  int64_t* str_begin, int& nstr)
 {
   dcol const& kc = P.keys[c];
   if (kc.storage == ST_STRING) {
-    if (live) str_sizes[int64_t(nstr) * G + g] = v ? int32_t(e - x) : 0;
+    if (live) {
+      str_sizes[int64_t(nstr) * G + g] = v ? int32_t(e - x) : 0;
+      str_begin[int64_t(nstr) * G + g] = int64_t(x);
+    }
     nstr++;
   } else if (live) {
     store_width(P.out_keys[c], g, storage_width(kc.storage), x);
@@ -972,7 +976,7 @@ __device__ __forceinline__ uint64_t raw_key(dcol const& kc, int64_t r)
 
 __global__ void __launch_bounds__(BLOCK)
   k_gather_keys(gb_params const* __restrict__ Pp, uint32_t const* __restrict__ rep_rows, int64_t G,
-                int32_t* str_sizes)
+                int32_t* str_sizes, int64_t* str_begin)
 {
   gb_params const& P = *Pp;
   int64_t const g    = int64_t(blockIdx.x) * BLOCK + threadIdx.x;
@@ -1001,7 +1005,7 @@ __global__ void __launch_bounds__(BLOCK)
     }
 #pragma unroll
     for (int c = 0; c < GATHER_KEYS_AT_ONCE; c++)
-      if (c < P.num_keys) gather_key_column(P, c, g, G, live, v[c], x[c], e[c], str_sizes, nstr);
+      if (c < P.num_keys) gather_key_column(P, c, g, G, live, v[c], x[c], e[c], str_sizes, str_begin, nstr);
     return;
   }
   for (int c = 0; c < P.num_keys; c++) {
@@ -1018,16 +1022,16 @@ __global__ void __launch_bounds__(BLOCK)
         x = raw_key(kc, r);
       }
     }
-    gather_key_column(P, c, g, G, live, v, x, e, str_sizes, nstr);
+    gather_key_column(P, c, g, G, live, v, x, e, str_sizes, str_begin, nstr);
   }
 }
 
 void launch_gather_keys(gb_params const* dP, gb_params const& hP, uint32_t const* rep_rows,
-                        int64_t G, int32_t* str_sizes, hipStream_t s)
+                        int64_t G, int32_t* str_sizes, int64_t* str_begin, hipStream_t s)
 {
   if (G <= 0) return;
   int64_t blocks = (G + BLOCK - 1) / BLOCK;
-  hipLaunchKernelGGL(k_gather_keys, dim3(blocks), dim3(BLOCK), 0, s, dP, rep_rows, G, str_sizes);
+  hipLaunchKernelGGL(k_gather_keys, dim3(blocks), dim3(BLOCK), 0, s, dP, rep_rows, G, str_sizes, str_begin);
 }
 
 // Exclusive scan of int32 values (three phases: tile sums, scan of sums, apply).
@@ -1102,6 +1106,32 @@ __global__ void __launch_bounds__(BLOCK)
   int64_t b, e;
   str_span(src, int64_t(rep_rows[g]), b, e);
   for (int64_t i = lane; i < len; i += GATHER_LANES) gp(out_chars)[o + i] = gp(src.data)[b + i];
+}
+
+// The same copy from source offsets k_gather_keys recorded (sequential reads
+// instead of a second random read of the string offsets).
+__global__ void __launch_bounds__(BLOCK)
+  k_gather_chars_from(uint8_t const* __restrict__ src_chars, int64_t const* __restrict__ src_begin, int64_t G,
+                      int32_t const* __restrict__ out_offsets, uint8_t* out_chars)
+{
+  int64_t const t = int64_t(blockIdx.x) * BLOCK + threadIdx.x;
+  int64_t const g = t / GATHER_LANES;
+  int const lane  = int(t % GATHER_LANES);
+  if (g >= G) return;
+  int64_t const o   = out_offsets[g];
+  int64_t const len = int64_t(out_offsets[g + 1]) - o;
+  if (len == 0) return;
+  int64_t const b = src_begin[g];
+  for (int64_t i = lane; i < len; i += GATHER_LANES) gp(out_chars)[o + i] = gp(src_chars)[b + i];
+}
+
+void launch_gather_chars_from(uint8_t const* src_chars, int64_t const* src_begin, int64_t G,
+                              int32_t const* out_offsets, uint8_t* out_chars, hipStream_t s)
+{
+  if (G <= 0) return;
+  int64_t blocks = (G * GATHER_LANES + BLOCK - 1) / BLOCK;
+  hipLaunchKernelGGL(k_gather_chars_from, dim3(blocks), dim3(BLOCK), 0, s, src_chars, src_begin, G, out_offsets,
+                     out_chars);
 }
 
 void launch_gather_string_chars(dcol const& src, uint32_t const* rep_rows, int64_t G,

@@ -326,9 +326,13 @@ void launch_compact(gb_params const* dP, gb_params const& hP, uint32_t const* bl
 // aggregation results of the G dense groups.
 void launch_outputs(gb_params const* dP, gb_params const& hP, uint32_t const* map, int64_t G,
                     uint32_t* rep_rows, hipStream_t s);
-// Gathers fixed-width key columns / string lengths by representative row.
+// Gathers fixed-width key columns / string lengths by representative row;
+// for string key columns also the source offset of each group's string.
 void launch_gather_keys(gb_params const* dP, gb_params const& hP, uint32_t const* rep_rows,
-                        int64_t G, int32_t* str_sizes_scratch, hipStream_t s);
+                        int64_t G, int32_t* str_sizes_scratch, int64_t* str_begin_scratch, hipStream_t s);
+// String chars from the source offsets launch_gather_keys recorded.
+void launch_gather_chars_from(uint8_t const* src_chars, int64_t const* src_begin, int64_t G,
+                              int32_t const* out_offsets, uint8_t* out_chars, hipStream_t s);
 void launch_exclusive_scan_i32(int32_t* data, int64_t n, int32_t* total, hipStream_t s);
 void launch_gather_string_chars(dcol const& src, uint32_t const* rep_rows, int64_t G,
                                 int32_t const* out_offsets, uint8_t* out_chars, hipStream_t s);

@@ -22,15 +22,21 @@ namespace mig {
 
 // ---------------------------------------------------------------------------
 // Rows per final bin.  Each block counts its slice of the rows in LDS and
-// writes its own histogram row; k_rpa_hist_reduce sums the rows.
+// writes its own histogram row; k_rpa_hist_reduce sums the rows.  The LDS
+// holds a window of at most RPA_HIST_WINDOW bins: more bins take one launch
+// per window, each reading the keys again.
+constexpr int RPA_HIST_WINDOW = 1 << 15;
+
 __global__ void __launch_bounds__(RPA_HIST_BLOCK)
-  k_rpa_hist(gb_params const* __restrict__ Pp, rpa_params const* __restrict__ Rp, int64_t n)
+  k_rpa_hist(gb_params const* __restrict__ Pp, rpa_params const* __restrict__ Rp, int64_t n, int window)
 {
   extern __shared__ uint32_t hist[];
   gb_params const& P = *Pp;
   rpa_params const& R = *Rp;
-  int const B = 1 << R.log_bins;
-  for (int i = threadIdx.x; i < B; i += RPA_HIST_BLOCK) hist[i] = 0;
+  int const B    = 1 << R.log_bins;
+  int const W    = B < RPA_HIST_WINDOW ? B : RPA_HIST_WINDOW;
+  uint32_t const lo = uint32_t(window) * uint32_t(W);
+  for (int i = threadIdx.x; i < W; i += RPA_HIST_BLOCK) hist[i] = 0;
   __syncthreads();
   int64_t const per = (n + gridDim.x - 1) / gridDim.x;
   int64_t const r0  = int64_t(blockIdx.x) * per;
@@ -49,12 +55,12 @@ __global__ void __launch_bounds__(RPA_HIST_BLOCK)
     }
 #pragma unroll
     for (int u = 0; u < U; u++)
-      if (bin[u] != 0xffffffffu)
-        __hip_atomic_fetch_add(&hist[bin[u]], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      if (bin[u] != 0xffffffffu && bin[u] - lo < uint32_t(W))
+        __hip_atomic_fetch_add(&hist[bin[u] - lo], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
   }
   __syncthreads();
-  auto out = gp(R.hist_partial + int64_t(blockIdx.x) * B);
-  for (int i = threadIdx.x; i < B; i += RPA_HIST_BLOCK) out[i] = hist[i];
+  auto out = gp(R.hist_partial + int64_t(blockIdx.x) * B + lo);
+  for (int i = threadIdx.x; i < W; i += RPA_HIST_BLOCK) out[i] = hist[i];
 }
 
 // Column sums of the per-block histogram rows: blockIdx.y takes one slice of
@@ -194,10 +200,12 @@ void launch_rpa_layout(rpa_params const* dR, hipStream_t s)
 void launch_rpa_hist(gb_params const* dP, rpa_params const* dR, rpa_params const& hR, int64_t n,
                      int num_cus, hipStream_t s)
 {
-  size_t lds = sizeof(uint32_t) << hR.log_bins;
-  allow_dynamic_lds(reinterpret_cast<void const*>(&k_rpa_hist), int(lds));
-  hipLaunchKernelGGL(k_rpa_hist, dim3(hR.hist_blocks), dim3(RPA_HIST_BLOCK), lds, s, dP, dR, n);
   int64_t const B = int64_t(1) << hR.log_bins;
+  int64_t const W = std::min<int64_t>(B, RPA_HIST_WINDOW);
+  size_t const lds = sizeof(uint32_t) * size_t(W);
+  allow_dynamic_lds(reinterpret_cast<void const*>(&k_rpa_hist), int(lds));
+  for (int w = 0; w < int(B / W); w++)
+    hipLaunchKernelGGL(k_rpa_hist, dim3(hR.hist_blocks), dim3(RPA_HIST_BLOCK), lds, s, dP, dR, n, w);
   hipLaunchKernelGGL(k_rpa_hist_reduce, dim3((B + RPA_SMALL_BLOCK - 1) / RPA_SMALL_BLOCK, RPA_HIST_SLICES),
                      dim3(RPA_SMALL_BLOCK), 0, s, dR);
 }

@@ -288,3 +288,24 @@ def test_general_key_forms(form, policy):
         assert st["verified_pairs"] > 0
     else:
         assert st["verified_pairs"] == 0
+
+
+@pytest.mark.timeout(600)
+def test_largest_bin_count_exact_layout():
+    """2^16 bins: 14M rows of nearly unique int64 keys and three value
+    columns with wide accumulators (few LDS slots per bin), so the estimate
+    needs the largest bin count; a key on 0.1% of the rows rules out the
+    speculative layout, so the histogram is counted in two windows of 2^15
+    bins.  Every group against the oracle."""
+    rng = np.random.default_rng(65536)
+    n = 14_000_000
+    k = rng.integers(1, 1 << 40, n, dtype=np.int64)
+    k[rng.random(n) < 0.001] = 0
+    vals = [host_col(rng.integers(-10**6, 10**6, n).astype(np.int64), rng.random(n) > 0.05),
+            host_col(rng.standard_normal(n), rng.random(n) > 0.05),
+            host_col(rng.integers(-10**12, 10**12, n).astype(np.int64))]
+    aggs = [(AGG["SUM"], 1), (AGG["MIN"], 1), (AGG["MAX"], 1)]
+    ng = assert_parity([host_col(k)], [(vals[0], aggs + [(AGG["COUNT_VALID"], 1)]), (vals[1], aggs), (vals[2], aggs)])
+    st = _stats()
+    assert st["strategy"] == 2 and st["capacity"] == 65536 and st["rpa_speculative"] == 0, st
+    assert ng == len(np.unique(k))
