@@ -516,7 +516,7 @@ def result_digest(query, out):
     return (n, tuple(digests))
 
 
-def bench_served(n_req, loads=(0.5, 1.0, 2.0, 4.0), workers=4):
+def bench_served(n_req, loads=(0.5, 1.0, 2.0, 4.0), workers=8):
     """Plain kernel run serially over the stream, then the continuous-batching
     server at Poisson arrival rates that are multiples of the serial rate and
     with every request arriving at once; p50/p99 latency and achieved rate."""
@@ -613,7 +613,7 @@ def main():
     ap.add_argument("--only-batched", action="store_true", help="run the batched workload alone")
     ap.add_argument("--served-requests", type=int, default=200,
                     help="requests of the served workload (0 skips it)")
-    ap.add_argument("--served-workers", type=int, default=4)
+    ap.add_argument("--served-workers", type=int, default=8)
     ap.add_argument("--only-served", action="store_true", help="run the served workload alone")
     ap.add_argument("--only-sorted", action="store_true",
                     help="run the pre-sorted-keys path against the hash path on sorted C2-shaped data")

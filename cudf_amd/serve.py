@@ -105,7 +105,7 @@ class Query:
 class GroupByServer:
     """Continuous-batching group-by server on one GPU (see the module doc)."""
 
-    def __init__(self, workers: int = 4, memory_budget: int = 200 * 10**9, max_batch: int = 1024,
+    def __init__(self, workers: int = 8, memory_budget: int = 200 * 10**9, max_batch: int = 1024,
                  batch_rows: int = 4_000_000, options: Optional[BatchOptions] = None,
                  device: Optional[int] = None):
         self.workers = workers
