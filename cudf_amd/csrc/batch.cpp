@@ -641,6 +641,7 @@ void run_solo(mig_batch_plan& p, hipStream_t stream)
 
 void batch_execute(mig_batch_plan* plan, hipStream_t stream)
 {
+  func_range range("groupby: batched plan");
   MIG_EXPECTS(plan != nullptr, "null batch plan");
   mig_batch_plan& p = *plan;
   launch_batched(p, stream);

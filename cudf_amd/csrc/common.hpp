@@ -2,6 +2,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <rocprofiler-sdk-roctx/roctx.h>
 
 #include <cstdint>
 #include <stdexcept>
@@ -10,6 +11,17 @@
 #include "../../include/mig_groupby.h"
 
 namespace mig {
+
+// A named range on the ROCm tools timeline (roctx) for the enclosing scope:
+// the reference's CUDF_FUNC_RANGE (NVTX, cpp/src/groupby/groupby.cu:209).
+// rocprofv3 --marker-trace records it; without a tool attached it costs a
+// call into an empty library function.
+struct func_range {
+  explicit func_range(char const* name) { roctxRangePushA(name); }
+  ~func_range() { roctxRangePop(); }
+  func_range(func_range const&)            = delete;
+  func_range& operator=(func_range const&) = delete;
+};
 
 // ---------------------------------------------------------------------------
 // Errors.  The C ABI converts these to mig_status codes; the classes follow

@@ -1762,13 +1762,26 @@ std::unique_ptr<mig_groupby_result> groupby_call::run()
   dP  = S->get<gb_params>(1);
   pin = mem_ ? host_staging() : static_cast<uint8_t*>(thread_pinned().get(HOST_STAGING_BYTES));
   T->mark(0);
-  estimate_groups();
+  {
+    func_range range("groupby: estimate groups");
+    estimate_groups();
+  }
   T->mark(1);
 
   int64_t G = -1;
-  if (keys_sorted_) G = aggregate_sorted_runs();
-  if (G < 0) G = aggregate_partitioned();
-  if (G < 0) G = aggregate_global();
+  if (keys_sorted_) {
+    func_range range("groupby: sorted runs");
+    G = aggregate_sorted_runs();
+  }
+  if (G < 0) {
+    func_range range("groupby: partitioned");
+    G = aggregate_partitioned();
+  }
+  if (G < 0) {
+    func_range range("groupby: device-wide table");
+    G = aggregate_global();
+  }
+  func_range range("groupby: finish");
   finish(G);
   return std::move(res_);
 }
@@ -1781,6 +1794,7 @@ std::unique_ptr<mig_groupby_result> groupby_aggregate(mig_column_view const* key
                                                       int32_t num_requests, hipStream_t stream,
                                                       mig_memory_resource const* mr, bool keys_sorted)
 {
+  func_range range("cudf::groupby::groupby::aggregate");
   groupby_call call(keys, num_keys, null_policy, requests, num_requests, stream, mr, keys_sorted, nullptr);
   return call.run();
 }
@@ -1789,6 +1803,7 @@ int32_t groupby_aggregate_into(mig_column_view const* keys, int32_t num_keys, in
                                mig_aggregation_request const* requests, int32_t num_requests, hipStream_t stream,
                                call_memory const& mem)
 {
+  func_range range("cudf::groupby::groupby::aggregate (allocation-free)");
   groupby_call call(keys, num_keys, null_policy, requests, num_requests, stream, nullptr, false, &mem);
   auto res = call.run();
   return res->num_groups;

@@ -316,6 +316,7 @@ std::unique_ptr<mig_table> hash_partition(mig_column_view const* input, int32_t 
                                           hipStream_t stream, mig_memory_resource const* mr,
                                           std::vector<int32_t>& offsets)
 {
+  func_range range("cudf::hash_partition");
   check_table(input, num_columns);
   MIG_EXPECTS(num_columns_to_hash >= 0 && (num_columns_to_hash == 0 || columns_to_hash != nullptr),
               "invalid columns_to_hash");
@@ -359,6 +360,7 @@ std::unique_ptr<mig_table> partition_by_map(mig_column_view const* input, int32_
                                             hipStream_t stream, mig_memory_resource const* mr,
                                             std::vector<int32_t>& offsets)
 {
+  func_range range("cudf::partition");
   check_table(input, num_columns);
   int64_t const n = num_columns > 0 ? input[0].size : 0;
   MIG_EXPECTS(partition_map.size == n, "Size mismatch between table and partition map.");
@@ -395,6 +397,7 @@ std::unique_ptr<mig_table> partition_by_map(mig_column_view const* input, int32_
 std::unique_ptr<mig_table> concatenate(mig_column_view const* columns, int32_t num_tables, int32_t num_columns,
                                        hipStream_t stream, mig_memory_resource const* mr)
 {
+  func_range range("cudf::concatenate");
   MIG_EXPECTS(num_tables >= 0 && num_columns >= 0, "negative table or column count");
   auto t = std::make_unique<mig_table>();
   init_table(*t, stream, mr);

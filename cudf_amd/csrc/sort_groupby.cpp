@@ -69,6 +69,7 @@ std::unique_ptr<mig_groupby_result> sort_groupby(mig_column_view const* keys, in
                                                  mig_aggregation_request const* requests, int32_t num_requests,
                                                  hipStream_t stream, mig_memory_resource const* mr, bool keys_sorted)
 {
+  func_range range("groupby: sort-based aggregations");
   int64_t const N = keys[0].size;
   MIG_EXPECTS(N > 0 && N < (int64_t(1) << 31), "sort-based group-by: rows out of range");
   for (int r = 0; r < num_requests; r++)
