@@ -408,11 +408,7 @@ void check_supported(int32_t src, int32_t kind)
     case MIG_AGG_VARIANCE:
     case MIG_AGG_STD: ok = is_numeric_type(src); break;
     case MIG_AGG_NUNIQUE:
-    case MIG_AGG_NTH_ELEMENT:
-      if (src == MIG_TYPE_STRING)
-        fail(MIG_ERROR_LOGIC, std::string(kind_name(kind)) + " of string values is not supported");
-      ok = is_numeric_type(src) || is_chrono_type(src);
-      break;
+    case MIG_AGG_NTH_ELEMENT: ok = is_numeric_type(src) || is_chrono_type(src) || src == MIG_TYPE_STRING; break;
     // arithmetic values only (group_quantiles.cu quantiles_functor)
     case MIG_AGG_MEDIAN:
     case MIG_AGG_QUANTILE: ok = is_numeric_type(src); break;
@@ -1071,6 +1067,24 @@ void groupby_call::alloc_outputs(int64_t rows_cap, int64_t G, bool zero_masks)
           O.field   = vp.nth_field(requests_[r].aggregations[a].n,
                                    requests_[r].aggregations[a].null_handling == MIG_NULL_INCLUDE);
           with_mask = true;
+          if (vp.storage == ST_STRING) {
+            // the group's n-th row into scratch; the strings are gathered
+            // from it once the rows are known (null: no such row, or a null)
+            O.kind = OK_ARG;
+            O.data = S->get<int32_t>(std::max<int64_t>(rows_cap, 1));
+            O.mask = nullptr;
+            owned_column oc;
+            oc.col.type_id         = MIG_TYPE_STRING;
+            oc.col.size            = int32_t(G);
+            oc.col.offsets_type_id = MIG_TYPE_INT32;
+            oc.offsets_bytes       = sizeof(int32_t) * size_t(G + 1);
+            oc.col.offsets         = A().allocate(oc.offsets_bytes);
+            oc.mask_bytes          = std::max<size_t>(8, size_t((G + 63) / 64) * 8);
+            oc.col.null_mask       = static_cast<uint32_t*>(A().allocate(oc.mask_bytes));
+            string_outs_.push_back({r, int(cols.size()), vcol_of_request_[r], static_cast<int32_t*>(O.data)});
+            cols.push_back(oc);
+            continue;
+          }
           break;
         case MIG_AGG_NUNIQUE:
           O.kind  = OK_COUNT;
@@ -1153,9 +1167,6 @@ void groupby_call::gather_general_keys(uint32_t const* rep_rows, int64_t G)
 int64_t groupby_call::aggregate_sorted_runs()
 {
   gb_params& p = P();
-  for (size_t j = 0; j < vals_.size(); j++)
-    MIG_EXPECTS(!(vals_[j].storage == ST_STRING && (vals_[j].want_argmin || vals_[j].want_argmax)),
-                "string MIN / MAX / ARGMIN / ARGMAX are not supported with pre-sorted keys");
   int const num_cus     = props().num_cus;
   p.key_mode            = KM_GENERAL;  // keys are gathered from each group's first row
   int64_t const ntiles  = sorted_tiles(N);
@@ -1190,6 +1201,7 @@ int64_t groupby_call::aggregate_sorted_runs()
   launch_sorted_assign(dP, N, tile_counts, gid, group_last, stream_);
   T->mark(2);
   launch_sorted_aggregate(dP, p, N, gid, num_cus, stream_);
+  launch_sorted_string_arg(dP, p, N, gid, stream_);
   T->mark(3);
   T->mark(4);
   if (p.need_second_pass) launch_sorted_second_pass(dP, p, N, gid, num_cus, stream_);
@@ -1208,6 +1220,7 @@ int64_t groupby_call::aggregate_sorted_runs()
   upload(1);
   launch_outputs(dP, p, map, G, rep_rows, stream_);
   gather_general_keys(rep_rows, G);
+  string_results(G);
   MIG_HIP_TRY(hipGetLastError());
   stats_[STAT_STRATEGY] = 3;  // runs of pre-sorted keys
   stats_[STAT_CAPACITY] = double(G);

@@ -1897,6 +1897,8 @@ static void launch_sorted_passes(gb_params const* dP, gb_params const& hP, int64
   for (int f = 0; f < hP.num_fields; f++) {
     int32_t const k = hP.fields[f].kind;
     if (k == FK_NUNIQUE || k == FK_NUNIQUE_NULLS || k == FK_NTH) continue;  // kernels of their own
+    if ((k == FK_ARGMIN || k == FK_ARGMAX) && hP.vals[hP.fields[f].vcol].storage == ST_STRING)
+      continue;  // launch_sorted_string_arg
     if ((hP.fields[f].kind == FK_M2) == m2) items.push_back(f);
   }
   int64_t per, W;
@@ -1932,6 +1934,66 @@ static void launch_sorted_passes(gb_params const* dP, gb_params const& hP, int64
     if (has_arg) hipLaunchKernelGGL(k_sorted_reduce<true>, dim3(blocks), dim3(BLOCK), 0, s, dP, gid, per, W, pass);
     else hipLaunchKernelGGL(k_sorted_reduce<false>, dim3(blocks), dim3(BLOCK), 0, s, dP, gid, per, W, pass);
   }
+}
+
+// ARGMIN / ARGMAX of a string column on runs (and through them string MIN /
+// MAX): each wave reduces its rows of every run to the run's best row with
+// string comparisons (segmented over lanes of equal group numbers, which are
+// contiguous), and the run's first lane folds that row into the group's
+// accumulator with the compare-and-swap of the device-wide path.  Equal
+// strings keep the smaller row, so the result does not depend on the order
+// of the updates.
+__global__ void __launch_bounds__(BLOCK)
+  k_sorted_string_arg(gb_params const* __restrict__ Pp, int32_t f, int64_t n, uint32_t const* __restrict__ gid)
+{
+  constexpr uint32_t NONE = 0xFFFFFFFFu;
+  gb_params const& P  = *Pp;
+  field_desc const& F = P.fields[f];
+  dcol const& c       = P.vals[F.vcol];
+  int const lane      = threadIdx.x & 63;
+  for (int64_t base = int64_t(blockIdx.x) * BLOCK + (threadIdx.x & ~63); base < n;
+       base += int64_t(gridDim.x) * BLOCK) {
+    int64_t const row = base + lane;
+    uint32_t const g  = row < n ? gid[row] : NO_GROUP;
+    uint32_t best     = (g != NO_GROUP && dcol_valid(c, row)) ? uint32_t(row) : NONE;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      uint32_t const other = uint32_t(__shfl_down(int(best), d));
+      uint32_t const og    = uint32_t(__shfl_down(int(g), d));
+      if (lane + d < 64 && og == g && other != NONE &&
+          (best == NONE || arg_better(c, F.kind, int64_t(other), int64_t(best))))
+        best = other;
+    }
+    uint32_t const gprev = uint32_t(__shfl_up(int(g), 1));
+    bool const head      = lane == 0 || gprev != g;
+    if (head && g != NO_GROUP && best != NONE) arg_update(F, int64_t(g), c, int64_t(best));
+  }
+}
+
+void launch_sorted_string_arg(gb_params const* dP, gb_params const& hP, int64_t n, uint32_t const* gid,
+                              hipStream_t s)
+{
+  if (n <= 0) return;
+  unsigned const blocks = unsigned(std::min<int64_t>((n + BLOCK - 1) / BLOCK, int64_t(1) << 20));
+  for (int f = 0; f < hP.num_fields; f++) {
+    int32_t const k = hP.fields[f].kind;
+    if ((k == FK_ARGMIN || k == FK_ARGMAX) && hP.vals[hP.fields[f].vcol].storage == ST_STRING)
+      hipLaunchKernelGGL(k_sorted_string_arg, dim3(blocks), dim3(BLOCK), 0, s, dP, f, n, gid);
+  }
+}
+
+// Group rows of the gathered (group-ordered) columns back to input rows:
+// rows[g] = perm[rows[g]] where the row is set (-1 stays).
+__global__ void __launch_bounds__(BLOCK)
+  k_map_rows(int32_t* __restrict__ rows, int64_t G, uint32_t const* __restrict__ perm)
+{
+  int64_t const g = int64_t(blockIdx.x) * BLOCK + threadIdx.x;
+  if (g < G && rows[g] >= 0) rows[g] = int32_t(perm[rows[g]]);
+}
+
+void launch_map_rows(int32_t* rows, int64_t G, uint32_t const* perm, hipStream_t s)
+{
+  if (G > 0) hipLaunchKernelGGL(k_map_rows, dim3(unsigned((G + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0, s, rows, G, perm);
 }
 
 void launch_sorted_aggregate(gb_params const* dP, gb_params const& hP, int64_t n, uint32_t const* gid, int num_cus,
@@ -1997,8 +2059,9 @@ __global__ void __launch_bounds__(BLOCK)
     uint32_t const g  = row < n ? gid[row] : NO_GROUP;
     bool const valid  = g != NO_GROUP && dcol_valid(c, row);
     bool inserted     = false;
+    bool const str = c.storage == ST_STRING;  // strings: hashed, then compared byte-wise
     if (g != NO_GROUP && (valid || with_null)) {
-      uint64_t const b     = valid ? nunique_bits(c, row) : 0ull;
+      uint64_t const b     = !valid ? 0ull : (str ? uint64_t(elem_hash(c, row, 0u)) : nunique_bits(c, row));
       uint32_t const h     = mm_fmix32(uint32_t(b) ^ mm_fmix32(uint32_t(b >> 32) ^ uint32_t(valid)));
       uint64_t const first = P.slot_rows[g];
       uint64_t const end   = int64_t(g) + 1 < G ? P.slot_rows[g + 1] : uint64_t(n);
@@ -2017,7 +2080,8 @@ __global__ void __launch_bounds__(BLOCK)
         }
         // the region holds rows of group g only
         bool const cv = dcol_valid(c, cur);
-        if (cv == valid && (!valid || nunique_bits(c, cur) == b)) break;  // already counted
+        if (cv == valid && (!valid || (str ? elem_equal(c, cur, row) : nunique_bits(c, cur) == b)))
+          break;  // already counted
         off = off + 1 == len ? 0 : off + 1;
       }
     }
@@ -2036,16 +2100,10 @@ __global__ void __launch_bounds__(BLOCK)
   }
 }
 
-// NTH_ELEMENT on runs (sort/group_nth_element.cu:50-140): one thread per
-// group walks the group's rows in row order (from its first row forward for
-// n >= 0, from the next group's first row backward for n < 0), counting every
-// row under INCLUDE and rows with a valid value under EXCLUDE, and keeps the
-// row where the count reaches the n-th.  A group's rows are contiguous (a
-// dropped null-key row ends a run), so the forward walk ends at the first row
-// of another group or a dropped row, and the backward walk skips the dropped
-// rows between the group's last row and the next group.  Cost per group:
-// O(|n|) rows plus rows without a counted value.
-// NTH_ELEMENT on runs.  Group g holds rows first..last (slot_rows[g],
+// NTH_ELEMENT on runs (sort/group_nth_element.cu:50-140): the group's n-th
+// row in row order (from the end for n < 0), counting every row under
+// INCLUDE and rows with a valid value under EXCLUDE; none when the group has
+// too few.  Group g holds rows first..last (slot_rows[g],
 // group_last[g]) with no dropped row among them, so when every row counts
 // (nulls included, or a value column without nulls) the n-th row is found by
 // index arithmetic, one thread per group.
@@ -2163,7 +2221,7 @@ __global__ void __launch_bounds__(BLOCK)
   bool valid      = false;
   if (g < G) {
     int32_t const r = rows[g];
-    valid           = r != -1;
+    valid           = r != -1 && dcol_valid(src, r);  // NTH_ELEMENT's row may hold a null
     int64_t b = 0, e = 0;
     if (valid) str_span(src, r, b, e);
     sizes[g] = int32_t(e - b);

@@ -364,6 +364,11 @@ void launch_sorted_nunique(gb_params const* dP, int32_t f, int64_t n, int64_t G,
 void launch_sorted_nth(gb_params const* dP, gb_params const& hP, int32_t f, int64_t G, uint32_t const* group_last,
                        hipStream_t s);
 void launch_iota_u32(uint32_t* out, int64_t n, hipStream_t s);
+// ARGMIN / ARGMAX fields of string columns on runs (the reduce passes skip them).
+void launch_sorted_string_arg(gb_params const* dP, gb_params const& hP, int64_t n, uint32_t const* gid,
+                              hipStream_t s);
+// rows[g] = perm[rows[g]] where rows[g] >= 0.
+void launch_map_rows(int32_t* rows, int64_t G, uint32_t const* perm, hipStream_t s);
 
 // ---- sort-based aggregations (sort_kernels.hip, label kernels in
 // groupby_kernels.hip) ---------------------------------------------------------

@@ -72,9 +72,6 @@ std::unique_ptr<mig_groupby_result> sort_groupby(mig_column_view const* keys, in
   func_range range("groupby: sort-based aggregations");
   int64_t const N = keys[0].size;
   MIG_EXPECTS(N > 0 && N < (int64_t(1) << 31), "sort-based group-by: rows out of range");
-  for (int r = 0; r < num_requests; r++)
-    if (storage_of(requests[r].values.type_id) == ST_STRING)
-      fail(MIG_ERROR_DATA_TYPE, "string values are not supported by the sort-based group-by");
   auto res = std::make_unique<mig_groupby_result>();
   res->alloc.stream = stream;
   if (mr) {
@@ -180,13 +177,30 @@ std::unique_ptr<mig_groupby_result> sort_groupby(mig_column_view const* keys, in
   auto gathered_view = [&](mig_column_view const& v) -> mig_column_view {
     for (auto const& g : gathered)
       if (same_view(g.src, v)) return g.view;
-    int32_t const w = type_width(v.type_id);
     mig_column_view out{};
     out.type_id = v.type_id;
     out.size    = kept;
-    auto* data  = S.get<uint8_t>(size_t(std::max(kept, 1)) * w);
-    launch_gather_fixed(static_cast<uint8_t const*>(v.data) + int64_t(w) * v.offset, w, perm, kept, data, stream);
-    out.data       = data;
+    if (storage_of(v.type_id) == ST_STRING) {
+      // sizes in group order, their offsets, then the characters
+      dcol const dv = make_dcol(v, false);
+      auto* offs    = S.get<int32_t>(size_t(kept) + 1);
+      launch_gather_str_sizes(dv, perm, kept, offs, stream);
+      launch_exclusive_scan_i32(offs, kept, offs + kept, stream);
+      int32_t total = 0;
+      MIG_HIP_TRY(hipMemcpyAsync(&total, offs + kept, 4, hipMemcpyDeviceToHost, stream));
+      MIG_HIP_TRY(hipStreamSynchronize(stream));
+      MIG_EXPECTS(total >= 0, "string values exceed 2^31 characters");
+      auto* chars = S.get<uint8_t>(std::max<size_t>(1, size_t(total)));
+      launch_gather_string_chars(dv, perm, kept, offs, chars, stream);
+      out.data            = chars;
+      out.offsets         = offs;
+      out.offsets_type_id = MIG_TYPE_INT32;
+    } else {
+      int32_t const w = type_width(v.type_id);
+      auto* data      = S.get<uint8_t>(size_t(std::max(kept, 1)) * w);
+      launch_gather_fixed(static_cast<uint8_t const*>(v.data) + int64_t(w) * v.offset, w, perm, kept, data, stream);
+      out.data = data;
+    }
     out.null_count = 0;
     if (v.null_mask && null_count_of(v, stream) > 0) {
       auto* mask  = S.get<uint32_t>(mask_words_bytes(kept) / 4);
@@ -233,8 +247,12 @@ std::unique_ptr<mig_groupby_result> sort_groupby(mig_column_view const* keys, in
     for (size_t i = 0; i < sub->results.size(); i++)
       for (auto& c : sub->results[i]) {
         auto const [r, a] = origin[k++];
-        results[r][a]     = c;
-        c                 = owned_column{};  // now owned by the result
+        int32_t const kind = requests[r].aggregations[a].kind;
+        // ARGMIN / ARGMAX name rows of the group-ordered columns: back to input rows
+        if ((kind == MIG_AGG_ARGMIN || kind == MIG_AGG_ARGMAX) && c.col.data != nullptr)
+          launch_map_rows(static_cast<int32_t*>(c.col.data), G, perm, stream);
+        results[r][a] = c;
+        c             = owned_column{};  // now owned by the result
       }
   }
 

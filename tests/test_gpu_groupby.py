@@ -392,8 +392,10 @@ def test_argmin_argmax_all_nan_groups_and_errors():
     s = orc.HostColumn.from_strings([b"x%d" % i for i in range(len(k))])
     with pytest.raises(_lib.DataTypeError):
         gpu_groupby([host_col(k)], [(s, [(AGG["SUM"], 1)])])
-    with pytest.raises(_lib.LogicError):  # string values on the pre-sorted path
-        gpu_groupby([host_col(np.sort(k))], [(s, [(AGG["ARGMIN"], 1)])], keys_sorted=True)
+    # string ARGMIN / ARGMAX on the pre-sorted path: the oracle's rows
+    assert_parity([host_col(np.sort(k))], [(s, [(AGG["ARGMIN"], 1), (AGG["ARGMAX"], 1)])],
+                  gpu=gpu_groupby([host_col(np.sort(k))], [(s, [(AGG["ARGMIN"], 1), (AGG["ARGMAX"], 1)])],
+                                  keys_sorted=True))
 
 
 # ------------------------------------------------------------------ PRODUCT --

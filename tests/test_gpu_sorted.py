@@ -222,7 +222,7 @@ def test_sorted_nunique_parity(vdt):
 def test_sorted_nunique_large_and_errors():
     """3M rows over 100k sorted keys against the oracle; NUNIQUE without
     keys_are_sorted takes the sort-based group-by (same answer), NUNIQUE of
-    strings raises a logic error."""
+    strings counts distinct strings per run."""
     from cudf_amd import _lib
     rng = np.random.default_rng(120)
     n = 3_000_000
@@ -230,9 +230,11 @@ def test_sorted_nunique_large_and_errors():
     v = rng.integers(0, 40, n).astype(np.int32)
     _check_sorted([host_col(k)], [(host_col(v, rng.random(n) > 0.2), [(AGG["NUNIQUE"], 0), (AGG["NUNIQUE"], 1)])], 0)
     assert_parity([host_col(k[:1000])], [(host_col(v[:1000]), [(AGG["NUNIQUE"], 0)])])
-    s = orc.HostColumn.from_strings([b"x%d" % (i % 3) for i in range(100)])
-    with pytest.raises(_lib.LogicError):
-        gpu_groupby([host_col(k[:100])], [(s, [(AGG["NUNIQUE"], 0)])], keys_sorted=True)
+    strs = [b"x%d" % (i % 3) for i in range(100)]
+    s = orc.HostColumn.from_strings(strs)
+    g_keys, g_res = gpu_groupby([host_col(k[:100])], [(s, [(AGG["NUNIQUE"], 0)])], keys_sorted=True)
+    runs = np.split(np.arange(100), np.flatnonzero(np.diff(k[:100])) + 1)
+    assert list(g_res[0][0][0]) == [len({strs[r] for r in run}) for run in runs]
 
 
 # ------------------------------------------------------------ NTH_ELEMENT --
@@ -377,9 +379,10 @@ def test_nth_element_errors():
     v = np.arange(100, dtype=np.int32)
     # unsorted keys: the sort-based group-by
     assert_parity([host_col(k[::-1].copy())], [(host_col(v), [(AGG["NTH_ELEMENT"], 1)])])
-    s = orc.HostColumn.from_strings([b"x%d" % i for i in range(100)])
-    with pytest.raises(_lib.LogicError):
-        gpu_groupby([host_col(k)], [(s, [(AGG["NTH_ELEMENT"], 1)])], keys_sorted=True)
+    strs = [b"x%d" % i for i in range(100)]
+    s = orc.HostColumn.from_strings(strs)
+    g_keys, g_res = gpu_groupby([host_col(k)], [(s, [(AGG["NTH_ELEMENT"], 1)])], keys_sorted=True)
+    assert list(g_res[0][0][0]) == strs  # one row per group: its own string
 
 
 @pytest.mark.parametrize("policy", [0, 1])
