@@ -90,6 +90,7 @@ class OutColumn(ctypes.Structure):
         ("data_bytes", ctypes.c_int64),
         ("mask_bytes", ctypes.c_int64),
         ("offsets_bytes", ctypes.c_int64),
+        ("scale", ctypes.c_int32),
     ]
 
 

@@ -45,7 +45,7 @@ class Column:
 
     def __init__(self, type_id: int, size: int, data: Optional[torch.Tensor],
                  mask: Optional[torch.Tensor] = None, null_count: int = 0, offset: int = 0,
-                 offsets: Optional[torch.Tensor] = None):
+                 offsets: Optional[torch.Tensor] = None, scale: int = 0):
         self.type_id = TypeId(type_id)
         self.size = int(size)
         self.data = data
@@ -53,11 +53,14 @@ class Column:
         self.null_count = int(null_count) if mask is not None else 0
         self.offset = int(offset)
         self.offsets = offsets
+        self.scale = int(scale)  # DECIMAL32 / DECIMAL64: value = data * 10**scale
 
     # -- construction -------------------------------------------------------
     @staticmethod
     def from_numpy(arr: np.ndarray, valid: Optional[np.ndarray] = None,
-                   type_id: Optional[int] = None) -> "Column":
+                   type_id: Optional[int] = None, scale: int = 0) -> "Column":
+        """A device column from host values; DECIMAL32 / DECIMAL64 take the
+        integer representation and the scale (value = rep * 10**scale)."""
         arr = np.ascontiguousarray(arr)
         tid = TypeId(type_id) if type_id is not None else type_id_of(arr.dtype)
         host = arr.astype(numpy_dtype(tid), copy=False)
@@ -68,7 +71,7 @@ class Column:
             valid = np.asarray(valid, dtype=bool)
             mask = torch.from_numpy(pack_validity(valid).view(np.int32).copy()).to(dev)
             nulls = int(len(valid) - valid.sum())
-        return Column(tid, len(arr), data, mask, nulls)
+        return Column(tid, len(arr), data, mask, nulls, scale=scale)
 
     @staticmethod
     def from_strings(strs: Sequence, valid: Optional[np.ndarray] = None,
@@ -90,7 +93,7 @@ class Column:
 
     def slice(self, offset: int, size: int, null_count: int = -1) -> "Column":
         """A view of rows [offset, offset+size) sharing the buffers."""
-        c = Column(self.type_id, size, self.data, self.mask, 0, self.offset + offset, self.offsets)
+        c = Column(self.type_id, size, self.data, self.mask, 0, self.offset + offset, self.offsets, self.scale)
         c.null_count = null_count if self.mask is not None else 0
         return c
 
@@ -103,7 +106,7 @@ class Column:
                 raise OverflowError(f"column {name} {x} is outside [0, 2^31 - 1] (cudf size_type)")
         v = _lib.ColumnView()
         v.type_id = int(self.type_id)
-        v.scale = 0
+        v.scale = self.scale
         v.size = self.size
         v.offset = self.offset
         v.data = self.data.data_ptr() if self.data is not None else None

@@ -323,8 +323,8 @@ mig_batch_plan* batch_plan_create(mig_batch_instance const* instances, int32_t n
   for (int32_t i = 0; i < num_instances; i++) {
     MIG_EXPECTS(instances[i].keys != nullptr, "null key array of an instance");
     mig_column_view const& k = instances[i].keys[0];
-    if (!fixed_width(k.type_id))
-      fail(MIG_ERROR_DATA_TYPE, "the batched group-by takes one fixed-width key column per instance");
+    if (!fixed_width(k.type_id) || is_decimal_type(k.type_id))
+      fail(MIG_ERROR_DATA_TYPE, "the batched group-by takes one fixed-width, non-fixed-point key column per instance");
     MIG_EXPECTS(k.size >= 0 && k.offset >= 0, "negative column size or offset");
     p.key_views[i]  = k;
     p.hinst[i].rows = k.size;
@@ -361,8 +361,8 @@ mig_batch_plan* batch_plan_create(mig_batch_instance const* instances, int32_t n
   for (int32_t v = 0; v < p.num_vals; v++)
     for (int32_t i = 0; i < num_instances; i++) {
       mig_column_view const& c = instances[i].values[req_of_vcol[v]];
-      if (!fixed_width(c.type_id))
-        fail(MIG_ERROR_DATA_TYPE, "the batched group-by takes fixed-width value columns");
+      if (!fixed_width(c.type_id) || is_decimal_type(c.type_id))
+        fail(MIG_ERROR_DATA_TYPE, "the batched group-by takes fixed-width, non-fixed-point value columns");
       MIG_EXPECTS(c.size == p.key_views[i].size, "Size mismatch between request values and groupby keys.");
       MIG_EXPECTS(c.offset >= 0, "negative column offset");
       p.val_views[size_t(i) * p.num_vals + v] = c;

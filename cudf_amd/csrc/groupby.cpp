@@ -339,6 +339,8 @@ bool is_numeric_type(int32_t t)
   return (t >= MIG_TYPE_INT8 && t <= MIG_TYPE_FLOAT64) || t == MIG_TYPE_BOOL8;
 }
 bool is_chrono_type(int32_t t) { return t >= MIG_TYPE_TIMESTAMP_DAYS && t <= MIG_TYPE_DURATION_NANOSECONDS; }
+// fixed-point types with a 32- or 64-bit representation (DECIMAL128 is not supported)
+bool is_decimal_type(int32_t t) { return t == MIG_TYPE_DECIMAL32 || t == MIG_TYPE_DECIMAL64; }
 
 char const* kind_name(int32_t k)
 {
@@ -376,6 +378,8 @@ int32_t target_type_of(int32_t src, int32_t kind)
     case MIG_AGG_ARGMAX:
     case MIG_AGG_NUNIQUE: return MIG_TYPE_INT32;  // size_type count (group_nunique.cu:90-91)
     case MIG_AGG_MEAN:
+      if (is_decimal_type(src)) return src;  // fixed point keeps its type (aggregation.hpp:1265-1272)
+      return MIG_TYPE_FLOAT64;
     case MIG_AGG_VARIANCE:
     case MIG_AGG_STD:
     case MIG_AGG_MEDIAN:    // always double (aggregation.hpp:1334-1344)
@@ -384,6 +388,7 @@ int32_t target_type_of(int32_t src, int32_t kind)
     case MIG_AGG_SUM_OF_SQUARES:
     case MIG_AGG_PRODUCT:  // aggregation.hpp:1276-1306
       if (src == MIG_TYPE_FLOAT32 || src == MIG_TYPE_FLOAT64) return src;
+      if (kind == MIG_AGG_SUM && is_decimal_type(src)) return src;  // aggregation.hpp:1286-1293
       return MIG_TYPE_INT64;
     default: return MIG_TYPE_EMPTY;
   }
@@ -393,6 +398,10 @@ int32_t target_type_of(int32_t src, int32_t kind)
 // type rules of cudf's is_valid_aggregation for the kinds implemented here.
 void check_supported(int32_t src, int32_t kind)
 {
+  if (src == MIG_TYPE_DECIMAL128)
+    fail(MIG_ERROR_DATA_TYPE, std::string("DECIMAL128 values are not supported (aggregation ") + kind_name(kind) + ")");
+  if (kind == MIG_AGG_PRODUCT && is_decimal_type(src))  // sum_tests.cpp:199-202: a logic_error
+    fail(MIG_ERROR_LOGIC, "PRODUCT of fixed-point values is not supported");
   bool ok = false;
   switch (kind) {
     case MIG_AGG_COUNT_VALID:
@@ -400,15 +409,19 @@ void check_supported(int32_t src, int32_t kind)
     case MIG_AGG_MIN:
     case MIG_AGG_MAX:
     case MIG_AGG_ARGMIN:
-    case MIG_AGG_ARGMAX: ok = is_numeric_type(src) || is_chrono_type(src) || src == MIG_TYPE_STRING; break;
+    case MIG_AGG_ARGMAX:
+      ok = is_numeric_type(src) || is_chrono_type(src) || is_decimal_type(src) || src == MIG_TYPE_STRING;
+      break;
     case MIG_AGG_SUM:
+    case MIG_AGG_MEAN: ok = is_numeric_type(src) || is_decimal_type(src); break;
     case MIG_AGG_SUM_OF_SQUARES:
     case MIG_AGG_PRODUCT:
-    case MIG_AGG_MEAN:
     case MIG_AGG_VARIANCE:
     case MIG_AGG_STD: ok = is_numeric_type(src); break;
     case MIG_AGG_NUNIQUE:
-    case MIG_AGG_NTH_ELEMENT: ok = is_numeric_type(src) || is_chrono_type(src) || src == MIG_TYPE_STRING; break;
+    case MIG_AGG_NTH_ELEMENT:
+      ok = is_numeric_type(src) || is_chrono_type(src) || is_decimal_type(src) || src == MIG_TYPE_STRING;
+      break;
     // arithmetic values only (group_quantiles.cu quantiles_functor)
     case MIG_AGG_MEDIAN:
     case MIG_AGG_QUANTILE: ok = is_numeric_type(src); break;
@@ -991,6 +1004,7 @@ void groupby_call::alloc_outputs(int64_t rows_cap, int64_t G, bool zero_masks)
       n_str_keys_++;
     } else {
       res_->keys.push_back(fixed_output(keys_[i].type_id, rows_cap, G, with_mask, zero_masks));
+      if (is_decimal_type(keys_[i].type_id)) res_->keys.back().col.scale = keys_[i].scale;
     }
     p.out_keys[i]     = storage_of(keys_[i].type_id) == ST_STRING ? nullptr : res_->keys.back().col.data;
     p.out_key_mask[i] = res_->keys.back().col.null_mask;
@@ -1016,6 +1030,7 @@ void groupby_call::alloc_outputs(int64_t rows_cap, int64_t G, bool zero_masks)
         case MIG_AGG_PRODUCT:
           O.field = kind == MIG_AGG_SUM ? vp.f_sum : (kind == MIG_AGG_PRODUCT ? vp.f_prod : vp.f_sumsq);
           O.kind  = tt == MIG_TYPE_FLOAT32 ? OK_COPY_F32 : (tt == MIG_TYPE_FLOAT64 ? OK_COPY_F64 : OK_COPY_I64);
+          if (tt == MIG_TYPE_DECIMAL32) O.kind = OK_SUM_I32;  // the representation's sum, as int32
           if (vp.has_nulls) { O.count_field = vp.f_count; with_mask = true; }
           break;
         case MIG_AGG_MIN:
@@ -1051,7 +1066,7 @@ void groupby_call::alloc_outputs(int64_t rows_cap, int64_t G, bool zero_masks)
           break;
         case MIG_AGG_COUNT_ALL: O.kind = OK_COUNT_ALL; break;
         case MIG_AGG_MEAN:
-          O.kind        = OK_MEAN;
+          O.kind        = is_decimal_type(tt) ? OK_MEAN_INT : OK_MEAN;
           O.field       = vp.f_sum;
           O.count_field = vp.f_count;
           with_mask     = vp.has_nulls;
@@ -1100,6 +1115,7 @@ void groupby_call::alloc_outputs(int64_t rows_cap, int64_t G, bool zero_masks)
         default: fail(MIG_ERROR_LOGIC, "unreachable aggregation kind");
       }
       owned_column oc = fixed_output(tt, rows_cap, G, with_mask, zero_masks);
+      if (is_decimal_type(tt)) oc.col.scale = vp.view.scale;
       O.data          = oc.col.data;
       O.mask          = oc.col.null_mask;
       cols.push_back(oc);

@@ -926,6 +926,15 @@ __global__ void __launch_bounds__(BLOCK)
           reinterpret_cast<double*>(O.data)[g] = valid ? sum / double(cnt) : 0.0;
           break;
         }
+        case OK_SUM_I32:
+          reinterpret_cast<int32_t*>(O.data)[g] = int32_t(uint32_t(field_bits(P.fields[O.field], slot)));
+          break;
+        case OK_MEAN_INT: {  // fixed-point mean: the representation's sum / count, truncated
+          int64_t sum = int64_t(field_bits(P.fields[O.field], slot));
+          if (O.out_width == 4) sum = int64_t(int32_t(uint32_t(sum)));
+          store_width(O.data, g, O.out_width, uint64_t(valid ? sum / int64_t(cnt) : 0));
+          break;
+        }
         case OK_ARG: {
           uint32_t const r = *reinterpret_cast<uint32_t const*>(field_at(P.fields[O.field], slot));
           valid            = r != 0xFFFFFFFFu;

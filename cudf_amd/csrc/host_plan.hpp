@@ -18,6 +18,7 @@ bool same_view(mig_column_view const& a, mig_column_view const& b);
 int32_t null_count_of(mig_column_view const& v, hipStream_t s);
 dcol make_dcol(mig_column_view const& v, bool use_mask);
 bool is_numeric_type(int32_t t);
+bool is_decimal_type(int32_t t);  // DECIMAL32 / DECIMAL64
 bool is_chrono_type(int32_t t);
 char const* kind_name(int32_t k);
 // Result type of (source type, aggregation): aggregation.hpp:1211-1331.

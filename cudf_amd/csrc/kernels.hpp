@@ -119,7 +119,9 @@ enum out_kind : int32_t {
   OK_VAR,        // m2 / (count - ddof)
   OK_STD,        // sqrt(var)
   OK_ARG,        // int32 row index from an ARGMIN / ARGMAX field, null when none
-  OK_NTH         // the value at an FK_NTH field's row, null when none or null
+  OK_NTH,        // the value at an FK_NTH field's row, null when none or null
+  OK_SUM_I32,    // DECIMAL32 SUM: the int64 sum wrapped to the 32-bit representation
+  OK_MEAN_INT    // DECIMAL32/64 MEAN: representation sum (wrapped to out_width) / count, truncated
 };
 
 struct out_desc {

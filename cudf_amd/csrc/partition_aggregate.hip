@@ -849,6 +849,13 @@ __global__ void __launch_bounds__(RPA_AGG_BLOCK) __attribute__((amdgpu_waves_per
               gp(reinterpret_cast<double*>(data))[pos] = valid ? sum / double(cnt[u]) : 0.0;
               break;
             }
+            case OK_SUM_I32: gp(reinterpret_cast<int32_t*>(data))[pos] = int32_t(uint32_t(raw[u])); break;
+            case OK_MEAN_INT: {  // fixed-point mean: the representation's sum / count, truncated
+              int64_t sum = int64_t(raw[u]);
+              if (ow == 4) sum = int64_t(int32_t(uint32_t(sum)));
+              store_width(data, int64_t(pos), ow, uint64_t(valid ? sum / int64_t(cnt[u]) : 0));
+              break;
+            }
             case OK_VAR:
             case OK_STD: {
               int64_t const dn = int64_t(cnt[u]) - int32_t(lane_value(uint32_t(out_ddof_l), o));

@@ -87,6 +87,7 @@ std::unique_ptr<mig_table> empty_like(mig_column_view const* input, int32_t num_
   for (int32_t c = 0; c < num_columns; c++) {
     owned_column& oc = t->columns[c];
     oc.col.type_id   = input[c].type_id;
+    oc.col.scale     = input[c].scale;
     oc.col.size      = 0;
     if (input[c].type_id == MIG_TYPE_STRING) {
       oc.col.offsets_type_id = MIG_TYPE_INT32;
@@ -121,6 +122,7 @@ std::unique_ptr<mig_table> regroup(mig_column_view const* input, int32_t num_col
     mig_column_view const& v = input[c];
     owned_column& oc         = t->columns[c];
     oc.col.type_id           = v.type_id;
+    oc.col.scale             = v.scale;
     oc.col.size              = int32_t(n);
     if (v.null_mask) {
       oc.mask_bytes    = mask_bytes_for(n);
@@ -431,6 +433,7 @@ std::unique_ptr<mig_table> concatenate(mig_column_view const* columns, int32_t n
     owned_column& oc = t->columns[c];
     auto& pieces     = host_pieces[c];
     oc.col.type_id   = at(0, c).type_id;
+    oc.col.scale     = at(0, c).scale;
     oc.col.size      = int32_t(total);
     bool any_nulls   = false;
     int64_t row      = 0;

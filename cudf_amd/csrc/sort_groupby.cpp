@@ -179,6 +179,7 @@ std::unique_ptr<mig_groupby_result> sort_groupby(mig_column_view const* keys, in
       if (same_view(g.src, v)) return g.view;
     mig_column_view out{};
     out.type_id = v.type_id;
+    out.scale   = v.scale;
     out.size    = kept;
     if (storage_of(v.type_id) == ST_STRING) {
       // sizes in group order, their offsets, then the characters
@@ -331,6 +332,7 @@ std::unique_ptr<mig_groupby_result> sort_groupby(mig_column_view const* keys, in
     mig_column_view const& v = keys[i];
     owned_column oc;
     oc.col.type_id = v.type_id;
+    oc.col.scale   = is_decimal_type(v.type_id) ? v.scale : 0;
     oc.col.size    = int32_t(G);
     if (v.null_mask) {
       oc.mask_bytes    = mask_words_bytes(G);

@@ -49,7 +49,7 @@ def _wrap(out: _lib.OutColumn, mr: TorchMemoryResource) -> Column:
         mask = mask.view(torch.int32)
     if offsets is not None:
         offsets = offsets.view(torch.int32)
-    return Column(out.type_id, out.size, data, mask, out.null_count, 0, offsets)
+    return Column(out.type_id, out.size, data, mask, out.null_count, 0, offsets, out.scale)
 
 
 class GroupBy:

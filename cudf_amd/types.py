@@ -61,6 +61,7 @@ _NP = {
     TypeId.TIMESTAMP_NANOSECONDS: np.int64, TypeId.DURATION_DAYS: np.int32,
     TypeId.DURATION_SECONDS: np.int64, TypeId.DURATION_MILLISECONDS: np.int64,
     TypeId.DURATION_MICROSECONDS: np.int64, TypeId.DURATION_NANOSECONDS: np.int64,
+    TypeId.DECIMAL32: np.int32, TypeId.DECIMAL64: np.int64,  # the fixed-point representation
 }
 
 

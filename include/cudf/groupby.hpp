@@ -36,7 +36,7 @@ namespace detail {
 /// through `mr` with `stream`.
 inline std::unique_ptr<column> adopt(mig_column const& c, device_memory_resource* mr, stream_view stream)
 {
-  auto const type = data_type{static_cast<type_id>(c.type_id)};
+  auto const type = data_type{static_cast<type_id>(c.type_id), c.scale};
   std::vector<std::unique_ptr<column>> children;
   if (type.id() == type_id::STRING) {
     children.push_back(std::make_unique<column>(data_type{type_id::INT32}, c.size + 1,

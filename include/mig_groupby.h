@@ -185,6 +185,8 @@ typedef struct mig_column {
   int64_t data_bytes;       /* bytes allocated for `data` (>= size * width)   */
   int64_t mask_bytes;       /* bytes allocated for `null_mask`                */
   int64_t offsets_bytes;    /* bytes allocated for `offsets`                  */
+  int32_t scale;            /* DECIMAL32 / DECIMAL64: fixed-point scale (the
+                               value is data * 10^scale); 0 otherwise        */
 } mig_column;
 
 typedef struct mig_groupby_result mig_groupby_result;
