@@ -1476,11 +1476,11 @@ int64_t groupby_call::aggregate_partitioned()
       if (R.two_level) R.rowsA = S->get<uint32_t>(rowsA);
       R.rep_stride    = (int64_t(S_slots) + 2 + 31) & ~int64_t(31);  // whole lines per bin
       R.rep_rows      = S->get<uint32_t>(size_t(B) * size_t(R.rep_stride));
-      R.rec_count     = rowsB;
-      R.rec_slot      = S->get<uint32_t>(size_t(rowsB));
+      R.rec_count     = N;  // rec_slot is indexed by input row
+      R.rec_slot      = S->get<uint32_t>(size_t(N));
       R.verified      = S->get<unsigned long long>(1);
       R.verify_failed = S->get<int32_t>(1);
-      MIG_HIP_TRY(hipMemsetAsync(R.rec_slot, 0xff, sizeof(uint32_t) * size_t(rowsB), stream_));
+      MIG_HIP_TRY(hipMemsetAsync(R.rec_slot, 0xff, sizeof(uint32_t) * size_t(N), stream_));
       MIG_HIP_TRY(hipMemsetAsync(R.verified, 0, sizeof(unsigned long long), stream_));
       MIG_HIP_TRY(hipMemsetAsync(R.verify_failed, 0, sizeof(int32_t), stream_));
       part_bytes += 4 * size_t(rowsB) * 2 + (R.two_level ? 4 * size_t(rowsA) : 0) + 4 * size_t(B) * R.rep_stride;

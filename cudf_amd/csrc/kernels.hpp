@@ -250,19 +250,20 @@ struct rpa_params {
   // General keys in hashed form (general_hashed = 1): the record word is
   // (row hash << 32) | a second, independent row hash, the row index travels
   // in rowsA / rowsB, and bins compare words instead of key rows.  A bin slot's
-  // representative row is rep_rows[bin * rep_stride + slot]; rec_slot[r]
-  // (buffer B record r; 0xFFFFFFFF where no record is) receives the slot of
-  // the record's group, and k_rpa_verify compares every record's key row with
-  // its representative's exactly afterwards (any difference reruns the call
-  // with general_hashed = 0, which compares key rows while probing).
+  // representative row is rep_rows[bin * rep_stride + slot]; rec_slot[row]
+  // (input row; 0xFFFFFFFF for dropped rows and representatives) receives the
+  // slot of the row's group, and k_rpa_verify compares every such row's key
+  // row with its representative's exactly afterwards, in row order (any
+  // difference reruns the call with general_hashed = 0, which compares key
+  // rows while probing).
   int32_t general_hashed;
   int32_t stage_row_off;
   int64_t rep_stride;
   uint32_t* rowsA;
   uint32_t* rowsB;
   uint32_t* rep_rows;
-  uint32_t* rec_slot;                // [records of buffer B]
-  int64_t rec_count;                 // records buffer B holds (its positions)
+  uint32_t* rec_slot;                // [input rows]
+  int64_t rec_count;                 // input rows (the verification's range)
   unsigned long long* verified;      // rows compared with a representative
   int32_t* verify_failed;
   // Pass-A plan of the aggregate kernel, read into registers once per

@@ -464,12 +464,13 @@ __device__ __forceinline__ void chunk_slots(gb_params const& P, acc_plan const& 
   }
   if (INSERT && inserted) __hip_atomic_fetch_add(&misc->fill, inserted, __ATOMIC_RELAXED, WG_SCOPE);
   if (KM == KF_HASHED && INSERT) {
-    // each record's group for the verification (coalesced stores); a group's
-    // representative is marked 0xFFFFFFFF, having nothing to compare
+    // each joined row's group for the verification, indexed by input row so
+    // that the verification reads the rows' own keys in order; a group's
+    // representative keeps 0xFFFFFFFF, having nothing to compare
 #pragma unroll
     for (int k = 0; k < RPA_CACHED; k++) {
       int64_t const r = r0 + int64_t(k) * RPA_AGG_BLOCK;
-      if (r < e && slot[k] >= 0) gp(R.rec_slot)[r] = ((reps >> k) & 1u) ? 0xffffffffu : uint32_t(rbase + slot[k]);
+      if (r < e && slot[k] >= 0 && !((reps >> k) & 1u)) gp(R.rec_slot)[in.row[k]] = uint32_t(rbase + slot[k]);
     }
   }
 }
@@ -896,8 +897,8 @@ void launch_rpa_aggregate(gb_params const* dP, rpa_params const* dR, rpa_params 
 // representative: rows whose group was found by equal 64-bit words must also
 // have equal keys (nulls equal, NaN equal NaN, -0.0 equal 0.0, strings
 // byte-wise), otherwise verify_failed is set and the call is redone comparing
-// key rows.  Records are read in order; the key loads of different records
-// are independent and in flight together.
+// key rows.  Rows are visited in input order, so a row's own keys are read
+// sequentially and only the representatives' are random reads.
 __global__ void __launch_bounds__(256) k_rpa_verify(gb_params const* __restrict__ Pp, rpa_params const* __restrict__ Rp,
                                                     int64_t records)
 {
@@ -910,8 +911,8 @@ __global__ void __launch_bounds__(256) k_rpa_verify(gb_params const* __restrict_
   uint32_t count = 0;
   for (int64_t r = int64_t(blockIdx.x) * 256 + threadIdx.x; r < records; r += int64_t(gridDim.x) * 256) {
     uint32_t const gs = gp(R.rec_slot)[r];
-    if (gs == 0xffffffffu) continue;  // no record here, or a representative
-    uint32_t const row = gp(R.rowsB)[r];
+    if (gs == 0xffffffffu) continue;  // a dropped row, or a representative
+    uint32_t const row = uint32_t(r);
     uint32_t const rep = __hip_atomic_load(gp(R.rep_rows) + gs, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (rep == row) continue;
     count++;
