@@ -1226,7 +1226,8 @@ int64_t groupby_call::aggregate_sorted_runs()
     launch_sorted_nunique(dP, f, N, G, gid, S->get<uint32_t>(2 * N), stream_);
   }
   for (int f = 0; f < nf_; f++)
-    if (p.fields[f].kind == FK_NTH) launch_sorted_nth(dP, p, f, G, group_last, stream_);
+    if (p.fields[f].kind == FK_NTH)
+      launch_sorted_nth(dP, p, f, G, group_last, S->get<uint32_t>(std::max<int64_t>(G, 1) + 1), stream_);
   T->mark(5);
   uint32_t* map = S->get<uint32_t>(std::max<int64_t>(G, 1));
   launch_iota_u32(map, G, stream_);

@@ -2144,12 +2144,75 @@ __device__ __forceinline__ int select_bit(uint32_t bits, int r, bool high)
   return __ffs(bits) - 1;
 }
 
-// NTH_ELEMENT on runs with null values excluded: one wave per group counts
-// the valid bits of the value column's mask over the group's rows, 64 mask
-// words (2,048 rows) per step, from the front for n >= 0 and from the back
-// for n < 0, and stops at the step holding the n-th valid row.
+// NTH_ELEMENT on runs with null values excluded.  The group's rows are bits
+// b0..b1 of the value column's mask; the n-th valid row (from the front for
+// n >= 0, from the back for n < 0) is found by counting set bits word by
+// word.  Groups of at most NTH_THREAD_WORDS mask words are counted by one
+// thread each; larger groups are listed (large[0] = count, large[1..] =
+// groups) for k_sorted_nth_scan, where a wave counts 64 words per step.
+constexpr int NTH_THREAD_WORDS = 8;
+
+struct nth_bits {
+  int64_t b0, b1, w0, w1;
+};
+
+__device__ __forceinline__ nth_bits nth_group_bits(gb_params const& P, dcol const& c, uint32_t const* group_last,
+                                                   int64_t g)
+{
+  nth_bits r;
+  r.b0 = int64_t(P.slot_rows[g]) + c.mask_offset;
+  r.b1 = min(int64_t(group_last[g]), P.num_rows - 1) + c.mask_offset;
+  r.w0 = r.b0 >> 5;
+  r.w1 = r.b1 >> 5;
+  return r;
+}
+
+// Mask word w restricted to the group's bits.
+__device__ __forceinline__ uint32_t nth_word(gmem<uint32_t const>* mask, nth_bits const& q, int64_t w)
+{
+  uint32_t bits = mask[w];
+  if (w == q.w0) bits &= ~0u << (q.b0 & 31);
+  if (w == q.w1 && (q.b1 & 31) != 31) bits &= (2u << (q.b1 & 31)) - 1u;
+  return bits;
+}
+
 __global__ void __launch_bounds__(BLOCK)
-  k_sorted_nth_scan(gb_params const* __restrict__ Pp, int32_t f, int64_t G, uint32_t const* __restrict__ group_last)
+  k_sorted_nth_small(gb_params const* __restrict__ Pp, int32_t f, int64_t G, uint32_t const* __restrict__ group_last,
+                     uint32_t* __restrict__ large)
+{
+  int64_t const g = int64_t(blockIdx.x) * BLOCK + threadIdx.x;
+  if (g >= G) return;
+  gb_params const& P  = *Pp;
+  field_desc const& F = P.fields[f];
+  dcol const& c       = P.vals[F.vcol];
+  nth_bits const q    = nth_group_bits(P, c, group_last, g);
+  if (q.w1 - q.w0 >= NTH_THREAD_WORDS) {
+    uint32_t const i = atomicAdd(&large[0], 1u);
+    large[1 + i]     = uint32_t(g);
+    return;
+  }
+  auto const mask    = gp(c.mask);
+  int64_t const nth  = F.aux_sum;
+  bool const back    = nth < 0;
+  int64_t const want = back ? -nth - 1 : nth;
+  int64_t seen       = 0;
+  uint32_t found     = 0xFFFFFFFFu;
+  for (int64_t k = 0; k <= q.w1 - q.w0; k++) {
+    int64_t const w     = back ? q.w1 - k : q.w0 + k;
+    uint32_t const bits = nth_word(mask, q, w);
+    int const cnt       = __popc(bits);
+    if (seen + cnt > want) {
+      found = uint32_t((w << 5) + select_bit(bits, int(want - seen), back) - c.mask_offset);
+      break;
+    }
+    seen += cnt;
+  }
+  *reinterpret_cast<uint32_t*>(field_at(F, g)) = found;
+}
+
+__global__ void __launch_bounds__(BLOCK)
+  k_sorted_nth_scan(gb_params const* __restrict__ Pp, int32_t f, uint32_t const* __restrict__ group_last,
+                    uint32_t const* __restrict__ large)
 {
   gb_params const& P  = *Pp;
   field_desc const& F = P.fields[f];
@@ -2160,20 +2223,16 @@ __global__ void __launch_bounds__(BLOCK)
   int64_t const want  = back ? -nth - 1 : nth;  // 0-based among the valid rows, from the chosen end
   int const lane      = threadIdx.x & 63;
   int64_t const waves = int64_t(gridDim.x) * (BLOCK / 64);
-  for (int64_t g = (int64_t(blockIdx.x) * BLOCK + threadIdx.x) >> 6; g < G; g += waves) {
-    int64_t const b0 = int64_t(P.slot_rows[g]) + c.mask_offset;  // the group's bits b0..b1 of the mask
-    int64_t const b1 = min(int64_t(group_last[g]), P.num_rows - 1) + c.mask_offset;
-    int64_t const w0 = b0 >> 5, w1 = b1 >> 5;
+  int64_t const count = int64_t(large[0]);
+  for (int64_t i = (int64_t(blockIdx.x) * BLOCK + threadIdx.x) >> 6; i < count; i += waves) {
+    int64_t const g  = int64_t(large[1 + i]);
+    nth_bits const q = nth_group_bits(P, c, group_last, g);
     int64_t seen     = 0;
     uint32_t found   = 0xFFFFFFFFu;
-    for (int64_t k = 0; k <= w1 - w0; k += 64) {
-      int64_t const w = back ? w1 - k - lane : w0 + k + lane;
+    for (int64_t k = 0; k <= q.w1 - q.w0; k += 64) {
+      int64_t const w = back ? q.w1 - k - lane : q.w0 + k + lane;
       uint32_t bits   = 0;
-      if (w >= w0 && w <= w1) {
-        bits = mask[w];
-        if (w == w0) bits &= ~0u << (b0 & 31);
-        if (w == w1 && (b1 & 31) != 31) bits &= (2u << (b1 & 31)) - 1u;
-      }
+      if (w >= q.w0 && w <= q.w1) bits = nth_word(mask, q, w);
       int const cnt = __popc(bits);
       int upto      = cnt;  // inclusive prefix over the lanes
 #pragma unroll
@@ -2198,17 +2257,19 @@ __global__ void __launch_bounds__(BLOCK)
 }
 
 void launch_sorted_nth(gb_params const* dP, gb_params const& hP, int32_t f, int64_t G, uint32_t const* group_last,
-                       hipStream_t s)
+                       uint32_t* large, hipStream_t s)
 {
   if (G <= 0) return;
   field_desc const& F = hP.fields[f];
+  unsigned const per_group = unsigned((G + BLOCK - 1) / BLOCK);
   if (F.aux_count != 0 || hP.vals[F.vcol].mask == nullptr) {
-    hipLaunchKernelGGL(k_sorted_nth_direct, dim3(unsigned((G + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0, s, dP, f, G,
-                       group_last);
-  } else {
-    int64_t const blocks = std::min<int64_t>((G + BLOCK / 64 - 1) / (BLOCK / 64), int64_t(1) << 18);
-    hipLaunchKernelGGL(k_sorted_nth_scan, dim3(unsigned(blocks)), dim3(BLOCK), 0, s, dP, f, G, group_last);
+    hipLaunchKernelGGL(k_sorted_nth_direct, dim3(per_group), dim3(BLOCK), 0, s, dP, f, G, group_last);
+    return;
   }
+  MIG_HIP_TRY(hipMemsetAsync(large, 0, sizeof(uint32_t), s));
+  hipLaunchKernelGGL(k_sorted_nth_small, dim3(per_group), dim3(BLOCK), 0, s, dP, f, G, group_last, large);
+  int64_t const blocks = std::min<int64_t>((G + BLOCK / 64 - 1) / (BLOCK / 64), 4096);
+  hipLaunchKernelGGL(k_sorted_nth_scan, dim3(unsigned(blocks)), dim3(BLOCK), 0, s, dP, f, group_last, large);
 }
 
 void launch_sorted_nunique(gb_params const* dP, int32_t f, int64_t n, int64_t G, uint32_t const* gid,

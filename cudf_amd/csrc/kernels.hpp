@@ -365,7 +365,7 @@ void launch_sorted_nunique(gb_params const* dP, int32_t f, int64_t n, int64_t G,
 // NTH_ELEMENT of field `f` on runs: the row of each group's n-th counted value,
 // from the groups' first and last rows (launch_sorted_assign's group_last).
 void launch_sorted_nth(gb_params const* dP, gb_params const& hP, int32_t f, int64_t G, uint32_t const* group_last,
-                       hipStream_t s);
+                       uint32_t* large_scratch /* G + 1 entries */, hipStream_t s);
 void launch_iota_u32(uint32_t* out, int64_t n, hipStream_t s);
 // ARGMIN / ARGMAX fields of string columns on runs (the reduce passes skip them).
 void launch_sorted_string_arg(gb_params const* dP, gb_params const& hP, int64_t n, uint32_t const* gid,
