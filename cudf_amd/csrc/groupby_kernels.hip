@@ -1593,7 +1593,8 @@ struct sorted_pass {
   int32_t col_of[SORTED_FPP];   // loaded column slot of each field (-1: the row count)
   int32_t kind[SORTED_FPP];     // accumulator kind (FK_COUNT for the row count)
   int32_t op[SORTED_FPP];       // combining operation (seg_op)
-  uint64_t ident[SORTED_FPP];   // identity in the accumulator encoding
+  uint64_t ident[SORTED_FPP];   // identity, in the scan domain (see scan_domain)
+  int32_t xf[SORTED_FPP];       // accumulator encoding -> scan domain (scan_domain)
   int32_t col[SORTED_COLS];     // value column (index into P.vals) of each slot
 };
 
@@ -1682,20 +1683,76 @@ __device__ __forceinline__ uint64_t sorted_row_part(gb_params const& P, int32_t 
   return uint64_t(__double_as_longlong(d * d));
 }
 
-// Segmented scan with the combining operation chosen once per call.
+// The runs reduction works in a "scan domain" with few combining operations:
+// every MIN / MAX accumulator becomes a signed 64-bit MIN / MAX (32-bit
+// signed values sign-extended, 32-bit unsigned values and float order codes
+// zero-extended, 64-bit unsigned values and float order codes with the sign
+// bit flipped, which maps unsigned order onto signed order).  Sums, counts
+// and products keep their operation.  Fewer operations, fewer instantiated
+// scans: the kernel's hot loop stays small for the instruction cache.
+enum scan_xf : int32_t { XF_NONE = 0, XF_SEXT32 = 1, XF_ZEXT32 = 2, XF_FLIP64 = 3 };
+
+__host__ __device__ inline void scan_domain(int32_t kind, int32_t& op, int32_t& xf)
+{
+  xf = XF_NONE;
+  switch (kind) {
+    case FK_MIN_S32: op = OP_MIN_S64; xf = XF_SEXT32; return;
+    case FK_MAX_S32: op = OP_MAX_S64; xf = XF_SEXT32; return;
+    case FK_MIN_U32:
+    case FK_MIN_F32: op = OP_MIN_S64; xf = XF_ZEXT32; return;
+    case FK_MAX_U32:
+    case FK_MAX_F32: op = OP_MAX_S64; xf = XF_ZEXT32; return;
+    case FK_MIN_U64:
+    case FK_MIN_F64: op = OP_MIN_S64; xf = XF_FLIP64; return;
+    case FK_MAX_U64:
+    case FK_MAX_F64: op = OP_MAX_S64; xf = XF_FLIP64; return;
+    default: op = seg_op_of(kind); return;
+  }
+}
+
+__host__ __device__ inline uint64_t xf_fwd(int32_t xf, uint64_t u)
+{
+  switch (xf) {
+    case XF_SEXT32: return uint64_t(int64_t(int32_t(uint32_t(u))));
+    case XF_ZEXT32: return uint64_t(uint32_t(u));
+    case XF_FLIP64: return u ^ 0x8000000000000000ull;
+    default: return u;
+  }
+}
+
+__host__ __device__ inline uint64_t xf_inv(int32_t xf, uint64_t t)
+{
+  switch (xf) {
+    case XF_SEXT32:
+    case XF_ZEXT32: return uint64_t(uint32_t(t));
+    case XF_FLIP64: return t ^ 0x8000000000000000ull;
+    default: return t;
+  }
+}
+
+// Combine two partials of the scan domain.
+__device__ __forceinline__ uint64_t domain_combine(int32_t op, uint64_t a, uint64_t b)
+{
+  switch (op) {
+    case OP_ADD_F32: return op_combine<OP_ADD_F32>(a, b);
+    case OP_ADD_F64: return op_combine<OP_ADD_F64>(a, b);
+    case OP_MIN_S64: return op_combine<OP_MIN_S64>(a, b);
+    case OP_MAX_S64: return op_combine<OP_MAX_S64>(a, b);
+    case OP_MUL_I64: return op_combine<OP_MUL_I64>(a, b);
+    case OP_MUL_F32: return op_combine<OP_MUL_F32>(a, b);
+    case OP_MUL_F64: return op_combine<OP_MUL_F64>(a, b);
+    default: return op_combine<OP_ADD_I64>(a, b);
+  }
+}
+
+// Segmented scan of the scan domain, the operation chosen once per call.
 __device__ __forceinline__ uint64_t segmented_scan_op(int32_t op, uint64_t part, uint32_t g)
 {
   switch (op) {
     case OP_ADD_F32: return seg_scan<OP_ADD_F32>(part, g);
     case OP_ADD_F64: return seg_scan<OP_ADD_F64>(part, g);
-    case OP_MIN_S32: return seg_scan<OP_MIN_S32>(part, g);
-    case OP_MAX_S32: return seg_scan<OP_MAX_S32>(part, g);
     case OP_MIN_S64: return seg_scan<OP_MIN_S64>(part, g);
     case OP_MAX_S64: return seg_scan<OP_MAX_S64>(part, g);
-    case OP_MIN_U32: return seg_scan<OP_MIN_U32>(part, g);
-    case OP_MAX_U32: return seg_scan<OP_MAX_U32>(part, g);
-    case OP_MIN_U64: return seg_scan<OP_MIN_U64>(part, g);
-    case OP_MAX_U64: return seg_scan<OP_MAX_U64>(part, g);
     case OP_MUL_I64: return seg_scan<OP_MUL_I64>(part, g);
     case OP_MUL_F32: return seg_scan<OP_MUL_F32>(part, g);
     case OP_MUL_F64: return seg_scan<OP_MUL_F64>(part, g);
@@ -1847,11 +1904,13 @@ __global__ void __launch_bounds__(BLOCK)
         }
         continue;
       }
-      if (carry_ends && lane == 0) sorted_flush(P, f, carry_g, c, carry_g == first_g);
-      uint64_t part         = sorted_row_part(P, f, kind, pass.ident[k], g, vv, v);
-      if (lane == 0 && carry_g != NO_GROUP && g == carry_g) part = sorted_combine(kind, part, c);
-      part = segmented_scan_op(pass.op[k], part, g);
-      if (ends_here) sorted_flush(P, f, g, part, g == first_g);
+      int32_t const op = pass.op[k], xf = pass.xf[k];
+      if (carry_ends && lane == 0) sorted_flush(P, f, carry_g, xf_inv(xf, c), carry_g == first_g);
+      uint64_t part = sorted_row_part(P, f, kind, pass.ident[k], g, vv, v);
+      if (f >= 0 && g != NO_GROUP && vv) part = xf_fwd(xf, part);  // the identity is already in the domain
+      if (lane == 0 && carry_g != NO_GROUP && g == carry_g) part = domain_combine(op, part, c);
+      part = segmented_scan_op(op, part, g);
+      if (ends_here) sorted_flush(P, f, g, xf_inv(xf, part), g == first_g);
       uint64_t const last = (uint64_t(uint32_t(__builtin_amdgcn_readlane(int(uint32_t(part >> 32)), 63))) << 32) |
                             uint32_t(__builtin_amdgcn_readlane(int(uint32_t(part)), 63));
       if (lane == 0) carry[wave][k] = last;
@@ -1863,7 +1922,7 @@ __global__ void __launch_bounds__(BLOCK)
       if (HAS_ARG && (pass.kind[k] == FK_ARGMIN || pass.kind[k] == FK_ARGMAX))
         arg_flush(P, pass.field[k], carry_g, carry_row[wave][k], true);
       else
-        sorted_flush(P, pass.field[k], carry_g, carry[wave][k], true);
+        sorted_flush(P, pass.field[k], carry_g, xf_inv(pass.xf[k], carry[wave][k]), true);
     }
   }
 }
@@ -1933,8 +1992,8 @@ static void launch_sorted_passes(gb_params const* dP, gb_params const& hP, int64
       pass.field[pass.n]  = f;
       pass.col_of[pass.n] = slot;
       pass.kind[pass.n]   = kind;
-      pass.op[pass.n]     = seg_op_of(kind);
-      pass.ident[pass.n]  = kind == FK_M2 ? 0ull : field_identity(kind);
+      scan_domain(kind, pass.op[pass.n], pass.xf[pass.n]);
+      pass.ident[pass.n]  = kind == FK_M2 ? 0ull : xf_fwd(pass.xf[pass.n], field_identity(kind));
       pass.n++;
       i++;
     }
