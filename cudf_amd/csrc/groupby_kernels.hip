@@ -10,6 +10,7 @@
 
 #include <map>
 #include <mutex>
+#include <type_traits>
 
 namespace mig {
 
@@ -1957,13 +1958,468 @@ static void sorted_ranges(int64_t n, int num_cus, int64_t& per, int64_t& W)
   W = (n + per - 1) / per;
 }
 
+// ---------------------------------------------------------------------------
+// Column statistics on runs.  A value column whose accumulators are all among
+// SUM, SUM_OF_SQUARES, COUNT, MIN, MAX and M2 (VARIANCE / STD), stored as int32, int64 or
+// float64, is reduced by a kernel compiled for its type: the combining
+// operations are fixed at compile time, the segment masks of the six scan
+// steps are computed once per chunk for all accumulators, and the partials
+// carried between chunks stay in registers.  M2 comes from the sums in the
+// same pass where that is exact or within the bound below; the groups where
+// it is not (and the groups a wave shares with its neighbours) are flagged
+// and get the exact two-pass M2 (the reference's mean, then squared
+// deviations: sort/group_m2.cu, sort/aggregate.cpp:286-306) in
+// k_sorted_stats_m2, which only touches the rows of flagged groups.
+template <int ST>
+struct stats_traits;
+template <>
+struct stats_traits<ST_I32> {
+  using T   = int32_t;
+  using Sum = int64_t;
+};
+template <>
+struct stats_traits<ST_I64> {
+  using T   = int64_t;
+  using Sum = int64_t;
+};
+template <>
+struct stats_traits<ST_F64> {
+  using T   = double;
+  using Sum = double;
+};
+
+// Value of a row in the signed 64-bit MIN / MAX domain (floats: the order
+// code with the sign bit flipped, as scan_domain maps them).
+template <int ST>
+__device__ __forceinline__ int64_t stats_order(typename stats_traits<ST>::T x)
+{
+  if constexpr (ST == ST_F64) return int64_t(ord64(uint64_t(__double_as_longlong(x))) ^ 0x8000000000000000ull);
+  else return int64_t(x);
+}
+
+// The accumulator encoding of a MIN / MAX domain value.
+template <int ST>
+__device__ __forceinline__ uint64_t stats_order_inv(int64_t t)
+{
+  if constexpr (ST == ST_F64) return uint64_t(t) ^ 0x8000000000000000ull;
+  else if constexpr (ST == ST_I32) return uint64_t(uint32_t(int32_t(t)));
+  else return uint64_t(t);
+}
+
+// Same-group masks of the six scan steps (see seg_scan): lane l combines the
+// partial the step's DPP control brings when that lane holds l's group.
+struct seg_masks {
+  bool s[6];
+};
+
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ bool seg_same(uint32_t g)
+{
+  constexpr int NOT_A_GROUP = int(0xFFFFFFFEu);
+  return uint32_t(__builtin_amdgcn_update_dpp(NOT_A_GROUP, int(g), CTRL, ROW_MASK, 0xF, false)) == g;
+}
+
+__device__ __forceinline__ seg_masks seg_masks_of(uint32_t g)
+{
+  return seg_masks{{seg_same<0x111, 0xF>(g), seg_same<0x112, 0xF>(g), seg_same<0x114, 0xF>(g),
+                    seg_same<0x118, 0xF>(g), seg_same<0x142, 0xA>(g), seg_same<0x143, 0xC>(g)}};
+}
+
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ uint64_t dpp64(uint64_t x)
+{
+  uint32_t const lo = uint32_t(__builtin_amdgcn_update_dpp(0, int(uint32_t(x)), CTRL, ROW_MASK, 0xF, false));
+  uint32_t const hi = uint32_t(__builtin_amdgcn_update_dpp(0, int(uint32_t(x >> 32)), CTRL, ROW_MASK, 0xF, false));
+  return (uint64_t(hi) << 32) | lo;
+}
+
+// Segmented inclusive scan with precomputed masks; `F` combines two partials.
+template <typename V, typename F>
+__device__ __forceinline__ V mscan(V x, seg_masks const& m, F&& f)
+{
+  auto step = [&](auto ctrl, auto rmask, bool same) {
+    constexpr int C = decltype(ctrl)::value, R = decltype(rmask)::value;
+    V o;
+    if constexpr (sizeof(V) == 8) {
+      uint64_t const u = dpp64<C, R>(__builtin_bit_cast(uint64_t, x));
+      o                = __builtin_bit_cast(V, u);
+    } else {
+      o = __builtin_bit_cast(V, uint32_t(__builtin_amdgcn_update_dpp(0, int(__builtin_bit_cast(uint32_t, x)), C, R,
+                                                                     0xF, false)));
+    }
+    if (same) x = f(x, o);
+  };
+  step(std::integral_constant<int, 0x111>{}, std::integral_constant<int, 0xF>{}, m.s[0]);
+  step(std::integral_constant<int, 0x112>{}, std::integral_constant<int, 0xF>{}, m.s[1]);
+  step(std::integral_constant<int, 0x114>{}, std::integral_constant<int, 0xF>{}, m.s[2]);
+  step(std::integral_constant<int, 0x118>{}, std::integral_constant<int, 0xF>{}, m.s[3]);
+  step(std::integral_constant<int, 0x142>{}, std::integral_constant<int, 0xA>{}, m.s[4]);
+  step(std::integral_constant<int, 0x143>{}, std::integral_constant<int, 0xC>{}, m.s[5]);
+  return x;
+}
+
+template <typename V>
+__device__ __forceinline__ V lane63(V x)
+{
+  if constexpr (sizeof(V) == 8) {
+    uint64_t const u = __builtin_bit_cast(uint64_t, x);
+    uint64_t const r = (uint64_t(uint32_t(__builtin_amdgcn_readlane(int(uint32_t(u >> 32)), 63))) << 32) |
+                       uint32_t(__builtin_amdgcn_readlane(int(uint32_t(u)), 63));
+    return __builtin_bit_cast(V, r);
+  } else {
+    return __builtin_bit_cast(V, uint32_t(__builtin_amdgcn_readlane(int(__builtin_bit_cast(uint32_t, x)), 63)));
+  }
+}
+
+// M2 of a whole group from its count n, sum s1 and sum of squares s2 (the
+// rule of the partitioned path's m2_from_sums): integer sources while s2 <
+// 2^53 (every partial then exact in double) through n·M2 = n·s2 − s1² in 128
+// bits; float64 sources with at most 256 values when the rounding error of
+// s2 − s1²/n, at most ~3·n·ε·s2, is below 1e-11 of the result.  False: the
+// group goes to the exact pass.
+template <int ST>
+__device__ __forceinline__ bool stats_m2(uint32_t n, typename stats_traits<ST>::Sum s1, double s2, double& m2)
+{
+  if (n <= 1) {
+    m2 = 0.0;
+    return true;
+  }
+  if constexpr (ST == ST_F64) {
+    if (n > 256) return false;
+    double const v = s2 - s1 * (s1 / double(n));
+    if (!(v > 0.0) || 3.0 * double(n) * 1.1102230246251565e-16 * s2 > 1e-11 * v) {
+      m2 = 0.0;
+      return v == 0.0 && s2 == 0.0;
+    }
+    m2 = v;
+    return true;
+  } else {
+    if (!(s2 < 9007199254740992.0)) return false;
+    __int128 const num = __int128(n) * __int128(uint64_t(s2)) - __int128(s1) * __int128(s1);
+    m2                 = double(num) / double(n);
+    return true;
+  }
+}
+
+struct sorted_stats_job {
+  int32_t vcol;
+  int32_t f_sum, f_count, f_min, f_max, f_m2;  // field index, -1 when not requested
+  int32_t f_sumsq;
+  int32_t count_rows;                          // also the group row count
+  int64_t min_ident, max_ident;                // identities in the MIN / MAX domain
+  uint8_t* m2_flag;                            // per group: 1 = M2 by the exact pass
+};
+
+template <int ST>
+struct stats_part {
+  uint32_t rows, cnt;
+  typename stats_traits<ST>::Sum sum;
+  int64_t mn, mx;
+  double sq;     // Σx² in double (M2; SUM_OF_SQUARES of float64)
+  uint64_t isq;  // Σx² of integers, wrapping (SUM_OF_SQUARES of int32 / int64)
+};
+
+// A group's partials: stored when the group lies inside one wave's range,
+// combined atomically when neighbouring ranges hold rows of it.
+template <int ST>
+__device__ __forceinline__ void stats_flush(gb_params const& P, sorted_stats_job const& J, uint32_t g,
+                                            stats_part<ST> const& a, bool shared)
+{
+  using Sum = typename stats_traits<ST>::Sum;
+  if (J.count_rows) {
+    if (shared) aadd(count_at(P, g), a.rows);
+    else *gp(count_at(P, g)) = a.rows;
+  }
+  if (J.f_count >= 0) {
+    auto* p = reinterpret_cast<uint32_t*>(field_at(P.fields[J.f_count], g));
+    if (shared) aadd(p, a.cnt);
+    else *gp(p) = a.cnt;
+  }
+  if (J.f_sum >= 0) {
+    auto* p = field_at(P.fields[J.f_sum], g);
+    if constexpr (ST == ST_F64) {
+      if (shared) gadd_f64(reinterpret_cast<double*>(p), a.sum);
+      else *gp(reinterpret_cast<double*>(p)) = a.sum;
+    } else {
+      if (shared) aadd(reinterpret_cast<uint64_t*>(p), uint64_t(a.sum));
+      else *gp(reinterpret_cast<Sum*>(p)) = a.sum;
+    }
+  }
+  auto minmax = [&](int32_t f, int64_t t, bool is_min) {
+    if (f < 0) return;
+    auto* p          = field_at(P.fields[f], g);
+    uint64_t const u = stats_order_inv<ST>(t);
+    if constexpr (ST == ST_I32) {
+      if (!shared) *gp(reinterpret_cast<int32_t*>(p)) = int32_t(uint32_t(u));
+      else if (is_min) amin(reinterpret_cast<int32_t*>(p), int32_t(uint32_t(u)));
+      else amax(reinterpret_cast<int32_t*>(p), int32_t(uint32_t(u)));
+    } else if constexpr (ST == ST_I64) {
+      if (!shared) *gp(reinterpret_cast<int64_t*>(p)) = int64_t(u);
+      else if (is_min) amin(reinterpret_cast<int64_t*>(p), int64_t(u));
+      else amax(reinterpret_cast<int64_t*>(p), int64_t(u));
+    } else {
+      if (!shared) *gp(reinterpret_cast<uint64_t*>(p)) = u;
+      else if (is_min) amin(reinterpret_cast<uint64_t*>(p), u);
+      else amax(reinterpret_cast<uint64_t*>(p), u);
+    }
+  };
+  minmax(J.f_min, a.mn, true);
+  minmax(J.f_max, a.mx, false);
+  if (J.f_sumsq >= 0) {
+    auto* p = field_at(P.fields[J.f_sumsq], g);
+    if constexpr (ST == ST_F64) {
+      if (shared) gadd_f64(reinterpret_cast<double*>(p), a.sq);
+      else *gp(reinterpret_cast<double*>(p)) = a.sq;
+    } else {
+      if (shared) aadd(reinterpret_cast<uint64_t*>(p), a.isq);
+      else *gp(reinterpret_cast<uint64_t*>(p)) = a.isq;
+    }
+  }
+  if (J.f_m2 >= 0) {
+    double m2  = 0.0;
+    bool exact = !shared && stats_m2<ST>(a.cnt, a.sum, a.sq, m2);
+    if (exact && m2 != 0.0) *gp(reinterpret_cast<double*>(field_at(P.fields[J.f_m2], g))) = m2;
+    *gp(J.m2_flag + g) = exact ? 0 : 1;
+  }
+}
+
+// One wave per contiguous row range, 64-row chunks (the range layout and the
+// carry between chunks of k_sorted_reduce).
+template <int ST>
+__global__ void __launch_bounds__(BLOCK)
+  k_sorted_stats(gb_params const* __restrict__ Pp, uint32_t const* __restrict__ gid, int64_t per, int64_t W,
+                 sorted_stats_job J)
+{
+  using T            = typename stats_traits<ST>::T;
+  using Sum          = typename stats_traits<ST>::Sum;
+  gb_params const& P = *Pp;
+  int const lane     = threadIdx.x & 63;
+  int64_t const w    = int64_t(blockIdx.x) * (BLOCK / 64) + (threadIdx.x >> 6);
+  if (w >= W) return;
+  int64_t const b = w * per;
+  int64_t const e = b + per < P.num_rows ? b + per : P.num_rows;
+  dcol const c    = P.vals[J.vcol];
+  auto const* data = reinterpret_cast<gmem<T const>*>(gp(c.data));
+  bool const want_sq  = J.f_m2 >= 0 || (ST == ST_F64 && J.f_sumsq >= 0);
+  bool const want_isq = ST != ST_F64 && J.f_sumsq >= 0;
+  uint32_t const first_g = gid[b];  // may continue from the previous range
+  uint32_t carry_g       = NO_GROUP;
+  stats_part<ST> cy{0, 0, Sum(0), J.min_ident, J.max_ident, 0.0, 0};
+  for (int64_t i0 = b; i0 < e; i0 += 64) {
+    int64_t const row = i0 + lane;
+    bool const live   = row < e;
+    int64_t const rc  = live ? row : e - 1;
+    uint32_t const gl = gid[rc];
+    T const x         = data[rc];
+    bool const vv     = dcol_valid(c, rc);
+    uint32_t const g  = live ? gl : NO_GROUP;
+    bool has          = g != NO_GROUP && vv;
+    stats_part<ST> a;
+    a.rows = g != NO_GROUP ? 1u : 0u;
+    a.cnt  = has ? 1u : 0u;
+    a.sum  = has ? Sum(x) : Sum(0);
+    a.sq   = has && want_sq ? double(x) * double(x) : 0.0;
+    a.isq  = has && want_isq ? uint64_t(int64_t(x)) * uint64_t(int64_t(x)) : 0;
+    bool ordered = has;
+    if constexpr (ST == ST_F64) ordered = has && !isnan(x);  // NaN takes no part in MIN / MAX
+    int64_t const t = stats_order<ST>(x);
+    a.mn            = ordered ? t : J.min_ident;
+    a.mx            = ordered ? t : J.max_ident;
+    uint32_t const gn = uint32_t(__builtin_amdgcn_update_dpp(int(NO_GROUP), int(g), 0x130, 0xF, 0xF, false));
+    uint32_t const g0 = uint32_t(__builtin_amdgcn_readlane(int(g), 0));
+    if (carry_g != NO_GROUP && g0 != carry_g) {
+      if (lane == 0) stats_flush<ST>(P, J, carry_g, cy, carry_g == first_g);
+    } else if (carry_g != NO_GROUP && lane == 0) {  // the carried group continues at lane 0
+      a.rows += cy.rows;
+      a.cnt += cy.cnt;
+      a.sum += cy.sum;
+      a.sq += cy.sq;
+      a.isq += cy.isq;
+      a.mn = a.mn < cy.mn ? a.mn : cy.mn;
+      a.mx = a.mx > cy.mx ? a.mx : cy.mx;
+    }
+    seg_masks const m = seg_masks_of(g);
+    auto add32        = [](uint32_t p, uint32_t q) { return p + q; };
+    if (J.count_rows) a.rows = mscan(a.rows, m, add32);
+    a.cnt = mscan(a.cnt, m, add32);
+    a.sum = mscan(a.sum, m, [](Sum p, Sum q) { return p + q; });
+    if (J.f_min >= 0) a.mn = mscan(a.mn, m, [](int64_t p, int64_t q) { return p < q ? p : q; });
+    if (J.f_max >= 0) a.mx = mscan(a.mx, m, [](int64_t p, int64_t q) { return p > q ? p : q; });
+    if (want_sq) a.sq = mscan(a.sq, m, [](double p, double q) { return p + q; });
+    if (want_isq) a.isq = mscan(a.isq, m, [](uint64_t p, uint64_t q) { return p + q; });
+    if (g != NO_GROUP && lane < 63 && gn != g) stats_flush<ST>(P, J, g, a, g == first_g);
+    cy.rows = lane63(a.rows);
+    cy.cnt  = lane63(a.cnt);
+    cy.sum  = lane63(a.sum);
+    cy.mn   = lane63(a.mn);
+    cy.mx   = lane63(a.mx);
+    cy.sq   = lane63(a.sq);
+    cy.isq  = lane63(a.isq);
+    carry_g = uint32_t(__builtin_amdgcn_readlane(int(g), 63));
+  }
+  if (carry_g != NO_GROUP && lane == 0) stats_flush<ST>(P, J, carry_g, cy, true);  // may continue in the next range
+}
+
+// The exact M2 of the flagged groups: Σ(x − mean)² with the mean from the
+// finished sum and count, summed over each run of a chunk and added to the
+// group's M2 once per chunk.  Chunks without a flagged row only read their
+// group numbers and flags.
+template <int ST>
+__global__ void __launch_bounds__(BLOCK)
+  k_sorted_stats_m2(gb_params const* __restrict__ Pp, uint32_t const* __restrict__ gid, sorted_stats_job J)
+{
+  using T            = typename stats_traits<ST>::T;
+  gb_params const& P = *Pp;
+  int64_t const n    = P.num_rows;
+  int const lane     = threadIdx.x & 63;
+  dcol const c       = P.vals[J.vcol];
+  auto const* data   = reinterpret_cast<gmem<T const>*>(gp(c.data));
+  field_desc const& F = P.fields[J.f_m2];
+  for (int64_t base = int64_t(blockIdx.x) * BLOCK + (threadIdx.x & ~63); base < n;
+       base += int64_t(gridDim.x) * BLOCK) {
+    int64_t const row = base + lane;
+    uint32_t const g  = row < n ? gid[row] : NO_GROUP;
+    bool const fl     = g != NO_GROUP && gp(J.m2_flag)[g] != 0;
+    if (__builtin_amdgcn_ballot_w64(fl) == 0) continue;
+    double d2 = 0.0;
+    if (fl && dcol_valid(c, row)) {
+      double const cnt  = double(count_of(P, F.aux_count, g));
+      double const mean = field_as_double(P.fields[F.aux_sum], g) / cnt;
+      double const d    = double(data[row]) - mean;
+      d2                = d * d;
+    }
+    uint32_t const gk  = fl ? g : NO_GROUP;
+    seg_masks const m  = seg_masks_of(gk);
+    d2                 = mscan(d2, m, [](double p, double q) { return p + q; });
+    uint32_t const gn  = uint32_t(__builtin_amdgcn_update_dpp(int(NO_GROUP), int(gk), 0x130, 0xF, 0xF, false));
+    if (fl && (lane == 63 || gn != gk) && d2 != 0.0) gadd_f64(reinterpret_cast<double*>(field_at(F, g)), d2);
+  }
+}
+
+template <int ST>
+static void launch_stats_job(gb_params const* dP, sorted_stats_job const& J, int64_t n, uint32_t const* gid,
+                             int num_cus, hipStream_t s)
+{
+  int64_t per, W;
+  sorted_ranges(n, num_cus, per, W);
+  unsigned const blocks = unsigned((W + BLOCK / 64 - 1) / (BLOCK / 64));
+  hipLaunchKernelGGL(k_sorted_stats<ST>, dim3(blocks), dim3(BLOCK), 0, s, dP, gid, per, W, J);
+}
+
+template <int ST>
+static void launch_stats_m2(gb_params const* dP, sorted_stats_job const& J, int64_t n, uint32_t const* gid,
+                            int num_cus, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_sorted_stats_m2<ST>, dim3(unsigned(row_grid(n, num_cus, 8))), dim3(BLOCK), 0, s, dP, gid, J);
+}
+
+static bool stats_kind_ok(int32_t st, int32_t kind)
+{
+  switch (kind) {
+    case FK_COUNT:
+    case FK_M2: return true;
+    case FK_SUM_I64: return st == ST_I32 || st == ST_I64;
+    case FK_SUM_F64: return st == ST_F64;
+    case FK_SUMSQ_I64: return st == ST_I32 || st == ST_I64;
+    case FK_SUMSQ_F64: return st == ST_F64;
+    case FK_MIN_S32:
+    case FK_MAX_S32: return st == ST_I32;
+    case FK_MIN_S64:
+    case FK_MAX_S64: return st == ST_I64;
+    case FK_MIN_F64:
+    case FK_MAX_F64: return st == ST_F64;
+    default: return false;
+  }
+}
+
+// The column jobs of a request: every value column whose fields all have a
+// specialised form.  The first job also counts the group rows when the
+// request needs them.
+static std::vector<std::pair<int32_t, sorted_stats_job>> stats_jobs(gb_params const& hP, uint8_t* m2_flag,
+                                                                   int64_t G)
+{
+  std::vector<std::pair<int32_t, sorted_stats_job>> jobs;
+  int64_t m2_jobs = 0;
+  for (int j = 0; j < hP.num_vals; j++) {
+    int32_t const st = hP.vals[j].storage;
+    int const fb = hP.val_field_begin[j], fe = hP.val_field_begin[j + 1];
+    if (fb == fe || (st != ST_I32 && st != ST_I64 && st != ST_F64)) continue;
+    sorted_stats_job J{};
+    J.vcol = j;
+    J.f_sum = J.f_count = J.f_min = J.f_max = J.f_m2 = J.f_sumsq = -1;
+    bool ok = true;
+    for (int f = fb; f < fe && ok; f++) {
+      int32_t const k = hP.fields[f].kind;
+      ok              = stats_kind_ok(st, k);
+      int32_t* slot   = k == FK_COUNT ? &J.f_count
+                        : k == FK_M2  ? &J.f_m2
+                        : (k == FK_SUMSQ_I64 || k == FK_SUMSQ_F64)                 ? &J.f_sumsq
+                        : (k == FK_SUM_I64 || k == FK_SUM_F64)                     ? &J.f_sum
+                        : (k == FK_MIN_S32 || k == FK_MIN_S64 || k == FK_MIN_F64) ? &J.f_min
+                                                                                   : &J.f_max;
+      if (ok) {
+        ok    = *slot < 0;
+        *slot = f;
+      }
+    }
+    if (!ok || (J.f_m2 >= 0 && (J.f_sum < 0 || hP.fields[J.f_m2].aux_sum != J.f_sum))) continue;
+    int32_t const kmin = st == ST_I32 ? FK_MIN_S32 : st == ST_I64 ? FK_MIN_S64 : FK_MIN_F64;
+    int32_t const kmax = st == ST_I32 ? FK_MAX_S32 : st == ST_I64 ? FK_MAX_S64 : FK_MAX_F64;
+    int32_t op, xf;
+    scan_domain(kmin, op, xf);
+    J.min_ident = int64_t(xf_fwd(xf, field_identity(kmin)));
+    scan_domain(kmax, op, xf);
+    J.max_ident = int64_t(xf_fwd(xf, field_identity(kmax)));
+    if (J.f_m2 >= 0 && m2_flag != nullptr) J.m2_flag = m2_flag + G * m2_jobs++;  // a flag array per column
+    jobs.push_back({st, J});
+  }
+  if (!jobs.empty() && hP.need_count_all) jobs.front().second.count_rows = 1;
+  return jobs;
+}
+
+sorted_stats_cover sorted_stats_plan(gb_params const& hP)
+{
+  sorted_stats_cover cover;
+  cover.field.assign(size_t(hP.num_fields), 0);
+  for (auto const& [st, J] : stats_jobs(hP, nullptr, 0)) {
+    for (int f = hP.val_field_begin[J.vcol]; f < hP.val_field_begin[J.vcol + 1]; f++) cover.field[size_t(f)] = 1;
+    cover.rows |= J.count_rows != 0;
+    cover.m2_columns += J.f_m2 >= 0 ? 1 : 0;
+  }
+  return cover;
+}
+
+void launch_sorted_stats(gb_params const* dP, gb_params const& hP, int64_t n, int64_t G, uint32_t const* gid,
+                         int num_cus, uint8_t* m2_flag, hipStream_t s)
+{
+  if (n <= 0) return;
+  for (auto const& [st, J] : stats_jobs(hP, m2_flag, G)) {
+    if (st == ST_I32) launch_stats_job<ST_I32>(dP, J, n, gid, num_cus, s);
+    else if (st == ST_I64) launch_stats_job<ST_I64>(dP, J, n, gid, num_cus, s);
+    else launch_stats_job<ST_F64>(dP, J, n, gid, num_cus, s);
+  }
+}
+
+void launch_sorted_stats_m2(gb_params const* dP, gb_params const& hP, int64_t n, int64_t G, uint32_t const* gid,
+                            int num_cus, uint8_t* m2_flag, hipStream_t s)
+{
+  if (n <= 0) return;
+  for (auto const& [st, J] : stats_jobs(hP, m2_flag, G)) {
+    if (J.f_m2 < 0) continue;
+    if (st == ST_I32) launch_stats_m2<ST_I32>(dP, J, n, gid, num_cus, s);
+    else if (st == ST_I64) launch_stats_m2<ST_I64>(dP, J, n, gid, num_cus, s);
+    else launch_stats_m2<ST_F64>(dP, J, n, gid, num_cus, s);
+  }
+}
+
 static void launch_sorted_passes(gb_params const* dP, gb_params const& hP, int64_t n, uint32_t const* gid,
-                                 int num_cus, bool m2, hipStream_t s)
+                                 int num_cus, bool m2, sorted_stats_cover const* cover, hipStream_t s)
 {
   std::vector<int32_t> items;
-  if (!m2 && hP.need_count_all) items.push_back(-1);
+  if (!m2 && hP.need_count_all && !(cover && cover->rows)) items.push_back(-1);
   for (int f = 0; f < hP.num_fields; f++) {
     int32_t const k = hP.fields[f].kind;
+    if (cover && cover->field[size_t(f)]) continue;  // launch_sorted_stats
     if (k == FK_NUNIQUE || k == FK_NUNIQUE_NULLS || k == FK_NTH) continue;  // kernels of their own
     if ((k == FK_ARGMIN || k == FK_ARGMAX) && hP.vals[hP.fields[f].vcol].storage == ST_STRING)
       continue;  // launch_sorted_string_arg
@@ -2065,15 +2521,15 @@ void launch_map_rows(int32_t* rows, int64_t G, uint32_t const* perm, hipStream_t
 }
 
 void launch_sorted_aggregate(gb_params const* dP, gb_params const& hP, int64_t n, uint32_t const* gid, int num_cus,
-                             hipStream_t s)
+                             sorted_stats_cover const* cover, hipStream_t s)
 {
-  launch_sorted_passes(dP, hP, n, gid, num_cus, false, s);
+  launch_sorted_passes(dP, hP, n, gid, num_cus, false, cover, s);
 }
 
 void launch_sorted_second_pass(gb_params const* dP, gb_params const& hP, int64_t n, uint32_t const* gid,
-                               int num_cus, hipStream_t s)
+                               int num_cus, sorted_stats_cover const* cover, hipStream_t s)
 {
-  launch_sorted_passes(dP, hP, n, gid, num_cus, true, s);
+  launch_sorted_passes(dP, hP, n, gid, num_cus, true, cover, s);
 }
 
 // NUNIQUE on runs.  The reference sorts each group's values and counts the

@@ -7,6 +7,8 @@
 
 #include "common.hpp"
 
+#include <vector>
+
 namespace mig {
 
 constexpr int MAX_KEYS   = 16;
@@ -352,11 +354,30 @@ void launch_sorted_count(gb_params const* dP, int64_t n, uint32_t* block_counts,
 // and, when group_last is not null, group_last[g] = its last row
 void launch_sorted_assign(gb_params const* dP, int64_t n, uint32_t const* block_offsets, uint32_t* gid,
                           uint32_t* group_last, hipStream_t s);
-// wave-range reductions with plain stores for groups inside one range
+// Fields (and the group row count) that launch_sorted_stats reduces: value
+// columns of int32 / int64 / float64 whose accumulators are all among SUM,
+// SUM_OF_SQUARES, COUNT, MIN, MAX and M2, each by a kernel compiled for the
+// column type.
+struct sorted_stats_cover {
+  std::vector<char> field;  // per field index: 1 = reduced by launch_sorted_stats
+  bool rows      = false;   // the group row count too
+  int m2_columns = 0;       // covered columns with M2 (each needs G bytes of per-group flags)
+};
+sorted_stats_cover sorted_stats_plan(gb_params const& hP);
+// The covered columns; M2 from the sums where exact (or within 1e-11 of the
+// result for float64), otherwise flagged in m2_flag (G bytes) for
+// launch_sorted_stats_m2, which runs after the sums and counts are final
+// (m2_flag: G bytes per covered column with M2, in column order).
+void launch_sorted_stats(gb_params const* dP, gb_params const& hP, int64_t n, int64_t G, uint32_t const* gid,
+                         int num_cus, uint8_t* m2_flag, hipStream_t s);
+void launch_sorted_stats_m2(gb_params const* dP, gb_params const& hP, int64_t n, int64_t G, uint32_t const* gid,
+                            int num_cus, uint8_t* m2_flag, hipStream_t s);
+// wave-range reductions with plain stores for groups inside one range (the
+// fields `cover` names are left to launch_sorted_stats)
 void launch_sorted_aggregate(gb_params const* dP, gb_params const& hP, int64_t n, uint32_t const* gid, int num_cus,
-                             hipStream_t s);
+                             sorted_stats_cover const* cover, hipStream_t s);
 void launch_sorted_second_pass(gb_params const* dP, gb_params const& hP, int64_t n, uint32_t const* gid,
-                               int num_cus, hipStream_t s);
+                               int num_cus, sorted_stats_cover const* cover, hipStream_t s);
 // NUNIQUE of field `f` on runs (`gid` = group of every row, NO_GROUP when
 // dropped; G groups): a set of (group, value) in `table` (2n u32 slots,
 // cleared here) counts each group's distinct values.

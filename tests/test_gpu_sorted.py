@@ -134,6 +134,40 @@ def test_sorted_c2_shape_properties():
     assert torch.equal(res.columns()[2].data.view(torch.int64)[:len(uk)], mx)
 
 
+STATS_AGGS = [(AGG["SUM"], 1), (AGG["MIN"], 1), (AGG["MAX"], 1), (AGG["COUNT_VALID"], 1), (AGG["COUNT_ALL"], 1),
+              (AGG["MEAN"], 1), (AGG["VARIANCE"], 1), (AGG["STD"], 0)]
+
+
+@pytest.mark.parametrize("policy", [0, 1])
+def test_sorted_column_statistics_m2_rules(policy):
+    """int32 / int64 / float64 columns whose aggregations all have the
+    type-specialised runs form (SUM, SUM_OF_SQUARES, MIN, MAX, COUNT, MEAN,
+    VARIANCE, STD), with the cases that decide where M2 comes from: groups of
+    1 to 3,000 rows (float64 groups above 256 values, and groups that
+    neighbouring waves share, take the exact two-pass M2), int64 values near
+    2^40 and int32 values over the whole range (Σx² at or above 2^53: exact
+    pass; SUM_OF_SQUARES wraps in 64 bits like the oracle), float64 values
+    1e9 + N(0, 1e-3) (ill-conditioned: exact pass), small values (M2 from the
+    sums), null values, a group without valid values, and null keys last."""
+    rng = np.random.default_rng(300 + policy)
+    sizes = np.concatenate([rng.integers(1, 20, 20_000), [300, 1000, 3000, 257, 256], rng.integers(1, 4, 5000)])
+    rng.shuffle(sizes)
+    k = np.repeat(np.arange(len(sizes), dtype=np.int64), sizes)
+    n = len(k)
+    kvalid = np.ones(n, bool)
+    kvalid[-300:] = False
+    i32 = np.where(k % 4 == 0, rng.integers(-2**31, 2**31, n), rng.integers(-10**4, 10**4, n)).astype(np.int32)
+    i64 = np.where(k % 3 == 0, rng.integers(-2**40, 2**40, n), rng.integers(-1000, 1000, n)).astype(np.int64)
+    f64 = np.where(k % 5 == 0, 1e9 + 1e-3 * rng.standard_normal(n), 100.0 * rng.standard_normal(n))
+    f64valid = rng.random(n) > 0.05
+    f64valid[k == 7] = False
+    reqs = [(host_col(i32, rng.random(n) > 0.1), STATS_AGGS + [(AGG["SUM_OF_SQUARES"], 1)]),
+            (host_col(i64, rng.random(n) > 0.1), STATS_AGGS),
+            (host_col(f64, f64valid), STATS_AGGS + [(AGG["SUM_OF_SQUARES"], 1)]),
+            (host_col(rng.integers(0, 50, n).astype(np.int64)), [(AGG["VARIANCE"], 1), (AGG["MEAN"], 1)])]
+    _check_sorted([host_col(k, kvalid)], reqs, policy)
+
+
 @pytest.mark.parametrize("vdt", [np.int32, np.int64, np.float32, np.float64])
 def test_sorted_product(vdt):
     """PRODUCT on runs (groups of ~20 rows): integers wrap in int64 exactly as
