@@ -1217,16 +1217,16 @@ int64_t groupby_call::aggregate_sorted_runs()
   launch_sorted_assign(dP, N, tile_counts, gid, group_last, stream_);
   T->mark(2);
   sorted_stats_cover const cover = sorted_stats_plan(p);
-  uint8_t* m2_flag = cover.m2_columns > 0 ? S->get<uint8_t>(size_t(std::max<int64_t>(G, 1)) * size_t(cover.m2_columns))
-                                           : nullptr;
-  launch_sorted_stats(dP, p, N, G, gid, num_cus, m2_flag, stream_);
+  uint8_t* m2_scratch =
+    cover.m2_columns > 0 ? S->get<uint8_t>(sorted_stats_scratch_bytes(G, cover.m2_columns)) : nullptr;
+  launch_sorted_stats(dP, p, N, G, gid, num_cus, m2_scratch, stream_);
   launch_sorted_aggregate(dP, p, N, gid, num_cus, &cover, stream_);
   launch_sorted_string_arg(dP, p, N, gid, stream_);
   T->mark(3);
   T->mark(4);
   if (p.need_second_pass) {
     launch_sorted_second_pass(dP, p, N, gid, num_cus, &cover, stream_);
-    launch_sorted_stats_m2(dP, p, N, G, gid, num_cus, m2_flag, stream_);
+    launch_sorted_stats_m2(dP, p, N, G, gid, num_cus, m2_scratch, stream_);
   }
   for (int f = 0; f < nf_; f++) {
     if (p.fields[f].kind != FK_NUNIQUE && p.fields[f].kind != FK_NUNIQUE_NULLS) continue;

@@ -1966,10 +1966,12 @@ static void sorted_ranges(int64_t n, int num_cus, int64_t& per, int64_t& W)
 // steps are computed once per chunk for all accumulators, and the partials
 // carried between chunks stay in registers.  M2 comes from the sums in the
 // same pass where that is exact or within the bound below; the groups where
-// it is not (and the groups a wave shares with its neighbours) are flagged
-// and get the exact two-pass M2 (the reference's mean, then squared
-// deviations: sort/group_m2.cu, sort/aggregate.cpp:286-306) in
-// k_sorted_stats_m2, which only touches the rows of flagged groups.
+// it is not (and the groups a wave shares with its neighbours) get the exact
+// two-pass M2 (the reference's mean, then squared deviations:
+// sort/group_m2.cu, sort/aggregate.cpp:286-306): groups of at most
+// M2_WAVE_ROWS rows are listed and reduced by one wave each
+// (k_sorted_stats_m2_list), longer ones are flagged for the row-parallel
+// k_sorted_stats_m2, which returns at once when none is.
 template <int ST>
 struct stats_traits;
 template <>
@@ -2107,8 +2109,14 @@ struct sorted_stats_job {
   int32_t f_sumsq;
   int32_t count_rows;                          // also the group row count
   int64_t min_ident, max_ident;                // identities in the MIN / MAX domain
-  uint8_t* m2_flag;                            // per group: 1 = M2 by the exact pass
+  uint8_t* m2_flag;                            // per group: 1 = M2 by the row-parallel exact pass
+  uint32_t* m2_list;                           // groups whose M2 one wave each recomputes
+  uint32_t* m2_counts;                         // [0] listed groups, [1] flagged groups
 };
+
+// Longest group whose exact M2 one wave recomputes; longer groups take the
+// row-parallel pass.
+constexpr int64_t M2_WAVE_ROWS = 4096;
 
 template <int ST>
 struct stats_part {
@@ -2121,9 +2129,11 @@ struct stats_part {
 
 // A group's partials: stored when the group lies inside one wave's range,
 // combined atomically when neighbouring ranges hold rows of it.
+// `ends`: no later row of the group follows (this is the group's last flush),
+// and `last_row` is then its last row.
 template <int ST>
 __device__ __forceinline__ void stats_flush(gb_params const& P, sorted_stats_job const& J, uint32_t g,
-                                            stats_part<ST> const& a, bool shared)
+                                            stats_part<ST> const& a, bool shared, bool ends, int64_t last_row)
 {
   using Sum = typename stats_traits<ST>::Sum;
   if (J.count_rows) {
@@ -2176,10 +2186,17 @@ __device__ __forceinline__ void stats_flush(gb_params const& P, sorted_stats_job
     }
   }
   if (J.f_m2 >= 0) {
-    double m2  = 0.0;
-    bool exact = !shared && stats_m2<ST>(a.cnt, a.sum, a.sq, m2);
-    if (exact && m2 != 0.0) *gp(reinterpret_cast<double*>(field_at(P.fields[J.f_m2], g))) = m2;
-    *gp(J.m2_flag + g) = exact ? 0 : 1;
+    double m2        = 0.0;
+    bool const exact = !shared && stats_m2<ST>(a.cnt, a.sum, a.sq, m2);  // a group not shared ends here
+    if (exact) {
+      if (m2 != 0.0) *gp(reinterpret_cast<double*>(field_at(P.fields[J.f_m2], g))) = m2;
+      *gp(J.m2_flag + g) = 0;
+    } else if (ends) {  // once per group: its last flush
+      bool const large   = last_row - int64_t(gp(P.slot_rows)[g]) + 1 > M2_WAVE_ROWS;
+      *gp(J.m2_flag + g) = large ? 1 : 0;
+      if (large) aadd(J.m2_counts + 1, 1u);
+      else *gp(J.m2_list + __hip_atomic_fetch_add(J.m2_counts, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) = g;
+    }
   }
 }
 
@@ -2228,7 +2245,7 @@ __global__ void __launch_bounds__(BLOCK)
     uint32_t const gn = uint32_t(__builtin_amdgcn_update_dpp(int(NO_GROUP), int(g), 0x130, 0xF, 0xF, false));
     uint32_t const g0 = uint32_t(__builtin_amdgcn_readlane(int(g), 0));
     if (carry_g != NO_GROUP && g0 != carry_g) {
-      if (lane == 0) stats_flush<ST>(P, J, carry_g, cy, carry_g == first_g);
+      if (lane == 0) stats_flush<ST>(P, J, carry_g, cy, carry_g == first_g, true, i0 - 1);
     } else if (carry_g != NO_GROUP && lane == 0) {  // the carried group continues at lane 0
       a.rows += cy.rows;
       a.cnt += cy.cnt;
@@ -2247,7 +2264,7 @@ __global__ void __launch_bounds__(BLOCK)
     if (J.f_max >= 0) a.mx = mscan(a.mx, m, [](int64_t p, int64_t q) { return p > q ? p : q; });
     if (want_sq) a.sq = mscan(a.sq, m, [](double p, double q) { return p + q; });
     if (want_isq) a.isq = mscan(a.isq, m, [](uint64_t p, uint64_t q) { return p + q; });
-    if (g != NO_GROUP && lane < 63 && gn != g) stats_flush<ST>(P, J, g, a, g == first_g);
+    if (g != NO_GROUP && lane < 63 && gn != g) stats_flush<ST>(P, J, g, a, g == first_g, true, row);
     cy.rows = lane63(a.rows);
     cy.cnt  = lane63(a.cnt);
     cy.sum  = lane63(a.sum);
@@ -2257,19 +2274,24 @@ __global__ void __launch_bounds__(BLOCK)
     cy.isq  = lane63(a.isq);
     carry_g = uint32_t(__builtin_amdgcn_readlane(int(g), 63));
   }
-  if (carry_g != NO_GROUP && lane == 0) stats_flush<ST>(P, J, carry_g, cy, true);  // may continue in the next range
+  if (carry_g != NO_GROUP && lane == 0) {  // may continue in the next range
+    bool const ends = e == P.num_rows || gid[e] != carry_g;
+    stats_flush<ST>(P, J, carry_g, cy, true, ends, e - 1);
+  }
 }
 
-// The exact M2 of the flagged groups: Σ(x − mean)² with the mean from the
-// finished sum and count, summed over each run of a chunk and added to the
-// group's M2 once per chunk.  Chunks without a flagged row only read their
-// group numbers and flags.
+// The exact M2 of the flagged (long) groups: Σ(x − mean)² with the mean from
+// the finished sum and count, summed over each run of a chunk and added to
+// the group's M2 once per chunk.  Chunks without a flagged row only read
+// their group numbers and flags; with no flagged group every wave returns
+// after reading the count.
 template <int ST>
 __global__ void __launch_bounds__(BLOCK)
   k_sorted_stats_m2(gb_params const* __restrict__ Pp, uint32_t const* __restrict__ gid, sorted_stats_job J)
 {
   using T            = typename stats_traits<ST>::T;
   gb_params const& P = *Pp;
+  if (aload(J.m2_counts + 1) == 0) return;
   int64_t const n    = P.num_rows;
   int const lane     = threadIdx.x & 63;
   dcol const c       = P.vals[J.vcol];
@@ -2296,6 +2318,48 @@ __global__ void __launch_bounds__(BLOCK)
   }
 }
 
+// The exact M2 of the listed groups (at most M2_WAVE_ROWS rows each): one
+// wave per group walks its rows from the first (the runs are contiguous),
+// sums the squared deviations in its lanes, reduces them across the wave and
+// stores the group's M2 (the only writer).
+template <int ST>
+__global__ void __launch_bounds__(BLOCK)
+  k_sorted_stats_m2_list(gb_params const* __restrict__ Pp, uint32_t const* __restrict__ gid, sorted_stats_job J)
+{
+  using T             = typename stats_traits<ST>::T;
+  gb_params const& P  = *Pp;
+  int64_t const n     = P.num_rows;
+  int const lane      = threadIdx.x & 63;
+  dcol const c        = P.vals[J.vcol];
+  auto const* data    = reinterpret_cast<gmem<T const>*>(gp(c.data));
+  field_desc const& F = P.fields[J.f_m2];
+  uint32_t const listed = aload(J.m2_counts);
+  for (uint32_t i = blockIdx.x * (BLOCK / 64) + (threadIdx.x >> 6); i < listed; i += gridDim.x * (BLOCK / 64)) {
+    uint32_t const g    = gp(J.m2_list)[i];
+    int64_t const first = int64_t(gp(P.slot_rows)[g]);
+    double const cnt    = double(count_of(P, F.aux_count, g));
+    double const mean   = field_as_double(P.fields[F.aux_sum], g) / cnt;
+    double acc          = 0.0;
+    for (int64_t r0 = first; r0 < n; r0 += 64) {
+      int64_t const row = r0 + lane;
+      bool const in     = row < n && gid[row] == g;
+      if (in && dcol_valid(c, row)) {
+        double const d = double(data[row]) - mean;
+        acc += d * d;
+      }
+      if (__builtin_amdgcn_ballot_w64(in) != ~0ull) break;  // the run ended in this chunk
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+    if (lane == 0) *gp(reinterpret_cast<double*>(field_at(F, g))) = acc;
+  }
+}
+
+__global__ void k_zero_u32(uint32_t* p, int n)
+{
+  if (int(threadIdx.x) < n) p[threadIdx.x] = 0;
+}
+
 template <int ST>
 static void launch_stats_job(gb_params const* dP, sorted_stats_job const& J, int64_t n, uint32_t const* gid,
                              int num_cus, hipStream_t s)
@@ -2310,6 +2374,7 @@ template <int ST>
 static void launch_stats_m2(gb_params const* dP, sorted_stats_job const& J, int64_t n, uint32_t const* gid,
                             int num_cus, hipStream_t s)
 {
+  hipLaunchKernelGGL(k_sorted_stats_m2_list<ST>, dim3(unsigned(num_cus * 8)), dim3(BLOCK), 0, s, dP, gid, J);
   hipLaunchKernelGGL(k_sorted_stats_m2<ST>, dim3(unsigned(row_grid(n, num_cus, 8))), dim3(BLOCK), 0, s, dP, gid, J);
 }
 
@@ -2335,7 +2400,17 @@ static bool stats_kind_ok(int32_t st, int32_t kind)
 // The column jobs of a request: every value column whose fields all have a
 // specialised form.  The first job also counts the group rows when the
 // request needs them.
-static std::vector<std::pair<int32_t, sorted_stats_job>> stats_jobs(gb_params const& hP, uint8_t* m2_flag,
+// Device scratch of one column with M2: per-group flags, the list of groups
+// and two counters.
+static size_t m2_column_bytes(int64_t G)
+{
+  size_t const g = size_t(G > 0 ? G : 1);
+  return ((g + 255) & ~size_t(255)) + ((4 * g + 255) & ~size_t(255)) + 256;
+}
+
+size_t sorted_stats_scratch_bytes(int64_t G, int m2_columns) { return m2_column_bytes(G) * size_t(m2_columns); }
+
+static std::vector<std::pair<int32_t, sorted_stats_job>> stats_jobs(gb_params const& hP, uint8_t* m2_scratch,
                                                                    int64_t G)
 {
   std::vector<std::pair<int32_t, sorted_stats_job>> jobs;
@@ -2370,7 +2445,13 @@ static std::vector<std::pair<int32_t, sorted_stats_job>> stats_jobs(gb_params co
     J.min_ident = int64_t(xf_fwd(xf, field_identity(kmin)));
     scan_domain(kmax, op, xf);
     J.max_ident = int64_t(xf_fwd(xf, field_identity(kmax)));
-    if (J.f_m2 >= 0 && m2_flag != nullptr) J.m2_flag = m2_flag + G * m2_jobs++;  // a flag array per column
+    if (J.f_m2 >= 0 && m2_scratch != nullptr) {  // scratch of its own per column
+      size_t const g  = size_t(G > 0 ? G : 1);
+      uint8_t* base   = m2_scratch + m2_column_bytes(G) * size_t(m2_jobs++);
+      J.m2_flag       = base;
+      J.m2_list       = reinterpret_cast<uint32_t*>(base + ((g + 255) & ~size_t(255)));
+      J.m2_counts     = reinterpret_cast<uint32_t*>(base + ((g + 255) & ~size_t(255)) + ((4 * g + 255) & ~size_t(255)));
+    }
     jobs.push_back({st, J});
   }
   if (!jobs.empty() && hP.need_count_all) jobs.front().second.count_rows = 1;
@@ -2390,10 +2471,11 @@ sorted_stats_cover sorted_stats_plan(gb_params const& hP)
 }
 
 void launch_sorted_stats(gb_params const* dP, gb_params const& hP, int64_t n, int64_t G, uint32_t const* gid,
-                         int num_cus, uint8_t* m2_flag, hipStream_t s)
+                         int num_cus, uint8_t* m2_scratch, hipStream_t s)
 {
   if (n <= 0) return;
-  for (auto const& [st, J] : stats_jobs(hP, m2_flag, G)) {
+  for (auto const& [st, J] : stats_jobs(hP, m2_scratch, G)) {
+    if (J.f_m2 >= 0) hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, J.m2_counts, 2);
     if (st == ST_I32) launch_stats_job<ST_I32>(dP, J, n, gid, num_cus, s);
     else if (st == ST_I64) launch_stats_job<ST_I64>(dP, J, n, gid, num_cus, s);
     else launch_stats_job<ST_F64>(dP, J, n, gid, num_cus, s);
@@ -2401,10 +2483,10 @@ void launch_sorted_stats(gb_params const* dP, gb_params const& hP, int64_t n, in
 }
 
 void launch_sorted_stats_m2(gb_params const* dP, gb_params const& hP, int64_t n, int64_t G, uint32_t const* gid,
-                            int num_cus, uint8_t* m2_flag, hipStream_t s)
+                            int num_cus, uint8_t* m2_scratch, hipStream_t s)
 {
   if (n <= 0) return;
-  for (auto const& [st, J] : stats_jobs(hP, m2_flag, G)) {
+  for (auto const& [st, J] : stats_jobs(hP, m2_scratch, G)) {
     if (J.f_m2 < 0) continue;
     if (st == ST_I32) launch_stats_m2<ST_I32>(dP, J, n, gid, num_cus, s);
     else if (st == ST_I64) launch_stats_m2<ST_I64>(dP, J, n, gid, num_cus, s);

@@ -361,17 +361,18 @@ void launch_sorted_assign(gb_params const* dP, int64_t n, uint32_t const* block_
 struct sorted_stats_cover {
   std::vector<char> field;  // per field index: 1 = reduced by launch_sorted_stats
   bool rows      = false;   // the group row count too
-  int m2_columns = 0;       // covered columns with M2 (each needs G bytes of per-group flags)
+  int m2_columns = 0;       // covered columns with M2 (each needs its per-group scratch)
 };
 sorted_stats_cover sorted_stats_plan(gb_params const& hP);
 // The covered columns; M2 from the sums where exact (or within 1e-11 of the
 // result for float64), otherwise flagged in m2_flag (G bytes) for
 // launch_sorted_stats_m2, which runs after the sums and counts are final
-// (m2_flag: G bytes per covered column with M2, in column order).
+// (m2_scratch: sorted_stats_scratch_bytes of device memory).
+size_t sorted_stats_scratch_bytes(int64_t G, int m2_columns);
 void launch_sorted_stats(gb_params const* dP, gb_params const& hP, int64_t n, int64_t G, uint32_t const* gid,
-                         int num_cus, uint8_t* m2_flag, hipStream_t s);
+                         int num_cus, uint8_t* m2_scratch, hipStream_t s);
 void launch_sorted_stats_m2(gb_params const* dP, gb_params const& hP, int64_t n, int64_t G, uint32_t const* gid,
-                            int num_cus, uint8_t* m2_flag, hipStream_t s);
+                            int num_cus, uint8_t* m2_scratch, hipStream_t s);
 // wave-range reductions with plain stores for groups inside one range (the
 // fields `cover` names are left to launch_sorted_stats)
 void launch_sorted_aggregate(gb_params const* dP, gb_params const& hP, int64_t n, uint32_t const* gid, int num_cus,

@@ -143,14 +143,16 @@ def test_sorted_column_statistics_m2_rules(policy):
     """int32 / int64 / float64 columns whose aggregations all have the
     type-specialised runs form (SUM, SUM_OF_SQUARES, MIN, MAX, COUNT, MEAN,
     VARIANCE, STD), with the cases that decide where M2 comes from: groups of
-    1 to 3,000 rows (float64 groups above 256 values, and groups that
-    neighbouring waves share, take the exact two-pass M2), int64 values near
+    1 to 9,000 rows (float64 groups above 256 values, and groups that
+    neighbouring waves share, take the exact two-pass M2: one wave per group
+    up to 4,096 rows, the row-parallel pass above), int64 values near
     2^40 and int32 values over the whole range (Σx² at or above 2^53: exact
     pass; SUM_OF_SQUARES wraps in 64 bits like the oracle), float64 values
     1e9 + N(0, 1e-3) (ill-conditioned: exact pass), small values (M2 from the
     sums), null values, a group without valid values, and null keys last."""
     rng = np.random.default_rng(300 + policy)
-    sizes = np.concatenate([rng.integers(1, 20, 20_000), [300, 1000, 3000, 257, 256], rng.integers(1, 4, 5000)])
+    sizes = np.concatenate([rng.integers(1, 20, 20_000), [300, 1000, 3000, 257, 256, 4096, 4097, 9000],
+                            rng.integers(1, 4, 5000)])
     rng.shuffle(sizes)
     k = np.repeat(np.arange(len(sizes), dtype=np.int64), sizes)
     n = len(k)
