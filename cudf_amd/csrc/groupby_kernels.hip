@@ -1978,34 +1978,36 @@ template <>
 struct stats_traits<ST_I32> {
   using T   = int32_t;
   using Sum = int64_t;
+  using Ord = int32_t;  // MIN / MAX domain
 };
 template <>
 struct stats_traits<ST_I64> {
   using T   = int64_t;
   using Sum = int64_t;
+  using Ord = int64_t;
 };
 template <>
 struct stats_traits<ST_F64> {
   using T   = double;
   using Sum = double;
+  using Ord = int64_t;
 };
 
-// Value of a row in the signed 64-bit MIN / MAX domain (floats: the order
-// code with the sign bit flipped, as scan_domain maps them).
+// Value of a row in the signed MIN / MAX domain (floats: the order code with
+// the sign bit flipped, as scan_domain maps them; integers: themselves).
 template <int ST>
-__device__ __forceinline__ int64_t stats_order(typename stats_traits<ST>::T x)
+__device__ __forceinline__ typename stats_traits<ST>::Ord stats_order(typename stats_traits<ST>::T x)
 {
   if constexpr (ST == ST_F64) return int64_t(ord64(uint64_t(__double_as_longlong(x))) ^ 0x8000000000000000ull);
-  else return int64_t(x);
+  else return x;
 }
 
-// The accumulator encoding of a MIN / MAX domain value.
+// The accumulator value of a MIN / MAX domain value.
 template <int ST>
-__device__ __forceinline__ uint64_t stats_order_inv(int64_t t)
+__device__ __forceinline__ typename stats_traits<ST>::Ord stats_order_inv(typename stats_traits<ST>::Ord t)
 {
-  if constexpr (ST == ST_F64) return uint64_t(t) ^ 0x8000000000000000ull;
-  else if constexpr (ST == ST_I32) return uint64_t(uint32_t(int32_t(t)));
-  else return uint64_t(t);
+  if constexpr (ST == ST_F64) return int64_t(uint64_t(t) ^ 0x8000000000000000ull);
+  else return t;
 }
 
 // Same-group masks of the six scan steps (see seg_scan): lane l combines the
@@ -2097,6 +2099,11 @@ __device__ __forceinline__ bool stats_m2(uint32_t n, typename stats_traits<ST>::
     return true;
   } else {
     if (!(s2 < 9007199254740992.0)) return false;
+    if (double(n) * s2 < 9.2e18) {  // n·s2 < 2^63, and s1² <= n·s2: 64 bits hold both
+      int64_t const num = int64_t(n) * int64_t(s2) - s1 * s1;
+      m2                = double(num) / double(n);
+      return true;
+    }
     __int128 const num = __int128(n) * __int128(uint64_t(s2)) - __int128(s1) * __int128(s1);
     m2                 = double(num) / double(n);
     return true;
@@ -2109,6 +2116,14 @@ struct sorted_stats_job {
   int32_t f_sumsq;
   int32_t count_rows;                          // also the group row count
   int64_t min_ident, max_ident;                // identities in the MIN / MAX domain
+  // the accumulators of the fields above (arrays of G entries)
+  uint32_t* p_rows;
+  uint32_t* p_count;
+  void* p_sum;
+  void* p_min;
+  void* p_max;
+  void* p_sumsq;
+  double* p_m2;
   uint8_t* m2_flag;                            // per group: 1 = M2 by the row-parallel exact pass
   uint32_t* m2_list;                           // groups whose M2 one wave each recomputes
   uint32_t* m2_counts;                         // [0] listed groups, [1] flagged groups
@@ -2122,7 +2137,7 @@ template <int ST>
 struct stats_part {
   uint32_t rows, cnt;
   typename stats_traits<ST>::Sum sum;
-  int64_t mn, mx;
+  typename stats_traits<ST>::Ord mn, mx;
   double sq;     // Σx² in double (M2; SUM_OF_SQUARES of float64)
   uint64_t isq;  // Σx² of integers, wrapping (SUM_OF_SQUARES of int32 / int64)
 };
@@ -2136,60 +2151,56 @@ __device__ __forceinline__ void stats_flush(gb_params const& P, sorted_stats_job
                                             stats_part<ST> const& a, bool shared, bool ends, int64_t last_row)
 {
   using Sum = typename stats_traits<ST>::Sum;
+  using Ord = typename stats_traits<ST>::Ord;
   if (J.count_rows) {
-    if (shared) aadd(count_at(P, g), a.rows);
-    else *gp(count_at(P, g)) = a.rows;
+    if (shared) aadd(J.p_rows + g, a.rows);
+    else *gp(J.p_rows + g) = a.rows;
   }
   if (J.f_count >= 0) {
-    auto* p = reinterpret_cast<uint32_t*>(field_at(P.fields[J.f_count], g));
-    if (shared) aadd(p, a.cnt);
-    else *gp(p) = a.cnt;
+    if (shared) aadd(J.p_count + g, a.cnt);
+    else *gp(J.p_count + g) = a.cnt;
   }
   if (J.f_sum >= 0) {
-    auto* p = field_at(P.fields[J.f_sum], g);
+    Sum* p = static_cast<Sum*>(J.p_sum) + g;
     if constexpr (ST == ST_F64) {
-      if (shared) gadd_f64(reinterpret_cast<double*>(p), a.sum);
-      else *gp(reinterpret_cast<double*>(p)) = a.sum;
+      if (shared) gadd_f64(p, a.sum);
+      else *gp(p) = a.sum;
     } else {
       if (shared) aadd(reinterpret_cast<uint64_t*>(p), uint64_t(a.sum));
-      else *gp(reinterpret_cast<Sum*>(p)) = a.sum;
+      else *gp(p) = a.sum;
     }
   }
-  auto minmax = [&](int32_t f, int64_t t, bool is_min) {
-    if (f < 0) return;
-    auto* p          = field_at(P.fields[f], g);
-    uint64_t const u = stats_order_inv<ST>(t);
-    if constexpr (ST == ST_I32) {
-      if (!shared) *gp(reinterpret_cast<int32_t*>(p)) = int32_t(uint32_t(u));
-      else if (is_min) amin(reinterpret_cast<int32_t*>(p), int32_t(uint32_t(u)));
-      else amax(reinterpret_cast<int32_t*>(p), int32_t(uint32_t(u)));
-    } else if constexpr (ST == ST_I64) {
-      if (!shared) *gp(reinterpret_cast<int64_t*>(p)) = int64_t(u);
-      else if (is_min) amin(reinterpret_cast<int64_t*>(p), int64_t(u));
-      else amax(reinterpret_cast<int64_t*>(p), int64_t(u));
-    } else {
-      if (!shared) *gp(reinterpret_cast<uint64_t*>(p)) = u;
-      else if (is_min) amin(reinterpret_cast<uint64_t*>(p), u);
-      else amax(reinterpret_cast<uint64_t*>(p), u);
-    }
-  };
-  minmax(J.f_min, a.mn, true);
-  minmax(J.f_max, a.mx, false);
+  // MIN / MAX accumulators: int32 / int64 values, float order codes (unsigned)
+  if (J.f_min >= 0) {
+    Ord const u = stats_order_inv<ST>(a.mn);
+    Ord* p      = static_cast<Ord*>(J.p_min) + g;
+    if (!shared) *gp(p) = u;
+    else if constexpr (ST == ST_F64) amin(reinterpret_cast<uint64_t*>(p), uint64_t(u));
+    else amin(p, u);
+  }
+  if (J.f_max >= 0) {
+    Ord const u = stats_order_inv<ST>(a.mx);
+    Ord* p      = static_cast<Ord*>(J.p_max) + g;
+    if (!shared) *gp(p) = u;
+    else if constexpr (ST == ST_F64) amax(reinterpret_cast<uint64_t*>(p), uint64_t(u));
+    else amax(p, u);
+  }
   if (J.f_sumsq >= 0) {
-    auto* p = field_at(P.fields[J.f_sumsq], g);
     if constexpr (ST == ST_F64) {
-      if (shared) gadd_f64(reinterpret_cast<double*>(p), a.sq);
-      else *gp(reinterpret_cast<double*>(p)) = a.sq;
+      double* p = static_cast<double*>(J.p_sumsq) + g;
+      if (shared) gadd_f64(p, a.sq);
+      else *gp(p) = a.sq;
     } else {
-      if (shared) aadd(reinterpret_cast<uint64_t*>(p), a.isq);
-      else *gp(reinterpret_cast<uint64_t*>(p)) = a.isq;
+      uint64_t* p = static_cast<uint64_t*>(J.p_sumsq) + g;
+      if (shared) aadd(p, a.isq);
+      else *gp(p) = a.isq;
     }
   }
   if (J.f_m2 >= 0) {
     double m2        = 0.0;
     bool const exact = !shared && stats_m2<ST>(a.cnt, a.sum, a.sq, m2);  // a group not shared ends here
     if (exact) {
-      if (m2 != 0.0) *gp(reinterpret_cast<double*>(field_at(P.fields[J.f_m2], g))) = m2;
+      if (m2 != 0.0) *gp(J.p_m2 + g) = m2;
       *gp(J.m2_flag + g) = 0;
     } else if (ends) {  // once per group: its last flush
       bool const large   = last_row - int64_t(gp(P.slot_rows)[g]) + 1 > M2_WAVE_ROWS;
@@ -2209,6 +2220,7 @@ __global__ void __launch_bounds__(BLOCK)
 {
   using T            = typename stats_traits<ST>::T;
   using Sum          = typename stats_traits<ST>::Sum;
+  using Ord          = typename stats_traits<ST>::Ord;
   gb_params const& P = *Pp;
   int const lane     = threadIdx.x & 63;
   int64_t const w    = int64_t(blockIdx.x) * (BLOCK / 64) + (threadIdx.x >> 6);
@@ -2221,7 +2233,9 @@ __global__ void __launch_bounds__(BLOCK)
   bool const want_isq = ST != ST_F64 && J.f_sumsq >= 0;
   uint32_t const first_g = gid[b];  // may continue from the previous range
   uint32_t carry_g       = NO_GROUP;
-  stats_part<ST> cy{0, 0, Sum(0), J.min_ident, J.max_ident, 0.0, 0};
+  Ord const min_ident    = Ord(J.min_ident);
+  Ord const max_ident    = Ord(J.max_ident);
+  stats_part<ST> cy{0, 0, Sum(0), min_ident, max_ident, 0.0, 0};
   for (int64_t i0 = b; i0 < e; i0 += 64) {
     int64_t const row = i0 + lane;
     bool const live   = row < e;
@@ -2239,9 +2253,9 @@ __global__ void __launch_bounds__(BLOCK)
     a.isq  = has && want_isq ? uint64_t(int64_t(x)) * uint64_t(int64_t(x)) : 0;
     bool ordered = has;
     if constexpr (ST == ST_F64) ordered = has && !isnan(x);  // NaN takes no part in MIN / MAX
-    int64_t const t = stats_order<ST>(x);
-    a.mn            = ordered ? t : J.min_ident;
-    a.mx            = ordered ? t : J.max_ident;
+    Ord const t = stats_order<ST>(x);
+    a.mn        = ordered ? t : min_ident;
+    a.mx        = ordered ? t : max_ident;
     uint32_t const gn = uint32_t(__builtin_amdgcn_update_dpp(int(NO_GROUP), int(g), 0x130, 0xF, 0xF, false));
     uint32_t const g0 = uint32_t(__builtin_amdgcn_readlane(int(g), 0));
     if (carry_g != NO_GROUP && g0 != carry_g) {
@@ -2260,8 +2274,8 @@ __global__ void __launch_bounds__(BLOCK)
     if (J.count_rows) a.rows = mscan(a.rows, m, add32);
     a.cnt = mscan(a.cnt, m, add32);
     a.sum = mscan(a.sum, m, [](Sum p, Sum q) { return p + q; });
-    if (J.f_min >= 0) a.mn = mscan(a.mn, m, [](int64_t p, int64_t q) { return p < q ? p : q; });
-    if (J.f_max >= 0) a.mx = mscan(a.mx, m, [](int64_t p, int64_t q) { return p > q ? p : q; });
+    if (J.f_min >= 0) a.mn = mscan(a.mn, m, [](Ord p, Ord q) { return p < q ? p : q; });
+    if (J.f_max >= 0) a.mx = mscan(a.mx, m, [](Ord p, Ord q) { return p > q ? p : q; });
     if (want_sq) a.sq = mscan(a.sq, m, [](double p, double q) { return p + q; });
     if (want_isq) a.isq = mscan(a.isq, m, [](uint64_t p, uint64_t q) { return p + q; });
     if (g != NO_GROUP && lane < 63 && gn != g) stats_flush<ST>(P, J, g, a, g == first_g, true, row);
@@ -2423,9 +2437,11 @@ static std::vector<std::pair<int32_t, sorted_stats_job>> stats_jobs(gb_params co
     J.vcol = j;
     J.f_sum = J.f_count = J.f_min = J.f_max = J.f_m2 = J.f_sumsq = -1;
     bool ok = true;
+    // the kernels index accumulators as dense arrays
+    if (hP.need_count_all && hP.count_stride != 4) continue;
     for (int f = fb; f < fe && ok; f++) {
       int32_t const k = hP.fields[f].kind;
-      ok              = stats_kind_ok(st, k);
+      ok              = stats_kind_ok(st, k) && hP.fields[f].stride == hP.fields[f].width;
       int32_t* slot   = k == FK_COUNT ? &J.f_count
                         : k == FK_M2  ? &J.f_m2
                         : (k == FK_SUMSQ_I64 || k == FK_SUMSQ_F64)                 ? &J.f_sumsq
@@ -2438,6 +2454,14 @@ static std::vector<std::pair<int32_t, sorted_stats_job>> stats_jobs(gb_params co
       }
     }
     if (!ok || (J.f_m2 >= 0 && (J.f_sum < 0 || hP.fields[J.f_m2].aux_sum != J.f_sum))) continue;
+    auto ptr = [&](int32_t f) { return f >= 0 ? hP.fields[f].ptr : nullptr; };
+    J.p_rows  = hP.slot_count;
+    J.p_count = static_cast<uint32_t*>(ptr(J.f_count));
+    J.p_sum   = ptr(J.f_sum);
+    J.p_min   = ptr(J.f_min);
+    J.p_max   = ptr(J.f_max);
+    J.p_sumsq = ptr(J.f_sumsq);
+    J.p_m2    = static_cast<double*>(ptr(J.f_m2));
     int32_t const kmin = st == ST_I32 ? FK_MIN_S32 : st == ST_I64 ? FK_MIN_S64 : FK_MIN_F64;
     int32_t const kmax = st == ST_I32 ? FK_MAX_S32 : st == ST_I64 ? FK_MAX_S64 : FK_MAX_F64;
     int32_t op, xf;
