@@ -1301,46 +1301,65 @@ __global__ void __launch_bounds__(BLOCK)
   k_sorted_assign(gb_params const* __restrict__ Pp, uint32_t const* __restrict__ block_offsets,
                   uint32_t* __restrict__ gid, uint32_t* __restrict__ group_last)
 {
-  __shared__ uint32_t wave_tot[BLOCK / 64];
+  constexpr int NW = BLOCK / 64;
+  __shared__ uint32_t wave_tot[NW];
+  __shared__ uint64_t head_bal[NW], incl_bal[NW];
   gb_params const& P = *Pp;
   int const wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   uint64_t const below = (1ull << lane) - 1ull;
   int64_t const base   = int64_t(blockIdx.x) * SORTED_TILE;
   uint32_t running     = block_offsets[blockIdx.x];  // heads before this iteration's rows
+  // the block's last thread: its row's group when that row is included and
+  // whether the group ends there is decided by the next iteration's first row
+  uint32_t pending_g = NO_GROUP;
+  uint32_t pending_row = 0;
   for (int j = 0; j < SORTED_ROWS; j++) {
     int64_t const row = base + int64_t(j) * BLOCK + threadIdx.x;
     bool const live   = row < P.num_rows;
     bool const incl   = live && sorted_included(P, row);
     bool const head   = incl && sorted_head(P, row);
-    uint64_t const bal = __ballot(head);
-    if (lane == 0) wave_tot[wave] = uint32_t(__popcll(bal));
+    uint64_t const bal  = __ballot(head);
+    uint64_t const ibal = __ballot(incl);
+    if (lane == 0) {
+      wave_tot[wave] = uint32_t(__popcll(bal));
+      head_bal[wave] = bal;
+      incl_bal[wave] = ibal;
+    }
     __syncthreads();
     uint32_t before = 0, all = 0;
-    for (int w = 0; w < BLOCK / 64; w++) {
+    for (int w = 0; w < NW; w++) {
       if (w < wave) before += wave_tot[w];
       all += wave_tot[w];
     }
     uint32_t const heads_before = running + before + uint32_t(__popcll(bal & below));
+    uint32_t const g            = !incl ? NO_GROUP : (head ? heads_before : heads_before - 1u);
     // A group's rows are one run with no dropped row inside (a dropped row
     // ends a run), so its last row is the included row whose successor is
-    // dropped, a head or past the end.  Lanes 0..62 read the successor's
-    // flags from the next lane; lane 63's successor is in another wave.
-    bool ends = false;
+    // dropped, a head or past the end.  The successor's flags come from the
+    // wave's ballots, the next wave's (LDS), or for the block's last row the
+    // next iteration's first row.
     if (group_last != nullptr) {
-      int const next_incl = __shfl_down(int(incl), 1);
-      int const next_head = __shfl_down(int(head), 1);
+      if (pending_g != NO_GROUP && ((incl_bal[0] & 1u) == 0 || (head_bal[0] & 1u) != 0))
+        group_last[pending_g] = pending_row;  // the previous iteration's last row (block's last thread only)
+      pending_g = NO_GROUP;
       if (incl) {
-        int64_t const nr = row + 1;
-        bool const ni    = lane < 63 ? next_incl != 0 : nr < P.num_rows && sorted_included(P, nr);
-        bool const nh    = lane < 63 ? next_head != 0 : ni && sorted_head(P, nr);
-        ends             = !ni || nh;
+        bool ends = false;
+        if (lane < 63) ends = ((ibal >> (lane + 1)) & 1u) == 0 || ((bal >> (lane + 1)) & 1u) != 0;
+        else if (wave < NW - 1) ends = (incl_bal[wave + 1] & 1u) == 0 || (head_bal[wave + 1] & 1u) != 0;
+        else if (j + 1 < SORTED_ROWS && row + 1 < P.num_rows) {
+          pending_g   = g;
+          pending_row = uint32_t(row);
+        } else {  // the tile's last row: its successor starts the next tile
+          int64_t const nr = row + 1;
+          bool const ni    = nr < P.num_rows && sorted_included(P, nr);
+          ends             = !ni || sorted_head(P, nr);
+        }
+        if (ends) group_last[g] = uint32_t(row);
       }
     }
     if (live) {
-      uint32_t const g = !incl ? NO_GROUP : (head ? heads_before : heads_before - 1u);
       gid[row] = g;
       if (head) P.slot_rows[g] = uint64_t(row);
-      if (ends) group_last[g] = uint32_t(row);
     }
     running += all;
     __syncthreads();
