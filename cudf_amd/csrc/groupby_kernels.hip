@@ -2658,9 +2658,13 @@ void launch_sorted_second_pass(gb_params const* dP, gb_params const& hP, int64_t
 }
 
 // NUNIQUE on runs.  The reference sorts each group's values and counts the
-// heads of equal runs (sort/group_nunique.cu:44-64); here every counted row
-// inserts (group, value) into an open-addressing set of row indices, and the
-// row that claims an empty slot adds one distinct value to its group.  Rows
+// heads of equal runs (sort/group_nunique.cu:44-64).  Here a group whose rows
+// all lie in one 64-row chunk of a wave is counted inside the wave: a counted
+// row is a new value when no earlier row of its group holds an equal one
+// (shuffles over at most the group's length).  Every other counted row (longer
+// or chunk-crossing groups, and strings) inserts (group, value) into an
+// open-addressing set of row indices, and the row that claims an empty slot
+// adds one distinct value to its group.  Rows
 // are equal when they share the group and both are null (counted only with
 // nulls included) or hold the same value, -0.0 equal to 0.0 and NaN equal to
 // NaN (the row comparator's defaults).  Group g probes only its own region
@@ -2699,6 +2703,7 @@ __global__ void __launch_bounds__(BLOCK)
   field_desc const& F  = P.fields[f];
   dcol const& c        = P.vals[F.vcol];
   bool const with_null = F.kind == FK_NUNIQUE_NULLS;
+  bool const str       = c.storage == ST_STRING;  // strings: hashed, then compared byte-wise
   int const lane       = threadIdx.x & 63;
   // each wave takes 64 consecutive rows per step (the loop bound is uniform
   // across the wave, so every lane reaches the ballots below)
@@ -2707,17 +2712,41 @@ __global__ void __launch_bounds__(BLOCK)
     int64_t const row = base0 + lane;
     uint32_t const g  = row < n ? gid[row] : NO_GROUP;
     bool const valid  = g != NO_GROUP && dcol_valid(c, row);
+    bool const counts = g != NO_GROUP && (valid || with_null);
     bool inserted     = false;
-    bool const str = c.storage == ST_STRING;  // strings: hashed, then compared byte-wise
-    if (g != NO_GROUP && (valid || with_null)) {
-      uint64_t const b     = !valid ? 0ull : (str ? uint64_t(elem_hash(c, row, 0u)) : nunique_bits(c, row));
+    uint64_t const b  = !valid ? 0ull : (str ? uint64_t(elem_hash(c, row, 0u)) : nunique_bits(c, row));
+    uint64_t first = 0, end = 0;
+    if (g != NO_GROUP) {
+      first = P.slot_rows[g];
+      end   = int64_t(g) + 1 < G ? P.slot_rows[g + 1] : uint64_t(n);
+    }
+    // A group whose rows all lie in this chunk is counted here: a counted
+    // lane is a new value when no lower lane of its group holds an equal one
+    // (the group's lanes are contiguous, from lane first - base0).
+    bool const local    = counts && !str && first >= uint64_t(base0) && end <= uint64_t(base0 + 64);
+    uint64_t const lmask = __ballot(local);
+    if (lmask != 0) {
+      uint64_t const cmask = __ballot(counts);
+      uint64_t const vmask = __ballot(valid);
+      int dist             = local ? lane - int(first - uint64_t(base0)) : 0;
+      int maxd             = dist;
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) maxd = max(maxd, __shfl_xor(maxd, off));
+      bool dup = false;
+      for (int d = 1; d <= maxd; d++) {
+        uint64_t const ob = __shfl_up(b, d);
+        int const o       = lane - d;  // the other lane, in the group when d <= dist
+        if (d <= dist && ((cmask >> o) & 1u) && (((vmask >> o) & 1u) != 0) == valid && (!valid || ob == b))
+          dup = true;
+      }
+      inserted = local && !dup;
+    }
+    if (counts && !local) {
       uint32_t const h     = mm_fmix32(uint32_t(b) ^ mm_fmix32(uint32_t(b >> 32) ^ uint32_t(valid)));
-      uint64_t const first = P.slot_rows[g];
-      uint64_t const end   = int64_t(g) + 1 < G ? P.slot_rows[g + 1] : uint64_t(n);
-      uint64_t const base  = 2 * first, len = 2 * (end - first);
+      uint64_t const hbase = 2 * first, len = 2 * (end - first);
       uint64_t off         = (uint64_t(h) * len) >> 32;  // in [0, len)
       for (;;) {
-        uint64_t const slot = base + off;
+        uint64_t const slot = hbase + off;
         uint32_t cur = __hip_atomic_load(&table[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (cur == EMPTY) {
           uint32_t const prev = atomicCAS(&table[slot], EMPTY, uint32_t(row));
