@@ -170,6 +170,60 @@ def test_sorted_column_statistics_m2_rules(policy):
     _check_sorted([host_col(k, kvalid)], reqs, policy)
 
 
+def test_sorted_c2_shape_float_statistics():
+    """10M rows over 1M sorted keys with a nullable float64 column: MEAN,
+    VARIANCE, STD, MIN and COUNT from the column kernel (M2 from the sums for
+    most groups, the exact pass for the groups neighbouring waves share)
+    against float64 torch on the device: two-pass variance within relative
+    1e-10, the mean within the normwise bound of its sum, MIN and COUNT exact."""
+    import torch
+    import cudf_amd as ca
+    from cudf_amd import GroupBy, GroupByRequest, NullPolicy, Sorted
+    from cudf_amd.aggregation import Aggregation, Kind
+    n = 10_000_000
+    gen = torch.Generator(device="cuda").manual_seed(16)
+    key = torch.sort(torch.randint(0, 1_000_000, (n,), device="cuda", dtype=torch.int64, generator=gen)).values
+    val = torch.randn(n, device="cuda", dtype=torch.float64, generator=gen) * 10.0 + 3.0
+    ok = torch.rand(n, device="cuda", generator=gen) > 0.05
+    bits = torch.zeros(((n + 31) // 32) * 32, dtype=torch.int64, device="cuda")
+    bits[:n] = ok.to(torch.int64)
+    words = (bits.view(-1, 32) << torch.arange(32, device="cuda", dtype=torch.int64)).sum(1)
+    mask = torch.where(words >= 2**31, words - 2**32, words).to(torch.int32)  # 32 validity bits per word
+    kc = ca.Column(ca.TypeId.INT64, n, key.view(torch.uint8))
+    vc = ca.Column(ca.TypeId.FLOAT64, n, val.view(torch.uint8), mask=mask, null_count=int((~ok).sum()))
+    aggs = [Aggregation(Kind.MEAN), Aggregation(Kind.VARIANCE), Aggregation(Kind.STD), Aggregation(Kind.MIN),
+            Aggregation(Kind.COUNT_VALID)]
+    out_keys, (res,) = GroupBy(ca.Table([kc]), NullPolicy.EXCLUDE, Sorted.YES).aggregate([GroupByRequest(vc, aggs)])
+    uk, counts = torch.unique_consecutive(key, return_counts=True)
+    G = len(uk)
+    assert out_keys.num_rows() == G
+    seg = torch.repeat_interleave(torch.arange(G, device="cuda"), counts)
+    okf = ok.to(torch.float64)
+    cnt = torch.zeros(G, dtype=torch.float64, device="cuda").index_add_(0, seg, okf)
+    s1 = torch.zeros(G, dtype=torch.float64, device="cuda").index_add_(0, seg, val * okf)
+    mean = s1 / cnt
+    dev = (val - mean[seg]) * okf
+    m2 = torch.zeros(G, dtype=torch.float64, device="cuda").index_add_(0, seg, dev * dev)
+    var = m2 / (cnt - 1)
+    cols = res.columns()
+    got_mean = cols[0].data.view(torch.float64)[:G]
+    got_var = cols[1].data.view(torch.float64)[:G]
+    got_std = cols[2].data.view(torch.float64)[:G]
+    got_min = cols[3].data.view(torch.float64)[:G]
+    got_cnt = cols[4].data.view(torch.int32)[:G].to(torch.float64)
+    assert torch.equal(got_cnt, cnt)
+    has2 = cnt > 1
+    # MEAN: the sum's normwise bound (summation order differs), divided by n
+    sabs = torch.zeros(G, dtype=torch.float64, device="cuda").index_add_(0, seg, val.abs() * okf)
+    c1 = cnt > 0
+    assert ((got_mean[c1] - mean[c1]).abs() <= 1e-12 * sabs[c1] / cnt[c1]).all()
+    assert torch.allclose(got_var[has2], var[has2], rtol=1e-10, atol=0)
+    assert torch.allclose(got_std[has2], var[has2].sqrt(), rtol=1e-10, atol=0)
+    mn = torch.full((G,), float("inf"), dtype=torch.float64, device="cuda").scatter_reduce_(
+        0, seg, torch.where(ok, val, torch.full_like(val, float("inf"))), "amin")
+    assert torch.equal(got_min[cnt > 0], mn[cnt > 0])
+
+
 @pytest.mark.parametrize("vdt", [np.int32, np.int64, np.float32, np.float64])
 def test_sorted_product(vdt):
     """PRODUCT on runs (groups of ~20 rows): integers wrap in int64 exactly as
