@@ -1283,15 +1283,46 @@ __device__ __forceinline__ bool sorted_head(gb_params const& P, int64_t row)
   return !rows_equal(P, row, row - 1);
 }
 
+// Inclusion and run-head flags of a row (sorted_included, sorted_head).  With
+// one fixed-width key column, the common case, the row's and its
+// predecessor's validity and key bits are read once each.
+__device__ __forceinline__ bool sorted_one_key(gb_params const& P)
+{
+  return P.num_keys == 1 && P.keys[0].storage != ST_STRING;
+}
+
+__device__ __forceinline__ void sorted_flags(gb_params const& P, bool one_key, int64_t row, bool& incl, bool& head)
+{
+  if (!one_key) {
+    incl = sorted_included(P, row);
+    head = incl && sorted_head(P, row);
+    return;
+  }
+  dcol const& k = P.keys[0];
+  bool const v  = dcol_valid(k, row);
+  incl          = v || !P.skip_null_keys;
+  head          = incl;
+  if (incl && row > 0) {
+    bool const pv = dcol_valid(k, row - 1);
+    if (pv || !P.skip_null_keys)  // the predecessor is kept: a head when the keys differ (nulls equal)
+      head = v != pv || (v && load_fixed_bits(k, row) != load_fixed_bits(k, row - 1));
+  }
+}
+
 __global__ void __launch_bounds__(BLOCK) k_sorted_count(gb_params const* __restrict__ Pp, uint32_t* block_counts)
 {
   __shared__ uint32_t red[BLOCK / 64];
   gb_params const& P = *Pp;
+  bool const one_key = sorted_one_key(P);
   int64_t const base = int64_t(blockIdx.x) * SORTED_TILE;
   uint32_t heads = 0;
   for (int j = 0; j < SORTED_ROWS; j++) {
     int64_t const row = base + int64_t(j) * BLOCK + threadIdx.x;
-    if (row < P.num_rows && sorted_head(P, row)) heads++;
+    if (row < P.num_rows) {
+      bool incl, head;
+      sorted_flags(P, one_key, row, incl, head);
+      heads += head ? 1u : 0u;
+    }
   }
   uint32_t const t = block_sum_u32<BLOCK>(heads, red);
   if (threadIdx.x == 0) block_counts[blockIdx.x] = t;
@@ -1313,11 +1344,12 @@ __global__ void __launch_bounds__(BLOCK)
   // whether the group ends there is decided by the next iteration's first row
   uint32_t pending_g = NO_GROUP;
   uint32_t pending_row = 0;
+  bool const one_key   = sorted_one_key(P);
   for (int j = 0; j < SORTED_ROWS; j++) {
     int64_t const row = base + int64_t(j) * BLOCK + threadIdx.x;
     bool const live   = row < P.num_rows;
-    bool const incl   = live && sorted_included(P, row);
-    bool const head   = incl && sorted_head(P, row);
+    bool incl = false, head = false;
+    if (live) sorted_flags(P, one_key, row, incl, head);
     uint64_t const bal  = __ballot(head);
     uint64_t const ibal = __ballot(incl);
     if (lane == 0) {
@@ -1351,8 +1383,9 @@ __global__ void __launch_bounds__(BLOCK)
           pending_row = uint32_t(row);
         } else {  // the tile's last row: its successor starts the next tile
           int64_t const nr = row + 1;
-          bool const ni    = nr < P.num_rows && sorted_included(P, nr);
-          ends             = !ni || sorted_head(P, nr);
+          bool ni = false, nh = false;
+          if (nr < P.num_rows) sorted_flags(P, one_key, nr, ni, nh);
+          ends = !ni || nh;
         }
         if (ends) group_last[g] = uint32_t(row);
       }
