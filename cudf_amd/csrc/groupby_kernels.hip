@@ -2020,7 +2020,7 @@ static void sorted_ranges(int64_t n, int num_cus, int64_t& per, int64_t& W)
 // same pass where that is exact or within the bound below; the groups where
 // it is not (and the groups a wave shares with its neighbours) get the exact
 // two-pass M2 (the reference's mean, then squared deviations:
-// sort/group_m2.cu, sort/aggregate.cpp:286-306): groups of at most
+// sort/group_std.cu:60-69 through sort/aggregate.cpp:286-306): groups of at most
 // M2_WAVE_ROWS rows are listed and reduced by one wave each
 // (k_sorted_stats_m2_list), longer ones are flagged for the row-parallel
 // k_sorted_stats_m2, which returns at once when none is.
