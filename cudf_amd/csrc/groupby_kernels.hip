@@ -2263,8 +2263,30 @@ __device__ __forceinline__ void stats_flush(gb_params const& P, sorted_stats_job
   }
 }
 
-// One wave per contiguous row range, 64-row chunks (the range layout and the
-// carry between chunks of k_sorted_reduce).
+// Rows per lane in k_sorted_stats: a lane first combines its consecutive rows
+// serially, so one cross-lane scan covers 64 * STATS_ROWS rows.
+constexpr int STATS_ROWS = 2;
+
+// Two partials of one group combined.
+template <int ST>
+__device__ __forceinline__ stats_part<ST> stats_combine(stats_part<ST> a, stats_part<ST> const& b)
+{
+  a.rows += b.rows;
+  a.cnt += b.cnt;
+  a.sum += b.sum;
+  a.sq += b.sq;
+  a.isq += b.isq;
+  a.mn = a.mn < b.mn ? a.mn : b.mn;
+  a.mx = a.mx > b.mx ? a.mx : b.mx;
+  return a;
+}
+
+// One wave per contiguous row range (the range layout and the carry between
+// chunks of k_sorted_reduce), chunks of 64 * STATS_ROWS rows with
+// STATS_ROWS consecutive rows per lane.  A lane's rows are combined serially;
+// a group that ends inside the lane (before its last row) is finished there
+// with the scan total of the lanes before it, and the partial of the lane's
+// last group takes part in the segmented scan across the wave.
 template <int ST>
 __global__ void __launch_bounds__(BLOCK)
   k_sorted_stats(gb_params const* __restrict__ Pp, uint32_t const* __restrict__ gid, int64_t per, int64_t W,
@@ -2273,6 +2295,7 @@ __global__ void __launch_bounds__(BLOCK)
   using T            = typename stats_traits<ST>::T;
   using Sum          = typename stats_traits<ST>::Sum;
   using Ord          = typename stats_traits<ST>::Ord;
+  constexpr int K    = STATS_ROWS;
   gb_params const& P = *Pp;
   int const lane     = threadIdx.x & 63;
   int64_t const w    = int64_t(blockIdx.x) * (BLOCK / 64) + (threadIdx.x >> 6);
@@ -2287,58 +2310,100 @@ __global__ void __launch_bounds__(BLOCK)
   uint32_t carry_g       = NO_GROUP;
   Ord const min_ident    = Ord(J.min_ident);
   Ord const max_ident    = Ord(J.max_ident);
-  stats_part<ST> cy{0, 0, Sum(0), min_ident, max_ident, 0.0, 0};
-  for (int64_t i0 = b; i0 < e; i0 += 64) {
-    int64_t const row = i0 + lane;
-    bool const live   = row < e;
-    int64_t const rc  = live ? row : e - 1;
-    uint32_t const gl = gid[rc];
-    T const x         = data[rc];
-    bool const vv     = dcol_valid(c, rc);
-    uint32_t const g  = live ? gl : NO_GROUP;
-    bool has          = g != NO_GROUP && vv;
-    stats_part<ST> a;
-    a.rows = g != NO_GROUP ? 1u : 0u;
-    a.cnt  = has ? 1u : 0u;
-    a.sum  = has ? Sum(x) : Sum(0);
-    a.sq   = has && want_sq ? double(x) * double(x) : 0.0;
-    a.isq  = has && want_isq ? uint64_t(int64_t(x)) * uint64_t(int64_t(x)) : 0;
-    bool ordered = has;
-    if constexpr (ST == ST_F64) ordered = has && !isnan(x);  // NaN takes no part in MIN / MAX
-    Ord const t = stats_order<ST>(x);
-    a.mn        = ordered ? t : min_ident;
-    a.mx        = ordered ? t : max_ident;
-    uint32_t const gn = uint32_t(__builtin_amdgcn_update_dpp(int(NO_GROUP), int(g), 0x130, 0xF, 0xF, false));
-    uint32_t const g0 = uint32_t(__builtin_amdgcn_readlane(int(g), 0));
+  stats_part<ST> const none{0, 0, Sum(0), min_ident, max_ident, 0.0, 0};
+  stats_part<ST> cy = none;
+  for (int64_t i0 = b; i0 < e; i0 += 64 * K) {
+    int64_t const r0 = i0 + int64_t(K) * lane;  // the lane's first row
+    uint32_t gg[K];
+    stats_part<ST> a[K];
+#pragma unroll
+    for (int u = 0; u < K; u++) {
+      int64_t const row = r0 + u;
+      bool const live   = row < e;
+      int64_t const rc  = live ? row : e - 1;
+      uint32_t const gl = gid[rc];
+      T const x         = data[rc];
+      bool const vv     = dcol_valid(c, rc);
+      gg[u]             = live ? gl : NO_GROUP;
+      bool const has    = gg[u] != NO_GROUP && vv;
+      a[u].rows         = gg[u] != NO_GROUP ? 1u : 0u;
+      a[u].cnt          = has ? 1u : 0u;
+      a[u].sum          = has ? Sum(x) : Sum(0);
+      a[u].sq           = has && want_sq ? double(x) * double(x) : 0.0;
+      a[u].isq          = has && want_isq ? uint64_t(int64_t(x)) * uint64_t(int64_t(x)) : 0;
+      bool ordered      = has;
+      if constexpr (ST == ST_F64) ordered = has && !isnan(x);  // NaN takes no part in MIN / MAX
+      Ord const t = stats_order<ST>(x);
+      a[u].mn     = ordered ? t : min_ident;
+      a[u].mx     = ordered ? t : max_ident;
+    }
+    uint32_t const g0 = uint32_t(__builtin_amdgcn_readlane(int(gg[0]), 0));
     if (carry_g != NO_GROUP && g0 != carry_g) {
       if (lane == 0) stats_flush<ST>(P, J, carry_g, cy, carry_g == first_g, true, i0 - 1);
     } else if (carry_g != NO_GROUP && lane == 0) {  // the carried group continues at lane 0
-      a.rows += cy.rows;
-      a.cnt += cy.cnt;
-      a.sum += cy.sum;
-      a.sq += cy.sq;
-      a.isq += cy.isq;
-      a.mn = a.mn < cy.mn ? a.mn : cy.mn;
-      a.mx = a.mx > cy.mx ? a.mx : cy.mx;
+      a[0] = stats_combine<ST>(a[0], cy);
     }
-    seg_masks const m = seg_masks_of(g);
+    // the lane's rows, serially: `first` holds the lane's first group when
+    // that group ends inside the lane
+    uint32_t cur = gg[0];
+    stats_part<ST> R = a[0], first = none;
+    bool split = false;
+#pragma unroll
+    for (int u = 1; u < K; u++) {
+      if (gg[u] == cur) {
+        R = stats_combine<ST>(R, a[u]);
+      } else {
+        if (!split) {
+          first = R;
+          split = true;
+        } else if (cur != NO_GROUP) {  // a group wholly inside the lane (more than two rows per lane)
+          stats_flush<ST>(P, J, cur, R, cur == first_g, true, r0 + u - 1);
+        }
+        cur = gg[u];
+        R   = a[u];
+      }
+    }
+    seg_masks const m = seg_masks_of(cur);
     auto add32        = [](uint32_t p, uint32_t q) { return p + q; };
-    if (J.count_rows) a.rows = mscan(a.rows, m, add32);
-    a.cnt = mscan(a.cnt, m, add32);
-    a.sum = mscan(a.sum, m, [](Sum p, Sum q) { return p + q; });
-    if (J.f_min >= 0) a.mn = mscan(a.mn, m, [](Ord p, Ord q) { return p < q ? p : q; });
-    if (J.f_max >= 0) a.mx = mscan(a.mx, m, [](Ord p, Ord q) { return p > q ? p : q; });
-    if (want_sq) a.sq = mscan(a.sq, m, [](double p, double q) { return p + q; });
-    if (want_isq) a.isq = mscan(a.isq, m, [](uint64_t p, uint64_t q) { return p + q; });
-    if (g != NO_GROUP && lane < 63 && gn != g) stats_flush<ST>(P, J, g, a, g == first_g, true, row);
-    cy.rows = lane63(a.rows);
-    cy.cnt  = lane63(a.cnt);
-    cy.sum  = lane63(a.sum);
-    cy.mn   = lane63(a.mn);
-    cy.mx   = lane63(a.mx);
-    cy.sq   = lane63(a.sq);
-    cy.isq  = lane63(a.isq);
-    carry_g = uint32_t(__builtin_amdgcn_readlane(int(g), 63));
+    if (J.count_rows) R.rows = mscan(R.rows, m, add32);
+    R.cnt = mscan(R.cnt, m, add32);
+    R.sum = mscan(R.sum, m, [](Sum p, Sum q) { return p + q; });
+    if (J.f_min >= 0) R.mn = mscan(R.mn, m, [](Ord p, Ord q) { return p < q ? p : q; });
+    if (J.f_max >= 0) R.mx = mscan(R.mx, m, [](Ord p, Ord q) { return p > q ? p : q; });
+    if (want_sq) R.sq = mscan(R.sq, m, [](double p, double q) { return p + q; });
+    if (want_isq) R.isq = mscan(R.isq, m, [](uint64_t p, uint64_t q) { return p + q; });
+    // a first group that ended inside the lane: its rows in the lanes before
+    // (the previous lane's scan total when that lane's last group is the same)
+    uint32_t const cprev = uint32_t(__builtin_amdgcn_update_dpp(int(NO_GROUP), int(cur), 0x138, 0xF, 0xF, false));
+    stats_part<ST> prev;
+    prev.rows = uint32_t(__builtin_amdgcn_update_dpp(0, int(R.rows), 0x138, 0xF, 0xF, false));
+    prev.cnt  = uint32_t(__builtin_amdgcn_update_dpp(0, int(R.cnt), 0x138, 0xF, 0xF, false));
+    prev.sum  = __builtin_bit_cast(Sum, dpp64<0x138, 0xF>(__builtin_bit_cast(uint64_t, R.sum)));
+    if constexpr (sizeof(Ord) == 8) {
+      prev.mn = __builtin_bit_cast(Ord, dpp64<0x138, 0xF>(__builtin_bit_cast(uint64_t, R.mn)));
+      prev.mx = __builtin_bit_cast(Ord, dpp64<0x138, 0xF>(__builtin_bit_cast(uint64_t, R.mx)));
+    } else {
+      prev.mn = Ord(__builtin_amdgcn_update_dpp(0, int(R.mn), 0x138, 0xF, 0xF, false));
+      prev.mx = Ord(__builtin_amdgcn_update_dpp(0, int(R.mx), 0x138, 0xF, 0xF, false));
+    }
+    prev.sq  = __builtin_bit_cast(double, dpp64<0x138, 0xF>(__builtin_bit_cast(uint64_t, R.sq)));
+    prev.isq = dpp64<0x138, 0xF>(R.isq);
+    if (split && gg[0] != NO_GROUP) {
+      stats_part<ST> const tot = lane > 0 && cprev == gg[0] ? stats_combine<ST>(first, prev) : first;
+      stats_flush<ST>(P, J, gg[0], tot, gg[0] == first_g, true, r0);
+    }
+    // the lane's last group ends at the lane's last row when the next lane
+    // starts another group
+    uint32_t const gnext = uint32_t(__builtin_amdgcn_update_dpp(int(NO_GROUP), int(gg[0]), 0x130, 0xF, 0xF, false));
+    if (cur != NO_GROUP && lane < 63 && gnext != cur) stats_flush<ST>(P, J, cur, R, cur == first_g, true, r0 + K - 1);
+    cy.rows = lane63(R.rows);
+    cy.cnt  = lane63(R.cnt);
+    cy.sum  = lane63(R.sum);
+    cy.mn   = lane63(R.mn);
+    cy.mx   = lane63(R.mx);
+    cy.sq   = lane63(R.sq);
+    cy.isq  = lane63(R.isq);
+    carry_g = uint32_t(__builtin_amdgcn_readlane(int(cur), 63));
   }
   if (carry_g != NO_GROUP && lane == 0) {  // may continue in the next range
     bool const ends = e == P.num_rows || gid[e] != carry_g;
@@ -2432,6 +2497,11 @@ static void launch_stats_job(gb_params const* dP, sorted_stats_job const& J, int
 {
   int64_t per, W;
   sorted_ranges(n, num_cus, per, W);
+  // ranges of whole chunks: a range then ends at a chunk's last lane, so a
+  // group at a range's end is always carried (and combined atomically)
+  constexpr int64_t CH = 64 * STATS_ROWS;
+  per = (per + CH - 1) / CH * CH;
+  W   = (n + per - 1) / per;
   unsigned const blocks = unsigned((W + BLOCK / 64 - 1) / (BLOCK / 64));
   hipLaunchKernelGGL(k_sorted_stats<ST>, dim3(blocks), dim3(BLOCK), 0, s, dP, gid, per, W, J);
 }
