@@ -2265,7 +2265,7 @@ __device__ __forceinline__ void stats_flush(gb_params const& P, sorted_stats_job
 
 // Rows per lane in k_sorted_stats: a lane first combines its consecutive rows
 // serially, so one cross-lane scan covers 64 * STATS_ROWS rows.
-constexpr int STATS_ROWS = 2;
+constexpr int STATS_ROWS = 4;
 
 // Two partials of one group combined.
 template <int ST>
