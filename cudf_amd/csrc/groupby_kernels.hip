@@ -2012,8 +2012,8 @@ static void sorted_ranges(int64_t n, int num_cus, int64_t& per, int64_t& W)
 
 // ---------------------------------------------------------------------------
 // Column statistics on runs.  A value column whose accumulators are all among
-// SUM, SUM_OF_SQUARES, COUNT, MIN, MAX and M2 (VARIANCE / STD), stored as int32, int64 or
-// float64, is reduced by a kernel compiled for its type: the combining
+// SUM, SUM_OF_SQUARES, COUNT, MIN, MAX and M2 (VARIANCE / STD), stored as int32, int64,
+// float32 or float64, is reduced by a kernel compiled for its type: the combining
 // operations are fixed at compile time, the segment masks of the six scan
 // steps are computed once per chunk for all accumulators, and the partials
 // carried between chunks stay in registers.  M2 comes from the sums in the
@@ -2044,6 +2044,12 @@ struct stats_traits<ST_F64> {
   using Sum = double;
   using Ord = int64_t;
 };
+template <>
+struct stats_traits<ST_F32> {
+  using T   = float;
+  using Sum = double;  // float32 sums and sums of squares accumulate in double, rounded once when stored
+  using Ord = int32_t;
+};
 
 // Value of a row in the signed MIN / MAX domain (floats: the order code with
 // the sign bit flipped, as scan_domain maps them; integers: themselves).
@@ -2051,6 +2057,7 @@ template <int ST>
 __device__ __forceinline__ typename stats_traits<ST>::Ord stats_order(typename stats_traits<ST>::T x)
 {
   if constexpr (ST == ST_F64) return int64_t(ord64(uint64_t(__double_as_longlong(x))) ^ 0x8000000000000000ull);
+  else if constexpr (ST == ST_F32) return int32_t(ord32(__float_as_uint(x)) ^ 0x80000000u);
   else return x;
 }
 
@@ -2059,6 +2066,7 @@ template <int ST>
 __device__ __forceinline__ typename stats_traits<ST>::Ord stats_order_inv(typename stats_traits<ST>::Ord t)
 {
   if constexpr (ST == ST_F64) return int64_t(uint64_t(t) ^ 0x8000000000000000ull);
+  else if constexpr (ST == ST_F32) return int32_t(uint32_t(t) ^ 0x80000000u);
   else return t;
 }
 
@@ -2136,11 +2144,11 @@ __device__ __forceinline__ V lane63(V x)
 template <int ST>
 __device__ __forceinline__ bool stats_m2(uint32_t n, typename stats_traits<ST>::Sum s1, double s2, double& m2)
 {
-  if (n <= 1) {
+  if (n <= 1 && isfinite(s2)) {  // one finite value (or none): no deviation; NaN / inf give NaN below
     m2 = 0.0;
     return true;
   }
-  if constexpr (ST == ST_F64) {
+  if constexpr (ST == ST_F64 || ST == ST_F32) {  // float32 values: exact squares, sums in double
     if (n > 256) return false;
     double const v = s2 - s1 * (s1 / double(n));
     if (!(v > 0.0) || 3.0 * double(n) * 1.1102230246251565e-16 * s2 > 1e-11 * v) {
@@ -2213,11 +2221,16 @@ __device__ __forceinline__ void stats_flush(gb_params const& P, sorted_stats_job
     else *gp(J.p_count + g) = a.cnt;
   }
   if (J.f_sum >= 0) {
-    Sum* p = static_cast<Sum*>(J.p_sum) + g;
-    if constexpr (ST == ST_F64) {
+    if constexpr (ST == ST_F32) {
+      float* p = static_cast<float*>(J.p_sum) + g;
+      if (shared) gadd_f32(p, float(a.sum));
+      else *gp(p) = float(a.sum);
+    } else if constexpr (ST == ST_F64) {
+      Sum* p = static_cast<Sum*>(J.p_sum) + g;
       if (shared) gadd_f64(p, a.sum);
       else *gp(p) = a.sum;
     } else {
+      Sum* p = static_cast<Sum*>(J.p_sum) + g;
       if (shared) aadd(reinterpret_cast<uint64_t*>(p), uint64_t(a.sum));
       else *gp(p) = a.sum;
     }
@@ -2228,6 +2241,7 @@ __device__ __forceinline__ void stats_flush(gb_params const& P, sorted_stats_job
     Ord* p      = static_cast<Ord*>(J.p_min) + g;
     if (!shared) *gp(p) = u;
     else if constexpr (ST == ST_F64) amin(reinterpret_cast<uint64_t*>(p), uint64_t(u));
+    else if constexpr (ST == ST_F32) amin(reinterpret_cast<uint32_t*>(p), uint32_t(u));
     else amin(p, u);
   }
   if (J.f_max >= 0) {
@@ -2235,10 +2249,15 @@ __device__ __forceinline__ void stats_flush(gb_params const& P, sorted_stats_job
     Ord* p      = static_cast<Ord*>(J.p_max) + g;
     if (!shared) *gp(p) = u;
     else if constexpr (ST == ST_F64) amax(reinterpret_cast<uint64_t*>(p), uint64_t(u));
+    else if constexpr (ST == ST_F32) amax(reinterpret_cast<uint32_t*>(p), uint32_t(u));
     else amax(p, u);
   }
   if (J.f_sumsq >= 0) {
-    if constexpr (ST == ST_F64) {
+    if constexpr (ST == ST_F32) {
+      float* p = static_cast<float*>(J.p_sumsq) + g;
+      if (shared) gadd_f32(p, float(a.sq));
+      else *gp(p) = float(a.sq);
+    } else if constexpr (ST == ST_F64) {
       double* p = static_cast<double*>(J.p_sumsq) + g;
       if (shared) gadd_f64(p, a.sq);
       else *gp(p) = a.sq;
@@ -2304,8 +2323,9 @@ __global__ void __launch_bounds__(BLOCK)
   int64_t const e = b + per < P.num_rows ? b + per : P.num_rows;
   dcol const c    = P.vals[J.vcol];
   auto const* data = reinterpret_cast<gmem<T const>*>(gp(c.data));
-  bool const want_sq  = J.f_m2 >= 0 || (ST == ST_F64 && J.f_sumsq >= 0);
-  bool const want_isq = ST != ST_F64 && J.f_sumsq >= 0;
+  constexpr bool FLOAT = ST == ST_F64 || ST == ST_F32;
+  bool const want_sq  = J.f_m2 >= 0 || (FLOAT && J.f_sumsq >= 0);
+  bool const want_isq = !FLOAT && J.f_sumsq >= 0;
   uint32_t const first_g = gid[b];  // may continue from the previous range
   uint32_t carry_g       = NO_GROUP;
   Ord const min_ident    = Ord(J.min_ident);
@@ -2332,7 +2352,7 @@ __global__ void __launch_bounds__(BLOCK)
       a[u].sq           = has && want_sq ? double(x) * double(x) : 0.0;
       a[u].isq          = has && want_isq ? uint64_t(int64_t(x)) * uint64_t(int64_t(x)) : 0;
       bool ordered      = has;
-      if constexpr (ST == ST_F64) ordered = has && !isnan(x);  // NaN takes no part in MIN / MAX
+      if constexpr (FLOAT) ordered = has && !isnan(x);  // NaN takes no part in MIN / MAX
       Ord const t = stats_order<ST>(x);
       a[u].mn     = ordered ? t : min_ident;
       a[u].mx     = ordered ? t : max_ident;
@@ -2523,6 +2543,10 @@ static bool stats_kind_ok(int32_t st, int32_t kind)
     case FK_SUM_F64: return st == ST_F64;
     case FK_SUMSQ_I64: return st == ST_I32 || st == ST_I64;
     case FK_SUMSQ_F64: return st == ST_F64;
+    case FK_SUM_F32:
+    case FK_SUMSQ_F32:
+    case FK_MIN_F32:
+    case FK_MAX_F32: return st == ST_F32;
     case FK_MIN_S32:
     case FK_MAX_S32: return st == ST_I32;
     case FK_MIN_S64:
@@ -2554,7 +2578,7 @@ static std::vector<std::pair<int32_t, sorted_stats_job>> stats_jobs(gb_params co
   for (int j = 0; j < hP.num_vals; j++) {
     int32_t const st = hP.vals[j].storage;
     int const fb = hP.val_field_begin[j], fe = hP.val_field_begin[j + 1];
-    if (fb == fe || (st != ST_I32 && st != ST_I64 && st != ST_F64)) continue;
+    if (fb == fe || (st != ST_I32 && st != ST_I64 && st != ST_F64 && st != ST_F32)) continue;
     sorted_stats_job J{};
     J.vcol = j;
     J.f_sum = J.f_count = J.f_min = J.f_max = J.f_m2 = J.f_sumsq = -1;
@@ -2566,9 +2590,9 @@ static std::vector<std::pair<int32_t, sorted_stats_job>> stats_jobs(gb_params co
       ok              = stats_kind_ok(st, k) && hP.fields[f].stride == hP.fields[f].width;
       int32_t* slot   = k == FK_COUNT ? &J.f_count
                         : k == FK_M2  ? &J.f_m2
-                        : (k == FK_SUMSQ_I64 || k == FK_SUMSQ_F64)                 ? &J.f_sumsq
-                        : (k == FK_SUM_I64 || k == FK_SUM_F64)                     ? &J.f_sum
-                        : (k == FK_MIN_S32 || k == FK_MIN_S64 || k == FK_MIN_F64) ? &J.f_min
+                        : (k == FK_SUMSQ_I64 || k == FK_SUMSQ_F64 || k == FK_SUMSQ_F32) ? &J.f_sumsq
+                        : (k == FK_SUM_I64 || k == FK_SUM_F64 || k == FK_SUM_F32)       ? &J.f_sum
+                        : (k == FK_MIN_S32 || k == FK_MIN_S64 || k == FK_MIN_F64 || k == FK_MIN_F32) ? &J.f_min
                                                                                    : &J.f_max;
       if (ok) {
         ok    = *slot < 0;
@@ -2584,13 +2608,18 @@ static std::vector<std::pair<int32_t, sorted_stats_job>> stats_jobs(gb_params co
     J.p_max   = ptr(J.f_max);
     J.p_sumsq = ptr(J.f_sumsq);
     J.p_m2    = static_cast<double*>(ptr(J.f_m2));
-    int32_t const kmin = st == ST_I32 ? FK_MIN_S32 : st == ST_I64 ? FK_MIN_S64 : FK_MIN_F64;
-    int32_t const kmax = st == ST_I32 ? FK_MAX_S32 : st == ST_I64 ? FK_MAX_S64 : FK_MAX_F64;
-    int32_t op, xf;
-    scan_domain(kmin, op, xf);
-    J.min_ident = int64_t(xf_fwd(xf, field_identity(kmin)));
-    scan_domain(kmax, op, xf);
-    J.max_ident = int64_t(xf_fwd(xf, field_identity(kmax)));
+    if (st == ST_F32) {  // float32 order codes with the sign bit flipped: a signed 32-bit domain
+      J.min_ident = int64_t(int32_t(uint32_t(field_identity(FK_MIN_F32)) ^ 0x80000000u));
+      J.max_ident = int64_t(int32_t(uint32_t(field_identity(FK_MAX_F32)) ^ 0x80000000u));
+    } else {
+      int32_t const kmin = st == ST_I32 ? FK_MIN_S32 : st == ST_I64 ? FK_MIN_S64 : FK_MIN_F64;
+      int32_t const kmax = st == ST_I32 ? FK_MAX_S32 : st == ST_I64 ? FK_MAX_S64 : FK_MAX_F64;
+      int32_t op, xf;
+      scan_domain(kmin, op, xf);
+      J.min_ident = int64_t(xf_fwd(xf, field_identity(kmin)));
+      scan_domain(kmax, op, xf);
+      J.max_ident = int64_t(xf_fwd(xf, field_identity(kmax)));
+    }
     if (J.f_m2 >= 0 && m2_scratch != nullptr) {  // scratch of its own per column
       size_t const g  = size_t(G > 0 ? G : 1);
       uint8_t* base   = m2_scratch + m2_column_bytes(G) * size_t(m2_jobs++);
@@ -2624,6 +2653,7 @@ void launch_sorted_stats(gb_params const* dP, gb_params const& hP, int64_t n, in
     if (J.f_m2 >= 0) hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, J.m2_counts, 2);
     if (st == ST_I32) launch_stats_job<ST_I32>(dP, J, n, gid, num_cus, s);
     else if (st == ST_I64) launch_stats_job<ST_I64>(dP, J, n, gid, num_cus, s);
+    else if (st == ST_F32) launch_stats_job<ST_F32>(dP, J, n, gid, num_cus, s);
     else launch_stats_job<ST_F64>(dP, J, n, gid, num_cus, s);
   }
 }
@@ -2636,6 +2666,7 @@ void launch_sorted_stats_m2(gb_params const* dP, gb_params const& hP, int64_t n,
     if (J.f_m2 < 0) continue;
     if (st == ST_I32) launch_stats_m2<ST_I32>(dP, J, n, gid, num_cus, s);
     else if (st == ST_I64) launch_stats_m2<ST_I64>(dP, J, n, gid, num_cus, s);
+    else if (st == ST_F32) launch_stats_m2<ST_F32>(dP, J, n, gid, num_cus, s);
     else launch_stats_m2<ST_F64>(dP, J, n, gid, num_cus, s);
   }
 }

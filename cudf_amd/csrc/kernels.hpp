@@ -355,7 +355,7 @@ void launch_sorted_count(gb_params const* dP, int64_t n, uint32_t* block_counts,
 void launch_sorted_assign(gb_params const* dP, int64_t n, uint32_t const* block_offsets, uint32_t* gid,
                           uint32_t* group_last, hipStream_t s);
 // Fields (and the group row count) that launch_sorted_stats reduces: value
-// columns of int32 / int64 / float64 whose accumulators are all among SUM,
+// columns of int32 / int64 / float32 / float64 whose accumulators are all among SUM,
 // SUM_OF_SQUARES, COUNT, MIN, MAX and M2, each by a kernel compiled for the
 // column type.
 struct sorted_stats_cover {

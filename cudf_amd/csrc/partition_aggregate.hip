@@ -573,10 +573,10 @@ __device__ __forceinline__ bool m2_from_sums(int32_t sum_kind, int32_t sum_off, 
                                              uint32_t n, double* m2)
 {
   double const s2 = *m2;
-  if (n <= 1) {  // no deviation from the mean of one value
+  if (n <= 1 && isfinite(s2)) {  // no deviation from the mean of one finite value
     *m2 = 0.0;
     return true;
-  }
+  }  // one NaN or infinite value: the exact pass gives (x - x)² = NaN, as the reference
   if (st == ST_F32) return false;
   if (st == ST_F64) {
     if (n > 256) return false;

@@ -177,6 +177,25 @@ def test_variance_one_pass_and_exact_fallback(case):
         assert abs(got - w) <= 1e-10 * abs(w) + (0.0 if w != 0 else 1e-300), (case, key, got, w)
 
 
+@pytest.mark.parametrize("vdt", [np.float32, np.float64])
+def test_variance_of_nan_and_infinite_values(vdt):
+    """Groups holding NaN or an infinity (many of one row): the reference's
+    two-pass variance gives NaN ((x - mean)² of an infinite or NaN mean), also
+    for a single value under ddof = 0, so such groups may not take the
+    one-value shortcut of the first-pass rule.  Against the oracle."""
+    rng = np.random.default_rng(41 if vdt == np.float32 else 42)
+    n = 300_000
+    k = rng.integers(0, 150_000, n).astype(np.int64)
+    v = rng.standard_normal(n).astype(vdt)
+    v[rng.random(n) < 0.02] = np.nan
+    v[rng.random(n) < 0.02] = np.inf
+    v[rng.random(n) < 0.02] = -np.inf
+    reqs = [(host_col(v, rng.random(n) > 0.05), [(AGG["VARIANCE"], 0), (AGG["STD"], 0), (AGG["VARIANCE"], 1),
+                                                 (AGG["MEAN"], 1), (AGG["COUNT_VALID"], 1)])]
+    assert_parity([host_col(k)], reqs)
+    assert _stats()["strategy"] == 2.0
+
+
 def _string_pool(rng, size, max_len=24):
     return [bytes(rng.integers(97, 123, int(rng.integers(0, max_len)), dtype=np.uint8)) for _ in range(size)]
 

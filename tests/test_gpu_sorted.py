@@ -140,7 +140,7 @@ STATS_AGGS = [(AGG["SUM"], 1), (AGG["MIN"], 1), (AGG["MAX"], 1), (AGG["COUNT_VAL
 
 @pytest.mark.parametrize("policy", [0, 1])
 def test_sorted_column_statistics_m2_rules(policy):
-    """int32 / int64 / float64 columns whose aggregations all have the
+    """int32 / int64 / float32 / float64 columns whose aggregations all have the
     type-specialised runs form (SUM, SUM_OF_SQUARES, MIN, MAX, COUNT, MEAN,
     VARIANCE, STD), with the cases that decide where M2 comes from: groups of
     1 to 9,000 rows (float64 groups above 256 values, and groups that
@@ -149,7 +149,8 @@ def test_sorted_column_statistics_m2_rules(policy):
     2^40 and int32 values over the whole range (Σx² at or above 2^53: exact
     pass; SUM_OF_SQUARES wraps in 64 bits like the oracle), float64 values
     1e9 + N(0, 1e-3) (ill-conditioned: exact pass), small values (M2 from the
-    sums), null values, a group without valid values, and null keys last."""
+    sums), float32 values with NaN, -0.0 and infinity, null values, a group
+    without valid values, and null keys last."""
     rng = np.random.default_rng(300 + policy)
     sizes = np.concatenate([rng.integers(1, 20, 20_000), [300, 1000, 3000, 257, 256, 4096, 4097, 9000],
                             rng.integers(1, 4, 5000)])
@@ -163,7 +164,12 @@ def test_sorted_column_statistics_m2_rules(policy):
     f64 = np.where(k % 5 == 0, 1e9 + 1e-3 * rng.standard_normal(n), 100.0 * rng.standard_normal(n))
     f64valid = rng.random(n) > 0.05
     f64valid[k == 7] = False
+    f32 = (10.0 * rng.standard_normal(n)).astype(np.float32)
+    f32[rng.random(n) < 0.01] = np.nan
+    f32[rng.random(n) < 0.01] = -0.0
+    f32[rng.random(n) < 0.002] = np.inf
     reqs = [(host_col(i32, rng.random(n) > 0.1), STATS_AGGS + [(AGG["SUM_OF_SQUARES"], 1)]),
+            (host_col(f32, rng.random(n) > 0.1), STATS_AGGS + [(AGG["SUM_OF_SQUARES"], 1)]),
             (host_col(i64, rng.random(n) > 0.1), STATS_AGGS),
             (host_col(f64, f64valid), STATS_AGGS + [(AGG["SUM_OF_SQUARES"], 1)]),
             (host_col(rng.integers(0, 50, n).astype(np.int64)), [(AGG["VARIANCE"], 1), (AGG["MEAN"], 1)])]
