@@ -422,7 +422,8 @@ def assert_parity(keys, requests, null_policy=0, gpu=None, stats=None):
             gnan, onan = np.isnan(g64), np.isnan(o64)
             assert np.array_equal(gnan, onan), f"agg {kind}: NaN results differ at {np.flatnonzero(gnan != onan)[:5]}"
             same = (g64 == o64) | gnan
-            err = np.where(same, 0.0, np.abs(np.where(same, 0.0, g64 - o64)))
+            with np.errstate(invalid="ignore"):  # inf - inf where both are the same infinity
+                err = np.where(same, 0.0, np.abs(np.where(same, 0.0, g64 - o64)))
             if stats is not None and col.type_id == orc.FLOAT64 and kind in (AGG["SUM"], AGG["MEAN"]):
                 rel = err / np.maximum(np.abs(o64), 1e-300)
                 stats[(r, kind)] = {"groups": int(len(rel)), "max_rel": float(rel.max(initial=0.0)),
