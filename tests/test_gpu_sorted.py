@@ -176,7 +176,7 @@ def test_sorted_column_statistics_m2_rules(policy):
     _check_sorted([host_col(k, kvalid)], reqs, policy)
 
 
-def test_sorted_c2_shape_float_statistics():
+def test_sorted_10m_rows_float_statistics():
     """10M rows over 1M sorted keys with a nullable float64 column: MEAN,
     VARIANCE, STD, MIN and COUNT from the column kernel (M2 from the sums for
     most groups, the exact pass for the groups neighbouring waves share)
